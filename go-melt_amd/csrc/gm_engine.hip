@@ -1,0 +1,1246 @@
+// Host side of libgomelt.so: the C ABI of include/gomelt.h and the engine
+// that drives one L1 step of the multilevel scheme on a HIP stream.
+//
+// The engine owns the device state of the three levels and of the Level0
+// state grid.  The host keeps the window bookkeeping (integer shifts, overlap
+// boxes) and writes it, with the toolpath rows of the step, into a GmStepDesc
+// that is uploaded once per call; every kernel reads the moving meshes from
+// that descriptor, so the ~330 launches of a subcycled L1 step are recorded
+// once into a hipGraph and replayed for every later step.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "gm_internal.h"
+
+using namespace gm;
+
+static thread_local std::string g_err;
+
+static int fail(const std::string& msg) {
+    g_err = msg;
+    return -1;
+}
+
+#define HIPCHK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(std::string(#call) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+extern "C" const char* gm_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Device, memory, streams
+// ---------------------------------------------------------------------------
+
+extern "C" int gm_device_init(int device) {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipFree(nullptr));
+    return 0;
+}
+
+extern "C" int gm_device_name(char* buf, int len) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return 0;
+}
+
+extern "C" int gm_malloc(void** ptr, size_t bytes) {
+    HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+    return 0;
+}
+extern "C" int gm_free(void* ptr) {
+    HIPCHK(hipFree(ptr));
+    return 0;
+}
+extern "C" int gm_memcpy_h2d(void* dst, const void* src, size_t n, void* s) {
+    HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, (hipStream_t)s));
+    return 0;
+}
+extern "C" int gm_memcpy_d2h(void* dst, const void* src, size_t n, void* s) {
+    HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, (hipStream_t)s));
+    HIPCHK(hipStreamSynchronize((hipStream_t)s));
+    return 0;
+}
+extern "C" int gm_memcpy_d2d(void* dst, const void* src, size_t n, void* s) {
+    HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    return 0;
+}
+extern "C" int gm_memset(void* dst, int v, size_t n, void* s) {
+    HIPCHK(hipMemsetAsync(dst, v, n, (hipStream_t)s));
+    return 0;
+}
+extern "C" int gm_stream_create(void** s) {
+    HIPCHK(hipStreamCreate((hipStream_t*)s));
+    return 0;
+}
+extern "C" int gm_stream_destroy(void* s) {
+    HIPCHK(hipStreamDestroy((hipStream_t)s));
+    return 0;
+}
+extern "C" int gm_stream_sync(void* s) {
+    HIPCHK(hipStreamSynchronize((hipStream_t)s));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+extern "C" int gm_event_create(void** ev) {
+    HIPCHK(hipEventCreate((hipEvent_t*)ev));
+    return 0;
+}
+extern "C" int gm_event_destroy(void* ev) {
+    HIPCHK(hipEventDestroy((hipEvent_t)ev));
+    return 0;
+}
+extern "C" int gm_event_record(void* ev, void* s) {
+    HIPCHK(hipEventRecord((hipEvent_t)ev, (hipStream_t)s));
+    return 0;
+}
+extern "C" int gm_event_elapsed_ms(void* a, void* b, float* ms) {
+    HIPCHK(hipEventSynchronize((hipEvent_t)b));
+    HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Helpers shared by the single operators and the engine
+// ---------------------------------------------------------------------------
+
+static int64_t grid_nn(const gm_grid& g) { return (int64_t)g.n[0] * g.n[1] * g.n[2]; }
+
+// Truncation toward zero of (v/h + 0.01) in FP32 (cF:1716-1718).
+static int shift_of(float v, float h) { return (int)truncf(v / h + 0.01f); }
+
+// Bound on the coarse elements a fine mesh can cover along one axis.
+static int cover_bound(const gm_grid& c, const gm_grid& f, int d) {
+    const int nec = c.n[d] - 1;
+    const double ext = (double)(f.n[d] - 1) * f.h[d];
+    const int b = (int)ceil(ext / c.h[d]) + 2;
+    return std::max(1, std::min(nec, b));
+}
+
+struct PairMem {
+    GmPair p{};
+    void* blob = nullptr;
+};
+
+static int pair_alloc(PairMem& pm, int coarse, int fine, const gm_grid& gc, const gm_grid& gf) {
+    GmPair& p = pm.p;
+    p.coarse = coarse;
+    p.fine = fine;
+    size_t bytes = 0;
+    for (int d = 0; d < 3; ++d) {
+        p.nef[d] = gf.n[d] - 1;
+        p.cmax[d] = cover_bound(gc, gf, d);
+        bytes += (size_t)p.nef[d] * 4 + (size_t)p.nef[d] * 2 * 16 + (size_t)(2 + 2 * p.cmax[d]) * 4;
+        bytes += 64;
+    }
+    HIPCHK(hipMalloc(&pm.blob, bytes));
+    HIPCHK(hipMemset(pm.blob, 0, bytes));
+    char* b = (char*)pm.blob;
+    for (int d = 0; d < 3; ++d) {
+        p.ax[d].q = (float4*)b;
+        b += (size_t)p.nef[d] * 2 * 16;
+        p.ax[d].c0 = (int32_t*)b;
+        b += (size_t)p.nef[d] * 4;
+        p.ax[d].info = (int32_t*)b;
+        b += (size_t)(2 + 2 * p.cmax[d]) * 4;
+        b = (char*)(((uintptr_t)b + 63) & ~(uintptr_t)63);
+    }
+    return 0;
+}
+
+static int64_t pair_nodes(const GmPair& p) {
+    return (int64_t)(p.cmax[0] + 1) * (p.cmax[1] + 1) * (p.cmax[2] + 1);
+}
+static int64_t pair_elems(const GmPair& p) { return (int64_t)p.cmax[0] * p.cmax[1] * p.cmax[2]; }
+static int pair_gmax(const GmPair& p) {
+    return std::max(p.cmax[0], std::max(p.cmax[1], p.cmax[2])) + 1;
+}
+
+static LSArgs ls_default() {
+    LSArgs a;
+    memset(&a, 0, sizeof(a));
+    a.use_desc = 1;
+    a.dt_mode = GM_DT_VALUE;
+    a.src_row = -1;
+    a.alpha = 1.0f;
+    return a;
+}
+
+static void ls_coeffs(LSArgs& a, const gm_grid& g) { level_coeffs(g, a.kc, &a.mrow, &a.w2, &a.w3); }
+
+// ---------------------------------------------------------------------------
+// Single operators
+// ---------------------------------------------------------------------------
+
+extern "C" int gm_state_properties(const gm_material* mat, const float* T, const float* S1,
+                                   int64_t nn, int64_t nsub, float* S1o, uint8_t* S2o, float* k,
+                                   float* c, void* stream) {
+    launch_props(make_devmat(*mat), T, S1, nn, nsub, S1o, S2o, k, c, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_solve_matrix_free(const gm_grid* grid, int64_t ne_active, const float* k,
+                                    const float* rhocp, float dt, const float* T, const float* F,
+                                    const float* Corr, float* T_out, void* stream) {
+    LSArgs a = ls_default();
+    a.use_desc = 0;
+    a.grid = *grid;
+    a.T = T;
+    a.kin = k;
+    a.cin = rhocp;
+    a.Tout = T_out;
+    a.ne_active = ne_active;
+    a.nn_active = grid_nn(*grid);
+    a.dt = dt;
+    a.Fext = F;
+    a.corr_full = Corr;
+    ls_coeffs(a, *grid);
+    gm_material dummy;
+    memset(&dummy, 0, sizeof(dummy));
+    launch_level_step(a, nullptr, *grid, make_devmat(dummy), (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_source_l3(const gm_grid* g, const gm_material* mat, const float* v, float P,
+                            float* F, void* stream) {
+    launch_source_l3_full(*g, make_devmat(*mat), v[0], v[1], v[2], P, F, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_conv_rad_bc(const gm_grid* g, const gm_material* mat, const float* T,
+                              int64_t ne_active, float* F, void* stream) {
+    launch_convrad_full(*g, make_devmat(*mat), T, ne_active, F, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_interpolate_points(const gm_grid* src, const float* u, const gm_grid* tgt,
+                                     float* out, void* stream) {
+    launch_interp_grid(nullptr, -1, src, u, nullptr, -1, tgt, grid_nn(*tgt), out,
+                       (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_interpolate_matrix(const gm_grid* src, const gm_grid* tgt, float* Nc,
+                                     int32_t* node, void* stream) {
+    launch_interp_matrix(*src, *tgt, Nc, node, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_interpolate_w_matrix(const float* Nc, const int32_t* node, int64_t n,
+                                       const float* T, float* out, void* stream) {
+    launch_interp_w(Nc, node, n, T, out, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// A temporary descriptor holding explicit grids, for the single operators
+// whose kernels read meshes from a descriptor.
+struct TmpDesc {
+    GmStepDesc* d = nullptr;
+    ~TmpDesc() {
+        if (d) (void)hipFree(d);
+    }
+};
+
+static int tmp_desc(TmpDesc& t, const gm_grid* g1, const gm_grid* g2, const gm_grid* g3,
+                    const GmStepDesc* extra, hipStream_t s) {
+    GmStepDesc h;
+    if (extra) h = *extra;
+    else memset(&h, 0, sizeof(h));
+    if (g1) h.g[1] = *g1;
+    if (g2) h.g[2] = *g2;
+    if (g3) h.g[3] = *g3;
+    HIPCHK(hipMalloc(&t.d, sizeof(GmStepDesc)));
+    HIPCHK(hipMemcpyAsync(t.d, &h, sizeof(GmStepDesc), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int gm_eval_tables(const LSArgs& a, const gm_grid& g, float* F, hipStream_t s);
+
+extern "C" int gm_level_source(const gm_grid* coarse, const gm_grid* fine, const gm_material* mat,
+                               const float* rows, int npos, float* F_out, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    TmpDesc td;
+    if (tmp_desc(td, coarse, fine, nullptr, nullptr, s)) return -1;
+    PairMem pm;
+    if (pair_alloc(pm, 1, 2, *coarse, *fine)) return -1;
+    float* rows_d = nullptr;
+    float* G = nullptr;
+    const int nmax = pair_gmax(pm.p);
+    HIPCHK(hipMalloc(&rows_d, (size_t)npos * 7 * 4));
+    HIPCHK(hipMalloc(&G, (size_t)npos * 3 * nmax * 4));
+    HIPCHK(hipMemcpyAsync(rows_d, rows, (size_t)npos * 7 * 4, hipMemcpyHostToDevice, s));
+    launch_axis_tables(pm.p, td.d, s);
+    launch_source_tables(pm.p, td.d, make_devmat(*mat), 0, npos, 1, rows_d, G, nmax, s);
+    LSArgs a = ls_default();
+    a.G = G;
+    a.G_npos = npos;
+    a.G_nmax = nmax;
+    float kc[8], mrow, w2, w3f;
+    level_coeffs(*fine, kc, &mrow, &w2, &w3f);
+    a.G_scale = w3f / (float)npos;
+    for (int d = 0; d < 3; ++d) a.G_info[d] = pm.p.ax[d].info;
+    const int rc = gm_eval_tables(a, *coarse, F_out, s);
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipFree(rows_d));
+    HIPCHK(hipFree(G));
+    HIPCHK(hipFree(pm.blob));
+    return rc;
+}
+
+// Evaluate F_A = G_scale * sum_pos Gx Gy Gz on a full coarse array.
+__global__ void k_eval_tables(LSArgs a, gm_grid g, float* F) {
+    const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ix = t % g.n[0], iy = (t / g.n[0]) % g.n[1], iz = t / ((int64_t)g.n[0] * g.n[1]);
+    const int bx = ix - a.G_info[0][0], by = iy - a.G_info[1][0], bz = iz - a.G_info[2][0];
+    float v = 0.0f;
+    if (bx >= 0 && bx <= a.G_info[0][1] && by >= 0 && by <= a.G_info[1][1] && bz >= 0 &&
+        bz <= a.G_info[2][1]) {
+        float s = 0.0f;
+        for (int p = 0; p < a.G_npos; ++p) {
+            const float* Gp = a.G + (int64_t)p * 3 * a.G_nmax;
+            s += Gp[bx] * Gp[a.G_nmax + by] * Gp[2 * a.G_nmax + bz];
+        }
+        v = s * a.G_scale;
+    }
+    F[t] = v;
+}
+
+int gm_eval_tables(const LSArgs& a, const gm_grid& g, float* F, hipStream_t s) {
+    const int64_t n = grid_nn(g);
+    hipLaunchKernelGGL(k_eval_tables, dim3((n + 255) / 256), dim3(256), 0, s, a, g, F);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Expand a node-box array of a pair onto the full coarse array.
+__global__ void k_box_to_full(GmPair p, const float* __restrict__ V, gm_grid g, float* out) {
+    const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ix = t % g.n[0], iy = (t / g.n[0]) % g.n[1], iz = t / ((int64_t)g.n[0] * g.n[1]);
+    const int bx = ix - p.ax[0].info[0], by = iy - p.ax[1].info[0], bz = iz - p.ax[2].info[0];
+    const int mx = p.cmax[0] + 1, my = p.cmax[1] + 1;
+    float v = 0.0f;
+    if (bx >= 0 && bx <= p.ax[0].info[1] && by >= 0 && by <= p.ax[1].info[1] && bz >= 0 &&
+        bz <= p.ax[2].info[1])
+        v = V[bx + mx * (by + my * bz)];
+    out[t] += v;
+}
+
+extern "C" int gm_restrict(const gm_grid* coarse, const gm_grid* fine, const gm_material* mat,
+                           const float* T, const float* S1, int64_t nsub, const float* Tp,
+                           const float* TpA, const float* TpB, float dt, float* V_out,
+                           void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    TmpDesc td;
+    if (tmp_desc(td, coarse, fine, nullptr, nullptr, s)) return -1;
+    PairMem pm;
+    if (pair_alloc(pm, 1, 2, *coarse, *fine)) return -1;
+    double* part = nullptr;
+    float* V = nullptr;
+    HIPCHK(hipMalloc(&part, pair_elems(pm.p) * 8 * sizeof(double)));
+    HIPCHK(hipMalloc(&V, pair_nodes(pm.p) * 4));
+    launch_axis_tables(pm.p, td.d, s);
+    RArgs r;
+    memset(&r, 0, sizeof(r));
+    r.T = T;
+    r.S1f = S1;
+    r.n_sub = nsub;
+    r.Tp = Tp;
+    r.TpA = TpA;
+    r.TpB = TpB;
+    r.dt = dt;
+    r.dt_mode = GM_DT_VALUE;
+    float kc[8], mrow, w2, w3;
+    level_coeffs(*fine, kc, &mrow, &w2, &w3);
+    r.wf = w3;
+    launch_restrict(pm.p, td.d, make_devmat(*mat), r, part, s);
+    launch_restrict_nodes(pm.p, part, nullptr, nullptr, nullptr, V, s);
+    const int64_t n = grid_nn(*coarse);
+    hipLaunchKernelGGL(k_box_to_full, dim3((n + 255) / 256), dim3(256), 0, s, pm.p, V, *coarse,
+                       V_out);
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipFree(part));
+    HIPCHK(hipFree(V));
+    HIPCHK(hipFree(pm.blob));
+    return 0;
+}
+
+extern "C" int gm_new_tprime(const gm_grid* fine, const float* FineT, const gm_grid* coarse,
+                             float* CoarseT, const int32_t* ov_lo, const int32_t* ov_n, float* Tp,
+                             void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    GmStepDesc h;
+    memset(&h, 0, sizeof(h));
+    for (int d = 0; d < 3; ++d) {
+        h.ov3.lo[d] = ov_lo[d];
+        h.ov3.n[d] = ov_n[d];
+        h.ov3_xyz.n[d] = ov_n[d];
+        h.ov3_xyz.x0[d] = coarse->x0[d] + coarse->h[d] * (float)ov_lo[d];
+        h.ov3_xyz.h[d] = coarse->h[d];
+    }
+    TmpDesc td;
+    if (tmp_desc(td, nullptr, coarse, fine, &h, s)) return -1;
+    launch_inject(td.d, 3, 2, FineT, CoarseT, (int64_t)ov_n[0] * ov_n[1] * ov_n[2], s);
+    launch_tprime(td.d, 3, 2, FineT, CoarseT, Tp, grid_nn(*fine), s);
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The engine
+// ---------------------------------------------------------------------------
+
+struct gm_engine {
+    gm_config cfg;
+    gm_material mat;
+    DevMat dm;
+    hipStream_t s;
+    int64_t nn[4];
+    int N2, N3, N23;
+    // level operator constants
+    float kc[4][8], mrow[4], w2[4], w3[4];
+    // state
+    float *T1 = nullptr, *S11 = nullptr, *S1st = nullptr;
+    int nst = 0;
+    float *T2 = nullptr, *Tp2 = nullptr;
+    uint8_t* S12 = nullptr;
+    float *T3 = nullptr, *Tp3 = nullptr;
+    uint8_t *S13 = nullptr, *S23 = nullptr;
+    uint8_t* L0S1 = nullptr;
+    float *acc = nullptr, *macc = nullptr;
+    uint8_t* s2snap = nullptr;
+    int32_t* s2box = nullptr;
+    // scratch
+    float *T1p = nullptr, *T1c = nullptr;
+    float *V1 = nullptr, *V1m = nullptr, *V2 = nullptr;
+    float *G1 = nullptr, *G2 = nullptr;
+    float* T2b[2] = {nullptr, nullptr};
+    uint8_t* S12b[2] = {nullptr, nullptr};
+    float* Tp2x = nullptr;
+    std::vector<float*> L2all, L3all, histTp;
+    float* T3b[2] = {nullptr, nullptr};
+    uint8_t* S13b[2] = {nullptr, nullptr};
+    uint8_t* S23b[2] = {nullptr, nullptr};
+    float* Tp3x = nullptr;
+    uint8_t* preS2 = nullptr;
+    float* mm = nullptr;
+    PairMem pair[3];
+    double* part[3] = {nullptr, nullptr, nullptr};
+    int gmax[3];
+    // descriptor
+    GmStepDesc host;
+    GmStepDesc* d_desc = nullptr;
+    GmStepDesc* ring[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ring_ev[4];
+    int ring_k = 0;
+    // window state
+    int s3[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+    // graph
+    bool use_graph = true;
+    bool graph_ok = false;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<void*> allocs;
+};
+
+static int eng_alloc(gm_engine* e, void** p, size_t bytes) {
+    HIPCHK(hipMalloc(p, bytes ? bytes : 1));
+    e->allocs.push_back(*p);
+    return 0;
+}
+
+#define EALLOC(ptr, bytes)                                                     \
+    do {                                                                       \
+        if (eng_alloc(e, (void**)&(ptr), (size_t)(bytes))) return -1;          \
+    } while (0)
+
+static int upload_desc(gm_engine* e, size_t bytes) {
+    e->ring_k = (e->ring_k + 1) & 3;
+    HIPCHK(hipEventSynchronize(e->ring_ev[e->ring_k]));
+    memcpy(e->ring[e->ring_k], &e->host, bytes);
+    HIPCHK(hipMemcpyAsync(e->d_desc, e->ring[e->ring_k], bytes, hipMemcpyHostToDevice, e->s));
+    HIPCHK(hipEventRecord(e->ring_ev[e->ring_k], e->s));
+    return 0;
+}
+
+static size_t desc_bytes(int nrows) {
+    return offsetof(GmStepDesc, rows) + (size_t)std::max(nrows, 1) * 7 * sizeof(float);
+}
+
+extern "C" int gm_engine_destroy(gm_engine* e) {
+    if (!e) return 0;
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    for (void* p : e->allocs) (void)hipFree(p);
+    for (int i = 0; i < 3; ++i)
+        if (e->pair[i].blob) (void)hipFree(e->pair[i].blob);
+    for (int i = 0; i < 4; ++i) {
+        if (e->ring[i]) (void)hipHostFree(e->ring[i]);
+        (void)hipEventDestroy(e->ring_ev[i]);
+    }
+    delete e;
+    return 0;
+}
+
+extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, void* stream,
+                                gm_engine** out) {
+    gm_engine* e = new gm_engine();
+    *out = nullptr;
+    e->cfg = *cfg;
+    e->mat = *mat;
+    e->dm = make_devmat(*mat);
+    e->s = (hipStream_t)stream;
+    e->N2 = cfg->n2;
+    e->N3 = cfg->n3;
+    e->N23 = cfg->n2 * cfg->n3;
+    if (e->N23 > GM_MAX_ROWS || e->N2 > GM_MAX_N2) {
+        delete e;
+        return fail("subcycle counts exceed the engine limits (N2 <= 64, N2*N3 <= 1024)");
+    }
+    for (int l = 0; l < 4; ++l) e->nn[l] = grid_nn(cfg->level[l]);
+    for (int l = 1; l < 4; ++l) level_coeffs(cfg->level[l], e->kc[l], &e->mrow[l], &e->w2[l], &e->w3[l]);
+    const int64_t n1 = e->nn[1], n2 = e->nn[2], n3 = e->nn[3], n0 = e->nn[0];
+    e->nst = std::max(1, (int)lround((double)cfg->level[1].h[2] / cfg->layer_height));
+    auto bad = [&](int rc) {
+        gm_engine_destroy(e);
+        return rc;
+    };
+    if (eng_alloc(e, (void**)&e->T1, n1 * 4) || eng_alloc(e, (void**)&e->S11, n1 * 4) ||
+        eng_alloc(e, (void**)&e->S1st, (size_t)e->nst * n1 * 4) ||
+        eng_alloc(e, (void**)&e->T2, n2 * 4) || eng_alloc(e, (void**)&e->Tp2, n2 * 4) ||
+        eng_alloc(e, (void**)&e->S12, n2) || eng_alloc(e, (void**)&e->T3, n3 * 4) ||
+        eng_alloc(e, (void**)&e->Tp3, n3 * 4) || eng_alloc(e, (void**)&e->S13, n3) ||
+        eng_alloc(e, (void**)&e->S23, n3) || eng_alloc(e, (void**)&e->L0S1, n0) ||
+        eng_alloc(e, (void**)&e->acc, n0 * 4) || eng_alloc(e, (void**)&e->macc, n0 * 4) ||
+        eng_alloc(e, (void**)&e->s2snap, n3) || eng_alloc(e, (void**)&e->s2box, 64) ||
+        eng_alloc(e, (void**)&e->T1p, n1 * 4) || eng_alloc(e, (void**)&e->T1c, n1 * 4) ||
+        eng_alloc(e, (void**)&e->Tp2x, n2 * 4) || eng_alloc(e, (void**)&e->Tp3x, n3 * 4) ||
+        eng_alloc(e, (void**)&e->preS2, n3) || eng_alloc(e, (void**)&e->mm, 64) ||
+        eng_alloc(e, (void**)&e->d_desc, sizeof(GmStepDesc)))
+        return bad(-1);
+    for (int i = 0; i < 2; ++i) {
+        if (eng_alloc(e, (void**)&e->T2b[i], n2 * 4) || eng_alloc(e, (void**)&e->S12b[i], n2) ||
+            eng_alloc(e, (void**)&e->T3b[i], n3 * 4) || eng_alloc(e, (void**)&e->S13b[i], n3) ||
+            eng_alloc(e, (void**)&e->S23b[i], n3))
+            return bad(-1);
+    }
+    e->L2all.resize(e->N2);
+    e->L3all.resize(e->N2);
+    e->histTp.resize(e->N2);
+    for (int i = 0; i < e->N2; ++i) {
+        if (eng_alloc(e, (void**)&e->L2all[i], n2 * 4) ||
+            eng_alloc(e, (void**)&e->L3all[i], n3 * 4) ||
+            eng_alloc(e, (void**)&e->histTp[i], n3 * 4))
+            return bad(-1);
+    }
+    // fine -> coarse pairs in the reference's Shapes order: L2->L1, L3->L1, L3->L2
+    const int pc[3] = {1, 1, 2}, pf[3] = {2, 3, 3};
+    for (int i = 0; i < 3; ++i) {
+        if (pair_alloc(e->pair[i], pc[i], pf[i], cfg->level[pc[i]], cfg->level[pf[i]]))
+            return bad(-1);
+        if (eng_alloc(e, (void**)&e->part[i], pair_elems(e->pair[i].p) * 8 * sizeof(double)))
+            return bad(-1);
+        e->gmax[i] = pair_gmax(e->pair[i].p);
+    }
+    if (eng_alloc(e, (void**)&e->V1, pair_nodes(e->pair[0].p) * 4) ||
+        eng_alloc(e, (void**)&e->V1m, pair_nodes(e->pair[0].p) * 4) ||
+        eng_alloc(e, (void**)&e->V2, pair_nodes(e->pair[2].p) * 4) ||
+        eng_alloc(e, (void**)&e->G1, (size_t)e->N23 * 3 * e->gmax[1] * 4) ||
+        eng_alloc(e, (void**)&e->G2, (size_t)e->N23 * 3 * e->gmax[2] * 4))
+        return bad(-1);
+    for (int i = 0; i < 4; ++i) {
+        if (hipHostMalloc((void**)&e->ring[i], sizeof(GmStepDesc), hipHostMallocDefault) !=
+            hipSuccess)
+            return bad(fail("hipHostMalloc of the step descriptor failed"));
+        if (hipEventCreateWithFlags(&e->ring_ev[i], hipEventDisableTiming) != hipSuccess)
+            return bad(fail("hipEventCreate failed"));
+        HIPCHK(hipEventRecord(e->ring_ev[i], e->s));
+    }
+    // initial state (SetupLevels, cF:150-157): T = T_amb, S1 = 0
+    launch_fill_f32(e->T1, mat->T_amb, n1, e->s);
+    launch_fill_f32(e->T2, mat->T_amb, n2, e->s);
+    launch_fill_f32(e->T3, mat->T_amb, n3, e->s);
+    HIPCHK(hipMemsetAsync(e->S11, 0, n1 * 4, e->s));
+    HIPCHK(hipMemsetAsync(e->S1st, 0, (size_t)e->nst * n1 * 4, e->s));
+    HIPCHK(hipMemsetAsync(e->Tp2, 0, n2 * 4, e->s));
+    HIPCHK(hipMemsetAsync(e->Tp3, 0, n3 * 4, e->s));
+    HIPCHK(hipMemsetAsync(e->S12, 0, n2, e->s));
+    HIPCHK(hipMemsetAsync(e->S13, 0, n3, e->s));
+    HIPCHK(hipMemsetAsync(e->S23, 0, n3, e->s));
+    HIPCHK(hipMemsetAsync(e->L0S1, 0, n0, e->s));
+    HIPCHK(hipMemsetAsync(e->acc, 0, n0 * 4, e->s));
+    HIPCHK(hipMemsetAsync(e->macc, 0, n0 * 4, e->s));
+    HIPCHK(hipMemsetAsync(e->s2box, 0, 64, e->s));
+    // descriptor: initial meshes and overlap boxes
+    GmStepDesc& h = e->host;
+    memset(&h, 0, sizeof(h));
+    for (int l = 0; l < 4; ++l) h.g[l] = h.gprev[l] = cfg->level[l];
+    for (int d = 0; d < 3; ++d) {
+        h.ov3.lo[d] = cfg->ov_lo[3][d];
+        h.ov3.n[d] = cfg->ov_n[3][d];
+        h.ov3_xyz.n[d] = cfg->ov_n[3][d];
+        h.ov3_xyz.x0[d] = cfg->ov_x0[3][d];
+        h.ov3_xyz.h[d] = cfg->level[2].h[d];
+        h.ov2.lo[d] = cfg->ov_lo[2][d];
+        h.ov2.n[d] = cfg->ov_n[2][d];
+        h.ov2_xyz.n[d] = cfg->ov_n[2][d];
+        h.ov2_xyz.x0[d] = cfg->ov_x0[2][d];
+        h.ov2_xyz.h[d] = cfg->level[1].h[d];
+        h.l0_lo3[d] = cfg->ov_lo[0][d];
+        h.l0_lo2[d] = cfg->ov_lo[1][d];
+        h.l0_st2[d] = cfg->l0_stride_l2[d];
+        h.l0_n[d] = cfg->level[0].n[d];
+    }
+    h.l1_ne_active = (int64_t)(cfg->level[1].n[0] - 1) * (cfg->level[1].n[1] - 1) *
+                     (cfg->level[1].n[2] - 1);
+    h.l1_nn_active = n1;
+    HIPCHK(hipStreamSynchronize(e->s));
+    if (upload_desc(e, desc_bytes(0))) return bad(-1);
+    for (int i = 0; i < 3; ++i) launch_axis_tables(e->pair[i].p, e->d_desc, e->s);
+    HIPCHK(hipStreamSynchronize(e->s));
+    HIPCHK(hipGetLastError());
+    *out = e;
+    return 0;
+}
+
+extern "C" int gm_engine_field(gm_engine* e, int level, int field, void** ptr, int64_t* count) {
+    void* p = nullptr;
+    int64_t n = (level >= 0 && level < 4) ? e->nn[level] : 0;
+    if (field >= GM_HIST_L2 && field <= GM_HIST_TP3) {
+        if (level < 0 || level >= e->N2) return fail("history index out of range");
+        p = field == GM_HIST_L2 ? (void*)e->L2all[level]
+            : field == GM_HIST_L3 ? (void*)e->L3all[level] : (void*)e->histTp[level];
+        *ptr = p;
+        *count = field == GM_HIST_L2 ? e->nn[2] : e->nn[3];
+        return 0;
+    }
+    switch (field) {
+        case GM_T0: p = level == 1 ? (void*)e->T1 : level == 2 ? (void*)e->T2 : level == 3 ? (void*)e->T3 : nullptr; break;
+        case GM_TPRIME0: p = level == 2 ? (void*)e->Tp2 : level == 3 ? (void*)e->Tp3 : nullptr; break;
+        case GM_S1: p = level == 0 ? (void*)e->L0S1 : level == 1 ? (void*)e->S11 : level == 2 ? (void*)e->S12 : (void*)e->S13; break;
+        case GM_S2: p = level == 3 ? (void*)e->S23 : nullptr; break;
+        case GM_ACCUM: p = level == 0 ? (void*)e->acc : nullptr; break;
+        case GM_MAX_ACCUM: p = level == 0 ? (void*)e->macc : nullptr; break;
+        default: break;
+    }
+    if (!p) return fail("no such field on this level");
+    *ptr = p;
+    *count = n;
+    return 0;
+}
+
+extern "C" int gm_engine_grid(gm_engine* e, int level, gm_grid* out) {
+    if (level < 0 || level > 3) return fail("level out of range");
+    *out = e->host.g[level];
+    return 0;
+}
+
+extern "C" int gm_engine_set_layer(gm_engine* e, int64_t ne_act, int64_t nn_act,
+                                   const int64_t* sub4) {
+    e->host.l1_ne_active = ne_act;
+    e->host.l1_nn_active = nn_act;
+    for (int i = 0; i < 4; ++i) e->host.sub[i] = sub4[i];
+    e->graph_ok = false;  // static launch arguments changed
+    return 0;
+}
+
+extern "C" int gm_engine_use_graph(gm_engine* e, int enable) {
+    e->use_graph = enable != 0;
+    e->graph_ok = false;
+    return 0;
+}
+
+// ---- moveEverything (cF:2400-2510) ----------------------------------------
+
+extern "C" int gm_engine_move(gm_engine* e, const float* v, const float* vstart,
+                              int32_t* move_v_out) {
+    const gm_config& c = e->cfg;
+    GmStepDesc& h = e->host;
+    float vtot[3];
+    for (int d = 0; d < 3; ++d) vtot[d] = v[d] - vstart[d];
+    float v3[3], v2[3];
+    for (int d = 0; d < 3; ++d) {
+        v3[d] = std::min(std::max(vtot[d], c.move_lo[1][d]), c.move_hi[1][d]);
+        v2[d] = std::min(std::max(vtot[d], c.move_lo[0][d]), c.move_hi[0][d]);
+    }
+    for (int l = 0; l < 4; ++l) h.gprev[l] = h.g[l];
+    const float h2[3] = {c.level[2].h[0], c.level[2].h[1], c.level[2].h[2]};
+    const float h1[3] = {c.level[1].h[0], c.level[1].h[1], c.level[1].h[2]};
+    const float hL1[3] = {h1[0], h1[1], c.layer_height};
+    int s3[3], s2[3], mv[3];
+    for (int d = 0; d < 3; ++d) {
+        s3[d] = shift_of(v3[d], h2[d]);
+        s2[d] = shift_of(v2[d], hL1[d]);
+        mv[d] = s2[d] * c.ratio12[d];
+    }
+    for (int d = 0; d < 3; ++d) {
+        h.g[3].x0[d] = c.level[3].x0[d] + h2[d] * (float)s3[d];
+        h.g[2].x0[d] = c.level[2].x0[d] + hL1[d] * (float)s2[d];
+    }
+    // L3 overlap (L2 nodes inside L3), cF:1727-1764 then cF:2472-2474
+    for (int d = 0; d < 3; ++d) {
+        h.ov3.lo[d] = c.ov_lo[3][d] + s3[d] - mv[d];
+        h.ov3_xyz.x0[d] = c.ov_x0[3][d] + h2[d] * (float)s3[d];
+    }
+    // L2 overlap (L1 nodes inside L2), cF:1767-1804
+    for (int d = 0; d < 3; ++d) {
+        const int sh = shift_of(v2[d], h1[d]);
+        h.ov2.lo[d] = c.ov_lo[2][d] + sh;
+        if (d < 2) h.ov2_xyz.x0[d] = c.ov_x0[2][d] + h1[d] * (float)sh;
+    }
+    h.ov2_xyz.x0[2] = c.ov_x0[2][2] + c.layer_height * (float)shift_of(v2[2], c.layer_height);
+    // Level0 boxes, cF:2477-2497
+    const float h0z[3] = {h1[0], h1[1], h2[2]};
+    const int r2[3] = {c.ratio12[0] * c.ratio23[0], c.ratio12[1] * c.ratio23[1], c.ratio23[2]};
+    for (int d = 0; d < 3; ++d) {
+        h.l0_lo3[d] = c.ov_lo[0][d] + c.ratio23[d] * shift_of(v3[d], h2[d]);
+        h.l0_lo2[d] = c.ov_lo[1][d] + r2[d] * shift_of(v2[d], h0z[d]);
+    }
+    h.l0_lo3[2] -= mv[2] * c.ratio23[2];
+    h.l0_lo2[2] -= mv[2] * c.ratio23[2];
+    for (int d = 0; d < 3; ++d) {
+        e->s3[d] = s3[d];
+        e->s2[d] = s2[d];
+        if (move_v_out) move_v_out[d] = mv[d];
+    }
+    if (upload_desc(e, desc_bytes(0))) return -1;
+    const int64_t n3 = e->nn[3], n2 = e->nn[2];
+    launch_move_l3(e->d_desc, e->T1, e->Tp2, e->Tp3, e->T3b[0], e->Tp3x, n3, e->s);
+    launch_move_l2(e->d_desc, e->T1, e->Tp2, e->T2b[0], e->Tp2x, n2, e->s);
+    launch_copy_f32(e->T3, e->T3b[0], n3, e->s);
+    launch_copy_f32(e->Tp3, e->Tp3x, n3, e->s);
+    launch_copy_f32(e->T2, e->T2b[0], n2, e->s);
+    launch_copy_f32(e->Tp2, e->Tp2x, n2, e->s);
+    for (int i = 0; i < 3; ++i) launch_axis_tables(e->pair[i].p, e->d_desc, e->s);
+    launch_l0_gather(e->d_desc, e->L0S1, e->s2snap, e->s2box, e->S13, e->S23, e->S12, n3, n2,
+                     e->s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---- level-step helpers -----------------------------------------------------
+
+// L1 solve (computeL1Temperature cF:2813-2854 + clamp)
+static void run_l1(gm_engine* e, const float* F_G, int npos, const float* corr, float* Tout,
+                   int dt_mode, float dt, int clamp) {
+    LSArgs a = ls_default();
+    a.level = 1;
+    a.T = e->T1;
+    a.S1 = e->S11;
+    a.s1_float = 1;
+    a.Tout = Tout;
+    a.n_sub = e->host.sub[1];
+    a.ne_active = e->host.l1_ne_active;
+    a.nn_active = e->host.l1_nn_active;
+    a.dt_mode = dt_mode;
+    a.dt = dt;
+    if (F_G) {
+        a.src_mode = GM_SRC_TABLES;
+        a.G = F_G;
+        a.G_npos = npos;
+        a.G_nmax = e->gmax[1];
+        a.G_scale = e->w3[3] / (float)npos;
+        for (int d = 0; d < 3; ++d) a.G_info[d] = e->pair[1].p.ax[d].info;
+    }
+    a.conv = 1;
+    if (corr) {
+        a.corr = corr;
+        for (int d = 0; d < 3; ++d) {
+            a.corr_info[d] = e->pair[0].p.ax[d].info;
+            a.corr_nmax[d] = e->pair[0].p.cmax[d] + 1;
+        }
+    }
+    a.bc_mode = GM_BC_L1;
+    for (int i = 0; i < 5; ++i) a.bc_vals[i] = e->cfg.l1_bc[i];
+    a.clamp = clamp;
+    memcpy(a.kc, e->kc[1], sizeof(a.kc));
+    a.mrow = e->mrow[1];
+    a.w2 = e->w2[1];
+    a.w3 = e->w3[1];
+    launch_level_step(a, e->d_desc, e->cfg.level[1], e->dm, e->s);
+}
+
+// L2 solve (computeL2Temperature cF:2917-2957 + clamp)
+static void run_l2(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
+                   const float* G, int npos, const float* corr, const float* bcA, const float* bcB,
+                   float alpha, float* Tout, int dt_mode, int dt_index, float dt, int clamp) {
+    LSArgs a = ls_default();
+    a.level = 2;
+    a.T = Tin;
+    a.S1 = S1in;
+    a.S1out = S1out;
+    a.Tout = Tout;
+    a.n_sub = e->host.sub[2];
+    a.ne_active = (int64_t)(e->cfg.level[2].n[0] - 1) * (e->cfg.level[2].n[1] - 1) *
+                  (e->cfg.level[2].n[2] - 1);
+    a.nn_active = e->nn[2];
+    a.dt_mode = dt_mode;
+    a.dt_index = dt_index;
+    a.dt = dt;
+    a.src_mode = GM_SRC_TABLES;
+    a.G = G;
+    a.G_npos = npos;
+    a.G_nmax = e->gmax[2];
+    a.G_scale = e->w3[3] / (float)npos;
+    for (int d = 0; d < 3; ++d) a.G_info[d] = e->pair[2].p.ax[d].info;
+    a.conv = 1;
+    a.corr = corr;
+    for (int d = 0; d < 3; ++d) {
+        a.corr_info[d] = e->pair[2].p.ax[d].info;
+        a.corr_nmax[d] = e->pair[2].p.cmax[d] + 1;
+    }
+    a.bc_mode = GM_BC_FINE;
+    a.bc_level = 1;
+    a.bcA = bcA;
+    a.bcB = bcB;
+    a.alpha = alpha;
+    a.clamp = clamp;
+    memcpy(a.kc, e->kc[2], sizeof(a.kc));
+    a.mrow = e->mrow[2];
+    a.w2 = e->w2[2];
+    a.w3 = e->w3[2];
+    launch_level_step(a, e->d_desc, e->cfg.level[2], e->dm, e->s);
+}
+
+// L3 solve (computeSolutions_L3 cF:3015-3054 + clamp [+ melt time])
+static void run_l3(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out, int row,
+                   const float* bcA, const float* bcB, float alpha, float* Tout, int clamp,
+                   const uint8_t* S2prev, uint8_t* S2out) {
+    LSArgs a = ls_default();
+    a.level = 3;
+    a.T = Tin;
+    a.S1 = S1in;
+    a.S1out = S1out;
+    a.Tout = Tout;
+    a.n_sub = e->host.sub[3];
+    a.ne_active = (int64_t)(e->cfg.level[3].n[0] - 1) * (e->cfg.level[3].n[1] - 1) *
+                  (e->cfg.level[3].n[2] - 1);
+    a.nn_active = e->nn[3];
+    a.dt_mode = GM_DT_ROW;
+    a.dt_index = row;
+    a.src_mode = GM_SRC_L3;
+    a.src_row = row;
+    a.conv = 1;
+    a.bc_mode = GM_BC_FINE;
+    a.bc_level = 2;
+    a.bcA = bcA;
+    a.bcB = bcB;
+    a.alpha = alpha;
+    a.clamp = clamp;
+    if (S2out) {
+        a.accum_on = 1;
+        a.S2prev = S2prev;
+        a.S2out = S2out;
+        a.accum = e->acc;
+        a.max_accum = e->macc;
+    }
+    memcpy(a.kc, e->kc[3], sizeof(a.kc));
+    a.mrow = e->mrow[3];
+    a.w2 = e->w2[3];
+    a.w3 = e->w3[3];
+    launch_level_step(a, e->d_desc, e->cfg.level[3], e->dm, e->s);
+}
+
+static RArgs rargs(gm_engine* e, int fine, const float* T, const uint8_t* S1, const float* Tp,
+                   const float* TpA, const float* TpB, int dt_mode, int dt_index, float dt) {
+    RArgs r;
+    memset(&r, 0, sizeof(r));
+    r.T = T;
+    r.S1 = S1;
+    r.n_sub = e->host.sub[fine];
+    r.Tp = Tp;
+    r.TpA = TpA;
+    r.TpB = TpB;
+    r.dt_mode = dt_mode;
+    r.dt_index = dt_index;
+    r.dt = dt;
+    r.wf = e->w3[fine];
+    return r;
+}
+
+// getNewTprime (cF:2060-2099) for fine level f in {2, 3}.
+static void new_tprime(gm_engine* e, int f, const float* fineT, float* coarseT, float* Tp) {
+    const GmBox& b = f == 3 ? e->host.ov3 : e->host.ov2;
+    const int64_t nb = (int64_t)b.n[0] * b.n[1] * b.n[2];
+    launch_inject(e->d_desc, f, f - 1, fineT, coarseT, nb, e->s);
+    launch_tprime(e->d_desc, f, f - 1, fineT, coarseT, Tp, e->nn[f], e->s);
+}
+
+static void l1_s1_update(gm_engine* e) {
+    const GmBox& b = e->host.ov2;
+    launch_l1_s1_from_l2(e->d_desc, e->S12, e->S11, (int64_t)b.n[0] * b.n[1] * b.n[2], e->s);
+    launch_fill_f32(e->S11, 1.0f, e->host.sub[1], e->s);
+}
+
+// ---- subcycleGOMELT (cF:3224-3632) -------------------------------------------
+
+static void subcycle_body(gm_engine* e) {
+    const int N2 = e->N2, N3 = e->N3, N23 = e->N23;
+    const int64_t n1 = e->nn[1], n2 = e->nn[2], n3 = e->nn[3];
+    GmStepDesc* d = e->d_desc;
+    hipStream_t s = e->s;
+    const float fN2 = (float)N2, fN3 = (float)N3;
+
+    // L1 S1 from the L2 overlap, then substrate (cF:3273-3278)
+    l1_s1_update(e);
+    // L1 source over all N2*N3 positions (cF:3285-3287)
+    launch_source_tables(e->pair[1].p, d, e->dm, 0, N23, 0, nullptr, e->G1, e->gmax[1], s);
+    // L2 sources per L2 substep (cF:3329-3331), shared by both passes
+    for (int k = 0; k < N2; ++k)
+        launch_source_tables(e->pair[2].p, d, e->dm, k * N3, N3, 0, nullptr,
+                             e->G2 + (size_t)k * N3 * 3 * e->gmax[2], e->gmax[2], s);
+    // L1 flux terms from L3 and L2 T' (computeL1TprimeTerms_Part1, cF:2733-2810)
+    launch_restrict(e->pair[1].p, d, e->dm,
+                    rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
+                    e->part[1], s);
+    launch_restrict(e->pair[0].p, d, e->dm,
+                    rargs(e, 2, e->T2, e->S12, e->Tp2, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
+                    e->part[0], s);
+    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], nullptr, e->V1, s);
+    // L1 predictor
+    run_l1(e, e->G1, N23, e->V1, e->T1p, GM_DT_L1, 0.0f, 1);
+
+    // ---- Part 1: L2 x N2 { L3 x N3 } -----------------------------------------
+    const float* T2cur = e->T2;
+    const uint8_t* S12cur = e->S12;
+    const float* T3cur = e->T3;
+    const float* Tp3cur = e->Tp3;
+    const uint8_t* S13cur = e->S13;
+    int b2 = 0, b3 = 0;
+    for (int k = 0; k < N2; ++k) {
+        const float* Gk = e->G2 + (size_t)k * N3 * 3 * e->gmax[2];
+        launch_restrict(e->pair[2].p, d, e->dm,
+                        rargs(e, 3, T3cur, S13cur, Tp3cur, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
+                        e->part[2], s);
+        launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2, s);
+        float* T2new = e->T2b[b2];
+        uint8_t* S12new = e->S12b[b2];
+        b2 ^= 1;
+        run_l2(e, T2cur, S12cur, S12new, Gk, N3, e->V2, e->T1p, e->T1, (float)(k + 1) / fN2,
+               T2new, GM_DT_L2, k, 0.0f, 1);
+        const float* T3in = T3cur;
+        const uint8_t* S13in = S13cur;
+        for (int j = 0; j < N3; ++j) {
+            float* T3o = e->T3b[b3];
+            uint8_t* S13o = e->S13b[b3];
+            b3 ^= 1;
+            run_l3(e, T3in, S13in, S13o, k * N3 + j, T2new, T2cur, (float)(j + 1) / fN3, T3o, 1,
+                   nullptr, nullptr);
+            T3in = T3o;
+            S13in = S13o;
+        }
+        new_tprime(e, 3, T3in, T2new, e->histTp[k]);
+        T2cur = T2new;
+        S12cur = S12new;
+        T3cur = T3in;
+        S13cur = S13in;
+        Tp3cur = e->histTp[k];
+    }
+    // L2 T' and L1 injection, then the L1 mass terms and corrector (cF:3433-3456)
+    new_tprime(e, 2, T2cur, e->T1p, e->Tp2x);
+    launch_restrict(e->pair[1].p, d, e->dm,
+                    rargs(e, 3, e->T3, e->S13, nullptr, Tp3cur, e->Tp3, GM_DT_L1, 0, 0.0f),
+                    e->part[1], s);
+    launch_restrict(e->pair[0].p, d, e->dm,
+                    rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_L1, 0, 0.0f),
+                    e->part[0], s);
+    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], e->V1, e->V1m, s);
+    run_l1(e, e->G1, N23, e->V1m, e->T1c, GM_DT_L1, 0.0f, 1);
+
+    // ---- Part 2: restart L2/L3 from the start-of-step state ------------------
+    T2cur = e->T2;
+    S12cur = e->S12;
+    T3cur = e->T3;
+    Tp3cur = e->Tp3;
+    S13cur = e->S13;
+    const uint8_t* S23cur = e->S23;
+    int b2s = 0, b3s = 0, bs2 = 0;
+    for (int k = 0; k < N2; ++k) {
+        const float* Gk = e->G2 + (size_t)k * N3 * 3 * e->gmax[2];
+        // flux + mass restriction onto L2 (cF:3496-3508)
+        launch_restrict(e->pair[2].p, d, e->dm,
+                        rargs(e, 3, T3cur, S13cur, Tp3cur, e->histTp[k], Tp3cur, GM_DT_L2, k, 0.0f),
+                        e->part[2], s);
+        launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2, s);
+        float* T2new = e->L2all[k];
+        uint8_t* S12new = e->S12b[b2s];
+        b2s ^= 1;
+        run_l2(e, T2cur, S12cur, S12new, Gk, N3, e->V2, e->T1c, e->T1, (float)(k + 1) / fN2,
+               T2new, GM_DT_L2, k, 0.0f, 1);
+        const float* T3in = T3cur;
+        const uint8_t* S13in = S13cur;
+        const uint8_t* S23in = S23cur;
+        for (int j = 0; j < N3; ++j) {
+            float* T3o = (j == N3 - 1) ? e->L3all[k] : e->T3b[b3s];
+            if (j != N3 - 1) b3s ^= 1;
+            uint8_t* S13o = e->S13b[bs2];
+            uint8_t* S23o = e->S23b[bs2];
+            bs2 ^= 1;
+            run_l3(e, T3in, S13in, S13o, k * N3 + j, T2new, T2cur, (float)(j + 1) / fN3, T3o, 1,
+                   S23in, S23o);
+            T3in = T3o;
+            S13in = S13o;
+            S23in = S23o;
+        }
+        new_tprime(e, 3, T3in, T2new, e->histTp[k]);
+        T2cur = T2new;
+        S12cur = S12new;
+        T3cur = T3in;
+        S13cur = S13in;
+        S23cur = S23in;
+        Tp3cur = e->histTp[k];
+    }
+    // commit the end-of-step L2/L3 state
+    launch_copy_f32(e->T2, T2cur, n2, s);
+    launch_copy_u8(e->S12, S12cur, n2, s);
+    launch_copy_f32(e->T3, T3cur, n3, s);
+    launch_copy_f32(e->Tp3, Tp3cur, n3, s);
+    launch_copy_u8(e->S13, S13cur, n3, s);
+    launch_copy_u8(e->S23, S23cur, n3, s);
+    // final L2 T' and L1 T0 (cF:3625-3627)
+    new_tprime(e, 2, e->T2, e->T1c, e->Tp2);
+    launch_copy_f32(e->T1, e->T1c, n1, s);
+    // Level0 S1/S2 (cF:3628-3630)
+    launch_l0_scatter(d, e->S13, e->S23, e->L0S1, e->s2snap, e->s2box, n3, s);
+}
+
+extern "C" int gm_engine_subcycle(gm_engine* e, const float* rows, int n_rows) {
+    if (n_rows != e->N23) return fail("subcycle needs exactly N2*N3 toolpath rows");
+    GmStepDesc& h = e->host;
+    h.nrows = n_rows;
+    float dt1 = 0.0f;
+    for (int i = 0; i < n_rows; ++i) {
+        for (int c = 0; c < 7; ++c) h.rows[i][c] = rows[7 * i + c];
+        dt1 += rows[7 * i + 5];
+    }
+    h.dt1 = dt1;
+    for (int k = 0; k < e->N2; ++k) {
+        float s2 = 0.0f;
+        for (int j = 0; j < e->N3; ++j) s2 += rows[7 * (k * e->N3 + j) + 5];
+        h.dt2[k] = s2;
+    }
+    if (upload_desc(e, desc_bytes(n_rows))) return -1;
+    if (!e->use_graph) {
+        subcycle_body(e);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    if (!e->graph_ok) {
+        if (e->gexec) {
+            HIPCHK(hipGraphExecDestroy(e->gexec));
+            e->gexec = nullptr;
+        }
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
+        subcycle_body(e);
+        HIPCHK(hipStreamEndCapture(e->s, &g));
+        HIPCHK(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+        e->graph_ok = true;
+    }
+    HIPCHK(hipGraphLaunch(e->gexec, e->s));
+    return 0;
+}
+
+// ---- stepGOMELT (cF:2304-2397) ----------------------------------------------
+
+static void solutions(gm_engine* e, const float* V1, const float* V2, float* T1o, float* T2o,
+                      float* T3o) {
+    // computeSolutions (cF:2135-2204): L1, then L2 from the new L1, then L3
+    // from the new L2; no clamp inside (the caller clamps afterwards).
+    run_l1(e, e->G1, 1, V1, T1o, GM_DT_ROW, 0.0f, 0);
+    run_l2(e, e->T2, e->S12, nullptr, e->G2, 1, V2, T1o, T1o, 1.0f, T2o, GM_DT_ROW, 0, 0.0f, 0);
+    run_l3(e, e->T3, e->S13, nullptr, 0, T2o, T2o, 1.0f, T3o, 0, nullptr, nullptr);
+}
+
+extern "C" int gm_engine_step(gm_engine* e, const float* row) {
+    GmStepDesc& h = e->host;
+    h.nrows = 1;
+    for (int c = 0; c < 7; ++c) h.rows[0][c] = row[c];
+    h.dt1 = row[5];
+    if (upload_desc(e, desc_bytes(1))) return -1;
+    GmStepDesc* d = e->d_desc;
+    hipStream_t s = e->s;
+    const int64_t n1 = e->nn[1], n2 = e->nn[2], n3 = e->nn[3];
+    const float Tamb = e->mat.T_amb;
+    launch_copy_u8(e->preS2, e->S23, n3, s);
+    // updateStateProperties (cF:2513-2564): persist L3 S1/S2 and L2 S1
+    extern void gm_props_u8(const DevMat& m, const float* T, uint8_t* S1, uint8_t* S2, int64_t n,
+                            int64_t nsub, hipStream_t s);
+    gm_props_u8(e->dm, e->T3, e->S13, e->S23, n3, h.sub[3], s);
+    gm_props_u8(e->dm, e->T2, e->S12, nullptr, n2, h.sub[2], s);
+    l1_s1_update(e);
+    // sources projected to L1 and L2 (computeSources cF:928-988)
+    launch_source_tables(e->pair[1].p, d, e->dm, 0, 1, 0, nullptr, e->G1, e->gmax[1], s);
+    launch_source_tables(e->pair[2].p, d, e->dm, 0, 1, 0, nullptr, e->G2, e->gmax[2], s);
+    // predictor T' flux terms (computeCoarseTprimeTerm_jax cF:1477-1565)
+    launch_restrict(e->pair[1].p, d, e->dm,
+                    rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
+                    e->part[1], s);
+    launch_restrict(e->pair[0].p, d, e->dm,
+                    rargs(e, 2, e->T2, e->S12, e->Tp2, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
+                    e->part[0], s);
+    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], nullptr, e->V1, s);
+    launch_restrict(e->pair[2].p, d, e->dm,
+                    rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
+                    e->part[2], s);
+    launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2, s);
+    // predictor
+    float* T1o = e->T1p;
+    float* T2o = e->T2b[0];
+    float* T3o = e->T3b[0];
+    solutions(e, e->V1, e->V2, T1o, T2o, T3o);
+    launch_clamp(T1o, n1, Tamb, s);
+    launch_clamp(T2o, n2, Tamb, s);
+    launch_clamp(T3o, n3, Tamb, s);
+    new_tprime(e, 3, T3o, T2o, e->Tp3x);
+    new_tprime(e, 2, T2o, T1o, e->Tp2x);
+    // mass terms (computeCoarseTprimeMassTerm_jax cF:1396-1474)
+    launch_restrict(e->pair[1].p, d, e->dm,
+                    rargs(e, 3, e->T3, e->S13, nullptr, e->Tp3x, e->Tp3, GM_DT_ROW, 0, 0.0f),
+                    e->part[1], s);
+    launch_restrict(e->pair[0].p, d, e->dm,
+                    rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_ROW, 0, 0.0f),
+                    e->part[0], s);
+    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], e->V1, e->V1m, s);
+    launch_restrict(e->pair[2].p, d, e->dm,
+                    rargs(e, 3, e->T3, e->S13, nullptr, e->Tp3x, e->Tp3, GM_DT_ROW, 0, 0.0f),
+                    e->part[2], s);
+    float* V2m = e->Tp2x;  // reuse: the L2 T' of the predictor is no longer needed
+    launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, e->V2, V2m, s);
+    // corrector
+    T1o = e->T1c;
+    T2o = e->T2b[1];
+    T3o = e->T3b[1];
+    solutions(e, e->V1m, V2m, T1o, T2o, T3o);
+    launch_clamp(T1o, n1, Tamb, s);
+    launch_clamp(T2o, n2, Tamb, s);
+    launch_clamp(T3o, n3, Tamb, s);
+    // final T' (getBothNewTprimes cF:2385-2387)
+    launch_copy_f32(e->T3, T3o, n3, s);
+    new_tprime(e, 3, e->T3, T2o, e->Tp3);
+    new_tprime(e, 2, T2o, T1o, e->Tp2);
+    launch_copy_f32(e->T2, T2o, n2, s);
+    launch_copy_f32(e->T1, T1o, n1, s);
+    // Level0 state and the driver's melt-time bookkeeping (go_melt.py:339-357)
+    launch_l0_scatter(d, e->S13, e->S23, e->L0S1, e->s2snap, e->s2box, n3, s);
+    launch_melt_single(d, e->T3, e->preS2, e->S23, row[5], e->mat.T_liquidus, e->acc, e->macc,
+                       n3, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// computeStateProperties on uint8 state, persisting S1 (and S2).
+__global__ void k_props_u8(DevMat m, const float* __restrict__ T, uint8_t* S1, uint8_t* S2,
+                           int64_t n, int64_t nsub) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float v = T[t];
+    const bool s2 = v >= m.T_liq;
+    S1[t] = (uint8_t)((S1[t] != 0) || s2 || t < nsub);
+    if (S2) S2[t] = (uint8_t)s2;
+}
+
+void gm_props_u8(const DevMat& m, const float* T, uint8_t* S1, uint8_t* S2, int64_t n,
+                 int64_t nsub, hipStream_t s) {
+    hipLaunchKernelGGL(k_props_u8, dim3((n + 255) / 256), dim3(256), 0, s, m, T, S1, S2, n, nsub);
+}
+
+// ---- stepGOMELTDwellTime (cF:2617-2664) -------------------------------------
+
+extern "C" int gm_engine_step_dwell(gm_engine* e, float dt) {
+    run_l1(e, nullptr, 1, nullptr, e->T1p, GM_DT_VALUE, dt, 0);
+    launch_copy_f32(e->T1, e->T1p, e->nn[1], e->s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_engine_clear_tprime(gm_engine* e) {
+    HIPCHK(hipMemsetAsync(e->Tp2, 0, e->nn[2] * 4, e->s));
+    HIPCHK(hipMemsetAsync(e->Tp3, 0, e->nn[3] * 4, e->s));
+    return 0;
+}
+
+extern "C" int gm_engine_level0_substrate(gm_engine* e, int64_t n) {
+    launch_fill_u8(e->L0S1, 1, std::min(n, e->nn[0]), e->s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---- layer change of the single-step driver path (go_melt.py:198-290) -------
+
+extern "C" int gm_engine_layer_change(gm_engine* e, int kidx, float new_l1_z0, float laser_z,
+                                      int shift_level0) {
+    (void)laser_z;
+    GmStepDesc& h = e->host;
+    hipStream_t s = e->s;
+    const int64_t n1 = e->nn[1];
+    // T0 re-interpolated onto the shifted L1 z axis, clamped at T_amb
+    gm_grid newg = h.g[1];
+    newg.x0[2] = new_l1_z0;
+    launch_interp_grid(nullptr, -1, &h.g[1], e->T1, nullptr, -1, &newg, n1, e->T1p, s);
+    launch_clamp(e->T1p, n1, e->mat.T_amb, s);
+    launch_copy_f32(e->T1, e->T1p, n1, s);
+    // S1 storage: store the current S1 in row kidx-1 and load row kidx
+    // (indices outside the storage clip to its ends, as JAX indexing does)
+    const int st = e->nst;
+    int wrow = kidx - 1;
+    if (wrow < 0) wrow += st;
+    wrow = std::min(std::max(wrow, 0), st - 1);
+    const int rrow = std::min(std::max(kidx, 0), st - 1);
+    launch_copy_f32(e->S1st + (size_t)wrow * n1, e->S11, n1, s);
+    launch_copy_f32(e->S11, e->S1st + (size_t)rrow * n1, n1, s);
+    h.g[1].x0[2] = new_l1_z0;
+    h.gprev[1] = h.g[1];
+    if (shift_level0) {
+        const gm_grid& g0 = e->cfg.level[0];
+        const int64_t nxy = (int64_t)g0.n[0] * g0.n[1];
+        const int64_t nA = nxy * e->cfg.l0_layer_delta;
+        const int64_t nB = nxy * (g0.n[2] - e->cfg.l0_layer_delta);
+        // accum = max(accum, max_accum), then shift down and clear the maxima
+        extern void gm_vmax(float* a, const float* b, int64_t n, hipStream_t s);
+        gm_vmax(e->acc, e->macc, e->nn[0], s);
+        if (nA > 0) launch_shift_l0(e->L0S1, e->acc, nA, nB, e->nn[0], s);
+        HIPCHK(hipMemsetAsync(e->macc, 0, e->nn[0] * 4, s));
+    }
+    e->graph_ok = false;
+    if (upload_desc(e, desc_bytes(0))) return -1;
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_vmax(float* a, const float* b, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && b[t] > a[t]) a[t] = b[t];
+}
+
+void gm_vmax(float* a, const float* b, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_vmax, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n);
+}
+
+extern "C" int gm_engine_minmax(gm_engine* e, float* out) {
+    launch_minmax(e->T1, e->nn[1], e->mm, e->s);
+    launch_minmax(e->T2, e->nn[2], e->mm + 2, e->s);
+    launch_minmax(e->T3, e->nn[3], e->mm + 4, e->s);
+    HIPCHK(hipMemcpyAsync(out, e->mm, 6 * 4, hipMemcpyDeviceToHost, e->s));
+    HIPCHK(hipStreamSynchronize(e->s));
+    return 0;
+}

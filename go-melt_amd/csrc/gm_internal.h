@@ -1,0 +1,203 @@
+// Internal types shared by the gfx950 kernels (gm_kernels.hip) and the host
+// engine / C ABI (gm_engine.hip).  Nothing here is part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gomelt.h"
+
+#define GM_MAX_ROWS 1024  // N2*N3 toolpath rows per L1 step
+#define GM_MAX_N2 64
+
+// Integer box of nodes (or elements): lo[d] .. lo[d]+n[d]-1.
+struct GmBox {
+    int32_t lo[3];
+    int32_t n[3];
+};
+
+// Everything that changes from one L1 step to the next.  It lives in device
+// memory and kernels read it through a pointer, so a recorded hipGraph stays
+// valid while the windows move: the host only re-uploads this struct.
+struct GmStepDesc {
+    gm_grid g[4];          // current meshes; [1] L1, [2] L2, [3] L3
+    gm_grid gprev[4];      // meshes before the last window move
+    GmBox ov3;             // L2 nodes overlapped by L3 (L2 index space)
+    gm_grid ov3_xyz;       // their coordinates (reference L3 overlapCoords)
+    GmBox ov2;             // L1 nodes overlapped by L2 (L1 index space)
+    gm_grid ov2_xyz;       // their coordinates (reference L2 overlapCoords)
+    int32_t l0_lo3[3];     // Level0 node of L3 node (0,0,0); stride 1
+    int32_t l0_lo2[3];     // Level0 node of L2 node (0,0,0)
+    int32_t l0_st2[3];     // Level0 stride of L2 nodes
+    int32_t l0_n[3];       // Level0 nodes per axis
+    int64_t l1_ne_active, l1_nn_active;
+    int64_t sub[4];        // substrate node counts per level
+    int32_t nrows;
+    float dt1;             // sum of row dt over the L1 step
+    float dt2[GM_MAX_N2];  // sum of row dt per L2 substep
+    float rows[GM_MAX_ROWS][7];
+};
+
+// Device copy of the material with derived constants (kernel argument).
+struct DevMat {
+    float T_amb, T_sol, T_liq, T_boil;
+    float k_p, k_a0, k_a1, k_f;
+    float cp_a0, cp_a1, cp_m, cp_f, rho;
+    int32_t k_n, cp_n;
+    float kT[GM_MAX_KNOTS], kv[GM_MAX_KNOTS], ks[GM_MAX_KNOTS];
+    float cT[GM_MAX_KNOTS], cv[GM_MAX_KNOTS], cs[GM_MAX_KNOTS];
+    float h_conv, sigma_eps, evc_cp, CT, CM, Lev, invTb, Tclip, Tamb4;
+    float eta, rsq, dsq, rc, dc, pc6;  // pc6 = 6*sqrt(3)*eta
+};
+
+// One axis of the tabulated restriction between a fine and a coarse mesh
+// (the per-axis factors of the reference's Shapes tensors, cF:1213-1358).
+// For fine element e and Gauss point g in {0,1} along the axis:
+//   xq  : Gauss point coordinate
+//   phi : coarse shape factors (x1 - xq)/hc, (xq - x0)/hc in the coarse
+//         element cq that holds the Gauss point (clipped into the mesh)
+//   c0  : coarse element holding Gauss point 0 (where the values are added)
+// Fine elements with c0 == c form the contiguous range [ex_lo, ex_hi).
+struct GmAxisTab {
+    int32_t* c0;       // [nef]
+    float4* q;         // [nef][2] : (xq, phi0, phi1, 0)
+    int32_t* info;     // [0] c_lo, [1] c_n, [2 .. 2+cmax) ex_lo, then ex_hi
+};
+
+struct GmPair {
+    int coarse, fine;  // level ids (grids read from the step descriptor)
+    int nef[3];        // fine elements per axis
+    int cmax[3];       // bound on covered coarse elements per axis
+    GmAxisTab ax[3];
+};
+
+enum { GM_SRC_NONE = 0, GM_SRC_L3 = 1, GM_SRC_TABLES = 2 };
+enum { GM_BC_NONE = 0, GM_BC_L1 = 1, GM_BC_FINE = 2 };
+// dt selectors
+enum { GM_DT_VALUE = -1, GM_DT_ROW = 0, GM_DT_L1 = 1, GM_DT_L2 = 2 };
+
+// Arguments of the fused level step (solveMatrixFreeFE + computeStateProperties
+// + sources + computeConvRadBC + Dirichlet + clamp [+ melt-time accumulation]).
+struct LSArgs {
+    int level;            // grid = desc->g[level] when use_desc
+    int use_desc;
+    gm_grid grid;         // used when !use_desc
+    const float* T;
+    const void* S1;       // float (s1_float) or uint8
+    int s1_float;
+    float* Tout;
+    uint8_t* S1out;       // nullable
+    const float* kin;     // optional precomputed nodal k, rho*cp
+    const float* cin;
+    int64_t n_sub;
+    int64_t ne_active;    // elements 0..ne_active-1 take part
+    int64_t nn_active;    // nodes >= nn_active are set to T_amb
+    int dt_mode, dt_index;
+    float dt;
+    // source
+    int src_mode;
+    int src_row;          // GM_SRC_L3: desc row (or -1: src_v/src_P)
+    float src_v[3], src_P;
+    const float* G;       // GM_SRC_TABLES: [npos][3][G_nmax]
+    int G_npos, G_nmax;
+    float G_scale;
+    const int32_t* G_info[3];
+    const float* Fext;    // nullable full-size extra source
+    int conv;             // top-face convection/radiation/evaporation
+    // correction term (restricted T' terms), stored on a node box
+    const float* corr;
+    const int32_t* corr_info[3];
+    int corr_nmax[3];
+    const float* corr_full;  // nullable full-size correction
+    // Dirichlet
+    int bc_mode;
+    float bc_vals[5];     // y0, y1, x0, x1, z0
+    int bc_level;         // coarse level of the prolonged field
+    const float* bcA;
+    const float* bcB;
+    float alpha;
+    int clamp;
+    // melt-time accumulation (Part 2 of the L3 subcycle)
+    int accum_on;
+    const uint8_t* S2prev;
+    uint8_t* S2out;
+    float* accum;
+    float* max_accum;
+    // element operator of the level: kc[p] for neighbour patterns p and the
+    // lumped mass factor (volume/8)
+    float kc[8];
+    float mrow;
+    float w2, w3;         // face and volume Gauss weights (detJ)
+};
+
+// Restriction (fine -> coarse) arguments.
+struct RArgs {
+    const float* T;       // fine temperature (for element k and rho*cp)
+    const uint8_t* S1;    // fine S1 (uint8)
+    const float* S1f;     // or FP32 S1
+    int64_t n_sub;
+    const float* Tp;      // flux T' (nullable)
+    const float* TpA;     // mass dT' = TpA - TpB (nullable)
+    const float* TpB;
+    float dt;
+    int dt_mode, dt_index;
+    float wf;             // fine volume Gauss weight
+};
+
+// ---- launchers (gm_kernels.hip) -------------------------------------------
+namespace gm {
+DevMat make_devmat(const gm_material& m);
+void level_coeffs(const gm_grid& g, float kc[8], float* mrow, float* w2, float* w3);
+
+void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& host_grid,
+                       const DevMat& m, hipStream_t s);
+void launch_axis_tables(const GmPair& p, const GmStepDesc* d, hipStream_t s);
+void launch_source_tables(const GmPair& p, const GmStepDesc* d, const DevMat& m,
+                          int row0, int npos, int use_rows, const float* rows_dev,
+                          float* G, int nmax, hipStream_t s);
+void launch_restrict(const GmPair& p, const GmStepDesc* d, const DevMat& m, const RArgs& r,
+                     double* partial, hipStream_t s);
+void launch_restrict_nodes(const GmPair& outer, const double* part_outer, const GmPair* inner,
+                           const double* part_inner, const float* Vbase, float* V,
+                           hipStream_t s);
+void launch_interp_grid(const GmStepDesc* d, int src_level, const gm_grid* src_host,
+                        const float* u, const uint8_t* u8, int tgt_mode, const gm_grid* tgt_host,
+                        int64_t n_target, float* out, hipStream_t s);
+void launch_inject(const GmStepDesc* d, int fine_level, int coarse_level, const float* fineT,
+                   float* coarseT, int64_t n_box, hipStream_t s);
+void launch_tprime(const GmStepDesc* d, int fine_level, int coarse_level, const float* fineT,
+                   const float* coarseT, float* Tp, int64_t n, hipStream_t s);
+void launch_move_l3(const GmStepDesc* d, const float* T1, const float* Tp2, const float* Tp3,
+                    float* T3n, float* Tp3n, int64_t nn3, hipStream_t s);
+void launch_move_l2(const GmStepDesc* d, const float* T1, const float* Tp2, float* T2n,
+                    float* Tp2n, int64_t nn2, hipStream_t s);
+void launch_l1_s1_from_l2(const GmStepDesc* d, const uint8_t* S1_2, float* S1_1, int64_t n_box,
+                          hipStream_t s);
+void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
+void launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s);
+void launch_copy_f32(float* dst, const float* src, int64_t n, hipStream_t s);
+void launch_copy_u8(uint8_t* dst, const uint8_t* src, int64_t n, hipStream_t s);
+void launch_clamp(float* T, int64_t n, float Tamb, hipStream_t s);
+void launch_l0_gather(const GmStepDesc* d, const uint8_t* L0S1, const uint8_t* s2snap,
+                      const int32_t* s2box, uint8_t* S1_3, uint8_t* S2_3, uint8_t* S1_2,
+                      int64_t nn3, int64_t nn2, hipStream_t s);
+void launch_l0_scatter(const GmStepDesc* d, const uint8_t* S1_3, const uint8_t* S2_3,
+                       uint8_t* L0S1, uint8_t* s2snap, int32_t* s2box, int64_t nn3,
+                       hipStream_t s);
+void launch_melt_single(const GmStepDesc* d, const float* T3, const uint8_t* preS2,
+                        const uint8_t* S2, float dt, float Tliq, float* accum, float* max_accum,
+                        int64_t nn3, hipStream_t s);
+void launch_minmax(const float* T, int64_t n, float* out2, hipStream_t s);
+void launch_props(const DevMat& m, const float* T, const float* S1, int64_t nn, int64_t nsub,
+                  float* S1o, uint8_t* S2o, float* k, float* c, hipStream_t s);
+void launch_source_l3_full(const gm_grid& g, const DevMat& m, float vx, float vy, float vz,
+                           float P, float* F, hipStream_t s);
+void launch_convrad_full(const gm_grid& g, const DevMat& m, const float* T, int64_t ne_active,
+                         float* F, hipStream_t s);
+void launch_interp_matrix(const gm_grid& src, const gm_grid& tgt, float* Nc, int32_t* node,
+                          hipStream_t s);
+void launch_interp_w(const float* Nc, const int32_t* node, int64_t n, const float* T,
+                     float* out, hipStream_t s);
+void launch_shift_l0(uint8_t* L0S1, float* accum, int64_t n1, int64_t n2, int64_t nn,
+                     hipStream_t s);
+}  // namespace gm

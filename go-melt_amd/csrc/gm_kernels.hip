@@ -1,0 +1,1232 @@
+// gfx950 kernels of the multilevel explicit FE heat step.
+//
+// Layout: every level field is a flat FP32 (or uint8) array in the x-fastest
+// node numbering of the reference (i = ix + nx*(iy + ny*iz), cF:645-689).
+// The hot kernel is k_level_step: one launch advances one level by one time
+// step, fusing the reference's computeStateProperties (cF:2567-2614), the
+// matrix-free element operator solveMatrixFreeFE (cF:582-642), the laser
+// source (cF:2960-3012 / 2667-2730), the top-face flux (cF:2207-2301), the
+// Dirichlet overwrite (cF:1568-1620), the inactive-node reset (cF:1848-1865)
+// and the T_amb clamp.  It is node-centric: each tile of 32x8x4 nodes stages
+// T, k and rho*cp of its nodes plus a one-node halo in LDS, forms the element
+// means once per element, and applies the 27-point stencil that the 8x8
+// element matrices of the box element reduce to.  No atomics, deterministic.
+#include <math.h>
+
+#include "gm_internal.h"
+
+namespace gm {
+
+#define TX 32
+#define TY 8
+#define TZ 4
+#define HX (TX + 2)
+#define HY (TY + 2)
+#define HZ (TZ + 2)
+#define EX (TX + 1)
+#define EY (TY + 1)
+#define EZ (TZ + 1)
+
+// 1-D Gauss factors at xi = -/+ 1/sqrt(3): N1 same side / other side.
+#define GA 0.78867513459481287f
+#define GB 0.21132486540518713f
+
+__device__ __forceinline__ float coordf(const gm_grid& g, int d, int i) {
+    return g.x0[d] + (float)i * g.h[d];
+}
+
+__device__ __forceinline__ int64_t nid(int ix, int iy, int iz, int nx, int ny) {
+    return (int64_t)ix + (int64_t)nx * ((int64_t)iy + (int64_t)ny * iz);
+}
+
+// Piecewise-linear table with linear extension beyond the end knots.
+__device__ __forceinline__ float tab_eval(const float* kT, const float* kv, const float* ks, int n,
+                                          float T) {
+    float b = kv[0], t0 = kT[0], sl = ks[0];
+#pragma unroll
+    for (int m = 1; m < GM_MAX_KNOTS - 1; ++m) {
+        if (m < n - 1 && T >= kT[m]) {
+            b = kv[m];
+            t0 = kT[m];
+            sl = ks[m];
+        }
+    }
+    return b + sl * (T - t0);
+}
+
+// computeStateProperties (cF:2584-2612) for one node.  With S1, S2, S3 in
+// {0,1} the reference's sums of products are exact selects.
+__device__ __forceinline__ void props(const DevMat& m, float T, float s1in, bool substrate,
+                                      float& k, float& c, bool& s1, bool& s2) {
+    s2 = T >= m.T_liq;
+    const bool s3 = (T > m.T_sol) && (T < m.T_liq);
+    s1 = (s1in > 0.499f) || s2 || substrate;
+    const float kb = m.k_n ? tab_eval(m.kT, m.kv, m.ks, m.k_n, T) : (m.k_a1 * T + m.k_a0);
+    const float kk = s2 ? m.k_f : (s1 ? kb : m.k_p);
+    k = kk / 1000.0f;
+    const float cs = m.cp_n ? tab_eval(m.cT, m.cv, m.cs, m.cp_n, T) : (m.cp_a1 * T + m.cp_a0);
+    const float cp = s2 ? m.cp_f : (s3 ? m.cp_m : cs);
+    c = m.rho * cp;
+}
+
+// interpolatePointsMatrix rule (cF:1060-1104): coarse element clipped into
+// the mesh, trilinear weights, all zero if any leaves [-0.01, 1.01].
+__device__ __forceinline__ void interp8(const gm_grid& c, float x, float y, float z, float N[8],
+                                        int64_t nd[8]) {
+    int e[3];
+    const float q[3] = {x, y, z};
+    float lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int ee = (int)floorf((q[d] - c.x0[d]) / c.h[d]);
+        ee = ee < 0 ? 0 : (ee > c.n[d] - 2 ? c.n[d] - 2 : ee);
+        e[d] = ee;
+        const float x0 = coordf(c, d, ee), x1 = coordf(c, d, ee + 1);
+        lo[d] = x1 - q[d];
+        hi[d] = q[d] - x0;
+    }
+    const float inv = 1.0f / (c.h[0] * c.h[1] * c.h[2]);
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int bx = ((a + 1) >> 1) & 1, by = (a >> 1) & 1, bz = a >> 2;
+        const float v = (bx ? hi[0] : lo[0]) * (by ? hi[1] : lo[1]) * (bz ? hi[2] : lo[2]) * inv;
+        N[a] = v;
+        ok = ok && (v >= -1e-2f) && (v <= 1.0f + 1e-2f);
+        nd[a] = nid(e[0] + bx, e[1] + by, e[2] + bz, c.n[0], c.n[1]);
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) N[a] = ok ? fminf(fmaxf(N[a], 0.0f), 1.0f) : 0.0f;
+}
+
+__device__ __forceinline__ float interp_f(const gm_grid& c, const float* u, float x, float y,
+                                          float z) {
+    float N[8];
+    int64_t nd[8];
+    interp8(c, x, y, z, N, nd);
+    float s = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) s += N[a] * u[nd[a]];
+    return s;
+}
+
+// Top-face flux density (cF:2269-2290), W/mm^2.
+__device__ __forceinline__ float face_flux(const DevMat& m, float Tq) {
+    Tq = (Tq > m.Tclip) ? m.Tclip : Tq;
+    const float invT = 1.0f / Tq;
+    const float E = m.Lev + m.cp_f * (Tq - m.T_amb);
+    const float mol = sqrtf(m.CM * invT);
+    const float S = m.evc_cp * expf(-m.CT * (invT - m.invTb)) * mol * E;
+    const float t2 = Tq * Tq;
+    const float q = m.h_conv * (m.T_amb - Tq) + m.sigma_eps * (m.Tamb4 - t2 * t2) - S;
+    return q * 1e-6f;
+}
+
+// 2x2 face Gauss point / node shape values (cF:778-815).
+__device__ __forceinline__ float n2(int q, int a) {
+    const float s = 0.62200846792814621f, j = 0.16666666666666667f, o = 0.04465819873852045f;
+    const int qx = ((q + 1) >> 1) & 1, qy = (q >> 1) & 1;
+    const int ax = ((a + 1) >> 1) & 1, ay = (a >> 1) & 1;
+    const int dif = (qx != ax) + (qy != ay);
+    return dif == 0 ? s : (dif == 1 ? j : o);
+}
+
+// Separable laser source factor along one axis at node i (cF:2986-3010 with
+// the Gaussian of cF:1013-1025): sum over the elements on either side and
+// their two Gauss points of N1 * coef * exp(-3 (xq - v)^2 / sq).
+__device__ __forceinline__ float src_axis(const gm_grid& g, int d, int i, float v, float sq,
+                                          float coef) {
+    float s = 0.0f;
+    const int ne = g.n[d] - 1;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int e = i - 1 + k;
+        if (e < 0 || e >= ne) continue;
+        const float xa = coordf(g, d, e), xb = coordf(g, d, e + 1);
+        const int a = i - e;
+#pragma unroll
+        for (int gg = 0; gg < 2; ++gg) {
+            const float xq = gg == 0 ? (xa * GA + xb * GB) : (xa * GB + xb * GA);
+            const float t = xq - v;
+            const float qv = coef * expf((-3.0f * (t * t)) / sq);
+            s += (a == gg ? GA : GB) * qv;
+        }
+    }
+    return s;
+}
+
+// Tile index of this block, mapped so that each XCD (blocks are dealt to the
+// 8 XCDs round-robin) works on a contiguous run of tiles and shares halos
+// through its own L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, k = b >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+template <bool S1F>
+__global__ void __launch_bounds__(256) k_level_step(LSArgs a, const GmStepDesc* __restrict__ d,
+                                                    DevMat m) {
+    __shared__ float sT[HX * HY * HZ];
+    __shared__ float sK[HX * HY * HZ];
+    __shared__ float sC[HX * HY * HZ];
+    __shared__ float sKe[EX * EY * EZ];
+    __shared__ float sCe[EX * EY * EZ];
+    __shared__ float sS[TX + TY + TZ];
+
+    const gm_grid g = a.use_desc ? d->g[a.level] : a.grid;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int ox = (t % ntx) * TX, oy = ((t / ntx) % nty) * TY, oz = (t / (ntx * nty)) * TZ;
+    const int nex = nx - 1, ney = ny - 1;
+    const int nez_act = (int)(a.ne_active / ((int64_t)nex * ney));
+    const int top = nez_act;  // node layer carrying the surface flux
+
+    float dt = a.dt;
+    if (a.dt_mode == GM_DT_ROW) dt = d->rows[a.dt_index][5];
+    else if (a.dt_mode == GM_DT_L1) dt = d->dt1;
+    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
+
+    // --- phase A: stage T, k, rho*cp of the tile and its halo ---------------
+    for (int l = threadIdx.x; l < HX * HY * HZ; l += blockDim.x) {
+        const int hx = l % HX, hy = (l / HX) % HY, hz = l / (HX * HY);
+        const int ix = ox + hx - 1, iy = oy + hy - 1, iz = oz + hz - 1;
+        float tv = 0.0f, kv = 0.0f, cv = 0.0f;
+        if (ix >= 0 && ix < nx && iy >= 0 && iy < ny && iz >= 0 && iz < nz) {
+            const int64_t idx = nid(ix, iy, iz, nx, ny);
+            tv = a.T[idx];
+            if (a.kin) {
+                kv = a.kin[idx];
+                cv = a.cin[idx];
+            } else {
+                const float s1in = S1F ? ((const float*)a.S1)[idx]
+                                       : (float)((const uint8_t*)a.S1)[idx];
+                bool s1, s2;
+                props(m, tv, s1in, idx < a.n_sub, kv, cv, s1, s2);
+                if (a.S1out && hx >= 1 && hx <= TX && hy >= 1 && hy <= TY && hz >= 1 && hz <= TZ)
+                    a.S1out[idx] = (uint8_t)s1;
+            }
+        }
+        sT[l] = tv;
+        sK[l] = kv;
+        sC[l] = cv;
+    }
+    // separable source factors of the tile's node rows/columns/layers
+    if (a.src_mode == GM_SRC_L3) {
+        float v[3], P;
+        if (a.src_row >= 0) {
+            v[0] = d->rows[a.src_row][0];
+            v[1] = d->rows[a.src_row][1];
+            v[2] = d->rows[a.src_row][2];
+            P = d->rows[a.src_row][6];
+        } else {
+            v[0] = a.src_v[0];
+            v[1] = a.src_v[1];
+            v[2] = a.src_v[2];
+            P = a.src_P;
+        }
+        for (int l = threadIdx.x; l < TX + TY + TZ; l += blockDim.x) {
+            int dd, i;
+            if (l < TX) { dd = 0; i = ox + l; }
+            else if (l < TX + TY) { dd = 1; i = oy + l - TX; }
+            else { dd = 2; i = oz + l - TX - TY; }
+            float s = 0.0f;
+            if (i < g.n[dd]) {
+                const float sq = dd == 2 ? m.dsq : m.rsq;
+                const float coef = dd == 2 ? m.dc : m.rc;
+                s = src_axis(g, dd, i, v[dd], sq, coef);
+                if (dd == 0) s *= m.pc6 * P;
+            }
+            sS[l] = s;
+        }
+    }
+    __syncthreads();
+
+    // --- phase B: element means of k and rho*cp ------------------------------
+    for (int l = threadIdx.x; l < EX * EY * EZ; l += blockDim.x) {
+        const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
+        const int ex = ox + lx - 1, ey = oy + ly - 1, ez = oz + lz - 1;
+        float ke = 0.0f, ce = 0.0f;
+        if (ex >= 0 && ex < nex && ey >= 0 && ey < ney && ez >= 0 && ez < nez_act) {
+            const int h0 = lx + HX * (ly + HY * lz);
+            const int o1 = 1, o2 = HX, o4 = HX * HY;
+            ke = (sK[h0] + sK[h0 + o1] + sK[h0 + o1 + o2] + sK[h0 + o2] + sK[h0 + o4] +
+                  sK[h0 + o1 + o4] + sK[h0 + o1 + o2 + o4] + sK[h0 + o2 + o4]) * 0.125f;
+            ce = (sC[h0] + sC[h0 + o1] + sC[h0 + o1 + o2] + sC[h0 + o2] + sC[h0 + o4] +
+                  sC[h0 + o1 + o4] + sC[h0 + o1 + o2 + o4] + sC[h0 + o2 + o4]) * 0.125f;
+        }
+        sKe[l] = ke;
+        sCe[l] = ce;
+    }
+    __syncthreads();
+
+    // --- phase C: one column of TZ nodes per thread -------------------------
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int ix = ox + tx, iy = oy + ty;
+    if (ix >= nx || iy >= ny) return;
+    const float mfac = a.mrow / dt;
+
+    // boxed correction / table-source bounds
+    int clo[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
+    if (a.corr) {
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) {
+            clo[dd] = a.corr_info[dd][0];
+            cn[dd] = a.corr_info[dd][1] + 1;
+        }
+    }
+    int glo[3] = {0, 0, 0}, gn[3] = {0, 0, 0};
+    if (a.src_mode == GM_SRC_TABLES) {
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) {
+            glo[dd] = a.G_info[dd][0];
+            gn[dd] = a.G_info[dd][1] + 1;
+        }
+    }
+
+    for (int z = 0; z < TZ; ++z) {
+        const int iz = oz + z;
+        if (iz >= nz) break;
+        const int hx = tx + 1, hy = ty + 1, hz = z + 1;
+        const int64_t idx = nid(ix, iy, iz, nx, ny);
+
+        float Tn[27];
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx)
+                    Tn[dx + 3 * (dy + 3 * dz)] =
+                        sT[(hx - 1 + dx) + HX * ((hy - 1 + dy) + HY * (hz - 1 + dz))];
+
+        float KT = 0.0f, Msum = 0.0f;
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            const int oxx = o & 1, oyy = (o >> 1) & 1, ozz = o >> 2;
+            const int el = (hx - 1 + oxx) + EX * ((hy - 1 + oyy) + EY * (hz - 1 + ozz));
+            const float ke = sKe[el];
+            const float ce = sCe[el];
+            float acc = 0.0f;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+                const int dxx = oxx - 1 + bx, dyy = oyy - 1 + by, dzz = ozz - 1 + bz;
+                const int p = (dxx != 0) | ((dyy != 0) << 1) | ((dzz != 0) << 2);
+                acc += a.kc[p] * Tn[(dxx + 1) + 3 * ((dyy + 1) + 3 * (dzz + 1))];
+            }
+            KT += ke * acc;
+            Msum += ce;
+        }
+        const float M = Msum * mfac;
+        const float Ti = Tn[13];
+
+        // source + surface flux
+        float F = 0.0f;
+        if (a.src_mode == GM_SRC_L3) {
+            F = sS[tx] * sS[TX + ty] * sS[TX + TY + z] * a.w3;
+        } else if (a.src_mode == GM_SRC_TABLES) {
+            const int bx = ix - glo[0], by = iy - glo[1], bz = iz - glo[2];
+            if (bx >= 0 && bx < gn[0] && by >= 0 && by < gn[1] && bz >= 0 && bz < gn[2]) {
+                float s = 0.0f;
+                for (int pq = 0; pq < a.G_npos; ++pq) {
+                    const float* Gp = a.G + (int64_t)pq * 3 * a.G_nmax;
+                    s += Gp[bx] * Gp[a.G_nmax + by] * Gp[2 * a.G_nmax + bz];
+                }
+                F = s * a.G_scale;
+            }
+        }
+        if (a.Fext) F += a.Fext[idx];
+        if (a.conv && iz == top) {
+            float cr = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int exx = ix - 1 + (e & 1), eyy = iy - 1 + (e >> 1);
+                if (exx < 0 || exx >= nex || eyy < 0 || eyy >= ney) continue;
+                const int fx = exx - ox + 1, fy = eyy - oy + 1;  // halo coords of face node 0
+                const float* row0 = sT + HX * (fy + HY * hz);
+                const float Tf[4] = {row0[fx], row0[fx + 1], row0[fx + 1 + HX], row0[fx + HX]};
+                const int ax = ix - exx, ay = iy - eyy;
+                const int ai = ay ? (ax ? 2 : 3) : (ax ? 1 : 0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float Tq = n2(q, 0) * Tf[0] + n2(q, 1) * Tf[1] + n2(q, 2) * Tf[2] +
+                                     n2(q, 3) * Tf[3];
+                    cr += n2(q, ai) * (face_flux(m, Tq) * a.w2);
+                }
+            }
+            F += cr;
+        }
+        float C = 0.0f;
+        if (a.corr) {
+            const int bx = ix - clo[0], by = iy - clo[1], bz = iz - clo[2];
+            if (bx >= 0 && bx < cn[0] && by >= 0 && by < cn[1] && bz >= 0 && bz < cn[2])
+                C = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
+        }
+        if (a.corr_full) C += a.corr_full[idx];
+
+        float Tnew = ((M * Ti - KT) + F + C) / M;
+        if (idx >= a.nn_active) Tnew = m.T_amb;
+
+        if (a.bc_mode == GM_BC_L1) {
+            if (iz == 0) Tnew = a.bc_vals[4];
+            else if (ix == nx - 1) Tnew = a.bc_vals[3];
+            else if (ix == 0) Tnew = a.bc_vals[2];
+            else if (iy == ny - 1) Tnew = a.bc_vals[1];
+            else if (iy == 0) Tnew = a.bc_vals[0];
+        } else if (a.bc_mode == GM_BC_FINE) {
+            if (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1 || iz == 0) {
+                const gm_grid& cg = d->g[a.bc_level];
+                float N[8];
+                int64_t nd[8];
+                interp8(cg, coordf(g, 0, ix), coordf(g, 1, iy), coordf(g, 2, iz), N, nd);
+                const float al = a.alpha, be = 1.0f - a.alpha;
+                float s = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) s += N[q] * (al * a.bcA[nd[q]] + be * a.bcB[nd[q]]);
+                Tnew = s;
+            }
+        }
+        if (a.clamp && Tnew < m.T_amb) Tnew = m.T_amb;
+        a.Tout[idx] = Tnew;
+
+        if (a.accum_on) {
+            const bool s2n = Ti >= m.T_liq;
+            const bool reset = (!a.S2prev[idx]) && s2n;
+            const int lx0 = d->l0_lo3[0] + ix, ly0 = d->l0_lo3[1] + iy, lz0 = d->l0_lo3[2] + iz;
+            if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] && lz0 >= 0 &&
+                lz0 < d->l0_n[2]) {
+                const int64_t l0 = nid(lx0, ly0, lz0, d->l0_n[0], d->l0_n[1]);
+                const float acc = a.accum[l0];
+                const float racc = reset ? acc : 0.0f;
+                const float mx = a.max_accum[l0];
+                a.max_accum[l0] = racc > mx ? racc : mx;
+                a.accum[l0] = acc + d->rows[a.src_row][5] * (s2n ? 1.0f : 0.0f) - racc;
+            }
+            a.S2out[idx] = (uint8_t)s2n;
+        }
+    }
+}
+
+void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat& m,
+                       hipStream_t s) {
+    const int ntx = (hg.n[0] + TX - 1) / TX, nty = (hg.n[1] + TY - 1) / TY,
+              ntz = (hg.n[2] + TZ - 1) / TZ;
+    const int nb = ntx * nty * ntz;
+    if (a.s1_float)
+        hipLaunchKernelGGL(k_level_step<true>, dim3(nb), dim3(256), 0, s, a, d, m);
+    else
+        hipLaunchKernelGGL(k_level_step<false>, dim3(nb), dim3(256), 0, s, a, d, m);
+}
+
+// ---------------------------------------------------------------------------
+// Fine -> coarse axis tables (the separable factors of the Shapes tensors)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int coarse_elem(const gm_grid& gc, int d, float xq) {
+    int c = (int)floorf((xq - gc.x0[d]) / gc.h[d]);
+    const int nec = gc.n[d] - 1;
+    return c < 0 ? 0 : (c > nec - 1 ? nec - 1 : c);
+}
+
+__device__ __forceinline__ float gauss_x(const gm_grid& gf, int d, int e, int gg) {
+    const float xa = coordf(gf, d, e), xb = coordf(gf, d, e + 1);
+    return gg == 0 ? (xa * GA + xb * GB) : (xa * GB + xb * GA);
+}
+
+__global__ void k_axis_tables(GmPair p, const GmStepDesc* __restrict__ d) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int ax = 0;
+    while (ax < 3 && t >= p.nef[ax]) {
+        t -= p.nef[ax];
+        ++ax;
+    }
+    if (ax >= 3) return;
+    const gm_grid& gf = d->g[p.fine];
+    const gm_grid& gc = d->g[p.coarse];
+    const int nef = p.nef[ax];
+    const GmAxisTab& tb = p.ax[ax];
+    float4 qq[2];
+    for (int gg = 0; gg < 2; ++gg) {
+        const float xq = gauss_x(gf, ax, t, gg);
+        const int c = coarse_elem(gc, ax, xq);
+        const float x0 = coordf(gc, ax, c), x1 = coordf(gc, ax, c + 1);
+        qq[gg] = make_float4(xq, (x1 - xq) / gc.h[ax], (xq - x0) / gc.h[ax], 0.0f);
+    }
+    tb.q[2 * t] = qq[0];
+    tb.q[2 * t + 1] = qq[1];
+    const int c = coarse_elem(gc, ax, gauss_x(gf, ax, t, 0));
+    tb.c0[t] = c;
+    const int clo = coarse_elem(gc, ax, gauss_x(gf, ax, 0, 0));
+    const int chi = coarse_elem(gc, ax, gauss_x(gf, ax, nef - 1, 0));
+    const int k = c - clo;
+    const int cmax = p.cmax[ax];
+    if (k >= 0 && k < cmax) {
+        if (t == 0 || coarse_elem(gc, ax, gauss_x(gf, ax, t - 1, 0)) != c) tb.info[2 + k] = t;
+        if (t == nef - 1 || coarse_elem(gc, ax, gauss_x(gf, ax, t + 1, 0)) != c)
+            tb.info[2 + cmax + k] = t + 1;
+    }
+    if (t == 0) {
+        tb.info[0] = clo;
+        const int n = chi - clo + 1;
+        tb.info[1] = n < cmax ? n : cmax;
+    }
+}
+
+void launch_axis_tables(const GmPair& p, const GmStepDesc* d, hipStream_t s) {
+    const int n = p.nef[0] + p.nef[1] + p.nef[2];
+    hipLaunchKernelGGL(k_axis_tables, dim3((n + 255) / 256), dim3(256), 0, s, p, d);
+}
+
+// ---------------------------------------------------------------------------
+// Coarse source tables (computeLevelSource, cF:2667-2730, separably):
+//   F_A = (w_f / npos) sum_pos Gx[pos][Ax] Gy[pos][Ay] Gz[pos][Az]
+//   Gd[pos][A] = sum over fine elements e with c0(e) + a = A, Gauss g:
+//                phi(e, g, a) * coef_d exp(-3 (xq - v_d)^2 / sq_d)
+// (the x table also carries 6 sqrt(3) P eta).
+// ---------------------------------------------------------------------------
+
+__global__ void k_source_tables(GmPair p, const GmStepDesc* __restrict__ d, DevMat m, int row0,
+                                int npos, int use_rows, const float* __restrict__ rows_dev,
+                                float* __restrict__ G, int nmax) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npos * 3 * nmax) return;
+    const int A = t % nmax, ax = (t / nmax) % 3, pos = t / (3 * nmax);
+    const GmAxisTab& tb = p.ax[ax];
+    const int cmax = p.cmax[ax];
+    const int cn = tb.info[1];
+    float s = 0.0f;
+    if (A <= cn) {
+        const float* row = use_rows ? rows_dev + 7 * (row0 + pos) : d->rows[row0 + pos];
+        const float v = row[ax];
+        const float sq = ax == 2 ? m.dsq : m.rsq;
+        const float coef = ax == 2 ? m.dc : m.rc;
+        for (int side = 0; side < 2; ++side) {
+            const int c = A - 1 + side;  // element below (local node 1) / above (node 0)
+            if (c < 0 || c >= cn) continue;
+            const int e0 = tb.info[2 + c], e1 = tb.info[2 + cmax + c];
+            for (int e = e0; e < e1; ++e) {
+#pragma unroll
+                for (int gg = 0; gg < 2; ++gg) {
+                    const float4 q = tb.q[2 * e + gg];
+                    const float tt = q.x - v;
+                    const float qv = coef * expf((-3.0f * (tt * tt)) / sq);
+                    s += (side == 0 ? q.z : q.y) * qv;
+                }
+            }
+        }
+        if (ax == 0) s *= m.pc6 * row[6];
+    }
+    G[t] = s;
+}
+
+void launch_source_tables(const GmPair& p, const GmStepDesc* d, const DevMat& m, int row0,
+                          int npos, int use_rows, const float* rows_dev, float* G, int nmax,
+                          hipStream_t s) {
+    const int n = npos * 3 * nmax;
+    hipLaunchKernelGGL(k_source_tables, dim3((n + 255) / 256), dim3(256), 0, s, p, d, m, row0,
+                       npos, use_rows, rows_dev, G, nmax);
+}
+
+// ---------------------------------------------------------------------------
+// Restriction of T' terms (cF:1396-1565, 2733-2810, 2857-2914, 3057-3132,
+// 3169-3221).  A group of GL lanes owns one covered coarse element c and sums,
+// in FP64, the contributions of the fine elements whose Gauss point 0 lies in
+// c to its 8 nodes:
+//   -w [ grad Nc . k_e grad T'(q) + Nc rho*cp_e dT'(q) / dt ]
+// with Nc and grad Nc evaluated (per axis) in the Gauss point's own coarse
+// element, as the reference's Shapes tensors are.
+// ---------------------------------------------------------------------------
+
+template <int GL>
+__global__ void __launch_bounds__(256) k_restrict(GmPair p, const GmStepDesc* __restrict__ d,
+                                                  DevMat m, RArgs r, double* __restrict__ part) {
+    const int lane = threadIdx.x & (GL - 1);
+    const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / GL;
+    const int cnx = p.ax[0].info[1], cny = p.ax[1].info[1], cnz = p.ax[2].info[1];
+    const bool live = gid < cnx * cny * cnz;
+    double acc[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) acc[a] = 0.0;
+    if (live) {
+        const int cx = gid % cnx, cy = (gid / cnx) % cny, cz = gid / (cnx * cny);
+        const int ex0 = p.ax[0].info[2 + cx], ex1 = p.ax[0].info[2 + p.cmax[0] + cx];
+        const int ey0 = p.ax[1].info[2 + cy], ey1 = p.ax[1].info[2 + p.cmax[1] + cy];
+        const int ez0 = p.ax[2].info[2 + cz], ez1 = p.ax[2].info[2 + p.cmax[2] + cz];
+        const int nfx = ex1 - ex0, nfy = ey1 - ey0, nfz = ez1 - ez0;
+        const int cnt = nfx * nfy * nfz;
+        const gm_grid& gf = d->g[p.fine];
+        const gm_grid& gc = d->g[p.coarse];
+        const int nx = gf.n[0], ny = gf.n[1];
+        const int64_t nxy = (int64_t)nx * ny;
+        const float ihf[3] = {1.0f / gf.h[0], 1.0f / gf.h[1], 1.0f / gf.h[2]};
+        const float ihc[3] = {1.0f / gc.h[0], 1.0f / gc.h[1], 1.0f / gc.h[2]};
+        float dt = r.dt;
+        if (r.dt_mode == GM_DT_L1) dt = d->dt1;
+        else if (r.dt_mode == GM_DT_L2) dt = d->dt2[r.dt_index];
+        else if (r.dt_mode == GM_DT_ROW) dt = d->rows[r.dt_index][5];
+        const float wflux = r.wf, wmass = (1.0f / dt) * r.wf;
+        for (int k = lane; k < cnt; k += GL) {
+            const int ex = ex0 + k % nfx, ey = ey0 + (k / nfx) % nfy, ez = ez0 + k / (nfx * nfy);
+            const int64_t base = ex + (int64_t)nx * ey + nxy * ez;
+            const int64_t off[8] = {0, 1, 1 + nx, nx, nxy, 1 + nxy, 1 + nx + nxy, nx + nxy};
+            float kk = 0.0f, cc = 0.0f, tp[8], dtp[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const int64_t n = base + off[a];
+                const float T = r.T[n];
+                const float s1 = r.S1 ? (float)r.S1[n] : r.S1f[n];
+                float kv, cv;
+                bool b1, b2;
+                props(m, T, s1, n < r.n_sub, kv, cv, b1, b2);
+                kk += kv;
+                cc += cv;
+                tp[a] = r.Tp ? r.Tp[n] : 0.0f;
+                dtp[a] = r.TpA ? (r.TpA[n] - r.TpB[n]) : 0.0f;
+            }
+            const float ke = kk * 0.125f, ce = cc * 0.125f;
+            const float4 qx[2] = {p.ax[0].q[2 * ex], p.ax[0].q[2 * ex + 1]};
+            const float4 qy[2] = {p.ax[1].q[2 * ey], p.ax[1].q[2 * ey + 1]};
+            const float4 qz[2] = {p.ax[2].q[2 * ez], p.ax[2].q[2 * ez + 1]};
+            float c8[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) c8[a] = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int gx = ((q + 1) >> 1) & 1, gy = (q >> 1) & 1, gz = q >> 2;
+                float gX = 0.0f, gY = 0.0f, gZ = 0.0f, ms = 0.0f;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const int bx = ((b + 1) >> 1) & 1, by = (b >> 1) & 1, bz = b >> 2;
+                    const float nxf = (bx == gx) ? GA : GB, nyf = (by == gy) ? GA : GB,
+                                nzf = (bz == gz) ? GA : GB;
+                    const float sx = bx ? ihf[0] : -ihf[0], sy = by ? ihf[1] : -ihf[1],
+                                sz = bz ? ihf[2] : -ihf[2];
+                    gX += sx * nyf * nzf * tp[b];
+                    gY += nxf * sy * nzf * tp[b];
+                    gZ += nxf * nyf * sz * tp[b];
+                    ms += nxf * nyf * nzf * dtp[b];
+                }
+                const float fX = ke * gX, fY = ke * gY, fZ = ke * gZ, mm = ce * ms;
+                const float px[2] = {gx ? qx[1].y : qx[0].y, gx ? qx[1].z : qx[0].z};
+                const float py[2] = {gy ? qy[1].y : qy[0].y, gy ? qy[1].z : qy[0].z};
+                const float pz[2] = {gz ? qz[1].y : qz[0].y, gz ? qz[1].z : qz[0].z};
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    const int ax = ((a + 1) >> 1) & 1, ay = (a >> 1) & 1, az = a >> 2;
+                    const float dpx = ax ? ihc[0] : -ihc[0], dpy = ay ? ihc[1] : -ihc[1],
+                                dpz = az ? ihc[2] : -ihc[2];
+                    const float Nc = px[ax] * py[ay] * pz[az];
+                    const float fl = (dpx * py[ay] * pz[az]) * fX + (px[ax] * dpy * pz[az]) * fY +
+                                     (px[ax] * py[ay] * dpz) * fZ;
+                    c8[a] += -fl * wflux - Nc * mm * wmass;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 8; ++a) acc[a] += (double)c8[a];
+        }
+    }
+#pragma unroll
+    for (int off = GL / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int a = 0; a < 8; ++a) acc[a] += __shfl_xor(acc[a], off, GL);
+    if (live && lane == 0) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) part[(int64_t)gid * 8 + a] = acc[a];
+    }
+}
+
+void launch_restrict(const GmPair& p, const GmStepDesc* d, const DevMat& m, const RArgs& r,
+                     double* part, hipStream_t s) {
+    // fine elements per coarse element decide the group width
+    int per = 1;
+    for (int dd = 0; dd < 3; ++dd) per *= (p.nef[dd] + p.cmax[dd] - 1) / p.cmax[dd];
+    const int64_t groups = (int64_t)p.cmax[0] * p.cmax[1] * p.cmax[2];
+    if (per >= 48) {
+        const int64_t thr = groups * 64;
+        hipLaunchKernelGGL(k_restrict<64>, dim3((thr + 255) / 256), dim3(256), 0, s, p, d, m, r,
+                           part);
+    } else if (per >= 12) {
+        const int64_t thr = groups * 16;
+        hipLaunchKernelGGL(k_restrict<16>, dim3((thr + 255) / 256), dim3(256), 0, s, p, d, m, r,
+                           part);
+    } else {
+        const int64_t thr = groups * 4;
+        hipLaunchKernelGGL(k_restrict<4>, dim3((thr + 255) / 256), dim3(256), 0, s, p, d, m, r,
+                           part);
+    }
+}
+
+// Sum the element partials of one (or two nested) pairs onto the coarse node
+// box of the outer pair; V has the layout [cmax+1]^3 of the outer pair.
+__global__ void k_restrict_nodes(GmPair po, const double* __restrict__ parto, GmPair pi,
+                                 const double* __restrict__ parti, int have_inner,
+                                 const float* __restrict__ Vbase, float* __restrict__ V) {
+    const int mx = po.cmax[0] + 1, my = po.cmax[1] + 1, mz = po.cmax[2] + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mx * my * mz) return;
+    const int bx = t % mx, by = (t / mx) % my, bz = t / (mx * my);
+    const int A[3] = {po.ax[0].info[0] + bx, po.ax[1].info[0] + by, po.ax[2].info[0] + bz};
+    double s = 0.0;
+    const GmPair* pp[2] = {&po, &pi};
+    const double* pa[2] = {parto, parti};
+    for (int w = 0; w < 1 + have_inner; ++w) {
+        const GmPair& P = *pp[w];
+        const int lo[3] = {P.ax[0].info[0], P.ax[1].info[0], P.ax[2].info[0]};
+        const int n[3] = {P.ax[0].info[1], P.ax[1].info[1], P.ax[2].info[1]};
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const int ax = ((a + 1) >> 1) & 1, ay = (a >> 1) & 1, az = a >> 2;
+            const int cx = A[0] - ax - lo[0], cy = A[1] - ay - lo[1], cz = A[2] - az - lo[2];
+            if (cx < 0 || cx >= n[0] || cy < 0 || cy >= n[1] || cz < 0 || cz >= n[2]) continue;
+            s += pa[w][((int64_t)cx + (int64_t)n[0] * (cy + (int64_t)n[1] * cz)) * 8 + a];
+        }
+    }
+    const float base = Vbase ? Vbase[t] : 0.0f;
+    V[t] = base + (float)s;
+}
+
+void launch_restrict_nodes(const GmPair& outer, const double* part_outer, const GmPair* inner,
+                           const double* part_inner, const float* Vbase, float* V,
+                           hipStream_t s) {
+    const int n = (outer.cmax[0] + 1) * (outer.cmax[1] + 1) * (outer.cmax[2] + 1);
+    GmPair pi = inner ? *inner : outer;
+    hipLaunchKernelGGL(k_restrict_nodes, dim3((n + 255) / 256), dim3(256), 0, s, outer, part_outer,
+                       pi, part_inner, inner ? 1 : 0, Vbase, V);
+}
+
+// ---------------------------------------------------------------------------
+// Gathers: interpolation onto grids, injection, T', window move
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ const gm_grid& pick_grid(const GmStepDesc* d, int mode,
+                                                    const gm_grid& host) {
+    if (mode == 10) return d->ov3_xyz;
+    if (mode == 11) return d->ov2_xyz;
+    if (mode >= 1 && mode <= 3) return d->g[mode];
+    return host;
+}
+
+__global__ void k_interp_grid(const GmStepDesc* __restrict__ d, int src_mode, gm_grid src_h,
+                              const float* __restrict__ u, const uint8_t* __restrict__ u8,
+                              int tgt_mode, gm_grid tgt_h, float* __restrict__ out) {
+    const gm_grid& tg = pick_grid(d, tgt_mode, tgt_h);
+    const gm_grid& sg = pick_grid(d, src_mode, src_h);
+    const int64_t n = (int64_t)tg.n[0] * tg.n[1] * tg.n[2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ix = t % tg.n[0], iy = (t / tg.n[0]) % tg.n[1], iz = t / ((int64_t)tg.n[0] * tg.n[1]);
+    float N[8];
+    int64_t nd[8];
+    interp8(sg, coordf(tg, 0, ix), coordf(tg, 1, iy), coordf(tg, 2, iz), N, nd);
+    float s = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) s += N[a] * (u ? u[nd[a]] : (float)u8[nd[a]]);
+    out[t] = s;
+}
+
+void launch_interp_grid(const GmStepDesc* d, int src_level, const gm_grid* src_host,
+                        const float* u, const uint8_t* u8, int tgt_mode, const gm_grid* tgt_host,
+                        int64_t n_target, float* out, hipStream_t s) {
+    if (n_target <= 0) return;
+    gm_grid sh = src_host ? *src_host : gm_grid{};
+    gm_grid th = tgt_host ? *tgt_host : gm_grid{};
+    hipLaunchKernelGGL(k_interp_grid, dim3((n_target + 255) / 256), dim3(256), 0, s, d,
+                       src_host ? -1 : src_level, sh, u, u8, tgt_host ? -1 : tgt_mode, th, out);
+}
+
+// CoarseT[overlap box] <- fine field at the overlap coordinates (cF:2086-2094)
+__global__ void k_inject(const GmStepDesc* __restrict__ d, int fine_level, int coarse_level,
+                         const float* __restrict__ fT, float* __restrict__ cT) {
+    const GmBox& b = fine_level == 3 ? d->ov3 : d->ov2;
+    const gm_grid& xyz = fine_level == 3 ? d->ov3_xyz : d->ov2_xyz;
+    const int n = b.n[0] * b.n[1] * b.n[2];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ix = t % b.n[0], iy = (t / b.n[0]) % b.n[1], iz = t / (b.n[0] * b.n[1]);
+    const float v = interp_f(d->g[fine_level], fT, coordf(xyz, 0, ix), coordf(xyz, 1, iy),
+                             coordf(xyz, 2, iz));
+    const gm_grid& cg = d->g[coarse_level];
+    const int cx = b.lo[0] + ix, cy = b.lo[1] + iy, cz = b.lo[2] + iz;
+    if (cx < 0 || cx >= cg.n[0] || cy < 0 || cy >= cg.n[1] || cz < 0 || cz >= cg.n[2]) return;
+    cT[nid(cx, cy, cz, cg.n[0], cg.n[1])] = v;
+}
+
+void launch_inject(const GmStepDesc* d, int fine_level, int coarse_level, const float* fineT,
+                   float* coarseT, int64_t n_box, hipStream_t s) {
+    if (n_box <= 0) return;
+    hipLaunchKernelGGL(k_inject, dim3((n_box + 255) / 256), dim3(256), 0, s, d, fine_level,
+                       coarse_level, fineT, coarseT);
+}
+
+// T' = fine - prolong(coarse) at every fine node (cF:2097)
+__global__ void k_tprime(const GmStepDesc* __restrict__ d, int fine_level, int coarse_level,
+                         const float* __restrict__ fT, const float* __restrict__ cT,
+                         float* __restrict__ Tp) {
+    const gm_grid& fg = d->g[fine_level];
+    const int64_t n = (int64_t)fg.n[0] * fg.n[1] * fg.n[2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ix = t % fg.n[0], iy = (t / fg.n[0]) % fg.n[1], iz = t / ((int64_t)fg.n[0] * fg.n[1]);
+    const float v = interp_f(d->g[coarse_level], cT, coordf(fg, 0, ix), coordf(fg, 1, iy),
+                             coordf(fg, 2, iz));
+    Tp[t] = fT[t] - v;
+}
+
+void launch_tprime(const GmStepDesc* d, int fine_level, int coarse_level, const float* fineT,
+                   const float* coarseT, float* Tp, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_tprime, dim3((n + 255) / 256), dim3(256), 0, s, d, fine_level,
+                       coarse_level, fineT, coarseT, Tp);
+}
+
+// moveEverything, L3 part (cF:2439-2444): T0 = I(L1 T0) + (I(L2 T'0) + I(L3 T'0))
+// on the moved L3 nodes, with the pre-move L2 and L3 meshes.
+__global__ void k_move_l3(const GmStepDesc* __restrict__ d, const float* __restrict__ T1, const float* __restrict__ Tp2,
+                          const float* __restrict__ Tp3, float* __restrict__ T3n,
+                          float* __restrict__ Tp3n, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const gm_grid& g3 = d->g[3];
+    const int ix = t % g3.n[0], iy = (t / g3.n[0]) % g3.n[1], iz = t / ((int64_t)g3.n[0] * g3.n[1]);
+    const float x = coordf(g3, 0, ix), y = coordf(g3, 1, iy), z = coordf(g3, 2, iz);
+    const float a = interp_f(d->g[1], T1, x, y, z);
+    const float b = interp_f(d->gprev[2], Tp2, x, y, z);
+    const float c = interp_f(d->gprev[3], Tp3, x, y, z);
+    T3n[t] = a + (b + c);
+    Tp3n[t] = c;
+}
+
+void launch_move_l3(const GmStepDesc* d, const float* T1, const float* Tp2, const float* Tp3,
+                    float* T3n, float* Tp3n, int64_t nn3, hipStream_t s) {
+    hipLaunchKernelGGL(k_move_l3, dim3((nn3 + 255) / 256), dim3(256), 0, s, d, T1, Tp2, Tp3, T3n,
+                       Tp3n, nn3);
+}
+
+// moveEverything, L2 part (cF:2460-2464)
+__global__ void k_move_l2(const GmStepDesc* __restrict__ d, const float* __restrict__ T1, const float* __restrict__ Tp2,
+                          float* __restrict__ T2n, float* __restrict__ Tp2n, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const gm_grid& g2 = d->g[2];
+    const int ix = t % g2.n[0], iy = (t / g2.n[0]) % g2.n[1], iz = t / ((int64_t)g2.n[0] * g2.n[1]);
+    const float x = coordf(g2, 0, ix), y = coordf(g2, 1, iy), z = coordf(g2, 2, iz);
+    const float tp = interp_f(d->gprev[2], Tp2, x, y, z);
+    const float a = interp_f(d->g[1], T1, x, y, z);
+    Tp2n[t] = tp;
+    T2n[t] = a + tp;
+}
+
+void launch_move_l2(const GmStepDesc* d, const float* T1, const float* Tp2, float* T2n,
+                    float* Tp2n, int64_t nn2, hipStream_t s) {
+    hipLaunchKernelGGL(k_move_l2, dim3((nn2 + 255) / 256), dim3(256), 0, s, d, T1, Tp2, T2n, Tp2n,
+                       nn2);
+}
+
+// L1 S1 on the L2 overlap <- interpolated L2 S1 (cF:3273-3277)
+__global__ void k_l1_s1_from_l2(const GmStepDesc* __restrict__ d, const uint8_t* __restrict__ S12,
+                                float* __restrict__ S11) {
+    const GmBox& b = d->ov2;
+    const int n = b.n[0] * b.n[1] * b.n[2];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ix = t % b.n[0], iy = (t / b.n[0]) % b.n[1], iz = t / (b.n[0] * b.n[1]);
+    const gm_grid& xyz = d->ov2_xyz;
+    float N[8];
+    int64_t nd[8];
+    interp8(d->g[2], coordf(xyz, 0, ix), coordf(xyz, 1, iy), coordf(xyz, 2, iz), N, nd);
+    float s = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) s += N[a] * (float)S12[nd[a]];
+    const gm_grid& g1 = d->g[1];
+    const int cx = b.lo[0] + ix, cy = b.lo[1] + iy, cz = b.lo[2] + iz;
+    if (cx < 0 || cx >= g1.n[0] || cy < 0 || cy >= g1.n[1] || cz < 0 || cz >= g1.n[2]) return;
+    S11[nid(cx, cy, cz, g1.n[0], g1.n[1])] = s;
+}
+
+void launch_l1_s1_from_l2(const GmStepDesc* d, const uint8_t* S1_2, float* S1_1, int64_t n_box,
+                          hipStream_t s) {
+    if (n_box <= 0) return;
+    hipLaunchKernelGGL(k_l1_s1_from_l2, dim3((n_box + 255) / 256), dim3(256), 0, s, d, S1_2, S1_1);
+}
+
+// ---------------------------------------------------------------------------
+// Small utilities
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_f32(float* p, float v, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+__global__ void k_fill_u8(uint8_t* p, uint8_t v, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+__global__ void k_copy_f32(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+__global__ void k_copy_u8(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+__global__ void k_clamp(float* T, int64_t n, float Tamb) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && T[t] < Tamb) T[t] = Tamb;
+}
+
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_f32, dim3(nblk(n)), dim3(256), 0, s, p, v, n);
+}
+void launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_u8, dim3(nblk(n)), dim3(256), 0, s, p, v, n);
+}
+void launch_copy_f32(float* dst, const float* src, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_copy_f32, dim3(nblk(n)), dim3(256), 0, s, dst, src, n);
+}
+void launch_copy_u8(uint8_t* dst, const uint8_t* src, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_copy_u8, dim3(nblk(n)), dim3(256), 0, s, dst, src, n);
+}
+void launch_clamp(float* T, int64_t n, float Tamb, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_clamp, dim3(nblk(n)), dim3(256), 0, s, T, n, Tamb);
+}
+
+// Level0 <-> L3/L2 state transfer (cF:2500-2502, 2390-2392, 3628-3630).
+// Level0 S2 is kept as a snapshot of the last L3 S2 and the Level0 box it was
+// written to: the reference clears all of Level0 S2 before each write, so
+// every Level0 node outside that box reads False.
+__global__ void k_l0_gather(const GmStepDesc* __restrict__ d, const uint8_t* __restrict__ L0S1,
+                            const uint8_t* __restrict__ snap, const int32_t* __restrict__ sbox,
+                            uint8_t* __restrict__ S13, uint8_t* __restrict__ S23,
+                            uint8_t* __restrict__ S12, int64_t nn3, int64_t nn2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nn3) {
+        const gm_grid& g3 = d->g[3];
+        const int ix = t % g3.n[0], iy = (t / g3.n[0]) % g3.n[1],
+                  iz = t / ((int64_t)g3.n[0] * g3.n[1]);
+        const int lx = d->l0_lo3[0] + ix, ly = d->l0_lo3[1] + iy, lz = d->l0_lo3[2] + iz;
+        const bool in0 = lx >= 0 && lx < d->l0_n[0] && ly >= 0 && ly < d->l0_n[1] && lz >= 0 &&
+                         lz < d->l0_n[2];
+        S13[t] = in0 ? L0S1[nid(lx, ly, lz, d->l0_n[0], d->l0_n[1])] : 0;
+        uint8_t s2 = 0;
+        if (sbox[3]) {
+            const int sx = lx - sbox[0], sy = ly - sbox[1], sz = lz - sbox[2];
+            if (sx >= 0 && sx < g3.n[0] && sy >= 0 && sy < g3.n[1] && sz >= 0 && sz < g3.n[2])
+                s2 = snap[nid(sx, sy, sz, g3.n[0], g3.n[1])];
+        }
+        S23[t] = s2;
+    }
+    if (t < nn2) {
+        const gm_grid& g2 = d->g[2];
+        const int ix = t % g2.n[0], iy = (t / g2.n[0]) % g2.n[1],
+                  iz = t / ((int64_t)g2.n[0] * g2.n[1]);
+        const int lx = d->l0_lo2[0] + d->l0_st2[0] * ix, ly = d->l0_lo2[1] + d->l0_st2[1] * iy,
+                  lz = d->l0_lo2[2] + d->l0_st2[2] * iz;
+        const bool in0 = lx >= 0 && lx < d->l0_n[0] && ly >= 0 && ly < d->l0_n[1] && lz >= 0 &&
+                         lz < d->l0_n[2];
+        S12[t] = in0 ? L0S1[nid(lx, ly, lz, d->l0_n[0], d->l0_n[1])] : 0;
+    }
+}
+
+void launch_l0_gather(const GmStepDesc* d, const uint8_t* L0S1, const uint8_t* s2snap,
+                      const int32_t* s2box, uint8_t* S1_3, uint8_t* S2_3, uint8_t* S1_2,
+                      int64_t nn3, int64_t nn2, hipStream_t s) {
+    const int64_t n = nn3 > nn2 ? nn3 : nn2;
+    hipLaunchKernelGGL(k_l0_gather, dim3(nblk(n)), dim3(256), 0, s, d, L0S1, s2snap, s2box, S1_3,
+                       S2_3, S1_2, nn3, nn2);
+}
+
+__global__ void k_l0_scatter(const GmStepDesc* __restrict__ d, const uint8_t* __restrict__ S13,
+                             const uint8_t* __restrict__ S23, uint8_t* __restrict__ L0S1,
+                             uint8_t* __restrict__ snap, int32_t* __restrict__ sbox, int64_t nn3) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn3) return;
+    const gm_grid& g3 = d->g[3];
+    const int ix = t % g3.n[0], iy = (t / g3.n[0]) % g3.n[1], iz = t / ((int64_t)g3.n[0] * g3.n[1]);
+    const int lx = d->l0_lo3[0] + ix, ly = d->l0_lo3[1] + iy, lz = d->l0_lo3[2] + iz;
+    if (lx >= 0 && lx < d->l0_n[0] && ly >= 0 && ly < d->l0_n[1] && lz >= 0 && lz < d->l0_n[2])
+        L0S1[nid(lx, ly, lz, d->l0_n[0], d->l0_n[1])] = S13[t];
+    snap[t] = S23[t];
+    if (t == 0) {
+        sbox[0] = d->l0_lo3[0];
+        sbox[1] = d->l0_lo3[1];
+        sbox[2] = d->l0_lo3[2];
+        sbox[3] = 1;
+    }
+}
+
+void launch_l0_scatter(const GmStepDesc* d, const uint8_t* S1_3, const uint8_t* S2_3, uint8_t* L0S1,
+                       uint8_t* s2snap, int32_t* s2box, int64_t nn3, hipStream_t s) {
+    hipLaunchKernelGGL(k_l0_scatter, dim3(nblk(nn3)), dim3(256), 0, s, d, S1_3, S2_3, L0S1, s2snap,
+                       s2box, nn3);
+}
+
+// Melt-time bookkeeping of the single-step driver path (go_melt.py:339-357).
+__global__ void k_melt_single(const GmStepDesc* __restrict__ d, const float* __restrict__ T3,
+                              const uint8_t* __restrict__ preS2, const uint8_t* __restrict__ S2,
+                              float dt, float Tliq, float* __restrict__ accum,
+                              float* __restrict__ maxacc, int64_t nn3) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn3) return;
+    const gm_grid& g3 = d->g[3];
+    const int ix = t % g3.n[0], iy = (t / g3.n[0]) % g3.n[1], iz = t / ((int64_t)g3.n[0] * g3.n[1]);
+    const int lx = d->l0_lo3[0] + ix, ly = d->l0_lo3[1] + iy, lz = d->l0_lo3[2] + iz;
+    if (lx < 0 || lx >= d->l0_n[0] || ly < 0 || ly >= d->l0_n[1] || lz < 0 || lz >= d->l0_n[2])
+        return;
+    const int64_t l0 = nid(lx, ly, lz, d->l0_n[0], d->l0_n[1]);
+    const bool reset = (!preS2[t]) && S2[t];
+    float acc = accum[l0];
+    const float racc = reset ? acc : 0.0f;
+    const float mx = maxacc[l0];
+    maxacc[l0] = racc > mx ? racc : mx;
+    acc = acc - racc;
+    acc = acc + ((T3[t] > Tliq) ? 1.0f : 0.0f) * dt;
+    accum[l0] = acc;
+}
+
+void launch_melt_single(const GmStepDesc* d, const float* T3, const uint8_t* preS2,
+                        const uint8_t* S2, float dt, float Tliq, float* accum, float* max_accum,
+                        int64_t nn3, hipStream_t s) {
+    hipLaunchKernelGGL(k_melt_single, dim3(nblk(nn3)), dim3(256), 0, s, d, T3, preS2, S2, dt, Tliq,
+                       accum, max_accum, nn3);
+}
+
+// min / max with NaN propagation (printLevelMaxMin, cF:3635-3665)
+__global__ void k_minmax(const float* __restrict__ T, int64_t n, float* out) {
+    __shared__ float smn[256], smx[256];
+    float mn = INFINITY, mx = -INFINITY;
+    bool nan = false;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const float v = T[i];
+        nan = nan || (v != v);
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    }
+    if (nan) {
+        mn = NAN;
+        mx = NAN;
+    }
+    smn[threadIdx.x] = mn;
+    smx[threadIdx.x] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float a = smn[0], b = smx[0];
+        for (int i = 1; i < 256; ++i) {
+            const float c = smn[i], e = smx[i];
+            if (c != c || a != a) a = NAN; else a = fminf(a, c);
+            if (e != e || b != b) b = NAN; else b = fmaxf(b, e);
+        }
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+void launch_minmax(const float* T, int64_t n, float* out2, hipStream_t s) {
+    hipLaunchKernelGGL(k_minmax, dim3(1), dim3(256), 0, s, T, n, out2);
+}
+
+// ---- single-operator kernels (the facade's one-function entry points) ----
+
+__global__ void k_props(DevMat m, const float* __restrict__ T, const float* __restrict__ S1,
+                        int64_t n, int64_t nsub, float* S1o, uint8_t* S2o, float* k, float* c) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float kv, cv;
+    bool s1, s2;
+    props(m, T[t], S1[t], t < nsub, kv, cv, s1, s2);
+    if (S1o) S1o[t] = s1 ? 1.0f : 0.0f;
+    if (S2o) S2o[t] = (uint8_t)s2;
+    if (k) k[t] = kv;
+    if (c) c[t] = cv;
+}
+
+void launch_props(const DevMat& m, const float* T, const float* S1, int64_t nn, int64_t nsub,
+                  float* S1o, uint8_t* S2o, float* k, float* c, hipStream_t s) {
+    hipLaunchKernelGGL(k_props, dim3(nblk(nn)), dim3(256), 0, s, m, T, S1, nn, nsub, S1o, S2o, k, c);
+}
+
+__global__ void k_source_l3_full(gm_grid g, DevMat m, float vx, float vy, float vz, float P,
+                                 float w3, float* __restrict__ F) {
+    const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ix = t % g.n[0], iy = (t / g.n[0]) % g.n[1], iz = t / ((int64_t)g.n[0] * g.n[1]);
+    const float sx = src_axis(g, 0, ix, vx, m.rsq, m.rc) * (m.pc6 * P);
+    const float sy = src_axis(g, 1, iy, vy, m.rsq, m.rc);
+    const float sz = src_axis(g, 2, iz, vz, m.dsq, m.dc);
+    F[t] = sx * sy * sz * w3;
+}
+
+void launch_source_l3_full(const gm_grid& g, const DevMat& m, float vx, float vy, float vz, float P,
+                           float* F, hipStream_t s) {
+    const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const float w3 = (g.h[0] * 0.5f) * (g.h[1] * 0.5f) * (g.h[2] * 0.5f);
+    hipLaunchKernelGGL(k_source_l3_full, dim3(nblk(n)), dim3(256), 0, s, g, m, vx, vy, vz, P, w3, F);
+}
+
+__global__ void k_convrad_full(gm_grid g, DevMat m, const float* __restrict__ T, int top,
+                               float w2, float* __restrict__ F) {
+    const int nx = g.n[0], ny = g.n[1];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nx * ny) return;
+    const int ix = t % nx, iy = t / nx;
+    float cr = 0.0f;
+    for (int e = 0; e < 4; ++e) {
+        const int ex = ix - 1 + (e & 1), ey = iy - 1 + (e >> 1);
+        if (ex < 0 || ex >= nx - 1 || ey < 0 || ey >= ny - 1) continue;
+        const float Tf[4] = {T[nid(ex, ey, top, nx, ny)], T[nid(ex + 1, ey, top, nx, ny)],
+                             T[nid(ex + 1, ey + 1, top, nx, ny)], T[nid(ex, ey + 1, top, nx, ny)]};
+        const int ax = ix - ex, ay = iy - ey;
+        const int ai = ay ? (ax ? 2 : 3) : (ax ? 1 : 0);
+        for (int q = 0; q < 4; ++q) {
+            const float Tq =
+                n2(q, 0) * Tf[0] + n2(q, 1) * Tf[1] + n2(q, 2) * Tf[2] + n2(q, 3) * Tf[3];
+            cr += n2(q, ai) * (face_flux(m, Tq) * w2);
+        }
+    }
+    F[nid(ix, iy, top, nx, ny)] += cr;
+}
+
+void launch_convrad_full(const gm_grid& g, const DevMat& m, const float* T, int64_t ne_active,
+                         float* F, hipStream_t s) {
+    const int top = (int)(ne_active / ((int64_t)(g.n[0] - 1) * (g.n[1] - 1)));
+    const float w2 = (g.h[0] * 0.5f) * (g.h[1] * 0.5f);
+    const int n = g.n[0] * g.n[1];
+    hipLaunchKernelGGL(k_convrad_full, dim3(nblk(n)), dim3(256), 0, s, g, m, T, top, w2, F);
+}
+
+__global__ void k_interp_matrix(gm_grid src, gm_grid tg, float* __restrict__ Nc,
+                                int32_t* __restrict__ node) {
+    const int64_t n = (int64_t)tg.n[0] * tg.n[1] * tg.n[2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ix = t % tg.n[0], iy = (t / tg.n[0]) % tg.n[1], iz = t / ((int64_t)tg.n[0] * tg.n[1]);
+    float N[8];
+    int64_t nd[8];
+    interp8(src, coordf(tg, 0, ix), coordf(tg, 1, iy), coordf(tg, 2, iz), N, nd);
+    for (int a = 0; a < 8; ++a) {
+        Nc[t * 8 + a] = N[a];
+        node[t * 8 + a] = (int32_t)nd[a];
+    }
+}
+
+void launch_interp_matrix(const gm_grid& src, const gm_grid& tgt, float* Nc, int32_t* node,
+                          hipStream_t s) {
+    const int64_t n = (int64_t)tgt.n[0] * tgt.n[1] * tgt.n[2];
+    hipLaunchKernelGGL(k_interp_matrix, dim3(nblk(n)), dim3(256), 0, s, src, tgt, Nc, node);
+}
+
+__global__ void k_interp_w(const float* __restrict__ Nc, const int32_t* __restrict__ node,
+                           int64_t n, const float* __restrict__ T, float* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float s = 0.0f;
+    for (int a = 0; a < 8; ++a) s += Nc[t * 8 + a] * T[node[t * 8 + a]];
+    out[t] = s;
+}
+
+void launch_interp_w(const float* Nc, const int32_t* node, int64_t n, const float* T, float* out,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_interp_w, dim3(nblk(n)), dim3(256), 0, s, Nc, node, n, T, out);
+}
+
+// Level0 layer shift (go_melt.py:264-290): v[:n2] = v[n1:]; v[n2:] = 0.
+// Run as two passes through a scratch copy held in the upper part.
+__global__ void k_shift_down_u8(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src,
+                                int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+
+void launch_shift_l0(uint8_t* L0S1, float* accum, int64_t n1, int64_t n2, int64_t nn,
+                     hipStream_t s) {
+    // n1 <= n2 and the copy goes downwards, so chunks of n1 nodes copied in
+    // order never overwrite a source chunk that is still to be read.
+    for (int64_t off = 0; off < n2; off += n1) {
+        const int64_t len = (off + n1 <= n2) ? n1 : (n2 - off);
+        hipLaunchKernelGGL(k_shift_down_u8, dim3(nblk(len)), dim3(256), 0, s, L0S1 + off,
+                           L0S1 + off + n1, len);
+        if (accum)
+            hipLaunchKernelGGL(k_copy_f32, dim3(nblk(len)), dim3(256), 0, s, accum + off,
+                               accum + off + n1, len);
+    }
+    launch_fill_u8(L0S1 + n2, 0, nn - n2, s);
+    if (accum) launch_fill_f32(accum + n2, 0.0f, nn - n2, s);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side constants
+// ---------------------------------------------------------------------------
+
+DevMat make_devmat(const gm_material& mm) {
+    DevMat m{};
+    m.T_amb = mm.T_amb;
+    m.T_sol = mm.T_solidus;
+    m.T_liq = mm.T_liquidus;
+    m.T_boil = mm.T_boiling;
+    m.k_p = mm.k_powder;
+    m.k_a0 = mm.k_bulk_a0;
+    m.k_a1 = mm.k_bulk_a1;
+    m.k_f = mm.k_fluid;
+    m.cp_a0 = mm.cp_solid_a0;
+    m.cp_a1 = mm.cp_solid_a1;
+    m.cp_m = mm.cp_mushy;
+    m.cp_f = mm.cp_fluid;
+    m.rho = mm.rho;
+    m.k_n = mm.k_table_n;
+    m.cp_n = mm.cp_table_n;
+    for (int i = 0; i < GM_MAX_KNOTS; ++i) {
+        m.kT[i] = mm.k_table_T[i];
+        m.kv[i] = mm.k_table_v[i];
+        m.cT[i] = mm.cp_table_T[i];
+        m.cv[i] = mm.cp_table_v[i];
+        m.ks[i] = 0.0f;
+        m.cs[i] = 0.0f;
+    }
+    for (int i = 0; i + 1 < mm.k_table_n; ++i)
+        m.ks[i] = (float)(((double)mm.k_table_v[i + 1] - mm.k_table_v[i]) /
+                          ((double)mm.k_table_T[i + 1] - mm.k_table_T[i]));
+    for (int i = 0; i + 1 < mm.cp_table_n; ++i)
+        m.cs[i] = (float)(((double)mm.cp_table_v[i + 1] - mm.cp_table_v[i]) /
+                          ((double)mm.cp_table_T[i + 1] - mm.cp_table_T[i]));
+    m.h_conv = mm.h_conv;
+    m.sigma_eps = mm.sigma_eps;
+    m.evc_cp = mm.evc_cp;
+    m.CT = mm.CT;
+    m.CM = mm.CM;
+    m.Lev = mm.Lev;
+    m.invTb = (float)(1.0 / (double)mm.T_boiling);
+    m.Tclip = (float)((double)mm.T_boiling + 1000.0);
+    m.Tamb4 = (float)((double)mm.T_amb * mm.T_amb * mm.T_amb * mm.T_amb);
+    m.eta = mm.laser_eta;
+    const double r = mm.laser_radius, dd = mm.laser_depth;
+    m.rsq = (float)(r * r);
+    m.dsq = (float)(dd * dd);
+    m.rc = (float)(1.0 / (r * sqrt(M_PI)));
+    m.dc = (float)(1.0 / (dd * sqrt(M_PI)));
+    m.pc6 = (float)(6.0 * sqrt(3.0) * mm.laser_eta);
+    return m;
+}
+
+// Box-element operator: K_ab = k * kc[p], p = which axes a and b differ on;
+// exact for the 2x2x2 Gauss rule of cF:692-775 (tensor products of the 1-D
+// linear element: stiffness (1/h)[1 -1; -1 1], mass (h/6)[2 1; 1 2]).
+void level_coeffs(const gm_grid& g, float kc[8], float* mrow, float* w2, float* w3) {
+    const double h[3] = {g.h[0], g.h[1], g.h[2]};
+    for (int p = 0; p < 8; ++p) {
+        double kk[3], mm[3];
+        for (int d = 0; d < 3; ++d) {
+            const bool dif = (p >> d) & 1;
+            kk[d] = (dif ? -1.0 : 1.0) / h[d];
+            mm[d] = h[d] * (dif ? 1.0 / 6.0 : 1.0 / 3.0);
+        }
+        kc[p] = (float)(kk[0] * mm[1] * mm[2] + mm[0] * kk[1] * mm[2] + mm[0] * mm[1] * kk[2]);
+    }
+    *mrow = (float)(h[0] * h[1] * h[2] / 8.0);
+    *w2 = (float)((h[0] / 2) * (h[1] / 2));
+    *w3 = (float)((h[0] / 2) * (h[1] / 2) * (h[2] / 2));
+}
+
+}  // namespace gm

@@ -1,0 +1,183 @@
+"""Input decks for the multilevel heat step, in the reference JSON schema.
+
+Each builder returns ``(solver_input, gcode_text)``.  ``solver_input`` follows
+examples/README.md of the reference (Level1/2/3, properties, nonmesh); the
+G-code is what ``nonmesh['gcode']`` would point at.  ``write_deck`` writes both
+to disk so that ``go_melt.py`` can run them from the command line.
+
+* ``single_track`` -- plain 3-level run, constant properties, one 5 mm track.
+  Levels 40x40x20 elements over 10x10x5 / 2x2x1 / 0.5x0.5x0.25 mm.
+* ``production`` -- plain 3-level run with temperature-dependent k(T), cp(T)
+  tables (8 knots, 300-3000 K), serpentine raster of 20 hatch lines of 5 mm at
+  0.1 mm spacing, P = 250 W, r = 40 um, radiation (emissivity 0.4) plus
+  convection.  L1 200x200x50 elements over 20x20x5 mm, L2 100x100x50,
+  L3 100x100x50 over 1x1x0.5 mm.  L2 spans 5x5x2.5 mm so that the L1/L2 and
+  L2/L3 spacing ratios are integers (2 and 5); a 4x4x2 mm L2 would give the
+  ratio 2.5, which the reference's integer window bookkeeping (go_melt.py:42-48,
+  computeFunctions.py:2453, 2472-2474) cannot represent.
+* ``tiny`` -- small decks for parity tests (seconds in the oracle).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+
+def _props(**kw):
+    base = {
+        "thermal_conductivity_powder": 20.0,
+        "thermal_conductivity_bulk_a0": 20.0,
+        "thermal_conductivity_bulk_a1": 0.0,
+        "thermal_conductivity_fluid_a0": 20.0,
+        "heat_capacity_solid_a0": 500.0,
+        "heat_capacity_solid_a1": 0.0,
+        "heat_capacity_mushy": 500.0,
+        "heat_capacity_fluid": 500.0,
+        "density": 8e-6,
+        "laser_radius": 0.05,
+        "laser_depth": 0.05,
+        "laser_power": 200.0,
+        "laser_absorptivity": 0.35,
+        "T_amb": 300.0,
+        "T_solidus": 1533.0,
+        "T_liquidus": 1609.0,
+        "T_boiling": 3038.0,
+        "h_conv": 1e-5,
+        "emissivity": 0.0,
+        "evaporation_coefficient": 0.0,
+        "boltzmann_constant": 1.38e-23,
+        "atomic_mass": 9.746e-26,
+        "latent_heat_evap": 6457000.0,
+        "molar_mass": 58.69,
+        "layer_height": 0.05,
+    }
+    base.update(kw)
+    return base
+
+
+def _nonmesh(**kw):
+    base = {
+        "timestep_L3": 1e-6,
+        "subcycle_num_L2": 10,
+        "subcycle_num_L3": 10,
+        "dwell_time": 0.0,
+        "wait_time": 0,
+        "output_files": 0,
+        "record_step": 100,
+        "Level1_record_step": 1,
+        "laser_velocity": 1000.0,
+        "save_path": "./results/",
+        "gcode": "./track.gcode",
+        "toolpath": "./results/toolpath.txt",
+        "layer_num": 0,
+        "info_T": 0,
+        "dwell_time_multiplier": 1,
+        "use_txt": 0,
+    }
+    base.update(kw)
+    return base
+
+
+def _level(el, x, y, z, cond=None):
+    d = {"elements": list(el), "bounds": {"x": list(x), "y": list(y), "z": list(z)}}
+    if cond is not None:
+        d["conditions"] = {"x": [cond, cond], "y": [cond, cond], "z": [cond, cond]}
+    return d
+
+
+def single_track():
+    """5 mm straight track at 1 m/s, P = 200 W, 3 x 40x40x20 elements."""
+    inp = {
+        "Level1": _level((40, 40, 20), (0, 10), (0, 10), (-5, 0), 300.0),
+        "Level2": _level((40, 40, 20), (1.5, 3.5), (4, 6), (-1, 0)),
+        "Level3": _level((40, 40, 20), (2.25, 2.75), (4.75, 5.25), (-0.25, 0)),
+        "properties": _props(laser_center=[2.5, 5.0, 0.0, 0, 0, 0, 0]),
+        "nonmesh": _nonmesh(),
+    }
+    gcode = "G0 X2.5 Y5 Z0\nG1 X7.5 Y5 Z0\n"
+    return inp, gcode
+
+
+# 8-knot tables, 300-3000 K (nickel-alloy-like; W/m K and J/kg K)
+K_TABLE = [[300.0, 500.0, 800.0, 1100.0, 1400.0, 1700.0, 2300.0, 3000.0],
+           [9.8, 12.5, 16.6, 20.5, 24.5, 28.0, 30.0, 31.5]]
+CP_TABLE = [[300.0, 500.0, 800.0, 1100.0, 1400.0, 1700.0, 2300.0, 3000.0],
+            [410.0, 460.0, 520.0, 580.0, 640.0, 700.0, 720.0, 730.0]]
+
+
+def raster_gcode(x0, x1, y0, hatch, n_lines, z=0.0):
+    """Serpentine raster: G1 scan lines joined by G0 hatch jumps."""
+    lines = [f"G0 X{x0:.4f} Y{y0:.4f} Z{z:g}"]
+    x_end = x1
+    for i in range(n_lines):
+        y = y0 + i * hatch
+        if i > 0:
+            lines.append(f"G0 X{x_start:.4f} Y{y:.4f} Z{z:g}")
+        lines.append(f"G1 X{x_end:.4f} Y{y:.4f} Z{z:g}")
+        x_start, x_end = x_end, (x0 if x_end == x1 else x1)
+    return "\n".join(lines) + "\n"
+
+
+def production():
+    """20-line raster on a 200x200x50 part-scale mesh (see module doc)."""
+    props = _props(
+        thermal_conductivity_bulk_a0=20.0, thermal_conductivity_fluid_a0=30.0,
+        thermal_conductivity_bulk_table=K_TABLE, heat_capacity_solid_table=CP_TABLE,
+        heat_capacity_mushy=800.0, heat_capacity_fluid=730.0,
+        laser_radius=0.04, laser_depth=0.04, laser_power=250.0, laser_absorptivity=0.35,
+        emissivity=0.4, laser_center=[7.5, 9.0, 0.0, 0, 0, 0, 0])
+    inp = {
+        "Level1": _level((200, 200, 50), (0, 20), (0, 20), (-5, 0), 300.0),
+        "Level2": _level((100, 100, 50), (5.0, 10.0), (6.5, 11.5), (-2.5, 0)),
+        "Level3": _level((100, 100, 50), (7.0, 8.0), (8.5, 9.5), (-0.5, 0)),
+        "properties": props,
+        "nonmesh": _nonmesh(),
+    }
+    return inp, raster_gcode(7.5, 12.5, 9.0, 0.1, 20)
+
+
+def tiny(n=(8, 8, 4), n2=2, n3=2, track=1.0, props=None, velocity=100.0):
+    """Small 3-level deck: L1 over 8x8x4 mm, L2 4x4x2 mm, L3 2x2x1 mm.
+
+    dt3 = 1 ms; at the default 100 mm/s the track takes 10 rows per mm."""
+    inp = {
+        "Level1": _level(n, (0, 8), (0, 8), (-4, 0), 300.0),
+        "Level2": _level(n, (2, 6), (2, 6), (-2, 0)),
+        "Level3": _level(n, (3, 5), (3, 5), (-1, 0)),
+        "properties": _props(**dict(dict(laser_center=[4.0, 4.0, 0.0, 0, 0, 0, 0],
+                                         laser_radius=0.5, laser_depth=0.5, laser_power=2000.0),
+                                    **(props or {}))),
+        "nonmesh": _nonmesh(subcycle_num_L2=n2, subcycle_num_L3=n3, timestep_L3=1e-3,
+                            laser_velocity=velocity),
+    }
+    gcode = f"G0 X4 Y4 Z0\nG1 X{4 + track:g} Y4 Z0\n"
+    return inp, gcode
+
+
+def write_deck(inp, gcode, directory):
+    """Write ``input.json`` and the G-code under ``directory``; paths are made
+    absolute so the deck runs from any working directory."""
+    os.makedirs(directory, exist_ok=True)
+    d = os.path.abspath(directory)
+    inp = json.loads(json.dumps(inp))
+    inp["nonmesh"]["gcode"] = os.path.join(d, "path.gcode")
+    inp["nonmesh"]["toolpath"] = os.path.join(d, "toolpath.txt")
+    inp["nonmesh"]["save_path"] = os.path.join(d, "results") + "/"
+    with open(inp["nonmesh"]["gcode"], "w") as fh:
+        fh.write(gcode)
+    path = os.path.join(d, "input.json")
+    with open(path, "w") as fh:
+        json.dump(inp, fh, indent=2)
+    return path
+
+
+def scan_length_mm(rows):
+    """Laser path length with P > 0: sum of |dxy| over rows whose power is on."""
+    rows = np.asarray(rows, dtype=np.float64)
+    if rows.shape[0] < 2:
+        return 0.0
+    d = np.hypot(np.diff(rows[:, 0]), np.diff(rows[:, 1]))
+    return float(d[rows[1:, 6] > 0].sum())

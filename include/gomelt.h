@@ -1,0 +1,252 @@
+/*
+ * gomelt.h -- C ABI of libgomelt.so, the MI355X (gfx950) implementation of
+ * GO-MELT's multilevel explicit finite-element heat step.
+ *
+ * The reference has no FFI: its boundary is the Python module surface of
+ * go_melt/computeFunctions.py, imported by go_melt/go_melt.py:10.  Each entry
+ * point below replaces one function (or one fused group of functions) of that
+ * surface; the comment above it cites the reference lines it stands in for.
+ * The Python facade go-melt_amd/computeFunctions.py binds these with ctypes
+ * and keeps the reference's names, argument order and return tuples.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative code on failure;
+ *    gm_last_error() returns the message of the last failure on this thread.
+ *  - All array arguments are device pointers (allocated with gm_malloc or any
+ *    HIP allocator) unless the name ends in _host.  Fields are FP32, node
+ *    numbering is x-fastest: i = ix + nx*(iy + ny*iz)  (reference cF:645-689).
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Calls are asynchronous with respect to the host unless stated.
+ *  - The library never frees caller buffers.
+ */
+#ifndef GOMELT_H
+#define GOMELT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Plain data                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* A uniform structured hexahedral mesh: n nodes per axis, node i of an axis
+ * at x0 + i*h (reference createMesh3D, cF:32-64; spacing from calc_length_h,
+ * cF:1916-1942). */
+typedef struct gm_grid {
+    int32_t n[3];
+    float x0[3];
+    float h[3];
+} gm_grid;
+
+/* Material, laser and surface constants, already in the units the kernels
+ * use (reference SetupProperties, cF:267-345).  k values in W/(m K) (divided
+ * by 1000 in the kernels, cF:2602), cp in J/(kg K), rho in kg/mm^3,
+ * h_conv in W/(m^2 K) (the input value times 1e6, cF:319).  When k_table_n
+ * (cp_table_n) is > 0 a piecewise-linear table of that many knots replaces
+ * the linear bulk-k (solid-cp) law; this is an extension, the reference model
+ * is the case k_table_n = cp_table_n = 0. */
+#define GM_MAX_KNOTS 16
+typedef struct gm_material {
+    float T_amb, T_solidus, T_liquidus, T_boiling;
+    float k_powder, k_bulk_a0, k_bulk_a1, k_fluid;
+    float cp_solid_a0, cp_solid_a1, cp_mushy, cp_fluid;
+    float rho;
+    int32_t k_table_n, cp_table_n;
+    float k_table_T[GM_MAX_KNOTS], k_table_v[GM_MAX_KNOTS];
+    float cp_table_T[GM_MAX_KNOTS], cp_table_v[GM_MAX_KNOTS];
+    float h_conv, sigma_eps;          /* sigma_sb * emissivity               */
+    float evc_cp, CT, CM, Lev;        /* evaporation model (cF:337-343)      */
+    float laser_eta, laser_radius, laser_depth;
+} gm_material;
+
+/* Static description of a three-level run (reference SetupLevels cF:115-264,
+ * SetupNonmesh cF:348-419, getStaticSubcycle cF:473-492). */
+typedef struct gm_config {
+    gm_grid level[4];          /* [1..3] initial L1/L2/L3 meshes; [0] Level0 */
+    float l1_bc[5];            /* L1 Dirichlet values: y0, y1, x0, x1, z0     */
+    float layer_height;        /* mm                                          */
+    int32_t n2, n3;            /* subcycle_num_L2, subcycle_num_L3            */
+    int32_t ratio12[3];        /* L1L2Eratio (go_melt.py:42-44)               */
+    int32_t ratio23[3];        /* L2L3Eratio (go_melt.py:46-48)               */
+    float move_lo[2][3];       /* window move limits of L2 and L3 (cF:1886)   */
+    float move_hi[2][3];
+    int32_t ov_lo[4][3];       /* original overlap node boxes: [2] L1 nodes in
+                                  L2, [3] L2 nodes in L3, [0] Level0 nodes in
+                                  L3, [1] Level0 nodes in L2 (strided)        */
+    int32_t ov_n[4][3];
+    float ov_x0[4][3];         /* original overlap coordinates (first node)   */
+    int32_t l0_stride_l2[3];   /* Level0 node stride of the L2 overlap        */
+    int32_t l0_layer_delta;    /* Level0 layer_idx_delta                      */
+} gm_config;
+
+/* ------------------------------------------------------------------------ */
+/* Device, memory and streams                                                */
+/* ------------------------------------------------------------------------ */
+
+const char* gm_last_error(void);
+int gm_device_init(int device);
+int gm_device_name(char* buf, int len);
+int gm_malloc(void** ptr, size_t bytes);
+int gm_free(void* ptr);
+int gm_memcpy_h2d(void* dst, const void* src_host, size_t bytes, void* stream);
+int gm_memcpy_d2h(void* dst_host, const void* src, size_t bytes, void* stream);
+int gm_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int gm_memset(void* dst, int value, size_t bytes, void* stream);
+int gm_stream_create(void** stream);
+int gm_stream_destroy(void* stream);
+int gm_stream_sync(void* stream);
+int gm_event_create(void** ev);
+int gm_event_destroy(void* ev);
+int gm_event_record(void* ev, void* stream);
+int gm_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ------------------------------------------------------------------------ */
+/* Single operators (one reference function each)                            */
+/* ------------------------------------------------------------------------ */
+
+/* computeStateProperties (cF:2567-2614): phase flags and nodal k (W/mm K),
+ * rho*cp.  S1 in/out is FP32 0/1 (S1_out may alias S1); S2_out is 0/1 bytes. */
+int gm_state_properties(const gm_material* mat, const float* T, const float* S1,
+                        int64_t nn, int64_t n_substrate, float* S1_out,
+                        uint8_t* S2_out, float* k_out, float* rhocp_out, void* stream);
+
+/* solveMatrixFreeFE (cF:582-642): T+ = [sum_e (diag M_e - K_e) T_e + F + Corr]
+ * / sum_e M_e over elements 0..ne_active-1 with element-mean k and rho*cp.
+ * F and Corr may be NULL (zero).  Nodes in no active element divide 0/0. */
+int gm_solve_matrix_free(const gm_grid* grid, int64_t ne_active, const float* k,
+                         const float* rhocp, float dt, const float* T, const float* F,
+                         const float* Corr, float* T_out, void* stream);
+
+/* computeSourcesL3 (cF:2960-3012): nodal Gaussian source of one laser
+ * position v = (x, y, z) and power P, integrated with 2x2x2 Gauss points. */
+int gm_source_l3(const gm_grid* grid, const gm_material* mat, const float* v_host,
+                 float P, float* F_out, void* stream);
+
+/* computeLevelSource (cF:2667-2730): coarse source averaged over n_pos laser
+ * rows (row stride 7 floats: x, y, z, Ljump, Ldwell, dt, P), evaluated at the
+ * fine mesh's Gauss points and restricted with the coarse shape functions. */
+int gm_level_source(const gm_grid* coarse, const gm_grid* fine, const gm_material* mat,
+                    const float* rows_host, int n_pos, float* F_out, void* stream);
+
+/* computeConvRadBC (cF:2207-2301): adds the top-face convection, radiation
+ * and evaporation flux of the top layer of elements 0..ne_active-1 to F. */
+int gm_conv_rad_bc(const gm_grid* grid, const gm_material* mat, const float* T,
+                   int64_t ne_active, float* F, void* stream);
+
+/* interpolatePoints (cF:1131-1210): trilinear interpolation of u (on src) at
+ * the nodes of the tensor grid `target`; a point whose weights leave
+ * [-0.01, 1.01] gets 0. */
+int gm_interpolate_points(const gm_grid* src, const float* u, const gm_grid* target,
+                          float* out, void* stream);
+
+/* interpolatePointsMatrix (cF:1028-1107): the weights (n, 8) and coarse node
+ * ids (n, 8) that gm_interpolate_points applies. */
+int gm_interpolate_matrix(const gm_grid* src, const gm_grid* target, float* Nc,
+                          int32_t* node, void* stream);
+
+/* interpolate_w_matrix (cF:1110-1128). */
+int gm_interpolate_w_matrix(const float* Nc, const int32_t* node, int64_t n,
+                            const float* T, float* out, void* stream);
+
+/* Restriction of fine-level T' terms onto a coarse level (the BCOO products
+ * of cF:1477-1565 / 2733-2810 / 2857-2914 for flux, cF:1396-1474 / 3057-3132 /
+ * 3169-3221 for mass):
+ *   V_A += -sum_e sum_q [grad N_A . k_e grad Tp + N_A rhocp_e (TpA - TpB)/dt] w
+ * k_e, rhocp_e are element means of the nodal properties of (T, S1) with
+ * n_substrate forced solid.  Tp (flux) or TpA/TpB (mass) may be NULL to drop
+ * that term.  Sums are accumulated in FP64.  V has the coarse node count. */
+int gm_restrict(const gm_grid* coarse, const gm_grid* fine, const gm_material* mat,
+                const float* T, const float* S1, int64_t n_substrate, const float* Tp,
+                const float* TpA, const float* TpB, float dt, float* V, void* stream);
+
+/* getNewTprime (cF:2060-2099): CoarseT[overlap box] <- fine field interpolated
+ * at those coarse nodes; then Tprime = FineT - prolong(CoarseT). */
+int gm_new_tprime(const gm_grid* fine, const float* FineT, const gm_grid* coarse,
+                  float* CoarseT, const int32_t* ov_lo_host, const int32_t* ov_n_host,
+                  float* Tprime, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* The time loop: a device-resident three-level state                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gm_engine gm_engine;
+
+/* Fields exposed by gm_engine_field. */
+enum {
+    GM_T0 = 0,        /* FP32 temperature (levels 1-3)                       */
+    GM_TPRIME0 = 1,   /* FP32 T' (levels 2-3)                                */
+    GM_S1 = 2,        /* level 1: FP32; levels 2-3: uint8; Level0: uint8     */
+    GM_S2 = 3,        /* level 3: uint8                                      */
+    GM_ACCUM = 4,     /* Level0 FP32 accumulated melt time                   */
+    GM_MAX_ACCUM = 5, /* Level0 FP32 max accumulated melt time               */
+    GM_HIST_L2 = 6,   /* corrector L2 T after L2 substep `level` (L2all)     */
+    GM_HIST_L3 = 7,   /* corrector L3 T after L2 substep `level` (L3all)     */
+    GM_HIST_TP3 = 8   /* corrector L3 T' after L2 substep `level` (L3pall)   */
+};
+
+/* SetupLevels (cF:115-264) + getStaticNodesAndElements/getStaticSubcycle:
+ * allocates all level state on the current device and initialises it to
+ * T_amb, S1 = 0.  `stream` is kept and used by every engine call. */
+int gm_engine_create(const gm_config* cfg, const gm_material* mat, void* stream,
+                     gm_engine** out);
+int gm_engine_destroy(gm_engine* e);
+
+/* Device pointer and element count of a field (see the enum above). */
+int gm_engine_field(gm_engine* e, int level, int field, void** ptr, int64_t* count);
+
+/* Current mesh of a level (1..3) after the last window move. */
+int gm_engine_grid(gm_engine* e, int level, gm_grid* out);
+
+/* Static per-layer quantities: calcStaticTmpNodesAndElements (cF:495-517)
+ * and getSubstrateNodes (cF:562-579), given by the caller. */
+int gm_engine_set_layer(gm_engine* e, int64_t l1_ne_active, int64_t l1_nn_active,
+                        const int64_t* substrate4_host);
+
+/* moveEverything (cF:2400-2510) for laser row v and start position vstart
+ * (3 floats each): moves L3 and L2 with the laser, re-interpolates T0/T' onto
+ * the moved meshes, pulls S1/S2 from Level0.  move_v_out (3 ints, host)
+ * receives the L2 move in L2 elements. */
+int gm_engine_move(gm_engine* e, const float* v_host, const float* vstart_host,
+                   int32_t* move_v_out_host);
+
+/* subcycleGOMELT (cF:3224-3632) for N2*N3 toolpath rows (row stride 7):
+ * one L1 step with L2/L3 subcycling, predictor and corrector, melt-time
+ * accumulation on Level0 and the Level0 S1/S2 update.  The first call with a
+ * given layer state records a hipGraph; later calls replay it. */
+int gm_engine_subcycle(gm_engine* e, const float* rows_host, int n_rows);
+
+/* stepGOMELT (cF:2304-2397) for one toolpath row; also applies the melt-time
+ * bookkeeping of the single-step driver path (go_melt.py:339-357). */
+int gm_engine_step(gm_engine* e, const float* row_host);
+
+/* stepGOMELTDwellTime (cF:2617-2664): L1-only step of length dt. */
+int gm_engine_step_dwell(gm_engine* e, float dt);
+
+/* Layer change of the single-step driver path (go_melt.py:198-290): shifts
+ * the L1 z axis by n_layers layer heights (re-interpolating T0), swaps the S1
+ * storage row, shifts Level0 down one layer and resets melt-time maxima. */
+int gm_engine_layer_change(gm_engine* e, int kidx, float new_l1_z0, float laser_z,
+                           int shift_level0);
+
+/* Zero Tprime0 of L2 and L3 (dwell hand-over, go_melt.py:360-368). */
+int gm_engine_clear_tprime(gm_engine* e);
+
+/* Set Level0 S1 = 1 on its first n nodes (go_melt.py:312-313). */
+int gm_engine_level0_substrate(gm_engine* e, int64_t n);
+
+/* min/max of T0 per level and a finite/range flag (printLevelMaxMin,
+ * cF:3635-3665).  Synchronous.  minmax_host: 6 floats (L1 min,max, L2, L3). */
+int gm_engine_minmax(gm_engine* e, float* minmax_host);
+
+/* Enable/disable hipGraph capture of gm_engine_subcycle (default enabled). */
+int gm_engine_use_graph(gm_engine* e, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOMELT_H */

@@ -1,0 +1,22 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "go-melt_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and libgomelt.so")
+    config.addinivalue_line("markers", "slow: long-running (skipped unless selected)")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import gomelt_oracle as O
+    O.set_precision("float64")
+    return O

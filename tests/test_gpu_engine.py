@@ -1,0 +1,84 @@
+"""GPU parity of the whole time loop (driver + engine) against the oracle.
+
+The facade driver (go-melt_amd/go_melt.py) and the oracle driver run the same
+toolpath rows on small decks; final T0 of every level is compared.  The
+kernels run FP32; against the FP64 oracle the stated bound is a relative
+max |dT| of 5e-4 (FP32 rounding of an explicit scheme over the run), and
+against the FP32 oracle (reference emulation) 1e-4.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(oracle, inp, gcode, tmp_path, precision="float64", graph=True):
+    import go_melt as G
+    import computeFunctions as cf
+    from createPath import read_rows
+    path = W.write_deck(inp, gcode, str(tmp_path))
+    import json
+    with open(path) as fh:
+        deck = json.load(fh)
+    st = {}
+    os.environ["GOMELT_NO_GRAPH"] = "" if graph else "1"
+    Lg = G.go_melt(deck, quiet=True, stats=st)
+    rows = read_rows(deck["nonmesh"]["toolpath"])
+    oracle.set_precision(precision)
+    try:
+        Lo, info = oracle.go_melt(deck, rows=rows)
+    finally:
+        oracle.set_precision("float64")
+    assert st["chunks"] == info["chunks"]
+    out = {}
+    for k in (1, 2, 3):
+        out[k] = (np.asarray(Lg[k]["T0"], np.float64), np.asarray(Lo[k]["T0"], np.float64))
+    out["S1_3"] = (np.asarray(Lg[3]["S1"]).astype(np.float64), Lo[3]["S1"])
+    out["Tp3"] = (np.asarray(Lg[3]["Tprime0"], np.float64), Lo[3]["Tprime0"])
+    out["acc"] = (np.asarray(Lg[0]["accum_time"], np.float64), info["accum"])
+    return out, st
+
+
+def _relT(a, b, Tamb=300.0):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b - Tamb)))
+
+
+@pytest.mark.parametrize("case", ["plain", "melt", "tables"])
+def test_driver_vs_oracle(oracle, tmp_path, case):
+    props = {}
+    if case == "melt":
+        props = {"laser_power": 600.0, "emissivity": 0.4, "evaporation_coefficient": 0.82,
+                 "thermal_conductivity_bulk_a1": 0.016, "heat_capacity_solid_a1": 0.174,
+                 "heat_capacity_mushy": 3235.0, "heat_capacity_fluid": 769.0,
+                 "thermal_conductivity_fluid_a0": 29.0}
+    if case == "tables":
+        props = {"laser_power": 600.0, "emissivity": 0.4,
+                 "thermal_conductivity_bulk_table": W.K_TABLE,
+                 "heat_capacity_solid_table": W.CP_TABLE}
+    inp, g = W.tiny(n=(8, 8, 4), n2=2, n3=3, track=1.6, props=props)
+    out, st = _run_both(oracle, inp, g, tmp_path)
+    assert st["subcycle"] >= 1 and st["single"] >= 1
+    for k in (1, 2, 3):
+        a, b = out[k]
+        assert _relT(a, b) < 5e-4, (k, _relT(a, b))
+    assert np.array_equal(out["S1_3"][0], out["S1_3"][1])
+
+
+def test_driver_vs_fp32_oracle(oracle, tmp_path):
+    inp, g = W.tiny(n=(8, 8, 4), n2=2, n3=2, track=1.2)
+    out, _ = _run_both(oracle, inp, g, tmp_path, precision="float32")
+    for k in (1, 2, 3):
+        a, b = out[k]
+        assert float(np.max(np.abs(a - b) / np.abs(b))) < 1e-4
+
+
+def test_graph_replay_matches_direct(oracle, tmp_path):
+    inp, g = W.tiny(n=(8, 8, 4), n2=2, n3=2, track=2.0)
+    a, _ = _run_both(oracle, inp, g, tmp_path / "g", graph=True)
+    b, _ = _run_both(oracle, inp, g, tmp_path / "d", graph=False)
+    for k in (1, 2, 3):
+        assert np.array_equal(a[k][0], b[k][0])
