@@ -1,0 +1,165 @@
+"""Benchmark: laser-scan mm simulated per wall-second on one MI355X.
+
+Workload: the production deck of go-melt_amd/workloads.py (plain 3-level run,
+FP32, temperature-dependent k(T)/cp(T) tables, 20-line serpentine raster at
+1 m/s, P = 250 W; L1 200x200x50, L2 100x100x50, L3 100x100x50 elements;
+N2 = N3 = 10).  A "step" is one L1 step of the reference driver: one chunk of
+N2*N3 = 100 toolpath rows, i.e. moveEverything + subcycleGOMELT (200 L3, 20 L2
+and 2 L1 solves with all cross-level terms).  W warmup steps (the first is the
+driver's single-step chunk; the hipGraph is recorded in the next) are followed
+by exactly K timed steps; the default K = 200 is the deck's 20,000 L3 steps.
+
+value = laser path length with the laser on over the timed rows / wall time.
+The JSON line also carries element updates per second, the roofline of the
+L3 level-step kernel (HBM bytes per launch from the algorithmic 10 B/node,
+duration from HIP events on the engine stream) and the CPU baseline (the
+NumPy oracle on one core, a bounded sample of the same L1 step, extrapolated).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "go-melt_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "scan mm per wall-s, plain vs surrogate (≤2% melt-pool err); % HBM peak"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--deck", default="production", choices=["production", "single_track"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernel-iters", type=int, default=400)
+    return ap.parse_args()
+
+
+def cpu_baseline(inp, gcode, n23, mm_per_step):
+    """NumPy oracle (FP32, one core) on a bounded sample of one L1 step."""
+    import numpy as np
+    from oracle import gomelt_oracle as O
+    O.set_precision(np.float32)
+    nm = O.SetupNonmesh(dict(inp["nonmesh"], save_path="/tmp/"))
+    p = O.SetupProperties(inp["properties"])
+    _, text = O.parsingGcode(dict(nm, toolpath=None), p, None, gcode_text=gcode)
+    lines = text.splitlines()[n23:2 * n23]
+    rows = np.array([[float(v) for v in ln.split(",")] for ln in lines])
+    t = O.sample_l1_step_cost(inp, rows)
+    return {
+        "value": mm_per_step / t["step_s"],
+        "unit": "mm/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": ("NumPy oracle, FP32, 1 core, same deck: moveEverything + L1 predictor "
+                   "(sources, T' flux restriction, solve) + one Part-1 L2 substep (L2 solve, "
+                   f"{t['N3']} L3 solves, T'), measured {t['measured_s']:.1f} s; L1-step time "
+                   f"extrapolated as move + 2*L1 + 2*N2*substep = {t['step_s']:.1f} s"),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(instances shard with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import workloads as W
+    import go_melt as G
+    import _native as N
+
+    inp, gcode = getattr(W, a.deck)()
+    work = tempfile.mkdtemp(prefix="gomelt_bench_")
+    deck_path = W.write_deck(inp, gcode, work)
+    with open(deck_path) as fh:
+        deck = json.load(fh)
+    run = G.TimeLoop(deck, quiet=True)
+    it = run.chunks()
+    for _ in range(a.warmup):
+        run.step_chunk(next(it))
+    run.eng.sync()
+    mm0 = run.stats["mm"]
+    nsub0 = run.stats["subcycle"]
+    nsing0 = run.stats["single"]
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        rows = next(it)
+        if not run.step_chunk(rows):
+            break
+    run.eng.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    mm = run.stats["mm"] - mm0
+    n_sub = run.stats["subcycle"] - nsub0
+    n_single = run.stats["single"] - nsing0
+    L = run.Levels
+    N2, N3 = run.subcycle[0], run.subcycle[1]
+    ne1, ne2, ne3 = L[1]["ne"], L[2]["ne"], L[3]["ne"]
+    elem_updates = n_sub * (2 * N2 * N3 * ne3 + 2 * N2 * ne2 + 2 * ne1) + \
+        n_single * 2 * (ne1 + ne2 + ne3)
+
+    # roofline of the dominant kernel: the L3 level step
+    import ctypes as C
+    ms = C.c_float()
+    N.check(N.lib().gm_engine_time_kernel(run.eng.h, 3, a.kernel_iters, C.byref(ms)))
+    nn3 = L[3]["nn"]
+    bytes_per_launch = 10 * nn3          # FP32 T read + write, uint8 S1 read + write
+    achieved = bytes_per_launch / (ms.value * 1e-3) / 1e9
+    mm_T = np.asarray(L[3]["T0"])
+    result = {
+        "metric": METRIC,
+        "value": mm / wall,
+        "unit": "mm/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1000.0 * wall / max(a.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (generated toolpath and input deck; no dataset)",
+        "config": {
+            "workload": f"{a.deck} deck: L1 {L[1]['elements']}, L2 {L[2]['elements']}, "
+                        f"L3 {L[3]['elements']} elements, N2={N2}, N3={N3}, "
+                        f"{a.steps * N2 * N3} L3 steps timed",
+            "scan_mm": mm,
+            "element_updates_per_s": elem_updates / wall,
+            "subcycled_steps": n_sub,
+            "single_steps": n_single,
+            "peak_T_L3": float(mm_T.max()),
+            "device": N.device_name(),
+        },
+        "roofline": {
+            "kernel": "k_level_step<false> (L3)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bytes_per_launch,
+            "us_per_launch": ms.value * 1000.0,
+        },
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        try:
+            result["cpu_baseline"] = cpu_baseline(inp, gcode, N2 * N3, mm / max(a.steps, 1))
+        except Exception as exc:  # the baseline must never hide the GPU line
+            result["cpu_baseline"] = {"error": repr(exc)}
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -127,6 +127,7 @@ _SIGS = {
     "gm_engine_level0_substrate": (_I, [_P, _I64]),
     "gm_engine_minmax": (_I, [_P, _FP]),
     "gm_engine_use_graph": (_I, [_P, _I]),
+    "gm_engine_time_kernel": (_I, [_P, _I, _I, _FP]),
 }
 
 SYMBOLS = tuple(_SIGS)

@@ -1236,6 +1236,56 @@ void gm_vmax(float* a, const float* b, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_vmax, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n);
 }
 
+// Times `iters` back-to-back launches of one kernel of the L1 step with the
+// engine's current state (the Part-1 arguments of the first substep), recorded
+// as a hipGraph and bracketed by HIP events on the engine stream.
+//   which = 3: L3 level step, 2: L2 level step, 1: L1 level step,
+//           4: L3 -> L2 restriction (elements + nodes)
+extern "C" int gm_engine_time_kernel(gm_engine* e, int which, int iters, float* ms_per_launch) {
+    if (iters < 1) return fail("iters must be >= 1");
+    hipGraph_t g;
+    hipGraphExec_t ge;
+    HIPCHK(hipStreamSynchronize(e->s));
+    HIPCHK(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < iters; ++i) {
+        const int b = i & 1;
+        if (which == 3)
+            run_l3(e, b ? e->T3b[0] : e->T3, e->S13, e->S13b[b], 0, e->T2, e->T2, 1.0f,
+                   e->T3b[b ? 1 : 0], 1, nullptr, nullptr);
+        else if (which == 2)
+            run_l2(e, b ? e->T2b[0] : e->T2, e->S12, e->S12b[b], e->G2, e->N3, e->V2, e->T1,
+                   e->T1, 1.0f, e->T2b[b ? 1 : 0], GM_DT_L2, 0, 0.0f, 1);
+        else if (which == 1)
+            run_l1(e, e->G1, e->N23, e->V1, b ? e->T1c : e->T1p, GM_DT_L1, 0.0f, 1);
+        else if (which == 4) {
+            launch_restrict(e->pair[2].p, e->d_desc, e->dm,
+                            rargs(e, 3, e->T3, e->S13, e->Tp3, e->histTp[0], e->Tp3, GM_DT_L2, 0,
+                                  0.0f),
+                            e->part[2], e->s);
+            launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2,
+                                  e->s);
+        }
+    }
+    HIPCHK(hipStreamEndCapture(e->s, &g));
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipGraphLaunch(ge, e->s));  // warm
+    HIPCHK(hipEventRecord(a, e->s));
+    HIPCHK(hipGraphLaunch(ge, e->s));
+    HIPCHK(hipEventRecord(b, e->s));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    *ms_per_launch = ms / (float)iters;
+    HIPCHK(hipEventDestroy(a));
+    HIPCHK(hipEventDestroy(b));
+    HIPCHK(hipGraphExecDestroy(ge));
+    HIPCHK(hipGraphDestroy(g));
+    return 0;
+}
+
 extern "C" int gm_engine_minmax(gm_engine* e, float* out) {
     launch_minmax(e->T1, e->nn[1], e->mm, e->s);
     launch_minmax(e->T2, e->nn[2], e->mm + 2, e->s);

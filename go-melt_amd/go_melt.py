@@ -11,6 +11,7 @@ go_melt/go_melt.py:
   * rows past ``wait_time`` laser-off steps run the L1-only dwell step
     (gm:358-379); layer changes shift the L1 z axis and Level0 (gm:198-290).
 Fields stay on the GPU; the host only reads them for output and checks.
+``TimeLoop`` exposes the loop one chunk at a time (the benchmark uses it).
 """
 
 from __future__ import annotations
@@ -46,181 +47,197 @@ def _chunks(path, n):
                 return
 
 
-def go_melt(solver_input: dict, quiet: bool = False, stats: dict = None):
-    """gm:16-530.  ``stats`` (optional dict) receives counters and timings."""
-    tstart = time.time()
-    say = (lambda *a, **k: None) if quiet else print
+class TimeLoop:
+    """The set-up and per-chunk body of the reference driver (gm:16-493)."""
+
     level_names = ["L1", "L2", "L3"]
-    Properties = SetupProperties(solver_input.get("properties", {}))
-    Levels = SetupLevels(solver_input, Properties)
-    Nonmesh = SetupNonmesh(solver_input.get("nonmesh", {}))
-    ne_nn = getStaticNodesAndElements(Levels)
-    subcycle = getStaticSubcycle(Nonmesh)
-    eng = cf._eng(Levels)
-    L1L2Eratio, L2L3Eratio = eng.r12, eng.r23
 
-    if Nonmesh["use_txt"]:
-        move_mesh = count_lines(Nonmesh["toolpath"])
-    else:
-        move_mesh = parsingGcode(Nonmesh, Properties, Levels[2]["h"])
-    total_t_inc = move_mesh
+    def __init__(self, solver_input, quiet=False):
+        self.tstart = time.time()
+        self.say = (lambda *a, **k: None) if quiet else print
+        self.quiet = quiet
+        self.Properties = SetupProperties(solver_input.get("properties", {}))
+        self.Levels = SetupLevels(solver_input, self.Properties)
+        self.Nonmesh = SetupNonmesh(solver_input.get("nonmesh", {}))
+        self.ne_nn = getStaticNodesAndElements(self.Levels)
+        self.subcycle = getStaticSubcycle(self.Nonmesh)
+        self.eng = cf._eng(self.Levels)
+        nm = self.Nonmesh
+        if nm["use_txt"]:
+            self.total_t_inc = count_lines(nm["toolpath"])
+        else:
+            self.total_t_inc = parsingGcode(nm, self.Properties, self.Levels[2]["h"])
+        if not self.Properties["laser_center"]:
+            with open(nm["toolpath"], "r") as fh:
+                self.laser_start = np.array([float(v) for v in fh.readline().split(",")])
+        else:
+            self.laser_start = np.array(self.Properties["laser_center"])
+        self.LInterp = [("L1->L2",), ("L2->L3",)]
+        self.time_inc = self.record_inc = self.wait_inc = 0
+        self.t_output = 0.0
+        saveResults(self.Levels, nm, int(self.time_inc / nm["record_step"]) + 1)
+        self.laser_prev_z = float("inf")
+        self.dwell_count = 0.0
+        self.move_hist = [0, 0, 0]
+        self.Shapes = self.tmp_ne_nn = self.substrate = None
+        self.new_checkpoint = self.load_chkpt = False
+        self.layer_check = nm["layer_num"] + nm["restart_layer_num"]
+        self.stats = {"single": 0, "subcycle": 0, "dwell": 0, "chunks": [], "mm": 0.0}
+        self.prev_row = None
+        self.done = False
 
-    if not Properties["laser_center"]:
-        with open(Nonmesh["toolpath"], "r") as fh:
-            laser_start = np.array([float(v) for v in fh.readline().split(",")])
-    else:
-        laser_start = np.array(Properties["laser_center"])
-    LInterp = [("L1->L2",), ("L2->L3",)]
+    def chunks(self):
+        return _chunks(self.Nonmesh["toolpath"], self.subcycle[2])
 
-    time_inc = record_inc = wait_inc = 0
-    t_output = 0.0
-    saveResults(Levels, Nonmesh, int(time_inc / Nonmesh["record_step"]) + 1)
-    laser_prev_z = float("inf")
-    dwell_count = 0.0
-    move_hist = [0, 0, 0]
-    Shapes = None
-    tmp_ne_nn = substrate = None
-    new_checkpoint = load_chkpt = False
-    layer_check = Nonmesh["layer_num"] + Nonmesh["restart_layer_num"]
-    st = {"single": 0, "subcycle": 0, "dwell": 0, "loop_s": 0.0, "rows": 0, "mm": 0.0,
-          "chunks": []}
-    prev_row = None
-    N23 = subcycle[2]
-    t_loop0 = time.time()
-    for rows in _chunks(Nonmesh["toolpath"], N23):
+    # -- one chunk of N2*N3 rows ------------------------------------------
+    def step_chunk(self, rows):
+        """Advance the simulation over one chunk; returns False when the loop
+        should stop (end of file or the checkpoint layer limit)."""
+        L, P, nm, sub = self.Levels, self.Properties, self.Nonmesh, self.subcycle
+        eng = self.eng
         t_loop = time.time()
         if not rows:
-            break
-        ongoing = len(rows) == N23
-        laser_all = np.array(rows, dtype=np.float64)
-        t_add = laser_all.shape[0]
-        la32 = laser_all.astype(np.float32)
+            return False
+        ongoing = len(rows) == sub[2]
+        la = np.array(rows, dtype=np.float64)
+        la32 = la.astype(np.float32)
         jump = False
-        if t_add > 1:
+        if la.shape[0] > 1:
             jump = bool((np.abs(np.diff(la32, axis=0)[:, :2] / la32[:-1, 5].max())
-                         > 100 * Nonmesh["laser_velocity"]).any())
-        single_step = (bool(np.any(laser_all[:, 2] != laser_prev_z)) or load_chkpt
-                       or wait_inc > max(0, Nonmesh["wait_time"] - subcycle[0] * subcycle[1] * 2)
-                       or not ongoing or t_add == 1 or jump)
-        st["chunks"].append("single" if single_step else "subcycle")
-        if single_step:
-            for laser_pos in laser_all:
-                wait_inc = wait_inc + 1 if laser_pos[4] == 0 else 0
-                if laser_pos[2] != laser_prev_z and time_inc > 0 and not load_chkpt:
-                    new_checkpoint = True
-                move_vert = False
-                if laser_pos[2] != laser_prev_z:
-                    tmp = copy.deepcopy(Levels[1]["orig_node_coords"])
-                    kidx = 0
-                    while not np.isclose(tmp[2] - laser_pos[2], 0, atol=1e-4).any():
-                        tmp[2] = (tmp[2] + np.float32(Properties["layer_height"])).astype(np.float32)
-                        kidx += 1
-                    if not load_chkpt:
-                        cf.N.check(cf.N.lib().gm_engine_layer_change(
-                            eng.h, kidx, float(tmp[2][0]), float(laser_pos[2]), 1))
-                        Levels[1]["node_coords"] = copy.deepcopy(tmp)
-                        L0 = Levels[0]
-                        L0["node_coords"][2] = (L0["orig_node_coords"][2] + laser_pos[2]
-                                                - L0["orig_node_coords"][2][-1]).astype(np.float32)
-                    tmp_ne_nn = calcStaticTmpNodesAndElements(Levels, laser_pos)
-                    laser_prev_z = laser_pos[2]
-                    wait_inc = 0
-                    move_vert = True
-                Levels, Shapes, LInterp, move_hist = moveEverything(
-                    laser_pos, laser_start, Levels, move_hist, LInterp, L1L2Eratio, L2L3Eratio,
-                    Properties["layer_height"])
-                if move_vert:
-                    eng.sync_coords()
-                    substrate = getSubstrateNodes(Levels)
-                    cf.N.check(cf.N.lib().gm_engine_level0_substrate(eng.h, substrate[0]))
-                    say("Start of new layer")
-                if wait_inc <= Nonmesh["wait_time"]:
-                    Levels, _ = stepGOMELT(Levels, ne_nn, tmp_ne_nn, Shapes, LInterp, laser_pos,
-                                           Properties, laser_pos[5], laser_pos[6], substrate)
-                    st["single"] += 1
-                    if Nonmesh["info_T"]:
-                        say(f"Step {time_inc + 1} / {total_t_inc}")
-                        printLevelMaxMin(Levels, level_names)
-                else:
-                    tp2 = np.asarray(Levels[2]["Tprime0"])
-                    tp3 = np.asarray(Levels[3]["Tprime0"])
-                    if not (tp2 == 0).all() and not (tp3 == 0).all():
-                        dwell_count = Nonmesh["wait_time"] * Nonmesh["timestep_L3"]
-                        cf.N.check(cf.N.lib().gm_engine_clear_tprime(eng.h))
-                    Levels = stepGOMELTDwellTime(Levels, tmp_ne_nn, ne_nn, Properties,
-                                                 laser_pos[5], substrate)
-                    dwell_count += laser_pos[5]
-                    st["dwell"] += 1
-                    say(f"Dwell Time {dwell_count:.6f} s")
-                time_inc += 1
-                record_inc += 1
-            if new_checkpoint:
-                Nonmesh["layer_num"] += 1
-                say(f"Saving checkpoint for Layer {Nonmesh['layer_num']}")
-                cdir = Path(Nonmesh["save_path"] + "checkpoint").absolute()
+                         > 100 * nm["laser_velocity"]).any())
+        single = (bool(np.any(la[:, 2] != self.laser_prev_z)) or self.load_chkpt
+                  or self.wait_inc > max(0, nm["wait_time"] - sub[0] * sub[1] * 2)
+                  or not ongoing or la.shape[0] == 1 or jump)
+        self.stats["chunks"].append("single" if single else "subcycle")
+        keep_going = True
+        if single:
+            for pos in la:
+                self._single_row(pos)
+            if self.new_checkpoint:
+                nm["layer_num"] += 1
+                self.say(f"Saving checkpoint for Layer {nm['layer_num']}")
+                cdir = Path(nm["save_path"] + "checkpoint").absolute()
                 os.makedirs(cdir, exist_ok=True)
-                save_object([Levels, time_inc, record_inc],
-                            cdir.joinpath(f"Checkpoint{str(Nonmesh['layer_num']).zfill(4)}.pkl"))
-                if Nonmesh["layer_num"] == layer_check:
-                    break
-                new_checkpoint = False
-                load_chkpt = True
+                save_object([L, self.time_inc, self.record_inc],
+                            cdir.joinpath(f"Checkpoint{str(nm['layer_num']).zfill(4)}.pkl"))
+                if nm["layer_num"] == self.layer_check:
+                    keep_going = False
+                self.new_checkpoint = False
+                self.load_chkpt = True
             else:
-                load_chkpt = False
+                self.load_chkpt = False
         else:
-            wait_inc = (wait_inc + len(laser_all) - laser_all[:, 4].sum()
-                        if (laser_all[:, 4] == 0).any() else 0)
-            Levels, Shapes, LInterp, move_hist = moveEverything(
-                laser_all[0, :], laser_start, Levels, move_hist, LInterp, L1L2Eratio, L2L3Eratio,
-                Properties["layer_height"])
-            Levels, _, _, _, _, _ = subcycleGOMELT(
-                Levels, ne_nn, Shapes, substrate, LInterp, tmp_ne_nn, laser_all, Properties,
-                laser_all[:, 6], subcycle)
-            st["subcycle"] += 1
-            time_inc += t_add
-            record_inc += t_add
-        t_output += laser_all[:, 5].sum()
-        if prev_row is not None:
-            seg = np.vstack([prev_row[None, :], laser_all])
-        else:
-            seg = laser_all
+            self.wait_inc = (self.wait_inc + len(la) - la[:, 4].sum()
+                             if (la[:, 4] == 0).any() else 0)
+            _, self.Shapes, self.LInterp, self.move_hist = moveEverything(
+                la[0, :], self.laser_start, L, self.move_hist, self.LInterp, eng.r12, eng.r23,
+                P["layer_height"])
+            subcycleGOMELT(L, self.ne_nn, self.Shapes, self.substrate, self.LInterp,
+                           self.tmp_ne_nn, la, P, la[:, 6], sub)
+            self.stats["subcycle"] += 1
+            self.time_inc += la.shape[0]
+            self.record_inc += la.shape[0]
+        self.t_output += la[:, 5].sum()
+        seg = la if self.prev_row is None else np.vstack([self.prev_row[None, :], la])
         d = np.hypot(np.diff(seg[:, 0]), np.diff(seg[:, 1]))
-        st["mm"] += float(d[seg[1:, 6] > 0].sum())
-        prev_row = laser_all[-1]
-        if record_inc >= Nonmesh["record_step"]:
-            record_inc = 0
-            saveResults(Levels, Nonmesh, int(time_inc / Nonmesh["record_step"]) + 1)
-        if Nonmesh["info_T"]:
-            printLevelMaxMin(Levels, level_names)
-        tend = time.time()
-        if not quiet:
+        self.stats["mm"] += float(d[seg[1:, 6] > 0].sum())
+        self.prev_row = la[-1]
+        if self.record_inc >= nm["record_step"]:
+            self.record_inc = 0
+            saveResults(L, nm, int(self.time_inc / nm["record_step"]) + 1)
+        if nm["info_T"]:
+            printLevelMaxMin(L, self.level_names)
+        if not self.quiet:
+            tend = time.time()
             t_now = 1000 * (tend - t_loop)
-            t_avg = 1000 * (tend - tstart) / max(time_inc, 1)
-            rem = ((tend - t_loop) / subcycle[2]) * (total_t_inc - time_inc) / 3600
+            t_avg = 1000 * (tend - self.tstart) / max(self.time_inc, 1)
+            rem = ((tend - t_loop) / sub[2]) * (self.total_t_inc - self.time_inc) / 3600
             print("%d/%d, Real: %.6f s, Wall: %.2f s, Loop: %5.2f ms, Avg: %5.2f ms/dt"
-                  % (time_inc, total_t_inc, t_output, tend - tstart, t_now, t_avg))
-            print("Laser location: X: %.2f, Y: %.2f, Z: %.2f"
-                  % (laser_all[-1, 0], laser_all[-1, 1], laser_all[-1, 2]))
+                  % (self.time_inc, self.total_t_inc, self.t_output, tend - self.tstart,
+                     t_now, t_avg))
+            print("Laser location: X: %.2f, Y: %.2f, Z: %.2f" % (la[-1, 0], la[-1, 1], la[-1, 2]))
             print(f"Estimated execution time remaining: {rem:.4f} hours")
-        if not ongoing:
+        return keep_going and ongoing
+
+    def _single_row(self, pos):
+        L, P, nm, eng = self.Levels, self.Properties, self.Nonmesh, self.eng
+        self.wait_inc = self.wait_inc + 1 if pos[4] == 0 else 0
+        if pos[2] != self.laser_prev_z and self.time_inc > 0 and not self.load_chkpt:
+            self.new_checkpoint = True
+        move_vert = False
+        if pos[2] != self.laser_prev_z:
+            tmp = copy.deepcopy(L[1]["orig_node_coords"])
+            kidx = 0
+            while not np.isclose(tmp[2] - pos[2], 0, atol=1e-4).any():
+                tmp[2] = (tmp[2] + np.float32(P["layer_height"])).astype(np.float32)
+                kidx += 1
+            if not self.load_chkpt:
+                cf.N.check(cf.N.lib().gm_engine_layer_change(eng.h, kidx, float(tmp[2][0]),
+                                                             float(pos[2]), 1))
+                L[1]["node_coords"] = copy.deepcopy(tmp)
+                L0 = L[0]
+                L0["node_coords"][2] = (L0["orig_node_coords"][2] + pos[2]
+                                        - L0["orig_node_coords"][2][-1]).astype(np.float32)
+            self.tmp_ne_nn = calcStaticTmpNodesAndElements(L, pos)
+            self.laser_prev_z = pos[2]
+            self.wait_inc = 0
+            move_vert = True
+        _, self.Shapes, self.LInterp, self.move_hist = moveEverything(
+            pos, self.laser_start, L, self.move_hist, self.LInterp, eng.r12, eng.r23,
+            P["layer_height"])
+        if move_vert:
+            eng.sync_coords()
+            self.substrate = getSubstrateNodes(L)
+            cf.N.check(cf.N.lib().gm_engine_level0_substrate(eng.h, self.substrate[0]))
+            self.say("Start of new layer")
+        if self.wait_inc <= nm["wait_time"]:
+            stepGOMELT(L, self.ne_nn, self.tmp_ne_nn, self.Shapes, self.LInterp, pos, P, pos[5],
+                       pos[6], self.substrate)
+            self.stats["single"] += 1
+            if nm["info_T"]:
+                self.say(f"Step {self.time_inc + 1} / {self.total_t_inc}")
+                printLevelMaxMin(L, self.level_names)
+        else:
+            tp2, tp3 = np.asarray(L[2]["Tprime0"]), np.asarray(L[3]["Tprime0"])
+            if not (tp2 == 0).all() and not (tp3 == 0).all():
+                self.dwell_count = nm["wait_time"] * nm["timestep_L3"]
+                cf.N.check(cf.N.lib().gm_engine_clear_tprime(eng.h))
+            stepGOMELTDwellTime(L, self.tmp_ne_nn, self.ne_nn, P, pos[5], self.substrate)
+            self.dwell_count += pos[5]
+            self.stats["dwell"] += 1
+            self.say(f"Dwell Time {self.dwell_count:.6f} s")
+        self.time_inc += 1
+        self.record_inc += 1
+
+    def finish(self):
+        """Final output (gm:495-530)."""
+        L, nm = self.Levels, self.Nonmesh
+        self.eng.sync()
+        self.eng.sync_coords()
+        if nm["output_files"] == 1:
+            saveState(L[0], "Level0_", nm["layer_num"], nm["save_path"], 0)
+        saveResultsFinal(L, nm)
+        np.savez(f"{nm['save_path']}FinalTemperatureFields", L1T=np.asarray(L[1]["T0"]),
+                 L2T=np.asarray(L[2]["T0"]), L3T=np.asarray(L[3]["T0"]))
+        acc = np.maximum(np.asarray(L[0]["accum_time"]), np.asarray(L[0]["max_accum_time"]))
+        np.savez(nm["save_path"] + "accum_time" + str(nm["layer_num"]).zfill(4), accum_time=acc)
+        self.say("End of simulation")
+
+
+def go_melt(solver_input: dict, quiet: bool = False, stats: dict = None):
+    """gm:16-530.  ``stats`` (optional dict) receives counters and timings."""
+    run = TimeLoop(solver_input, quiet=quiet)
+    t0 = time.time()
+    for rows in run.chunks():
+        if not run.step_chunk(rows):
             break
-    eng.sync()
-    st["loop_s"] = time.time() - t_loop0
-    st["rows"] = time_inc
-    eng.sync_coords()
-    if Nonmesh["output_files"] == 1:
-        saveState(Levels[0], "Level0_", Nonmesh["layer_num"], Nonmesh["save_path"], 0)
-    saveResultsFinal(Levels, Nonmesh)
-    np.savez(f"{Nonmesh['save_path']}FinalTemperatureFields",
-             L1T=np.asarray(Levels[1]["T0"]), L2T=np.asarray(Levels[2]["T0"]),
-             L3T=np.asarray(Levels[3]["T0"]))
-    acc = np.maximum(np.asarray(Levels[0]["accum_time"]), np.asarray(Levels[0]["max_accum_time"]))
-    np.savez(Nonmesh["save_path"] + "accum_time" + str(Nonmesh["layer_num"]).zfill(4),
-             accum_time=acc)
-    say("End of simulation")
+    run.eng.sync()
+    run.stats["loop_s"] = time.time() - t0
+    run.stats["rows"] = run.time_inc
+    run.finish()
     if stats is not None:
-        stats.update(st)
-    return Levels
+        stats.update(run.stats)
+    return run.Levels
 
 
 if __name__ == "__main__":

@@ -242,6 +242,13 @@ int gm_engine_level0_substrate(gm_engine* e, int64_t n);
  * cF:3635-3665).  Synchronous.  minmax_host: 6 floats (L1 min,max, L2, L3). */
 int gm_engine_minmax(gm_engine* e, float* minmax_host);
 
+/* Measurement helper: average duration (ms) of `iters` back-to-back launches
+ * of one kernel of the L1 step on the engine's current state, recorded as a
+ * hipGraph and timed with HIP events on the engine stream.  which = 3: L3
+ * level step, 2: L2 level step, 1: L1 level step, 4: L3->L2 restriction.
+ * Overwrites scratch buffers only.  Synchronous. */
+int gm_engine_time_kernel(gm_engine* e, int which, int iters, float* ms_per_launch);
+
 /* Enable/disable hipGraph capture of gm_engine_subcycle (default enabled). */
 int gm_engine_use_graph(gm_engine* e, int enable);
 
