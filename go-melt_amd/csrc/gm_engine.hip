@@ -205,10 +205,22 @@ extern "C" int gm_solve_matrix_free(const gm_grid* grid, int64_t ne_active, cons
     a.Fext = F;
     a.corr_full = Corr;
     ls_coeffs(a, *grid);
+    // the given-k path reads only T_amb of the material (zero here: every
+    // node is active)
     gm_material dummy;
     memset(&dummy, 0, sizeof(dummy));
-    launch_level_step(a, nullptr, *grid, make_devmat(dummy), (hipStream_t)stream);
-    HIPCHK(hipGetLastError());
+    const DevMat hm = make_devmat(dummy);
+    DevMat* dm = nullptr;
+    HIPCHK(hipMalloc((void**)&dm, sizeof(DevMat)));
+    hipError_t rc = hipMemcpyAsync(dm, &hm, sizeof(DevMat), hipMemcpyHostToDevice,
+                                   (hipStream_t)stream);
+    if (rc == hipSuccess) {
+        launch_level_step(a, nullptr, *grid, dm, (hipStream_t)stream);
+        rc = hipGetLastError();
+    }
+    if (rc == hipSuccess) rc = hipStreamSynchronize((hipStream_t)stream);
+    (void)hipFree(dm);
+    HIPCHK(rc);
     return 0;
 }
 
@@ -374,7 +386,8 @@ extern "C" int gm_restrict(const gm_grid* coarse, const gm_grid* fine, const gm_
     level_coeffs(*fine, kc, &mrow, &w2, &w3);
     r.wf = w3;
     launch_restrict(pm.p, td.d, make_devmat(*mat), r, part, s);
-    launch_restrict_nodes(pm.p, part, nullptr, nullptr, nullptr, V, s);
+    launch_restrict_nodes(pm.p, part, nullptr, nullptr, nullptr, V, nullptr, nullptr, 0, 0, 0.0f,
+                          s);
     const int64_t n = grid_nn(*coarse);
     hipLaunchKernelGGL(k_box_to_full, dim3((n + 255) / 256), dim3(256), 0, s, pm.p, V, *coarse,
                        V_out);
@@ -414,6 +427,7 @@ struct gm_engine {
     gm_config cfg;
     gm_material mat;
     DevMat dm;
+    DevMat* d_dm = nullptr;  // device copy read by the level steps
     hipStream_t s;
     int64_t nn[4];
     int N2, N3, N23;
@@ -443,6 +457,7 @@ struct gm_engine {
     uint8_t* S23b[2] = {nullptr, nullptr};
     float* Tp3x = nullptr;
     uint8_t* preS2 = nullptr;
+    float *fa2 = nullptr, *fb2 = nullptr, *fa3 = nullptr, *fb3 = nullptr;
     float* mm = nullptr;
     PairMem pair[3];
     double* part[3] = {nullptr, nullptr, nullptr};
@@ -516,7 +531,13 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
         return fail("subcycle counts exceed the engine limits (N2 <= 64, N2*N3 <= 1024)");
     }
     for (int l = 0; l < 4; ++l) e->nn[l] = grid_nn(cfg->level[l]);
-    for (int l = 1; l < 4; ++l) level_coeffs(cfg->level[l], e->kc[l], &e->mrow[l], &e->w2[l], &e->w3[l]);
+    // the level-step kernel takes k in W/(m K) and cp in J/(kg K): the 1/1000
+    // of cF:2602 and rho of cF:2612 go into the stencil and mass constants
+    for (int l = 1; l < 4; ++l) {
+        level_coeffs(cfg->level[l], e->kc[l], &e->mrow[l], &e->w2[l], &e->w3[l]);
+        for (int p = 0; p < 8; ++p) e->kc[l][p] = (float)((double)e->kc[l][p] / 1000.0);
+        e->mrow[l] = (float)((double)e->mrow[l] * mat->rho);
+    }
     const int64_t n1 = e->nn[1], n2 = e->nn[2], n3 = e->nn[3], n0 = e->nn[0];
     e->nst = std::max(1, (int)lround((double)cfg->level[1].h[2] / cfg->layer_height));
     auto bad = [&](int rc) {
@@ -534,8 +555,13 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
         eng_alloc(e, (void**)&e->T1p, n1 * 4) || eng_alloc(e, (void**)&e->T1c, n1 * 4) ||
         eng_alloc(e, (void**)&e->Tp2x, n2 * 4) || eng_alloc(e, (void**)&e->Tp3x, n3 * 4) ||
         eng_alloc(e, (void**)&e->preS2, n3) || eng_alloc(e, (void**)&e->mm, 64) ||
-        eng_alloc(e, (void**)&e->d_desc, sizeof(GmStepDesc)))
+        eng_alloc(e, (void**)&e->fa2, n2 * 4) || eng_alloc(e, (void**)&e->fb2, n2 * 4) ||
+        eng_alloc(e, (void**)&e->fa3, n3 * 4) || eng_alloc(e, (void**)&e->fb3, n3 * 4) ||
+        eng_alloc(e, (void**)&e->d_desc, sizeof(GmStepDesc)) ||
+        eng_alloc(e, (void**)&e->d_dm, sizeof(DevMat)))
         return bad(-1);
+    if (hipMemcpy(e->d_dm, &e->dm, sizeof(DevMat), hipMemcpyHostToDevice) != hipSuccess)
+        return bad(fail("copy of the material constants failed"));
     for (int i = 0; i < 2; ++i) {
         if (eng_alloc(e, (void**)&e->T2b[i], n2 * 4) || eng_alloc(e, (void**)&e->S12b[i], n2) ||
             eng_alloc(e, (void**)&e->T3b[i], n3 * 4) || eng_alloc(e, (void**)&e->S13b[i], n3) ||
@@ -739,8 +765,8 @@ extern "C" int gm_engine_move(gm_engine* e, const float* v, const float* vstart,
 // ---- level-step helpers -----------------------------------------------------
 
 // L1 solve (computeL1Temperature cF:2813-2854 + clamp)
-static void run_l1(gm_engine* e, const float* F_G, int npos, const float* corr, float* Tout,
-                   int dt_mode, float dt, int clamp) {
+static void run_l1(gm_engine* e, const float* corr, float* Tout, int dt_mode, float dt,
+                   int clamp) {
     LSArgs a = ls_default();
     a.level = 1;
     a.T = e->T1;
@@ -752,14 +778,6 @@ static void run_l1(gm_engine* e, const float* F_G, int npos, const float* corr, 
     a.nn_active = e->host.l1_nn_active;
     a.dt_mode = dt_mode;
     a.dt = dt;
-    if (F_G) {
-        a.src_mode = GM_SRC_TABLES;
-        a.G = F_G;
-        a.G_npos = npos;
-        a.G_nmax = e->gmax[1];
-        a.G_scale = e->w3[3] / (float)npos;
-        for (int d = 0; d < 3; ++d) a.G_info[d] = e->pair[1].p.ax[d].info;
-    }
     a.conv = 1;
     if (corr) {
         a.corr = corr;
@@ -775,13 +793,13 @@ static void run_l1(gm_engine* e, const float* F_G, int npos, const float* corr, 
     a.mrow = e->mrow[1];
     a.w2 = e->w2[1];
     a.w3 = e->w3[1];
-    launch_level_step(a, e->d_desc, e->cfg.level[1], e->dm, e->s);
+    launch_level_step(a, e->d_desc, e->cfg.level[1], e->d_dm, e->s);
 }
 
 // L2 solve (computeL2Temperature cF:2917-2957 + clamp)
 static void run_l2(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
-                   const float* G, int npos, const float* corr, const float* bcA, const float* bcB,
-                   float alpha, float* Tout, int dt_mode, int dt_index, float dt, int clamp) {
+                   const float* corr, float alpha, float* Tout, int dt_mode, int dt_index,
+                   float dt, int clamp) {
     LSArgs a = ls_default();
     a.level = 2;
     a.T = Tin;
@@ -795,12 +813,6 @@ static void run_l2(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t*
     a.dt_mode = dt_mode;
     a.dt_index = dt_index;
     a.dt = dt;
-    a.src_mode = GM_SRC_TABLES;
-    a.G = G;
-    a.G_npos = npos;
-    a.G_nmax = e->gmax[2];
-    a.G_scale = e->w3[3] / (float)npos;
-    for (int d = 0; d < 3; ++d) a.G_info[d] = e->pair[2].p.ax[d].info;
     a.conv = 1;
     a.corr = corr;
     for (int d = 0; d < 3; ++d) {
@@ -809,21 +821,20 @@ static void run_l2(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t*
     }
     a.bc_mode = GM_BC_FINE;
     a.bc_level = 1;
-    a.bcA = bcA;
-    a.bcB = bcB;
+    a.bcA = e->fa2;
+    a.bcB = e->fb2;
     a.alpha = alpha;
     a.clamp = clamp;
     memcpy(a.kc, e->kc[2], sizeof(a.kc));
     a.mrow = e->mrow[2];
     a.w2 = e->w2[2];
     a.w3 = e->w3[2];
-    launch_level_step(a, e->d_desc, e->cfg.level[2], e->dm, e->s);
+    launch_level_step(a, e->d_desc, e->cfg.level[2], e->d_dm, e->s);
 }
 
 // L3 solve (computeSolutions_L3 cF:3015-3054 + clamp [+ melt time])
 static void run_l3(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out, int row,
-                   const float* bcA, const float* bcB, float alpha, float* Tout, int clamp,
-                   const uint8_t* S2prev, uint8_t* S2out) {
+                   float alpha, float* Tout, int clamp, const uint8_t* S2prev, uint8_t* S2out) {
     LSArgs a = ls_default();
     a.level = 3;
     a.T = Tin;
@@ -841,8 +852,8 @@ static void run_l3(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t*
     a.conv = 1;
     a.bc_mode = GM_BC_FINE;
     a.bc_level = 2;
-    a.bcA = bcA;
-    a.bcB = bcB;
+    a.bcA = e->fa3;
+    a.bcB = e->fb3;
     a.alpha = alpha;
     a.clamp = clamp;
     if (S2out) {
@@ -856,7 +867,7 @@ static void run_l3(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t*
     a.mrow = e->mrow[3];
     a.w2 = e->w2[3];
     a.w3 = e->w3[3];
-    launch_level_step(a, e->d_desc, e->cfg.level[3], e->dm, e->s);
+    launch_level_step(a, e->d_desc, e->cfg.level[3], e->d_dm, e->s);
 }
 
 static RArgs rargs(gm_engine* e, int fine, const float* T, const uint8_t* S1, const float* Tp,
@@ -914,9 +925,11 @@ static void subcycle_body(gm_engine* e) {
     launch_restrict(e->pair[0].p, d, e->dm,
                     rargs(e, 2, e->T2, e->S12, e->Tp2, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
                     e->part[0], s);
-    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], nullptr, e->V1, s);
-    // L1 predictor
-    run_l1(e, e->G1, N23, e->V1, e->T1p, GM_DT_L1, 0.0f, 1);
+    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], nullptr, e->V1,
+                          &e->pair[1].p, e->G1, N23, e->gmax[1], e->w3[3] / (float)N23, s);
+    // L1 predictor; its L2 face values for the Part-1 L2 substeps
+    run_l1(e, e->V1, e->T1p, GM_DT_L1, 0.0f, 1);
+    launch_face_bc(d, 2, 1, e->cfg.level[2], e->T1p, e->T1, e->fa2, e->fb2, s);
 
     // ---- Part 1: L2 x N2 { L3 x N3 } -----------------------------------------
     const float* T2cur = e->T2;
@@ -930,20 +943,21 @@ static void subcycle_body(gm_engine* e) {
         launch_restrict(e->pair[2].p, d, e->dm,
                         rargs(e, 3, T3cur, S13cur, Tp3cur, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
                         e->part[2], s);
-        launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2, s);
+        launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2,
+                              &e->pair[2].p, Gk, N3, e->gmax[2], e->w3[3] / (float)N3, s);
         float* T2new = e->T2b[b2];
         uint8_t* S12new = e->S12b[b2];
         b2 ^= 1;
-        run_l2(e, T2cur, S12cur, S12new, Gk, N3, e->V2, e->T1p, e->T1, (float)(k + 1) / fN2,
-               T2new, GM_DT_L2, k, 0.0f, 1);
+        run_l2(e, T2cur, S12cur, S12new, e->V2, (float)(k + 1) / fN2, T2new, GM_DT_L2, k, 0.0f, 1);
+        launch_face_bc(d, 3, 2, e->cfg.level[3], T2new, T2cur, e->fa3, e->fb3, s);
         const float* T3in = T3cur;
         const uint8_t* S13in = S13cur;
         for (int j = 0; j < N3; ++j) {
             float* T3o = e->T3b[b3];
             uint8_t* S13o = e->S13b[b3];
             b3 ^= 1;
-            run_l3(e, T3in, S13in, S13o, k * N3 + j, T2new, T2cur, (float)(j + 1) / fN3, T3o, 1,
-                   nullptr, nullptr);
+            run_l3(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, 1, nullptr,
+                   nullptr);
             T3in = T3o;
             S13in = S13o;
         }
@@ -962,8 +976,10 @@ static void subcycle_body(gm_engine* e) {
     launch_restrict(e->pair[0].p, d, e->dm,
                     rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_L1, 0, 0.0f),
                     e->part[0], s);
-    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], e->V1, e->V1m, s);
-    run_l1(e, e->G1, N23, e->V1m, e->T1c, GM_DT_L1, 0.0f, 1);
+    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], e->V1, e->V1m,
+                          nullptr, nullptr, 0, 0, 0.0f, s);
+    run_l1(e, e->V1m, e->T1c, GM_DT_L1, 0.0f, 1);
+    launch_face_bc(d, 2, 1, e->cfg.level[2], e->T1c, e->T1, e->fa2, e->fb2, s);
 
     // ---- Part 2: restart L2/L3 from the start-of-step state ------------------
     T2cur = e->T2;
@@ -979,12 +995,13 @@ static void subcycle_body(gm_engine* e) {
         launch_restrict(e->pair[2].p, d, e->dm,
                         rargs(e, 3, T3cur, S13cur, Tp3cur, e->histTp[k], Tp3cur, GM_DT_L2, k, 0.0f),
                         e->part[2], s);
-        launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2, s);
+        launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2,
+                              &e->pair[2].p, Gk, N3, e->gmax[2], e->w3[3] / (float)N3, s);
         float* T2new = e->L2all[k];
         uint8_t* S12new = e->S12b[b2s];
         b2s ^= 1;
-        run_l2(e, T2cur, S12cur, S12new, Gk, N3, e->V2, e->T1c, e->T1, (float)(k + 1) / fN2,
-               T2new, GM_DT_L2, k, 0.0f, 1);
+        run_l2(e, T2cur, S12cur, S12new, e->V2, (float)(k + 1) / fN2, T2new, GM_DT_L2, k, 0.0f, 1);
+        launch_face_bc(d, 3, 2, e->cfg.level[3], T2new, T2cur, e->fa3, e->fb3, s);
         const float* T3in = T3cur;
         const uint8_t* S13in = S13cur;
         const uint8_t* S23in = S23cur;
@@ -994,8 +1011,7 @@ static void subcycle_body(gm_engine* e) {
             uint8_t* S13o = e->S13b[bs2];
             uint8_t* S23o = e->S23b[bs2];
             bs2 ^= 1;
-            run_l3(e, T3in, S13in, S13o, k * N3 + j, T2new, T2cur, (float)(j + 1) / fN3, T3o, 1,
-                   S23in, S23o);
+            run_l3(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, 1, S23in, S23o);
             T3in = T3o;
             S13in = S13o;
             S23in = S23o;
@@ -1066,9 +1082,12 @@ static void solutions(gm_engine* e, const float* V1, const float* V2, float* T1o
                       float* T3o) {
     // computeSolutions (cF:2135-2204): L1, then L2 from the new L1, then L3
     // from the new L2; no clamp inside (the caller clamps afterwards).
-    run_l1(e, e->G1, 1, V1, T1o, GM_DT_ROW, 0.0f, 0);
-    run_l2(e, e->T2, e->S12, nullptr, e->G2, 1, V2, T1o, T1o, 1.0f, T2o, GM_DT_ROW, 0, 0.0f, 0);
-    run_l3(e, e->T3, e->S13, nullptr, 0, T2o, T2o, 1.0f, T3o, 0, nullptr, nullptr);
+    GmStepDesc* d = e->d_desc;
+    run_l1(e, V1, T1o, GM_DT_ROW, 0.0f, 0);
+    launch_face_bc(d, 2, 1, e->cfg.level[2], T1o, T1o, e->fa2, e->fb2, e->s);
+    run_l2(e, e->T2, e->S12, nullptr, V2, 1.0f, T2o, GM_DT_ROW, 0, 0.0f, 0);
+    launch_face_bc(d, 3, 2, e->cfg.level[3], T2o, T2o, e->fa3, e->fb3, e->s);
+    run_l3(e, e->T3, e->S13, nullptr, 0, 1.0f, T3o, 0, nullptr, nullptr);
 }
 
 extern "C" int gm_engine_step(gm_engine* e, const float* row) {
@@ -1098,11 +1117,13 @@ extern "C" int gm_engine_step(gm_engine* e, const float* row) {
     launch_restrict(e->pair[0].p, d, e->dm,
                     rargs(e, 2, e->T2, e->S12, e->Tp2, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
                     e->part[0], s);
-    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], nullptr, e->V1, s);
+    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], nullptr, e->V1,
+                          &e->pair[1].p, e->G1, 1, e->gmax[1], e->w3[3], s);
     launch_restrict(e->pair[2].p, d, e->dm,
                     rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
                     e->part[2], s);
-    launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2, s);
+    launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2,
+                          &e->pair[2].p, e->G2, 1, e->gmax[2], e->w3[3], s);
     // predictor
     float* T1o = e->T1p;
     float* T2o = e->T2b[0];
@@ -1120,12 +1141,14 @@ extern "C" int gm_engine_step(gm_engine* e, const float* row) {
     launch_restrict(e->pair[0].p, d, e->dm,
                     rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_ROW, 0, 0.0f),
                     e->part[0], s);
-    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], e->V1, e->V1m, s);
+    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], e->V1, e->V1m,
+                          nullptr, nullptr, 0, 0, 0.0f, s);
     launch_restrict(e->pair[2].p, d, e->dm,
                     rargs(e, 3, e->T3, e->S13, nullptr, e->Tp3x, e->Tp3, GM_DT_ROW, 0, 0.0f),
                     e->part[2], s);
     float* V2m = e->Tp2x;  // reuse: the L2 T' of the predictor is no longer needed
-    launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, e->V2, V2m, s);
+    launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, e->V2, V2m, nullptr, nullptr,
+                          0, 0, 0.0f, s);
     // corrector
     T1o = e->T1c;
     T2o = e->T2b[1];
@@ -1167,7 +1190,7 @@ void gm_props_u8(const DevMat& m, const float* T, uint8_t* S1, uint8_t* S2, int6
 // ---- stepGOMELTDwellTime (cF:2617-2664) -------------------------------------
 
 extern "C" int gm_engine_step_dwell(gm_engine* e, float dt) {
-    run_l1(e, nullptr, 1, nullptr, e->T1p, GM_DT_VALUE, dt, 0);
+    run_l1(e, nullptr, e->T1p, GM_DT_VALUE, dt, 0);
     launch_copy_f32(e->T1, e->T1p, e->nn[1], e->s);
     HIPCHK(hipGetLastError());
     return 0;
@@ -1250,20 +1273,20 @@ extern "C" int gm_engine_time_kernel(gm_engine* e, int which, int iters, float* 
     for (int i = 0; i < iters; ++i) {
         const int b = i & 1;
         if (which == 3)
-            run_l3(e, b ? e->T3b[0] : e->T3, e->S13, e->S13b[b], 0, e->T2, e->T2, 1.0f,
-                   e->T3b[b ? 1 : 0], 1, nullptr, nullptr);
+            run_l3(e, b ? e->T3b[0] : e->T3, e->S13, e->S13b[b], 0, 1.0f, e->T3b[b ? 1 : 0], 1,
+                   nullptr, nullptr);
         else if (which == 2)
-            run_l2(e, b ? e->T2b[0] : e->T2, e->S12, e->S12b[b], e->G2, e->N3, e->V2, e->T1,
-                   e->T1, 1.0f, e->T2b[b ? 1 : 0], GM_DT_L2, 0, 0.0f, 1);
+            run_l2(e, b ? e->T2b[0] : e->T2, e->S12, e->S12b[b], e->V2, 1.0f, e->T2b[b ? 1 : 0],
+                   GM_DT_L2, 0, 0.0f, 1);
         else if (which == 1)
-            run_l1(e, e->G1, e->N23, e->V1, b ? e->T1c : e->T1p, GM_DT_L1, 0.0f, 1);
+            run_l1(e, e->V1, b ? e->T1c : e->T1p, GM_DT_L1, 0.0f, 1);
         else if (which == 4) {
             launch_restrict(e->pair[2].p, e->d_desc, e->dm,
                             rargs(e, 3, e->T3, e->S13, e->Tp3, e->histTp[0], e->Tp3, GM_DT_L2, 0,
                                   0.0f),
                             e->part[2], e->s);
             launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2,
-                                  e->s);
+                                  nullptr, nullptr, 0, 0, 0.0f, e->s);
         }
     }
     HIPCHK(hipStreamEndCapture(e->s, &g));

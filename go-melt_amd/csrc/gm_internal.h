@@ -39,6 +39,7 @@ struct GmStepDesc {
 };
 
 // Device copy of the material with derived constants (kernel argument).
+#define GM_LUT_BINS 64
 struct DevMat {
     float T_amb, T_sol, T_liq, T_boil;
     float k_p, k_a0, k_a1, k_f;
@@ -48,6 +49,11 @@ struct DevMat {
     float cT[GM_MAX_KNOTS], cv[GM_MAX_KNOTS], cs[GM_MAX_KNOTS];
     float h_conv, sigma_eps, evc_cp, CT, CM, Lev, invTb, Tclip, Tamb4;
     float eta, rsq, dsq, rc, dc, pc6;  // pc6 = 6*sqrt(3)*eta
+    // binned lookup of the table segment: bin b = (T - lut_T0) * lut_iw,
+    // segment = lut[b], then one step up if T passed the next knot
+    float lut_T0[2], lut_iw[2];
+    int32_t lut_nb[2];
+    int32_t lut[2][GM_LUT_BINS];
 };
 
 // One axis of the tabulated restriction between a fine and a coarse mesh
@@ -113,7 +119,7 @@ struct LSArgs {
     int bc_mode;
     float bc_vals[5];     // y0, y1, x0, x1, z0
     int bc_level;         // coarse level of the prolonged field
-    const float* bcA;
+    const float* bcA;     // face values of the prolonged coarse fields (full size)
     const float* bcB;
     float alpha;
     int clamp;
@@ -150,7 +156,7 @@ DevMat make_devmat(const gm_material& m);
 void level_coeffs(const gm_grid& g, float kc[8], float* mrow, float* w2, float* w3);
 
 void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& host_grid,
-                       const DevMat& m, hipStream_t s);
+                       const DevMat* m_dev, hipStream_t s);
 void launch_axis_tables(const GmPair& p, const GmStepDesc* d, hipStream_t s);
 void launch_source_tables(const GmPair& p, const GmStepDesc* d, const DevMat& m,
                           int row0, int npos, int use_rows, const float* rows_dev,
@@ -159,7 +165,10 @@ void launch_restrict(const GmPair& p, const GmStepDesc* d, const DevMat& m, cons
                      double* partial, hipStream_t s);
 void launch_restrict_nodes(const GmPair& outer, const double* part_outer, const GmPair* inner,
                            const double* part_inner, const float* Vbase, float* V,
+                           const GmPair* gpair, const float* G, int npos, int nmax, float scale,
                            hipStream_t s);
+void launch_face_bc(const GmStepDesc* d, int fine, int coarse, const gm_grid& gf, const float* A,
+                    const float* B, float* fa, float* fb, hipStream_t s);
 void launch_interp_grid(const GmStepDesc* d, int src_level, const gm_grid* src_host,
                         const float* u, const uint8_t* u8, int tgt_mode, const gm_grid* tgt_host,
                         int64_t n_target, float* out, hipStream_t s);

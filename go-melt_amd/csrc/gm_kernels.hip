@@ -7,25 +7,16 @@
 // matrix-free element operator solveMatrixFreeFE (cF:582-642), the laser
 // source (cF:2960-3012 / 2667-2730), the top-face flux (cF:2207-2301), the
 // Dirichlet overwrite (cF:1568-1620), the inactive-node reset (cF:1848-1865)
-// and the T_amb clamp.  It is node-centric: each tile of 32x8x4 nodes stages
-// T, k and rho*cp of its nodes plus a one-node halo in LDS, forms the element
-// means once per element, and applies the 27-point stencil that the 8x8
-// element matrices of the box element reduce to.  No atomics, deterministic.
+// and the T_amb clamp.  It is node-centric and marches along z: a block owns
+// 32x8 node columns and 4 output planes, keeps three T planes and two element
+// planes (k, rho*cp means) in LDS, evaluates the node properties once per
+// loaded plane node, and applies the 27-point stencil the box element's 8x8
+// matrices reduce to, summed by neighbour class.  No atomics, deterministic.
 #include <math.h>
 
 #include "gm_internal.h"
 
 namespace gm {
-
-#define TX 32
-#define TY 8
-#define TZ 4
-#define HX (TX + 2)
-#define HY (TY + 2)
-#define HZ (TZ + 2)
-#define EX (TX + 1)
-#define EY (TY + 1)
-#define EZ (TZ + 1)
 
 // 1-D Gauss factors at xi = -/+ 1/sqrt(3): N1 same side / other side.
 #define GA 0.78867513459481287f
@@ -164,261 +155,447 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
-template <bool S1F>
+// Tiles of LX x LY node columns; ZCT output planes per block (template).
+#define LX 32
+#define LY 8
+#define PX (LX + 2)
+#define PY (LY + 2)
+#define PN (PX * PY)
+#define QX (LX + 1)
+#define QY (LY + 1)
+#define QN (QX * QY)
+#define LUTW (GM_LUT_BINS + 3 * GM_MAX_KNOTS)  // LDS words per property table
+
+// Piecewise-linear property table through its binned segment lookup (LDS).
+__device__ __forceinline__ float tab_lds(const float* __restrict__ t, float T0, float iw, int nb,
+                                         int n, float T) {
+    int b = (int)((T - T0) * iw);
+    b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+    int j = __float_as_int(t[b]);
+    const float* sg = t + GM_LUT_BINS;
+    while (j + 1 < n - 1 && T >= sg[3 * (j + 1) + 1]) ++j;
+    return sg[3 * j] + sg[3 * j + 2] * (T - sg[3 * j + 1]);
+}
+
+// computeStateProperties (cF:2584-2612) for one node, without the unit
+// factors: k in W/(m K), cp in J/(kg K) (the 1/1000 of cF:2602 and rho of
+// cF:2612 are folded into the host-side stencil and mass constants).  With
+// S1, S2, S3 in {0,1} the reference's sums of products are exact selects.
+struct PropC {
+    float Tliq, Tsol, kp, kf, ka0, ka1, cpf, cpm, ca0, ca1;
+    int kn, cn;
+    float kT0, kiw, cT0, ciw;
+    int knb, cnb;
+};
+
+__device__ __forceinline__ void props_c(const PropC& c, const float* __restrict__ tab, float T,
+                                        bool s1in, bool substrate, float& kk, float& cp, bool& s1) {
+    const bool s2 = T >= c.Tliq;
+    const bool s3 = (T > c.Tsol) && (T < c.Tliq);
+    s1 = s1in || s2 || substrate;
+    const float kb = c.kn ? tab_lds(tab, c.kT0, c.kiw, c.knb, c.kn, T) : (c.ka1 * T + c.ka0);
+    const float cs = c.cn ? tab_lds(tab + LUTW, c.cT0, c.ciw, c.cnb, c.cn, T) : (c.ca1 * T + c.ca0);
+    kk = s2 ? c.kf : (s1 ? kb : c.kp);
+    cp = s2 ? c.cpf : (s3 ? c.cpm : cs);
+}
+
+// Modes fixed at compile time: S1 storage, laser source, Dirichlet kind,
+// melt-time accumulation, given k / rho*cp arrays, planes per block.
+template <bool S1F, bool SRC, int BC, bool ACC, bool KIN, int ZCT>
 __global__ void __launch_bounds__(256) k_level_step(LSArgs a, const GmStepDesc* __restrict__ d,
-                                                    DevMat m) {
-    __shared__ float sT[HX * HY * HZ];
-    __shared__ float sK[HX * HY * HZ];
-    __shared__ float sC[HX * HY * HZ];
-    __shared__ float sKe[EX * EY * EZ];
-    __shared__ float sCe[EX * EY * EZ];
-    __shared__ float sS[TX + TY + TZ];
+                                                    const DevMat* __restrict__ m) {
+    constexpr int NP = ZCT + 2;              // planes loaded per block
+    __shared__ float sT[3][PN];              // T of node planes (ring)
+    __shared__ float sKn[PN], sCn[PN];       // k, cp of the plane being loaded
+    __shared__ float sSk[2][QN], sSc[2][QN];  // x-y quad sums of k, cp (ring)
+    __shared__ float sKe[2][QN], sCe[2][QN];  // element planes below / above (ring)
+    __shared__ float sSrc[LX + LY + ZCT];
+    __shared__ float sTab[2 * LUTW];
 
     const gm_grid g = a.use_desc ? d->g[a.level] : a.grid;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
+    const int nxy = nx * ny;
+    const int ntx = (nx + LX - 1) / LX, nty = (ny + LY - 1) / LY;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int ox = (t % ntx) * TX, oy = ((t / ntx) % nty) * TY, oz = (t / (ntx * nty)) * TZ;
+    const int ox = (t % ntx) * LX, oy = ((t / ntx) % nty) * LY, oz = (t / (ntx * nty)) * ZCT;
     const int nex = nx - 1, ney = ny - 1;
     const int nez_act = (int)(a.ne_active / ((int64_t)nex * ney));
-    const int top = nez_act;  // node layer carrying the surface flux
+    const int tid = threadIdx.x;
+    const int tx = tid % LX, ty = tid / LX;
+    const int ix = ox + tx, iy = oy + ty;
+    const bool col_ok = ix < nx && iy < ny;
+
+    PropC pc;
+    if (!KIN) {
+        pc.Tliq = m->T_liq; pc.Tsol = m->T_sol; pc.kp = m->k_p; pc.kf = m->k_f;
+        pc.ka0 = m->k_a0; pc.ka1 = m->k_a1; pc.cpf = m->cp_f; pc.cpm = m->cp_m;
+        pc.ca0 = m->cp_a0; pc.ca1 = m->cp_a1; pc.kn = m->k_n; pc.cn = m->cp_n;
+        pc.kT0 = m->lut_T0[0]; pc.kiw = m->lut_iw[0]; pc.cT0 = m->lut_T0[1]; pc.ciw = m->lut_iw[1];
+        pc.knb = m->lut_nb[0]; pc.cnb = m->lut_nb[1];
+    }
+    const float Tamb = m->T_amb;
+
+    // ---- prologue: every global load of the block is issued here -----------
+    // plane entries of this thread: l = tid and tid + 256
+    int eoff[2];
+    bool eok[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int l = tid + 256 * e;
+        const int jx = ox + l % PX - 1, jy = oy + l / PX - 1;
+        eok[e] = l < PN && jx >= 0 && jx < nx && jy >= 0 && jy < ny;
+        eoff[e] = jx + nx * jy;
+    }
+    float rT[NP][2], rS[NP][2], rK[NP][2];
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+        const int p = oz - 1 + pl;
+        const bool pin = p >= 0 && p < nz;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            rT[pl][e] = 0.0f;
+            rS[pl][e] = 0.0f;
+            rK[pl][e] = 0.0f;
+            if (pin && eok[e]) {
+                const int idx = eoff[e] + nxy * p;
+                rT[pl][e] = a.T[idx];
+                if (KIN) {
+                    rK[pl][e] = a.kin[idx];
+                    rS[pl][e] = a.cin[idx];
+                } else {
+                    rS[pl][e] = S1F ? ((const float*)a.S1)[idx]
+                                    : (float)((const uint8_t*)a.S1)[idx];
+                }
+            }
+        }
+    }
+    // per output node: correction (+ extra source), Dirichlet face values,
+    // melt-time state
+    const int cidx = ix + nx * iy;
+    const bool face_col = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
+    float rC[ZCT], rFa[ZCT], rFb[ZCT], rAcc[ZCT], rMax[ZCT];
+    bool rS2[ZCT];
+    int clo0 = 0, clo1 = 0, clo2 = 0, cn0 = 0, cn1 = 0, cn2 = 0;
+    if (a.corr) {
+        clo0 = a.corr_info[0][0]; cn0 = a.corr_info[0][1] + 1;
+        clo1 = a.corr_info[1][0]; cn1 = a.corr_info[1][1] + 1;
+        clo2 = a.corr_info[2][0]; cn2 = a.corr_info[2][1] + 1;
+    }
+    int l0base = 0;
+    bool l0xy = false;
+    if (ACC) {
+        const int lx0 = d->l0_lo3[0] + ix, ly0 = d->l0_lo3[1] + iy;
+        l0xy = lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1];
+        l0base = lx0 + d->l0_n[0] * ly0;
+    }
+#pragma unroll
+    for (int z = 0; z < ZCT; ++z) {
+        const int zo = oz + z;
+        const bool ok = col_ok && zo < nz;
+        const int idx = cidx + nxy * zo;
+        float c = 0.0f;
+        if (ok && a.corr) {
+            const int bx = ix - clo0, by = iy - clo1, bz = zo - clo2;
+            if (bx >= 0 && bx < cn0 && by >= 0 && by < cn1 && bz >= 0 && bz < cn2)
+                c = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
+        }
+        if (ok && a.corr_full) c += a.corr_full[idx];
+        if (ok && a.Fext) c += a.Fext[idx];
+        rC[z] = c;
+        rFa[z] = rFb[z] = 0.0f;
+        if (BC == GM_BC_FINE && ok && (face_col || zo == 0)) {
+            rFa[z] = a.bcA[idx];
+            rFb[z] = a.bcB[idx];
+        }
+        rAcc[z] = rMax[z] = 0.0f;
+        rS2[z] = false;
+        if (ACC && ok) {
+            rS2[z] = a.S2prev[idx] != 0;
+            const int lz0 = d->l0_lo3[2] + zo;
+            if (l0xy && lz0 >= 0 && lz0 < d->l0_n[2]) {
+                const int64_t l0 = (int64_t)l0base + (int64_t)d->l0_n[0] * d->l0_n[1] * lz0;
+                rAcc[z] = a.accum[l0];
+                rMax[z] = a.max_accum[l0];
+            }
+        }
+    }
 
     float dt = a.dt;
     if (a.dt_mode == GM_DT_ROW) dt = d->rows[a.dt_index][5];
     else if (a.dt_mode == GM_DT_L1) dt = d->dt1;
     else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
-
-    // --- phase A: stage T, k, rho*cp of the tile and its halo ---------------
-    for (int l = threadIdx.x; l < HX * HY * HZ; l += blockDim.x) {
-        const int hx = l % HX, hy = (l / HX) % HY, hz = l / (HX * HY);
-        const int ix = ox + hx - 1, iy = oy + hy - 1, iz = oz + hz - 1;
-        float tv = 0.0f, kv = 0.0f, cv = 0.0f;
-        if (ix >= 0 && ix < nx && iy >= 0 && iy < ny && iz >= 0 && iz < nz) {
-            const int64_t idx = nid(ix, iy, iz, nx, ny);
-            tv = a.T[idx];
-            if (a.kin) {
-                kv = a.kin[idx];
-                cv = a.cin[idx];
-            } else {
-                const float s1in = S1F ? ((const float*)a.S1)[idx]
-                                       : (float)((const uint8_t*)a.S1)[idx];
-                bool s1, s2;
-                props(m, tv, s1in, idx < a.n_sub, kv, cv, s1, s2);
-                if (a.S1out && hx >= 1 && hx <= TX && hy >= 1 && hy <= TY && hz >= 1 && hz <= TZ)
-                    a.S1out[idx] = (uint8_t)s1;
-            }
-        }
-        sT[l] = tv;
-        sK[l] = kv;
-        sC[l] = cv;
-    }
-    // separable source factors of the tile's node rows/columns/layers
-    if (a.src_mode == GM_SRC_L3) {
-        float v[3], P;
-        if (a.src_row >= 0) {
-            v[0] = d->rows[a.src_row][0];
-            v[1] = d->rows[a.src_row][1];
-            v[2] = d->rows[a.src_row][2];
-            P = d->rows[a.src_row][6];
-        } else {
-            v[0] = a.src_v[0];
-            v[1] = a.src_v[1];
-            v[2] = a.src_v[2];
-            P = a.src_P;
-        }
-        for (int l = threadIdx.x; l < TX + TY + TZ; l += blockDim.x) {
-            int dd, i;
-            if (l < TX) { dd = 0; i = ox + l; }
-            else if (l < TX + TY) { dd = 1; i = oy + l - TX; }
-            else { dd = 2; i = oz + l - TX - TY; }
-            float s = 0.0f;
-            if (i < g.n[dd]) {
-                const float sq = dd == 2 ? m.dsq : m.rsq;
-                const float coef = dd == 2 ? m.dc : m.rc;
-                s = src_axis(g, dd, i, v[dd], sq, coef);
-                if (dd == 0) s *= m.pc6 * P;
-            }
-            sS[l] = s;
-        }
-    }
-    __syncthreads();
-
-    // --- phase B: element means of k and rho*cp ------------------------------
-    for (int l = threadIdx.x; l < EX * EY * EZ; l += blockDim.x) {
-        const int lx = l % EX, ly = (l / EX) % EY, lz = l / (EX * EY);
-        const int ex = ox + lx - 1, ey = oy + ly - 1, ez = oz + lz - 1;
-        float ke = 0.0f, ce = 0.0f;
-        if (ex >= 0 && ex < nex && ey >= 0 && ey < ney && ez >= 0 && ez < nez_act) {
-            const int h0 = lx + HX * (ly + HY * lz);
-            const int o1 = 1, o2 = HX, o4 = HX * HY;
-            ke = (sK[h0] + sK[h0 + o1] + sK[h0 + o1 + o2] + sK[h0 + o2] + sK[h0 + o4] +
-                  sK[h0 + o1 + o4] + sK[h0 + o1 + o2 + o4] + sK[h0 + o2 + o4]) * 0.125f;
-            ce = (sC[h0] + sC[h0 + o1] + sC[h0 + o1 + o2] + sC[h0 + o2] + sC[h0 + o4] +
-                  sC[h0 + o1 + o4] + sC[h0 + o1 + o2 + o4] + sC[h0 + o2 + o4]) * 0.125f;
-        }
-        sKe[l] = ke;
-        sCe[l] = ce;
-    }
-    __syncthreads();
-
-    // --- phase C: one column of TZ nodes per thread -------------------------
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int ix = ox + tx, iy = oy + ty;
-    if (ix >= nx || iy >= ny) return;
     const float mfac = a.mrow / dt;
 
-    // boxed correction / table-source bounds
-    int clo[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
-    if (a.corr) {
-#pragma unroll
-        for (int dd = 0; dd < 3; ++dd) {
-            clo[dd] = a.corr_info[dd][0];
-            cn[dd] = a.corr_info[dd][1] + 1;
-        }
-    }
-    int glo[3] = {0, 0, 0}, gn[3] = {0, 0, 0};
-    if (a.src_mode == GM_SRC_TABLES) {
-#pragma unroll
-        for (int dd = 0; dd < 3; ++dd) {
-            glo[dd] = a.G_info[dd][0];
-            gn[dd] = a.G_info[dd][1] + 1;
-        }
-    }
-
-    for (int z = 0; z < TZ; ++z) {
-        const int iz = oz + z;
-        if (iz >= nz) break;
-        const int hx = tx + 1, hy = ty + 1, hz = z + 1;
-        const int64_t idx = nid(ix, iy, iz, nx, ny);
-
-        float Tn[27];
-#pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                for (int dx = 0; dx < 3; ++dx)
-                    Tn[dx + 3 * (dy + 3 * dz)] =
-                        sT[(hx - 1 + dx) + HX * ((hy - 1 + dy) + HY * (hz - 1 + dz))];
-
-        float KT = 0.0f, Msum = 0.0f;
-#pragma unroll
-        for (int o = 0; o < 8; ++o) {
-            const int oxx = o & 1, oyy = (o >> 1) & 1, ozz = o >> 2;
-            const int el = (hx - 1 + oxx) + EX * ((hy - 1 + oyy) + EY * (hz - 1 + ozz));
-            const float ke = sKe[el];
-            const float ce = sCe[el];
-            float acc = 0.0f;
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
-                const int dxx = oxx - 1 + bx, dyy = oyy - 1 + by, dzz = ozz - 1 + bz;
-                const int p = (dxx != 0) | ((dyy != 0) << 1) | ((dzz != 0) << 2);
-                acc += a.kc[p] * Tn[(dxx + 1) + 3 * ((dyy + 1) + 3 * (dzz + 1))];
+    // per-block constants: table lookups and the separable source
+    if (!KIN) {
+        for (int l = tid; l < 2 * LUTW; l += 256) {
+            const int w = l / LUTW, r = l % LUTW;
+            float v;
+            if (r < GM_LUT_BINS) {
+                v = __int_as_float(m->lut[w][r]);
+            } else {
+                const int j = (r - GM_LUT_BINS) / 3, f = (r - GM_LUT_BINS) % 3;
+                const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
+                                                                          : (w ? m->cs : m->ks);
+                v = src[j];
             }
-            KT += ke * acc;
-            Msum += ce;
+            sTab[l] = v;
         }
-        const float M = Msum * mfac;
-        const float Ti = Tn[13];
+    }
+    if (SRC) {
+        float v0, v1, v2, P;
+        if (a.src_row >= 0) {
+            v0 = d->rows[a.src_row][0];
+            v1 = d->rows[a.src_row][1];
+            v2 = d->rows[a.src_row][2];
+            P = d->rows[a.src_row][6];
+        } else {
+            v0 = a.src_v[0];
+            v1 = a.src_v[1];
+            v2 = a.src_v[2];
+            P = a.src_P;
+        }
+        if (tid < LX) {
+            const int i = ox + tid;
+            sSrc[tid] = i < nx ? src_axis(g, 0, i, v0, m->rsq, m->rc) * (m->pc6 * P) : 0.0f;
+        } else if (tid < LX + LY) {
+            const int i = oy + tid - LX;
+            sSrc[tid] = i < ny ? src_axis(g, 1, i, v1, m->rsq, m->rc) : 0.0f;
+        } else if (tid < LX + LY + ZCT) {
+            const int i = oz + tid - LX - LY;
+            sSrc[tid] = i < nz ? src_axis(g, 2, i, v2, m->dsq, m->dc) : 0.0f;
+        }
+    }
+    // quad-sum entries of this thread: q = tid and tid + 256
+    int qh[2], qo[2];
+    bool qxy[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int l = tid + 256 * e;
+        const int qx = l % QX, qy = l / QX;
+        qh[e] = qx + PX * qy;
+        const int ex = ox + qx - 1, ey = oy + qy - 1;
+        qxy[e] = l < QN && ex >= 0 && ex < nex && ey >= 0 && ey < ney;
+        qo[e] = l;
+    }
+    const int hc = (tx + 1) + PX * (ty + 1);
+    const int q0 = tx + QX * ty;
+    __syncthreads();
 
-        // source + surface flux
-        float F = 0.0f;
-        if (a.src_mode == GM_SRC_L3) {
-            F = sS[tx] * sS[TX + ty] * sS[TX + TY + z] * a.w3;
-        } else if (a.src_mode == GM_SRC_TABLES) {
-            const int bx = ix - glo[0], by = iy - glo[1], bz = iz - glo[2];
-            if (bx >= 0 && bx < gn[0] && by >= 0 && by < gn[1] && bz >= 0 && bz < gn[2]) {
-                float s = 0.0f;
-                for (int pq = 0; pq < a.G_npos; ++pq) {
-                    const float* Gp = a.G + (int64_t)pq * 3 * a.G_nmax;
-                    s += Gp[bx] * Gp[a.G_nmax + by] * Gp[2 * a.G_nmax + bz];
+    // ---- march through the planes ------------------------------------------
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+        const int p = oz - 1 + pl;
+        const bool pin = p >= 0 && p < nz;
+        if (pl > 0) __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int l = tid + 256 * e;
+            if (l < PN) {
+                float kv = 0.0f, cv = 0.0f;
+                if (pin && eok[e]) {
+                    if (KIN) {
+                        kv = rK[pl][e];
+                        cv = rS[pl][e];
+                    } else {
+                        const int idx = eoff[e] + nxy * p;
+                        bool s1;
+                        props_c(pc, sTab, rT[pl][e], rS[pl][e] > 0.499f, idx < a.n_sub, kv, cv,
+                                s1);
+                        if (a.S1out && p >= oz && l % PX >= 1 && l % PX <= LX && l / PX >= 1 &&
+                            l / PX <= LY)
+                            a.S1out[idx] = (uint8_t)s1;
+                    }
                 }
-                F = s * a.G_scale;
+                sT[pl % 3][l] = rT[pl][e];
+                sKn[l] = kv;
+                sCn[l] = cv;
             }
         }
-        if (a.Fext) F += a.Fext[idx];
-        if (a.conv && iz == top) {
-            float cr = 0.0f;
+        __syncthreads();
+        const bool ezok = p - 1 >= 0 && p - 1 < nez_act;
 #pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            if (qo[e] < QN) {
+                const int h0 = qh[e], l = qo[e];
+                const float Sk = (sKn[h0] + sKn[h0 + 1]) + (sKn[h0 + PX] + sKn[h0 + PX + 1]);
+                const float Sc = (sCn[h0] + sCn[h0 + 1]) + (sCn[h0 + PX] + sCn[h0 + PX + 1]);
+                const bool ok = qxy[e] && ezok;
+                sKe[pl & 1][l] = ok ? (sSk[(pl + 1) & 1][l] + Sk) * 0.125f : 0.0f;
+                sCe[pl & 1][l] = ok ? (sSc[(pl + 1) & 1][l] + Sc) * 0.125f : 0.0f;
+                sSk[pl & 1][l] = Sk;
+                sSc[pl & 1][l] = Sc;
+            }
+        }
+        __syncthreads();
+        if (pl < 2) continue;
+        const int z = pl - 2;
+        const int zo = oz + z;
+        if (zo >= nz || !col_ok) continue;
+        const int idx = cidx + nxy * zo;
+        const float* Tm = sT[(pl + 1) % 3];
+        const float* T0 = sT[(pl + 2) % 3];
+        const float* Tp = sT[pl % 3];
+        const float* kb = sKe[(pl + 1) & 1];  // elements (zo-1, zo)
+        const float* ka = sKe[pl & 1];        // elements (zo, zo+1)
+        const float* cb = sCe[(pl + 1) & 1];
+        const float* ca = sCe[pl & 1];
+        const float k000 = kb[q0], k100 = kb[q0 + 1], k010 = kb[q0 + QX], k110 = kb[q0 + QX + 1];
+        const float k001 = ka[q0], k101 = ka[q0 + 1], k011 = ka[q0 + QX], k111 = ka[q0 + QX + 1];
+        const float Msum = ((cb[q0] + cb[q0 + 1]) + (cb[q0 + QX] + cb[q0 + QX + 1])) +
+                           ((ca[q0] + ca[q0 + 1]) + (ca[q0 + QX] + ca[q0 + QX + 1]));
+        const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011, Pz11 = k110 + k111;
+        const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011, Py11 = k101 + k111;
+        const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101, Px11 = k011 + k111;
+        const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
+        const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
+        const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
+        const float Kc = Fx0 + Fx1;
+        const float Ti = T0[hc];
+        float KT = a.kc[0] * (Kc * Ti);
+        KT += a.kc[1] * (Fx0 * T0[hc - 1] + Fx1 * T0[hc + 1]);
+        KT += a.kc[2] * (Fy0 * T0[hc - PX] + Fy1 * T0[hc + PX]);
+        KT += a.kc[4] * (Fz0 * Tm[hc] + Fz1 * Tp[hc]);
+        KT += a.kc[3] * (Pz00 * T0[hc - 1 - PX] + Pz10 * T0[hc + 1 - PX] + Pz01 * T0[hc - 1 + PX] +
+                         Pz11 * T0[hc + 1 + PX]);
+        KT += a.kc[5] * (Py00 * Tm[hc - 1] + Py10 * Tm[hc + 1] + Py01 * Tp[hc - 1] +
+                         Py11 * Tp[hc + 1]);
+        KT += a.kc[6] * (Px00 * Tm[hc - PX] + Px10 * Tm[hc + PX] + Px01 * Tp[hc - PX] +
+                         Px11 * Tp[hc + PX]);
+        KT += a.kc[7] * (k000 * Tm[hc - 1 - PX] + k100 * Tm[hc + 1 - PX] + k010 * Tm[hc - 1 + PX] +
+                         k110 * Tm[hc + 1 + PX] + k001 * Tp[hc - 1 - PX] + k101 * Tp[hc + 1 - PX] +
+                         k011 * Tp[hc - 1 + PX] + k111 * Tp[hc + 1 + PX]);
+        const float M = Msum * mfac;
+        float F = 0.0f;
+        if (SRC) F = sSrc[tx] * sSrc[LX + ty] * sSrc[LX + LY + z] * a.w3;
+        if (a.conv && zo == nez_act) {
+            float cr = 0.0f;
             for (int e = 0; e < 4; ++e) {
                 const int exx = ix - 1 + (e & 1), eyy = iy - 1 + (e >> 1);
                 if (exx < 0 || exx >= nex || eyy < 0 || eyy >= ney) continue;
-                const int fx = exx - ox + 1, fy = eyy - oy + 1;  // halo coords of face node 0
-                const float* row0 = sT + HX * (fy + HY * hz);
-                const float Tf[4] = {row0[fx], row0[fx + 1], row0[fx + 1 + HX], row0[fx + HX]};
+                const int f0 = (exx - ox + 1) + PX * (eyy - oy + 1);
+                const float Tf[4] = {T0[f0], T0[f0 + 1], T0[f0 + 1 + PX], T0[f0 + PX]};
                 const int ax = ix - exx, ay = iy - eyy;
                 const int ai = ay ? (ax ? 2 : 3) : (ax ? 1 : 0);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const float Tq = n2(q, 0) * Tf[0] + n2(q, 1) * Tf[1] + n2(q, 2) * Tf[2] +
                                      n2(q, 3) * Tf[3];
-                    cr += n2(q, ai) * (face_flux(m, Tq) * a.w2);
+                    cr += n2(q, ai) * (face_flux(*m, Tq) * a.w2);
                 }
             }
             F += cr;
         }
-        float C = 0.0f;
-        if (a.corr) {
-            const int bx = ix - clo[0], by = iy - clo[1], bz = iz - clo[2];
-            if (bx >= 0 && bx < cn[0] && by >= 0 && by < cn[1] && bz >= 0 && bz < cn[2])
-                C = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
-        }
-        if (a.corr_full) C += a.corr_full[idx];
-
-        float Tnew = ((M * Ti - KT) + F + C) / M;
-        if (idx >= a.nn_active) Tnew = m.T_amb;
-
-        if (a.bc_mode == GM_BC_L1) {
-            if (iz == 0) Tnew = a.bc_vals[4];
+        float Tnew = ((M * Ti - KT) + F + rC[z]) / M;
+        if (idx >= a.nn_active) Tnew = Tamb;
+        if (BC == GM_BC_L1) {
+            if (zo == 0) Tnew = a.bc_vals[4];
             else if (ix == nx - 1) Tnew = a.bc_vals[3];
             else if (ix == 0) Tnew = a.bc_vals[2];
             else if (iy == ny - 1) Tnew = a.bc_vals[1];
             else if (iy == 0) Tnew = a.bc_vals[0];
-        } else if (a.bc_mode == GM_BC_FINE) {
-            if (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1 || iz == 0) {
-                const gm_grid& cg = d->g[a.bc_level];
-                float N[8];
-                int64_t nd[8];
-                interp8(cg, coordf(g, 0, ix), coordf(g, 1, iy), coordf(g, 2, iz), N, nd);
-                const float al = a.alpha, be = 1.0f - a.alpha;
-                float s = 0.0f;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) s += N[q] * (al * a.bcA[nd[q]] + be * a.bcB[nd[q]]);
-                Tnew = s;
-            }
+        } else if (BC == GM_BC_FINE) {
+            if (face_col || zo == 0) Tnew = a.alpha * rFa[z] + (1.0f - a.alpha) * rFb[z];
         }
-        if (a.clamp && Tnew < m.T_amb) Tnew = m.T_amb;
+        if (a.clamp && Tnew < Tamb) Tnew = Tamb;
         a.Tout[idx] = Tnew;
-
-        if (a.accum_on) {
-            const bool s2n = Ti >= m.T_liq;
-            const bool reset = (!a.S2prev[idx]) && s2n;
-            const int lx0 = d->l0_lo3[0] + ix, ly0 = d->l0_lo3[1] + iy, lz0 = d->l0_lo3[2] + iz;
-            if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] && lz0 >= 0 &&
-                lz0 < d->l0_n[2]) {
-                const int64_t l0 = nid(lx0, ly0, lz0, d->l0_n[0], d->l0_n[1]);
-                const float acc = a.accum[l0];
-                const float racc = reset ? acc : 0.0f;
-                const float mx = a.max_accum[l0];
-                a.max_accum[l0] = racc > mx ? racc : mx;
-                a.accum[l0] = acc + d->rows[a.src_row][5] * (s2n ? 1.0f : 0.0f) - racc;
+        if (ACC) {
+            const bool s2n = Ti >= pc.Tliq;
+            const bool reset = (!rS2[z]) && s2n;
+            const int lz0 = d->l0_lo3[2] + zo;
+            if (l0xy && lz0 >= 0 && lz0 < d->l0_n[2]) {
+                const int64_t l0 = (int64_t)l0base + (int64_t)d->l0_n[0] * d->l0_n[1] * lz0;
+                const float racc = reset ? rAcc[z] : 0.0f;
+                a.max_accum[l0] = racc > rMax[z] ? racc : rMax[z];
+                a.accum[l0] = rAcc[z] + d->rows[a.src_row][5] * (s2n ? 1.0f : 0.0f) - racc;
             }
             a.S2out[idx] = (uint8_t)s2n;
         }
     }
 }
 
-void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat& m,
-                       hipStream_t s) {
-    const int ntx = (hg.n[0] + TX - 1) / TX, nty = (hg.n[1] + TY - 1) / TY,
-              ntz = (hg.n[2] + TZ - 1) / TZ;
-    const int nb = ntx * nty * ntz;
-    if (a.s1_float)
-        hipLaunchKernelGGL(k_level_step<true>, dim3(nb), dim3(256), 0, s, a, d, m);
-    else
-        hipLaunchKernelGGL(k_level_step<false>, dim3(nb), dim3(256), 0, s, a, d, m);
+template <bool S1F, bool SRC, int BC, bool ACC, bool KIN>
+static void launch_ls(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
+                      hipStream_t s) {
+    const int ntx = (hg.n[0] + LX - 1) / LX, nty = (hg.n[1] + LY - 1) / LY;
+    // 4 planes per block on meshes up to ~1M nodes (enough blocks for 256
+    // CUs), 8 planes on larger ones (less halo work per node)
+    if ((int64_t)hg.n[0] * hg.n[1] * hg.n[2] > (1 << 20)) {
+        const int nb = ntx * nty * ((hg.n[2] + 7) / 8);
+        hipLaunchKernelGGL((k_level_step<S1F, SRC, BC, ACC, KIN, 8>), dim3(nb), dim3(256), 0, s, a,
+                           d, m);
+    } else {
+        const int nb = ntx * nty * ((hg.n[2] + 3) / 4);
+        hipLaunchKernelGGL((k_level_step<S1F, SRC, BC, ACC, KIN, 4>), dim3(nb), dim3(256), 0, s, a,
+                           d, m);
+    }
 }
+
+void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
+                       hipStream_t s) {
+    if (a.kin) {
+        launch_ls<false, false, GM_BC_NONE, false, true>(a, d, hg, m, s);
+    } else if (a.s1_float) {  // L1
+        if (a.bc_mode == GM_BC_L1) launch_ls<true, false, GM_BC_L1, false, false>(a, d, hg, m, s);
+        else launch_ls<true, false, GM_BC_NONE, false, false>(a, d, hg, m, s);
+    } else if (a.src_mode == GM_SRC_L3) {  // L3
+        if (a.accum_on) launch_ls<false, true, GM_BC_FINE, true, false>(a, d, hg, m, s);
+        else launch_ls<false, true, GM_BC_FINE, false, false>(a, d, hg, m, s);
+    } else {  // L2
+        launch_ls<false, false, GM_BC_FINE, false, false>(a, d, hg, m, s);
+    }
+}
+
+// Dirichlet values on the W, E, S, N and bottom faces of a fine level:
+// A and B prolonged separately (interpolatePointsMatrix rule, cF:1060-1104)
+// onto full-size arrays at the face nodes (cF:1598-1620); the level step
+// blends them as alpha*A + (1-alpha)*B.
+__global__ void k_face_bc(const GmStepDesc* __restrict__ d, int fine, int coarse,
+                          const float* __restrict__ A, const float* __restrict__ B,
+                          float* __restrict__ fa, float* __restrict__ fb) {
+    const gm_grid& g = d->g[fine];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int nW = ny * nz, nS = nx * nz, nB = nx * ny;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int ix, iy, iz;
+    if (t < 2 * nW) {
+        ix = t < nW ? 0 : nx - 1;
+        t %= nW;
+        iy = t % ny;
+        iz = t / ny;
+    } else if ((t -= 2 * nW) < 2 * nS) {
+        iy = t < nS ? 0 : ny - 1;
+        t %= nS;
+        ix = t % nx;
+        iz = t / nx;
+    } else if ((t -= 2 * nS) < nB) {
+        iz = 0;
+        ix = t % nx;
+        iy = t / nx;
+    } else {
+        return;
+    }
+    float N[8];
+    int64_t nd[8];
+    interp8(d->g[coarse], coordf(g, 0, ix), coordf(g, 1, iy), coordf(g, 2, iz), N, nd);
+    float sa = 0.0f, sb = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        sa += N[q] * A[nd[q]];
+        sb += N[q] * B[nd[q]];
+    }
+    const int idx = ix + nx * iy + nx * ny * iz;
+    fa[idx] = sa;
+    fb[idx] = sb;
+}
+
+void launch_face_bc(const GmStepDesc* d, int fine, int coarse, const gm_grid& gf, const float* A,
+                    const float* B, float* fa, float* fb, hipStream_t s) {
+    const int n = 2 * gf.n[1] * gf.n[2] + 2 * gf.n[0] * gf.n[2] + gf.n[0] * gf.n[1];
+    hipLaunchKernelGGL(k_face_bc, dim3((n + 255) / 256), dim3(256), 0, s, d, fine, coarse, A, B,
+                       fa, fb);
+}
+
 
 // ---------------------------------------------------------------------------
 // Fine -> coarse axis tables (the separable factors of the Shapes tensors)
@@ -659,10 +836,15 @@ void launch_restrict(const GmPair& p, const GmStepDesc* d, const DevMat& m, cons
 }
 
 // Sum the element partials of one (or two nested) pairs onto the coarse node
-// box of the outer pair; V has the layout [cmax+1]^3 of the outer pair.
+// box of the outer pair, plus (optionally) the coarse laser source
+//   F_A = scale * sum_pos Gx Gy Gz
+// of computeLevelSource on the node box of the source pair.  V has the
+// layout [cmax+1]^3 of the outer pair.
 __global__ void k_restrict_nodes(GmPair po, const double* __restrict__ parto, GmPair pi,
                                  const double* __restrict__ parti, int have_inner,
-                                 const float* __restrict__ Vbase, float* __restrict__ V) {
+                                 const float* __restrict__ Vbase, float* __restrict__ V,
+                                 GmPair pg, const float* __restrict__ G, int npos, int nmax,
+                                 float scale) {
     const int mx = po.cmax[0] + 1, my = po.cmax[1] + 1, mz = po.cmax[2] + 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= mx * my * mz) return;
@@ -683,17 +865,33 @@ __global__ void k_restrict_nodes(GmPair po, const double* __restrict__ parto, Gm
             s += pa[w][((int64_t)cx + (int64_t)n[0] * (cy + (int64_t)n[1] * cz)) * 8 + a];
         }
     }
-    const float base = Vbase ? Vbase[t] : 0.0f;
-    V[t] = base + (float)s;
+    float v = (Vbase ? Vbase[t] : 0.0f) + (float)s;
+    if (G) {
+        const int gx = A[0] - pg.ax[0].info[0], gy = A[1] - pg.ax[1].info[0],
+                  gz = A[2] - pg.ax[2].info[0];
+        if (gx >= 0 && gx <= pg.ax[0].info[1] && gy >= 0 && gy <= pg.ax[1].info[1] && gz >= 0 &&
+            gz <= pg.ax[2].info[1]) {
+            float f = 0.0f;
+            for (int q = 0; q < npos; ++q) {
+                const float* Gq = G + q * 3 * nmax;
+                f += Gq[gx] * Gq[nmax + gy] * Gq[2 * nmax + gz];
+            }
+            v += f * scale;
+        }
+    }
+    V[t] = v;
 }
 
 void launch_restrict_nodes(const GmPair& outer, const double* part_outer, const GmPair* inner,
                            const double* part_inner, const float* Vbase, float* V,
+                           const GmPair* gpair, const float* G, int npos, int nmax, float scale,
                            hipStream_t s) {
     const int n = (outer.cmax[0] + 1) * (outer.cmax[1] + 1) * (outer.cmax[2] + 1);
     GmPair pi = inner ? *inner : outer;
+    GmPair pg = gpair ? *gpair : outer;
     hipLaunchKernelGGL(k_restrict_nodes, dim3((n + 255) / 256), dim3(256), 0, s, outer, part_outer,
-                       pi, part_inner, inner ? 1 : 0, Vbase, V);
+                       pi, part_inner, inner ? 1 : 0, Vbase, V, pg, gpair ? G : nullptr, npos,
+                       nmax, scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -1207,6 +1405,38 @@ DevMat make_devmat(const gm_material& mm) {
     m.rc = (float)(1.0 / (r * sqrt(M_PI)));
     m.dc = (float)(1.0 / (dd * sqrt(M_PI)));
     m.pc6 = (float)(6.0 * sqrt(3.0) * mm.laser_eta);
+    // Segment lookup: bins of width w over [T_first, T_last]; lut[b] is the
+    // segment of a temperature half a bin below the bin's lower edge, a lower
+    // bound of the segment of every T that lands in bin b, so the kernel's
+    // forward search from it ends at the same segment as a full scan.
+    const int tn[2] = {mm.k_table_n, mm.cp_table_n};
+    const float* tT[2] = {mm.k_table_T, mm.cp_table_T};
+    for (int w = 0; w < 2; ++w) {
+        const int n = tn[w];
+        const float* kT = tT[w];
+        int nb = 1;
+        double T0 = n > 0 ? kT[0] : 0.0, iw = 0.0;
+        if (n >= 2 && kT[n - 1] > kT[0]) {
+            double dmin = kT[n - 1] - kT[0];
+            for (int i = 0; i + 1 < n; ++i)
+                if (kT[i + 1] - kT[i] > 0.0) dmin = std::min(dmin, (double)kT[i + 1] - kT[i]);
+            nb = (int)std::min<double>(GM_LUT_BINS, std::ceil(2.0 * (kT[n - 1] - kT[0]) / dmin));
+            nb = std::max(nb, 1);
+            iw = nb / ((double)kT[n - 1] - kT[0]);
+        }
+        m.lut_T0[w] = (float)T0;
+        m.lut_iw[w] = (float)iw;
+        m.lut_nb[w] = nb;
+        for (int b = 0; b < GM_LUT_BINS; ++b) {
+            int j = 0;
+            if (iw > 0.0) {
+                const double Tb = T0 + (b - 0.5) / iw;
+                for (int i = 1; i < n - 1; ++i)
+                    if ((double)kT[i] <= Tb) j = i;
+            }
+            m.lut[w][b] = j;
+        }
+    }
     return m;
 }
 
