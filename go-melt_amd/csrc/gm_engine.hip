@@ -456,6 +456,13 @@ struct gm_engine {
     uint8_t* S13b[2] = {nullptr, nullptr};
     uint8_t* S23b[2] = {nullptr, nullptr};
     float* Tp3x = nullptr;
+    // k, cp companions of the L3 temperature buffers that feed L3 steps
+    // (T3, T3b[0..1], L3all[0..N2-1]): kc_T[i] belongs to kc_key[i]
+    std::vector<const float*> kc_key;
+    std::vector<float*> kc_T;
+    bool use_kc = false;      // L3 restrictions read the carried k, cp
+    bool fast_restrict = false;  // all pairs node-aligned: node-centric restriction
+    float mcs[4][8];          // consistent mass per neighbour pattern (x rho)
     uint8_t* preS2 = nullptr;
     float *fa2 = nullptr, *fb2 = nullptr, *fa3 = nullptr, *fb3 = nullptr;
     float* mm = nullptr;
@@ -515,6 +522,8 @@ extern "C" int gm_engine_destroy(gm_engine* e) {
     return 0;
 }
 
+static bool pairs_aligned(gm_engine* e);
+
 extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, void* stream,
                                 gm_engine** out) {
     gm_engine* e = new gm_engine();
@@ -537,6 +546,14 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
         level_coeffs(cfg->level[l], e->kc[l], &e->mrow[l], &e->w2[l], &e->w3[l]);
         for (int p = 0; p < 8; ++p) e->kc[l][p] = (float)((double)e->kc[l][p] / 1000.0);
         e->mrow[l] = (float)((double)e->mrow[l] * mat->rho);
+        // consistent mass of a box element, per neighbour pattern: rho *
+        // prod_d h_d (2 or 1)/6 (the 1-D mass matrix (h/6)[2 1; 1 2])
+        for (int p = 0; p < 8; ++p) {
+            double v = mat->rho;
+            for (int d = 0; d < 3; ++d)
+                v *= (double)cfg->level[l].h[d] * (((p >> d) & 1) ? 1.0 : 2.0) / 6.0;
+            e->mcs[l][p] = (float)v;
+        }
     }
     const int64_t n1 = e->nn[1], n2 = e->nn[2], n3 = e->nn[3], n0 = e->nn[0];
     e->nst = std::max(1, (int)lround((double)cfg->level[1].h[2] / cfg->layer_height));
@@ -576,6 +593,16 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
             eng_alloc(e, (void**)&e->L3all[i], n3 * 4) ||
             eng_alloc(e, (void**)&e->histTp[i], n3 * 4))
             return bad(-1);
+    }
+    {
+        std::vector<const float*> keys = {e->T3, e->T3b[0], e->T3b[1]};
+        for (int i = 0; i < e->N2; ++i) keys.push_back(e->L3all[i]);
+        for (const float* k : keys) {
+            float* b = nullptr;
+            if (eng_alloc(e, (void**)&b, n3 * 8)) return bad(-1);
+            e->kc_key.push_back(k);
+            e->kc_T.push_back(b);
+        }
     }
     // fine -> coarse pairs in the reference's Shapes order: L2->L1, L3->L1, L3->L2
     const int pc[3] = {1, 1, 2}, pf[3] = {2, 3, 3};
@@ -641,6 +668,9 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
     HIPCHK(hipStreamSynchronize(e->s));
     if (upload_desc(e, desc_bytes(0))) return bad(-1);
     for (int i = 0; i < 3; ++i) launch_axis_tables(e->pair[i].p, e->d_desc, e->s);
+    for (int i = 0; i < 3; ++i)
+        HIPCHK(hipMemsetAsync(e->part[i], 0, pair_elems(e->pair[i].p) * 8 * sizeof(double), e->s));
+    e->fast_restrict = pairs_aligned(e);
     HIPCHK(hipStreamSynchronize(e->s));
     HIPCHK(hipGetLastError());
     *out = e;
@@ -691,6 +721,20 @@ extern "C" int gm_engine_set_layer(gm_engine* e, int64_t ne_act, int64_t nn_act,
 extern "C" int gm_engine_use_graph(gm_engine* e, int enable) {
     e->use_graph = enable != 0;
     e->graph_ok = false;
+    return 0;
+}
+
+// Re-checks the node alignment of the pairs after the meshes changed; a
+// change of restriction path clears the partial buffers and the graph.
+static int update_alignment(gm_engine* e) {
+    const bool a = pairs_aligned(e);
+    if (a != e->fast_restrict) {
+        e->fast_restrict = a;
+        e->graph_ok = false;
+        for (int i = 0; i < 3; ++i)
+            HIPCHK(hipMemsetAsync(e->part[i], 0, pair_elems(e->pair[i].p) * 8 * sizeof(double),
+                                  e->s));
+    }
     return 0;
 }
 
@@ -747,6 +791,7 @@ extern "C" int gm_engine_move(gm_engine* e, const float* v, const float* vstart,
         e->s2[d] = s2[d];
         if (move_v_out) move_v_out[d] = mv[d];
     }
+    if (update_alignment(e)) return -1;
     if (upload_desc(e, desc_bytes(0))) return -1;
     const int64_t n3 = e->nn[3], n2 = e->nn[2];
     launch_move_l3(e->d_desc, e->T1, e->Tp2, e->Tp3, e->T3b[0], e->Tp3x, n3, e->s);
@@ -870,6 +915,48 @@ static void run_l3(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t*
     launch_level_step(a, e->d_desc, e->cfg.level[3], e->d_dm, e->s);
 }
 
+// k, cp companion of an L3 temperature buffer
+static float* kc_of(gm_engine* e, const float* T) {
+    for (size_t i = 0; i < e->kc_key.size(); ++i)
+        if (e->kc_key[i] == T) return e->kc_T[i];
+    return nullptr;
+}
+
+// L3 solve of the subcycle with the nodal properties carried between steps
+static void run_l3_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
+                      int row, float alpha, float* Tout, const uint8_t* S2prev, uint8_t* S2out) {
+    LSArgs a = ls_default();
+    a.level = 3;
+    a.T = Tin;
+    a.S1 = S1in;
+    a.S1out = S1out;
+    a.Tout = Tout;
+    a.kcin = kc_of(e, Tin);
+    a.kcout = kc_of(e, Tout);
+    a.n_sub = e->host.sub[3];
+    a.dt_mode = GM_DT_ROW;
+    a.dt_index = row;
+    a.src_mode = GM_SRC_L3;
+    a.src_row = row;
+    a.conv = 1;
+    a.bcA = e->fa3;
+    a.bcB = e->fb3;
+    a.alpha = alpha;
+    a.clamp = 1;
+    if (S2out) {
+        a.accum_on = 1;
+        a.S2prev = S2prev;
+        a.S2out = S2out;
+        a.accum = e->acc;
+        a.max_accum = e->macc;
+    }
+    memcpy(a.kc, e->kc[3], sizeof(a.kc));
+    a.mrow = e->mrow[3];
+    a.w2 = e->w2[3];
+    a.w3 = e->w3[3];
+    launch_level_step_kc(a, e->d_desc, e->cfg.level[3], e->d_dm, e->s);
+}
+
 static RArgs rargs(gm_engine* e, int fine, const float* T, const uint8_t* S1, const float* Tp,
                    const float* TpA, const float* TpB, int dt_mode, int dt_index, float dt) {
     RArgs r;
@@ -885,6 +972,55 @@ static RArgs rargs(gm_engine* e, int fine, const float* T, const uint8_t* S1, co
     r.dt = dt;
     r.wf = e->w3[fine];
     return r;
+}
+
+// Fine nodes on every coarse node of the overlap along each axis: integer
+// spacing ratio, fine origin an integer number of fine spacings from the
+// coarse origin, fine extent inside the coarse mesh.
+static bool grids_aligned(const gm_grid& gc, const gm_grid& gf) {
+    for (int d = 0; d < 3; ++d) {
+        const double R = (double)gc.h[d] / gf.h[d];
+        const double off = ((double)gf.x0[d] - gc.x0[d]) / gf.h[d];
+        if (std::fabs(R - std::round(R)) > 1e-3 || std::round(R) < 1.0) return false;
+        if (std::fabs(off - std::round(off)) > 1e-3 || std::round(off) < 0.0) return false;
+        if (std::round(off) + (gf.n[d] - 1) > std::round(R) * (gc.n[d] - 1) + 1e-6) return false;
+    }
+    return true;
+}
+
+static bool pairs_aligned(gm_engine* e) {
+    bool ok = true;
+    for (int i = 0; i < 3; ++i)
+        ok = ok && grids_aligned(e->host.g[e->pair[i].p.coarse], e->host.g[e->pair[i].p.fine]);
+    return ok;
+}
+
+// Restricted T' terms of pair i into its partial buffer: node-centric FP64
+// node box when the pairs are aligned, element partials otherwise.
+static void eng_restrict(gm_engine* e, int i, RArgs r) {
+    const GmPair& p = e->pair[i].p;
+    if (e->fast_restrict) {
+        if (e->use_kc && p.fine == 3) r.KC = kc_of(e, r.T);
+        memcpy(r.kcs, e->kc[p.fine], sizeof(r.kcs));
+        memcpy(r.mcs, e->mcs[p.fine], sizeof(r.mcs));
+        launch_restrict_fast(p, e->d_desc, e->d_dm, r, e->part[i], e->s);
+    } else {
+        launch_restrict(p, e->d_desc, e->dm, r, e->part[i], e->s);
+    }
+}
+
+static void eng_restrict_nodes(gm_engine* e, int outer, int inner, const float* Vbase, float* V,
+                               int gpair, const float* G, int npos, int nmax, float scale) {
+    const GmPair* pi = inner >= 0 ? &e->pair[inner].p : nullptr;
+    const GmPair* pg = gpair >= 0 ? &e->pair[gpair].p : nullptr;
+    if (e->fast_restrict)
+        launch_restrict_nodes_acc(e->pair[outer].p, e->part[outer], pi,
+                                  inner >= 0 ? e->part[inner] : nullptr, Vbase, V, pg, G, npos,
+                                  nmax, scale, e->s);
+    else
+        launch_restrict_nodes(e->pair[outer].p, e->part[outer], pi,
+                              inner >= 0 ? e->part[inner] : nullptr, Vbase, V, pg, G, npos, nmax,
+                              scale, e->s);
 }
 
 // getNewTprime (cF:2060-2099) for fine level f in {2, 3}.
@@ -912,6 +1048,9 @@ static void subcycle_body(gm_engine* e) {
 
     // L1 S1 from the L2 overlap, then substrate (cF:3273-3278)
     l1_s1_update(e);
+    // k, cp of the start-of-step L3 state for the carried-property L3 steps
+    launch_props_kc(e->d_dm, e->T3, e->S13, n3, e->host.sub[3], kc_of(e, e->T3), s);
+    e->use_kc = true;
     // L1 source over all N2*N3 positions (cF:3285-3287)
     launch_source_tables(e->pair[1].p, d, e->dm, 0, N23, 0, nullptr, e->G1, e->gmax[1], s);
     // L2 sources per L2 substep (cF:3329-3331), shared by both passes
@@ -919,14 +1058,12 @@ static void subcycle_body(gm_engine* e) {
         launch_source_tables(e->pair[2].p, d, e->dm, k * N3, N3, 0, nullptr,
                              e->G2 + (size_t)k * N3 * 3 * e->gmax[2], e->gmax[2], s);
     // L1 flux terms from L3 and L2 T' (computeL1TprimeTerms_Part1, cF:2733-2810)
-    launch_restrict(e->pair[1].p, d, e->dm,
-                    rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
-                    e->part[1], s);
-    launch_restrict(e->pair[0].p, d, e->dm,
-                    rargs(e, 2, e->T2, e->S12, e->Tp2, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
-                    e->part[0], s);
-    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], nullptr, e->V1,
-                          &e->pair[1].p, e->G1, N23, e->gmax[1], e->w3[3] / (float)N23, s);
+    eng_restrict(e, 1,
+                    rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));
+    eng_restrict(e, 0,
+                    rargs(e, 2, e->T2, e->S12, e->Tp2, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));
+    eng_restrict_nodes(e, 0, 1, nullptr, e->V1,
+                          1, e->G1, N23, e->gmax[1], e->w3[3] / (float)N23);
     // L1 predictor; its L2 face values for the Part-1 L2 substeps
     run_l1(e, e->V1, e->T1p, GM_DT_L1, 0.0f, 1);
     launch_face_bc(d, 2, 1, e->cfg.level[2], e->T1p, e->T1, e->fa2, e->fb2, s);
@@ -940,11 +1077,10 @@ static void subcycle_body(gm_engine* e) {
     int b2 = 0, b3 = 0;
     for (int k = 0; k < N2; ++k) {
         const float* Gk = e->G2 + (size_t)k * N3 * 3 * e->gmax[2];
-        launch_restrict(e->pair[2].p, d, e->dm,
-                        rargs(e, 3, T3cur, S13cur, Tp3cur, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
-                        e->part[2], s);
-        launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2,
-                              &e->pair[2].p, Gk, N3, e->gmax[2], e->w3[3] / (float)N3, s);
+        eng_restrict(e, 2,
+                        rargs(e, 3, T3cur, S13cur, Tp3cur, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));
+        eng_restrict_nodes(e, 2, -1, nullptr, e->V2,
+                              2, Gk, N3, e->gmax[2], e->w3[3] / (float)N3);
         float* T2new = e->T2b[b2];
         uint8_t* S12new = e->S12b[b2];
         b2 ^= 1;
@@ -956,8 +1092,8 @@ static void subcycle_body(gm_engine* e) {
             float* T3o = e->T3b[b3];
             uint8_t* S13o = e->S13b[b3];
             b3 ^= 1;
-            run_l3(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, 1, nullptr,
-                   nullptr);
+            run_l3_kc(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, nullptr,
+                      nullptr);
             T3in = T3o;
             S13in = S13o;
         }
@@ -970,14 +1106,12 @@ static void subcycle_body(gm_engine* e) {
     }
     // L2 T' and L1 injection, then the L1 mass terms and corrector (cF:3433-3456)
     new_tprime(e, 2, T2cur, e->T1p, e->Tp2x);
-    launch_restrict(e->pair[1].p, d, e->dm,
-                    rargs(e, 3, e->T3, e->S13, nullptr, Tp3cur, e->Tp3, GM_DT_L1, 0, 0.0f),
-                    e->part[1], s);
-    launch_restrict(e->pair[0].p, d, e->dm,
-                    rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_L1, 0, 0.0f),
-                    e->part[0], s);
-    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], e->V1, e->V1m,
-                          nullptr, nullptr, 0, 0, 0.0f, s);
+    eng_restrict(e, 1,
+                    rargs(e, 3, e->T3, e->S13, nullptr, Tp3cur, e->Tp3, GM_DT_L1, 0, 0.0f));
+    eng_restrict(e, 0,
+                    rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_L1, 0, 0.0f));
+    eng_restrict_nodes(e, 0, 1, e->V1, e->V1m,
+                          -1, nullptr, 0, 0, 0.0f);
     run_l1(e, e->V1m, e->T1c, GM_DT_L1, 0.0f, 1);
     launch_face_bc(d, 2, 1, e->cfg.level[2], e->T1c, e->T1, e->fa2, e->fb2, s);
 
@@ -992,11 +1126,10 @@ static void subcycle_body(gm_engine* e) {
     for (int k = 0; k < N2; ++k) {
         const float* Gk = e->G2 + (size_t)k * N3 * 3 * e->gmax[2];
         // flux + mass restriction onto L2 (cF:3496-3508)
-        launch_restrict(e->pair[2].p, d, e->dm,
-                        rargs(e, 3, T3cur, S13cur, Tp3cur, e->histTp[k], Tp3cur, GM_DT_L2, k, 0.0f),
-                        e->part[2], s);
-        launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2,
-                              &e->pair[2].p, Gk, N3, e->gmax[2], e->w3[3] / (float)N3, s);
+        eng_restrict(e, 2,
+                        rargs(e, 3, T3cur, S13cur, Tp3cur, e->histTp[k], Tp3cur, GM_DT_L2, k, 0.0f));
+        eng_restrict_nodes(e, 2, -1, nullptr, e->V2,
+                              2, Gk, N3, e->gmax[2], e->w3[3] / (float)N3);
         float* T2new = e->L2all[k];
         uint8_t* S12new = e->S12b[b2s];
         b2s ^= 1;
@@ -1011,7 +1144,7 @@ static void subcycle_body(gm_engine* e) {
             uint8_t* S13o = e->S13b[bs2];
             uint8_t* S23o = e->S23b[bs2];
             bs2 ^= 1;
-            run_l3(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, 1, S23in, S23o);
+            run_l3_kc(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, S23in, S23o);
             T3in = T3o;
             S13in = S13o;
             S23in = S23o;
@@ -1036,6 +1169,7 @@ static void subcycle_body(gm_engine* e) {
     launch_copy_f32(e->T1, e->T1c, n1, s);
     // Level0 S1/S2 (cF:3628-3630)
     launch_l0_scatter(d, e->S13, e->S23, e->L0S1, e->s2snap, e->s2box, n3, s);
+    e->use_kc = false;
 }
 
 extern "C" int gm_engine_subcycle(gm_engine* e, const float* rows, int n_rows) {
@@ -1111,19 +1245,16 @@ extern "C" int gm_engine_step(gm_engine* e, const float* row) {
     launch_source_tables(e->pair[1].p, d, e->dm, 0, 1, 0, nullptr, e->G1, e->gmax[1], s);
     launch_source_tables(e->pair[2].p, d, e->dm, 0, 1, 0, nullptr, e->G2, e->gmax[2], s);
     // predictor T' flux terms (computeCoarseTprimeTerm_jax cF:1477-1565)
-    launch_restrict(e->pair[1].p, d, e->dm,
-                    rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
-                    e->part[1], s);
-    launch_restrict(e->pair[0].p, d, e->dm,
-                    rargs(e, 2, e->T2, e->S12, e->Tp2, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
-                    e->part[0], s);
-    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], nullptr, e->V1,
-                          &e->pair[1].p, e->G1, 1, e->gmax[1], e->w3[3], s);
-    launch_restrict(e->pair[2].p, d, e->dm,
-                    rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f),
-                    e->part[2], s);
-    launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2,
-                          &e->pair[2].p, e->G2, 1, e->gmax[2], e->w3[3], s);
+    eng_restrict(e, 1,
+                    rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));
+    eng_restrict(e, 0,
+                    rargs(e, 2, e->T2, e->S12, e->Tp2, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));
+    eng_restrict_nodes(e, 0, 1, nullptr, e->V1,
+                          1, e->G1, 1, e->gmax[1], e->w3[3]);
+    eng_restrict(e, 2,
+                    rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));
+    eng_restrict_nodes(e, 2, -1, nullptr, e->V2,
+                          2, e->G2, 1, e->gmax[2], e->w3[3]);
     // predictor
     float* T1o = e->T1p;
     float* T2o = e->T2b[0];
@@ -1135,20 +1266,17 @@ extern "C" int gm_engine_step(gm_engine* e, const float* row) {
     new_tprime(e, 3, T3o, T2o, e->Tp3x);
     new_tprime(e, 2, T2o, T1o, e->Tp2x);
     // mass terms (computeCoarseTprimeMassTerm_jax cF:1396-1474)
-    launch_restrict(e->pair[1].p, d, e->dm,
-                    rargs(e, 3, e->T3, e->S13, nullptr, e->Tp3x, e->Tp3, GM_DT_ROW, 0, 0.0f),
-                    e->part[1], s);
-    launch_restrict(e->pair[0].p, d, e->dm,
-                    rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_ROW, 0, 0.0f),
-                    e->part[0], s);
-    launch_restrict_nodes(e->pair[0].p, e->part[0], &e->pair[1].p, e->part[1], e->V1, e->V1m,
-                          nullptr, nullptr, 0, 0, 0.0f, s);
-    launch_restrict(e->pair[2].p, d, e->dm,
-                    rargs(e, 3, e->T3, e->S13, nullptr, e->Tp3x, e->Tp3, GM_DT_ROW, 0, 0.0f),
-                    e->part[2], s);
+    eng_restrict(e, 1,
+                    rargs(e, 3, e->T3, e->S13, nullptr, e->Tp3x, e->Tp3, GM_DT_ROW, 0, 0.0f));
+    eng_restrict(e, 0,
+                    rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_ROW, 0, 0.0f));
+    eng_restrict_nodes(e, 0, 1, e->V1, e->V1m,
+                          -1, nullptr, 0, 0, 0.0f);
+    eng_restrict(e, 2,
+                    rargs(e, 3, e->T3, e->S13, nullptr, e->Tp3x, e->Tp3, GM_DT_ROW, 0, 0.0f));
     float* V2m = e->Tp2x;  // reuse: the L2 T' of the predictor is no longer needed
-    launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, e->V2, V2m, nullptr, nullptr,
-                          0, 0, 0.0f, s);
+    eng_restrict_nodes(e, 2, -1, e->V2, V2m, -1, nullptr,
+                          0, 0, 0.0f);
     // corrector
     T1o = e->T1c;
     T2o = e->T2b[1];
@@ -1245,6 +1373,7 @@ extern "C" int gm_engine_layer_change(gm_engine* e, int kidx, float new_l1_z0, f
         HIPCHK(hipMemsetAsync(e->macc, 0, e->nn[0] * 4, s));
     }
     e->graph_ok = false;
+    if (update_alignment(e)) return -1;
     if (upload_desc(e, desc_bytes(0))) return -1;
     HIPCHK(hipGetLastError());
     return 0;
@@ -1270,25 +1399,26 @@ extern "C" int gm_engine_time_kernel(gm_engine* e, int which, int iters, float* 
     hipGraphExec_t ge;
     HIPCHK(hipStreamSynchronize(e->s));
     HIPCHK(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
+    e->use_kc = true;
     for (int i = 0; i < iters; ++i) {
         const int b = i & 1;
         if (which == 3)
-            run_l3(e, b ? e->T3b[0] : e->T3, e->S13, e->S13b[b], 0, 1.0f, e->T3b[b ? 1 : 0], 1,
-                   nullptr, nullptr);
+            run_l3_kc(e, b ? e->T3b[0] : e->T3, e->S13, e->S13b[b], 0, 1.0f, e->T3b[b ? 1 : 0],
+                      nullptr, nullptr);
         else if (which == 2)
             run_l2(e, b ? e->T2b[0] : e->T2, e->S12, e->S12b[b], e->V2, 1.0f, e->T2b[b ? 1 : 0],
                    GM_DT_L2, 0, 0.0f, 1);
         else if (which == 1)
             run_l1(e, e->V1, b ? e->T1c : e->T1p, GM_DT_L1, 0.0f, 1);
         else if (which == 4) {
-            launch_restrict(e->pair[2].p, e->d_desc, e->dm,
+            eng_restrict(e, 2,
                             rargs(e, 3, e->T3, e->S13, e->Tp3, e->histTp[0], e->Tp3, GM_DT_L2, 0,
-                                  0.0f),
-                            e->part[2], e->s);
-            launch_restrict_nodes(e->pair[2].p, e->part[2], nullptr, nullptr, nullptr, e->V2,
-                                  nullptr, nullptr, 0, 0, 0.0f, e->s);
+                                  0.0f));
+            eng_restrict_nodes(e, 2, -1, nullptr, e->V2,
+                                  -1, nullptr, 0, 0, 0.0f);
         }
     }
+    e->use_kc = false;
     HIPCHK(hipStreamEndCapture(e->s, &g));
     HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     hipEvent_t a, b;

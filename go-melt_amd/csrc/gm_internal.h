@@ -134,6 +134,10 @@ struct LSArgs {
     float kc[8];
     float mrow;
     float w2, w3;         // face and volume Gauss weights (detJ)
+    // nodal properties carried between steps: k[nn] then cp[nn] of T (in),
+    // of Tout (out), in the raw units of the material tables
+    const float* kcin;
+    float* kcout;
 };
 
 // Restriction (fine -> coarse) arguments.
@@ -148,6 +152,11 @@ struct RArgs {
     float dt;
     int dt_mode, dt_index;
     float wf;             // fine volume Gauss weight
+    // node-centric path (fine nodes on every coarse node): the fine element
+    // operators and, optionally, the carried nodal k, cp of T
+    const float* KC;      // k[nn] then cp[nn] (nullable: computed from T, S1)
+    float kcs[8];         // stiffness per neighbour pattern (unit k)
+    float mcs[8];         // consistent mass per neighbour pattern (unit rho*cp)
 };
 
 // ---- launchers (gm_kernels.hip) -------------------------------------------
@@ -157,12 +166,27 @@ void level_coeffs(const gm_grid& g, float kc[8], float* mrow, float* w2, float* 
 
 void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& host_grid,
                        const DevMat* m_dev, hipStream_t s);
+// fine-level step that reads k, cp of its input and writes those of its output
+void launch_level_step_kc(const LSArgs& a, const GmStepDesc* d, const gm_grid& host_grid,
+                          const DevMat* m_dev, hipStream_t s);
+// k, cp of (T, S1) for a level step that reads them
+void launch_props_kc(const DevMat* m_dev, const float* T, const uint8_t* S1, int64_t nn,
+                     int64_t n_sub, float* kc, hipStream_t s);
 void launch_axis_tables(const GmPair& p, const GmStepDesc* d, hipStream_t s);
 void launch_source_tables(const GmPair& p, const GmStepDesc* d, const DevMat& m,
                           int row0, int npos, int use_rows, const float* rows_dev,
                           float* G, int nmax, hipStream_t s);
 void launch_restrict(const GmPair& p, const GmStepDesc* d, const DevMat& m, const RArgs& r,
                      double* partial, hipStream_t s);
+// node-centric restriction; adds into the FP64 node box acc of the pair
+void launch_restrict_fast(const GmPair& p, const GmStepDesc* d, const DevMat* m, const RArgs& r,
+                          double* acc, hipStream_t s);
+// V = Vbase + sum of the pairs' FP64 node boxes (+ coarse source); the node
+// boxes are cleared for the next accumulation
+void launch_restrict_nodes_acc(const GmPair& outer, double* acc_outer, const GmPair* inner,
+                               double* acc_inner, const float* Vbase, float* V,
+                               const GmPair* gpair, const float* G, int npos, int nmax,
+                               float scale, hipStream_t s);
 void launch_restrict_nodes(const GmPair& outer, const double* part_outer, const GmPair* inner,
                            const double* part_inner, const float* Vbase, float* V,
                            const GmPair* gpair, const float* G, int npos, int nmax, float scale,

@@ -546,6 +546,335 @@ void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, 
     }
 }
 
+// Fine-level step with the nodal properties carried between steps.  The
+// input's k and cp come from the previous step (or from k_props_kc), so each
+// node's table lookups happen once, where its temperature is produced:
+//   S1 state   : S1 | (T >= T_liq) | substrate        (cF:2585-2592)
+//   T+         : the explicit update of k_level_step   (cF:582-642)
+//   k+, cp+    : properties of T+ with S1 | (T+ >= T_liq), i.e. what the next
+//                step's computeStateProperties returns.
+// Dirichlet faces take the time-blended coarse field, all nodes are active.
+template <bool SRC, bool ACC, int ZCT>
+__global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __restrict__ d,
+                                                 const DevMat* __restrict__ m) {
+    constexpr int NP = ZCT + 2;
+    __shared__ float sT[3][PN];
+    __shared__ float sKn[PN], sCn[PN];
+    __shared__ float sSk[2][QN], sSc[2][QN];
+    __shared__ float sKe[2][QN], sCe[2][QN];
+    __shared__ float sSrc[LX + LY + ZCT];
+    __shared__ float sTab[2 * LUTW];
+
+    const gm_grid g = d->g[a.level];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int nxy = nx * ny;
+    const int nn = nxy * nz;
+    const int ntx = (nx + LX - 1) / LX, nty = (ny + LY - 1) / LY;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int ox = (t % ntx) * LX, oy = ((t / ntx) % nty) * LY, oz = (t / (ntx * nty)) * ZCT;
+    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
+    const int tid = threadIdx.x;
+    const int tx = tid % LX, ty = tid / LX;
+    const int ix = ox + tx, iy = oy + ty;
+    const bool col_ok = ix < nx && iy < ny;
+    const float* __restrict__ Kin = a.kcin;
+    const float* __restrict__ Cin = a.kcin + nn;
+
+    // ---- prologue: all global loads of the block -----------------------------
+    int eoff[2];
+    bool eok[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int l = tid + 256 * e;
+        const int jx = ox + l % PX - 1, jy = oy + l / PX - 1;
+        eok[e] = l < PN && jx >= 0 && jx < nx && jy >= 0 && jy < ny;
+        eoff[e] = jx + nx * jy;
+    }
+    float rT[NP][2], rK[NP][2], rC[NP][2];
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+        const int p = oz - 1 + pl;
+        const bool pin = p >= 0 && p < nz;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            rT[pl][e] = rK[pl][e] = rC[pl][e] = 0.0f;
+            if (pin && eok[e]) {
+                const int idx = eoff[e] + nxy * p;
+                rT[pl][e] = a.T[idx];
+                rK[pl][e] = Kin[idx];
+                rC[pl][e] = Cin[idx];
+            }
+        }
+    }
+    const int cidx = ix + nx * iy;
+    const bool face_col = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
+    float rCorr[ZCT], rFa[ZCT], rFb[ZCT], rAcc[ZCT], rMax[ZCT];
+    bool rS1[ZCT], rS2[ZCT];
+    int clo0 = 0, clo1 = 0, clo2 = 0, cn0 = 0, cn1 = 0, cn2 = 0;
+    if (a.corr) {
+        clo0 = a.corr_info[0][0]; cn0 = a.corr_info[0][1] + 1;
+        clo1 = a.corr_info[1][0]; cn1 = a.corr_info[1][1] + 1;
+        clo2 = a.corr_info[2][0]; cn2 = a.corr_info[2][1] + 1;
+    }
+    int l0base = 0;
+    bool l0xy = false;
+    if (ACC) {
+        const int lx0 = d->l0_lo3[0] + ix, ly0 = d->l0_lo3[1] + iy;
+        l0xy = lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1];
+        l0base = lx0 + d->l0_n[0] * ly0;
+    }
+#pragma unroll
+    for (int z = 0; z < ZCT; ++z) {
+        const int zo = oz + z;
+        const bool ok = col_ok && zo < nz;
+        const int idx = cidx + nxy * zo;
+        float c = 0.0f;
+        if (ok && a.corr) {
+            const int bx = ix - clo0, by = iy - clo1, bz = zo - clo2;
+            if (bx >= 0 && bx < cn0 && by >= 0 && by < cn1 && bz >= 0 && bz < cn2)
+                c = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
+        }
+        rCorr[z] = c;
+        rFa[z] = rFb[z] = 0.0f;
+        if (ok && (face_col || zo == 0)) {
+            rFa[z] = a.bcA[idx];
+            rFb[z] = a.bcB[idx];
+        }
+        rS1[z] = ok && ((const uint8_t*)a.S1)[idx] != 0;
+        rAcc[z] = rMax[z] = 0.0f;
+        rS2[z] = false;
+        if (ACC && ok) {
+            rS2[z] = a.S2prev[idx] != 0;
+            const int lz0 = d->l0_lo3[2] + zo;
+            if (l0xy && lz0 >= 0 && lz0 < d->l0_n[2]) {
+                const int64_t l0 = (int64_t)l0base + (int64_t)d->l0_n[0] * d->l0_n[1] * lz0;
+                rAcc[z] = a.accum[l0];
+                rMax[z] = a.max_accum[l0];
+            }
+        }
+    }
+    float dt = a.dt;
+    if (a.dt_mode == GM_DT_ROW) dt = d->rows[a.dt_index][5];
+    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
+    const float mfac = a.mrow / dt;
+
+    PropC pc;
+    pc.Tliq = m->T_liq; pc.Tsol = m->T_sol; pc.kp = m->k_p; pc.kf = m->k_f;
+    pc.ka0 = m->k_a0; pc.ka1 = m->k_a1; pc.cpf = m->cp_f; pc.cpm = m->cp_m;
+    pc.ca0 = m->cp_a0; pc.ca1 = m->cp_a1; pc.kn = m->k_n; pc.cn = m->cp_n;
+    pc.kT0 = m->lut_T0[0]; pc.kiw = m->lut_iw[0]; pc.cT0 = m->lut_T0[1]; pc.ciw = m->lut_iw[1];
+    pc.knb = m->lut_nb[0]; pc.cnb = m->lut_nb[1];
+    const float Tamb = m->T_amb;
+    for (int l = tid; l < 2 * LUTW; l += 256) {
+        const int w = l / LUTW, r = l % LUTW;
+        float v;
+        if (r < GM_LUT_BINS) {
+            v = __int_as_float(m->lut[w][r]);
+        } else {
+            const int j = (r - GM_LUT_BINS) / 3, f = (r - GM_LUT_BINS) % 3;
+            const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
+                                                                      : (w ? m->cs : m->ks);
+            v = src[j];
+        }
+        sTab[l] = v;
+    }
+    if (SRC) {
+        const float* row = d->rows[a.src_row];
+        if (tid < LX) {
+            const int i = ox + tid;
+            sSrc[tid] = i < nx ? src_axis(g, 0, i, row[0], m->rsq, m->rc) * (m->pc6 * row[6])
+                               : 0.0f;
+        } else if (tid < LX + LY) {
+            const int i = oy + tid - LX;
+            sSrc[tid] = i < ny ? src_axis(g, 1, i, row[1], m->rsq, m->rc) : 0.0f;
+        } else if (tid < LX + LY + ZCT) {
+            const int i = oz + tid - LX - LY;
+            sSrc[tid] = i < nz ? src_axis(g, 2, i, row[2], m->dsq, m->dc) : 0.0f;
+        }
+    }
+    int qh[2];
+    bool qok[2], qxy[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int l = tid + 256 * e;
+        const int qx = l % QX, qy = l / QX;
+        qh[e] = qx + PX * qy;
+        const int ex = ox + qx - 1, ey = oy + qy - 1;
+        qok[e] = l < QN;
+        qxy[e] = l < QN && ex >= 0 && ex < nex && ey >= 0 && ey < ney;
+    }
+    const int hc = (tx + 1) + PX * (ty + 1);
+    const int q0 = tx + QX * ty;
+
+    // ---- march through the planes ---------------------------------------------
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+        const int p = oz - 1 + pl;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int l = tid + 256 * e;
+            if (l < PN) {
+                sT[pl % 3][l] = rT[pl][e];
+                sKn[l] = rK[pl][e];
+                sCn[l] = rC[pl][e];
+            }
+        }
+        __syncthreads();
+        const bool ezok = p >= 1 && p - 1 < nez;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            if (qok[e]) {
+                const int h0 = qh[e], l = tid + 256 * e;
+                const float Sk = (sKn[h0] + sKn[h0 + 1]) + (sKn[h0 + PX] + sKn[h0 + PX + 1]);
+                const float Sc = (sCn[h0] + sCn[h0 + 1]) + (sCn[h0 + PX] + sCn[h0 + PX + 1]);
+                const bool ok = qxy[e] && ezok;
+                sKe[pl & 1][l] = ok ? (sSk[(pl + 1) & 1][l] + Sk) * 0.125f : 0.0f;
+                sCe[pl & 1][l] = ok ? (sSc[(pl + 1) & 1][l] + Sc) * 0.125f : 0.0f;
+                sSk[pl & 1][l] = Sk;
+                sSc[pl & 1][l] = Sc;
+            }
+        }
+        __syncthreads();
+        if (pl < 2) continue;
+        const int z = pl - 2;
+        const int zo = oz + z;
+        if (zo >= nz || !col_ok) continue;
+        const int idx = cidx + nxy * zo;
+        const float* Tm = sT[(pl + 1) % 3];
+        const float* T0 = sT[(pl + 2) % 3];
+        const float* Tp = sT[pl % 3];
+        const float* kb = sKe[(pl + 1) & 1];
+        const float* ka = sKe[pl & 1];
+        const float* cb = sCe[(pl + 1) & 1];
+        const float* ca = sCe[pl & 1];
+        const float k000 = kb[q0], k100 = kb[q0 + 1], k010 = kb[q0 + QX], k110 = kb[q0 + QX + 1];
+        const float k001 = ka[q0], k101 = ka[q0 + 1], k011 = ka[q0 + QX], k111 = ka[q0 + QX + 1];
+        const float Msum = ((cb[q0] + cb[q0 + 1]) + (cb[q0 + QX] + cb[q0 + QX + 1])) +
+                           ((ca[q0] + ca[q0 + 1]) + (ca[q0 + QX] + ca[q0 + QX + 1]));
+        const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011, Pz11 = k110 + k111;
+        const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011, Py11 = k101 + k111;
+        const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101, Px11 = k011 + k111;
+        const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
+        const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
+        const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
+        const float Kc = Fx0 + Fx1;
+        const float Ti = T0[hc];
+        float KT = a.kc[0] * (Kc * Ti);
+        KT += a.kc[1] * (Fx0 * T0[hc - 1] + Fx1 * T0[hc + 1]);
+        KT += a.kc[2] * (Fy0 * T0[hc - PX] + Fy1 * T0[hc + PX]);
+        KT += a.kc[4] * (Fz0 * Tm[hc] + Fz1 * Tp[hc]);
+        KT += a.kc[3] * (Pz00 * T0[hc - 1 - PX] + Pz10 * T0[hc + 1 - PX] + Pz01 * T0[hc - 1 + PX] +
+                         Pz11 * T0[hc + 1 + PX]);
+        KT += a.kc[5] * (Py00 * Tm[hc - 1] + Py10 * Tm[hc + 1] + Py01 * Tp[hc - 1] +
+                         Py11 * Tp[hc + 1]);
+        KT += a.kc[6] * (Px00 * Tm[hc - PX] + Px10 * Tm[hc + PX] + Px01 * Tp[hc - PX] +
+                         Px11 * Tp[hc + PX]);
+        KT += a.kc[7] * (k000 * Tm[hc - 1 - PX] + k100 * Tm[hc + 1 - PX] + k010 * Tm[hc - 1 + PX] +
+                         k110 * Tm[hc + 1 + PX] + k001 * Tp[hc - 1 - PX] + k101 * Tp[hc + 1 - PX] +
+                         k011 * Tp[hc - 1 + PX] + k111 * Tp[hc + 1 + PX]);
+        const float M = Msum * mfac;
+        float F = 0.0f;
+        if (SRC) F = sSrc[tx] * sSrc[LX + ty] * sSrc[LX + LY + z] * a.w3;
+        if (zo == nez) {
+            float cr = 0.0f;
+            for (int e = 0; e < 4; ++e) {
+                const int exx = ix - 1 + (e & 1), eyy = iy - 1 + (e >> 1);
+                if (exx < 0 || exx >= nex || eyy < 0 || eyy >= ney) continue;
+                const int f0 = (exx - ox + 1) + PX * (eyy - oy + 1);
+                const float Tf[4] = {T0[f0], T0[f0 + 1], T0[f0 + 1 + PX], T0[f0 + PX]};
+                const int ax = ix - exx, ay = iy - eyy;
+                const int ai = ay ? (ax ? 2 : 3) : (ax ? 1 : 0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float Tq = n2(q, 0) * Tf[0] + n2(q, 1) * Tf[1] + n2(q, 2) * Tf[2] +
+                                     n2(q, 3) * Tf[3];
+                    cr += n2(q, ai) * (face_flux(*m, Tq) * a.w2);
+                }
+            }
+            F += cr;
+        }
+        float Tnew = ((M * Ti - KT) + F + rCorr[z]) / M;
+        if (face_col || zo == 0) Tnew = a.alpha * rFa[z] + (1.0f - a.alpha) * rFb[z];
+        if (a.clamp && Tnew < Tamb) Tnew = Tamb;
+        a.Tout[idx] = Tnew;
+        // S1 state of this step's properties, then k, cp of T+
+        const bool s1 = rS1[z] || Ti >= pc.Tliq || idx < a.n_sub;
+        a.S1out[idx] = (uint8_t)s1;
+        float kn, cn;
+        bool s1n;
+        props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
+        a.kcout[idx] = kn;
+        a.kcout[nn + idx] = cn;
+        if (ACC) {
+            const bool s2n = Ti >= pc.Tliq;
+            const bool reset = (!rS2[z]) && s2n;
+            const int lz0 = d->l0_lo3[2] + zo;
+            if (l0xy && lz0 >= 0 && lz0 < d->l0_n[2]) {
+                const int64_t l0 = (int64_t)l0base + (int64_t)d->l0_n[0] * d->l0_n[1] * lz0;
+                const float racc = reset ? rAcc[z] : 0.0f;
+                a.max_accum[l0] = racc > rMax[z] ? racc : rMax[z];
+                a.accum[l0] = rAcc[z] + d->rows[a.src_row][5] * (s2n ? 1.0f : 0.0f) - racc;
+            }
+            a.S2out[idx] = (uint8_t)s2n;
+        }
+    }
+}
+
+void launch_level_step_kc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg,
+                          const DevMat* m, hipStream_t s) {
+    const int ntx = (hg.n[0] + LX - 1) / LX, nty = (hg.n[1] + LY - 1) / LY;
+    const int nb = ntx * nty * ((hg.n[2] + 3) / 4);
+    const bool src = a.src_mode == GM_SRC_L3;
+    if (src && a.accum_on)
+        hipLaunchKernelGGL((k_step_kc<true, true, 4>), dim3(nb), dim3(256), 0, s, a, d, m);
+    else if (src)
+        hipLaunchKernelGGL((k_step_kc<true, false, 4>), dim3(nb), dim3(256), 0, s, a, d, m);
+    else
+        hipLaunchKernelGGL((k_step_kc<false, false, 4>), dim3(nb), dim3(256), 0, s, a, d, m);
+}
+
+// k, cp of (T, S1) as the next level step's computeStateProperties returns them
+__global__ void k_props_kc(const DevMat* __restrict__ m, const float* __restrict__ T,
+                           const uint8_t* __restrict__ S1, int64_t nn, int64_t n_sub,
+                           float* __restrict__ kc) {
+    __shared__ float sTab[2 * LUTW];
+    for (int l = threadIdx.x; l < 2 * LUTW; l += blockDim.x) {
+        const int w = l / LUTW, r = l % LUTW;
+        float v;
+        if (r < GM_LUT_BINS) {
+            v = __int_as_float(m->lut[w][r]);
+        } else {
+            const int j = (r - GM_LUT_BINS) / 3, f = (r - GM_LUT_BINS) % 3;
+            const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
+                                                                      : (w ? m->cs : m->ks);
+            v = src[j];
+        }
+        sTab[l] = v;
+    }
+    __syncthreads();
+    PropC pc;
+    pc.Tliq = m->T_liq; pc.Tsol = m->T_sol; pc.kp = m->k_p; pc.kf = m->k_f;
+    pc.ka0 = m->k_a0; pc.ka1 = m->k_a1; pc.cpf = m->cp_f; pc.cpm = m->cp_m;
+    pc.ca0 = m->cp_a0; pc.ca1 = m->cp_a1; pc.kn = m->k_n; pc.cn = m->cp_n;
+    pc.kT0 = m->lut_T0[0]; pc.kiw = m->lut_iw[0]; pc.cT0 = m->lut_T0[1]; pc.ciw = m->lut_iw[1];
+    pc.knb = m->lut_nb[0]; pc.cnb = m->lut_nb[1];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float kk, cc;
+        bool s1;
+        props_c(pc, sTab, T[i], S1[i] != 0, i < n_sub, kk, cc, s1);
+        kc[i] = kk;
+        kc[nn + i] = cc;
+    }
+}
+
+void launch_props_kc(const DevMat* m, const float* T, const uint8_t* S1, int64_t nn,
+                     int64_t n_sub, float* kc, hipStream_t s) {
+    const int64_t nb = std::min<int64_t>((nn + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_props_kc, dim3((unsigned)nb), dim3(256), 0, s, m, T, S1, nn, n_sub, kc);
+}
+
 // Dirichlet values on the W, E, S, N and bottom faces of a fine level:
 // A and B prolonged separately (interpolatePointsMatrix rule, cF:1060-1104)
 // onto full-size arrays at the face nodes (cF:1598-1620); the level step
@@ -892,6 +1221,352 @@ void launch_restrict_nodes(const GmPair& outer, const double* part_outer, const 
     hipLaunchKernelGGL(k_restrict_nodes, dim3((n + 255) / 256), dim3(256), 0, s, outer, part_outer,
                        pi, part_inner, inner ? 1 : 0, Vbase, V, pg, gpair ? G : nullptr, npos,
                        nmax, scale);
+}
+
+// ---------------------------------------------------------------------------
+// Node-centric restriction, for pairs whose fine nodes lie on every coarse
+// node of the overlap (integer spacing ratio, aligned windows).  There the
+// coarse shape function N_A restricted to a fine element is the fine
+// trilinear interpolant of its nodal values, and the 2x2x2 Gauss rule of the
+// reference integrates its products exactly, so the restricted T' terms of
+// cF:1396-1565 are
+//   V_A = sum_i N_A(x_i) r_i,
+//   r_i = -[ (K(k_e) T')_i + (M(rho*cp_e) dT')_i / dt ]
+// with K and M the fine stiffness and consistent mass operators built from
+// the element means (cF:625-626, 2773, 2895, 3106).  r is a 27-point stencil
+// of the level step's form; N_A(x_i) factors per axis into (R - j)/R and j/R
+// for a fine node j spacings past coarse node c (ratio R).  Sums over fine
+// nodes run in FP64: per column over z in registers, then x and y in LDS,
+// then one FP64 atomic add per coarse node and block.
+// ---------------------------------------------------------------------------
+#define RZ 4
+#define NZA (RZ + 1)
+#define NAXM (LX + 3)
+
+__device__ __forceinline__ float stencil27(const float* kb, const float* ka, int q0,
+                                           const float* Tm, const float* T0, const float* Tp,
+                                           int hc, const float* co) {
+    const float k000 = kb[q0], k100 = kb[q0 + 1], k010 = kb[q0 + QX], k110 = kb[q0 + QX + 1];
+    const float k001 = ka[q0], k101 = ka[q0 + 1], k011 = ka[q0 + QX], k111 = ka[q0 + QX + 1];
+    const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011, Pz11 = k110 + k111;
+    const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011, Py11 = k101 + k111;
+    const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101, Px11 = k011 + k111;
+    const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
+    const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
+    const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
+    float s = co[0] * ((Fx0 + Fx1) * T0[hc]);
+    s += co[1] * (Fx0 * T0[hc - 1] + Fx1 * T0[hc + 1]);
+    s += co[2] * (Fy0 * T0[hc - PX] + Fy1 * T0[hc + PX]);
+    s += co[4] * (Fz0 * Tm[hc] + Fz1 * Tp[hc]);
+    s += co[3] * (Pz00 * T0[hc - 1 - PX] + Pz10 * T0[hc + 1 - PX] + Pz01 * T0[hc - 1 + PX] +
+                  Pz11 * T0[hc + 1 + PX]);
+    s += co[5] * (Py00 * Tm[hc - 1] + Py10 * Tm[hc + 1] + Py01 * Tp[hc - 1] + Py11 * Tp[hc + 1]);
+    s += co[6] * (Px00 * Tm[hc - PX] + Px10 * Tm[hc + PX] + Px01 * Tp[hc - PX] +
+                  Px11 * Tp[hc + PX]);
+    s += co[7] * (k000 * Tm[hc - 1 - PX] + k100 * Tm[hc + 1 - PX] + k010 * Tm[hc - 1 + PX] +
+                  k110 * Tm[hc + 1 + PX] + k001 * Tp[hc - 1 - PX] + k101 * Tp[hc + 1 - PX] +
+                  k011 * Tp[hc - 1 + PX] + k111 * Tp[hc + 1 + PX]);
+    return s;
+}
+
+template <bool KCM, bool FLUX, bool MASS>
+__global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc* __restrict__ d,
+                                                     const DevMat* __restrict__ m, RArgs r,
+                                                     double* __restrict__ acc) {
+    constexpr int NP = RZ + 2;
+    __shared__ float sA[FLUX ? 3 : 1][PN];
+    __shared__ float sB[MASS ? 3 : 1][PN];
+    __shared__ float sKn[PN], sCn[PN];
+    __shared__ float sSk[2][QN], sSc[2][QN];
+    __shared__ float sKe[2][QN], sCe[2][QN];
+    __shared__ float sTab[KCM ? 1 : 2 * LUTW];
+    __shared__ double sZ[NZA][LY * LX];
+    __shared__ double sX[NZA][LY][NAXM];
+
+    const gm_grid gf = d->g[pp.fine];
+    const gm_grid gc = d->g[pp.coarse];
+    const int nx = gf.n[0], ny = gf.n[1], nz = gf.n[2];
+    const int nxy = nx * ny;
+    const int nn = nxy * nz;
+    const int ntx = (nx + LX - 1) / LX, nty = (ny + LY - 1) / LY;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int ox = (t % ntx) * LX, oy = ((t / ntx) % nty) * LY, oz = (t / (ntx * nty)) * RZ;
+    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
+    const int tid = threadIdx.x;
+    const int tx = tid % LX, ty = tid / LX;
+    const int ix = ox + tx, iy = oy + ty;
+    const bool col_ok = ix < nx && iy < ny;
+    int R[3], off[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        R[a] = (int)rintf(gc.h[a] / gf.h[a]);
+        off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) / gf.h[a]);
+    }
+
+    int eoff[2];
+    bool eok[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int l = tid + 256 * e;
+        const int jx = ox + l % PX - 1, jy = oy + l / PX - 1;
+        eok[e] = l < PN && jx >= 0 && jx < nx && jy >= 0 && jy < ny;
+        eoff[e] = jx + nx * jy;
+    }
+    // all global loads of the block first
+    float rA[NP][2], rB[NP][2], rK[NP][2], rC[NP][2];
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+        const int p = oz - 1 + pl;
+        const bool pin = p >= 0 && p < nz;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            rA[pl][e] = rB[pl][e] = rK[pl][e] = rC[pl][e] = 0.0f;
+            if (pin && eok[e]) {
+                const int idx = eoff[e] + nxy * p;
+                if (FLUX) rA[pl][e] = r.Tp[idx];
+                if (MASS) rB[pl][e] = r.TpA[idx] - r.TpB[idx];
+                if (KCM) {
+                    rK[pl][e] = r.KC[idx];
+                    rC[pl][e] = r.KC[nn + idx];
+                } else {
+                    rK[pl][e] = r.T[idx];
+                    rC[pl][e] = (float)r.S1[idx];
+                }
+            }
+        }
+    }
+    float dt = r.dt;
+    if (r.dt_mode == GM_DT_L1) dt = d->dt1;
+    else if (r.dt_mode == GM_DT_L2) dt = d->dt2[r.dt_index];
+    else if (r.dt_mode == GM_DT_ROW) dt = d->rows[r.dt_index][5];
+    const float idt = 1.0f / dt;
+    PropC pc;
+    if (!KCM) {
+        pc.Tliq = m->T_liq; pc.Tsol = m->T_sol; pc.kp = m->k_p; pc.kf = m->k_f;
+        pc.ka0 = m->k_a0; pc.ka1 = m->k_a1; pc.cpf = m->cp_f; pc.cpm = m->cp_m;
+        pc.ca0 = m->cp_a0; pc.ca1 = m->cp_a1; pc.kn = m->k_n; pc.cn = m->cp_n;
+        pc.kT0 = m->lut_T0[0]; pc.kiw = m->lut_iw[0]; pc.cT0 = m->lut_T0[1];
+        pc.ciw = m->lut_iw[1]; pc.knb = m->lut_nb[0]; pc.cnb = m->lut_nb[1];
+        for (int l = tid; l < 2 * LUTW; l += 256) {
+            const int w = l / LUTW, rr = l % LUTW;
+            float v;
+            if (rr < GM_LUT_BINS) {
+                v = __int_as_float(m->lut[w][rr]);
+            } else {
+                const int j = (rr - GM_LUT_BINS) / 3, f = (rr - GM_LUT_BINS) % 3;
+                const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
+                                                                          : (w ? m->cs : m->ks);
+                v = src[j];
+            }
+            sTab[l] = v;
+        }
+    }
+    int qh[2];
+    bool qok[2], qxy[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int l = tid + 256 * e;
+        const int qx = l % QX, qy = l / QX;
+        qh[e] = qx + PX * qy;
+        const int ex = ox + qx - 1, ey = oy + qy - 1;
+        qok[e] = l < QN;
+        qxy[e] = l < QN && ex >= 0 && ex < nex && ey >= 0 && ey < ney;
+    }
+    const int hc = (tx + 1) + PX * (ty + 1);
+    const int q0 = tx + QX * ty;
+    const int AzB = (oz + off[2]) / R[2];
+    double zacc[NZA];
+#pragma unroll
+    for (int a = 0; a < NZA; ++a) zacc[a] = 0.0;
+
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+        const int p = oz - 1 + pl;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int l = tid + 256 * e;
+            if (l < PN) {
+                if (FLUX) sA[pl % 3][l] = rA[pl][e];
+                if (MASS) sB[pl % 3][l] = rB[pl][e];
+                float kv = rK[pl][e], cv = rC[pl][e];
+                if (!KCM && eok[e] && p >= 0 && p < nz) {
+                    bool s1;
+                    props_c(pc, sTab, rK[pl][e], rC[pl][e] > 0.5f, eoff[e] + nxy * p < r.n_sub,
+                            kv, cv, s1);
+                }
+                sKn[l] = kv;
+                sCn[l] = cv;
+            }
+        }
+        __syncthreads();
+        const bool ezok = p >= 1 && p - 1 < nez;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            if (qok[e]) {
+                const int h0 = qh[e], l = tid + 256 * e;
+                const float Sk = (sKn[h0] + sKn[h0 + 1]) + (sKn[h0 + PX] + sKn[h0 + PX + 1]);
+                const float Sc = (sCn[h0] + sCn[h0 + 1]) + (sCn[h0 + PX] + sCn[h0 + PX + 1]);
+                const bool ok = qxy[e] && ezok;
+                sKe[pl & 1][l] = ok ? (sSk[(pl + 1) & 1][l] + Sk) * 0.125f : 0.0f;
+                sCe[pl & 1][l] = ok ? (sSc[(pl + 1) & 1][l] + Sc) * 0.125f : 0.0f;
+                sSk[pl & 1][l] = Sk;
+                sSc[pl & 1][l] = Sc;
+            }
+        }
+        __syncthreads();
+        if (pl < 2) continue;
+        const int zo = oz + pl - 2;
+        if (zo >= nz || !col_ok) continue;
+        float rr = 0.0f;
+        if (FLUX)
+            rr -= stencil27(sKe[(pl + 1) & 1], sKe[pl & 1], q0, sA[(pl + 1) % 3], sA[(pl + 2) % 3],
+                            sA[pl % 3], hc, r.kcs);
+        if (MASS)
+            rr -= idt * stencil27(sCe[(pl + 1) & 1], sCe[pl & 1], q0, sB[(pl + 1) % 3],
+                                  sB[(pl + 2) % 3], sB[pl % 3], hc, r.mcs);
+        const int u = zo + off[2];
+        const int c = u / R[2];
+        const int j = u - c * R[2];
+        const int lz = c - AzB;
+        const double w1 = (double)j / (double)R[2], w0 = 1.0 - w1;
+        const double v = (double)rr;
+#pragma unroll
+        for (int a = 0; a < NZA; ++a)
+            zacc[a] += (a == lz ? w0 * v : 0.0) + (a == lz + 1 ? w1 * v : 0.0);
+    }
+#pragma unroll
+    for (int a = 0; a < NZA; ++a) sZ[a][tid] = zacc[a];
+    __syncthreads();
+    // x: coarse node Ax gets sum over the tile's fine columns with weight 1 - |u - Ax R| / R
+    const int AxB = (ox + off[0]) / R[0], AyB = (oy + off[1]) / R[1];
+    const int xl = min(ox + LX, nx) - 1, yl = min(oy + LY, ny) - 1, zl = min(oz + RZ, nz) - 1;
+    const int NAX = (xl + off[0]) / R[0] - AxB + 2;
+    const int NAY = (yl + off[1]) / R[1] - AyB + 2;
+    const int NAZ = (zl + off[2]) / R[2] - AzB + 2;
+    for (int cI = tid; cI < NAZ * LY * NAX; cI += 256) {
+        const int az = cI / (LY * NAX), rem = cI % (LY * NAX);
+        const int yy = rem / NAX, ax = rem % NAX;
+        const int Au = (AxB + ax) * R[0] - off[0];  // fine index of coarse node Ax
+        double sum = 0.0;
+        const int lo = max(Au - R[0] + 1, ox), hi = min(Au + R[0] - 1, xl);
+        for (int i = lo; i <= hi; ++i) {
+            const int dd = i - Au;
+            sum += (double)(R[0] - (dd < 0 ? -dd : dd)) / (double)R[0] * sZ[az][yy * LX + i - ox];
+        }
+        sX[az][yy][ax] = sum;
+    }
+    __syncthreads();
+    const int lo0 = pp.ax[0].info[0], lo1 = pp.ax[1].info[0], lo2 = pp.ax[2].info[0];
+    const int nb0 = pp.ax[0].info[1] + 1, nb1 = pp.ax[1].info[1] + 1, nb2 = pp.ax[2].info[1] + 1;
+    const int st0 = pp.cmax[0] + 1, st1 = pp.cmax[1] + 1;
+    for (int cI = tid; cI < NAZ * NAY * NAX; cI += 256) {
+        const int az = cI / (NAY * NAX), rem = cI % (NAY * NAX);
+        const int ay = rem / NAX, ax = rem % NAX;
+        const int Au = (AyB + ay) * R[1] - off[1];
+        double sum = 0.0;
+        const int lo = max(Au - R[1] + 1, oy), hi = min(Au + R[1] - 1, yl);
+        for (int i = lo; i <= hi; ++i) {
+            const int dd = i - Au;
+            sum += (double)(R[1] - (dd < 0 ? -dd : dd)) / (double)R[1] * sX[az][i - oy][ax];
+        }
+        const int bx = AxB + ax - lo0, by = AyB + ay - lo1, bz = AzB + az - lo2;
+        if (sum != 0.0 && bx >= 0 && bx < nb0 && by >= 0 && by < nb1 && bz >= 0 && bz < nb2)
+            unsafeAtomicAdd(acc + (bx + (int64_t)st0 * (by + (int64_t)st1 * bz)), sum);
+    }
+}
+
+void launch_restrict_fast(const GmPair& p, const GmStepDesc* d, const DevMat* m, const RArgs& r,
+                          double* acc, hipStream_t s) {
+    // grid of the fine level: the pair's element counts + 1
+    const int nx = p.nef[0] + 1, ny = p.nef[1] + 1, nz = p.nef[2] + 1;
+    const int nb = ((nx + LX - 1) / LX) * ((ny + LY - 1) / LY) * ((nz + RZ - 1) / RZ);
+    const bool fl = r.Tp != nullptr, ms = r.TpA != nullptr;
+    if (r.KC) {
+        if (fl && ms)
+            hipLaunchKernelGGL((k_restrict_kc<true, true, true>), dim3(nb), dim3(256), 0, s, p, d,
+                               m, r, acc);
+        else if (fl)
+            hipLaunchKernelGGL((k_restrict_kc<true, true, false>), dim3(nb), dim3(256), 0, s, p,
+                               d, m, r, acc);
+        else
+            hipLaunchKernelGGL((k_restrict_kc<true, false, true>), dim3(nb), dim3(256), 0, s, p,
+                               d, m, r, acc);
+    } else {
+        if (fl && ms)
+            hipLaunchKernelGGL((k_restrict_kc<false, true, true>), dim3(nb), dim3(256), 0, s, p,
+                               d, m, r, acc);
+        else if (fl)
+            hipLaunchKernelGGL((k_restrict_kc<false, true, false>), dim3(nb), dim3(256), 0, s, p,
+                               d, m, r, acc);
+        else
+            hipLaunchKernelGGL((k_restrict_kc<false, false, true>), dim3(nb), dim3(256), 0, s, p,
+                               d, m, r, acc);
+    }
+}
+
+// V = Vbase + outer node box (+ inner node box) (+ coarse source), then the
+// node boxes are cleared for the next accumulation.
+__global__ void k_restrict_nodes_acc(GmPair po, double* __restrict__ acco, GmPair pi,
+                                     double* __restrict__ acci, int have_inner,
+                                     const float* __restrict__ Vbase, float* __restrict__ V,
+                                     GmPair pg, const float* __restrict__ G, int npos, int nmax,
+                                     float scale) {
+    const int mx = po.cmax[0] + 1, my = po.cmax[1] + 1, mz = po.cmax[2] + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mx * my * mz) return;
+    const int bx = t % mx, by = (t / mx) % my, bz = t / (mx * my);
+    const int A[3] = {po.ax[0].info[0] + bx, po.ax[1].info[0] + by, po.ax[2].info[0] + bz};
+    double s = acco[t];
+    acco[t] = 0.0;
+    if (have_inner) {
+        const int ix = A[0] - pi.ax[0].info[0], iy = A[1] - pi.ax[1].info[0],
+                  iz = A[2] - pi.ax[2].info[0];
+        if (ix >= 0 && ix <= pi.ax[0].info[1] && iy >= 0 && iy <= pi.ax[1].info[1] && iz >= 0 &&
+            iz <= pi.ax[2].info[1]) {
+            const int64_t k = ix + (int64_t)(pi.cmax[0] + 1) * (iy + (int64_t)(pi.cmax[1] + 1) * iz);
+            s += acci[k];
+        }
+    }
+    float v = (Vbase ? Vbase[t] : 0.0f) + (float)s;
+    if (G) {
+        const int gx = A[0] - pg.ax[0].info[0], gy = A[1] - pg.ax[1].info[0],
+                  gz = A[2] - pg.ax[2].info[0];
+        if (gx >= 0 && gx <= pg.ax[0].info[1] && gy >= 0 && gy <= pg.ax[1].info[1] && gz >= 0 &&
+            gz <= pg.ax[2].info[1]) {
+            float f = 0.0f;
+            for (int q = 0; q < npos; ++q) {
+                const float* Gq = G + q * 3 * nmax;
+                f += Gq[gx] * Gq[nmax + gy] * Gq[2 * nmax + gz];
+            }
+            v += f * scale;
+        }
+    }
+    V[t] = v;
+}
+
+// the inner node box is cleared by a second pass (its layout differs)
+__global__ void k_clear_f64(double* __restrict__ a, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+void launch_restrict_nodes_acc(const GmPair& outer, double* acc_outer, const GmPair* inner,
+                               double* acc_inner, const float* Vbase, float* V,
+                               const GmPair* gpair, const float* G, int npos, int nmax,
+                               float scale, hipStream_t s) {
+    const int n = (outer.cmax[0] + 1) * (outer.cmax[1] + 1) * (outer.cmax[2] + 1);
+    GmPair pi = inner ? *inner : outer;
+    GmPair pg = gpair ? *gpair : outer;
+    hipLaunchKernelGGL(k_restrict_nodes_acc, dim3((n + 255) / 256), dim3(256), 0, s, outer,
+                       acc_outer, pi, acc_inner, inner ? 1 : 0, Vbase, V, pg, gpair ? G : nullptr,
+                       npos, nmax, scale);
+    if (inner) {
+        const int64_t ni = (int64_t)(inner->cmax[0] + 1) * (inner->cmax[1] + 1) *
+                           (inner->cmax[2] + 1);
+        hipLaunchKernelGGL(k_clear_f64, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0, s,
+                           acc_inner, ni);
+    }
 }
 
 // ---------------------------------------------------------------------------
