@@ -564,14 +564,19 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
     __shared__ float sKe[2][QN], sCe[2][QN];
     __shared__ float sSrc[LX + LY + ZCT];
     __shared__ float sTab[2 * LUTW];
+    __shared__ float sFq[4][QN];  // surface flux at the 4 Gauss points of each top face
 
     const gm_grid g = d->g[a.level];
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int nxy = nx * ny;
     const int nn = nxy * nz;
-    const int ntx = (nx + LX - 1) / LX, nty = (ny + LY - 1) / LY;
+    const int ntx = (nx + LX - 1) / LX;
+    const int nzc = (nz + ZCT - 1) / ZCT;
+    // z chunk fastest: each XCD owns whole tile columns (its share of the
+    // top-surface work and its z halos)
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int ox = (t % ntx) * LX, oy = ((t / ntx) % nty) * LY, oz = (t / (ntx * nty)) * ZCT;
+    const int oz = (t % nzc) * ZCT, txy = t / nzc;
+    const int ox = (txy % ntx) * LX, oy = (txy / ntx) * LY;
     const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
     const int tid = threadIdx.x;
     const int tx = tid % LX, ty = tid / LX;
@@ -739,6 +744,26 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
         if (pl < 2) continue;
         const int z = pl - 2;
         const int zo = oz + z;
+        if (zo == nez) {
+            // top faces (cF:2254-2299): flux at the 4 face Gauss points of
+            // each element face of the tile, once per face
+            const float* Tc = sT[(pl + 2) % 3];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                if (qxy[e]) {
+                    const int l = tid + 256 * e;
+                    const int f0 = qh[e];
+                    const float Tf[4] = {Tc[f0], Tc[f0 + 1], Tc[f0 + 1 + PX], Tc[f0 + PX]};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float Tq = n2(q, 0) * Tf[0] + n2(q, 1) * Tf[1] + n2(q, 2) * Tf[2] +
+                                         n2(q, 3) * Tf[3];
+                        sFq[q][l] = face_flux(*m, Tq);
+                    }
+                }
+            }
+            __syncthreads();
+        }
         if (zo >= nz || !col_ok) continue;
         const int idx = cidx + nxy * zo;
         const float* Tm = sT[(pl + 1) % 3];
@@ -781,16 +806,11 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
             for (int e = 0; e < 4; ++e) {
                 const int exx = ix - 1 + (e & 1), eyy = iy - 1 + (e >> 1);
                 if (exx < 0 || exx >= nex || eyy < 0 || eyy >= ney) continue;
-                const int f0 = (exx - ox + 1) + PX * (eyy - oy + 1);
-                const float Tf[4] = {T0[f0], T0[f0 + 1], T0[f0 + 1 + PX], T0[f0 + PX]};
+                const int qe = (exx - ox + 1) + QX * (eyy - oy + 1);
                 const int ax = ix - exx, ay = iy - eyy;
                 const int ai = ay ? (ax ? 2 : 3) : (ax ? 1 : 0);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float Tq = n2(q, 0) * Tf[0] + n2(q, 1) * Tf[1] + n2(q, 2) * Tf[2] +
-                                     n2(q, 3) * Tf[3];
-                    cr += n2(q, ai) * (face_flux(*m, Tq) * a.w2);
-                }
+                for (int q = 0; q < 4; ++q) cr += n2(q, ai) * (sFq[q][qe] * a.w2);
             }
             F += cr;
         }
@@ -824,8 +844,8 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
 void launch_level_step_kc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg,
                           const DevMat* m, hipStream_t s) {
     const int ntx = (hg.n[0] + LX - 1) / LX, nty = (hg.n[1] + LY - 1) / LY;
-    const int nb = ntx * nty * ((hg.n[2] + 3) / 4);
     const bool src = a.src_mode == GM_SRC_L3;
+    const int nb = ntx * nty * ((hg.n[2] + 3) / 4);
     if (src && a.accum_on)
         hipLaunchKernelGGL((k_step_kc<true, true, 4>), dim3(nb), dim3(256), 0, s, a, d, m);
     else if (src)
