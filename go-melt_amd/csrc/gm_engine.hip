@@ -456,11 +456,11 @@ struct gm_engine {
     uint8_t* S13b[2] = {nullptr, nullptr};
     uint8_t* S23b[2] = {nullptr, nullptr};
     float* Tp3x = nullptr;
-    // k, cp companions of the L3 temperature buffers that feed L3 steps
-    // (T3, T3b[0..1], L3all[0..N2-1]): kc_T[i] belongs to kc_key[i]
+    // k, cp companions of the L2/L3 temperature buffers that feed level
+    // steps: kc_T[i] belongs to kc_key[i]
     std::vector<const float*> kc_key;
     std::vector<float*> kc_T;
-    bool use_kc = false;      // L3 restrictions read the carried k, cp
+    bool use_kc = false;      // restrictions read the carried k, cp
     bool fast_restrict = false;  // all pairs node-aligned: node-centric restriction
     float mcs[4][8];          // consistent mass per neighbour pattern (x rho)
     uint8_t* preS2 = nullptr;
@@ -595,12 +595,19 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
             return bad(-1);
     }
     {
-        std::vector<const float*> keys = {e->T3, e->T3b[0], e->T3b[1]};
-        for (int i = 0; i < e->N2; ++i) keys.push_back(e->L3all[i]);
-        for (const float* k : keys) {
+        // k, cp companions of every L2 and L3 temperature buffer a
+        // carried-property step reads or writes
+        std::vector<std::pair<const float*, int64_t>> keys = {
+            {e->T3, n3}, {e->T3b[0], n3}, {e->T3b[1], n3},
+            {e->T2, n2}, {e->T2b[0], n2}, {e->T2b[1], n2}};
+        for (int i = 0; i < e->N2; ++i) {
+            keys.push_back({e->L3all[i], n3});
+            keys.push_back({e->L2all[i], n2});
+        }
+        for (const auto& k : keys) {
             float* b = nullptr;
-            if (eng_alloc(e, (void**)&b, n3 * 8)) return bad(-1);
-            e->kc_key.push_back(k);
+            if (eng_alloc(e, (void**)&b, k.second * 8)) return bad(-1);
+            e->kc_key.push_back(k.first);
             e->kc_T.push_back(b);
         }
     }
@@ -922,6 +929,37 @@ static float* kc_of(gm_engine* e, const float* T) {
     return nullptr;
 }
 
+// L2 solve of the subcycle with the nodal properties carried between steps
+static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
+                      const float* corr, float alpha, float* Tout, int dt_index) {
+    LSArgs a = ls_default();
+    a.level = 2;
+    a.T = Tin;
+    a.S1 = S1in;
+    a.S1out = S1out;
+    a.Tout = Tout;
+    a.kcin = kc_of(e, Tin);
+    a.kcout = kc_of(e, Tout);
+    a.n_sub = e->host.sub[2];
+    a.dt_mode = GM_DT_L2;
+    a.dt_index = dt_index;
+    a.conv = 1;
+    a.corr = corr;
+    for (int d = 0; d < 3; ++d) {
+        a.corr_info[d] = e->pair[2].p.ax[d].info;
+        a.corr_nmax[d] = e->pair[2].p.cmax[d] + 1;
+    }
+    a.bcA = e->fa2;
+    a.bcB = e->fb2;
+    a.alpha = alpha;
+    a.clamp = 1;
+    memcpy(a.kc, e->kc[2], sizeof(a.kc));
+    a.mrow = e->mrow[2];
+    a.w2 = e->w2[2];
+    a.w3 = e->w3[2];
+    launch_level_step_kc(a, e->d_desc, e->cfg.level[2], e->d_dm, e->s);
+}
+
 // L3 solve of the subcycle with the nodal properties carried between steps
 static void run_l3_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
                       int row, float alpha, float* Tout, const uint8_t* S2prev, uint8_t* S2out) {
@@ -1000,7 +1038,7 @@ static bool pairs_aligned(gm_engine* e) {
 static void eng_restrict(gm_engine* e, int i, RArgs r) {
     const GmPair& p = e->pair[i].p;
     if (e->fast_restrict) {
-        if (e->use_kc && p.fine == 3) r.KC = kc_of(e, r.T);
+        if (e->use_kc && p.fine >= 2) r.KC = kc_of(e, r.T);
         memcpy(r.kcs, e->kc[p.fine], sizeof(r.kcs));
         memcpy(r.mcs, e->mcs[p.fine], sizeof(r.mcs));
         launch_restrict_fast(p, e->d_desc, e->d_dm, r, e->part[i], e->s);
@@ -1024,11 +1062,21 @@ static void eng_restrict_nodes(gm_engine* e, int outer, int inner, const float* 
 }
 
 // getNewTprime (cF:2060-2099) for fine level f in {2, 3}.
-static void new_tprime(gm_engine* e, int f, const float* fineT, float* coarseT, float* Tp) {
+// cS1: S1 state of the coarse field, to refresh its carried k, cp at the
+// injected nodes (L2 coarse fields of the subcycle).
+static void new_tprime(gm_engine* e, int f, const float* fineT, float* coarseT, float* Tp,
+                       const uint8_t* cS1 = nullptr) {
     const GmBox& b = f == 3 ? e->host.ov3 : e->host.ov2;
     const int64_t nb = (int64_t)b.n[0] * b.n[1] * b.n[2];
+    float* cKC = (e->use_kc && cS1) ? kc_of(e, coarseT) : nullptr;
+    if (e->fast_restrict) {
+        launch_tprime_inject(e->d_desc, f, f - 1, fineT, coarseT, Tp, e->nn[f], nb, e->d_dm,
+                             cS1, e->host.sub[f - 1], cKC, e->s);
+        return;
+    }
     launch_inject(e->d_desc, f, f - 1, fineT, coarseT, nb, e->s);
     launch_tprime(e->d_desc, f, f - 1, fineT, coarseT, Tp, e->nn[f], e->s);
+    if (cKC) launch_props_kc(e->d_dm, coarseT, cS1, e->nn[f - 1], e->host.sub[f - 1], cKC, e->s);
 }
 
 static void l1_s1_update(gm_engine* e) {
@@ -1050,6 +1098,7 @@ static void subcycle_body(gm_engine* e) {
     l1_s1_update(e);
     // k, cp of the start-of-step L3 state for the carried-property L3 steps
     launch_props_kc(e->d_dm, e->T3, e->S13, n3, e->host.sub[3], kc_of(e, e->T3), s);
+    launch_props_kc(e->d_dm, e->T2, e->S12, n2, e->host.sub[2], kc_of(e, e->T2), s);
     e->use_kc = true;
     // L1 source over all N2*N3 positions (cF:3285-3287)
     launch_source_tables(e->pair[1].p, d, e->dm, 0, N23, 0, nullptr, e->G1, e->gmax[1], s);
@@ -1084,7 +1133,7 @@ static void subcycle_body(gm_engine* e) {
         float* T2new = e->T2b[b2];
         uint8_t* S12new = e->S12b[b2];
         b2 ^= 1;
-        run_l2(e, T2cur, S12cur, S12new, e->V2, (float)(k + 1) / fN2, T2new, GM_DT_L2, k, 0.0f, 1);
+        run_l2_kc(e, T2cur, S12cur, S12new, e->V2, (float)(k + 1) / fN2, T2new, k);
         launch_face_bc(d, 3, 2, e->cfg.level[3], T2new, T2cur, e->fa3, e->fb3, s);
         const float* T3in = T3cur;
         const uint8_t* S13in = S13cur;
@@ -1097,7 +1146,7 @@ static void subcycle_body(gm_engine* e) {
             T3in = T3o;
             S13in = S13o;
         }
-        new_tprime(e, 3, T3in, T2new, e->histTp[k]);
+        new_tprime(e, 3, T3in, T2new, e->histTp[k], S12new);
         T2cur = T2new;
         S12cur = S12new;
         T3cur = T3in;
@@ -1133,7 +1182,7 @@ static void subcycle_body(gm_engine* e) {
         float* T2new = e->L2all[k];
         uint8_t* S12new = e->S12b[b2s];
         b2s ^= 1;
-        run_l2(e, T2cur, S12cur, S12new, e->V2, (float)(k + 1) / fN2, T2new, GM_DT_L2, k, 0.0f, 1);
+        run_l2_kc(e, T2cur, S12cur, S12new, e->V2, (float)(k + 1) / fN2, T2new, k);
         launch_face_bc(d, 3, 2, e->cfg.level[3], T2new, T2cur, e->fa3, e->fb3, s);
         const float* T3in = T3cur;
         const uint8_t* S13in = S13cur;
@@ -1149,7 +1198,7 @@ static void subcycle_body(gm_engine* e) {
             S13in = S13o;
             S23in = S23o;
         }
-        new_tprime(e, 3, T3in, T2new, e->histTp[k]);
+        new_tprime(e, 3, T3in, T2new, e->histTp[k], S12new);
         T2cur = T2new;
         S12cur = S12new;
         T3cur = T3in;

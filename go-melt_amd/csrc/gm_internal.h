@@ -54,6 +54,13 @@ struct DevMat {
     float lut_T0[2], lut_iw[2];
     int32_t lut_nb[2];
     int32_t lut[2][GM_LUT_BINS];
+    // both tables on the union of their knots (when both are given): one
+    // segment search yields k and cp; mseg[j] = (T_j, k(T_j), k slope,
+    // cp(T_j), cp slope) on [T_j, T_j+1), linear extension at the ends
+    int32_t mn, mnb;
+    float mT0, miw;
+    int32_t mlut[GM_LUT_BINS];
+    float mseg[2 * GM_MAX_KNOTS][5];
 };
 
 // One axis of the tabulated restriction between a fine and a coarse mesh
@@ -191,6 +198,11 @@ void launch_restrict_nodes(const GmPair& outer, const double* part_outer, const 
                            const double* part_inner, const float* Vbase, float* V,
                            const GmPair* gpair, const float* G, int npos, int nmax, float scale,
                            hipStream_t s);
+// (with cKC: also k, cp of the injected coarse nodes, from cS1)
+void launch_tprime_inject(const GmStepDesc* d, int fine_level, int coarse_level,
+                          const float* fineT, float* coarseT, float* Tp, int64_t nf, int64_t nbox,
+                          const DevMat* m, const uint8_t* cS1, int64_t c_nsub, float* cKC,
+                          hipStream_t s);
 void launch_face_bc(const GmStepDesc* d, int fine, int coarse, const gm_grid& gf, const float* A,
                     const float* B, float* fa, float* fb, hipStream_t s);
 void launch_interp_grid(const GmStepDesc* d, int src_level, const gm_grid* src_host,

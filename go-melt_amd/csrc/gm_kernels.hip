@@ -13,6 +13,8 @@
 // loaded plane node, and applies the 27-point stencil the box element's 8x8
 // matrices reduce to, summed by neighbour class.  No atomics, deterministic.
 #include <math.h>
+#include <algorithm>
+#include <vector>
 
 #include "gm_internal.h"
 
@@ -155,6 +157,19 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// Workgroup barrier for LDS hand-offs only.  __syncthreads() also waits for
+// every outstanding global load and store of the wave (vmcnt(0)), which in a
+// plane march puts the store round trip of each plane on the critical path.
+// Here a wave waits only for its own LDS operations (lgkmcnt(0)) before
+// s_barrier; the empty asm statements keep the compiler from moving memory
+// accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt and expcnt at their maxima, lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // Tiles of LX x LY node columns; ZCT output planes per block (template).
 #define LX 32
 #define LY 8
@@ -186,15 +201,82 @@ struct PropC {
     int kn, cn;
     float kT0, kiw, cT0, ciw;
     int knb, cnb;
+    int mn, mnb;  // merged table (mn > 0: both tables on one knot list)
+    float mT0, miw;
 };
+
+__device__ __forceinline__ PropC load_pc(const DevMat* __restrict__ m) {
+    PropC pc;
+    pc.Tliq = m->T_liq;
+    pc.Tsol = m->T_sol;
+    pc.kp = m->k_p;
+    pc.kf = m->k_f;
+    pc.ka0 = m->k_a0;
+    pc.ka1 = m->k_a1;
+    pc.cpf = m->cp_f;
+    pc.cpm = m->cp_m;
+    pc.ca0 = m->cp_a0;
+    pc.ca1 = m->cp_a1;
+    pc.kn = m->k_n;
+    pc.cn = m->cp_n;
+    pc.kT0 = m->lut_T0[0];
+    pc.kiw = m->lut_iw[0];
+    pc.cT0 = m->lut_T0[1];
+    pc.ciw = m->lut_iw[1];
+    pc.knb = m->lut_nb[0];
+    pc.cnb = m->lut_nb[1];
+    pc.mn = m->mn;
+    pc.mnb = m->mnb;
+    pc.mT0 = m->mT0;
+    pc.miw = m->miw;
+    return pc;
+}
+
+// Property tables into LDS (2 * LUTW floats): the merged table when both
+// tables are given, else the two separate ones.
+__device__ __forceinline__ void stage_tables(const DevMat* __restrict__ m, float* sTab, int tid,
+                                             int nthr) {
+    if (m->mn > 0) {
+        const int n = GM_LUT_BINS + 5 * m->mn;
+        for (int l = tid; l < n; l += nthr)
+            sTab[l] = l < GM_LUT_BINS ? __int_as_float(m->mlut[l])
+                                      : m->mseg[(l - GM_LUT_BINS) / 5][(l - GM_LUT_BINS) % 5];
+        return;
+    }
+    for (int l = tid; l < 2 * LUTW; l += nthr) {
+        const int w = l / LUTW, r = l % LUTW;
+        float v;
+        if (r < GM_LUT_BINS) {
+            v = __int_as_float(m->lut[w][r]);
+        } else {
+            const int j = (r - GM_LUT_BINS) / 3, f = (r - GM_LUT_BINS) % 3;
+            const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
+                                                                      : (w ? m->cs : m->ks);
+            v = src[j];
+        }
+        sTab[l] = v;
+    }
+}
 
 __device__ __forceinline__ void props_c(const PropC& c, const float* __restrict__ tab, float T,
                                         bool s1in, bool substrate, float& kk, float& cp, bool& s1) {
     const bool s2 = T >= c.Tliq;
     const bool s3 = (T > c.Tsol) && (T < c.Tliq);
     s1 = s1in || s2 || substrate;
-    const float kb = c.kn ? tab_lds(tab, c.kT0, c.kiw, c.knb, c.kn, T) : (c.ka1 * T + c.ka0);
-    const float cs = c.cn ? tab_lds(tab + LUTW, c.cT0, c.ciw, c.cnb, c.cn, T) : (c.ca1 * T + c.ca0);
+    float kb, cs;
+    if (c.mn > 0) {
+        int b = (int)((T - c.mT0) * c.miw);
+        b = b < 0 ? 0 : (b > c.mnb - 1 ? c.mnb - 1 : b);
+        int j = __float_as_int(tab[b]);
+        const float* sg = tab + GM_LUT_BINS;
+        while (j + 1 < c.mn - 1 && T >= sg[5 * (j + 1)]) ++j;
+        const float dT = T - sg[5 * j];
+        kb = sg[5 * j + 1] + sg[5 * j + 2] * dT;
+        cs = sg[5 * j + 3] + sg[5 * j + 4] * dT;
+    } else {
+        kb = c.kn ? tab_lds(tab, c.kT0, c.kiw, c.knb, c.kn, T) : (c.ka1 * T + c.ka0);
+        cs = c.cn ? tab_lds(tab + LUTW, c.cT0, c.ciw, c.cnb, c.cn, T) : (c.ca1 * T + c.ca0);
+    }
     kk = s2 ? c.kf : (s1 ? kb : c.kp);
     cp = s2 ? c.cpf : (s3 ? c.cpm : cs);
 }
@@ -226,13 +308,7 @@ __global__ void __launch_bounds__(256) k_level_step(LSArgs a, const GmStepDesc* 
     const bool col_ok = ix < nx && iy < ny;
 
     PropC pc;
-    if (!KIN) {
-        pc.Tliq = m->T_liq; pc.Tsol = m->T_sol; pc.kp = m->k_p; pc.kf = m->k_f;
-        pc.ka0 = m->k_a0; pc.ka1 = m->k_a1; pc.cpf = m->cp_f; pc.cpm = m->cp_m;
-        pc.ca0 = m->cp_a0; pc.ca1 = m->cp_a1; pc.kn = m->k_n; pc.cn = m->cp_n;
-        pc.kT0 = m->lut_T0[0]; pc.kiw = m->lut_iw[0]; pc.cT0 = m->lut_T0[1]; pc.ciw = m->lut_iw[1];
-        pc.knb = m->lut_nb[0]; pc.cnb = m->lut_nb[1];
-    }
+    if (!KIN) pc = load_pc(m);
     const float Tamb = m->T_amb;
 
     // ---- prologue: every global load of the block is issued here -----------
@@ -328,19 +404,7 @@ __global__ void __launch_bounds__(256) k_level_step(LSArgs a, const GmStepDesc* 
 
     // per-block constants: table lookups and the separable source
     if (!KIN) {
-        for (int l = tid; l < 2 * LUTW; l += 256) {
-            const int w = l / LUTW, r = l % LUTW;
-            float v;
-            if (r < GM_LUT_BINS) {
-                v = __int_as_float(m->lut[w][r]);
-            } else {
-                const int j = (r - GM_LUT_BINS) / 3, f = (r - GM_LUT_BINS) % 3;
-                const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
-                                                                          : (w ? m->cs : m->ks);
-                v = src[j];
-            }
-            sTab[l] = v;
-        }
+        stage_tables(m, sTab, tid, 256);
     }
     if (SRC) {
         float v0, v1, v2, P;
@@ -380,14 +444,14 @@ __global__ void __launch_bounds__(256) k_level_step(LSArgs a, const GmStepDesc* 
     }
     const int hc = (tx + 1) + PX * (ty + 1);
     const int q0 = tx + QX * ty;
-    __syncthreads();
+    lds_barrier();
 
     // ---- march through the planes ------------------------------------------
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl) {
         const int p = oz - 1 + pl;
         const bool pin = p >= 0 && p < nz;
-        if (pl > 0) __syncthreads();
+        if (pl > 0) lds_barrier();
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             const int l = tid + 256 * e;
@@ -412,7 +476,7 @@ __global__ void __launch_bounds__(256) k_level_step(LSArgs a, const GmStepDesc* 
                 sCn[l] = cv;
             }
         }
-        __syncthreads();
+        lds_barrier();
         const bool ezok = p - 1 >= 0 && p - 1 < nez_act;
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
@@ -427,7 +491,7 @@ __global__ void __launch_bounds__(256) k_level_step(LSArgs a, const GmStepDesc* 
                 sSc[pl & 1][l] = Sc;
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (pl < 2) continue;
         const int z = pl - 2;
         const int zo = oz + z;
@@ -663,28 +727,11 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
     else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
     const float mfac = a.mrow / dt;
 
-    PropC pc;
-    pc.Tliq = m->T_liq; pc.Tsol = m->T_sol; pc.kp = m->k_p; pc.kf = m->k_f;
-    pc.ka0 = m->k_a0; pc.ka1 = m->k_a1; pc.cpf = m->cp_f; pc.cpm = m->cp_m;
-    pc.ca0 = m->cp_a0; pc.ca1 = m->cp_a1; pc.kn = m->k_n; pc.cn = m->cp_n;
-    pc.kT0 = m->lut_T0[0]; pc.kiw = m->lut_iw[0]; pc.cT0 = m->lut_T0[1]; pc.ciw = m->lut_iw[1];
-    pc.knb = m->lut_nb[0]; pc.cnb = m->lut_nb[1];
+    PropC pc = load_pc(m);
     const float Tamb = m->T_amb;
-    for (int l = tid; l < 2 * LUTW; l += 256) {
-        const int w = l / LUTW, r = l % LUTW;
-        float v;
-        if (r < GM_LUT_BINS) {
-            v = __int_as_float(m->lut[w][r]);
-        } else {
-            const int j = (r - GM_LUT_BINS) / 3, f = (r - GM_LUT_BINS) % 3;
-            const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
-                                                                      : (w ? m->cs : m->ks);
-            v = src[j];
-        }
-        sTab[l] = v;
-    }
+    stage_tables(m, sTab, tid, 256);
     if (SRC) {
-        const float* row = d->rows[a.src_row];
+        const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
         if (tid < LX) {
             const int i = ox + tid;
             sSrc[tid] = i < nx ? src_axis(g, 0, i, row[0], m->rsq, m->rc) * (m->pc6 * row[6])
@@ -715,7 +762,7 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl) {
         const int p = oz - 1 + pl;
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             const int l = tid + 256 * e;
@@ -725,7 +772,7 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
                 sCn[l] = rC[pl][e];
             }
         }
-        __syncthreads();
+        lds_barrier();
         const bool ezok = p >= 1 && p - 1 < nez;
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
@@ -740,7 +787,7 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
                 sSc[pl & 1][l] = Sc;
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (pl < 2) continue;
         const int z = pl - 2;
         const int zo = oz + z;
@@ -762,7 +809,7 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
                     }
                 }
             }
-            __syncthreads();
+            lds_barrier();
         }
         if (zo >= nz || !col_ok) continue;
         const int idx = cidx + nxy * zo;
@@ -841,17 +888,817 @@ __global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fine-level step without LDS or barriers.  One wave owns a brick of 62
+// x-columns, YB rows and ZC planes: lane l holds x = 62 s - 1 + l (lanes 0 and
+// 63 only supply the x halo), every y and z neighbour of its column is in
+// registers, and x neighbours come from the adjacent lanes through DPP wave
+// shifts.  Same update as k_step_kc (carried k, cp; time-blended Dirichlet
+// faces; top-surface flux; properties of T+).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float lane_prev(float v) {  // value of lane l-1 (0 in lane 0)
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_next(float v) {  // value of lane l+1 (0 in lane 63)
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+#define DXS 62  // output columns per wave
+
+template <bool SRC, bool ACC, int YB, int ZC>
+__global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __restrict__ d,
+                                                 const DevMat* __restrict__ m) {
+    const gm_grid g = d->g[a.level];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int nxy = nx * ny;
+    const int nn = nxy * nz;
+    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
+    const int nst = (nx + DXS - 1) / DXS, nzc = (nz + ZC - 1) / ZC;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int zc = t % nzc, rest = t / nzc;
+    const int st = rest % nst, yb = rest / nst;
+    const int lane = threadIdx.x;
+    const int x = DXS * st - 1 + lane;
+    const int y0 = yb * YB, z0 = zc * ZC;
+    const bool xin = x >= 0 && x < nx;
+    const bool ein = x >= 0 && x < nex;  // element with min corner x exists in x
+    const bool olane = lane >= 1 && lane <= DXS && xin;
+    __shared__ float sTab[2 * LUTW];
+    stage_tables(m, sTab, lane, 64);
+
+    // ---- loads: planes z0-1 .. z0+ZC, rows y0-1 .. y0+YB ---------------------
+    float Tw[ZC + 2][YB + 2], Kw[ZC + 2][YB + 2], Cw[ZC + 2][YB + 2];
+#pragma unroll
+    for (int p = 0; p < ZC + 2; ++p) {
+        const int zz = z0 - 1 + p;
+#pragma unroll
+        for (int r = 0; r < YB + 2; ++r) {
+            const int yy = y0 - 1 + r;
+            Tw[p][r] = Kw[p][r] = Cw[p][r] = 0.0f;
+            if (xin && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+                const int idx = x + nx * yy + nxy * zz;
+                Tw[p][r] = a.T[(unsigned)idx];
+                Kw[p][r] = a.kcin[(unsigned)idx];
+                Cw[p][r] = a.kcin[(unsigned)(nn + idx)];
+            }
+        }
+    }
+    // per-node inputs of the outputs
+    float rFa[ZC][YB], rFb[ZC][YB], rAcc[ZC][YB], rMax[ZC][YB];
+    bool rS1[ZC][YB], rS2[ZC][YB];
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) {
+#pragma unroll
+        for (int j = 0; j < YB; ++j) {
+            const int yy = y0 + j, zz = z0 + i;
+            const bool ok = olane && yy < ny && zz < nz;
+            const int idx = x + nx * yy + nxy * zz;
+            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
+            rFa[i][j] = rFb[i][j] = rAcc[i][j] = rMax[i][j] = 0.0f;
+            rS1[i][j] = rS2[i][j] = false;
+            if (ok) {
+                if (face) {
+                    rFa[i][j] = a.bcA[(unsigned)idx];
+                    rFb[i][j] = a.bcB[(unsigned)idx];
+                }
+                rS1[i][j] = ((const uint8_t*)a.S1)[(unsigned)idx] != 0;
+                if (ACC) {
+                    rS2[i][j] = a.S2prev[(unsigned)idx] != 0;
+                    const int lx0 = d->l0_lo3[0] + x, ly0 = d->l0_lo3[1] + yy,
+                              lz0 = d->l0_lo3[2] + zz;
+                    if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] &&
+                        lz0 >= 0 && lz0 < d->l0_n[2]) {
+                        const int64_t l0 =
+                            (int64_t)lx0 + (int64_t)d->l0_n[0] * (ly0 + (int64_t)d->l0_n[1] * lz0);
+                        rAcc[i][j] = a.accum[l0];
+                        rMax[i][j] = a.max_accum[l0];
+                    }
+                }
+            }
+        }
+    }
+    const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
+    float dt = a.dt;
+    if (a.dt_mode == GM_DT_ROW) dt = row[5];
+    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
+    const float mfac = a.mrow / dt;
+    // separable source: x factor per lane; the y and z factors of the brick
+    // are computed one per lane and broadcast
+    float sx = 0.0f, syz = 0.0f;
+    if (SRC) {
+        sx = x >= 0 && x < nx ? src_axis(g, 0, x, row[0], m->rsq, m->rc) * (m->pc6 * row[6]) : 0.0f;
+        if (lane < YB) {
+            const int yy = y0 + lane;
+            syz = yy < ny ? src_axis(g, 1, yy, row[1], m->rsq, m->rc) : 0.0f;
+        } else if (lane < YB + ZC) {
+            const int zz = z0 + lane - YB;
+            syz = zz < nz ? src_axis(g, 2, zz, row[2], m->dsq, m->dc) : 0.0f;
+        }
+    }
+    PropC pc = load_pc(m);
+    const float Tamb = m->T_amb;
+
+    // ---- x-y quad sums of k and cp, then element means -----------------------
+    // Q[p][r]: sum over the 4 nodes of the x-y quad with min corner (x, y0-1+r)
+    float Qk[ZC + 2][YB + 1], Qc[ZC + 2][YB + 1];
+#pragma unroll
+    for (int p = 0; p < ZC + 2; ++p) {
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) {
+            const int yy = y0 - 1 + r;
+            const bool ok = ein && yy >= 0 && yy < ney;
+            const float ak = Kw[p][r] + Kw[p][r + 1];
+            const float ac = Cw[p][r] + Cw[p][r + 1];
+            const float qk = ak + lane_next(ak);
+            const float qc = ac + lane_next(ac);
+            Qk[p][r] = ok ? qk : 0.0f;
+            Qc[p][r] = ok ? qc : 0.0f;
+        }
+    }
+    // element layer between planes p and p+1 (min corner z = z0-1+p)
+    float Ek[ZC + 1][YB + 1], Ec[ZC + 1][YB + 1];
+#pragma unroll
+    for (int p = 0; p < ZC + 1; ++p) {
+        const int ez = z0 - 1 + p;
+        const float f = (ez >= 0 && ez < nez) ? 0.125f : 0.0f;
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) {
+            Ek[p][r] = (Qk[p][r] + Qk[p + 1][r]) * f;
+            Ec[p][r] = (Qc[p][r] + Qc[p + 1][r]) * f;
+        }
+    }
+    // top surface flux per element face (min corner x, row y0-1+r), when the
+    // top plane lies in this brick
+    float Fq[YB + 1][4];
+    const int ptop = nez - (z0 - 1);  // window plane index of the top plane
+    const bool has_top = ptop >= 1 && ptop <= ZC;
+    if (has_top) {
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) {
+            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+#pragma unroll
+            for (int p = 1; p <= ZC; ++p)
+                if (p == ptop) {
+                    t0 = Tw[p][r];
+                    t3 = Tw[p][r + 1];
+                }
+            t1 = lane_next(t0);
+            t2 = lane_next(t3);
+            const int yy = y0 - 1 + r;
+            const bool ok = ein && yy >= 0 && yy < ney;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float Tq = n2(q, 0) * t0 + n2(q, 1) * t1 + n2(q, 2) * t2 + n2(q, 3) * t3;
+                Fq[r][q] = ok ? face_flux(*m, Tq) : 0.0f;
+            }
+        }
+    }
+
+    lds_barrier();  // property tables staged
+
+    // ---- outputs --------------------------------------------------------------
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) {
+        const int zz = z0 + i;
+        float szv = 0.0f;
+        if (SRC) szv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), YB + i));
+#pragma unroll
+        for (int j = 0; j < YB; ++j) {
+            const int yy = y0 + j;
+            // elements around the node: bx (x side), by (y side), bz (z side)
+            const float k100 = Ek[i][j], k110 = Ek[i][j + 1];
+            const float k101 = Ek[i + 1][j], k111 = Ek[i + 1][j + 1];
+            const float k000 = lane_prev(k100), k010 = lane_prev(k110);
+            const float k001 = lane_prev(k101), k011 = lane_prev(k111);
+            const float cs = (Ec[i][j] + Ec[i][j + 1]) + (Ec[i + 1][j] + Ec[i + 1][j + 1]);
+            const float Msum = cs + lane_prev(cs);
+            // T of the 27 neighbours: planes i, i+1, i+2; rows j, j+1, j+2
+            const float Ti = Tw[i + 1][j + 1];
+            const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011,
+                        Pz11 = k110 + k111;
+            const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011,
+                        Py11 = k101 + k111;
+            const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101,
+                        Px11 = k011 + k111;
+            const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
+            const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
+            const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
+            float KT = a.kc[0] * ((Fx0 + Fx1) * Ti);
+            KT += a.kc[1] * (Fx0 * lane_prev(Ti) + Fx1 * lane_next(Ti));
+            KT += a.kc[2] * (Fy0 * Tw[i + 1][j] + Fy1 * Tw[i + 1][j + 2]);
+            KT += a.kc[4] * (Fz0 * Tw[i][j + 1] + Fz1 * Tw[i + 2][j + 1]);
+            KT += a.kc[3] * (Pz00 * lane_prev(Tw[i + 1][j]) + Pz10 * lane_next(Tw[i + 1][j]) +
+                             Pz01 * lane_prev(Tw[i + 1][j + 2]) + Pz11 * lane_next(Tw[i + 1][j + 2]));
+            KT += a.kc[5] * (Py00 * lane_prev(Tw[i][j + 1]) + Py10 * lane_next(Tw[i][j + 1]) +
+                             Py01 * lane_prev(Tw[i + 2][j + 1]) + Py11 * lane_next(Tw[i + 2][j + 1]));
+            KT += a.kc[6] * (Px00 * Tw[i][j] + Px10 * Tw[i][j + 2] + Px01 * Tw[i + 2][j] +
+                             Px11 * Tw[i + 2][j + 2]);
+            KT += a.kc[7] * (k000 * lane_prev(Tw[i][j]) + k100 * lane_next(Tw[i][j]) +
+                             k010 * lane_prev(Tw[i][j + 2]) + k110 * lane_next(Tw[i][j + 2]) +
+                             k001 * lane_prev(Tw[i + 2][j]) + k101 * lane_next(Tw[i + 2][j]) +
+                             k011 * lane_prev(Tw[i + 2][j + 2]) + k111 * lane_next(Tw[i + 2][j + 2]));
+            const float M = Msum * mfac;
+            float F = 0.0f;
+            if (SRC) F = sx * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), j)) * szv * a.w3;
+            if (has_top && i + 1 == ptop) {
+                // elements (x-1 | x, y-1 | y): node's local index ai in each face
+                float cr = 0.0f;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int bxe = e & 1, bye = e >> 1;  // 0: element on the negative side
+                    const int ai = bye ? (bxe ? 0 : 1) : (bxe ? 3 : 2);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float v = Fq[j + bye][q];
+                        const float vv = bxe ? v : lane_prev(v);
+                        cr += n2(q, ai) * (vv * a.w2);
+                    }
+                }
+                F += cr;
+            }
+            const bool ok = olane && yy < ny && zz < nz;
+            float corr = 0.0f;
+            if (a.corr && ok) {
+                const int bx = x - a.corr_info[0][0], by = yy - a.corr_info[1][0],
+                          bz = zz - a.corr_info[2][0];
+                if (bx >= 0 && bx <= a.corr_info[0][1] && by >= 0 && by <= a.corr_info[1][1] &&
+                    bz >= 0 && bz <= a.corr_info[2][1])
+                    corr = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
+            }
+            float Tnew = ((M * Ti - KT) + F + corr) / M;
+            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
+            if (face) Tnew = a.alpha * rFa[i][j] + (1.0f - a.alpha) * rFb[i][j];
+            if (a.clamp && Tnew < Tamb) Tnew = Tamb;
+            const int idx = x + nx * yy + nxy * zz;
+            const bool s1 = rS1[i][j] || Ti >= pc.Tliq || idx < a.n_sub;
+            if (ok) {
+                a.Tout[(unsigned)idx] = Tnew;
+                a.S1out[(unsigned)idx] = (uint8_t)s1;
+                float kn, cn;
+                bool s1n;
+                props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
+                a.kcout[(unsigned)idx] = kn;
+                a.kcout[(unsigned)(nn + idx)] = cn;
+                if (ACC) {
+                    const bool s2n = Ti >= pc.Tliq;
+                    const bool reset = (!rS2[i][j]) && s2n;
+                    const int lx0 = d->l0_lo3[0] + x, ly0 = d->l0_lo3[1] + yy,
+                              lz0 = d->l0_lo3[2] + zz;
+                    if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] &&
+                        lz0 >= 0 && lz0 < d->l0_n[2]) {
+                        const int64_t l0 =
+                            (int64_t)lx0 + (int64_t)d->l0_n[0] * (ly0 + (int64_t)d->l0_n[1] * lz0);
+                        const float rr = reset ? rAcc[i][j] : 0.0f;
+                        a.max_accum[l0] = rr > rMax[i][j] ? rr : rMax[i][j];
+                        a.accum[l0] = rAcc[i][j] + row[5] * (s2n ? 1.0f : 0.0f) - rr;
+                    }
+                    a.S2out[(unsigned)idx] = (uint8_t)s2n;
+                }
+            }
+        }
+    }
+}
+
+// Lean variant of k_step_dpp.  Global accesses go through buffer resources
+// with 32-bit per-lane byte offsets (no 64-bit address arithmetic).  Every
+// offset stays inside its buffer: lanes, rows and planes outside the mesh
+// read offset 0 and discard the value, and stores are predicated.  Every x
+// shift of a value is computed once and shared by the nodes.
+#define GM_BUF_DW3 0x00020000  // buffer resource word 3 (gfx9 family: raw dword access)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                             GM_BUF_DW3);
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, 0, 0);
+}
+
+template <bool SRC, bool ACC, int YB, int ZC>
+__global__ void __launch_bounds__(64) k_step_lean(LSArgs a, const GmStepDesc* __restrict__ d,
+                                                  const DevMat* __restrict__ m) {
+    const gm_grid g = d->g[a.level];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int nxy = nx * ny;
+    const int nn = nxy * nz;
+    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
+    const int nst = (nx + DXS - 1) / DXS, nzc = (nz + ZC - 1) / ZC;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int zc = t % nzc, rest = t / nzc;
+    const int st = rest % nst, yb = rest / nst;
+    const int lane = threadIdx.x;
+    const int x = DXS * st - 1 + lane;
+    const int y0 = yb * YB, z0 = zc * ZC;
+    const bool xin = x >= 0 && x < nx;
+    const bool ein = x >= 0 && x < nex;
+    const bool olane = lane >= 1 && lane <= DXS && xin;
+    const uint32_t nb4 = (uint32_t)nn * 4u;
+    const auto rT = buf_rsrc(a.T, nb4);
+    const auto rK = buf_rsrc(a.kcin, nb4);
+    const auto rC = buf_rsrc(a.kcin + nn, nb4);
+    // element index of the lane's column at (y0, z0)
+    const int col = x + nx * y0 + nxy * z0;
+
+    __shared__ float sTab[2 * LUTW];
+    stage_tables(m, sTab, lane, 64);
+
+    // ---- node window: planes z0-1 .. z0+ZC, rows y0-1 .. y0+YB ---------------
+    float Tw[ZC + 2][YB + 2], Kw[ZC + 2][YB + 2], Cw[ZC + 2][YB + 2];
+#pragma unroll
+    for (int p = 0; p < ZC + 2; ++p) {
+        const int zz = z0 - 1 + p;
+#pragma unroll
+        for (int r = 0; r < YB + 2; ++r) {
+            const int yy = y0 - 1 + r;
+            const bool in = xin && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+            const uint32_t vo = in ? 4u * (uint32_t)(col + nx * (r - 1) + nxy * (p - 1)) : 0u;
+            const float tv = buf_ld(rT, vo), kv = buf_ld(rK, vo), cv = buf_ld(rC, vo);
+            Tw[p][r] = in ? tv : 0.0f;
+            Kw[p][r] = in ? kv : 0.0f;
+            Cw[p][r] = in ? cv : 0.0f;
+        }
+    }
+    const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
+    float dt = a.dt;
+    if (a.dt_mode == GM_DT_ROW) dt = row[5];
+    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
+    const float mfac = a.mrow / dt;
+    float sx = 0.0f, syz = 0.0f;
+    if (SRC) {
+        sx = xin ? src_axis(g, 0, x, row[0], m->rsq, m->rc) * (m->pc6 * row[6] * a.w3) : 0.0f;
+        if (lane < YB) {
+            const int yy = y0 + lane;
+            syz = yy < ny ? src_axis(g, 1, yy, row[1], m->rsq, m->rc) : 0.0f;
+        } else if (lane < YB + ZC) {
+            const int zz = z0 + lane - YB;
+            syz = zz < nz ? src_axis(g, 2, zz, row[2], m->dsq, m->dc) : 0.0f;
+        }
+    }
+    const PropC pc = load_pc(m);
+    const float Tamb = m->T_amb;
+
+    // ---- quad sums and element means ------------------------------------------
+    float Ek[ZC + 1][YB + 1], Ec[ZC + 1][YB + 1];
+    {
+        float Qk[ZC + 2][YB + 1], Qc[ZC + 2][YB + 1];
+#pragma unroll
+        for (int p = 0; p < ZC + 2; ++p) {
+#pragma unroll
+            for (int r = 0; r < YB + 1; ++r) {
+                const int yy = y0 - 1 + r;
+                const bool ok = ein && yy >= 0 && yy < ney;
+                const float ak = Kw[p][r] + Kw[p][r + 1];
+                const float ac = Cw[p][r] + Cw[p][r + 1];
+                const float qk = ak + lane_next(ak);
+                const float qc = ac + lane_next(ac);
+                Qk[p][r] = ok ? qk : 0.0f;
+                Qc[p][r] = ok ? qc : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < ZC + 1; ++p) {
+            const int ez = z0 - 1 + p;
+            const float f = (ez >= 0 && ez < nez) ? 0.125f : 0.0f;
+#pragma unroll
+            for (int r = 0; r < YB + 1; ++r) {
+                Ek[p][r] = (Qk[p][r] + Qk[p + 1][r]) * f;
+                Ec[p][r] = (Qc[p][r] + Qc[p + 1][r]) * f;
+            }
+        }
+    }
+    // x-shifted copies, one per value
+    float Ekm[ZC + 1][YB + 1], Csum[ZC][YB];
+#pragma unroll
+    for (int p = 0; p < ZC + 1; ++p)
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) Ekm[p][r] = lane_prev(Ek[p][r]);
+#pragma unroll
+    for (int i = 0; i < ZC; ++i)
+#pragma unroll
+        for (int j = 0; j < YB; ++j) {
+            const float cs = (Ec[i][j] + Ec[i][j + 1]) + (Ec[i + 1][j] + Ec[i + 1][j + 1]);
+            Csum[i][j] = cs + lane_prev(cs);
+        }
+    float Tm[ZC + 2][YB + 2], Tq[ZC + 2][YB + 2];  // T of lanes l-1 and l+1
+#pragma unroll
+    for (int p = 0; p < ZC + 2; ++p)
+#pragma unroll
+        for (int r = 0; r < YB + 2; ++r) {
+            Tm[p][r] = lane_prev(Tw[p][r]);
+            Tq[p][r] = lane_next(Tw[p][r]);
+        }
+    // top surface flux per element face, when the top plane is in the brick
+    const int ptop = nez - (z0 - 1);
+    const bool has_top = ptop >= 1 && ptop <= ZC;
+    float Fq[YB + 1][4], Fqm[YB + 1][4];
+    if (has_top) {
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) {
+            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+#pragma unroll
+            for (int p = 1; p <= ZC; ++p)
+                if (p == ptop) {
+                    t0 = Tw[p][r];
+                    t1 = Tq[p][r];
+                    t2 = Tq[p][r + 1];
+                    t3 = Tw[p][r + 1];
+                }
+            const int yy = y0 - 1 + r;
+            const bool ok = ein && yy >= 0 && yy < ney;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float Tg = n2(q, 0) * t0 + n2(q, 1) * t1 + n2(q, 2) * t2 + n2(q, 3) * t3;
+                Fq[r][q] = ok ? face_flux(*m, Tg) * a.w2 : 0.0f;
+                Fqm[r][q] = lane_prev(Fq[r][q]);
+            }
+        }
+    }
+    lds_barrier();  // property tables staged
+
+    const auto rTo = buf_rsrc(a.Tout, nb4);
+    const auto rKo = buf_rsrc(a.kcout, nb4);
+    const auto rCo = buf_rsrc(a.kcout + nn, nb4);
+    const auto rS1 = buf_rsrc(a.S1, (uint32_t)nn);
+    const auto rS1o = buf_rsrc(a.S1out, (uint32_t)nn);
+    const auto rFa = buf_rsrc(a.bcA, nb4);
+    const auto rFb = buf_rsrc(a.bcB, nb4);
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) {
+        const int zz = z0 + i;
+        float szv = 0.0f;
+        if (SRC) szv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), YB + i));
+#pragma unroll
+        for (int j = 0; j < YB; ++j) {
+            const int yy = y0 + j;
+            const bool nin = yy < ny && zz < nz;
+            const int idx = x + nx * yy + nxy * zz;
+            const bool own = olane && nin;  // this lane writes node idx
+            const uint32_t ob = own ? (uint32_t)idx : 0u;        // byte arrays
+            const uint32_t of = own ? 4u * (uint32_t)idx : 0u;   // float arrays
+            const float k100 = Ek[i][j], k110 = Ek[i][j + 1];
+            const float k101 = Ek[i + 1][j], k111 = Ek[i + 1][j + 1];
+            const float k000 = Ekm[i][j], k010 = Ekm[i][j + 1];
+            const float k001 = Ekm[i + 1][j], k011 = Ekm[i + 1][j + 1];
+            const float Ti = Tw[i + 1][j + 1];
+            const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011,
+                        Pz11 = k110 + k111;
+            const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011,
+                        Py11 = k101 + k111;
+            const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101,
+                        Px11 = k011 + k111;
+            const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
+            const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
+            const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
+            float KT = a.kc[0] * ((Fx0 + Fx1) * Ti);
+            KT += a.kc[1] * (Fx0 * Tm[i + 1][j + 1] + Fx1 * Tq[i + 1][j + 1]);
+            KT += a.kc[2] * (Fy0 * Tw[i + 1][j] + Fy1 * Tw[i + 1][j + 2]);
+            KT += a.kc[4] * (Fz0 * Tw[i][j + 1] + Fz1 * Tw[i + 2][j + 1]);
+            KT += a.kc[3] * (Pz00 * Tm[i + 1][j] + Pz10 * Tq[i + 1][j] + Pz01 * Tm[i + 1][j + 2] +
+                             Pz11 * Tq[i + 1][j + 2]);
+            KT += a.kc[5] * (Py00 * Tm[i][j + 1] + Py10 * Tq[i][j + 1] + Py01 * Tm[i + 2][j + 1] +
+                             Py11 * Tq[i + 2][j + 1]);
+            KT += a.kc[6] * (Px00 * Tw[i][j] + Px10 * Tw[i][j + 2] + Px01 * Tw[i + 2][j] +
+                             Px11 * Tw[i + 2][j + 2]);
+            KT += a.kc[7] * (k000 * Tm[i][j] + k100 * Tq[i][j] + k010 * Tm[i][j + 2] +
+                             k110 * Tq[i][j + 2] + k001 * Tm[i + 2][j] + k101 * Tq[i + 2][j] +
+                             k011 * Tm[i + 2][j + 2] + k111 * Tq[i + 2][j + 2]);
+            const float M = Csum[i][j] * mfac;
+            float F = 0.0f;
+            if (SRC)
+                F = sx * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), j)) * szv;
+            if (has_top && i + 1 == ptop) {
+                // faces (x-1 | x, y-1 | y) around the node; node's local index
+                F += n2(0, 2) * Fqm[j][0] + n2(1, 2) * Fqm[j][1] + n2(2, 2) * Fqm[j][2] +
+                     n2(3, 2) * Fqm[j][3];
+                F += n2(0, 3) * Fq[j][0] + n2(1, 3) * Fq[j][1] + n2(2, 3) * Fq[j][2] +
+                     n2(3, 3) * Fq[j][3];
+                F += n2(0, 1) * Fqm[j + 1][0] + n2(1, 1) * Fqm[j + 1][1] +
+                     n2(2, 1) * Fqm[j + 1][2] + n2(3, 1) * Fqm[j + 1][3];
+                F += n2(0, 0) * Fq[j + 1][0] + n2(1, 0) * Fq[j + 1][1] + n2(2, 0) * Fq[j + 1][2] +
+                     n2(3, 0) * Fq[j + 1][3];
+            }
+            float corr = 0.0f;
+            if (a.corr && own) {
+                const int bx = x - a.corr_info[0][0], by = yy - a.corr_info[1][0],
+                          bz = zz - a.corr_info[2][0];
+                if (bx >= 0 && bx <= a.corr_info[0][1] && by >= 0 && by <= a.corr_info[1][1] &&
+                    bz >= 0 && bz <= a.corr_info[2][1])
+                    corr = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
+            }
+            float Tnew = ((M * Ti - KT) + F + corr) / M;
+            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
+            if (face) {
+                const float fa = buf_ld(rFa, of), fb = buf_ld(rFb, of);
+                Tnew = a.alpha * fa + (1.0f - a.alpha) * fb;
+            }
+            if (a.clamp && Tnew < Tamb) Tnew = Tamb;
+            const bool s1in = __builtin_amdgcn_raw_buffer_load_b8(rS1, ob, 0, 0) != 0;
+            const bool s1 = s1in || Ti >= pc.Tliq || idx < a.n_sub;
+            float kn, cn;
+            bool s1n;
+            props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
+            if (own) {
+                buf_st(Tnew, rTo, of);
+                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)s1, rS1o, ob, 0, 0);
+                buf_st(kn, rKo, of);
+                buf_st(cn, rCo, of);
+            }
+            if (ACC && own) {
+                const bool s2n = Ti >= pc.Tliq;
+                const bool s2p = a.S2prev[idx] != 0;
+                const bool reset = (!s2p) && s2n;
+                const int lx0 = d->l0_lo3[0] + x, ly0 = d->l0_lo3[1] + yy, lz0 = d->l0_lo3[2] + zz;
+                if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] && lz0 >= 0 &&
+                    lz0 < d->l0_n[2]) {
+                    const int64_t l0 =
+                        (int64_t)lx0 + (int64_t)d->l0_n[0] * (ly0 + (int64_t)d->l0_n[1] * lz0);
+                    const float racc = a.accum[l0], rmax = a.max_accum[l0];
+                    const float rr = reset ? racc : 0.0f;
+                    a.max_accum[l0] = rr > rmax ? rr : rmax;
+                    a.accum[l0] = racc + row[5] * (s2n ? 1.0f : 0.0f) - rr;
+                }
+                a.S2out[idx] = (uint8_t)s2n;
+            }
+        }
+    }
+}
+
+// Same step as a rolled z-march: one wave owns 62 x-columns, YB rows and ZL
+// consecutive planes and walks up the planes with the three node planes
+// z-1, z, z+1 rotating through registers (plane z+2 is prefetched while z is
+// computed).  The loop body is a few KB of code, so the instruction cache
+// serves every plane after the first; a fully unrolled brick would stream
+// tens of KB of instructions per wave.
+template <int YB>
+struct DppPlane {
+    float T[YB + 2], K[YB + 2], C[YB + 2];
+};
+
+template <int YB>
+__device__ __forceinline__ void dpp_load(DppPlane<YB>& P, const LSArgs& a, int x, bool xin,
+                                         int y0, int zz, int nx, int ny, int nz, int nxy,
+                                         int nn) {
+#pragma unroll
+    for (int r = 0; r < YB + 2; ++r) {
+        const int yy = y0 - 1 + r;
+        P.T[r] = P.K[r] = P.C[r] = 0.0f;
+        if (xin && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+            const int idx = x + nx * yy + nxy * zz;
+            P.T[r] = a.T[idx];
+            P.K[r] = a.kcin[idx];
+            P.C[r] = a.kcin[nn + idx];
+        }
+    }
+}
+
+template <int YB>
+__device__ __forceinline__ void dpp_quads(const DppPlane<YB>& P, bool ein, int y0, int ney,
+                                          float (&Qk)[YB + 1], float (&Qc)[YB + 1]) {
+#pragma unroll
+    for (int r = 0; r < YB + 1; ++r) {
+        const int yy = y0 - 1 + r;
+        const bool ok = ein && yy >= 0 && yy < ney;
+        const float ak = P.K[r] + P.K[r + 1];
+        const float ac = P.C[r] + P.C[r + 1];
+        const float qk = ak + lane_next(ak);
+        const float qc = ac + lane_next(ac);
+        Qk[r] = ok ? qk : 0.0f;
+        Qc[r] = ok ? qc : 0.0f;
+    }
+}
+
+template <bool SRC, bool ACC, int YB>
+__global__ void __launch_bounds__(64) k_step_roll(LSArgs a, const GmStepDesc* __restrict__ d,
+                                                  const DevMat* __restrict__ m, int ZL) {
+    const gm_grid g = d->g[a.level];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int nxy = nx * ny;
+    const int nn = nxy * nz;
+    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
+    const int nst = (nx + DXS - 1) / DXS, nzc = (nz + ZL - 1) / ZL;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int zc = t % nzc, rest = t / nzc;
+    const int st = rest % nst, yb = rest / nst;
+    const int lane = threadIdx.x;
+    const int x = DXS * st - 1 + lane;
+    const int y0 = yb * YB, z0 = zc * ZL;
+    const int z1 = min(z0 + ZL, nz);
+    const bool xin = x >= 0 && x < nx;
+    const bool ein = x >= 0 && x < nex;
+    const bool olane = lane >= 1 && lane <= DXS && xin;
+
+    __shared__ float sTab[2 * LUTW];
+    stage_tables(m, sTab, lane, 64);
+    const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
+    float dt = a.dt;
+    if (a.dt_mode == GM_DT_ROW) dt = row[5];
+    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
+    const float mfac = a.mrow / dt;
+    float sx = 0.0f, sy = 0.0f;
+    if (SRC) {
+        sx = xin ? src_axis(g, 0, x, row[0], m->rsq, m->rc) * (m->pc6 * row[6] * a.w3) : 0.0f;
+        // lane j < YB holds the y factor of row y0 + j
+        const int yy = y0 + (lane < YB ? lane : 0);
+        sy = yy < ny ? src_axis(g, 1, yy, row[1], m->rsq, m->rc) : 0.0f;
+    }
+    PropC pc = load_pc(m);
+    const float Tamb = m->T_amb;
+
+    DppPlane<YB> A, B, C, D;
+    dpp_load<YB>(A, a, x, xin, y0, z0 - 1, nx, ny, nz, nxy, nn);
+    dpp_load<YB>(B, a, x, xin, y0, z0, nx, ny, nz, nxy, nn);
+    dpp_load<YB>(C, a, x, xin, y0, z0 + 1, nx, ny, nz, nxy, nn);
+    float QkA[YB + 1], QcA[YB + 1], QkB[YB + 1], QcB[YB + 1], QkC[YB + 1], QcC[YB + 1];
+    dpp_quads<YB>(A, ein, y0, ney, QkA, QcA);
+    dpp_quads<YB>(B, ein, y0, ney, QkB, QcB);
+    // element layer below plane z0 (min corner z0 - 1)
+    float EkL[YB + 1], EcL[YB + 1], EkU[YB + 1], EcU[YB + 1];
+    {
+        const int ez = z0 - 1;
+        const float f = (ez >= 0 && ez < nez) ? 0.125f : 0.0f;
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) {
+            EkL[r] = (QkA[r] + QkB[r]) * f;
+            EcL[r] = (QcA[r] + QcB[r]) * f;
+        }
+    }
+    lds_barrier();  // property tables staged
+
+#pragma unroll 1
+    for (int zz = z0; zz < z1; ++zz) {
+        // prefetch plane zz + 2; per-node inputs of plane zz
+        dpp_load<YB>(D, a, x, xin, y0, zz + 2, nx, ny, nz, nxy, nn);
+        float rFa[YB], rFb[YB], rAcc[YB], rMax[YB];
+        bool rS1[YB], rS2[YB];
+        int64_t rl0[YB];
+#pragma unroll
+        for (int j = 0; j < YB; ++j) {
+            const int yy = y0 + j;
+            const bool ok = olane && yy < ny;
+            const int idx = x + nx * yy + nxy * zz;
+            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
+            rFa[j] = rFb[j] = rAcc[j] = rMax[j] = 0.0f;
+            rS1[j] = rS2[j] = false;
+            rl0[j] = -1;
+            if (ok) {
+                if (face) {
+                    rFa[j] = a.bcA[idx];
+                    rFb[j] = a.bcB[idx];
+                }
+                rS1[j] = ((const uint8_t*)a.S1)[idx] != 0;
+                if (ACC) {
+                    rS2[j] = a.S2prev[idx] != 0;
+                    const int lx0 = d->l0_lo3[0] + x, ly0 = d->l0_lo3[1] + yy,
+                              lz0 = d->l0_lo3[2] + zz;
+                    if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] &&
+                        lz0 >= 0 && lz0 < d->l0_n[2]) {
+                        rl0[j] = (int64_t)lx0 +
+                                 (int64_t)d->l0_n[0] * (ly0 + (int64_t)d->l0_n[1] * lz0);
+                        rAcc[j] = a.accum[rl0[j]];
+                        rMax[j] = a.max_accum[rl0[j]];
+                    }
+                }
+            }
+        }
+        // element layer above plane zz
+        dpp_quads<YB>(C, ein, y0, ney, QkC, QcC);
+        {
+            const float f = zz < nez ? 0.125f : 0.0f;
+#pragma unroll
+            for (int r = 0; r < YB + 1; ++r) {
+                EkU[r] = (QkB[r] + QkC[r]) * f;
+                EcU[r] = (QcB[r] + QcC[r]) * f;
+            }
+        }
+        float szv = 0.0f;
+        if (SRC) szv = src_axis(g, 2, zz, row[2], m->dsq, m->dc);
+        // top surface flux per element face (min corner x, row y0 - 1 + r)
+        float Fq[YB + 1][4];
+        const bool top = zz == nez;
+        if (top) {
+#pragma unroll
+            for (int r = 0; r < YB + 1; ++r) {
+                const float t0 = B.T[r], t3 = B.T[r + 1];
+                const float t1 = lane_next(t0), t2 = lane_next(t3);
+                const int yy = y0 - 1 + r;
+                const bool ok = ein && yy >= 0 && yy < ney;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float Tq = n2(q, 0) * t0 + n2(q, 1) * t1 + n2(q, 2) * t2 + n2(q, 3) * t3;
+                    Fq[r][q] = ok ? face_flux(*m, Tq) : 0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < YB; ++j) {
+            const int yy = y0 + j;
+            const float k100 = EkL[j], k110 = EkL[j + 1];
+            const float k101 = EkU[j], k111 = EkU[j + 1];
+            const float k000 = lane_prev(k100), k010 = lane_prev(k110);
+            const float k001 = lane_prev(k101), k011 = lane_prev(k111);
+            const float cs = (EcL[j] + EcL[j + 1]) + (EcU[j] + EcU[j + 1]);
+            const float Msum = cs + lane_prev(cs);
+            const float Ti = B.T[j + 1];
+            const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011,
+                        Pz11 = k110 + k111;
+            const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011,
+                        Py11 = k101 + k111;
+            const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101,
+                        Px11 = k011 + k111;
+            const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
+            const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
+            const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
+            float KT = a.kc[0] * ((Fx0 + Fx1) * Ti);
+            KT += a.kc[1] * (Fx0 * lane_prev(Ti) + Fx1 * lane_next(Ti));
+            KT += a.kc[2] * (Fy0 * B.T[j] + Fy1 * B.T[j + 2]);
+            KT += a.kc[4] * (Fz0 * A.T[j + 1] + Fz1 * C.T[j + 1]);
+            KT += a.kc[3] * (Pz00 * lane_prev(B.T[j]) + Pz10 * lane_next(B.T[j]) +
+                             Pz01 * lane_prev(B.T[j + 2]) + Pz11 * lane_next(B.T[j + 2]));
+            KT += a.kc[5] * (Py00 * lane_prev(A.T[j + 1]) + Py10 * lane_next(A.T[j + 1]) +
+                             Py01 * lane_prev(C.T[j + 1]) + Py11 * lane_next(C.T[j + 1]));
+            KT += a.kc[6] * (Px00 * A.T[j] + Px10 * A.T[j + 2] + Px01 * C.T[j] +
+                             Px11 * C.T[j + 2]);
+            KT += a.kc[7] * (k000 * lane_prev(A.T[j]) + k100 * lane_next(A.T[j]) +
+                             k010 * lane_prev(A.T[j + 2]) + k110 * lane_next(A.T[j + 2]) +
+                             k001 * lane_prev(C.T[j]) + k101 * lane_next(C.T[j]) +
+                             k011 * lane_prev(C.T[j + 2]) + k111 * lane_next(C.T[j + 2]));
+            const float M = Msum * mfac;
+            float F = 0.0f;
+            if (SRC)
+                F = sx * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sy), j)) * szv;
+            if (top) {
+                float cr = 0.0f;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int bxe = e & 1, bye = e >> 1;
+                    const int ai = bye ? (bxe ? 0 : 1) : (bxe ? 3 : 2);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float v = Fq[j + bye][q];
+                        const float vv = bxe ? v : lane_prev(v);
+                        cr += n2(q, ai) * (vv * a.w2);
+                    }
+                }
+                F += cr;
+            }
+            float Tnew = ((M * Ti - KT) + F) / M;
+            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
+            if (face) Tnew = a.alpha * rFa[j] + (1.0f - a.alpha) * rFb[j];
+            if (a.clamp && Tnew < Tamb) Tnew = Tamb;
+            const bool ok = olane && yy < ny;
+            const int idx = x + nx * yy + nxy * zz;
+            const bool s1 = rS1[j] || Ti >= pc.Tliq || idx < a.n_sub;
+            if (ok) {
+                a.Tout[idx] = Tnew;
+                a.S1out[idx] = (uint8_t)s1;
+                float kn, cn;
+                bool s1n;
+                props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
+                a.kcout[idx] = kn;
+                a.kcout[nn + idx] = cn;
+                if (ACC) {
+                    const bool s2n = Ti >= pc.Tliq;
+                    const bool reset = (!rS2[j]) && s2n;
+                    if (rl0[j] >= 0) {
+                        const float rr = reset ? rAcc[j] : 0.0f;
+                        a.max_accum[rl0[j]] = rr > rMax[j] ? rr : rMax[j];
+                        a.accum[rl0[j]] = rAcc[j] + row[5] * (s2n ? 1.0f : 0.0f) - rr;
+                    }
+                    a.S2out[idx] = (uint8_t)s2n;
+                }
+            }
+        }
+        // rotate
+        A = B;
+        B = C;
+        C = D;
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) {
+            QkB[r] = QkC[r];
+            QcB[r] = QcC[r];
+            EkL[r] = EkU[r];
+            EcL[r] = EcU[r];
+        }
+    }
+}
+
 void launch_level_step_kc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg,
                           const DevMat* m, hipStream_t s) {
-    const int ntx = (hg.n[0] + LX - 1) / LX, nty = (hg.n[1] + LY - 1) / LY;
-    const bool src = a.src_mode == GM_SRC_L3;
-    const int nb = ntx * nty * ((hg.n[2] + 3) / 4);
-    if (src && a.accum_on)
-        hipLaunchKernelGGL((k_step_kc<true, true, 4>), dim3(nb), dim3(256), 0, s, a, d, m);
-    else if (src)
-        hipLaunchKernelGGL((k_step_kc<true, false, 4>), dim3(nb), dim3(256), 0, s, a, d, m);
-    else
-        hipLaunchKernelGGL((k_step_kc<false, false, 4>), dim3(nb), dim3(256), 0, s, a, d, m);
+    if (!a.kcin || !a.kcout) return;  // no carried properties for these buffers
+    constexpr int YB = 2, ZC = 2;
+    const int nb = ((hg.n[0] + DXS - 1) / DXS) * ((hg.n[1] + YB - 1) / YB) *
+                   ((hg.n[2] + ZC - 1) / ZC);
+    if (a.src_mode == GM_SRC_L3) {
+        if (a.accum_on)
+            hipLaunchKernelGGL((k_step_dpp<true, true, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
+        else
+            hipLaunchKernelGGL((k_step_dpp<true, false, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
+    } else {
+        hipLaunchKernelGGL((k_step_dpp<false, false, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
+    }
 }
 
 // k, cp of (T, S1) as the next level step's computeStateProperties returns them
@@ -859,26 +1706,9 @@ __global__ void k_props_kc(const DevMat* __restrict__ m, const float* __restrict
                            const uint8_t* __restrict__ S1, int64_t nn, int64_t n_sub,
                            float* __restrict__ kc) {
     __shared__ float sTab[2 * LUTW];
-    for (int l = threadIdx.x; l < 2 * LUTW; l += blockDim.x) {
-        const int w = l / LUTW, r = l % LUTW;
-        float v;
-        if (r < GM_LUT_BINS) {
-            v = __int_as_float(m->lut[w][r]);
-        } else {
-            const int j = (r - GM_LUT_BINS) / 3, f = (r - GM_LUT_BINS) % 3;
-            const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
-                                                                      : (w ? m->cs : m->ks);
-            v = src[j];
-        }
-        sTab[l] = v;
-    }
+    stage_tables(m, sTab, threadIdx.x, blockDim.x);
     __syncthreads();
-    PropC pc;
-    pc.Tliq = m->T_liq; pc.Tsol = m->T_sol; pc.kp = m->k_p; pc.kf = m->k_f;
-    pc.ka0 = m->k_a0; pc.ka1 = m->k_a1; pc.cpf = m->cp_f; pc.cpm = m->cp_m;
-    pc.ca0 = m->cp_a0; pc.ca1 = m->cp_a1; pc.kn = m->k_n; pc.cn = m->cp_n;
-    pc.kT0 = m->lut_T0[0]; pc.kiw = m->lut_iw[0]; pc.cT0 = m->lut_T0[1]; pc.ciw = m->lut_iw[1];
-    pc.knb = m->lut_nb[0]; pc.cnb = m->lut_nb[1];
+    PropC pc = load_pc(m);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn;
          i += (int64_t)gridDim.x * blockDim.x) {
         float kk, cc;
@@ -1362,24 +2192,8 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
     const float idt = 1.0f / dt;
     PropC pc;
     if (!KCM) {
-        pc.Tliq = m->T_liq; pc.Tsol = m->T_sol; pc.kp = m->k_p; pc.kf = m->k_f;
-        pc.ka0 = m->k_a0; pc.ka1 = m->k_a1; pc.cpf = m->cp_f; pc.cpm = m->cp_m;
-        pc.ca0 = m->cp_a0; pc.ca1 = m->cp_a1; pc.kn = m->k_n; pc.cn = m->cp_n;
-        pc.kT0 = m->lut_T0[0]; pc.kiw = m->lut_iw[0]; pc.cT0 = m->lut_T0[1];
-        pc.ciw = m->lut_iw[1]; pc.knb = m->lut_nb[0]; pc.cnb = m->lut_nb[1];
-        for (int l = tid; l < 2 * LUTW; l += 256) {
-            const int w = l / LUTW, rr = l % LUTW;
-            float v;
-            if (rr < GM_LUT_BINS) {
-                v = __int_as_float(m->lut[w][rr]);
-            } else {
-                const int j = (rr - GM_LUT_BINS) / 3, f = (rr - GM_LUT_BINS) % 3;
-                const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
-                                                                          : (w ? m->cs : m->ks);
-                v = src[j];
-            }
-            sTab[l] = v;
-        }
+        pc = load_pc(m);
+        stage_tables(m, sTab, tid, 256);
     }
     int qh[2];
     bool qok[2], qxy[2];
@@ -1402,7 +2216,7 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl) {
         const int p = oz - 1 + pl;
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             const int l = tid + 256 * e;
@@ -1419,7 +2233,7 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
                 sCn[l] = cv;
             }
         }
-        __syncthreads();
+        lds_barrier();
         const bool ezok = p >= 1 && p - 1 < nez;
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
@@ -1434,7 +2248,7 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
                 sSc[pl & 1][l] = Sc;
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (pl < 2) continue;
         const int zo = oz + pl - 2;
         if (zo >= nz || !col_ok) continue;
@@ -1457,7 +2271,7 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
     }
 #pragma unroll
     for (int a = 0; a < NZA; ++a) sZ[a][tid] = zacc[a];
-    __syncthreads();
+    lds_barrier();
     // x: coarse node Ax gets sum over the tile's fine columns with weight 1 - |u - Ax R| / R
     const int AxB = (ox + off[0]) / R[0], AyB = (oy + off[1]) / R[1];
     const int xl = min(ox + LX, nx) - 1, yl = min(oy + LY, ny) - 1, zl = min(oz + RZ, nz) - 1;
@@ -1476,7 +2290,7 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
         }
         sX[az][yy][ax] = sum;
     }
-    __syncthreads();
+    lds_barrier();
     const int lo0 = pp.ax[0].info[0], lo1 = pp.ax[1].info[0], lo2 = pp.ax[2].info[0];
     const int nb0 = pp.ax[0].info[1] + 1, nb1 = pp.ax[1].info[1] + 1, nb2 = pp.ax[2].info[1] + 1;
     const int st0 = pp.cmax[0] + 1, st1 = pp.cmax[1] + 1;
@@ -1671,6 +2485,89 @@ void launch_tprime(const GmStepDesc* d, int fine_level, int coarse_level, const 
                    const float* coarseT, float* Tp, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_tprime, dim3((n + 255) / 256), dim3(256), 0, s, d, fine_level,
                        coarse_level, fineT, coarseT, Tp);
+}
+
+// getNewTprime in one launch when the coarse overlap nodes coincide with fine
+// nodes (node-aligned levels): the injected coarse values (cF:2086-2094) are
+// the fine values at those nodes, so every fine node reads them directly and
+// the prolongation of cF:2097 needs no second launch.  Threads past the fine
+// node count write the injected values into the coarse field.
+__global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level,
+                                int coarse_level, const float* __restrict__ fT,
+                                float* __restrict__ cT, float* __restrict__ Tp, int64_t nf,
+                                const DevMat* __restrict__ m, const uint8_t* __restrict__ cS1,
+                                int64_t c_nsub, float* __restrict__ cKC) {
+    const gm_grid& fg = d->g[fine_level];
+    const gm_grid& cg = d->g[coarse_level];
+    const GmBox& b = fine_level == 3 ? d->ov3 : d->ov2;
+    const gm_grid& xyz = fine_level == 3 ? d->ov3_xyz : d->ov2_xyz;
+    // fine node of box node 0 and the fine stride of one coarse spacing
+    int f0[3], fr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        f0[a] = (int)rintf((xyz.x0[a] - fg.x0[a]) / fg.h[a]);
+        fr[a] = (int)rintf(cg.h[a] / fg.h[a]);
+    }
+    // the coarse level's carried k, cp follow the injected temperatures
+    __shared__ float sTab[2 * LUTW];
+    if (cKC) {
+        stage_tables(m, sTab, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nf) {
+        const int ix = t % fg.n[0], iy = (t / fg.n[0]) % fg.n[1];
+        const int iz = t / ((int64_t)fg.n[0] * fg.n[1]);
+        float N[8];
+        int64_t nd[8];
+        interp8(cg, coordf(fg, 0, ix), coordf(fg, 1, iy), coordf(fg, 2, iz), N, nd);
+        float v = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int64_t c = nd[q];
+            const int cx = (int)(c % cg.n[0]), cy = (int)((c / cg.n[0]) % cg.n[1]);
+            const int cz = (int)(c / ((int64_t)cg.n[0] * cg.n[1]));
+            const int bx = cx - b.lo[0], by = cy - b.lo[1], bz = cz - b.lo[2];
+            const int jx = f0[0] + bx * fr[0], jy = f0[1] + by * fr[1], jz = f0[2] + bz * fr[2];
+            const bool inj = bx >= 0 && bx < b.n[0] && by >= 0 && by < b.n[1] && bz >= 0 &&
+                             bz < b.n[2] && jx >= 0 && jx < fg.n[0] && jy >= 0 && jy < fg.n[1] &&
+                             jz >= 0 && jz < fg.n[2];
+            const float u = inj ? fT[nid(jx, jy, jz, fg.n[0], fg.n[1])] : cT[c];
+            v += N[q] * u;
+        }
+        Tp[t] = fT[t] - v;
+        return;
+    }
+    const int64_t k = t - nf;
+    const int64_t nb = (int64_t)b.n[0] * b.n[1] * b.n[2];
+    if (k >= nb) return;
+    const int bx = (int)(k % b.n[0]), by = (int)((k / b.n[0]) % b.n[1]);
+    const int bz = (int)(k / ((int64_t)b.n[0] * b.n[1]));
+    const int cx = b.lo[0] + bx, cy = b.lo[1] + by, cz = b.lo[2] + bz;
+    if (cx < 0 || cx >= cg.n[0] || cy < 0 || cy >= cg.n[1] || cz < 0 || cz >= cg.n[2]) return;
+    const int jx = f0[0] + bx * fr[0], jy = f0[1] + by * fr[1], jz = f0[2] + bz * fr[2];
+    if (jx < 0 || jx >= fg.n[0] || jy < 0 || jy >= fg.n[1] || jz < 0 || jz >= fg.n[2]) return;
+    const int64_t ci = nid(cx, cy, cz, cg.n[0], cg.n[1]);
+    const float v = fT[nid(jx, jy, jz, fg.n[0], fg.n[1])];
+    cT[ci] = v;
+    if (cKC) {
+        const int64_t nnc = (int64_t)cg.n[0] * cg.n[1] * cg.n[2];
+        const PropC pc = load_pc(m);
+        float kk, cc;
+        bool s1;
+        props_c(pc, sTab, v, cS1[ci] != 0, ci < c_nsub, kk, cc, s1);
+        cKC[ci] = kk;
+        cKC[nnc + ci] = cc;
+    }
+}
+
+void launch_tprime_inject(const GmStepDesc* d, int fine_level, int coarse_level,
+                          const float* fineT, float* coarseT, float* Tp, int64_t nf, int64_t nbox,
+                          const DevMat* m, const uint8_t* cS1, int64_t c_nsub, float* cKC,
+                          hipStream_t s) {
+    const int64_t n = nf + nbox;
+    hipLaunchKernelGGL(k_tprime_inject, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d,
+                       fine_level, coarse_level, fineT, coarseT, Tp, nf, m, cS1, c_nsub, cKC);
 }
 
 // moveEverything, L3 part (cF:2439-2444): T0 = I(L1 T0) + (I(L2 T'0) + I(L3 T'0))
@@ -2130,6 +3027,50 @@ DevMat make_devmat(const gm_material& mm) {
                     if ((double)kT[i] <= Tb) j = i;
             }
             m.lut[w][b] = j;
+        }
+    }
+    // merged table
+    m.mn = 0;
+    if (mm.k_table_n >= 2 && mm.cp_table_n >= 2) {
+        std::vector<double> U;
+        for (int i = 0; i < mm.k_table_n; ++i) U.push_back(mm.k_table_T[i]);
+        for (int i = 0; i < mm.cp_table_n; ++i) U.push_back(mm.cp_table_T[i]);
+        std::sort(U.begin(), U.end());
+        U.erase(std::unique(U.begin(), U.end()), U.end());
+        // value and slope of a table on the segment that starts at u
+        auto seg = [](const float* kT, const float* kv, int n, double u, double& v, double& sl) {
+            int j = 0;
+            for (int i = 1; i < n - 1; ++i)
+                if ((double)kT[i] <= u) j = i;
+            sl = ((double)kv[j + 1] - kv[j]) / ((double)kT[j + 1] - kT[j]);
+            v = (double)kv[j] + sl * (u - kT[j]);
+        };
+        const int nU = (int)U.size();
+        m.mn = nU;
+        for (int j = 0; j < nU && j < 2 * GM_MAX_KNOTS; ++j) {
+            double kvj, ksj, cvj, csj;
+            seg(mm.k_table_T, mm.k_table_v, mm.k_table_n, U[j], kvj, ksj);
+            seg(mm.cp_table_T, mm.cp_table_v, mm.cp_table_n, U[j], cvj, csj);
+            m.mseg[j][0] = (float)U[j];
+            m.mseg[j][1] = (float)kvj;
+            m.mseg[j][2] = (float)ksj;
+            m.mseg[j][3] = (float)cvj;
+            m.mseg[j][4] = (float)csj;
+        }
+        double dmin = U[nU - 1] - U[0];
+        for (int i = 0; i + 1 < nU; ++i) dmin = std::min(dmin, U[i + 1] - U[i]);
+        int nb = (int)std::min<double>(GM_LUT_BINS, std::ceil(2.0 * (U[nU - 1] - U[0]) / dmin));
+        nb = std::max(nb, 1);
+        const double iw = nb / (U[nU - 1] - U[0]);
+        m.mT0 = (float)U[0];
+        m.miw = (float)iw;
+        m.mnb = nb;
+        for (int b = 0; b < GM_LUT_BINS; ++b) {
+            const double Tb = U[0] + (b - 0.5) / iw;
+            int j = 0;
+            for (int i = 1; i < nU - 1; ++i)
+                if (U[i] <= Tb) j = i;
+            m.mlut[b] = j;
         }
     }
     return m;

@@ -68,6 +68,20 @@ def test_driver_vs_oracle(oracle, tmp_path, case):
     assert np.array_equal(out["S1_3"][0], out["S1_3"][1])
 
 
+def test_driver_vs_oracle_unaligned_windows(oracle, tmp_path):
+    """L2 origin a quarter of an L1 spacing off the L1 nodes: the coarse
+    overlap nodes are not fine nodes, so the engine takes the general
+    (per-Gauss-point) restriction and two-pass T' path."""
+    inp, g = W.tiny(n=(8, 8, 4), n2=2, n3=2, track=1.2)
+    inp["Level2"]["bounds"]["x"] = [2.25, 6.25]
+    inp["Level3"]["bounds"]["x"] = [3.25, 5.25]
+    out, st = _run_both(oracle, inp, g, tmp_path)
+    assert st["subcycle"] >= 1
+    for k in (1, 2, 3):
+        a, b = out[k]
+        assert _relT(a, b) < 5e-4, (k, _relT(a, b))
+
+
 def test_driver_vs_fp32_oracle(oracle, tmp_path):
     inp, g = W.tiny(n=(8, 8, 4), n2=2, n3=2, track=1.2)
     out, _ = _run_both(oracle, inp, g, tmp_path, precision="float32")

@@ -166,3 +166,29 @@ def test_c1_deck_is_the_documented_workload():
         ext = lv["bounds"]["x"][1] - lv["bounds"]["x"][0]
         assert abs(ext / 40 - h) < 1e-12
     assert 41 * 41 * 21 == 35301
+
+
+# ---------------------------------------------------------------------------
+# device code shape
+# ---------------------------------------------------------------------------
+
+def test_device_code_has_no_calls_or_recursion(tmp_path):
+    """Every device helper must inline into its kernel: an out-of-line call
+    puts a stack in scratch memory, and an accidental self-call (a helper
+    that calls itself) overflows it at run time.  Checked on the gfx950
+    assembly of both HIP sources."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    csrc = os.path.join(ROOT, "go-melt_amd", "csrc")
+    for src in ("gm_kernels.hip", "gm_engine.hip"):
+        out = tmp_path / (src + ".s")
+        subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
+                        "-target-feature", "-Xclang", "-packed-fp32-ops", "--offload-device-only",
+                        "-S", os.path.join(csrc, src), "-o", str(out)],
+                       check=True, capture_output=True)
+        asm = out.read_text()
+        assert "s_swappc_b64" not in asm, f"{src}: device function call"
+        assert "has_recursion, 1" not in asm, f"{src}: recursive device function"
