@@ -92,6 +92,43 @@ __device__ __forceinline__ void interp8(const gm_grid& c, float x, float y, floa
     for (int a = 0; a < 8; ++a) N[a] = ok ? fminf(fmaxf(N[a], 0.0f), 1.0f) : 0.0f;
 }
 
+// Node index -> (ix, iy, iz) in 32-bit unsigned arithmetic (every mesh and
+// box here has fewer than 2^32 nodes; 64-bit division costs ~10x more).
+__device__ __forceinline__ void dec3(int64_t t, int nx, int ny, int& ix, int& iy, int& iz) {
+    uint32_t u = (uint32_t)t;
+    const uint32_t q = u / (uint32_t)nx;
+    ix = (int)(u - q * (uint32_t)nx);
+    iz = (int)(q / (uint32_t)ny);
+    iy = (int)(q - (uint32_t)iz * (uint32_t)ny);
+}
+
+// interp8 returning the clipped coarse element per axis instead of node ids
+__device__ __forceinline__ void interp8e(const gm_grid& c, float x, float y, float z, float N[8],
+                                         int e[3]) {
+    const float q[3] = {x, y, z};
+    float lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int ee = (int)floorf((q[d] - c.x0[d]) / c.h[d]);
+        ee = ee < 0 ? 0 : (ee > c.n[d] - 2 ? c.n[d] - 2 : ee);
+        e[d] = ee;
+        const float x0 = coordf(c, d, ee), x1 = coordf(c, d, ee + 1);
+        lo[d] = x1 - q[d];
+        hi[d] = q[d] - x0;
+    }
+    const float inv = 1.0f / (c.h[0] * c.h[1] * c.h[2]);
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int bx = ((a + 1) >> 1) & 1, by = (a >> 1) & 1, bz = a >> 2;
+        const float v = (bx ? hi[0] : lo[0]) * (by ? hi[1] : lo[1]) * (bz ? hi[2] : lo[2]) * inv;
+        N[a] = v;
+        ok = ok && (v >= -1e-2f) && (v <= 1.0f + 1e-2f);
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) N[a] = ok ? fminf(fmaxf(N[a], 0.0f), 1.0f) : 0.0f;
+}
+
 __device__ __forceinline__ float interp_f(const gm_grid& c, const float* u, float x, float y,
                                           float z) {
     float N[8];
@@ -2423,7 +2460,8 @@ __global__ void k_interp_grid(const GmStepDesc* __restrict__ d, int src_mode, gm
     const int64_t n = (int64_t)tg.n[0] * tg.n[1] * tg.n[2];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int ix = t % tg.n[0], iy = (t / tg.n[0]) % tg.n[1], iz = t / ((int64_t)tg.n[0] * tg.n[1]);
+    int ix, iy, iz;
+    dec3(t, tg.n[0], tg.n[1], ix, iy, iz);
     float N[8];
     int64_t nd[8];
     interp8(sg, coordf(tg, 0, ix), coordf(tg, 1, iy), coordf(tg, 2, iz), N, nd);
@@ -2451,7 +2489,8 @@ __global__ void k_inject(const GmStepDesc* __restrict__ d, int fine_level, int c
     const int n = b.n[0] * b.n[1] * b.n[2];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int ix = t % b.n[0], iy = (t / b.n[0]) % b.n[1], iz = t / (b.n[0] * b.n[1]);
+    int ix, iy, iz;
+    dec3(t, b.n[0], b.n[1], ix, iy, iz);
     const float v = interp_f(d->g[fine_level], fT, coordf(xyz, 0, ix), coordf(xyz, 1, iy),
                              coordf(xyz, 2, iz));
     const gm_grid& cg = d->g[coarse_level];
@@ -2475,7 +2514,8 @@ __global__ void k_tprime(const GmStepDesc* __restrict__ d, int fine_level, int c
     const int64_t n = (int64_t)fg.n[0] * fg.n[1] * fg.n[2];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int ix = t % fg.n[0], iy = (t / fg.n[0]) % fg.n[1], iz = t / ((int64_t)fg.n[0] * fg.n[1]);
+    int ix, iy, iz;
+    dec3(t, fg.n[0], fg.n[1], ix, iy, iz);
     const float v = interp_f(d->g[coarse_level], cT, coordf(fg, 0, ix), coordf(fg, 1, iy),
                              coordf(fg, 2, iz));
     Tp[t] = fT[t] - v;
@@ -2516,17 +2556,17 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
     }
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nf) {
-        const int ix = t % fg.n[0], iy = (t / fg.n[0]) % fg.n[1];
-        const int iz = t / ((int64_t)fg.n[0] * fg.n[1]);
+        int ix, iy, iz;
+        dec3(t, fg.n[0], fg.n[1], ix, iy, iz);
         float N[8];
-        int64_t nd[8];
-        interp8(cg, coordf(fg, 0, ix), coordf(fg, 1, iy), coordf(fg, 2, iz), N, nd);
+        int ce[3];
+        interp8e(cg, coordf(fg, 0, ix), coordf(fg, 1, iy), coordf(fg, 2, iz), N, ce);
         float v = 0.0f;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int64_t c = nd[q];
-            const int cx = (int)(c % cg.n[0]), cy = (int)((c / cg.n[0]) % cg.n[1]);
-            const int cz = (int)(c / ((int64_t)cg.n[0] * cg.n[1]));
+            const int cx = ce[0] + (((q + 1) >> 1) & 1), cy = ce[1] + ((q >> 1) & 1);
+            const int cz = ce[2] + (q >> 2);
+            const int64_t c = nid(cx, cy, cz, cg.n[0], cg.n[1]);
             const int bx = cx - b.lo[0], by = cy - b.lo[1], bz = cz - b.lo[2];
             const int jx = f0[0] + bx * fr[0], jy = f0[1] + by * fr[1], jz = f0[2] + bz * fr[2];
             const bool inj = bx >= 0 && bx < b.n[0] && by >= 0 && by < b.n[1] && bz >= 0 &&
@@ -2578,7 +2618,8 @@ __global__ void k_move_l3(const GmStepDesc* __restrict__ d, const float* __restr
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const gm_grid& g3 = d->g[3];
-    const int ix = t % g3.n[0], iy = (t / g3.n[0]) % g3.n[1], iz = t / ((int64_t)g3.n[0] * g3.n[1]);
+    int ix, iy, iz;
+    dec3(t, g3.n[0], g3.n[1], ix, iy, iz);
     const float x = coordf(g3, 0, ix), y = coordf(g3, 1, iy), z = coordf(g3, 2, iz);
     const float a = interp_f(d->g[1], T1, x, y, z);
     const float b = interp_f(d->gprev[2], Tp2, x, y, z);
@@ -2599,7 +2640,8 @@ __global__ void k_move_l2(const GmStepDesc* __restrict__ d, const float* __restr
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const gm_grid& g2 = d->g[2];
-    const int ix = t % g2.n[0], iy = (t / g2.n[0]) % g2.n[1], iz = t / ((int64_t)g2.n[0] * g2.n[1]);
+    int ix, iy, iz;
+    dec3(t, g2.n[0], g2.n[1], ix, iy, iz);
     const float x = coordf(g2, 0, ix), y = coordf(g2, 1, iy), z = coordf(g2, 2, iz);
     const float tp = interp_f(d->gprev[2], Tp2, x, y, z);
     const float a = interp_f(d->g[1], T1, x, y, z);
@@ -2620,7 +2662,8 @@ __global__ void k_l1_s1_from_l2(const GmStepDesc* __restrict__ d, const uint8_t*
     const int n = b.n[0] * b.n[1] * b.n[2];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int ix = t % b.n[0], iy = (t / b.n[0]) % b.n[1], iz = t / (b.n[0] * b.n[1]);
+    int ix, iy, iz;
+    dec3(t, b.n[0], b.n[1], ix, iy, iz);
     const gm_grid& xyz = d->ov2_xyz;
     float N[8];
     int64_t nd[8];
@@ -2694,8 +2737,8 @@ __global__ void k_l0_gather(const GmStepDesc* __restrict__ d, const uint8_t* __r
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nn3) {
         const gm_grid& g3 = d->g[3];
-        const int ix = t % g3.n[0], iy = (t / g3.n[0]) % g3.n[1],
-                  iz = t / ((int64_t)g3.n[0] * g3.n[1]);
+        int ix, iy, iz;
+        dec3(t, g3.n[0], g3.n[1], ix, iy, iz);
         const int lx = d->l0_lo3[0] + ix, ly = d->l0_lo3[1] + iy, lz = d->l0_lo3[2] + iz;
         const bool in0 = lx >= 0 && lx < d->l0_n[0] && ly >= 0 && ly < d->l0_n[1] && lz >= 0 &&
                          lz < d->l0_n[2];
@@ -2710,8 +2753,8 @@ __global__ void k_l0_gather(const GmStepDesc* __restrict__ d, const uint8_t* __r
     }
     if (t < nn2) {
         const gm_grid& g2 = d->g[2];
-        const int ix = t % g2.n[0], iy = (t / g2.n[0]) % g2.n[1],
-                  iz = t / ((int64_t)g2.n[0] * g2.n[1]);
+        int ix, iy, iz;
+        dec3(t, g2.n[0], g2.n[1], ix, iy, iz);
         const int lx = d->l0_lo2[0] + d->l0_st2[0] * ix, ly = d->l0_lo2[1] + d->l0_st2[1] * iy,
                   lz = d->l0_lo2[2] + d->l0_st2[2] * iz;
         const bool in0 = lx >= 0 && lx < d->l0_n[0] && ly >= 0 && ly < d->l0_n[1] && lz >= 0 &&
@@ -2734,7 +2777,8 @@ __global__ void k_l0_scatter(const GmStepDesc* __restrict__ d, const uint8_t* __
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nn3) return;
     const gm_grid& g3 = d->g[3];
-    const int ix = t % g3.n[0], iy = (t / g3.n[0]) % g3.n[1], iz = t / ((int64_t)g3.n[0] * g3.n[1]);
+    int ix, iy, iz;
+    dec3(t, g3.n[0], g3.n[1], ix, iy, iz);
     const int lx = d->l0_lo3[0] + ix, ly = d->l0_lo3[1] + iy, lz = d->l0_lo3[2] + iz;
     if (lx >= 0 && lx < d->l0_n[0] && ly >= 0 && ly < d->l0_n[1] && lz >= 0 && lz < d->l0_n[2])
         L0S1[nid(lx, ly, lz, d->l0_n[0], d->l0_n[1])] = S13[t];
@@ -2761,7 +2805,8 @@ __global__ void k_melt_single(const GmStepDesc* __restrict__ d, const float* __r
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nn3) return;
     const gm_grid& g3 = d->g[3];
-    const int ix = t % g3.n[0], iy = (t / g3.n[0]) % g3.n[1], iz = t / ((int64_t)g3.n[0] * g3.n[1]);
+    int ix, iy, iz;
+    dec3(t, g3.n[0], g3.n[1], ix, iy, iz);
     const int lx = d->l0_lo3[0] + ix, ly = d->l0_lo3[1] + iy, lz = d->l0_lo3[2] + iz;
     if (lx < 0 || lx >= d->l0_n[0] || ly < 0 || ly >= d->l0_n[1] || lz < 0 || lz >= d->l0_n[2])
         return;
@@ -2842,7 +2887,8 @@ __global__ void k_source_l3_full(gm_grid g, DevMat m, float vx, float vy, float 
     const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int ix = t % g.n[0], iy = (t / g.n[0]) % g.n[1], iz = t / ((int64_t)g.n[0] * g.n[1]);
+    int ix, iy, iz;
+    dec3(t, g.n[0], g.n[1], ix, iy, iz);
     const float sx = src_axis(g, 0, ix, vx, m.rsq, m.rc) * (m.pc6 * P);
     const float sy = src_axis(g, 1, iy, vy, m.rsq, m.rc);
     const float sz = src_axis(g, 2, iz, vz, m.dsq, m.dc);
@@ -2892,7 +2938,8 @@ __global__ void k_interp_matrix(gm_grid src, gm_grid tg, float* __restrict__ Nc,
     const int64_t n = (int64_t)tg.n[0] * tg.n[1] * tg.n[2];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int ix = t % tg.n[0], iy = (t / tg.n[0]) % tg.n[1], iz = t / ((int64_t)tg.n[0] * tg.n[1]);
+    int ix, iy, iz;
+    dec3(t, tg.n[0], tg.n[1], ix, iy, iz);
     float N[8];
     int64_t nd[8];
     interp8(src, coordf(tg, 0, ix), coordf(tg, 1, iy), coordf(tg, 2, iz), N, nd);
