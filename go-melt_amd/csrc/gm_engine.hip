@@ -456,6 +456,8 @@ struct gm_engine {
     uint8_t* S13b[2] = {nullptr, nullptr};
     uint8_t* S23b[2] = {nullptr, nullptr};
     float* Tp3x = nullptr;
+    float* src3 = nullptr;  // L3 source factors of the N2*N3 rows of a subcycle
+    float *acc3 = nullptr, *macc3 = nullptr;  // melt time on the L3 nodes during Part 2
     // k, cp companions of the L2/L3 temperature buffers that feed level
     // steps: kc_T[i] belongs to kc_key[i]
     std::vector<const float*> kc_key;
@@ -575,7 +577,11 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
         eng_alloc(e, (void**)&e->fa2, n2 * 4) || eng_alloc(e, (void**)&e->fb2, n2 * 4) ||
         eng_alloc(e, (void**)&e->fa3, n3 * 4) || eng_alloc(e, (void**)&e->fb3, n3 * 4) ||
         eng_alloc(e, (void**)&e->d_desc, sizeof(GmStepDesc)) ||
-        eng_alloc(e, (void**)&e->d_dm, sizeof(DevMat)))
+        eng_alloc(e, (void**)&e->d_dm, sizeof(DevMat)) ||
+        eng_alloc(e, (void**)&e->acc3, n3 * 4) || eng_alloc(e, (void**)&e->macc3, n3 * 4) ||
+        eng_alloc(e, (void**)&e->src3,
+                  (size_t)e->N23 *
+                      (cfg->level[3].n[0] + cfg->level[3].n[1] + cfg->level[3].n[2]) * 4))
         return bad(-1);
     if (hipMemcpy(e->d_dm, &e->dm, sizeof(DevMat), hipMemcpyHostToDevice) != hipSuccess)
         return bad(fail("copy of the material constants failed"));
@@ -971,6 +977,7 @@ static void run_l3_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8
     a.Tout = Tout;
     a.kcin = kc_of(e, Tin);
     a.kcout = kc_of(e, Tout);
+    a.srcT = e->src3;
     a.n_sub = e->host.sub[3];
     a.dt_mode = GM_DT_ROW;
     a.dt_index = row;
@@ -985,8 +992,8 @@ static void run_l3_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8
         a.accum_on = 1;
         a.S2prev = S2prev;
         a.S2out = S2out;
-        a.accum = e->acc;
-        a.max_accum = e->macc;
+        a.accum = e->acc3;  // per L3 node (gathered from Level0 for Part 2)
+        a.max_accum = e->macc3;
     }
     memcpy(a.kc, e->kc[3], sizeof(a.kc));
     a.mrow = e->mrow[3];
@@ -1099,6 +1106,12 @@ static void subcycle_body(gm_engine* e) {
     // k, cp of the start-of-step L3 state for the carried-property L3 steps
     launch_props_kc(e->d_dm, e->T3, e->S13, n3, e->host.sub[3], kc_of(e, e->T3), s);
     launch_props_kc(e->d_dm, e->T2, e->S12, n2, e->host.sub[2], kc_of(e, e->T2), s);
+    // separable L3 source factors of all N2*N3 rows (the L3 mesh is fixed
+    // during the step)
+    {
+        const gm_grid& g3 = e->cfg.level[3];
+        launch_src3_tables(d, e->d_dm, N23, g3.n[0] + g3.n[1] + g3.n[2], e->w3[3], e->src3, s);
+    }
     e->use_kc = true;
     // L1 source over all N2*N3 positions (cF:3285-3287)
     launch_source_tables(e->pair[1].p, d, e->dm, 0, N23, 0, nullptr, e->G1, e->gmax[1], s);
@@ -1165,6 +1178,7 @@ static void subcycle_body(gm_engine* e) {
     launch_face_bc(d, 2, 1, e->cfg.level[2], e->T1c, e->T1, e->fa2, e->fb2, s);
 
     // ---- Part 2: restart L2/L3 from the start-of-step state ------------------
+    launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, n3, 1, s);
     T2cur = e->T2;
     S12cur = e->S12;
     T3cur = e->T3;
@@ -1206,6 +1220,7 @@ static void subcycle_body(gm_engine* e) {
         S23cur = S23in;
         Tp3cur = e->histTp[k];
     }
+    launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, n3, 0, s);
     // commit the end-of-step L2/L3 state
     launch_copy_f32(e->T2, T2cur, n2, s);
     launch_copy_u8(e->S12, S12cur, n2, s);

@@ -145,6 +145,9 @@ struct LSArgs {
     // of Tout (out), in the raw units of the material tables
     const float* kcin;
     float* kcout;
+    // tabulated separable L3 source factors of the desc rows: row r at
+    // srcT + r * (nx + ny + nz): x factors (times 6 sqrt(3) eta P w3), then y, z
+    const float* srcT;
 };
 
 // Restriction (fine -> coarse) arguments.
@@ -177,6 +180,11 @@ void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& host
 void launch_level_step_kc(const LSArgs& a, const GmStepDesc* d, const gm_grid& host_grid,
                           const DevMat* m_dev, hipStream_t s);
 // k, cp of (T, S1) for a level step that reads them
+// separable source factors of the L3 mesh for desc rows [0, nrows)
+void launch_src3_tables(const GmStepDesc* d, const DevMat* m_dev, int nrows, int ntot, float w3,
+                        float* out, hipStream_t s);
+void launch_acc_l3(const GmStepDesc* d, float* accum, float* max_accum, float* acc3, float* macc3,
+                   int64_t n3, int to_l3, hipStream_t s);
 void launch_props_kc(const DevMat* m_dev, const float* T, const uint8_t* S1, int64_t nn,
                      int64_t n_sub, float* kc, hipStream_t s);
 void launch_axis_tables(const GmPair& p, const GmStepDesc* d, hipStream_t s);

@@ -647,291 +647,22 @@ void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, 
     }
 }
 
-// Fine-level step with the nodal properties carried between steps.  The
-// input's k and cp come from the previous step (or from k_props_kc), so each
-// node's table lookups happen once, where its temperature is produced:
+// ---------------------------------------------------------------------------
+// Fine-level (L2, L3) step of the subcycle, without LDS or barriers.
+// The nodal properties are carried between steps: the input's k and cp come
+// from the previous step (or from k_props_kc), so each node's table lookups
+// happen once, where its temperature is produced:
 //   S1 state   : S1 | (T >= T_liq) | substrate        (cF:2585-2592)
-//   T+         : the explicit update of k_level_step   (cF:582-642)
+//   T+         : the explicit update of k_level_step   (cF:582-642), with the
+//                time-blended coarse field on the Dirichlet faces
 //   k+, cp+    : properties of T+ with S1 | (T+ >= T_liq), i.e. what the next
 //                step's computeStateProperties returns.
-// Dirichlet faces take the time-blended coarse field, all nodes are active.
-template <bool SRC, bool ACC, int ZCT>
-__global__ void __launch_bounds__(256) k_step_kc(LSArgs a, const GmStepDesc* __restrict__ d,
-                                                 const DevMat* __restrict__ m) {
-    constexpr int NP = ZCT + 2;
-    __shared__ float sT[3][PN];
-    __shared__ float sKn[PN], sCn[PN];
-    __shared__ float sSk[2][QN], sSc[2][QN];
-    __shared__ float sKe[2][QN], sCe[2][QN];
-    __shared__ float sSrc[LX + LY + ZCT];
-    __shared__ float sTab[2 * LUTW];
-    __shared__ float sFq[4][QN];  // surface flux at the 4 Gauss points of each top face
-
-    const gm_grid g = d->g[a.level];
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const int nxy = nx * ny;
-    const int nn = nxy * nz;
-    const int ntx = (nx + LX - 1) / LX;
-    const int nzc = (nz + ZCT - 1) / ZCT;
-    // z chunk fastest: each XCD owns whole tile columns (its share of the
-    // top-surface work and its z halos)
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int oz = (t % nzc) * ZCT, txy = t / nzc;
-    const int ox = (txy % ntx) * LX, oy = (txy / ntx) * LY;
-    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
-    const int tid = threadIdx.x;
-    const int tx = tid % LX, ty = tid / LX;
-    const int ix = ox + tx, iy = oy + ty;
-    const bool col_ok = ix < nx && iy < ny;
-    const float* __restrict__ Kin = a.kcin;
-    const float* __restrict__ Cin = a.kcin + nn;
-
-    // ---- prologue: all global loads of the block -----------------------------
-    int eoff[2];
-    bool eok[2];
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const int l = tid + 256 * e;
-        const int jx = ox + l % PX - 1, jy = oy + l / PX - 1;
-        eok[e] = l < PN && jx >= 0 && jx < nx && jy >= 0 && jy < ny;
-        eoff[e] = jx + nx * jy;
-    }
-    float rT[NP][2], rK[NP][2], rC[NP][2];
-#pragma unroll
-    for (int pl = 0; pl < NP; ++pl) {
-        const int p = oz - 1 + pl;
-        const bool pin = p >= 0 && p < nz;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            rT[pl][e] = rK[pl][e] = rC[pl][e] = 0.0f;
-            if (pin && eok[e]) {
-                const int idx = eoff[e] + nxy * p;
-                rT[pl][e] = a.T[idx];
-                rK[pl][e] = Kin[idx];
-                rC[pl][e] = Cin[idx];
-            }
-        }
-    }
-    const int cidx = ix + nx * iy;
-    const bool face_col = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
-    float rCorr[ZCT], rFa[ZCT], rFb[ZCT], rAcc[ZCT], rMax[ZCT];
-    bool rS1[ZCT], rS2[ZCT];
-    int clo0 = 0, clo1 = 0, clo2 = 0, cn0 = 0, cn1 = 0, cn2 = 0;
-    if (a.corr) {
-        clo0 = a.corr_info[0][0]; cn0 = a.corr_info[0][1] + 1;
-        clo1 = a.corr_info[1][0]; cn1 = a.corr_info[1][1] + 1;
-        clo2 = a.corr_info[2][0]; cn2 = a.corr_info[2][1] + 1;
-    }
-    int l0base = 0;
-    bool l0xy = false;
-    if (ACC) {
-        const int lx0 = d->l0_lo3[0] + ix, ly0 = d->l0_lo3[1] + iy;
-        l0xy = lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1];
-        l0base = lx0 + d->l0_n[0] * ly0;
-    }
-#pragma unroll
-    for (int z = 0; z < ZCT; ++z) {
-        const int zo = oz + z;
-        const bool ok = col_ok && zo < nz;
-        const int idx = cidx + nxy * zo;
-        float c = 0.0f;
-        if (ok && a.corr) {
-            const int bx = ix - clo0, by = iy - clo1, bz = zo - clo2;
-            if (bx >= 0 && bx < cn0 && by >= 0 && by < cn1 && bz >= 0 && bz < cn2)
-                c = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
-        }
-        rCorr[z] = c;
-        rFa[z] = rFb[z] = 0.0f;
-        if (ok && (face_col || zo == 0)) {
-            rFa[z] = a.bcA[idx];
-            rFb[z] = a.bcB[idx];
-        }
-        rS1[z] = ok && ((const uint8_t*)a.S1)[idx] != 0;
-        rAcc[z] = rMax[z] = 0.0f;
-        rS2[z] = false;
-        if (ACC && ok) {
-            rS2[z] = a.S2prev[idx] != 0;
-            const int lz0 = d->l0_lo3[2] + zo;
-            if (l0xy && lz0 >= 0 && lz0 < d->l0_n[2]) {
-                const int64_t l0 = (int64_t)l0base + (int64_t)d->l0_n[0] * d->l0_n[1] * lz0;
-                rAcc[z] = a.accum[l0];
-                rMax[z] = a.max_accum[l0];
-            }
-        }
-    }
-    float dt = a.dt;
-    if (a.dt_mode == GM_DT_ROW) dt = d->rows[a.dt_index][5];
-    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
-    const float mfac = a.mrow / dt;
-
-    PropC pc = load_pc(m);
-    const float Tamb = m->T_amb;
-    stage_tables(m, sTab, tid, 256);
-    if (SRC) {
-        const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
-        if (tid < LX) {
-            const int i = ox + tid;
-            sSrc[tid] = i < nx ? src_axis(g, 0, i, row[0], m->rsq, m->rc) * (m->pc6 * row[6])
-                               : 0.0f;
-        } else if (tid < LX + LY) {
-            const int i = oy + tid - LX;
-            sSrc[tid] = i < ny ? src_axis(g, 1, i, row[1], m->rsq, m->rc) : 0.0f;
-        } else if (tid < LX + LY + ZCT) {
-            const int i = oz + tid - LX - LY;
-            sSrc[tid] = i < nz ? src_axis(g, 2, i, row[2], m->dsq, m->dc) : 0.0f;
-        }
-    }
-    int qh[2];
-    bool qok[2], qxy[2];
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const int l = tid + 256 * e;
-        const int qx = l % QX, qy = l / QX;
-        qh[e] = qx + PX * qy;
-        const int ex = ox + qx - 1, ey = oy + qy - 1;
-        qok[e] = l < QN;
-        qxy[e] = l < QN && ex >= 0 && ex < nex && ey >= 0 && ey < ney;
-    }
-    const int hc = (tx + 1) + PX * (ty + 1);
-    const int q0 = tx + QX * ty;
-
-    // ---- march through the planes ---------------------------------------------
-#pragma unroll
-    for (int pl = 0; pl < NP; ++pl) {
-        const int p = oz - 1 + pl;
-        lds_barrier();
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const int l = tid + 256 * e;
-            if (l < PN) {
-                sT[pl % 3][l] = rT[pl][e];
-                sKn[l] = rK[pl][e];
-                sCn[l] = rC[pl][e];
-            }
-        }
-        lds_barrier();
-        const bool ezok = p >= 1 && p - 1 < nez;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            if (qok[e]) {
-                const int h0 = qh[e], l = tid + 256 * e;
-                const float Sk = (sKn[h0] + sKn[h0 + 1]) + (sKn[h0 + PX] + sKn[h0 + PX + 1]);
-                const float Sc = (sCn[h0] + sCn[h0 + 1]) + (sCn[h0 + PX] + sCn[h0 + PX + 1]);
-                const bool ok = qxy[e] && ezok;
-                sKe[pl & 1][l] = ok ? (sSk[(pl + 1) & 1][l] + Sk) * 0.125f : 0.0f;
-                sCe[pl & 1][l] = ok ? (sSc[(pl + 1) & 1][l] + Sc) * 0.125f : 0.0f;
-                sSk[pl & 1][l] = Sk;
-                sSc[pl & 1][l] = Sc;
-            }
-        }
-        lds_barrier();
-        if (pl < 2) continue;
-        const int z = pl - 2;
-        const int zo = oz + z;
-        if (zo == nez) {
-            // top faces (cF:2254-2299): flux at the 4 face Gauss points of
-            // each element face of the tile, once per face
-            const float* Tc = sT[(pl + 2) % 3];
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                if (qxy[e]) {
-                    const int l = tid + 256 * e;
-                    const int f0 = qh[e];
-                    const float Tf[4] = {Tc[f0], Tc[f0 + 1], Tc[f0 + 1 + PX], Tc[f0 + PX]};
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const float Tq = n2(q, 0) * Tf[0] + n2(q, 1) * Tf[1] + n2(q, 2) * Tf[2] +
-                                         n2(q, 3) * Tf[3];
-                        sFq[q][l] = face_flux(*m, Tq);
-                    }
-                }
-            }
-            lds_barrier();
-        }
-        if (zo >= nz || !col_ok) continue;
-        const int idx = cidx + nxy * zo;
-        const float* Tm = sT[(pl + 1) % 3];
-        const float* T0 = sT[(pl + 2) % 3];
-        const float* Tp = sT[pl % 3];
-        const float* kb = sKe[(pl + 1) & 1];
-        const float* ka = sKe[pl & 1];
-        const float* cb = sCe[(pl + 1) & 1];
-        const float* ca = sCe[pl & 1];
-        const float k000 = kb[q0], k100 = kb[q0 + 1], k010 = kb[q0 + QX], k110 = kb[q0 + QX + 1];
-        const float k001 = ka[q0], k101 = ka[q0 + 1], k011 = ka[q0 + QX], k111 = ka[q0 + QX + 1];
-        const float Msum = ((cb[q0] + cb[q0 + 1]) + (cb[q0 + QX] + cb[q0 + QX + 1])) +
-                           ((ca[q0] + ca[q0 + 1]) + (ca[q0 + QX] + ca[q0 + QX + 1]));
-        const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011, Pz11 = k110 + k111;
-        const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011, Py11 = k101 + k111;
-        const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101, Px11 = k011 + k111;
-        const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
-        const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
-        const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
-        const float Kc = Fx0 + Fx1;
-        const float Ti = T0[hc];
-        float KT = a.kc[0] * (Kc * Ti);
-        KT += a.kc[1] * (Fx0 * T0[hc - 1] + Fx1 * T0[hc + 1]);
-        KT += a.kc[2] * (Fy0 * T0[hc - PX] + Fy1 * T0[hc + PX]);
-        KT += a.kc[4] * (Fz0 * Tm[hc] + Fz1 * Tp[hc]);
-        KT += a.kc[3] * (Pz00 * T0[hc - 1 - PX] + Pz10 * T0[hc + 1 - PX] + Pz01 * T0[hc - 1 + PX] +
-                         Pz11 * T0[hc + 1 + PX]);
-        KT += a.kc[5] * (Py00 * Tm[hc - 1] + Py10 * Tm[hc + 1] + Py01 * Tp[hc - 1] +
-                         Py11 * Tp[hc + 1]);
-        KT += a.kc[6] * (Px00 * Tm[hc - PX] + Px10 * Tm[hc + PX] + Px01 * Tp[hc - PX] +
-                         Px11 * Tp[hc + PX]);
-        KT += a.kc[7] * (k000 * Tm[hc - 1 - PX] + k100 * Tm[hc + 1 - PX] + k010 * Tm[hc - 1 + PX] +
-                         k110 * Tm[hc + 1 + PX] + k001 * Tp[hc - 1 - PX] + k101 * Tp[hc + 1 - PX] +
-                         k011 * Tp[hc - 1 + PX] + k111 * Tp[hc + 1 + PX]);
-        const float M = Msum * mfac;
-        float F = 0.0f;
-        if (SRC) F = sSrc[tx] * sSrc[LX + ty] * sSrc[LX + LY + z] * a.w3;
-        if (zo == nez) {
-            float cr = 0.0f;
-            for (int e = 0; e < 4; ++e) {
-                const int exx = ix - 1 + (e & 1), eyy = iy - 1 + (e >> 1);
-                if (exx < 0 || exx >= nex || eyy < 0 || eyy >= ney) continue;
-                const int qe = (exx - ox + 1) + QX * (eyy - oy + 1);
-                const int ax = ix - exx, ay = iy - eyy;
-                const int ai = ay ? (ax ? 2 : 3) : (ax ? 1 : 0);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) cr += n2(q, ai) * (sFq[q][qe] * a.w2);
-            }
-            F += cr;
-        }
-        float Tnew = ((M * Ti - KT) + F + rCorr[z]) / M;
-        if (face_col || zo == 0) Tnew = a.alpha * rFa[z] + (1.0f - a.alpha) * rFb[z];
-        if (a.clamp && Tnew < Tamb) Tnew = Tamb;
-        a.Tout[idx] = Tnew;
-        // S1 state of this step's properties, then k, cp of T+
-        const bool s1 = rS1[z] || Ti >= pc.Tliq || idx < a.n_sub;
-        a.S1out[idx] = (uint8_t)s1;
-        float kn, cn;
-        bool s1n;
-        props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
-        a.kcout[idx] = kn;
-        a.kcout[nn + idx] = cn;
-        if (ACC) {
-            const bool s2n = Ti >= pc.Tliq;
-            const bool reset = (!rS2[z]) && s2n;
-            const int lz0 = d->l0_lo3[2] + zo;
-            if (l0xy && lz0 >= 0 && lz0 < d->l0_n[2]) {
-                const int64_t l0 = (int64_t)l0base + (int64_t)d->l0_n[0] * d->l0_n[1] * lz0;
-                const float racc = reset ? rAcc[z] : 0.0f;
-                a.max_accum[l0] = racc > rMax[z] ? racc : rMax[z];
-                a.accum[l0] = rAcc[z] + d->rows[a.src_row][5] * (s2n ? 1.0f : 0.0f) - racc;
-            }
-            a.S2out[idx] = (uint8_t)s2n;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Fine-level step without LDS or barriers.  One wave owns a brick of 62
-// x-columns, YB rows and ZC planes: lane l holds x = 62 s - 1 + l (lanes 0 and
-// 63 only supply the x halo), every y and z neighbour of its column is in
-// registers, and x neighbours come from the adjacent lanes through DPP wave
-// shifts.  Same update as k_step_kc (carried k, cp; time-blended Dirichlet
-// faces; top-surface flux; properties of T+).
+// One wave owns a brick of 62 x-columns, YB rows and ZC planes: lane l holds
+// x = 62 s - 1 + l (lanes 0 and 63 only supply the x halo), every y and z
+// neighbour of its column is in registers, and x neighbours come from the
+// adjacent lanes through DPP wave shifts.  The brick is unrolled: with few
+// waves per SIMD on a 0.5M-node mesh, independent nodes in one wave hide
+// latency that a rolled loop or LDS barriers expose.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float lane_prev(float v) {  // value of lane l-1 (0 in lane 0)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
@@ -1000,16 +731,11 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
                 }
                 rS1[i][j] = ((const uint8_t*)a.S1)[(unsigned)idx] != 0;
                 if (ACC) {
+                    // melt-time state on the L3 nodes (gathered from Level0
+                    // for the step, scattered back after it)
                     rS2[i][j] = a.S2prev[(unsigned)idx] != 0;
-                    const int lx0 = d->l0_lo3[0] + x, ly0 = d->l0_lo3[1] + yy,
-                              lz0 = d->l0_lo3[2] + zz;
-                    if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] &&
-                        lz0 >= 0 && lz0 < d->l0_n[2]) {
-                        const int64_t l0 =
-                            (int64_t)lx0 + (int64_t)d->l0_n[0] * (ly0 + (int64_t)d->l0_n[1] * lz0);
-                        rAcc[i][j] = a.accum[l0];
-                        rMax[i][j] = a.max_accum[l0];
-                    }
+                    rAcc[i][j] = a.accum[(unsigned)idx];
+                    rMax[i][j] = a.max_accum[(unsigned)idx];
                 }
             }
         }
@@ -1023,13 +749,28 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
     // are computed one per lane and broadcast
     float sx = 0.0f, syz = 0.0f;
     if (SRC) {
-        sx = x >= 0 && x < nx ? src_axis(g, 0, x, row[0], m->rsq, m->rc) * (m->pc6 * row[6]) : 0.0f;
-        if (lane < YB) {
-            const int yy = y0 + lane;
-            syz = yy < ny ? src_axis(g, 1, yy, row[1], m->rsq, m->rc) : 0.0f;
-        } else if (lane < YB + ZC) {
-            const int zz = z0 + lane - YB;
-            syz = zz < nz ? src_axis(g, 2, zz, row[2], m->dsq, m->dc) : 0.0f;
+        if (a.srcT) {
+            // tabulated per row: x factors (with 6 sqrt(3) eta P w3), y, z
+            const float* st = a.srcT + (size_t)a.src_row * (nx + ny + nz);
+            sx = x >= 0 && x < nx ? st[x] : 0.0f;
+            if (lane < YB) {
+                const int yy = y0 + lane;
+                syz = yy < ny ? st[nx + yy] : 0.0f;
+            } else if (lane < YB + ZC) {
+                const int zz = z0 + lane - YB;
+                syz = zz < nz ? st[nx + ny + zz] : 0.0f;
+            }
+        } else {
+            sx = x >= 0 && x < nx
+                     ? src_axis(g, 0, x, row[0], m->rsq, m->rc) * (m->pc6 * row[6] * a.w3)
+                     : 0.0f;
+            if (lane < YB) {
+                const int yy = y0 + lane;
+                syz = yy < ny ? src_axis(g, 1, yy, row[1], m->rsq, m->rc) : 0.0f;
+            } else if (lane < YB + ZC) {
+                const int zz = z0 + lane - YB;
+                syz = zz < nz ? src_axis(g, 2, zz, row[2], m->dsq, m->dc) : 0.0f;
+            }
         }
     }
     PropC pc = load_pc(m);
@@ -1136,7 +877,7 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
                              k011 * lane_prev(Tw[i + 2][j + 2]) + k111 * lane_next(Tw[i + 2][j + 2]));
             const float M = Msum * mfac;
             float F = 0.0f;
-            if (SRC) F = sx * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), j)) * szv * a.w3;
+            if (SRC) F = sx * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), j)) * szv;
             if (has_top && i + 1 == ptop) {
                 // elements (x-1 | x, y-1 | y): node's local index ai in each face
                 float cr = 0.0f;
@@ -1179,16 +920,9 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
                 if (ACC) {
                     const bool s2n = Ti >= pc.Tliq;
                     const bool reset = (!rS2[i][j]) && s2n;
-                    const int lx0 = d->l0_lo3[0] + x, ly0 = d->l0_lo3[1] + yy,
-                              lz0 = d->l0_lo3[2] + zz;
-                    if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] &&
-                        lz0 >= 0 && lz0 < d->l0_n[2]) {
-                        const int64_t l0 =
-                            (int64_t)lx0 + (int64_t)d->l0_n[0] * (ly0 + (int64_t)d->l0_n[1] * lz0);
-                        const float rr = reset ? rAcc[i][j] : 0.0f;
-                        a.max_accum[l0] = rr > rMax[i][j] ? rr : rMax[i][j];
-                        a.accum[l0] = rAcc[i][j] + row[5] * (s2n ? 1.0f : 0.0f) - rr;
-                    }
+                    const float rr = reset ? rAcc[i][j] : 0.0f;
+                    a.max_accum[(unsigned)idx] = rr > rMax[i][j] ? rr : rMax[i][j];
+                    a.accum[(unsigned)idx] = rAcc[i][j] + row[5] * (s2n ? 1.0f : 0.0f) - rr;
                     a.S2out[(unsigned)idx] = (uint8_t)s2n;
                 }
             }
@@ -1196,530 +930,63 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
     }
 }
 
-// Lean variant of k_step_dpp.  Global accesses go through buffer resources
-// with 32-bit per-lane byte offsets (no 64-bit address arithmetic).  Every
-// offset stays inside its buffer: lanes, rows and planes outside the mesh
-// read offset 0 and discard the value, and stores are predicated.  Every x
-// shift of a value is computed once and shared by the nodes.
-#define GM_BUF_DW3 0x00020000  // buffer resource word 3 (gfx9 family: raw dword access)
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
-                                             GM_BUF_DW3);
+// Separable source factors of every L3 node line for the desc rows:
+// src_axis of x (times 6 sqrt(3) eta P w3), y and z (cF:1013-1025, 2986-3010).
+__global__ void k_src3_tables(const GmStepDesc* __restrict__ d, const DevMat* __restrict__ m,
+                              int nrows, float w3, float* __restrict__ out) {
+    const gm_grid& g = d->g[3];
+    const int n = g.n[0] + g.n[1] + g.n[2];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows * n) return;
+    const int r = t / n, i = t % n;
+    const float* row = d->rows[r];
+    float v;
+    if (i < g.n[0]) {
+        v = src_axis(g, 0, i, row[0], m->rsq, m->rc) * (m->pc6 * row[6] * w3);
+    } else if (i < g.n[0] + g.n[1]) {
+        v = src_axis(g, 1, i - g.n[0], row[1], m->rsq, m->rc);
+    } else {
+        v = src_axis(g, 2, i - g.n[0] - g.n[1], row[2], m->dsq, m->dc);
+    }
+    out[t] = v;
 }
-__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+
+void launch_src3_tables(const GmStepDesc* d, const DevMat* m, int nrows, int ntot, float w3,
+                        float* out, hipStream_t s) {
+    const int n = nrows * ntot;
+    if (n > 0)
+        hipLaunchKernelGGL(k_src3_tables, dim3((n + 255) / 256), dim3(256), 0, s, d, m, nrows, w3,
+                           out);
 }
-__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, 0, 0);
-}
 
-template <bool SRC, bool ACC, int YB, int ZC>
-__global__ void __launch_bounds__(64) k_step_lean(LSArgs a, const GmStepDesc* __restrict__ d,
-                                                  const DevMat* __restrict__ m) {
-    const gm_grid g = d->g[a.level];
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const int nxy = nx * ny;
-    const int nn = nxy * nz;
-    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
-    const int nst = (nx + DXS - 1) / DXS, nzc = (nz + ZC - 1) / ZC;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int zc = t % nzc, rest = t / nzc;
-    const int st = rest % nst, yb = rest / nst;
-    const int lane = threadIdx.x;
-    const int x = DXS * st - 1 + lane;
-    const int y0 = yb * YB, z0 = zc * ZC;
-    const bool xin = x >= 0 && x < nx;
-    const bool ein = x >= 0 && x < nex;
-    const bool olane = lane >= 1 && lane <= DXS && xin;
-    const uint32_t nb4 = (uint32_t)nn * 4u;
-    const auto rT = buf_rsrc(a.T, nb4);
-    const auto rK = buf_rsrc(a.kcin, nb4);
-    const auto rC = buf_rsrc(a.kcin + nn, nb4);
-    // element index of the lane's column at (y0, z0)
-    const int col = x + nx * y0 + nxy * z0;
-
-    __shared__ float sTab[2 * LUTW];
-    stage_tables(m, sTab, lane, 64);
-
-    // ---- node window: planes z0-1 .. z0+ZC, rows y0-1 .. y0+YB ---------------
-    float Tw[ZC + 2][YB + 2], Kw[ZC + 2][YB + 2], Cw[ZC + 2][YB + 2];
-#pragma unroll
-    for (int p = 0; p < ZC + 2; ++p) {
-        const int zz = z0 - 1 + p;
-#pragma unroll
-        for (int r = 0; r < YB + 2; ++r) {
-            const int yy = y0 - 1 + r;
-            const bool in = xin && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
-            const uint32_t vo = in ? 4u * (uint32_t)(col + nx * (r - 1) + nxy * (p - 1)) : 0u;
-            const float tv = buf_ld(rT, vo), kv = buf_ld(rK, vo), cv = buf_ld(rC, vo);
-            Tw[p][r] = in ? tv : 0.0f;
-            Kw[p][r] = in ? kv : 0.0f;
-            Cw[p][r] = in ? cv : 0.0f;
-        }
-    }
-    const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
-    float dt = a.dt;
-    if (a.dt_mode == GM_DT_ROW) dt = row[5];
-    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
-    const float mfac = a.mrow / dt;
-    float sx = 0.0f, syz = 0.0f;
-    if (SRC) {
-        sx = xin ? src_axis(g, 0, x, row[0], m->rsq, m->rc) * (m->pc6 * row[6] * a.w3) : 0.0f;
-        if (lane < YB) {
-            const int yy = y0 + lane;
-            syz = yy < ny ? src_axis(g, 1, yy, row[1], m->rsq, m->rc) : 0.0f;
-        } else if (lane < YB + ZC) {
-            const int zz = z0 + lane - YB;
-            syz = zz < nz ? src_axis(g, 2, zz, row[2], m->dsq, m->dc) : 0.0f;
-        }
-    }
-    const PropC pc = load_pc(m);
-    const float Tamb = m->T_amb;
-
-    // ---- quad sums and element means ------------------------------------------
-    float Ek[ZC + 1][YB + 1], Ec[ZC + 1][YB + 1];
-    {
-        float Qk[ZC + 2][YB + 1], Qc[ZC + 2][YB + 1];
-#pragma unroll
-        for (int p = 0; p < ZC + 2; ++p) {
-#pragma unroll
-            for (int r = 0; r < YB + 1; ++r) {
-                const int yy = y0 - 1 + r;
-                const bool ok = ein && yy >= 0 && yy < ney;
-                const float ak = Kw[p][r] + Kw[p][r + 1];
-                const float ac = Cw[p][r] + Cw[p][r + 1];
-                const float qk = ak + lane_next(ak);
-                const float qc = ac + lane_next(ac);
-                Qk[p][r] = ok ? qk : 0.0f;
-                Qc[p][r] = ok ? qc : 0.0f;
-            }
-        }
-#pragma unroll
-        for (int p = 0; p < ZC + 1; ++p) {
-            const int ez = z0 - 1 + p;
-            const float f = (ez >= 0 && ez < nez) ? 0.125f : 0.0f;
-#pragma unroll
-            for (int r = 0; r < YB + 1; ++r) {
-                Ek[p][r] = (Qk[p][r] + Qk[p + 1][r]) * f;
-                Ec[p][r] = (Qc[p][r] + Qc[p + 1][r]) * f;
-            }
-        }
-    }
-    // x-shifted copies, one per value
-    float Ekm[ZC + 1][YB + 1], Csum[ZC][YB];
-#pragma unroll
-    for (int p = 0; p < ZC + 1; ++p)
-#pragma unroll
-        for (int r = 0; r < YB + 1; ++r) Ekm[p][r] = lane_prev(Ek[p][r]);
-#pragma unroll
-    for (int i = 0; i < ZC; ++i)
-#pragma unroll
-        for (int j = 0; j < YB; ++j) {
-            const float cs = (Ec[i][j] + Ec[i][j + 1]) + (Ec[i + 1][j] + Ec[i + 1][j + 1]);
-            Csum[i][j] = cs + lane_prev(cs);
-        }
-    float Tm[ZC + 2][YB + 2], Tq[ZC + 2][YB + 2];  // T of lanes l-1 and l+1
-#pragma unroll
-    for (int p = 0; p < ZC + 2; ++p)
-#pragma unroll
-        for (int r = 0; r < YB + 2; ++r) {
-            Tm[p][r] = lane_prev(Tw[p][r]);
-            Tq[p][r] = lane_next(Tw[p][r]);
-        }
-    // top surface flux per element face, when the top plane is in the brick
-    const int ptop = nez - (z0 - 1);
-    const bool has_top = ptop >= 1 && ptop <= ZC;
-    float Fq[YB + 1][4], Fqm[YB + 1][4];
-    if (has_top) {
-#pragma unroll
-        for (int r = 0; r < YB + 1; ++r) {
-            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
-#pragma unroll
-            for (int p = 1; p <= ZC; ++p)
-                if (p == ptop) {
-                    t0 = Tw[p][r];
-                    t1 = Tq[p][r];
-                    t2 = Tq[p][r + 1];
-                    t3 = Tw[p][r + 1];
-                }
-            const int yy = y0 - 1 + r;
-            const bool ok = ein && yy >= 0 && yy < ney;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float Tg = n2(q, 0) * t0 + n2(q, 1) * t1 + n2(q, 2) * t2 + n2(q, 3) * t3;
-                Fq[r][q] = ok ? face_flux(*m, Tg) * a.w2 : 0.0f;
-                Fqm[r][q] = lane_prev(Fq[r][q]);
-            }
-        }
-    }
-    lds_barrier();  // property tables staged
-
-    const auto rTo = buf_rsrc(a.Tout, nb4);
-    const auto rKo = buf_rsrc(a.kcout, nb4);
-    const auto rCo = buf_rsrc(a.kcout + nn, nb4);
-    const auto rS1 = buf_rsrc(a.S1, (uint32_t)nn);
-    const auto rS1o = buf_rsrc(a.S1out, (uint32_t)nn);
-    const auto rFa = buf_rsrc(a.bcA, nb4);
-    const auto rFb = buf_rsrc(a.bcB, nb4);
-#pragma unroll
-    for (int i = 0; i < ZC; ++i) {
-        const int zz = z0 + i;
-        float szv = 0.0f;
-        if (SRC) szv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), YB + i));
-#pragma unroll
-        for (int j = 0; j < YB; ++j) {
-            const int yy = y0 + j;
-            const bool nin = yy < ny && zz < nz;
-            const int idx = x + nx * yy + nxy * zz;
-            const bool own = olane && nin;  // this lane writes node idx
-            const uint32_t ob = own ? (uint32_t)idx : 0u;        // byte arrays
-            const uint32_t of = own ? 4u * (uint32_t)idx : 0u;   // float arrays
-            const float k100 = Ek[i][j], k110 = Ek[i][j + 1];
-            const float k101 = Ek[i + 1][j], k111 = Ek[i + 1][j + 1];
-            const float k000 = Ekm[i][j], k010 = Ekm[i][j + 1];
-            const float k001 = Ekm[i + 1][j], k011 = Ekm[i + 1][j + 1];
-            const float Ti = Tw[i + 1][j + 1];
-            const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011,
-                        Pz11 = k110 + k111;
-            const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011,
-                        Py11 = k101 + k111;
-            const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101,
-                        Px11 = k011 + k111;
-            const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
-            const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
-            const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
-            float KT = a.kc[0] * ((Fx0 + Fx1) * Ti);
-            KT += a.kc[1] * (Fx0 * Tm[i + 1][j + 1] + Fx1 * Tq[i + 1][j + 1]);
-            KT += a.kc[2] * (Fy0 * Tw[i + 1][j] + Fy1 * Tw[i + 1][j + 2]);
-            KT += a.kc[4] * (Fz0 * Tw[i][j + 1] + Fz1 * Tw[i + 2][j + 1]);
-            KT += a.kc[3] * (Pz00 * Tm[i + 1][j] + Pz10 * Tq[i + 1][j] + Pz01 * Tm[i + 1][j + 2] +
-                             Pz11 * Tq[i + 1][j + 2]);
-            KT += a.kc[5] * (Py00 * Tm[i][j + 1] + Py10 * Tq[i][j + 1] + Py01 * Tm[i + 2][j + 1] +
-                             Py11 * Tq[i + 2][j + 1]);
-            KT += a.kc[6] * (Px00 * Tw[i][j] + Px10 * Tw[i][j + 2] + Px01 * Tw[i + 2][j] +
-                             Px11 * Tw[i + 2][j + 2]);
-            KT += a.kc[7] * (k000 * Tm[i][j] + k100 * Tq[i][j] + k010 * Tm[i][j + 2] +
-                             k110 * Tq[i][j + 2] + k001 * Tm[i + 2][j] + k101 * Tq[i + 2][j] +
-                             k011 * Tm[i + 2][j + 2] + k111 * Tq[i + 2][j + 2]);
-            const float M = Csum[i][j] * mfac;
-            float F = 0.0f;
-            if (SRC)
-                F = sx * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), j)) * szv;
-            if (has_top && i + 1 == ptop) {
-                // faces (x-1 | x, y-1 | y) around the node; node's local index
-                F += n2(0, 2) * Fqm[j][0] + n2(1, 2) * Fqm[j][1] + n2(2, 2) * Fqm[j][2] +
-                     n2(3, 2) * Fqm[j][3];
-                F += n2(0, 3) * Fq[j][0] + n2(1, 3) * Fq[j][1] + n2(2, 3) * Fq[j][2] +
-                     n2(3, 3) * Fq[j][3];
-                F += n2(0, 1) * Fqm[j + 1][0] + n2(1, 1) * Fqm[j + 1][1] +
-                     n2(2, 1) * Fqm[j + 1][2] + n2(3, 1) * Fqm[j + 1][3];
-                F += n2(0, 0) * Fq[j + 1][0] + n2(1, 0) * Fq[j + 1][1] + n2(2, 0) * Fq[j + 1][2] +
-                     n2(3, 0) * Fq[j + 1][3];
-            }
-            float corr = 0.0f;
-            if (a.corr && own) {
-                const int bx = x - a.corr_info[0][0], by = yy - a.corr_info[1][0],
-                          bz = zz - a.corr_info[2][0];
-                if (bx >= 0 && bx <= a.corr_info[0][1] && by >= 0 && by <= a.corr_info[1][1] &&
-                    bz >= 0 && bz <= a.corr_info[2][1])
-                    corr = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
-            }
-            float Tnew = ((M * Ti - KT) + F + corr) / M;
-            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
-            if (face) {
-                const float fa = buf_ld(rFa, of), fb = buf_ld(rFb, of);
-                Tnew = a.alpha * fa + (1.0f - a.alpha) * fb;
-            }
-            if (a.clamp && Tnew < Tamb) Tnew = Tamb;
-            const bool s1in = __builtin_amdgcn_raw_buffer_load_b8(rS1, ob, 0, 0) != 0;
-            const bool s1 = s1in || Ti >= pc.Tliq || idx < a.n_sub;
-            float kn, cn;
-            bool s1n;
-            props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
-            if (own) {
-                buf_st(Tnew, rTo, of);
-                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)s1, rS1o, ob, 0, 0);
-                buf_st(kn, rKo, of);
-                buf_st(cn, rCo, of);
-            }
-            if (ACC && own) {
-                const bool s2n = Ti >= pc.Tliq;
-                const bool s2p = a.S2prev[idx] != 0;
-                const bool reset = (!s2p) && s2n;
-                const int lx0 = d->l0_lo3[0] + x, ly0 = d->l0_lo3[1] + yy, lz0 = d->l0_lo3[2] + zz;
-                if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] && lz0 >= 0 &&
-                    lz0 < d->l0_n[2]) {
-                    const int64_t l0 =
-                        (int64_t)lx0 + (int64_t)d->l0_n[0] * (ly0 + (int64_t)d->l0_n[1] * lz0);
-                    const float racc = a.accum[l0], rmax = a.max_accum[l0];
-                    const float rr = reset ? racc : 0.0f;
-                    a.max_accum[l0] = rr > rmax ? rr : rmax;
-                    a.accum[l0] = racc + row[5] * (s2n ? 1.0f : 0.0f) - rr;
-                }
-                a.S2out[idx] = (uint8_t)s2n;
-            }
-        }
+// Melt-time state between Level0 and the L3 nodes (the L3 window maps one to
+// one onto Level0 nodes, cF:3568-3578): to_l3 = 1 gathers, 0 scatters back
+// (nodes of the window outside Level0 are left alone / read 0).
+__global__ void k_acc_l3(const GmStepDesc* __restrict__ d, float* __restrict__ accum,
+                         float* __restrict__ max_accum, float* __restrict__ acc3,
+                         float* __restrict__ macc3, int64_t n3, int to_l3) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n3) return;
+    const gm_grid& g3 = d->g[3];
+    int ix, iy, iz;
+    dec3(t, g3.n[0], g3.n[1], ix, iy, iz);
+    const int lx = d->l0_lo3[0] + ix, ly = d->l0_lo3[1] + iy, lz = d->l0_lo3[2] + iz;
+    const bool in0 = lx >= 0 && lx < d->l0_n[0] && ly >= 0 && ly < d->l0_n[1] && lz >= 0 &&
+                     lz < d->l0_n[2];
+    const int64_t l0 = in0 ? nid(lx, ly, lz, d->l0_n[0], d->l0_n[1]) : 0;
+    if (to_l3) {
+        acc3[t] = in0 ? accum[l0] : 0.0f;
+        macc3[t] = in0 ? max_accum[l0] : 0.0f;
+    } else if (in0) {
+        accum[l0] = acc3[t];
+        max_accum[l0] = macc3[t];
     }
 }
 
-// Same step as a rolled z-march: one wave owns 62 x-columns, YB rows and ZL
-// consecutive planes and walks up the planes with the three node planes
-// z-1, z, z+1 rotating through registers (plane z+2 is prefetched while z is
-// computed).  The loop body is a few KB of code, so the instruction cache
-// serves every plane after the first; a fully unrolled brick would stream
-// tens of KB of instructions per wave.
-template <int YB>
-struct DppPlane {
-    float T[YB + 2], K[YB + 2], C[YB + 2];
-};
-
-template <int YB>
-__device__ __forceinline__ void dpp_load(DppPlane<YB>& P, const LSArgs& a, int x, bool xin,
-                                         int y0, int zz, int nx, int ny, int nz, int nxy,
-                                         int nn) {
-#pragma unroll
-    for (int r = 0; r < YB + 2; ++r) {
-        const int yy = y0 - 1 + r;
-        P.T[r] = P.K[r] = P.C[r] = 0.0f;
-        if (xin && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
-            const int idx = x + nx * yy + nxy * zz;
-            P.T[r] = a.T[idx];
-            P.K[r] = a.kcin[idx];
-            P.C[r] = a.kcin[nn + idx];
-        }
-    }
-}
-
-template <int YB>
-__device__ __forceinline__ void dpp_quads(const DppPlane<YB>& P, bool ein, int y0, int ney,
-                                          float (&Qk)[YB + 1], float (&Qc)[YB + 1]) {
-#pragma unroll
-    for (int r = 0; r < YB + 1; ++r) {
-        const int yy = y0 - 1 + r;
-        const bool ok = ein && yy >= 0 && yy < ney;
-        const float ak = P.K[r] + P.K[r + 1];
-        const float ac = P.C[r] + P.C[r + 1];
-        const float qk = ak + lane_next(ak);
-        const float qc = ac + lane_next(ac);
-        Qk[r] = ok ? qk : 0.0f;
-        Qc[r] = ok ? qc : 0.0f;
-    }
-}
-
-template <bool SRC, bool ACC, int YB>
-__global__ void __launch_bounds__(64) k_step_roll(LSArgs a, const GmStepDesc* __restrict__ d,
-                                                  const DevMat* __restrict__ m, int ZL) {
-    const gm_grid g = d->g[a.level];
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const int nxy = nx * ny;
-    const int nn = nxy * nz;
-    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
-    const int nst = (nx + DXS - 1) / DXS, nzc = (nz + ZL - 1) / ZL;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int zc = t % nzc, rest = t / nzc;
-    const int st = rest % nst, yb = rest / nst;
-    const int lane = threadIdx.x;
-    const int x = DXS * st - 1 + lane;
-    const int y0 = yb * YB, z0 = zc * ZL;
-    const int z1 = min(z0 + ZL, nz);
-    const bool xin = x >= 0 && x < nx;
-    const bool ein = x >= 0 && x < nex;
-    const bool olane = lane >= 1 && lane <= DXS && xin;
-
-    __shared__ float sTab[2 * LUTW];
-    stage_tables(m, sTab, lane, 64);
-    const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
-    float dt = a.dt;
-    if (a.dt_mode == GM_DT_ROW) dt = row[5];
-    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
-    const float mfac = a.mrow / dt;
-    float sx = 0.0f, sy = 0.0f;
-    if (SRC) {
-        sx = xin ? src_axis(g, 0, x, row[0], m->rsq, m->rc) * (m->pc6 * row[6] * a.w3) : 0.0f;
-        // lane j < YB holds the y factor of row y0 + j
-        const int yy = y0 + (lane < YB ? lane : 0);
-        sy = yy < ny ? src_axis(g, 1, yy, row[1], m->rsq, m->rc) : 0.0f;
-    }
-    PropC pc = load_pc(m);
-    const float Tamb = m->T_amb;
-
-    DppPlane<YB> A, B, C, D;
-    dpp_load<YB>(A, a, x, xin, y0, z0 - 1, nx, ny, nz, nxy, nn);
-    dpp_load<YB>(B, a, x, xin, y0, z0, nx, ny, nz, nxy, nn);
-    dpp_load<YB>(C, a, x, xin, y0, z0 + 1, nx, ny, nz, nxy, nn);
-    float QkA[YB + 1], QcA[YB + 1], QkB[YB + 1], QcB[YB + 1], QkC[YB + 1], QcC[YB + 1];
-    dpp_quads<YB>(A, ein, y0, ney, QkA, QcA);
-    dpp_quads<YB>(B, ein, y0, ney, QkB, QcB);
-    // element layer below plane z0 (min corner z0 - 1)
-    float EkL[YB + 1], EcL[YB + 1], EkU[YB + 1], EcU[YB + 1];
-    {
-        const int ez = z0 - 1;
-        const float f = (ez >= 0 && ez < nez) ? 0.125f : 0.0f;
-#pragma unroll
-        for (int r = 0; r < YB + 1; ++r) {
-            EkL[r] = (QkA[r] + QkB[r]) * f;
-            EcL[r] = (QcA[r] + QcB[r]) * f;
-        }
-    }
-    lds_barrier();  // property tables staged
-
-#pragma unroll 1
-    for (int zz = z0; zz < z1; ++zz) {
-        // prefetch plane zz + 2; per-node inputs of plane zz
-        dpp_load<YB>(D, a, x, xin, y0, zz + 2, nx, ny, nz, nxy, nn);
-        float rFa[YB], rFb[YB], rAcc[YB], rMax[YB];
-        bool rS1[YB], rS2[YB];
-        int64_t rl0[YB];
-#pragma unroll
-        for (int j = 0; j < YB; ++j) {
-            const int yy = y0 + j;
-            const bool ok = olane && yy < ny;
-            const int idx = x + nx * yy + nxy * zz;
-            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
-            rFa[j] = rFb[j] = rAcc[j] = rMax[j] = 0.0f;
-            rS1[j] = rS2[j] = false;
-            rl0[j] = -1;
-            if (ok) {
-                if (face) {
-                    rFa[j] = a.bcA[idx];
-                    rFb[j] = a.bcB[idx];
-                }
-                rS1[j] = ((const uint8_t*)a.S1)[idx] != 0;
-                if (ACC) {
-                    rS2[j] = a.S2prev[idx] != 0;
-                    const int lx0 = d->l0_lo3[0] + x, ly0 = d->l0_lo3[1] + yy,
-                              lz0 = d->l0_lo3[2] + zz;
-                    if (lx0 >= 0 && lx0 < d->l0_n[0] && ly0 >= 0 && ly0 < d->l0_n[1] &&
-                        lz0 >= 0 && lz0 < d->l0_n[2]) {
-                        rl0[j] = (int64_t)lx0 +
-                                 (int64_t)d->l0_n[0] * (ly0 + (int64_t)d->l0_n[1] * lz0);
-                        rAcc[j] = a.accum[rl0[j]];
-                        rMax[j] = a.max_accum[rl0[j]];
-                    }
-                }
-            }
-        }
-        // element layer above plane zz
-        dpp_quads<YB>(C, ein, y0, ney, QkC, QcC);
-        {
-            const float f = zz < nez ? 0.125f : 0.0f;
-#pragma unroll
-            for (int r = 0; r < YB + 1; ++r) {
-                EkU[r] = (QkB[r] + QkC[r]) * f;
-                EcU[r] = (QcB[r] + QcC[r]) * f;
-            }
-        }
-        float szv = 0.0f;
-        if (SRC) szv = src_axis(g, 2, zz, row[2], m->dsq, m->dc);
-        // top surface flux per element face (min corner x, row y0 - 1 + r)
-        float Fq[YB + 1][4];
-        const bool top = zz == nez;
-        if (top) {
-#pragma unroll
-            for (int r = 0; r < YB + 1; ++r) {
-                const float t0 = B.T[r], t3 = B.T[r + 1];
-                const float t1 = lane_next(t0), t2 = lane_next(t3);
-                const int yy = y0 - 1 + r;
-                const bool ok = ein && yy >= 0 && yy < ney;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float Tq = n2(q, 0) * t0 + n2(q, 1) * t1 + n2(q, 2) * t2 + n2(q, 3) * t3;
-                    Fq[r][q] = ok ? face_flux(*m, Tq) : 0.0f;
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < YB; ++j) {
-            const int yy = y0 + j;
-            const float k100 = EkL[j], k110 = EkL[j + 1];
-            const float k101 = EkU[j], k111 = EkU[j + 1];
-            const float k000 = lane_prev(k100), k010 = lane_prev(k110);
-            const float k001 = lane_prev(k101), k011 = lane_prev(k111);
-            const float cs = (EcL[j] + EcL[j + 1]) + (EcU[j] + EcU[j + 1]);
-            const float Msum = cs + lane_prev(cs);
-            const float Ti = B.T[j + 1];
-            const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011,
-                        Pz11 = k110 + k111;
-            const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011,
-                        Py11 = k101 + k111;
-            const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101,
-                        Px11 = k011 + k111;
-            const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
-            const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
-            const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
-            float KT = a.kc[0] * ((Fx0 + Fx1) * Ti);
-            KT += a.kc[1] * (Fx0 * lane_prev(Ti) + Fx1 * lane_next(Ti));
-            KT += a.kc[2] * (Fy0 * B.T[j] + Fy1 * B.T[j + 2]);
-            KT += a.kc[4] * (Fz0 * A.T[j + 1] + Fz1 * C.T[j + 1]);
-            KT += a.kc[3] * (Pz00 * lane_prev(B.T[j]) + Pz10 * lane_next(B.T[j]) +
-                             Pz01 * lane_prev(B.T[j + 2]) + Pz11 * lane_next(B.T[j + 2]));
-            KT += a.kc[5] * (Py00 * lane_prev(A.T[j + 1]) + Py10 * lane_next(A.T[j + 1]) +
-                             Py01 * lane_prev(C.T[j + 1]) + Py11 * lane_next(C.T[j + 1]));
-            KT += a.kc[6] * (Px00 * A.T[j] + Px10 * A.T[j + 2] + Px01 * C.T[j] +
-                             Px11 * C.T[j + 2]);
-            KT += a.kc[7] * (k000 * lane_prev(A.T[j]) + k100 * lane_next(A.T[j]) +
-                             k010 * lane_prev(A.T[j + 2]) + k110 * lane_next(A.T[j + 2]) +
-                             k001 * lane_prev(C.T[j]) + k101 * lane_next(C.T[j]) +
-                             k011 * lane_prev(C.T[j + 2]) + k111 * lane_next(C.T[j + 2]));
-            const float M = Msum * mfac;
-            float F = 0.0f;
-            if (SRC)
-                F = sx * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sy), j)) * szv;
-            if (top) {
-                float cr = 0.0f;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int bxe = e & 1, bye = e >> 1;
-                    const int ai = bye ? (bxe ? 0 : 1) : (bxe ? 3 : 2);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const float v = Fq[j + bye][q];
-                        const float vv = bxe ? v : lane_prev(v);
-                        cr += n2(q, ai) * (vv * a.w2);
-                    }
-                }
-                F += cr;
-            }
-            float Tnew = ((M * Ti - KT) + F) / M;
-            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
-            if (face) Tnew = a.alpha * rFa[j] + (1.0f - a.alpha) * rFb[j];
-            if (a.clamp && Tnew < Tamb) Tnew = Tamb;
-            const bool ok = olane && yy < ny;
-            const int idx = x + nx * yy + nxy * zz;
-            const bool s1 = rS1[j] || Ti >= pc.Tliq || idx < a.n_sub;
-            if (ok) {
-                a.Tout[idx] = Tnew;
-                a.S1out[idx] = (uint8_t)s1;
-                float kn, cn;
-                bool s1n;
-                props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
-                a.kcout[idx] = kn;
-                a.kcout[nn + idx] = cn;
-                if (ACC) {
-                    const bool s2n = Ti >= pc.Tliq;
-                    const bool reset = (!rS2[j]) && s2n;
-                    if (rl0[j] >= 0) {
-                        const float rr = reset ? rAcc[j] : 0.0f;
-                        a.max_accum[rl0[j]] = rr > rMax[j] ? rr : rMax[j];
-                        a.accum[rl0[j]] = rAcc[j] + row[5] * (s2n ? 1.0f : 0.0f) - rr;
-                    }
-                    a.S2out[idx] = (uint8_t)s2n;
-                }
-            }
-        }
-        // rotate
-        A = B;
-        B = C;
-        C = D;
-#pragma unroll
-        for (int r = 0; r < YB + 1; ++r) {
-            QkB[r] = QkC[r];
-            QcB[r] = QcC[r];
-            EkL[r] = EkU[r];
-            EcL[r] = EcU[r];
-        }
-    }
+void launch_acc_l3(const GmStepDesc* d, float* accum, float* max_accum, float* acc3, float* macc3,
+                   int64_t n3, int to_l3, hipStream_t s) {
+    hipLaunchKernelGGL(k_acc_l3, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, d, accum,
+                       max_accum, acc3, macc3, n3, to_l3);
 }
 
 void launch_level_step_kc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg,
