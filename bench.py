@@ -11,8 +11,9 @@ by exactly K timed steps; the default K = 200 is the deck's 20,000 L3 steps.
 
 value = laser path length with the laser on over the timed rows / wall time.
 The JSON line also carries element updates per second, the roofline of the
-L3 level-step kernel (HBM bytes per launch from the algorithmic 10 B/node,
-duration from HIP events on the engine stream) and the CPU baseline (the
+L3 level-step kernel (algorithmic 10 B/node over the mean launch duration,
+measured here with HIP events on the engine stream; HBM traffic per launch
+from the FETCH_SIZE/WRITE_SIZE profile in profiles/) and the CPU baseline (the
 NumPy oracle on one core, a bounded sample of the same L1 step, extrapolated).
 """
 
@@ -30,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "go-melt_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "l3_step_hbm_traffic.txt")
 METRIC = "scan mm per wall-s, plain vs surrogate (≤2% melt-pool err); % HBM peak"
 
 
@@ -115,6 +117,11 @@ def main():
     nn3 = L[3]["nn"]
     bytes_per_launch = 10 * nn3          # FP32 T read + write, uint8 S1 read + write
     achieved = bytes_per_launch / (ms.value * 1e-3) / 1e9
+    traffic = None                       # HBM bytes per launch from FETCH_SIZE/WRITE_SIZE runs
+    if os.path.exists(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = int(line.split(":")[1])
     mm_T = np.asarray(L[3]["T0"])
     result = {
         "metric": METRIC,
@@ -141,13 +148,13 @@ def main():
             "device": N.device_name(),
         },
         "roofline": {
-            "kernel": "k_level_step<false> (L3)",
+            "kernel": "k_step_dpp (L3 level step)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": bytes_per_launch,
             "us_per_launch": ms.value * 1000.0,
         },
