@@ -542,6 +542,11 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
         return fail("subcycle counts exceed the engine limits (N2 <= 64, N2*N3 <= 1024)");
     }
     for (int l = 0; l < 4; ++l) e->nn[l] = grid_nn(cfg->level[l]);
+    // the L2/L3 subcycle step addresses k|cp companions with 32-bit offsets
+    if (e->nn[2] >= ((int64_t)1 << 28) || e->nn[3] >= ((int64_t)1 << 28)) {
+        delete e;
+        return fail("Level2/Level3 meshes are limited to 2^28 nodes");
+    }
     // the level-step kernel takes k in W/(m K) and cp in J/(kg K): the 1/1000
     // of cF:2602 and rho of cF:2612 go into the stencil and mass constants
     for (int l = 1; l < 4; ++l) {
@@ -963,6 +968,7 @@ static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8
     a.mrow = e->mrow[2];
     a.w2 = e->w2[2];
     a.w3 = e->w3[2];
+    a.prec_ok = e->dm.pnb > 0;
     launch_level_step_kc(a, e->d_desc, e->cfg.level[2], e->d_dm, e->s);
 }
 
@@ -999,6 +1005,7 @@ static void run_l3_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8
     a.mrow = e->mrow[3];
     a.w2 = e->w2[3];
     a.w3 = e->w3[3];
+    a.prec_ok = e->dm.pnb > 0;
     launch_level_step_kc(a, e->d_desc, e->cfg.level[3], e->d_dm, e->s);
 }
 

@@ -40,6 +40,7 @@ struct GmStepDesc {
 
 // Device copy of the material with derived constants (kernel argument).
 #define GM_LUT_BINS 64
+#define GM_PREC_BINS 32
 struct DevMat {
     float T_amb, T_sol, T_liq, T_boil;
     float k_p, k_a0, k_a1, k_f;
@@ -47,20 +48,25 @@ struct DevMat {
     int32_t k_n, cp_n;
     float kT[GM_MAX_KNOTS], kv[GM_MAX_KNOTS], ks[GM_MAX_KNOTS];
     float cT[GM_MAX_KNOTS], cv[GM_MAX_KNOTS], cs[GM_MAX_KNOTS];
-    float h_conv, sigma_eps, evc_cp, CT, CM, Lev, invTb, Tclip, Tamb4;
+    float h_conv, sigma_eps, evc_cp, CT, CM, Lev, invTb, Tclip, Tamb4, sqCM;
     float eta, rsq, dsq, rc, dc, pc6;  // pc6 = 6*sqrt(3)*eta
-    // binned lookup of the table segment: bin b = (T - lut_T0) * lut_iw,
-    // segment = lut[b], then one step up if T passed the next knot
-    float lut_T0[2], lut_iw[2];
-    int32_t lut_nb[2];
-    int32_t lut[2][GM_LUT_BINS];
-    // both tables on the union of their knots (when both are given): one
-    // segment search yields k and cp; mseg[j] = (T_j, k(T_j), k slope,
-    // cp(T_j), cp slope) on [T_j, T_j+1), linear extension at the ends
+    // k and cp on the union of the table knots (a property without a table
+    // contributes its linear law): mseg[j] = (T_j, k(T_j), k slope, cp(T_j),
+    // cp slope) on [T_j, T_j+1), linear extension at the ends; mn = 0 when
+    // neither property has a table.  Segment of T: start at mlut[b] of the
+    // bin b = (T - mT0) * miw, step up while T passed the next knot.
     int32_t mn, mnb;
     float mT0, miw;
     int32_t mlut[GM_LUT_BINS];
     float mseg[2 * GM_MAX_KNOTS][5];
+    // Per-bin segment records (pnb > 0 when every bin of width 1/piw over
+    // the knots, widened by a quarter bin on each side, meets at most two
+    // segments A and A+1).  prec[b] = (T_{A+1} or +inf, T_A, k(T_A), k slope,
+    // cp(T_A), cp slope, k(T_{A+1}), k slope, cp(T_{A+1}), cp slope, 0, 0):
+    // the segment of T in bin b is A+1 when T >= prec[b][0], else A.
+    int32_t pnb;
+    float pT0, piw;
+    float prec[GM_PREC_BINS][12];
 };
 
 // One axis of the tabulated restriction between a fine and a coarse mesh
@@ -148,6 +154,7 @@ struct LSArgs {
     // tabulated separable L3 source factors of the desc rows: row r at
     // srcT + r * (nx + ny + nz): x factors (times 6 sqrt(3) eta P w3), then y, z
     const float* srcT;
+    int prec_ok;          // the material has per-bin property records (pnb > 0)
 };
 
 // Restriction (fine -> coarse) arguments.

@@ -216,30 +216,21 @@ __device__ __forceinline__ void lds_barrier() {
 #define QX (LX + 1)
 #define QY (LY + 1)
 #define QN (QX * QY)
-#define LUTW (GM_LUT_BINS + 3 * GM_MAX_KNOTS)  // LDS words per property table
-
-// Piecewise-linear property table through its binned segment lookup (LDS).
-__device__ __forceinline__ float tab_lds(const float* __restrict__ t, float T0, float iw, int nb,
-                                         int n, float T) {
-    int b = (int)((T - T0) * iw);
-    b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
-    int j = __float_as_int(t[b]);
-    const float* sg = t + GM_LUT_BINS;
-    while (j + 1 < n - 1 && T >= sg[3 * (j + 1) + 1]) ++j;
-    return sg[3 * j] + sg[3 * j + 2] * (T - sg[3 * j + 1]);
-}
+// LDS words of the staged property table: the per-bin segment records, or
+// the loop lookup (bin -> first segment, then the segment rows).
+#define PTW (GM_PREC_BINS * 12 > GM_LUT_BINS + 10 * GM_MAX_KNOTS ? GM_PREC_BINS * 12 \
+                                                                 : GM_LUT_BINS + 10 * GM_MAX_KNOTS)
 
 // computeStateProperties (cF:2584-2612) for one node, without the unit
 // factors: k in W/(m K), cp in J/(kg K) (the 1/1000 of cF:2602 and rho of
 // cF:2612 are folded into the host-side stencil and mass constants).  With
 // S1, S2, S3 in {0,1} the reference's sums of products are exact selects.
 struct PropC {
-    float Tliq, Tsol, kp, kf, ka0, ka1, cpf, cpm, ca0, ca1;
-    int kn, cn;
-    float kT0, kiw, cT0, ciw;
-    int knb, cnb;
-    int mn, mnb;  // merged table (mn > 0: both tables on one knot list)
+    float Tliq, Tsol, kp, kf, cpf, cpm;
+    int mn, mnb;   // loop lookup (used when pnb == 0)
     float mT0, miw;
+    int pnb;       // per-bin records
+    float pT0, piw;
 };
 
 __device__ __forceinline__ PropC load_pc(const DevMat* __restrict__ m) {
@@ -248,74 +239,73 @@ __device__ __forceinline__ PropC load_pc(const DevMat* __restrict__ m) {
     pc.Tsol = m->T_sol;
     pc.kp = m->k_p;
     pc.kf = m->k_f;
-    pc.ka0 = m->k_a0;
-    pc.ka1 = m->k_a1;
     pc.cpf = m->cp_f;
     pc.cpm = m->cp_m;
-    pc.ca0 = m->cp_a0;
-    pc.ca1 = m->cp_a1;
-    pc.kn = m->k_n;
-    pc.cn = m->cp_n;
-    pc.kT0 = m->lut_T0[0];
-    pc.kiw = m->lut_iw[0];
-    pc.cT0 = m->lut_T0[1];
-    pc.ciw = m->lut_iw[1];
-    pc.knb = m->lut_nb[0];
-    pc.cnb = m->lut_nb[1];
     pc.mn = m->mn;
     pc.mnb = m->mnb;
     pc.mT0 = m->mT0;
     pc.miw = m->miw;
+    pc.pnb = m->pnb;
+    pc.pT0 = m->pT0;
+    pc.piw = m->piw;
     return pc;
 }
 
-// Property tables into LDS (2 * LUTW floats): the merged table when both
-// tables are given, else the two separate ones.
+// Property table into LDS (PTW floats): the records when the material has
+// them, else the loop lookup.
 __device__ __forceinline__ void stage_tables(const DevMat* __restrict__ m, float* sTab, int tid,
                                              int nthr) {
-    if (m->mn > 0) {
-        const int n = GM_LUT_BINS + 5 * m->mn;
-        for (int l = tid; l < n; l += nthr)
-            sTab[l] = l < GM_LUT_BINS ? __int_as_float(m->mlut[l])
-                                      : m->mseg[(l - GM_LUT_BINS) / 5][(l - GM_LUT_BINS) % 5];
+    if (m->pnb > 0) {
+        const float4* src = (const float4*)&m->prec[0][0];
+        float4* dst = (float4*)sTab;
+        for (int l = tid; l < GM_PREC_BINS * 3; l += nthr) dst[l] = src[l];
         return;
     }
-    for (int l = tid; l < 2 * LUTW; l += nthr) {
-        const int w = l / LUTW, r = l % LUTW;
-        float v;
-        if (r < GM_LUT_BINS) {
-            v = __int_as_float(m->lut[w][r]);
-        } else {
-            const int j = (r - GM_LUT_BINS) / 3, f = (r - GM_LUT_BINS) % 3;
-            const float* src = f == 0 ? (w ? m->cv : m->kv) : f == 1 ? (w ? m->cT : m->kT)
-                                                                      : (w ? m->cs : m->ks);
-            v = src[j];
-        }
-        sTab[l] = v;
-    }
+    const int n = GM_LUT_BINS + 5 * m->mn;
+    for (int l = tid; l < n; l += nthr)
+        sTab[l] = l < GM_LUT_BINS ? __int_as_float(m->mlut[l])
+                                  : m->mseg[(l - GM_LUT_BINS) / 5][(l - GM_LUT_BINS) % 5];
+}
+
+__device__ __forceinline__ void phase_select(const PropC& c, float T, bool s1in, bool substrate,
+                                             float kb, float cs, float& kk, float& cp, bool& s1) {
+    const bool s2 = T >= c.Tliq;
+    const bool s3 = (T > c.Tsol) && (T < c.Tliq);
+    s1 = s1in || s2 || substrate;
+    kk = s2 ? c.kf : (s1 ? kb : c.kp);
+    cp = s2 ? c.cpf : (s3 ? c.cpm : cs);
+}
+
+// Record lookup: one bin read, one compare, no loop.
+__device__ __forceinline__ void props_rec(const PropC& c, const float* __restrict__ tab, float T,
+                                          bool s1in, bool substrate, float& kk, float& cp,
+                                          bool& s1) {
+    int b = (int)((T - c.pT0) * c.piw);
+    b = b < 0 ? 0 : (b > c.pnb - 1 ? c.pnb - 1 : b);
+    const float4* r = (const float4*)(tab + 12 * b);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+    const bool up = T >= r0.x;
+    const float Ts = up ? r0.x : r0.y;
+    const float k0 = up ? r1.z : r0.z, ks = up ? r1.w : r0.w;
+    const float c0 = up ? r2.x : r1.x, cs = up ? r2.y : r1.y;
+    const float dT = T - Ts;
+    phase_select(c, T, s1in, substrate, k0 + ks * dT, c0 + cs * dT, kk, cp, s1);
 }
 
 __device__ __forceinline__ void props_c(const PropC& c, const float* __restrict__ tab, float T,
                                         bool s1in, bool substrate, float& kk, float& cp, bool& s1) {
-    const bool s2 = T >= c.Tliq;
-    const bool s3 = (T > c.Tsol) && (T < c.Tliq);
-    s1 = s1in || s2 || substrate;
-    float kb, cs;
-    if (c.mn > 0) {
-        int b = (int)((T - c.mT0) * c.miw);
-        b = b < 0 ? 0 : (b > c.mnb - 1 ? c.mnb - 1 : b);
-        int j = __float_as_int(tab[b]);
-        const float* sg = tab + GM_LUT_BINS;
-        while (j + 1 < c.mn - 1 && T >= sg[5 * (j + 1)]) ++j;
-        const float dT = T - sg[5 * j];
-        kb = sg[5 * j + 1] + sg[5 * j + 2] * dT;
-        cs = sg[5 * j + 3] + sg[5 * j + 4] * dT;
-    } else {
-        kb = c.kn ? tab_lds(tab, c.kT0, c.kiw, c.knb, c.kn, T) : (c.ka1 * T + c.ka0);
-        cs = c.cn ? tab_lds(tab + LUTW, c.cT0, c.ciw, c.cnb, c.cn, T) : (c.ca1 * T + c.ca0);
+    if (c.pnb > 0) {
+        props_rec(c, tab, T, s1in, substrate, kk, cp, s1);
+        return;
     }
-    kk = s2 ? c.kf : (s1 ? kb : c.kp);
-    cp = s2 ? c.cpf : (s3 ? c.cpm : cs);
+    int b = (int)((T - c.mT0) * c.miw);
+    b = b < 0 ? 0 : (b > c.mnb - 1 ? c.mnb - 1 : b);
+    int j = __float_as_int(tab[b]);
+    const float* sg = tab + GM_LUT_BINS;
+    while (j + 1 < c.mn - 1 && T >= sg[5 * (j + 1)]) ++j;
+    const float dT = T - sg[5 * j];
+    phase_select(c, T, s1in, substrate, sg[5 * j + 1] + sg[5 * j + 2] * dT,
+                 sg[5 * j + 3] + sg[5 * j + 4] * dT, kk, cp, s1);
 }
 
 // Modes fixed at compile time: S1 storage, laser source, Dirichlet kind,
@@ -329,7 +319,7 @@ __global__ void __launch_bounds__(256) k_level_step(LSArgs a, const GmStepDesc* 
     __shared__ float sSk[2][QN], sSc[2][QN];  // x-y quad sums of k, cp (ring)
     __shared__ float sKe[2][QN], sCe[2][QN];  // element planes below / above (ring)
     __shared__ float sSrc[LX + LY + ZCT];
-    __shared__ float sTab[2 * LUTW];
+    __shared__ float sTab[PTW] __attribute__((aligned(16)));
 
     const gm_grid g = a.use_desc ? d->g[a.level] : a.grid;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -657,12 +647,16 @@ void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, 
 //                time-blended coarse field on the Dirichlet faces
 //   k+, cp+    : properties of T+ with S1 | (T+ >= T_liq), i.e. what the next
 //                step's computeStateProperties returns.
-// One wave owns a brick of 62 x-columns, YB rows and ZC planes: lane l holds
-// x = 62 s - 1 + l (lanes 0 and 63 only supply the x halo), every y and z
-// neighbour of its column is in registers, and x neighbours come from the
-// adjacent lanes through DPP wave shifts.  The brick is unrolled: with few
-// waves per SIMD on a 0.5M-node mesh, independent nodes in one wave hide
-// latency that a rolled loop or LDS barriers expose.
+// A wave owns DXS consecutive positions of the plane numbering
+// q = x + nx * band (bands of YB rows), over ZC planes: lane l holds
+// q = DXS s - 1 + l (lanes 0 and 63 only supply the halo).  Consecutive q
+// run across band ends, so the lanes of a wave stay busy whatever nx is; the
+// pair (nx-1, next band's 0) is no element, so its coefficients are zero.
+// Every y and z neighbour of a lane's nodes is in its registers and x
+// neighbours come from the adjacent lanes through DPP wave shifts.  Global
+// memory goes through buffer resources with 32-bit offsets: positions
+// outside the mesh read zero and their stores are dropped (offset past the
+// end), so the body has no branches and the nodes of a brick interleave.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float lane_prev(float v) {  // value of lane l-1 (0 in lane 0)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
@@ -671,9 +665,41 @@ __device__ __forceinline__ float lane_next(float v) {  // value of lane l+1 (0 i
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
-#define DXS 62  // output columns per wave
+#define DXS 62             // output positions per wave
+#define GM_BUF_DW3 0x00020000  // buffer resource word 3 (gfx9 family, raw dword access)
+#define GM_OOB 0x80000000u     // an offset past the end of every buffer
 
-template <bool SRC, bool ACC, int YB, int ZC>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, GM_BUF_DW3);
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff = 0) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                       uint32_t soff = 0) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ uint32_t buf_ld8(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, voff, 0, 0);
+}
+__device__ __forceinline__ void buf_st8(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff) {
+    __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, r, voff, 0, 0);
+}
+
+// Top-face flux density of face_flux with hardware reciprocal, reciprocal
+// square root and exponential (relative error a few 1e-7).
+__device__ __forceinline__ float face_flux_hw(const DevMat& m, float Tq) {
+    Tq = (Tq > m.Tclip) ? m.Tclip : Tq;
+    const float invT = __builtin_amdgcn_rcpf(Tq);
+    const float E = m.Lev + m.cp_f * (Tq - m.T_amb);
+    const float mol = m.sqCM * __builtin_amdgcn_rsqf(Tq);
+    const float S = m.evc_cp * __expf(-m.CT * (invT - m.invTb)) * mol * E;
+    const float t2 = Tq * Tq;
+    const float q = m.h_conv * (m.T_amb - Tq) + m.sigma_eps * (m.Tamb4 - t2 * t2) - S;
+    return q * 1e-6f;
+}
+
+template <bool SRC, bool ACC, bool REC, int YB, int ZC>
 __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __restrict__ d,
                                                  const DevMat* __restrict__ m) {
     const gm_grid g = d->g[a.level];
@@ -681,99 +707,118 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
     const int nxy = nx * ny;
     const int nn = nxy * nz;
     const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
-    const int nst = (nx + DXS - 1) / DXS, nzc = (nz + ZC - 1) / ZC;
+    const int nband = (ny + YB - 1) / YB;
+    const int nq = nx * nband;
+    const int nzc = (nz + ZC - 1) / ZC;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int zc = t % nzc, rest = t / nzc;
-    const int st = rest % nst, yb = rest / nst;
+    const int zc = t % nzc, qs = t / nzc;
     const int lane = threadIdx.x;
-    const int x = DXS * st - 1 + lane;
-    const int y0 = yb * YB, z0 = zc * ZC;
-    const bool xin = x >= 0 && x < nx;
-    const bool ein = x >= 0 && x < nex;  // element with min corner x exists in x
-    const bool olane = lane >= 1 && lane <= DXS && xin;
-    __shared__ float sTab[2 * LUTW];
-    stage_tables(m, sTab, lane, 64);
+    const int q = DXS * qs - 1 + lane;
+    const bool qv = q >= 0 && q < nq;
+    const int band = qv ? (int)((unsigned)q / (unsigned)nx) : 0;
+    const int x = qv ? q - band * nx : -1;
+    const int y0 = band * YB, z0 = zc * ZC;
+    const bool ein = qv && x < nex;  // element with min corner x exists in x
+    const bool olane = lane >= 1 && lane <= DXS && qv;
+    const int base = x + nx * y0 + nxy * z0;  // node (x, y0, z0)
+    const uint32_t nb4 = (uint32_t)nn * 4u;
 
     // ---- loads: planes z0-1 .. z0+ZC, rows y0-1 .. y0+YB ---------------------
+    const auto rT = buf_rsrc(a.T, nb4), rKC = buf_rsrc(a.kcin, 2u * nb4);
     float Tw[ZC + 2][YB + 2], Kw[ZC + 2][YB + 2], Cw[ZC + 2][YB + 2];
+    bool rowok[YB + 2];
+#pragma unroll
+    for (int r = 0; r < YB + 2; ++r) {
+        const int yy = y0 - 1 + r;
+        rowok[r] = qv && yy >= 0 && yy < ny;
+    }
 #pragma unroll
     for (int p = 0; p < ZC + 2; ++p) {
         const int zz = z0 - 1 + p;
+        const bool pok = zz >= 0 && zz < nz;
 #pragma unroll
         for (int r = 0; r < YB + 2; ++r) {
-            const int yy = y0 - 1 + r;
-            Tw[p][r] = Kw[p][r] = Cw[p][r] = 0.0f;
-            if (xin && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
-                const int idx = x + nx * yy + nxy * zz;
-                Tw[p][r] = a.T[(unsigned)idx];
-                Kw[p][r] = a.kcin[(unsigned)idx];
-                Cw[p][r] = a.kcin[(unsigned)(nn + idx)];
-            }
+            const uint32_t vo =
+                (pok && rowok[r]) ? 4u * (uint32_t)(base + nx * (r - 1) + nxy * (p - 1)) : GM_OOB;
+            Tw[p][r] = buf_ld(rT, vo);
+            Kw[p][r] = buf_ld(rKC, vo);
+            Cw[p][r] = buf_ld(rKC, vo, nb4);
         }
     }
     // per-node inputs of the outputs
-    float rFa[ZC][YB], rFb[ZC][YB], rAcc[ZC][YB], rMax[ZC][YB];
-    bool rS1[ZC][YB], rS2[ZC][YB];
+    const auto rS1 = buf_rsrc(a.S1, (uint32_t)nn);
+    const auto rFa = buf_rsrc(a.bcA, nb4), rFb = buf_rsrc(a.bcB, nb4);
+    float rFa_[ZC][YB], rFb_[ZC][YB], rAcc[ZC][YB], rMax[ZC][YB];
+    uint32_t rS1_[ZC][YB], rS2_[ZC][YB], oF[ZC][YB], oB[ZC][YB];
+    bool face_[ZC][YB];
 #pragma unroll
     for (int i = 0; i < ZC; ++i) {
 #pragma unroll
         for (int j = 0; j < YB; ++j) {
             const int yy = y0 + j, zz = z0 + i;
             const bool ok = olane && yy < ny && zz < nz;
-            const int idx = x + nx * yy + nxy * zz;
-            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
-            rFa[i][j] = rFb[i][j] = rAcc[i][j] = rMax[i][j] = 0.0f;
-            rS1[i][j] = rS2[i][j] = false;
-            if (ok) {
-                if (face) {
-                    rFa[i][j] = a.bcA[(unsigned)idx];
-                    rFb[i][j] = a.bcB[(unsigned)idx];
-                }
-                rS1[i][j] = ((const uint8_t*)a.S1)[(unsigned)idx] != 0;
-                if (ACC) {
-                    // melt-time state on the L3 nodes (gathered from Level0
-                    // for the step, scattered back after it)
-                    rS2[i][j] = a.S2prev[(unsigned)idx] != 0;
-                    rAcc[i][j] = a.accum[(unsigned)idx];
-                    rMax[i][j] = a.max_accum[(unsigned)idx];
-                }
+            const bool face = x == 0 || x == nex || yy == 0 || yy == ney || zz == 0;
+            const uint32_t idx = (uint32_t)(base + nx * j + nxy * i);
+            face_[i][j] = face;
+            oB[i][j] = ok ? idx : GM_OOB;
+            oF[i][j] = ok ? 4u * idx : GM_OOB;
+            const uint32_t fo = face ? oF[i][j] : GM_OOB;
+            rFa_[i][j] = buf_ld(rFa, fo);
+            rFb_[i][j] = buf_ld(rFb, fo);
+            rS1_[i][j] = buf_ld8(rS1, oB[i][j]);
+            if (ACC) {
+                // melt-time state on the L3 nodes (gathered from Level0 for
+                // the step, scattered back after it)
+                const auto rS2p = buf_rsrc(a.S2prev, (uint32_t)nn);
+                const auto rA = buf_rsrc(a.accum, nb4), rM = buf_rsrc(a.max_accum, nb4);
+                rS2_[i][j] = buf_ld8(rS2p, oB[i][j]);
+                rAcc[i][j] = buf_ld(rA, oF[i][j]);
+                rMax[i][j] = buf_ld(rM, oF[i][j]);
+            } else {
+                rS2_[i][j] = 0;
+                rAcc[i][j] = rMax[i][j] = 0.0f;
             }
         }
     }
+    __shared__ float sTab[PTW] __attribute__((aligned(16)));
+    stage_tables(m, sTab, lane, 64);
+
     const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
     float dt = a.dt;
     if (a.dt_mode == GM_DT_ROW) dt = row[5];
     else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
-    const float mfac = a.mrow / dt;
-    // separable source: x factor per lane; the y and z factors of the brick
-    // are computed one per lane and broadcast
-    float sx = 0.0f, syz = 0.0f;
+    const float rmfac = dt / a.mrow;
+    // separable source: x and y factors per lane, z factors uniform
+    float sx = 0.0f, sy[YB], sz[ZC];
     if (SRC) {
-        if (a.srcT) {
-            // tabulated per row: x factors (with 6 sqrt(3) eta P w3), y, z
-            const float* st = a.srcT + (size_t)a.src_row * (nx + ny + nz);
-            sx = x >= 0 && x < nx ? st[x] : 0.0f;
-            if (lane < YB) {
-                const int yy = y0 + lane;
-                syz = yy < ny ? st[nx + yy] : 0.0f;
-            } else if (lane < YB + ZC) {
-                const int zz = z0 + lane - YB;
-                syz = zz < nz ? st[nx + ny + zz] : 0.0f;
-            }
-        } else {
-            sx = x >= 0 && x < nx
-                     ? src_axis(g, 0, x, row[0], m->rsq, m->rc) * (m->pc6 * row[6] * a.w3)
-                     : 0.0f;
-            if (lane < YB) {
-                const int yy = y0 + lane;
-                syz = yy < ny ? src_axis(g, 1, yy, row[1], m->rsq, m->rc) : 0.0f;
-            } else if (lane < YB + ZC) {
-                const int zz = z0 + lane - YB;
-                syz = zz < nz ? src_axis(g, 2, zz, row[2], m->dsq, m->dc) : 0.0f;
-            }
-        }
+        const float* st = a.srcT + (size_t)a.src_row * (nx + ny + nz);
+        sx = qv ? st[x] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < YB; ++j) sy[j] = (qv && y0 + j < ny) ? st[nx + y0 + j] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < ZC; ++i) sz[i] = (z0 + i < nz) ? st[nx + ny + z0 + i] : 0.0f;
     }
-    PropC pc = load_pc(m);
+    // restricted correction on its node box
+    float corr[ZC][YB];
+#pragma unroll
+    for (int i = 0; i < ZC; ++i)
+#pragma unroll
+        for (int j = 0; j < YB; ++j) corr[i][j] = 0.0f;
+    if (a.corr) {
+        const int c0x = a.corr_info[0][0], c1x = a.corr_info[0][1];
+        const int c0y = a.corr_info[1][0], c1y = a.corr_info[1][1];
+        const int c0z = a.corr_info[2][0], c1z = a.corr_info[2][1];
+#pragma unroll
+        for (int i = 0; i < ZC; ++i)
+#pragma unroll
+            for (int j = 0; j < YB; ++j) {
+                const int bx = x - c0x, by = y0 + j - c0y, bz = z0 + i - c0z;
+                if (oB[i][j] != GM_OOB && bx >= 0 && bx <= c1x && by >= 0 && by <= c1y &&
+                    bz >= 0 && bz <= c1z)
+                    corr[i][j] = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
+            }
+    }
+    const PropC pc = load_pc(m);
     const float Tamb = m->T_amb;
 
     // ---- x-y quad sums of k and cp, then element means -----------------------
@@ -805,45 +850,62 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
             Ec[p][r] = (Qc[p][r] + Qc[p + 1][r]) * f;
         }
     }
-    // top surface flux per element face (min corner x, row y0-1+r), when the
-    // top plane lies in this brick
-    float Fq[YB + 1][4];
+    // top surface flux (cF:2207-2301) of the nodes of the top plane, when it
+    // lies in this brick: face Gauss point values per face (min corner x, row
+    // y0-1+r), then each node's share from its four faces
+    float Ftop[YB];
+#pragma unroll
+    for (int j = 0; j < YB; ++j) Ftop[j] = 0.0f;
     const int ptop = nez - (z0 - 1);  // window plane index of the top plane
-    const bool has_top = ptop >= 1 && ptop <= ZC;
-    if (has_top) {
+    if (ptop >= 1 && ptop <= ZC) {
+        float Fq[YB + 1][4];
 #pragma unroll
         for (int r = 0; r < YB + 1; ++r) {
-            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+            float t0 = 0.0f, t3 = 0.0f;
 #pragma unroll
             for (int p = 1; p <= ZC; ++p)
                 if (p == ptop) {
                     t0 = Tw[p][r];
                     t3 = Tw[p][r + 1];
                 }
-            t1 = lane_next(t0);
-            t2 = lane_next(t3);
+            const float t1 = lane_next(t0), t2 = lane_next(t3);
             const int yy = y0 - 1 + r;
             const bool ok = ein && yy >= 0 && yy < ney;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float Tq = n2(q, 0) * t0 + n2(q, 1) * t1 + n2(q, 2) * t2 + n2(q, 3) * t3;
-                Fq[r][q] = ok ? face_flux(*m, Tq) : 0.0f;
+            for (int qd = 0; qd < 4; ++qd) {
+                const float Tq = n2(qd, 0) * t0 + n2(qd, 1) * t1 + n2(qd, 2) * t2 + n2(qd, 3) * t3;
+                const float fl = face_flux_hw(*m, Tq) * a.w2;
+                Fq[r][qd] = ok ? fl : 0.0f;
             }
+        }
+#pragma unroll
+        for (int j = 0; j < YB; ++j) {
+            float cr = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int bxe = e & 1, bye = e >> 1;  // 0: face on the negative side
+                const int ai = bye ? (bxe ? 0 : 1) : (bxe ? 3 : 2);
+#pragma unroll
+                for (int qd = 0; qd < 4; ++qd) {
+                    const float v = Fq[j + bye][qd];
+                    cr += n2(qd, ai) * (bxe ? v : lane_prev(v));
+                }
+            }
+            Ftop[j] = cr;
         }
     }
 
-    lds_barrier();  // property tables staged
+    lds_barrier();  // property table staged
 
     // ---- outputs --------------------------------------------------------------
+    const auto rTo = buf_rsrc(a.Tout, nb4), rKCo = buf_rsrc(a.kcout, 2u * nb4);
+    const auto rS1o = buf_rsrc(a.S1out, (uint32_t)nn);
+    const int nsub = (int)(a.n_sub < (int64_t)nn ? a.n_sub : (int64_t)nn);
 #pragma unroll
     for (int i = 0; i < ZC; ++i) {
-        const int zz = z0 + i;
-        float szv = 0.0f;
-        if (SRC) szv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), YB + i));
 #pragma unroll
         for (int j = 0; j < YB; ++j) {
-            const int yy = y0 + j;
-            // elements around the node: bx (x side), by (y side), bz (z side)
+            // elements around the node: k000 .. k111 by (x, y, z) side
             const float k100 = Ek[i][j], k110 = Ek[i][j + 1];
             const float k101 = Ek[i + 1][j], k111 = Ek[i + 1][j + 1];
             const float k000 = lane_prev(k100), k010 = lane_prev(k110);
@@ -875,59 +937,89 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
                              k010 * lane_prev(Tw[i][j + 2]) + k110 * lane_next(Tw[i][j + 2]) +
                              k001 * lane_prev(Tw[i + 2][j]) + k101 * lane_next(Tw[i + 2][j]) +
                              k011 * lane_prev(Tw[i + 2][j + 2]) + k111 * lane_next(Tw[i + 2][j + 2]));
-            const float M = Msum * mfac;
-            float F = 0.0f;
-            if (SRC) F = sx * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(syz), j)) * szv;
-            if (has_top && i + 1 == ptop) {
-                // elements (x-1 | x, y-1 | y): node's local index ai in each face
-                float cr = 0.0f;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int bxe = e & 1, bye = e >> 1;  // 0: element on the negative side
-                    const int ai = bye ? (bxe ? 0 : 1) : (bxe ? 3 : 2);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const float v = Fq[j + bye][q];
-                        const float vv = bxe ? v : lane_prev(v);
-                        cr += n2(q, ai) * (vv * a.w2);
-                    }
-                }
-                F += cr;
-            }
-            const bool ok = olane && yy < ny && zz < nz;
-            float corr = 0.0f;
-            if (a.corr && ok) {
-                const int bx = x - a.corr_info[0][0], by = yy - a.corr_info[1][0],
-                          bz = zz - a.corr_info[2][0];
-                if (bx >= 0 && bx <= a.corr_info[0][1] && by >= 0 && by <= a.corr_info[1][1] &&
-                    bz >= 0 && bz <= a.corr_info[2][1])
-                    corr = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
-            }
-            float Tnew = ((M * Ti - KT) + F + corr) / M;
-            const bool face = x == 0 || x == nx - 1 || yy == 0 || yy == ny - 1 || zz == 0;
-            if (face) Tnew = a.alpha * rFa[i][j] + (1.0f - a.alpha) * rFb[i][j];
+            float F = corr[i][j];
+            if (SRC) F += sx * sy[j] * sz[i];
+            if (i + 1 == ptop) F += Ftop[j];
+            // explicit update Ti + dt (F - K T) / M with the lumped mass
+            // M = mrow * sum of the 8 element means of rho*cp
+            float Tnew = Ti + (F - KT) * (__builtin_amdgcn_rcpf(Msum) * rmfac);
+            if (face_[i][j]) Tnew = a.alpha * rFa_[i][j] + (1.0f - a.alpha) * rFb_[i][j];
             if (a.clamp && Tnew < Tamb) Tnew = Tamb;
-            const int idx = x + nx * yy + nxy * zz;
-            const bool s1 = rS1[i][j] || Ti >= pc.Tliq || idx < a.n_sub;
-            if (ok) {
-                a.Tout[(unsigned)idx] = Tnew;
-                a.S1out[(unsigned)idx] = (uint8_t)s1;
-                float kn, cn;
-                bool s1n;
-                props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
-                a.kcout[(unsigned)idx] = kn;
-                a.kcout[(unsigned)(nn + idx)] = cn;
-                if (ACC) {
-                    const bool s2n = Ti >= pc.Tliq;
-                    const bool reset = (!rS2[i][j]) && s2n;
-                    const float rr = reset ? rAcc[i][j] : 0.0f;
-                    a.max_accum[(unsigned)idx] = rr > rMax[i][j] ? rr : rMax[i][j];
-                    a.accum[(unsigned)idx] = rAcc[i][j] + row[5] * (s2n ? 1.0f : 0.0f) - rr;
-                    a.S2out[(unsigned)idx] = (uint8_t)s2n;
-                }
+            const int idx = base + nx * j + nxy * i;
+            const bool s1 = rS1_[i][j] != 0 || Ti >= pc.Tliq || idx < nsub;
+            float kn, cn;
+            bool s1n;
+            if (REC) props_rec(pc, sTab, Tnew, s1, false, kn, cn, s1n);
+            else props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
+            buf_st(Tnew, rTo, oF[i][j]);
+            buf_st8(s1, rS1o, oB[i][j]);
+            buf_st(kn, rKCo, oF[i][j]);
+            buf_st(cn, rKCo, oF[i][j], nb4);
+            if (ACC) {
+                const auto rS2o = buf_rsrc(a.S2out, (uint32_t)nn);
+                const auto rA = buf_rsrc(a.accum, nb4), rM = buf_rsrc(a.max_accum, nb4);
+                const bool s2n = Ti >= pc.Tliq;
+                const bool reset = rS2_[i][j] == 0 && s2n;
+                const float rr = reset ? rAcc[i][j] : 0.0f;
+                buf_st(rr > rMax[i][j] ? rr : rMax[i][j], rM, oF[i][j]);
+                buf_st(rAcc[i][j] + row[5] * (s2n ? 1.0f : 0.0f) - rr, rA, oF[i][j]);
+                buf_st8(s2n, rS2o, oB[i][j]);
             }
         }
     }
+}
+
+template <bool SRC, bool ACC, bool REC>
+static void launch_dpp(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
+                       hipStream_t s) {
+    constexpr int YB = 1, ZC = 3;  // measured best on a 101 x 101 x 51 mesh
+    const int nq = hg.n[0] * ((hg.n[1] + YB - 1) / YB);
+    const int nb = ((nq + DXS - 1) / DXS) * ((hg.n[2] + ZC - 1) / ZC);
+    hipLaunchKernelGGL((k_step_dpp<SRC, ACC, REC, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
+}
+
+void launch_level_step_kc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg,
+                          const DevMat* m, hipStream_t s) {
+    if (!a.kcin || !a.kcout) return;  // no carried properties for these buffers
+    if (a.src_mode == GM_SRC_L3 && !a.srcT) return;  // the source comes from tables
+    // the buffer offsets are 32-bit: node counts up to 2^28
+    if ((int64_t)hg.n[0] * hg.n[1] * hg.n[2] * 8 >= (int64_t)1 << 31) return;
+    if (a.src_mode == GM_SRC_L3) {
+        if (a.accum_on) {
+            if (a.prec_ok) launch_dpp<true, true, true>(a, d, hg, m, s);
+            else launch_dpp<true, true, false>(a, d, hg, m, s);
+        } else {
+            if (a.prec_ok) launch_dpp<true, false, true>(a, d, hg, m, s);
+            else launch_dpp<true, false, false>(a, d, hg, m, s);
+        }
+    } else {
+        if (a.prec_ok) launch_dpp<false, false, true>(a, d, hg, m, s);
+        else launch_dpp<false, false, false>(a, d, hg, m, s);
+    }
+}
+
+// k, cp of (T, S1) as the next level step's computeStateProperties returns them
+__global__ void k_props_kc(const DevMat* __restrict__ m, const float* __restrict__ T,
+                           const uint8_t* __restrict__ S1, int64_t nn, int64_t n_sub,
+                           float* __restrict__ kc) {
+    __shared__ float sTab[PTW] __attribute__((aligned(16)));
+    stage_tables(m, sTab, threadIdx.x, blockDim.x);
+    __syncthreads();
+    PropC pc = load_pc(m);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float kk, cc;
+        bool s1;
+        props_c(pc, sTab, T[i], S1[i] != 0, i < n_sub, kk, cc, s1);
+        kc[i] = kk;
+        kc[nn + i] = cc;
+    }
+}
+
+void launch_props_kc(const DevMat* m, const float* T, const uint8_t* S1, int64_t nn,
+                     int64_t n_sub, float* kc, hipStream_t s) {
+    const int64_t nb = std::min<int64_t>((nn + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_props_kc, dim3((unsigned)nb), dim3(256), 0, s, m, T, S1, nn, n_sub, kc);
 }
 
 // Separable source factors of every L3 node line for the desc rows:
@@ -987,46 +1079,6 @@ void launch_acc_l3(const GmStepDesc* d, float* accum, float* max_accum, float* a
                    int64_t n3, int to_l3, hipStream_t s) {
     hipLaunchKernelGGL(k_acc_l3, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, d, accum,
                        max_accum, acc3, macc3, n3, to_l3);
-}
-
-void launch_level_step_kc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg,
-                          const DevMat* m, hipStream_t s) {
-    if (!a.kcin || !a.kcout) return;  // no carried properties for these buffers
-    constexpr int YB = 2, ZC = 2;
-    const int nb = ((hg.n[0] + DXS - 1) / DXS) * ((hg.n[1] + YB - 1) / YB) *
-                   ((hg.n[2] + ZC - 1) / ZC);
-    if (a.src_mode == GM_SRC_L3) {
-        if (a.accum_on)
-            hipLaunchKernelGGL((k_step_dpp<true, true, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
-        else
-            hipLaunchKernelGGL((k_step_dpp<true, false, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
-    } else {
-        hipLaunchKernelGGL((k_step_dpp<false, false, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
-    }
-}
-
-// k, cp of (T, S1) as the next level step's computeStateProperties returns them
-__global__ void k_props_kc(const DevMat* __restrict__ m, const float* __restrict__ T,
-                           const uint8_t* __restrict__ S1, int64_t nn, int64_t n_sub,
-                           float* __restrict__ kc) {
-    __shared__ float sTab[2 * LUTW];
-    stage_tables(m, sTab, threadIdx.x, blockDim.x);
-    __syncthreads();
-    PropC pc = load_pc(m);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        float kk, cc;
-        bool s1;
-        props_c(pc, sTab, T[i], S1[i] != 0, i < n_sub, kk, cc, s1);
-        kc[i] = kk;
-        kc[nn + i] = cc;
-    }
-}
-
-void launch_props_kc(const DevMat* m, const float* T, const uint8_t* S1, int64_t nn,
-                     int64_t n_sub, float* kc, hipStream_t s) {
-    const int64_t nb = std::min<int64_t>((nn + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_props_kc, dim3((unsigned)nb), dim3(256), 0, s, m, T, S1, nn, n_sub, kc);
 }
 
 // Dirichlet values on the W, E, S, N and bottom faces of a fine level:
@@ -1433,7 +1485,7 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
     __shared__ float sKn[PN], sCn[PN];
     __shared__ float sSk[2][QN], sSc[2][QN];
     __shared__ float sKe[2][QN], sCe[2][QN];
-    __shared__ float sTab[KCM ? 1 : 2 * LUTW];
+    __shared__ float sTab[KCM ? 4 : PTW] __attribute__((aligned(16)));
     __shared__ double sZ[NZA][LY * LX];
     __shared__ double sX[NZA][LY][NAXM];
 
@@ -1816,7 +1868,7 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
         fr[a] = (int)rintf(cg.h[a] / fg.h[a]);
     }
     // the coarse level's carried k, cp follow the injected temperatures
-    __shared__ float sTab[2 * LUTW];
+    __shared__ float sTab[PTW] __attribute__((aligned(16)));
     if (cKC) {
         stage_tables(m, sTab, threadIdx.x, blockDim.x);
         __syncthreads();
@@ -2304,6 +2356,7 @@ DevMat make_devmat(const gm_material& mm) {
     m.invTb = (float)(1.0 / (double)mm.T_boiling);
     m.Tclip = (float)((double)mm.T_boiling + 1000.0);
     m.Tamb4 = (float)((double)mm.T_amb * mm.T_amb * mm.T_amb * mm.T_amb);
+    m.sqCM = (float)sqrt((double)mm.CM);
     m.eta = mm.laser_eta;
     const double r = mm.laser_radius, dd = mm.laser_depth;
     m.rsq = (float)(r * r);
@@ -2311,80 +2364,102 @@ DevMat make_devmat(const gm_material& mm) {
     m.rc = (float)(1.0 / (r * sqrt(M_PI)));
     m.dc = (float)(1.0 / (dd * sqrt(M_PI)));
     m.pc6 = (float)(6.0 * sqrt(3.0) * mm.laser_eta);
-    // Segment lookup: bins of width w over [T_first, T_last]; lut[b] is the
-    // segment of a temperature half a bin below the bin's lower edge, a lower
-    // bound of the segment of every T that lands in bin b, so the kernel's
-    // forward search from it ends at the same segment as a full scan.
-    const int tn[2] = {mm.k_table_n, mm.cp_table_n};
-    const float* tT[2] = {mm.k_table_T, mm.cp_table_T};
-    for (int w = 0; w < 2; ++w) {
-        const int n = tn[w];
-        const float* kT = tT[w];
-        int nb = 1;
-        double T0 = n > 0 ? kT[0] : 0.0, iw = 0.0;
-        if (n >= 2 && kT[n - 1] > kT[0]) {
-            double dmin = kT[n - 1] - kT[0];
-            for (int i = 0; i + 1 < n; ++i)
-                if (kT[i + 1] - kT[i] > 0.0) dmin = std::min(dmin, (double)kT[i + 1] - kT[i]);
-            nb = (int)std::min<double>(GM_LUT_BINS, std::ceil(2.0 * (kT[n - 1] - kT[0]) / dmin));
-            nb = std::max(nb, 1);
-            iw = nb / ((double)kT[n - 1] - kT[0]);
+    // Union of the table knots; k and cp are linear on every segment between
+    // consecutive knots (a property without a table keeps its linear law).
+    std::vector<double> U;
+    for (int i = 0; i < mm.k_table_n && mm.k_table_n >= 2; ++i) U.push_back(mm.k_table_T[i]);
+    for (int i = 0; i < mm.cp_table_n && mm.cp_table_n >= 2; ++i) U.push_back(mm.cp_table_T[i]);
+    std::sort(U.begin(), U.end());
+    U.erase(std::unique(U.begin(), U.end()), U.end());
+    if (U.size() > 2 * GM_MAX_KNOTS) U.resize(2 * GM_MAX_KNOTS);
+    const int nU = (int)U.size();
+    // value and slope on the segment that starts at u: table segment j of the
+    // largest interior knot <= u (cF:2584-2612 with a table for the law)
+    auto seg = [](const float* kT, const float* kv, int n, double a0, double a1, double u,
+                  double& v, double& sl) {
+        if (n < 2) {
+            sl = a1;
+            v = a0 + a1 * u;
+            return;
         }
-        m.lut_T0[w] = (float)T0;
-        m.lut_iw[w] = (float)iw;
-        m.lut_nb[w] = nb;
-        for (int b = 0; b < GM_LUT_BINS; ++b) {
-            int j = 0;
-            if (iw > 0.0) {
-                const double Tb = T0 + (b - 0.5) / iw;
-                for (int i = 1; i < n - 1; ++i)
-                    if ((double)kT[i] <= Tb) j = i;
-            }
-            m.lut[w][b] = j;
-        }
+        int j = 0;
+        for (int i = 1; i < n - 1; ++i)
+            if ((double)kT[i] <= u) j = i;
+        sl = ((double)kv[j + 1] - kv[j]) / ((double)kT[j + 1] - kT[j]);
+        v = (double)kv[j] + sl * (u - kT[j]);
+    };
+    // segment of T on the union knots: largest i in [0, nU-2] with U_i <= T
+    auto useg = [&](double T) {
+        int j = 0;
+        for (int i = 1; i < nU - 1; ++i)
+            if (U[i] <= T) j = i;
+        return j;
+    };
+    double segv[2 * GM_MAX_KNOTS][5];
+    for (int j = 0; j < nU; ++j) {
+        segv[j][0] = U[j];
+        seg(mm.k_table_T, mm.k_table_v, mm.k_table_n, mm.k_bulk_a0, mm.k_bulk_a1, U[j],
+            segv[j][1], segv[j][2]);
+        seg(mm.cp_table_T, mm.cp_table_v, mm.cp_table_n, mm.cp_solid_a0, mm.cp_solid_a1, U[j],
+            segv[j][3], segv[j][4]);
+        for (int f = 0; f < 5; ++f) m.mseg[j][f] = (float)segv[j][f];
     }
-    // merged table
-    m.mn = 0;
-    if (mm.k_table_n >= 2 && mm.cp_table_n >= 2) {
-        std::vector<double> U;
-        for (int i = 0; i < mm.k_table_n; ++i) U.push_back(mm.k_table_T[i]);
-        for (int i = 0; i < mm.cp_table_n; ++i) U.push_back(mm.cp_table_T[i]);
-        std::sort(U.begin(), U.end());
-        U.erase(std::unique(U.begin(), U.end()), U.end());
-        // value and slope of a table on the segment that starts at u
-        auto seg = [](const float* kT, const float* kv, int n, double u, double& v, double& sl) {
-            int j = 0;
-            for (int i = 1; i < n - 1; ++i)
-                if ((double)kT[i] <= u) j = i;
-            sl = ((double)kv[j + 1] - kv[j]) / ((double)kT[j + 1] - kT[j]);
-            v = (double)kv[j] + sl * (u - kT[j]);
-        };
-        const int nU = (int)U.size();
-        m.mn = nU;
-        for (int j = 0; j < nU && j < 2 * GM_MAX_KNOTS; ++j) {
-            double kvj, ksj, cvj, csj;
-            seg(mm.k_table_T, mm.k_table_v, mm.k_table_n, U[j], kvj, ksj);
-            seg(mm.cp_table_T, mm.cp_table_v, mm.cp_table_n, U[j], cvj, csj);
-            m.mseg[j][0] = (float)U[j];
-            m.mseg[j][1] = (float)kvj;
-            m.mseg[j][2] = (float)ksj;
-            m.mseg[j][3] = (float)cvj;
-            m.mseg[j][4] = (float)csj;
-        }
-        double dmin = U[nU - 1] - U[0];
+    m.mn = nU >= 2 ? nU : 0;
+    m.pnb = 0;
+    if (nU >= 2) {
+        const double range = U[nU - 1] - U[0];
+        double dmin = range;
         for (int i = 0; i + 1 < nU; ++i) dmin = std::min(dmin, U[i + 1] - U[i]);
-        int nb = (int)std::min<double>(GM_LUT_BINS, std::ceil(2.0 * (U[nU - 1] - U[0]) / dmin));
+        // loop lookup: lut[b] is the segment half a bin below the bin's lower
+        // edge, a lower bound of the segment of every T in bin b
+        int nb = (int)std::min<double>(GM_LUT_BINS, std::ceil(2.0 * range / dmin));
         nb = std::max(nb, 1);
-        const double iw = nb / (U[nU - 1] - U[0]);
+        double iw = nb / range;
         m.mT0 = (float)U[0];
         m.miw = (float)iw;
         m.mnb = nb;
-        for (int b = 0; b < GM_LUT_BINS; ++b) {
-            const double Tb = U[0] + (b - 0.5) / iw;
-            int j = 0;
-            for (int i = 1; i < nU - 1; ++i)
-                if (U[i] <= Tb) j = i;
-            m.mlut[b] = j;
+        for (int b = 0; b < GM_LUT_BINS; ++b) m.mlut[b] = useg(U[0] + (b - 0.5) / iw);
+        // records: the fewest bins for which every widened bin meets at most
+        // two segments (also for T beyond the end knots in the end bins)
+        for (int pnb = std::max(1, (int)std::ceil(range / dmin)); pnb <= GM_PREC_BINS; ++pnb) {
+            const double w = range / pnb, dl = 0.25 * w;
+            bool ok = true;
+            int A[GM_PREC_BINS];
+            for (int b = 0; b < pnb && ok; ++b) {
+                const double lo = U[0] + b * w, hi = lo + w;
+                A[b] = useg(lo - dl);
+                const int top = (b == pnb - 1) ? nU - 2 : useg(hi + dl);
+                ok = top <= A[b] + 1;
+            }
+            if (!ok) continue;
+            m.pnb = pnb;
+            m.pT0 = (float)U[0];
+            m.piw = (float)(pnb / range);
+            for (int b = 0; b < GM_PREC_BINS; ++b) {
+                const int a = A[std::min(b, pnb - 1)];
+                const int c = std::min(a + 1, nU - 2);
+                float* r = m.prec[b];
+                r[0] = a + 1 <= nU - 2 ? (float)U[a + 1] : INFINITY;
+                for (int f = 0; f < 5; ++f) r[1 + f] = (float)segv[a][f];
+                for (int f = 0; f < 4; ++f) r[6 + f] = (float)segv[c][1 + f];
+                r[10] = r[11] = 0.0f;
+            }
+            break;
+        }
+    } else {
+        // no table: one record of the two linear laws
+        m.pnb = 1;
+        m.pT0 = 0.0f;
+        m.piw = 0.0f;
+        for (int b = 0; b < GM_PREC_BINS; ++b) {
+            float* r = m.prec[b];
+            r[0] = INFINITY;
+            r[1] = 0.0f;
+            r[2] = mm.k_bulk_a0;
+            r[3] = mm.k_bulk_a1;
+            r[4] = mm.cp_solid_a0;
+            r[5] = mm.cp_solid_a1;
+            for (int f = 6; f < 12; ++f) r[f] = 0.0f;
         }
     }
     return m;
