@@ -47,7 +47,13 @@ def _relT(a, b, Tamb=300.0):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b - Tamb)))
 
 
-@pytest.mark.parametrize("case", ["plain", "melt", "tables"])
+# k table with knots 1 K apart (no per-bin segment records: loop lookup) and a
+# cp table on other knots; a k table alone (cp keeps its linear law)
+CLOSE_K = [[300.0, 1000.0, 1001.0, 1600.0, 1610.0, 3000.0], [10.0, 18.0, 18.5, 24.0, 27.0, 32.0]]
+CLOSE_CP = [[300.0, 700.0, 1500.0, 2500.0], [420.0, 500.0, 650.0, 700.0]]
+
+
+@pytest.mark.parametrize("case", ["plain", "melt", "tables", "tables_close", "table_k_only"])
 def test_driver_vs_oracle(oracle, tmp_path, case):
     props = {}
     if case == "melt":
@@ -59,6 +65,12 @@ def test_driver_vs_oracle(oracle, tmp_path, case):
         props = {"laser_power": 600.0, "emissivity": 0.4,
                  "thermal_conductivity_bulk_table": W.K_TABLE,
                  "heat_capacity_solid_table": W.CP_TABLE}
+    if case == "tables_close":
+        props = {"laser_power": 600.0, "thermal_conductivity_bulk_table": CLOSE_K,
+                 "heat_capacity_solid_table": CLOSE_CP}
+    if case == "table_k_only":
+        props = {"laser_power": 600.0, "thermal_conductivity_bulk_table": W.K_TABLE,
+                 "heat_capacity_solid_a1": 0.174}
     inp, g = W.tiny(n=(8, 8, 4), n2=2, n3=3, track=1.6, props=props)
     out, st = _run_both(oracle, inp, g, tmp_path)
     assert st["subcycle"] >= 1 and st["single"] >= 1
