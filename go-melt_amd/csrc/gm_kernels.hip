@@ -1851,6 +1851,22 @@ void launch_tprime(const GmStepDesc* d, int fine_level, int coarse_level, const 
 // the fine values at those nodes, so every fine node reads them directly and
 // the prolongation of cF:2097 needs no second launch.  Threads past the fine
 // node count write the injected values into the coarse field.
+__device__ __forceinline__ void tp_axis(int i, int f0, int R, float invR, int blo, int bn, int nc,
+                                        int nf, int& c0, int& c1, float& w, bool& in0, bool& in1,
+                                        int& j0, int& j1) {
+    const int rel = i - f0 + R;                           // >= 1: f0 < R
+    const int q = (int)(((float)rel + 0.5f) * invR);      // floor(rel / R), rel small
+    w = (float)(rel - q * R) * invR;
+    c0 = blo + q - 1;                                     // coarse node at or below node i
+    c0 = c0 < 0 ? 0 : (c0 > nc - 1 ? nc - 1 : c0);        // inside the mesh when aligned
+    c1 = c0 + 1 < nc ? c0 + 1 : nc - 1;                   // weight w = 0 when clamped
+    j0 = f0 + (c0 - blo) * R;
+    j1 = f0 + (c1 - blo) * R;
+    // injected: in the overlap box and on a fine node
+    in0 = c0 - blo >= 0 && c0 - blo < bn && j0 >= 0 && j0 < nf;
+    in1 = c1 - blo >= 0 && c1 - blo < bn && j1 >= 0 && j1 < nf;
+}
+
 __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level,
                                 int coarse_level, const float* __restrict__ fT,
                                 float* __restrict__ cT, float* __restrict__ Tp, int64_t nf,
@@ -1868,31 +1884,42 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
         fr[a] = (int)rintf(cg.h[a] / fg.h[a]);
     }
     // the coarse level's carried k, cp follow the injected temperatures
+    // (staged only by the blocks that inject)
     __shared__ float sTab[PTW] __attribute__((aligned(16)));
-    if (cKC) {
+    if (cKC && (int64_t)(blockIdx.x + 1) * blockDim.x > nf) {
         stage_tables(m, sTab, threadIdx.x, blockDim.x);
         __syncthreads();
     }
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nf) {
+        // T' = T_fine - I(T_coarse after injection) at fine node t: with the
+        // coarse nodes on every R-th fine node, the coarse element of a fine
+        // node and its trilinear weights follow from the fine index; corners
+        // inside the injected box take the fine value (cF:2060-2099).
         int ix, iy, iz;
         dec3(t, fg.n[0], fg.n[1], ix, iy, iz);
-        float N[8];
-        int ce[3];
-        interp8e(cg, coordf(fg, 0, ix), coordf(fg, 1, iy), coordf(fg, 2, iz), N, ce);
+        int cx0, cx1, cy0, cy1, cz0, cz1, jx0, jx1, jy0, jy1, jz0, jz1;
+        float wx, wy, wz;
+        bool ix0, ix1, iy0, iy1, iz0, iz1;
+        tp_axis(ix, f0[0], fr[0], 1.0f / fr[0], b.lo[0], b.n[0], cg.n[0], fg.n[0], cx0, cx1, wx, ix0, ix1,
+                jx0, jx1);
+        tp_axis(iy, f0[1], fr[1], 1.0f / fr[1], b.lo[1], b.n[1], cg.n[1], fg.n[1], cy0, cy1, wy, iy0, iy1,
+                jy0, jy1);
+        tp_axis(iz, f0[2], fr[2], 1.0f / fr[2], b.lo[2], b.n[2], cg.n[2], fg.n[2], cz0, cz1, wz, iz0, iz1,
+                jz0, jz1);
+        const int fnx = fg.n[0], fnxy = fg.n[0] * fg.n[1];
+        const int cnx = cg.n[0], cnxy = cg.n[0] * cg.n[1];
         float v = 0.0f;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int cx = ce[0] + (((q + 1) >> 1) & 1), cy = ce[1] + ((q >> 1) & 1);
-            const int cz = ce[2] + (q >> 2);
-            const int64_t c = nid(cx, cy, cz, cg.n[0], cg.n[1]);
-            const int bx = cx - b.lo[0], by = cy - b.lo[1], bz = cz - b.lo[2];
-            const int jx = f0[0] + bx * fr[0], jy = f0[1] + by * fr[1], jz = f0[2] + bz * fr[2];
-            const bool inj = bx >= 0 && bx < b.n[0] && by >= 0 && by < b.n[1] && bz >= 0 &&
-                             bz < b.n[2] && jx >= 0 && jx < fg.n[0] && jy >= 0 && jy < fg.n[1] &&
-                             jz >= 0 && jz < fg.n[2];
-            const float u = inj ? fT[nid(jx, jy, jz, fg.n[0], fg.n[1])] : cT[c];
-            v += N[q] * u;
+            const int dx = q & 1, dy = (q >> 1) & 1, dz = q >> 2;
+            const bool inj = (dx ? ix1 : ix0) && (dy ? iy1 : iy0) && (dz ? iz1 : iz0);
+            const float u = inj ? fT[(dx ? jx1 : jx0) + fnx * (dy ? jy1 : jy0) +
+                                     fnxy * (dz ? jz1 : jz0)]
+                                : cT[(dx ? cx1 : cx0) + cnx * (dy ? cy1 : cy0) +
+                                     cnxy * (dz ? cz1 : cz0)];
+            const float N = (dx ? wx : 1.0f - wx) * (dy ? wy : 1.0f - wy) * (dz ? wz : 1.0f - wz);
+            v += N * u;
         }
         Tp[t] = fT[t] - v;
         return;
