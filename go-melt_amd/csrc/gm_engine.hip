@@ -942,7 +942,8 @@ static float* kc_of(gm_engine* e, const float* T) {
 
 // L2 solve of the subcycle with the nodal properties carried between steps
 static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
-                      const float* corr, float alpha, float* Tout, int dt_index) {
+                      const float* corr, float alpha, float* Tout, int dt_index,
+                      const float* G = nullptr, int npos = 0, float gscale = 0.0f) {
     LSArgs a = ls_default();
     a.level = 2;
     a.T = Tin;
@@ -955,7 +956,17 @@ static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8
     a.dt_mode = GM_DT_L2;
     a.dt_index = dt_index;
     a.conv = 1;
-    a.corr = corr;
+    if (corr) {
+        a.corr = corr;
+    } else {
+        // restricted terms straight from the pair's FP64 node box, plus the
+        // L2 source of the substep's positions (k_restrict_nodes_acc's sum)
+        a.corr64 = e->part[2];
+        a.corrG = G;
+        a.corrG_npos = npos;
+        a.corrG_nmax = e->gmax[2];
+        a.corrG_scale = gscale;
+    }
     for (int d = 0; d < 3; ++d) {
         a.corr_info[d] = e->pair[2].p.ax[d].info;
         a.corr_nmax[d] = e->pair[2].p.cmax[d] + 1;
@@ -1101,6 +1112,20 @@ static void l1_s1_update(gm_engine* e) {
 
 // ---- subcycleGOMELT (cF:3224-3632) -------------------------------------------
 
+// L2 solve of substep k with its restricted terms and source: read straight
+// from the pair's FP64 node box by the step (aligned levels), or finished
+// into V2 first
+static void l2_substep(gm_engine* e, const float* T2cur, const uint8_t* S12cur, uint8_t* S12new,
+                       const float* Gk, float alpha, float* T2new, int k) {
+    const float gscale = e->w3[3] / (float)e->N3;
+    if (e->fast_restrict) {
+        run_l2_kc(e, T2cur, S12cur, S12new, nullptr, alpha, T2new, k, Gk, e->N3, gscale);
+        return;
+    }
+    eng_restrict_nodes(e, 2, -1, nullptr, e->V2, 2, Gk, e->N3, e->gmax[2], gscale);
+    run_l2_kc(e, T2cur, S12cur, S12new, e->V2, alpha, T2new, k);
+}
+
 static void subcycle_body(gm_engine* e) {
     const int N2 = e->N2, N3 = e->N3, N23 = e->N23;
     const int64_t n1 = e->nn[1], n2 = e->nn[2], n3 = e->nn[3];
@@ -1122,10 +1147,9 @@ static void subcycle_body(gm_engine* e) {
     e->use_kc = true;
     // L1 source over all N2*N3 positions (cF:3285-3287)
     launch_source_tables(e->pair[1].p, d, e->dm, 0, N23, 0, nullptr, e->G1, e->gmax[1], s);
-    // L2 sources per L2 substep (cF:3329-3331), shared by both passes
-    for (int k = 0; k < N2; ++k)
-        launch_source_tables(e->pair[2].p, d, e->dm, k * N3, N3, 0, nullptr,
-                             e->G2 + (size_t)k * N3 * 3 * e->gmax[2], e->gmax[2], s);
+    // L2 sources per L2 substep (cF:3329-3331), shared by both passes: the
+    // tables of substep k are positions k*N3 .. k*N3+N3-1, one launch for all
+    launch_source_tables(e->pair[2].p, d, e->dm, 0, N23, 0, nullptr, e->G2, e->gmax[2], s);
     // L1 flux terms from L3 and L2 T' (computeL1TprimeTerms_Part1, cF:2733-2810)
     eng_restrict(e, 1,
                     rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));
@@ -1148,12 +1172,10 @@ static void subcycle_body(gm_engine* e) {
         const float* Gk = e->G2 + (size_t)k * N3 * 3 * e->gmax[2];
         eng_restrict(e, 2,
                         rargs(e, 3, T3cur, S13cur, Tp3cur, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));
-        eng_restrict_nodes(e, 2, -1, nullptr, e->V2,
-                              2, Gk, N3, e->gmax[2], e->w3[3] / (float)N3);
         float* T2new = e->T2b[b2];
         uint8_t* S12new = e->S12b[b2];
         b2 ^= 1;
-        run_l2_kc(e, T2cur, S12cur, S12new, e->V2, (float)(k + 1) / fN2, T2new, k);
+        l2_substep(e, T2cur, S12cur, S12new, Gk, (float)(k + 1) / fN2, T2new, k);
         launch_face_bc(d, 3, 2, e->cfg.level[3], T2new, T2cur, e->fa3, e->fb3, s);
         const float* T3in = T3cur;
         const uint8_t* S13in = S13cur;
@@ -1198,12 +1220,10 @@ static void subcycle_body(gm_engine* e) {
         // flux + mass restriction onto L2 (cF:3496-3508)
         eng_restrict(e, 2,
                         rargs(e, 3, T3cur, S13cur, Tp3cur, e->histTp[k], Tp3cur, GM_DT_L2, k, 0.0f));
-        eng_restrict_nodes(e, 2, -1, nullptr, e->V2,
-                              2, Gk, N3, e->gmax[2], e->w3[3] / (float)N3);
         float* T2new = e->L2all[k];
         uint8_t* S12new = e->S12b[b2s];
         b2s ^= 1;
-        run_l2_kc(e, T2cur, S12cur, S12new, e->V2, (float)(k + 1) / fN2, T2new, k);
+        l2_substep(e, T2cur, S12cur, S12new, Gk, (float)(k + 1) / fN2, T2new, k);
         launch_face_bc(d, 3, 2, e->cfg.level[3], T2new, T2cur, e->fa3, e->fb3, s);
         const float* T3in = T3cur;
         const uint8_t* S13in = S13cur;

@@ -155,6 +155,14 @@ struct LSArgs {
     // srcT + r * (nx + ny + nz): x factors (times 6 sqrt(3) eta P w3), then y, z
     const float* srcT;
     int prec_ok;          // the material has per-bin property records (pnb > 0)
+    // correction read straight from the FP64 restriction accumulator (same
+    // node box as corr), cleared as it is read, plus the coarse laser source
+    // from per-axis tables G (npos positions, nmax entries per axis) times
+    // corrG_scale; replaces a separate finishing pass
+    double* corr64;
+    const float* corrG;
+    int corrG_npos, corrG_nmax;
+    float corrG_scale;
 };
 
 // Restriction (fine -> coarse) arguments.

@@ -804,7 +804,7 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
     for (int i = 0; i < ZC; ++i)
 #pragma unroll
         for (int j = 0; j < YB; ++j) corr[i][j] = 0.0f;
-    if (a.corr) {
+    if (a.corr || a.corr64) {
         const int c0x = a.corr_info[0][0], c1x = a.corr_info[0][1];
         const int c0y = a.corr_info[1][0], c1y = a.corr_info[1][1];
         const int c0z = a.corr_info[2][0], c1z = a.corr_info[2][1];
@@ -814,8 +814,26 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
             for (int j = 0; j < YB; ++j) {
                 const int bx = x - c0x, by = y0 + j - c0y, bz = z0 + i - c0z;
                 if (oB[i][j] != GM_OOB && bx >= 0 && bx <= c1x && by >= 0 && by <= c1y &&
-                    bz >= 0 && bz <= c1z)
-                    corr[i][j] = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
+                    bz >= 0 && bz <= c1z) {
+                    const int k = bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz);
+                    if (a.corr) {
+                        corr[i][j] = a.corr[k];
+                    } else {
+                        // each box node is read by exactly one lane of the grid
+                        const double acc = a.corr64[k];
+                        a.corr64[k] = 0.0;
+                        float v = (float)acc;
+                        if (a.corrG) {
+                            float f = 0.0f;
+                            for (int qd = 0; qd < a.corrG_npos; ++qd) {
+                                const float* Gq = a.corrG + qd * 3 * a.corrG_nmax;
+                                f += Gq[bx] * Gq[a.corrG_nmax + by] * Gq[2 * a.corrG_nmax + bz];
+                            }
+                            v += f * a.corrG_scale;
+                        }
+                        corr[i][j] = v;
+                    }
+                }
             }
     }
     const PropC pc = load_pc(m);
@@ -1503,10 +1521,12 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
     const int ix = ox + tx, iy = oy + ty;
     const bool col_ok = ix < nx && iy < ny;
     int R[3], off[3];
+    double invR[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         R[a] = (int)rintf(gc.h[a] / gf.h[a]);
         off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) / gf.h[a]);
+        invR[a] = 1.0 / (double)R[a];
     }
 
     int eoff[2];
@@ -1619,11 +1639,10 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
         const int c = u / R[2];
         const int j = u - c * R[2];
         const int lz = c - AzB;
-        const double w1 = (double)j / (double)R[2], w0 = 1.0 - w1;
         const double v = (double)rr;
+        const double v1 = (double)j * invR[2] * v, v0 = v - v1;
 #pragma unroll
-        for (int a = 0; a < NZA; ++a)
-            zacc[a] += (a == lz ? w0 * v : 0.0) + (a == lz + 1 ? w1 * v : 0.0);
+        for (int a = 0; a < NZA; ++a) zacc[a] += (a == lz ? v0 : (a == lz + 1 ? v1 : 0.0));
     }
 #pragma unroll
     for (int a = 0; a < NZA; ++a) sZ[a][tid] = zacc[a];
@@ -1642,7 +1661,7 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
         const int lo = max(Au - R[0] + 1, ox), hi = min(Au + R[0] - 1, xl);
         for (int i = lo; i <= hi; ++i) {
             const int dd = i - Au;
-            sum += (double)(R[0] - (dd < 0 ? -dd : dd)) / (double)R[0] * sZ[az][yy * LX + i - ox];
+            sum += (double)(R[0] - (dd < 0 ? -dd : dd)) * invR[0] * sZ[az][yy * LX + i - ox];
         }
         sX[az][yy][ax] = sum;
     }
@@ -1658,7 +1677,7 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
         const int lo = max(Au - R[1] + 1, oy), hi = min(Au + R[1] - 1, yl);
         for (int i = lo; i <= hi; ++i) {
             const int dd = i - Au;
-            sum += (double)(R[1] - (dd < 0 ? -dd : dd)) / (double)R[1] * sX[az][i - oy][ax];
+            sum += (double)(R[1] - (dd < 0 ? -dd : dd)) * invR[1] * sX[az][i - oy][ax];
         }
         const int bx = AxB + ax - lo0, by = AyB + ay - lo1, bz = AzB + az - lo2;
         if (sum != 0.0 && bx >= 0 && bx < nb0 && by >= 0 && by < nb1 && bz >= 0 && bz < nb2)
