@@ -464,6 +464,7 @@ struct gm_engine {
     std::vector<float*> kc_T;
     bool use_kc = false;      // restrictions read the carried k, cp
     bool fast_restrict = false;  // all pairs node-aligned: node-centric restriction
+    int strip_R0[3] = {-1, -1, -1}, strip_a0[3] = {-1, -1, -1};  // restriction strip layout
     float mcs[4][8];          // consistent mass per neighbour pattern (x rho)
     uint8_t* preS2 = nullptr;
     float *fa2 = nullptr, *fb2 = nullptr, *fa3 = nullptr, *fb3 = nullptr;
@@ -525,6 +526,7 @@ extern "C" int gm_engine_destroy(gm_engine* e) {
 }
 
 static bool pairs_aligned(gm_engine* e);
+static void strip_phase(gm_engine* e, int i, int& R0, int& a0);
 
 extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, void* stream,
                                 gm_engine** out) {
@@ -689,6 +691,7 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
     for (int i = 0; i < 3; ++i)
         HIPCHK(hipMemsetAsync(e->part[i], 0, pair_elems(e->pair[i].p) * 8 * sizeof(double), e->s));
     e->fast_restrict = pairs_aligned(e);
+    for (int i = 0; i < 3; ++i) strip_phase(e, i, e->strip_R0[i], e->strip_a0[i]);
     HIPCHK(hipStreamSynchronize(e->s));
     HIPCHK(hipGetLastError());
     *out = e;
@@ -752,6 +755,17 @@ static int update_alignment(gm_engine* e) {
         for (int i = 0; i < 3; ++i)
             HIPCHK(hipMemsetAsync(e->part[i], 0, pair_elems(e->pair[i].p) * 8 * sizeof(double),
                                   e->s));
+    }
+    // the wave-brick restriction's strip layout (its grid size) follows the
+    // pairs' x ratio and phase; a recorded graph holds the old grid
+    for (int i = 0; i < 3; ++i) {
+        int R0, a0;
+        strip_phase(e, i, R0, a0);
+        if (R0 != e->strip_R0[i] || a0 != e->strip_a0[i]) {
+            e->strip_R0[i] = R0;
+            e->strip_a0[i] = a0;
+            e->graph_ok = false;
+        }
     }
     return 0;
 }
@@ -1051,6 +1065,21 @@ static bool grids_aligned(const gm_grid& gc, const gm_grid& gf) {
     return true;
 }
 
+// x spacing ratio of pair i and the first fine x index on a coarse node
+// (R0 = 0 when the pair is not node-aligned)
+static void strip_phase(gm_engine* e, int i, int& R0, int& a0) {
+    const gm_grid& gc = e->host.g[e->pair[i].p.coarse];
+    const gm_grid& gf = e->host.g[e->pair[i].p.fine];
+    R0 = 0;
+    a0 = 0;
+    if (!grids_aligned(gc, gf)) return;
+    const int R = (int)std::lround((double)gc.h[0] / gf.h[0]);
+    const int off = (int)std::lround(((double)gf.x0[0] - gc.x0[0]) / gf.h[0]);
+    if (R < 1 || R > 61 || off < 0) return;
+    R0 = R;
+    a0 = (R - off % R) % R;
+}
+
 static bool pairs_aligned(gm_engine* e) {
     bool ok = true;
     for (int i = 0; i < 3; ++i)
@@ -1064,6 +1093,9 @@ static void eng_restrict(gm_engine* e, int i, RArgs r) {
     const GmPair& p = e->pair[i].p;
     if (e->fast_restrict) {
         if (e->use_kc && p.fine >= 2) r.KC = kc_of(e, r.T);
+        r.R0 = e->strip_R0[i];
+        r.a0 = e->strip_a0[i];
+        r.dpp_ok = r.R0 > 0;
         memcpy(r.kcs, e->kc[p.fine], sizeof(r.kcs));
         memcpy(r.mcs, e->mcs[p.fine], sizeof(r.mcs));
         launch_restrict_fast(p, e->d_desc, e->d_dm, r, e->part[i], e->s);

@@ -182,6 +182,10 @@ struct RArgs {
     const float* KC;      // k[nn] then cp[nn] (nullable: computed from T, S1)
     float kcs[8];         // stiffness per neighbour pattern (unit k)
     float mcs[8];         // consistent mass per neighbour pattern (unit rho*cp)
+    // wave-brick kernel (carried k, cp): the x ratio and the first fine x on a
+    // coarse node, fixed while the pair's meshes do not move relative to each
+    // other by a fraction of a coarse spacing (host-checked)
+    int dpp_ok, R0, a0;
 };
 
 // ---- launchers (gm_kernels.hip) -------------------------------------------

@@ -1685,13 +1685,231 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
     }
 }
 
+// Node-centric restriction with the carried k, cp of the fine level, one
+// wave per brick like the level step.  A wave owns an x-strip of W fine
+// columns (W a multiple of the x ratio R, starting on a coarse node), one
+// fine row and ZC fine planes; lane l holds x = X - 1 + l.  Per fine node
+// the residual r of the flux and mass terms is the 27-point stencil of the
+// level step; the z weights (uniform in the wave) fold it into per-lane FP64
+// sums per coarse z node, then the lane on each coarse node gathers the
+// x weights (R - |d|)/R of its 2R-1 neighbours through DPP shifts, applies
+// the y weight and adds into the pair's FP64 node box.
+__device__ __forceinline__ double dprev(double v) {  // lane l-1 (0 in lane 0)
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x138, 0xf, 0xf, true);
+    const uint32_t hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0x138, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double dnext(double v) {  // lane l+1 (0 in lane 63)
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x130, 0xf, 0xf, true);
+    const uint32_t hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0x130, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// 27-point stencil of the level-step form at the lane's node of window
+// plane i+1, row 1: element values E (planes i, i+1; rows 0, 1 = y-1, y
+// sides), field U over planes i..i+2, rows 0..2, pattern coefficients co.
+template <int NR>
+__device__ __forceinline__ float stencil_dpp(const float (*E)[2], const float (*U)[NR], int i,
+                                             const float* co) {
+    const float k100 = E[i][0], k110 = E[i][1], k101 = E[i + 1][0], k111 = E[i + 1][1];
+    const float k000 = lane_prev(k100), k010 = lane_prev(k110);
+    const float k001 = lane_prev(k101), k011 = lane_prev(k111);
+    const float Ti = U[i + 1][1];
+    const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011, Pz11 = k110 + k111;
+    const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011, Py11 = k101 + k111;
+    const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101, Px11 = k011 + k111;
+    const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
+    const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
+    const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
+    float s = co[0] * ((Fx0 + Fx1) * Ti);
+    s += co[1] * (Fx0 * lane_prev(Ti) + Fx1 * lane_next(Ti));
+    s += co[2] * (Fy0 * U[i + 1][0] + Fy1 * U[i + 1][2]);
+    s += co[4] * (Fz0 * U[i][1] + Fz1 * U[i + 2][1]);
+    s += co[3] * (Pz00 * lane_prev(U[i + 1][0]) + Pz10 * lane_next(U[i + 1][0]) +
+                  Pz01 * lane_prev(U[i + 1][2]) + Pz11 * lane_next(U[i + 1][2]));
+    s += co[5] * (Py00 * lane_prev(U[i][1]) + Py10 * lane_next(U[i][1]) +
+                  Py01 * lane_prev(U[i + 2][1]) + Py11 * lane_next(U[i + 2][1]));
+    s += co[6] * (Px00 * U[i][0] + Px10 * U[i][2] + Px01 * U[i + 2][0] + Px11 * U[i + 2][2]);
+    s += co[7] * (k000 * lane_prev(U[i][0]) + k100 * lane_next(U[i][0]) +
+                  k010 * lane_prev(U[i][2]) + k110 * lane_next(U[i][2]) +
+                  k001 * lane_prev(U[i + 2][0]) + k101 * lane_next(U[i + 2][0]) +
+                  k011 * lane_prev(U[i + 2][2]) + k111 * lane_next(U[i + 2][2]));
+    return s;
+}
+
+template <bool FLUX, bool MASS, int ZC>
+__global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc* __restrict__ d,
+                                                     RArgs r, double* __restrict__ acc) {
+    constexpr int NZC = ZC + 1;  // coarse z nodes a brick can touch (ratio 1)
+    const gm_grid gf = d->g[pp.fine];
+    const gm_grid gc = d->g[pp.coarse];
+    const int nx = gf.n[0], ny = gf.n[1], nz = gf.n[2];
+    const int nxy = nx * ny;
+    const int nn = nxy * nz;
+    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
+    int R[3], off[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        R[a] = (int)rintf(gc.h[a] / gf.h[a]);
+        off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) / gf.h[a]);
+    }
+    const int W = R[0] * (61 / R[0]);
+    const int a0 = (R[0] - off[0] % R[0]) % R[0];  // first fine x on a coarse node
+    const int s0 = a0 > 0 ? 0 : 1;
+    const int nzc = (nz + ZC - 1) / ZC;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int zc = t % nzc, rest = t / nzc;
+    const int y = rest % ny, st = rest / ny;
+    const int z0 = zc * ZC;
+    const int X = a0 + (st + s0 - 1) * W;  // strip start (a coarse node)
+    const int lane = threadIdx.x;
+    const int x = X - 1 + lane;
+    const bool xin = x >= 0 && x < nx;
+    const bool ein = x >= 0 && x < nex;
+    const bool own = lane >= 1 && lane <= W && xin;
+    const uint32_t nb4 = (uint32_t)nn * 4u;
+    const int base = x + nx * y + nxy * z0;
+
+    // ---- loads: planes z0-1 .. z0+ZC, rows y-1 .. y+1 ------------------------
+    const auto rKC = buf_rsrc(r.KC, 2u * nb4);
+    const auto rP = buf_rsrc(FLUX ? r.Tp : r.KC, nb4);
+    const auto rA = buf_rsrc(MASS ? r.TpA : r.KC, nb4), rB = buf_rsrc(MASS ? r.TpB : r.KC, nb4);
+    float Pw[ZC + 2][3], Dw[ZC + 2][3], Kw[ZC + 2][3], Cw[ZC + 2][3];
+#pragma unroll
+    for (int p = 0; p < ZC + 2; ++p) {
+        const int zz = z0 - 1 + p;
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) {
+            const int yy = y - 1 + rr;
+            const bool in = xin && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+            const uint32_t vo = in ? 4u * (uint32_t)(base + nx * (rr - 1) + nxy * (p - 1)) : GM_OOB;
+            Kw[p][rr] = buf_ld(rKC, vo);
+            Cw[p][rr] = buf_ld(rKC, vo, nb4);
+            Pw[p][rr] = FLUX ? buf_ld(rP, vo) : 0.0f;
+            Dw[p][rr] = MASS ? buf_ld(rA, vo) - buf_ld(rB, vo) : 0.0f;
+        }
+    }
+    float dt = r.dt;
+    if (r.dt_mode == GM_DT_L1) dt = d->dt1;
+    else if (r.dt_mode == GM_DT_L2) dt = d->dt2[r.dt_index];
+    else if (r.dt_mode == GM_DT_ROW) dt = d->rows[r.dt_index][5];
+    const float idt = 1.0f / dt;
+
+    // ---- element means of k and cp (as the level step) ---------------------
+    float Qk[ZC + 2][2], Qc[ZC + 2][2];
+#pragma unroll
+    for (int p = 0; p < ZC + 2; ++p) {
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr) {
+            const int yy = y - 1 + rr;
+            const bool ok = ein && yy >= 0 && yy < ney;
+            const float ak = Kw[p][rr] + Kw[p][rr + 1];
+            const float ac = Cw[p][rr] + Cw[p][rr + 1];
+            const float qk = ak + lane_next(ak);
+            const float qc = ac + lane_next(ac);
+            Qk[p][rr] = ok ? qk : 0.0f;
+            Qc[p][rr] = ok ? qc : 0.0f;
+        }
+    }
+    float Ek[ZC + 1][2], Ec[ZC + 1][2];
+#pragma unroll
+    for (int p = 0; p < ZC + 1; ++p) {
+        const int ez = z0 - 1 + p;
+        const float f = (ez >= 0 && ez < nez) ? 0.125f : 0.0f;
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr) {
+            Ek[p][rr] = (Qk[p][rr] + Qk[p + 1][rr]) * f;
+            Ec[p][rr] = (Qc[p][rr] + Qc[p + 1][rr]) * f;
+        }
+    }
+
+    // ---- residuals, folded over z into coarse z nodes -------------------------
+    const int czb = (z0 + off[2]) / R[2];
+    const double invR2 = 1.0 / (double)R[2];
+    double zacc[NZC];
+#pragma unroll
+    for (int b = 0; b < NZC; ++b) zacc[b] = 0.0;
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) {
+        const int zz = z0 + i;
+        float rv = 0.0f;
+        if (FLUX) rv -= stencil_dpp<3>(Ek, Pw, i, r.kcs);
+        if (MASS) rv -= idt * stencil_dpp<3>(Ec, Dw, i, r.mcs);
+        const double v = (own && zz < nz) ? (double)rv : 0.0;
+        const int uz = zz + off[2];
+        const int cz = uz / R[2];
+        const int jz = uz - cz * R[2];
+        const int lz = cz - czb;
+        const double w1 = (double)jz * invR2, w0 = 1.0 - w1;
+#pragma unroll
+        for (int b = 0; b < NZC; ++b) {
+            const double w = b == lz ? w0 : (b == lz + 1 ? w1 : 0.0);
+            zacc[b] += w * v;
+        }
+    }
+
+    // ---- x gather on the coarse-node lanes, y weight, box add -----------------
+    const int uy = y + off[1];
+    const int cy = uy / R[1];
+    const int jy = uy - cy * R[1];
+    const double wy1 = (double)jy / (double)R[1], wy0 = 1.0 - wy1;
+    const double invR0 = 1.0 / (double)R[0];
+    // coarse-node lanes, also those just outside the fine mesh whose support
+    // reaches into it (x + off is a multiple of R there, so the division is exact)
+    const bool owner = lane >= 1 && lane <= W + 1 && (lane - 1) % R[0] == 0;
+    const int cx = (x + off[0]) / R[0];
+    const int lo0 = pp.ax[0].info[0], lo1 = pp.ax[1].info[0], lo2 = pp.ax[2].info[0];
+    const int nb0 = pp.ax[0].info[1] + 1, nb1 = pp.ax[1].info[1] + 1, nb2 = pp.ax[2].info[1] + 1;
+    const int st0 = pp.cmax[0] + 1, st1 = pp.cmax[1] + 1;
+#pragma unroll
+    for (int b = 0; b < NZC; ++b) {
+        const int bz = czb + b - lo2;
+        if (b > (ZC + R[2] - 2) / R[2] + 1 || bz < 0 || bz >= nb2) continue;  // uniform
+        double V = zacc[b], vp = zacc[b], vn = zacc[b];
+        for (int dd = 1; dd < R[0]; ++dd) {
+            vp = dprev(vp);
+            vn = dnext(vn);
+            V += (double)(R[0] - dd) * invR0 * (vp + vn);
+        }
+        const int bx = cx - lo0;
+        if (owner && V != 0.0 && bx >= 0 && bx < nb0) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int by = cy + a - lo1;
+                const double w = a ? wy1 : wy0;
+                if (w != 0.0 && by >= 0 && by < nb1)
+                    unsafeAtomicAdd(acc + (bx + (int64_t)st0 * (by + (int64_t)st1 * bz)), w * V);
+            }
+        }
+    }
+}
+
 void launch_restrict_fast(const GmPair& p, const GmStepDesc* d, const DevMat* m, const RArgs& r,
                           double* acc, hipStream_t s) {
     // grid of the fine level: the pair's element counts + 1
     const int nx = p.nef[0] + 1, ny = p.nef[1] + 1, nz = p.nef[2] + 1;
     const int nb = ((nx + LX - 1) / LX) * ((ny + LY - 1) / LY) * ((nz + RZ - 1) / RZ);
     const bool fl = r.Tp != nullptr, ms = r.TpA != nullptr;
-    if (r.KC) {
+    if (r.KC && r.dpp_ok) {
+        // x strips of W = R * floor(61 / R) columns starting on coarse nodes,
+        // one per fine row and ZC planes
+        constexpr int ZC = 4;
+        const int R0 = r.R0, a0 = r.a0;
+        const int W = R0 * (61 / R0);
+        const int nst = (nx - a0 + W - 1) / W + (a0 > 0 ? 1 : 0);
+        const int nbd = nst * ny * ((nz + ZC - 1) / ZC);
+        if (fl && ms)
+            hipLaunchKernelGGL((k_restrict_dpp<true, true, ZC>), dim3(nbd), dim3(64), 0, s, p, d, r,
+                               acc);
+        else if (fl)
+            hipLaunchKernelGGL((k_restrict_dpp<true, false, ZC>), dim3(nbd), dim3(64), 0, s, p, d,
+                               r, acc);
+        else
+            hipLaunchKernelGGL((k_restrict_dpp<false, true, ZC>), dim3(nbd), dim3(64), 0, s, p, d,
+                               r, acc);
+    } else if (r.KC) {
         if (fl && ms)
             hipLaunchKernelGGL((k_restrict_kc<true, true, true>), dim3(nb), dim3(256), 0, s, p, d,
                                m, r, acc);
