@@ -448,6 +448,9 @@ struct gm_engine {
     float *T1p = nullptr, *T1c = nullptr;
     float *V1 = nullptr, *V1m = nullptr, *V2 = nullptr;
     float *G1 = nullptr, *G2 = nullptr;
+    // coarse source on the node boxes: per L2 substep (pair 2), and a scratch
+    // box for the node finishing passes
+    float *F2 = nullptr, *Fg = nullptr;
     float* T2b[2] = {nullptr, nullptr};
     uint8_t* S12b[2] = {nullptr, nullptr};
     float* Tp2x = nullptr;
@@ -637,7 +640,10 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
         eng_alloc(e, (void**)&e->V1m, pair_nodes(e->pair[0].p) * 4) ||
         eng_alloc(e, (void**)&e->V2, pair_nodes(e->pair[2].p) * 4) ||
         eng_alloc(e, (void**)&e->G1, (size_t)e->N23 * 3 * e->gmax[1] * 4) ||
-        eng_alloc(e, (void**)&e->G2, (size_t)e->N23 * 3 * e->gmax[2] * 4))
+        eng_alloc(e, (void**)&e->G2, (size_t)e->N23 * 3 * e->gmax[2] * 4) ||
+        eng_alloc(e, (void**)&e->F2, (size_t)e->N2 * pair_nodes(e->pair[2].p) * 4) ||
+        eng_alloc(e, (void**)&e->Fg, std::max(pair_nodes(e->pair[1].p),
+                                              pair_nodes(e->pair[2].p)) * 4))
         return bad(-1);
     for (int i = 0; i < 4; ++i) {
         if (hipHostMalloc((void**)&e->ring[i], sizeof(GmStepDesc), hipHostMallocDefault) !=
@@ -957,7 +963,7 @@ static float* kc_of(gm_engine* e, const float* T) {
 // L2 solve of the subcycle with the nodal properties carried between steps
 static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
                       const float* corr, float alpha, float* Tout, int dt_index,
-                      const float* G = nullptr, int npos = 0, float gscale = 0.0f) {
+                      const float* F = nullptr, float gscale = 0.0f) {
     LSArgs a = ls_default();
     a.level = 2;
     a.T = Tin;
@@ -976,9 +982,7 @@ static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8
         // restricted terms straight from the pair's FP64 node box, plus the
         // L2 source of the substep's positions (k_restrict_nodes_acc's sum)
         a.corr64 = e->part[2];
-        a.corrG = G;
-        a.corrG_npos = npos;
-        a.corrG_nmax = e->gmax[2];
+        a.corrG = F;
         a.corrG_scale = gscale;
     }
     for (int d = 0; d < 3; ++d) {
@@ -1108,11 +1112,12 @@ static void eng_restrict_nodes(gm_engine* e, int outer, int inner, const float* 
                                int gpair, const float* G, int npos, int nmax, float scale) {
     const GmPair* pi = inner >= 0 ? &e->pair[inner].p : nullptr;
     const GmPair* pg = gpair >= 0 ? &e->pair[gpair].p : nullptr;
-    if (e->fast_restrict)
+    if (e->fast_restrict) {
+        if (pg && G) launch_src_box(*pg, G, 1, npos, nmax, e->Fg, e->s);
         launch_restrict_nodes_acc(e->pair[outer].p, e->part[outer], pi,
-                                  inner >= 0 ? e->part[inner] : nullptr, Vbase, V, pg, G, npos,
-                                  nmax, scale, e->s);
-    else
+                                  inner >= 0 ? e->part[inner] : nullptr, Vbase, V, pg,
+                                  (pg && G) ? e->Fg : nullptr, scale, e->s);
+    } else
         launch_restrict_nodes(e->pair[outer].p, e->part[outer], pi,
                               inner >= 0 ? e->part[inner] : nullptr, Vbase, V, pg, G, npos, nmax,
                               scale, e->s);
@@ -1151,7 +1156,9 @@ static void l2_substep(gm_engine* e, const float* T2cur, const uint8_t* S12cur, 
                        const float* Gk, float alpha, float* T2new, int k) {
     const float gscale = e->w3[3] / (float)e->N3;
     if (e->fast_restrict) {
-        run_l2_kc(e, T2cur, S12cur, S12new, nullptr, alpha, T2new, k, Gk, e->N3, gscale);
+        // source box of substep k, from launch_src_box at the start of the step
+        run_l2_kc(e, T2cur, S12cur, S12new, nullptr, alpha, T2new, k,
+                  e->F2 + (size_t)k * pair_nodes(e->pair[2].p), gscale);
         return;
     }
     eng_restrict_nodes(e, 2, -1, nullptr, e->V2, 2, Gk, e->N3, e->gmax[2], gscale);
@@ -1182,6 +1189,7 @@ static void subcycle_body(gm_engine* e) {
     // L2 sources per L2 substep (cF:3329-3331), shared by both passes: the
     // tables of substep k are positions k*N3 .. k*N3+N3-1, one launch for all
     launch_source_tables(e->pair[2].p, d, e->dm, 0, N23, 0, nullptr, e->G2, e->gmax[2], s);
+    if (e->fast_restrict) launch_src_box(e->pair[2].p, e->G2, N2, N3, e->gmax[2], e->F2, s);
     // L1 flux terms from L3 and L2 T' (computeL1TprimeTerms_Part1, cF:2733-2810)
     eng_restrict(e, 1,
                     rargs(e, 3, e->T3, e->S13, e->Tp3, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));

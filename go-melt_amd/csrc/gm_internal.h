@@ -157,11 +157,9 @@ struct LSArgs {
     int prec_ok;          // the material has per-bin property records (pnb > 0)
     // correction read straight from the FP64 restriction accumulator (same
     // node box as corr), cleared as it is read, plus the coarse laser source
-    // from per-axis tables G (npos positions, nmax entries per axis) times
-    // corrG_scale; replaces a separate finishing pass
+    // on the same box (corrG, times corrG_scale); replaces a finishing pass
     double* corr64;
     const float* corrG;
-    int corrG_npos, corrG_nmax;
     float corrG_scale;
 };
 
@@ -219,8 +217,10 @@ void launch_restrict_fast(const GmPair& p, const GmStepDesc* d, const DevMat* m,
 // boxes are cleared for the next accumulation
 void launch_restrict_nodes_acc(const GmPair& outer, double* acc_outer, const GmPair* inner,
                                double* acc_inner, const float* Vbase, float* V,
-                               const GmPair* gpair, const float* G, int npos, int nmax,
-                               float scale, hipStream_t s);
+                               const GmPair* gpair, const float* F, float scale, hipStream_t s);
+// coarse source of nk position groups (npos each) on the node box of pair p
+void launch_src_box(const GmPair& p, const float* G, int nk, int npos, int nmax, float* F,
+                    hipStream_t s);
 void launch_restrict_nodes(const GmPair& outer, const double* part_outer, const GmPair* inner,
                            const double* part_inner, const float* Vbase, float* V,
                            const GmPair* gpair, const float* G, int npos, int nmax, float scale,

@@ -823,14 +823,7 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
                         const double acc = a.corr64[k];
                         a.corr64[k] = 0.0;
                         float v = (float)acc;
-                        if (a.corrG) {
-                            float f = 0.0f;
-                            for (int qd = 0; qd < a.corrG_npos; ++qd) {
-                                const float* Gq = a.corrG + qd * 3 * a.corrG_nmax;
-                                f += Gq[bx] * Gq[a.corrG_nmax + by] * Gq[2 * a.corrG_nmax + bz];
-                            }
-                            v += f * a.corrG_scale;
-                        }
+                        if (a.corrG) v += a.corrG[k] * a.corrG_scale;
                         corr[i][j] = v;
                     }
                 }
@@ -1937,8 +1930,7 @@ void launch_restrict_fast(const GmPair& p, const GmStepDesc* d, const DevMat* m,
 __global__ void k_restrict_nodes_acc(GmPair po, double* __restrict__ acco, GmPair pi,
                                      double* __restrict__ acci, int have_inner,
                                      const float* __restrict__ Vbase, float* __restrict__ V,
-                                     GmPair pg, const float* __restrict__ G, int npos, int nmax,
-                                     float scale) {
+                                     GmPair pg, const float* __restrict__ F, float scale) {
     const int mx = po.cmax[0] + 1, my = po.cmax[1] + 1, mz = po.cmax[2] + 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= mx * my * mz) return;
@@ -1956,20 +1948,59 @@ __global__ void k_restrict_nodes_acc(GmPair po, double* __restrict__ acco, GmPai
         }
     }
     float v = (Vbase ? Vbase[t] : 0.0f) + (float)s;
-    if (G) {
+    if (F) {
+        // coarse source summed over the positions on the node box of pg
         const int gx = A[0] - pg.ax[0].info[0], gy = A[1] - pg.ax[1].info[0],
                   gz = A[2] - pg.ax[2].info[0];
         if (gx >= 0 && gx <= pg.ax[0].info[1] && gy >= 0 && gy <= pg.ax[1].info[1] && gz >= 0 &&
-            gz <= pg.ax[2].info[1]) {
-            float f = 0.0f;
-            for (int q = 0; q < npos; ++q) {
-                const float* Gq = G + q * 3 * nmax;
-                f += Gq[gx] * Gq[nmax + gy] * Gq[2 * nmax + gz];
-            }
-            v += f * scale;
-        }
+            gz <= pg.ax[2].info[1])
+            v += F[gx + (pg.cmax[0] + 1) * (gy + (pg.cmax[1] + 1) * gz)] * scale;
     }
     V[t] = v;
+}
+
+// Coarse laser source on the node box of pair p (cF:2667-2730 summed over
+// positions): F[k][t] = sum over the npos positions of group k of the
+// per-axis table products, for every node t of the box (zero outside the
+// covered range).  WAVE: one wave per node, lanes over the positions.
+template <bool WAVE>
+__global__ void k_src_box(GmPair p, const float* __restrict__ G, int nk, int npos, int nmax,
+                          float* __restrict__ F) {
+    const int mx = p.cmax[0] + 1, my = p.cmax[1] + 1, mz = p.cmax[2] + 1;
+    const int nbox = mx * my * mz;
+    const int o = WAVE ? blockIdx.x : blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nk * nbox) return;
+    const int k = o / nbox, t = o - k * nbox;
+    const int gx = t % mx, gy = (t / mx) % my, gz = t / (mx * my);
+    const bool in = gx <= p.ax[0].info[1] && gy <= p.ax[1].info[1] && gz <= p.ax[2].info[1];
+    float f = 0.0f;
+    if (in) {
+        const float* Gk = G + (size_t)k * npos * 3 * nmax;
+        if (WAVE) {
+            for (int q = threadIdx.x; q < npos; q += 64) {
+                const float* Gq = Gk + q * 3 * nmax;
+                f += Gq[gx] * Gq[nmax + gy] * Gq[2 * nmax + gz];
+            }
+#pragma unroll
+            for (int sh = 32; sh >= 1; sh >>= 1) f += __shfl_xor(f, sh, 64);
+        } else {
+            for (int q = 0; q < npos; ++q) {
+                const float* Gq = Gk + q * 3 * nmax;
+                f += Gq[gx] * Gq[nmax + gy] * Gq[2 * nmax + gz];
+            }
+        }
+    }
+    if (!WAVE || threadIdx.x == 0) F[o] = f;
+}
+
+void launch_src_box(const GmPair& p, const float* G, int nk, int npos, int nmax, float* F,
+                    hipStream_t s) {
+    const int n = nk * (p.cmax[0] + 1) * (p.cmax[1] + 1) * (p.cmax[2] + 1);
+    if (npos > 16)
+        hipLaunchKernelGGL(k_src_box<true>, dim3(n), dim3(64), 0, s, p, G, nk, npos, nmax, F);
+    else
+        hipLaunchKernelGGL(k_src_box<false>, dim3((n + 255) / 256), dim3(256), 0, s, p, G, nk,
+                           npos, nmax, F);
 }
 
 // the inner node box is cleared by a second pass (its layout differs)
@@ -1980,14 +2011,13 @@ __global__ void k_clear_f64(double* __restrict__ a, int64_t n) {
 
 void launch_restrict_nodes_acc(const GmPair& outer, double* acc_outer, const GmPair* inner,
                                double* acc_inner, const float* Vbase, float* V,
-                               const GmPair* gpair, const float* G, int npos, int nmax,
-                               float scale, hipStream_t s) {
+                               const GmPair* gpair, const float* F, float scale, hipStream_t s) {
     const int n = (outer.cmax[0] + 1) * (outer.cmax[1] + 1) * (outer.cmax[2] + 1);
     GmPair pi = inner ? *inner : outer;
     GmPair pg = gpair ? *gpair : outer;
     hipLaunchKernelGGL(k_restrict_nodes_acc, dim3((n + 255) / 256), dim3(256), 0, s, outer,
-                       acc_outer, pi, acc_inner, inner ? 1 : 0, Vbase, V, pg, gpair ? G : nullptr,
-                       npos, nmax, scale);
+                       acc_outer, pi, acc_inner, inner ? 1 : 0, Vbase, V, pg, gpair ? F : nullptr,
+                       scale);
     if (inner) {
         const int64_t ni = (int64_t)(inner->cmax[0] + 1) * (inner->cmax[1] + 1) *
                            (inner->cmax[2] + 1);
