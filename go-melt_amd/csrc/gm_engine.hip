@@ -1272,7 +1272,7 @@ static void subcycle_body(gm_engine* e) {
             float* T3o = (j == N3 - 1) ? e->L3all[k] : e->T3b[b3s];
             if (j != N3 - 1) b3s ^= 1;
             uint8_t* S13o = e->S13b[bs2];
-            uint8_t* S23o = e->S23b[bs2];
+            uint8_t* S23o = e->S23;  // melt state updated in place (written where it flips)
             bs2 ^= 1;
             run_l3_kc(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, S23in, S23o);
             T3in = T3o;
@@ -1294,7 +1294,6 @@ static void subcycle_body(gm_engine* e) {
     launch_copy_f32(e->T3, T3cur, n3, s);
     launch_copy_f32(e->Tp3, Tp3cur, n3, s);
     launch_copy_u8(e->S13, S13cur, n3, s);
-    launch_copy_u8(e->S23, S23cur, n3, s);
     // final L2 T' and L1 T0 (cF:3625-3627)
     new_tprime(e, 2, e->T2, e->T1c, e->Tp2);
     launch_copy_f32(e->T1, e->T1c, n1, s);

@@ -967,14 +967,19 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
             buf_st(kn, rKCo, oF[i][j]);
             buf_st(cn, rKCo, oF[i][j], nb4);
             if (ACC) {
+                // melt time: only molten nodes change accum and max_accum
+                // (max with 0 keeps max_accum >= 0), and S2 is written where
+                // it flips, so S2out may alias S2prev
                 const auto rS2o = buf_rsrc(a.S2out, (uint32_t)nn);
                 const auto rA = buf_rsrc(a.accum, nb4), rM = buf_rsrc(a.max_accum, nb4);
+                const bool s2p = rS2_[i][j] != 0;
                 const bool s2n = Ti >= pc.Tliq;
-                const bool reset = rS2_[i][j] == 0 && s2n;
+                const bool reset = !s2p && s2n;
                 const float rr = reset ? rAcc[i][j] : 0.0f;
-                buf_st(rr > rMax[i][j] ? rr : rMax[i][j], rM, oF[i][j]);
-                buf_st(rAcc[i][j] + row[5] * (s2n ? 1.0f : 0.0f) - rr, rA, oF[i][j]);
-                buf_st8(s2n, rS2o, oB[i][j]);
+                const uint32_t oa = s2n ? oF[i][j] : GM_OOB;
+                buf_st(rr > rMax[i][j] ? rr : rMax[i][j], rM, oa);
+                buf_st(rAcc[i][j] + row[5] - rr, rA, oa);
+                buf_st8(s2n, rS2o, s2n != s2p ? oB[i][j] : GM_OOB);
             }
         }
     }
