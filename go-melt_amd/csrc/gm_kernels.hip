@@ -702,8 +702,9 @@ __device__ __forceinline__ float face_flux_hw(const DevMat& m, float Tq) {
 template <bool SRC, bool ACC, bool REC, int YB, int ZC>
 __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __restrict__ d,
                                                  const DevMat* __restrict__ m) {
-    const gm_grid g = d->g[a.level];
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    // node counts from the arguments (a window move changes only the origin,
+    // which this kernel does not use): no dependent descriptor read
+    const int nx = a.grid.n[0], ny = a.grid.n[1], nz = a.grid.n[2];
     const int nxy = nx * ny;
     const int nn = nxy * nz;
     const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
@@ -781,7 +782,16 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
         }
     }
     __shared__ float sTab[PTW] __attribute__((aligned(16)));
-    stage_tables(m, sTab, lane, 64);
+    if (REC) {
+        // records known present: copy them without reading the bin count
+        const float4* src = (const float4*)&m->prec[0][0];
+        float4* dst = (float4*)sTab;
+#pragma unroll
+        for (int l = 0; l < (GM_PREC_BINS * 3 + 63) / 64; ++l)
+            if (lane + 64 * l < GM_PREC_BINS * 3) dst[lane + 64 * l] = src[lane + 64 * l];
+    } else {
+        stage_tables(m, sTab, lane, 64);
+    }
 
     const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
     float dt = a.dt;
@@ -994,8 +1004,10 @@ static void launch_dpp(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, 
     hipLaunchKernelGGL((k_step_dpp<SRC, ACC, REC, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
 }
 
-void launch_level_step_kc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg,
+void launch_level_step_kc(const LSArgs& a0, const GmStepDesc* d, const gm_grid& hg,
                           const DevMat* m, hipStream_t s) {
+    LSArgs a = a0;
+    a.grid = hg;  // the kernel takes the node counts from here
     if (!a.kcin || !a.kcout) return;  // no carried properties for these buffers
     if (a.src_mode == GM_SRC_L3 && !a.srcT) return;  // the source comes from tables
     // the buffer offsets are 32-bit: node counts up to 2^28
