@@ -1132,8 +1132,8 @@ static void new_tprime(gm_engine* e, int f, const float* fineT, float* coarseT, 
     const int64_t nb = (int64_t)b.n[0] * b.n[1] * b.n[2];
     float* cKC = (e->use_kc && cS1) ? kc_of(e, coarseT) : nullptr;
     if (e->fast_restrict) {
-        launch_tprime_inject(e->d_desc, f, f - 1, fineT, coarseT, Tp, e->nn[f], nb, e->d_dm,
-                             cS1, e->host.sub[f - 1], cKC, e->s);
+        launch_tprime_inject(e->d_desc, f, f - 1, fineT, coarseT, Tp, e->cfg.level[f], nb,
+                             e->d_dm, cS1, e->host.sub[f - 1], cKC, e->s);
         return;
     }
     launch_inject(e->d_desc, f, f - 1, fineT, coarseT, nb, e->s);

@@ -227,9 +227,9 @@ void launch_restrict_nodes(const GmPair& outer, const double* part_outer, const 
                            hipStream_t s);
 // (with cKC: also k, cp of the injected coarse nodes, from cS1)
 void launch_tprime_inject(const GmStepDesc* d, int fine_level, int coarse_level,
-                          const float* fineT, float* coarseT, float* Tp, int64_t nf, int64_t nbox,
-                          const DevMat* m, const uint8_t* cS1, int64_t c_nsub, float* cKC,
-                          hipStream_t s);
+                          const float* fineT, float* coarseT, float* Tp, const gm_grid& fine_grid,
+                          int64_t nbox, const DevMat* m, const uint8_t* cS1, int64_t c_nsub,
+                          float* cKC, hipStream_t s);
 void launch_face_bc(const GmStepDesc* d, int fine, int coarse, const gm_grid& gf, const float* A,
                     const float* B, float* fa, float* fb, hipStream_t s);
 void launch_interp_grid(const GmStepDesc* d, int src_level, const gm_grid* src_host,

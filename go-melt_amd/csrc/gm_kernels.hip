@@ -2153,7 +2153,7 @@ __device__ __forceinline__ void tp_axis(int i, int f0, int R, float invR, int bl
 
 __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level,
                                 int coarse_level, const float* __restrict__ fT,
-                                float* __restrict__ cT, float* __restrict__ Tp, int64_t nf,
+                                float* __restrict__ cT, float* __restrict__ Tp, int ntp_blocks,
                                 const DevMat* __restrict__ m, const uint8_t* __restrict__ cS1,
                                 int64_t c_nsub, float* __restrict__ cKC) {
     const gm_grid& fg = d->g[fine_level];
@@ -2167,58 +2167,75 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
         f0[a] = (int)rintf((xyz.x0[a] - fg.x0[a]) / fg.h[a]);
         fr[a] = (int)rintf(cg.h[a] / fg.h[a]);
     }
-    // the coarse level's carried k, cp follow the injected temperatures
-    // (staged only by the blocks that inject)
-    __shared__ float sTab[PTW] __attribute__((aligned(16)));
-    if (cKC && (int64_t)(blockIdx.x + 1) * blockDim.x > nf) {
-        stage_tables(m, sTab, threadIdx.x, blockDim.x);
-        __syncthreads();
-    }
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nf) {
-        // T' = T_fine - I(T_coarse after injection) at fine node t: with the
-        // coarse nodes on every R-th fine node, the coarse element of a fine
-        // node and its trilinear weights follow from the fine index; corners
-        // inside the injected box take the fine value (cF:2060-2099).
-        int ix, iy, iz;
-        dec3(t, fg.n[0], fg.n[1], ix, iy, iz);
+    const int fnx = fg.n[0], fny = fg.n[1], fnxy = fnx * fny;
+    if ((int)blockIdx.x < ntp_blocks) {
+        // T' = T_fine - I(T_coarse after injection) at the fine nodes of one
+        // plane (z uniform in the block): with the coarse nodes on every R-th
+        // fine node, the coarse element of a fine node and its trilinear
+        // weights follow from the fine index; corners inside the injected box
+        // take the fine value (cF:2060-2099).
+        const int nqb = (fnxy + 255) / 256;
+        const int iz = blockIdx.x / nqb;
+        const int q = (blockIdx.x - iz * nqb) * 256 + threadIdx.x;
+        if (q >= fnxy) return;
+        const int iy = (int)((unsigned)q / (unsigned)fnx), ix = q - iy * fnx;
         int cx0, cx1, cy0, cy1, cz0, cz1, jx0, jx1, jy0, jy1, jz0, jz1;
         float wx, wy, wz;
         bool ix0, ix1, iy0, iy1, iz0, iz1;
-        tp_axis(ix, f0[0], fr[0], 1.0f / fr[0], b.lo[0], b.n[0], cg.n[0], fg.n[0], cx0, cx1, wx, ix0, ix1,
-                jx0, jx1);
-        tp_axis(iy, f0[1], fr[1], 1.0f / fr[1], b.lo[1], b.n[1], cg.n[1], fg.n[1], cy0, cy1, wy, iy0, iy1,
-                jy0, jy1);
-        tp_axis(iz, f0[2], fr[2], 1.0f / fr[2], b.lo[2], b.n[2], cg.n[2], fg.n[2], cz0, cz1, wz, iz0, iz1,
-                jz0, jz1);
-        const int fnx = fg.n[0], fnxy = fg.n[0] * fg.n[1];
-        const int cnx = cg.n[0], cnxy = cg.n[0] * cg.n[1];
-        float v = 0.0f;
+        tp_axis(ix, f0[0], fr[0], 1.0f / fr[0], b.lo[0], b.n[0], cg.n[0], fg.n[0], cx0, cx1, wx,
+                ix0, ix1, jx0, jx1);
+        tp_axis(iy, f0[1], fr[1], 1.0f / fr[1], b.lo[1], b.n[1], cg.n[1], fg.n[1], cy0, cy1, wy,
+                iy0, iy1, jy0, jy1);
+        tp_axis(iz, f0[2], fr[2], 1.0f / fr[2], b.lo[2], b.n[2], cg.n[2], fg.n[2], cz0, cz1, wz,
+                iz0, iz1, jz0, jz1);
+        const int t = q + fnxy * iz;
+        float v;
+        if (ix0 && ix1 && iy0 && iy1 && iz0 && iz1) {
+            // all corners injected (inside the window): fine values only
+            const int r00 = fnx * jy0 + fnxy * jz0, r10 = fnx * jy1 + fnxy * jz0;
+            const int r01 = fnx * jy0 + fnxy * jz1, r11 = fnx * jy1 + fnxy * jz1;
+            const float a00 = (1.0f - wx) * fT[jx0 + r00] + wx * fT[jx1 + r00];
+            const float a10 = (1.0f - wx) * fT[jx0 + r10] + wx * fT[jx1 + r10];
+            const float a01 = (1.0f - wx) * fT[jx0 + r01] + wx * fT[jx1 + r01];
+            const float a11 = (1.0f - wx) * fT[jx0 + r11] + wx * fT[jx1 + r11];
+            v = (1.0f - wz) * ((1.0f - wy) * a00 + wy * a10) + wz * ((1.0f - wy) * a01 + wy * a11);
+        } else {
+            const int cnx = cg.n[0], cnxy = cg.n[0] * cg.n[1];
+            v = 0.0f;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int dx = q & 1, dy = (q >> 1) & 1, dz = q >> 2;
-            const bool inj = (dx ? ix1 : ix0) && (dy ? iy1 : iy0) && (dz ? iz1 : iz0);
-            const float u = inj ? fT[(dx ? jx1 : jx0) + fnx * (dy ? jy1 : jy0) +
-                                     fnxy * (dz ? jz1 : jz0)]
-                                : cT[(dx ? cx1 : cx0) + cnx * (dy ? cy1 : cy0) +
-                                     cnxy * (dz ? cz1 : cz0)];
-            const float N = (dx ? wx : 1.0f - wx) * (dy ? wy : 1.0f - wy) * (dz ? wz : 1.0f - wz);
-            v += N * u;
+            for (int qd = 0; qd < 8; ++qd) {
+                const int dx = qd & 1, dy = (qd >> 1) & 1, dz = qd >> 2;
+                const bool inj = (dx ? ix1 : ix0) && (dy ? iy1 : iy0) && (dz ? iz1 : iz0);
+                const float u = inj ? fT[(dx ? jx1 : jx0) + fnx * (dy ? jy1 : jy0) +
+                                         fnxy * (dz ? jz1 : jz0)]
+                                    : cT[(dx ? cx1 : cx0) + cnx * (dy ? cy1 : cy0) +
+                                         cnxy * (dz ? cz1 : cz0)];
+                const float N =
+                    (dx ? wx : 1.0f - wx) * (dy ? wy : 1.0f - wy) * (dz ? wz : 1.0f - wz);
+                v += N * u;
+            }
         }
         Tp[t] = fT[t] - v;
         return;
     }
-    const int64_t k = t - nf;
-    const int64_t nb = (int64_t)b.n[0] * b.n[1] * b.n[2];
+    // injection of the fine values into the coarse overlap box; the coarse
+    // level's carried k, cp follow the injected temperatures
+    __shared__ float sTab[PTW] __attribute__((aligned(16)));
+    if (cKC) {
+        stage_tables(m, sTab, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
+    const int k = ((int)blockIdx.x - ntp_blocks) * 256 + threadIdx.x;
+    const int nb = b.n[0] * b.n[1] * b.n[2];
     if (k >= nb) return;
-    const int bx = (int)(k % b.n[0]), by = (int)((k / b.n[0]) % b.n[1]);
-    const int bz = (int)(k / ((int64_t)b.n[0] * b.n[1]));
+    const int bz = k / (b.n[0] * b.n[1]), rem = k - bz * (b.n[0] * b.n[1]);
+    const int by = rem / b.n[0], bx = rem - by * b.n[0];
     const int cx = b.lo[0] + bx, cy = b.lo[1] + by, cz = b.lo[2] + bz;
     if (cx < 0 || cx >= cg.n[0] || cy < 0 || cy >= cg.n[1] || cz < 0 || cz >= cg.n[2]) return;
     const int jx = f0[0] + bx * fr[0], jy = f0[1] + by * fr[1], jz = f0[2] + bz * fr[2];
     if (jx < 0 || jx >= fg.n[0] || jy < 0 || jy >= fg.n[1] || jz < 0 || jz >= fg.n[2]) return;
     const int64_t ci = nid(cx, cy, cz, cg.n[0], cg.n[1]);
-    const float v = fT[nid(jx, jy, jz, fg.n[0], fg.n[1])];
+    const float v = fT[jx + fnx * jy + fnxy * jz];
     cT[ci] = v;
     if (cKC) {
         const int64_t nnc = (int64_t)cg.n[0] * cg.n[1] * cg.n[2];
@@ -2232,12 +2249,15 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
 }
 
 void launch_tprime_inject(const GmStepDesc* d, int fine_level, int coarse_level,
-                          const float* fineT, float* coarseT, float* Tp, int64_t nf, int64_t nbox,
-                          const DevMat* m, const uint8_t* cS1, int64_t c_nsub, float* cKC,
-                          hipStream_t s) {
-    const int64_t n = nf + nbox;
-    hipLaunchKernelGGL(k_tprime_inject, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d,
-                       fine_level, coarse_level, fineT, coarseT, Tp, nf, m, cS1, c_nsub, cKC);
+                          const float* fineT, float* coarseT, float* Tp, const gm_grid& fine_grid,
+                          int64_t nbox, const DevMat* m, const uint8_t* cS1, int64_t c_nsub,
+                          float* cKC, hipStream_t s) {
+    // fine node counts are fixed for a level (moves change the origin only)
+    const int nxy = fine_grid.n[0] * fine_grid.n[1];
+    const int ntp = ((nxy + 255) / 256) * fine_grid.n[2];
+    const int ninj = (int)((nbox + 255) / 256);
+    hipLaunchKernelGGL(k_tprime_inject, dim3(ntp + ninj), dim3(256), 0, s, d, fine_level,
+                       coarse_level, fineT, coarseT, Tp, ntp, m, cS1, c_nsub, cKC);
 }
 
 // moveEverything, L3 part (cF:2439-2444): T0 = I(L1 T0) + (I(L2 T'0) + I(L3 T'0))
