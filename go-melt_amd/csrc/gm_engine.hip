@@ -298,7 +298,7 @@ extern "C" int gm_level_source(const gm_grid* coarse, const gm_grid* fine, const
     HIPCHK(hipMalloc(&rows_d, (size_t)npos * 7 * 4));
     HIPCHK(hipMalloc(&G, (size_t)npos * 3 * nmax * 4));
     HIPCHK(hipMemcpyAsync(rows_d, rows, (size_t)npos * 7 * 4, hipMemcpyHostToDevice, s));
-    launch_axis_tables(pm.p, td.d, s);
+    launch_axis_tables(&pm.p, 1, td.d, s);
     launch_source_tables(pm.p, td.d, make_devmat(*mat), 0, npos, 1, rows_d, G, nmax, s);
     LSArgs a = ls_default();
     a.G = G;
@@ -371,7 +371,7 @@ extern "C" int gm_restrict(const gm_grid* coarse, const gm_grid* fine, const gm_
     float* V = nullptr;
     HIPCHK(hipMalloc(&part, pair_elems(pm.p) * 8 * sizeof(double)));
     HIPCHK(hipMalloc(&V, pair_nodes(pm.p) * 4));
-    launch_axis_tables(pm.p, td.d, s);
+    launch_axis_tables(&pm.p, 1, td.d, s);
     RArgs r;
     memset(&r, 0, sizeof(r));
     r.T = T;
@@ -693,7 +693,10 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
     h.l1_nn_active = n1;
     HIPCHK(hipStreamSynchronize(e->s));
     if (upload_desc(e, desc_bytes(0))) return bad(-1);
-    for (int i = 0; i < 3; ++i) launch_axis_tables(e->pair[i].p, e->d_desc, e->s);
+    {
+        const GmPair ps[3] = {e->pair[0].p, e->pair[1].p, e->pair[2].p};
+        launch_axis_tables(ps, 3, e->d_desc, e->s);
+    }
     for (int i = 0; i < 3; ++i)
         HIPCHK(hipMemsetAsync(e->part[i], 0, pair_elems(e->pair[i].p) * 8 * sizeof(double), e->s));
     e->fast_restrict = pairs_aligned(e);
@@ -834,11 +837,18 @@ extern "C" int gm_engine_move(gm_engine* e, const float* v, const float* vstart,
     const int64_t n3 = e->nn[3], n2 = e->nn[2];
     launch_move_l3(e->d_desc, e->T1, e->Tp2, e->Tp3, e->T3b[0], e->Tp3x, n3, e->s);
     launch_move_l2(e->d_desc, e->T1, e->Tp2, e->T2b[0], e->Tp2x, n2, e->s);
-    launch_copy_f32(e->T3, e->T3b[0], n3, e->s);
-    launch_copy_f32(e->Tp3, e->Tp3x, n3, e->s);
-    launch_copy_f32(e->T2, e->T2b[0], n2, e->s);
-    launch_copy_f32(e->Tp2, e->Tp2x, n2, e->s);
-    for (int i = 0; i < 3; ++i) launch_axis_tables(e->pair[i].p, e->d_desc, e->s);
+    {
+        CopyList L;
+        L.c[0] = CopyItem{e->T3, e->T3b[0], n3 * 4};
+        L.c[1] = CopyItem{e->Tp3, e->Tp3x, n3 * 4};
+        L.c[2] = CopyItem{e->T2, e->T2b[0], n2 * 4};
+        L.c[3] = CopyItem{e->Tp2, e->Tp2x, n2 * 4};
+        launch_copy_multi(L, 4, e->s);
+    }
+    {
+        const GmPair ps[3] = {e->pair[0].p, e->pair[1].p, e->pair[2].p};
+        launch_axis_tables(ps, 3, e->d_desc, e->s);
+    }
     launch_l0_gather(e->d_desc, e->L0S1, e->s2snap, e->s2box, e->S13, e->S23, e->S12, n3, n2,
                      e->s);
     HIPCHK(hipGetLastError());
@@ -1175,8 +1185,8 @@ static void subcycle_body(gm_engine* e) {
     // L1 S1 from the L2 overlap, then substrate (cF:3273-3278)
     l1_s1_update(e);
     // k, cp of the start-of-step L3 state for the carried-property L3 steps
-    launch_props_kc(e->d_dm, e->T3, e->S13, n3, e->host.sub[3], kc_of(e, e->T3), s);
-    launch_props_kc(e->d_dm, e->T2, e->S12, n2, e->host.sub[2], kc_of(e, e->T2), s);
+    launch_props_kc(e->d_dm, e->T3, e->S13, n3, e->host.sub[3], kc_of(e, e->T3), s, e->T2,
+                    e->S12, n2, e->host.sub[2], kc_of(e, e->T2));
     // separable L3 source factors of all N2*N3 rows (the L3 mesh is fixed
     // during the step)
     {
@@ -1288,15 +1298,18 @@ static void subcycle_body(gm_engine* e) {
         Tp3cur = e->histTp[k];
     }
     launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, n3, 0, s);
-    // commit the end-of-step L2/L3 state
-    launch_copy_f32(e->T2, T2cur, n2, s);
-    launch_copy_u8(e->S12, S12cur, n2, s);
-    launch_copy_f32(e->T3, T3cur, n3, s);
-    launch_copy_f32(e->Tp3, Tp3cur, n3, s);
-    launch_copy_u8(e->S13, S13cur, n3, s);
-    // final L2 T' and L1 T0 (cF:3625-3627)
-    new_tprime(e, 2, e->T2, e->T1c, e->Tp2);
-    launch_copy_f32(e->T1, e->T1c, n1, s);
+    // final L2 T' and L1 T0 (cF:3625-3627), then commit the end-of-step state
+    new_tprime(e, 2, T2cur, e->T1c, e->Tp2);
+    {
+        CopyList L;
+        L.c[0] = CopyItem{e->T2, T2cur, n2 * 4};
+        L.c[1] = CopyItem{e->S12, S12cur, n2};
+        L.c[2] = CopyItem{e->T3, T3cur, n3 * 4};
+        L.c[3] = CopyItem{e->Tp3, Tp3cur, n3 * 4};
+        L.c[4] = CopyItem{e->S13, S13cur, n3};
+        L.c[5] = CopyItem{e->T1, e->T1c, n1 * 4};
+        launch_copy_multi(L, 6, s);
+    }
     // Level0 S1/S2 (cF:3628-3630)
     launch_l0_scatter(d, e->S13, e->S23, e->L0S1, e->s2snap, e->s2box, n3, s);
     e->use_kc = false;

@@ -203,8 +203,20 @@ void launch_src3_tables(const GmStepDesc* d, const DevMat* m_dev, int nrows, int
 void launch_acc_l3(const GmStepDesc* d, float* accum, float* max_accum, float* acc3, float* macc3,
                    int64_t n3, int to_l3, hipStream_t s);
 void launch_props_kc(const DevMat* m_dev, const float* T, const uint8_t* S1, int64_t nn,
-                     int64_t n_sub, float* kc, hipStream_t s);
-void launch_axis_tables(const GmPair& p, const GmStepDesc* d, hipStream_t s);
+                     int64_t n_sub, float* kc, hipStream_t s, const float* T2 = nullptr,
+                     const uint8_t* S12 = nullptr, int64_t nn2 = 0, int64_t n_sub2 = 0,
+                     float* kc2 = nullptr);
+// device-to-device copies batched into one launch (hipMalloc'd buffers)
+struct CopyItem {
+    void* dst;
+    const void* src;
+    int64_t bytes;
+};
+struct CopyList {
+    CopyItem c[8];
+};
+void launch_copy_multi(const CopyList& L, int n, hipStream_t s);
+void launch_axis_tables(const GmPair* p, int np, const GmStepDesc* d, hipStream_t s);
 void launch_source_tables(const GmPair& p, const GmStepDesc* d, const DevMat& m,
                           int row0, int npos, int use_rows, const float* rows_dev,
                           float* G, int nmax, hipStream_t s);

@@ -1026,28 +1026,71 @@ void launch_level_step_kc(const LSArgs& a0, const GmStepDesc* d, const gm_grid& 
     }
 }
 
-// k, cp of (T, S1) as the next level step's computeStateProperties returns them
-__global__ void k_props_kc(const DevMat* __restrict__ m, const float* __restrict__ T,
-                           const uint8_t* __restrict__ S1, int64_t nn, int64_t n_sub,
-                           float* __restrict__ kc) {
+// k, cp of (T, S1) as the next level step's computeStateProperties returns them;
+// blockIdx.y picks one of up to two fields
+struct PropsJob {
+    const float* T;
+    const uint8_t* S1;
+    int64_t nn, n_sub;
+    float* kc;
+};
+struct PropsJobs {
+    PropsJob j[2];
+};
+
+__global__ void k_props_kc(const DevMat* __restrict__ m, PropsJobs jobs) {
     __shared__ float sTab[PTW] __attribute__((aligned(16)));
     stage_tables(m, sTab, threadIdx.x, blockDim.x);
     __syncthreads();
+    const PropsJob& J = jobs.j[blockIdx.y];
     PropC pc = load_pc(m);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < J.nn;
          i += (int64_t)gridDim.x * blockDim.x) {
         float kk, cc;
         bool s1;
-        props_c(pc, sTab, T[i], S1[i] != 0, i < n_sub, kk, cc, s1);
-        kc[i] = kk;
-        kc[nn + i] = cc;
+        props_c(pc, sTab, J.T[i], J.S1[i] != 0, i < J.n_sub, kk, cc, s1);
+        J.kc[i] = kk;
+        J.kc[J.nn + i] = cc;
     }
 }
 
 void launch_props_kc(const DevMat* m, const float* T, const uint8_t* S1, int64_t nn,
-                     int64_t n_sub, float* kc, hipStream_t s) {
-    const int64_t nb = std::min<int64_t>((nn + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_props_kc, dim3((unsigned)nb), dim3(256), 0, s, m, T, S1, nn, n_sub, kc);
+                     int64_t n_sub, float* kc, hipStream_t s, const float* T2,
+                     const uint8_t* S12, int64_t nn2, int64_t n_sub2, float* kc2) {
+    PropsJobs jobs;
+    jobs.j[0] = PropsJob{T, S1, nn, n_sub, kc};
+    jobs.j[1] = PropsJob{T2, S12, nn2, n_sub2, kc2};
+    const int nj = T2 ? 2 : 1;
+    const int64_t nb = std::min<int64_t>((std::max(nn, T2 ? nn2 : 0) + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_props_kc, dim3((unsigned)nb, nj), dim3(256), 0, s, m, jobs);
+}
+
+// Up to 8 device-to-device copies in one launch (blockIdx.y picks the copy):
+// 16-byte chunks, then the tail bytes.
+__global__ void k_copy_multi(CopyList L) {
+    const CopyItem& c = L.c[blockIdx.y];
+    if ((((uintptr_t)c.dst) | ((uintptr_t)c.src)) & 15) {  // not 16-byte aligned: bytes
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < c.bytes;
+             i += (int64_t)gridDim.x * blockDim.x)
+            ((uint8_t*)c.dst)[i] = ((const uint8_t*)c.src)[i];
+        return;
+    }
+    const int64_t n16 = c.bytes >> 4;
+    const uint4* src = (const uint4*)c.src;
+    uint4* dst = (uint4*)c.dst;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+    const int64_t tail = c.bytes - (n16 << 4);
+    if (blockIdx.x == 0 && threadIdx.x < tail)
+        ((uint8_t*)c.dst)[(n16 << 4) + threadIdx.x] = ((const uint8_t*)c.src)[(n16 << 4) + threadIdx.x];
+}
+
+void launch_copy_multi(const CopyList& L, int n, hipStream_t s) {
+    int64_t mx = 0;
+    for (int i = 0; i < n; ++i) mx = std::max(mx, L.c[i].bytes >> 4);
+    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>((mx + 255) / 256, 2048));
+    if (n > 0) hipLaunchKernelGGL(k_copy_multi, dim3((unsigned)nb, n), dim3(256), 0, s, L);
 }
 
 // Separable source factors of every L3 node line for the desc rows:
@@ -1175,7 +1218,12 @@ __device__ __forceinline__ float gauss_x(const gm_grid& gf, int d, int e, int gg
     return gg == 0 ? (xa * GA + xb * GB) : (xa * GB + xb * GA);
 }
 
-__global__ void k_axis_tables(GmPair p, const GmStepDesc* __restrict__ d) {
+struct GmPairs3 {
+    GmPair p[3];
+};
+
+__global__ void k_axis_tables(GmPairs3 ps, const GmStepDesc* __restrict__ d) {
+    const GmPair& p = ps.p[blockIdx.y];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int ax = 0;
     while (ax < 3 && t >= p.nef[ax]) {
@@ -1214,9 +1262,14 @@ __global__ void k_axis_tables(GmPair p, const GmStepDesc* __restrict__ d) {
     }
 }
 
-void launch_axis_tables(const GmPair& p, const GmStepDesc* d, hipStream_t s) {
-    const int n = p.nef[0] + p.nef[1] + p.nef[2];
-    hipLaunchKernelGGL(k_axis_tables, dim3((n + 255) / 256), dim3(256), 0, s, p, d);
+void launch_axis_tables(const GmPair* p, int np, const GmStepDesc* d, hipStream_t s) {
+    GmPairs3 ps;
+    int n = 0;
+    for (int i = 0; i < np && i < 3; ++i) {
+        ps.p[i] = p[i];
+        n = std::max(n, p[i].nef[0] + p[i].nef[1] + p[i].nef[2]);
+    }
+    hipLaunchKernelGGL(k_axis_tables, dim3((n + 255) / 256, np), dim3(256), 0, s, ps, d);
 }
 
 // ---------------------------------------------------------------------------
