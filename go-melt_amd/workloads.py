@@ -139,10 +139,11 @@ def production():
     return inp, raster_gcode(7.5, 12.5, 9.0, 0.1, 20)
 
 
-def tiny(n=(8, 8, 4), n2=2, n3=2, track=1.0, props=None, velocity=100.0):
+def tiny(n=(8, 8, 4), n2=2, n3=2, track=1.0, props=None, velocity=100.0, dt3=1e-3):
     """Small 3-level deck: L1 over 8x8x4 mm, L2 4x4x2 mm, L3 2x2x1 mm.
 
-    dt3 = 1 ms; at the default 100 mm/s the track takes 10 rows per mm."""
+    dt3 = 1 ms by default; at the default 100 mm/s the track then takes 10
+    rows per mm.  Finer meshes need a smaller dt3 for the explicit scheme."""
     inp = {
         "Level1": _level(n, (0, 8), (0, 8), (-4, 0), 300.0),
         "Level2": _level(n, (2, 6), (2, 6), (-2, 0)),
@@ -150,7 +151,7 @@ def tiny(n=(8, 8, 4), n2=2, n3=2, track=1.0, props=None, velocity=100.0):
         "properties": _props(**dict(dict(laser_center=[4.0, 4.0, 0.0, 0, 0, 0, 0],
                                          laser_radius=0.5, laser_depth=0.5, laser_power=2000.0),
                                     **(props or {}))),
-        "nonmesh": _nonmesh(subcycle_num_L2=n2, subcycle_num_L3=n3, timestep_L3=1e-3,
+        "nonmesh": _nonmesh(subcycle_num_L2=n2, subcycle_num_L3=n3, timestep_L3=dt3,
                             laser_velocity=velocity),
     }
     gcode = f"G0 X4 Y4 Z0\nG1 X{4 + track:g} Y4 Z0\n"

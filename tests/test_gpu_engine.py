@@ -54,7 +54,11 @@ CLOSE_K = [[300.0, 1000.0, 1001.0, 1600.0, 1610.0, 3000.0], [10.0, 18.0, 18.5, 2
 CLOSE_CP = [[300.0, 700.0, 1500.0, 2500.0], [420.0, 500.0, 650.0, 700.0]]
 
 
-@pytest.mark.parametrize("case", ["plain", "melt", "tables", "tables_close", "table_k_only"])
+# wide_x / wide_y: 81 nodes along one axis, so the subcycle kernels' lane
+# strips (62 positions per wave, restriction strips of 60 fine columns) split
+# the mesh and coarse nodes collect partial sums from two strips
+@pytest.mark.parametrize("case", ["plain", "melt", "tables", "tables_close", "table_k_only",
+                                  "wide_x", "wide_y"])
 def test_driver_vs_oracle(oracle, tmp_path, case):
     props = {}
     if case == "melt":
@@ -72,7 +76,12 @@ def test_driver_vs_oracle(oracle, tmp_path, case):
     if case == "table_k_only":
         props = {"laser_power": 600.0, "thermal_conductivity_bulk_table": W.K_TABLE,
                  "heat_capacity_solid_a1": 0.174}
-    inp, g = W.tiny(n=(8, 8, 4), n2=2, n3=3, track=1.6, props=props)
+    if case.startswith("wide"):
+        # 0.025 mm L3 spacing: dt3 = 10 us keeps the explicit scheme stable
+        n = (80, 8, 4) if case == "wide_x" else (8, 80, 4)
+        inp, g = W.tiny(n=n, n2=2, n3=3, track=0.05, props=props, dt3=1e-5)
+    else:
+        inp, g = W.tiny(n=(8, 8, 4), n2=2, n3=3, track=1.6, props=props)
     out, st = _run_both(oracle, inp, g, tmp_path)
     assert st["subcycle"] >= 1 and st["single"] >= 1
     for k in (1, 2, 3):
