@@ -1587,8 +1587,10 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
     double invR[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        R[a] = (int)rintf(gc.h[a] / gf.h[a]);
-        off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) / gf.h[a]);
+        // integer ratio and offset: one hardware reciprocal is exact enough to round
+        const float ih = __builtin_amdgcn_rcpf(gf.h[a]);
+        R[a] = (int)rintf(gc.h[a] * ih);
+        off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) * ih);
         invR[a] = 1.0 / (double)R[a];
     }
 
@@ -1815,8 +1817,10 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
     int R[3], off[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        R[a] = (int)rintf(gc.h[a] / gf.h[a]);
-        off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) / gf.h[a]);
+        // integer ratio and offset: one hardware reciprocal is exact enough to round
+        const float ih = __builtin_amdgcn_rcpf(gf.h[a]);
+        R[a] = (int)rintf(gc.h[a] * ih);
+        off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) * ih);
     }
     const int W = R[0] * (61 / R[0]);
     const int a0 = (R[0] - off[0] % R[0]) % R[0];  // first fine x on a coarse node
@@ -2217,8 +2221,10 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
     int f0[3], fr[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        f0[a] = (int)rintf((xyz.x0[a] - fg.x0[a]) / fg.h[a]);
-        fr[a] = (int)rintf(cg.h[a] / fg.h[a]);
+        // integer offset and ratio: one hardware reciprocal is exact enough to round
+        const float ih = __builtin_amdgcn_rcpf(fg.h[a]);
+        f0[a] = (int)rintf((xyz.x0[a] - fg.x0[a]) * ih);
+        fr[a] = (int)rintf(cg.h[a] * ih);
     }
     const int fnx = fg.n[0], fny = fg.n[1], fnxy = fnx * fny;
     if ((int)blockIdx.x < ntp_blocks) {
@@ -2235,11 +2241,11 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
         int cx0, cx1, cy0, cy1, cz0, cz1, jx0, jx1, jy0, jy1, jz0, jz1;
         float wx, wy, wz;
         bool ix0, ix1, iy0, iy1, iz0, iz1;
-        tp_axis(ix, f0[0], fr[0], 1.0f / fr[0], b.lo[0], b.n[0], cg.n[0], fg.n[0], cx0, cx1, wx,
+        tp_axis(ix, f0[0], fr[0], __builtin_amdgcn_rcpf((float)fr[0]), b.lo[0], b.n[0], cg.n[0], fg.n[0], cx0, cx1, wx,
                 ix0, ix1, jx0, jx1);
-        tp_axis(iy, f0[1], fr[1], 1.0f / fr[1], b.lo[1], b.n[1], cg.n[1], fg.n[1], cy0, cy1, wy,
+        tp_axis(iy, f0[1], fr[1], __builtin_amdgcn_rcpf((float)fr[1]), b.lo[1], b.n[1], cg.n[1], fg.n[1], cy0, cy1, wy,
                 iy0, iy1, jy0, jy1);
-        tp_axis(iz, f0[2], fr[2], 1.0f / fr[2], b.lo[2], b.n[2], cg.n[2], fg.n[2], cz0, cz1, wz,
+        tp_axis(iz, f0[2], fr[2], __builtin_amdgcn_rcpf((float)fr[2]), b.lo[2], b.n[2], cg.n[2], fg.n[2], cz0, cz1, wz,
                 iz0, iz1, jz0, jz1);
         const int t = q + fnxy * iz;
         float v;
