@@ -1231,7 +1231,9 @@ static void subcycle_body(gm_engine* e) {
         const uint8_t* S13in = S13cur;
         for (int j = 0; j < N3; ++j) {
             float* T3o = e->T3b[b3];
-            uint8_t* S13o = e->S13b[b3];
+            // L3 S1 of Part 1 in a work buffer: the first step writes all of
+            // it, later steps update it in place (written where it flips)
+            uint8_t* S13o = e->S13b[0];
             b3 ^= 1;
             run_l3_kc(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, nullptr,
                       nullptr);
@@ -1264,7 +1266,7 @@ static void subcycle_body(gm_engine* e) {
     Tp3cur = e->Tp3;
     S13cur = e->S13;
     const uint8_t* S23cur = e->S23;
-    int b2s = 0, b3s = 0, bs2 = 0;
+    int b2s = 0, b3s = 0;
     for (int k = 0; k < N2; ++k) {
         const float* Gk = e->G2 + (size_t)k * N3 * 3 * e->gmax[2];
         // flux + mass restriction onto L2 (cF:3496-3508)
@@ -1281,9 +1283,8 @@ static void subcycle_body(gm_engine* e) {
         for (int j = 0; j < N3; ++j) {
             float* T3o = (j == N3 - 1) ? e->L3all[k] : e->T3b[b3s];
             if (j != N3 - 1) b3s ^= 1;
-            uint8_t* S13o = e->S13b[bs2];
-            uint8_t* S23o = e->S23;  // melt state updated in place (written where it flips)
-            bs2 ^= 1;
+            uint8_t* S13o = e->S13;  // S1 and the melt state of Part 2 updated in place
+            uint8_t* S23o = e->S23;
             run_l3_kc(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, S23in, S23o);
             T3in = T3o;
             S13in = S13o;
@@ -1306,9 +1307,8 @@ static void subcycle_body(gm_engine* e) {
         L.c[1] = CopyItem{e->S12, S12cur, n2};
         L.c[2] = CopyItem{e->T3, T3cur, n3 * 4};
         L.c[3] = CopyItem{e->Tp3, Tp3cur, n3 * 4};
-        L.c[4] = CopyItem{e->S13, S13cur, n3};
-        L.c[5] = CopyItem{e->T1, e->T1c, n1 * 4};
-        launch_copy_multi(L, 6, s);
+        L.c[4] = CopyItem{e->T1, e->T1c, n1 * 4};
+        launch_copy_multi(L, 5, s);
     }
     // Level0 S1/S2 (cF:3628-3630)
     launch_l0_scatter(d, e->S13, e->S23, e->L0S1, e->s2snap, e->s2box, n3, s);

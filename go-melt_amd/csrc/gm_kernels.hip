@@ -921,6 +921,7 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
     // ---- outputs --------------------------------------------------------------
     const auto rTo = buf_rsrc(a.Tout, nb4), rKCo = buf_rsrc(a.kcout, 2u * nb4);
     const auto rS1o = buf_rsrc(a.S1out, (uint32_t)nn);
+    const bool s1_inplace = (const void*)a.S1out == a.S1;
     const int nsub = (int)(a.n_sub < (int64_t)nn ? a.n_sub : (int64_t)nn);
 #pragma unroll
     for (int i = 0; i < ZC; ++i) {
@@ -973,7 +974,8 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
             if (REC) props_rec(pc, sTab, Tnew, s1, false, kn, cn, s1n);
             else props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
             buf_st(Tnew, rTo, oF[i][j]);
-            buf_st8(s1, rS1o, oB[i][j]);
+            // S1 updated in place (S1out == S1) is written only where it flips
+            buf_st8(s1, rS1o, (s1_inplace && s1 == (rS1_[i][j] != 0)) ? GM_OOB : oB[i][j]);
             buf_st(kn, rKCo, oF[i][j]);
             buf_st(cn, rKCo, oF[i][j], nb4);
             if (ACC) {
