@@ -886,6 +886,7 @@ static void run_l1(gm_engine* e, const float* corr, float* Tout, int dt_mode, fl
     a.mrow = e->mrow[1];
     a.w2 = e->w2[1];
     a.w3 = e->w3[1];
+    a.prec_ok = e->dm.pnb > 0;
     launch_level_step(a, e->d_desc, e->cfg.level[1], e->d_dm, e->s);
 }
 

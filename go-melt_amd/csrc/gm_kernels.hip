@@ -622,13 +622,22 @@ static void launch_ls(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, c
     }
 }
 
+static void launch_step_l1(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg,
+                           const DevMat* m, hipStream_t s);
+
 void launch_level_step(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
                        hipStream_t s) {
     if (a.kin) {
         launch_ls<false, false, GM_BC_NONE, false, true>(a, d, hg, m, s);
     } else if (a.s1_float) {  // L1
-        if (a.bc_mode == GM_BC_L1) launch_ls<true, false, GM_BC_L1, false, false>(a, d, hg, m, s);
-        else launch_ls<true, false, GM_BC_NONE, false, false>(a, d, hg, m, s);
+        if (a.bc_mode == GM_BC_L1 && !a.S1out && !a.corr_full && !a.Fext &&
+            a.src_mode == GM_SRC_NONE && (int64_t)hg.n[0] * hg.n[1] * hg.n[2] < ((int64_t)1 << 29)) {
+            launch_step_l1(a, d, hg, m, s);
+        } else if (a.bc_mode == GM_BC_L1) {
+            launch_ls<true, false, GM_BC_L1, false, false>(a, d, hg, m, s);
+        } else {
+            launch_ls<true, false, GM_BC_NONE, false, false>(a, d, hg, m, s);
+        }
     } else if (a.src_mode == GM_SRC_L3) {  // L3
         if (a.accum_on) launch_ls<false, true, GM_BC_FINE, true, false>(a, d, hg, m, s);
         else launch_ls<false, true, GM_BC_FINE, false, false>(a, d, hg, m, s);
@@ -995,6 +1004,235 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
             }
         }
     }
+}
+
+// L1 step (computeL1Temperature, cF:2135-2204 with the L1 faces of
+// cF:1568-1597) in the same wave-brick form: properties from (T, S1) of every
+// loaded position (L1 carries no k, cp), active element layers below
+// nez_act = ne_active / (nex ney), the surface flux on the top active plane,
+// the restricted correction box, inactive nodes (idx >= nn_active) at T_amb
+// and the Dirichlet values of the five L1 faces.
+template <bool REC, int YB, int ZC>
+__global__ void __launch_bounds__(64) k_step_l1(LSArgs a, const GmStepDesc* __restrict__ d,
+                                                const DevMat* __restrict__ m) {
+    const int nx = a.grid.n[0], ny = a.grid.n[1], nz = a.grid.n[2];
+    const int nxy = nx * ny;
+    const int nn = nxy * nz;
+    const int nex = nx - 1, ney = ny - 1;
+    const int nez_act = (int)(a.ne_active / ((int64_t)nex * ney));
+    const int nband = (ny + YB - 1) / YB;
+    const int nq = nx * nband;
+    const int nzc = (nz + ZC - 1) / ZC;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int zc = t % nzc, qs = t / nzc;
+    const int lane = threadIdx.x;
+    const int q = DXS * qs - 1 + lane;
+    const bool qv = q >= 0 && q < nq;
+    const int band = qv ? (int)((unsigned)q / (unsigned)nx) : 0;
+    const int x = qv ? q - band * nx : -1;
+    const int y0 = band * YB, z0 = zc * ZC;
+    const bool ein = qv && x < nex;
+    const bool olane = lane >= 1 && lane <= DXS && qv;
+    const int base = x + nx * y0 + nxy * z0;
+    const uint32_t nb4 = (uint32_t)nn * 4u;
+    const int nsub = (int)(a.n_sub < (int64_t)nn ? a.n_sub : (int64_t)nn);
+
+    const auto rT = buf_rsrc(a.T, nb4), rS = buf_rsrc(a.S1, nb4);
+    float Tw[ZC + 2][YB + 2], Sw[ZC + 2][YB + 2];
+    bool rowok[YB + 2];
+#pragma unroll
+    for (int r = 0; r < YB + 2; ++r) {
+        const int yy = y0 - 1 + r;
+        rowok[r] = qv && yy >= 0 && yy < ny;
+    }
+#pragma unroll
+    for (int p = 0; p < ZC + 2; ++p) {
+        const int zz = z0 - 1 + p;
+        const bool pok = zz >= 0 && zz < nz;
+#pragma unroll
+        for (int r = 0; r < YB + 2; ++r) {
+            const uint32_t vo =
+                (pok && rowok[r]) ? 4u * (uint32_t)(base + nx * (r - 1) + nxy * (p - 1)) : GM_OOB;
+            Tw[p][r] = buf_ld(rT, vo);
+            Sw[p][r] = buf_ld(rS, vo);
+        }
+    }
+    float corr[ZC][YB];
+#pragma unroll
+    for (int i = 0; i < ZC; ++i)
+#pragma unroll
+        for (int j = 0; j < YB; ++j) corr[i][j] = 0.0f;
+    if (a.corr) {
+        const int c0x = a.corr_info[0][0], c1x = a.corr_info[0][1];
+        const int c0y = a.corr_info[1][0], c1y = a.corr_info[1][1];
+        const int c0z = a.corr_info[2][0], c1z = a.corr_info[2][1];
+#pragma unroll
+        for (int i = 0; i < ZC; ++i)
+#pragma unroll
+            for (int j = 0; j < YB; ++j) {
+                const int bx = x - c0x, by = y0 + j - c0y, bz = z0 + i - c0z;
+                if (olane && y0 + j < ny && z0 + i < nz && bx >= 0 && bx <= c1x && by >= 0 &&
+                    by <= c1y && bz >= 0 && bz <= c1z)
+                    corr[i][j] = a.corr[bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz)];
+            }
+    }
+    __shared__ float sTab[PTW] __attribute__((aligned(16)));
+    stage_tables(m, sTab, lane, 64);
+    float dt = a.dt;
+    if (a.dt_mode == GM_DT_ROW) dt = d->rows[a.dt_index][5];
+    else if (a.dt_mode == GM_DT_L1) dt = d->dt1;
+    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
+    const float rmfac = dt / a.mrow;
+    const PropC pc = load_pc(m);
+    const float Tamb = m->T_amb;
+
+    lds_barrier();  // property table staged
+    // properties of every loaded position (computeStateProperties, cF:2567-2614)
+    float Kw[ZC + 2][YB + 2], Cw[ZC + 2][YB + 2];
+#pragma unroll
+    for (int p = 0; p < ZC + 2; ++p) {
+#pragma unroll
+        for (int r = 0; r < YB + 2; ++r) {
+            const int idx = base + nx * (r - 1) + nxy * (p - 1);
+            bool s1;
+            if (REC) props_rec(pc, sTab, Tw[p][r], Sw[p][r] > 0.499f, idx < nsub, Kw[p][r], Cw[p][r], s1);
+            else props_c(pc, sTab, Tw[p][r], Sw[p][r] > 0.499f, idx < nsub, Kw[p][r], Cw[p][r], s1);
+        }
+    }
+    // element means over the active layers
+    float Qk[ZC + 2][YB + 1], Qc[ZC + 2][YB + 1];
+#pragma unroll
+    for (int p = 0; p < ZC + 2; ++p) {
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) {
+            const int yy = y0 - 1 + r;
+            const bool ok = ein && yy >= 0 && yy < ney;
+            const float ak = Kw[p][r] + Kw[p][r + 1];
+            const float ac = Cw[p][r] + Cw[p][r + 1];
+            const float qk = ak + lane_next(ak);
+            const float qc = ac + lane_next(ac);
+            Qk[p][r] = ok ? qk : 0.0f;
+            Qc[p][r] = ok ? qc : 0.0f;
+        }
+    }
+    float Ek[ZC + 1][YB + 1], Ec[ZC + 1][YB + 1];
+#pragma unroll
+    for (int p = 0; p < ZC + 1; ++p) {
+        const int ez = z0 - 1 + p;
+        const float f = (ez >= 0 && ez < nez_act) ? 0.125f : 0.0f;
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) {
+            Ek[p][r] = (Qk[p][r] + Qk[p + 1][r]) * f;
+            Ec[p][r] = (Qc[p][r] + Qc[p + 1][r]) * f;
+        }
+    }
+    // surface flux of the top active plane (cF:2207-2301)
+    float Ftop[YB];
+#pragma unroll
+    for (int j = 0; j < YB; ++j) Ftop[j] = 0.0f;
+    const int ptop = nez_act - (z0 - 1);
+    if (a.conv && ptop >= 1 && ptop <= ZC) {
+        float Fq[YB + 1][4];
+#pragma unroll
+        for (int r = 0; r < YB + 1; ++r) {
+            float t0 = 0.0f, t3 = 0.0f;
+#pragma unroll
+            for (int p = 1; p <= ZC; ++p)
+                if (p == ptop) {
+                    t0 = Tw[p][r];
+                    t3 = Tw[p][r + 1];
+                }
+            const float t1 = lane_next(t0), t2 = lane_next(t3);
+            const int yy = y0 - 1 + r;
+            const bool ok = ein && yy >= 0 && yy < ney;
+#pragma unroll
+            for (int qd = 0; qd < 4; ++qd) {
+                const float Tq = n2(qd, 0) * t0 + n2(qd, 1) * t1 + n2(qd, 2) * t2 + n2(qd, 3) * t3;
+                const float fl = face_flux_hw(*m, Tq) * a.w2;
+                Fq[r][qd] = ok ? fl : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < YB; ++j) {
+            float cr = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int bxe = e & 1, bye = e >> 1;
+                const int ai = bye ? (bxe ? 0 : 1) : (bxe ? 3 : 2);
+#pragma unroll
+                for (int qd = 0; qd < 4; ++qd) {
+                    const float v = Fq[j + bye][qd];
+                    cr += n2(qd, ai) * (bxe ? v : lane_prev(v));
+                }
+            }
+            Ftop[j] = cr;
+        }
+    }
+
+    const auto rTo = buf_rsrc(a.Tout, nb4);
+#pragma unroll
+    for (int i = 0; i < ZC; ++i) {
+#pragma unroll
+        for (int j = 0; j < YB; ++j) {
+            const int yy = y0 + j, zz = z0 + i;
+            const float k100 = Ek[i][j], k110 = Ek[i][j + 1];
+            const float k101 = Ek[i + 1][j], k111 = Ek[i + 1][j + 1];
+            const float k000 = lane_prev(k100), k010 = lane_prev(k110);
+            const float k001 = lane_prev(k101), k011 = lane_prev(k111);
+            const float cs = (Ec[i][j] + Ec[i][j + 1]) + (Ec[i + 1][j] + Ec[i + 1][j + 1]);
+            const float Msum = cs + lane_prev(cs);
+            const float Ti = Tw[i + 1][j + 1];
+            const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011,
+                        Pz11 = k110 + k111;
+            const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011,
+                        Py11 = k101 + k111;
+            const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101,
+                        Px11 = k011 + k111;
+            const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
+            const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
+            const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
+            float KT = a.kc[0] * ((Fx0 + Fx1) * Ti);
+            KT += a.kc[1] * (Fx0 * lane_prev(Ti) + Fx1 * lane_next(Ti));
+            KT += a.kc[2] * (Fy0 * Tw[i + 1][j] + Fy1 * Tw[i + 1][j + 2]);
+            KT += a.kc[4] * (Fz0 * Tw[i][j + 1] + Fz1 * Tw[i + 2][j + 1]);
+            KT += a.kc[3] * (Pz00 * lane_prev(Tw[i + 1][j]) + Pz10 * lane_next(Tw[i + 1][j]) +
+                             Pz01 * lane_prev(Tw[i + 1][j + 2]) + Pz11 * lane_next(Tw[i + 1][j + 2]));
+            KT += a.kc[5] * (Py00 * lane_prev(Tw[i][j + 1]) + Py10 * lane_next(Tw[i][j + 1]) +
+                             Py01 * lane_prev(Tw[i + 2][j + 1]) + Py11 * lane_next(Tw[i + 2][j + 1]));
+            KT += a.kc[6] * (Px00 * Tw[i][j] + Px10 * Tw[i][j + 2] + Px01 * Tw[i + 2][j] +
+                             Px11 * Tw[i + 2][j + 2]);
+            KT += a.kc[7] * (k000 * lane_prev(Tw[i][j]) + k100 * lane_next(Tw[i][j]) +
+                             k010 * lane_prev(Tw[i][j + 2]) + k110 * lane_next(Tw[i][j + 2]) +
+                             k001 * lane_prev(Tw[i + 2][j]) + k101 * lane_next(Tw[i + 2][j]) +
+                             k011 * lane_prev(Tw[i + 2][j + 2]) + k111 * lane_next(Tw[i + 2][j + 2]));
+            float F = corr[i][j];
+            if (i + 1 == ptop) F += Ftop[j];
+            float Tnew = Ti + (F - KT) * (__builtin_amdgcn_rcpf(Msum) * rmfac);
+            const int idx = base + nx * j + nxy * i;
+            if (idx >= a.nn_active) Tnew = Tamb;
+            if (zz == 0) Tnew = a.bc_vals[4];
+            else if (x == nex) Tnew = a.bc_vals[3];
+            else if (x == 0) Tnew = a.bc_vals[2];
+            else if (yy == ney) Tnew = a.bc_vals[1];
+            else if (yy == 0) Tnew = a.bc_vals[0];
+            if (a.clamp && Tnew < Tamb) Tnew = Tamb;
+            const bool ok = olane && yy < ny && zz < nz;
+            buf_st(Tnew, rTo, ok ? 4u * (uint32_t)idx : GM_OOB);
+        }
+    }
+}
+
+static void launch_step_l1(const LSArgs& a0, const GmStepDesc* d, const gm_grid& hg,
+                           const DevMat* m, hipStream_t s) {
+    constexpr int YB = 1, ZC = 3;
+    LSArgs a = a0;
+    a.grid = hg;  // node counts (fixed for a level)
+    const int nq = hg.n[0] * ((hg.n[1] + YB - 1) / YB);
+    const int nb = ((nq + DXS - 1) / DXS) * ((hg.n[2] + ZC - 1) / ZC);
+    if (a.prec_ok)
+        hipLaunchKernelGGL((k_step_l1<true, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
+    else
+        hipLaunchKernelGGL((k_step_l1<false, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
 }
 
 template <bool SRC, bool ACC, bool REC>

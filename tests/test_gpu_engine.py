@@ -57,8 +57,10 @@ CLOSE_CP = [[300.0, 700.0, 1500.0, 2500.0], [420.0, 500.0, 650.0, 700.0]]
 # wide_x / wide_y: 81 nodes along one axis, so the subcycle kernels' lane
 # strips (62 positions per wave, restriction strips of 60 fine columns) split
 # the mesh and coarse nodes collect partial sums from two strips
+# l1_layers: Level1 reaches 2 mm above the laser plane, so its top layers are
+# inactive (the L1 step's active-element and inactive-node rules)
 @pytest.mark.parametrize("case", ["plain", "melt", "tables", "tables_close", "table_k_only",
-                                  "wide_x", "wide_y"])
+                                  "wide_x", "wide_y", "l1_layers"])
 def test_driver_vs_oracle(oracle, tmp_path, case):
     props = {}
     if case == "melt":
@@ -82,6 +84,9 @@ def test_driver_vs_oracle(oracle, tmp_path, case):
         inp, g = W.tiny(n=n, n2=2, n3=3, track=0.05, props=props, dt3=1e-5)
     else:
         inp, g = W.tiny(n=(8, 8, 4), n2=2, n3=3, track=1.6, props=props)
+    if case == "l1_layers":
+        inp["Level1"]["elements"] = [8, 8, 6]
+        inp["Level1"]["bounds"]["z"] = [-4, 2]
     out, st = _run_both(oracle, inp, g, tmp_path)
     assert st["subcycle"] >= 1 and st["single"] >= 1
     for k in (1, 2, 3):
