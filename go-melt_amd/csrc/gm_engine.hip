@@ -1111,6 +1111,17 @@ static void eng_restrict(gm_engine* e, int i, RArgs r) {
         r.R0 = e->strip_R0[i];
         r.a0 = e->strip_a0[i];
         r.dpp_ok = r.R0 > 0;
+        if (r.dpp_ok) {
+            const gm_grid& gc = e->host.g[p.coarse];
+            const gm_grid& gf = e->host.g[p.fine];
+            r.R1 = (int)std::lround((double)gc.h[1] / gf.h[1]);
+            r.R2 = (int)std::lround((double)gc.h[2] / gf.h[2]);
+            r.W = r.R0 * (61 / r.R0);
+            r.invR[0] = 1.0 / r.R0;
+            r.invR[1] = 1.0 / r.R1;
+            r.invR[2] = 1.0 / r.R2;
+            r.dpp_ok = r.R1 >= 1 && r.R2 >= 1;
+        }
         memcpy(r.kcs, e->kc[p.fine], sizeof(r.kcs));
         memcpy(r.mcs, e->mcs[p.fine], sizeof(r.mcs));
         launch_restrict_fast(p, e->d_desc, e->d_dm, r, e->part[i], e->s);

@@ -184,6 +184,8 @@ struct RArgs {
     // coarse node, fixed while the pair's meshes do not move relative to each
     // other by a fraction of a coarse spacing (host-checked)
     int dpp_ok, R0, a0;
+    int R1, R2, W;        // y, z ratios; strip width R0 * floor(61 / R0)
+    double invR[3];       // 1 / ratio per axis (FP64 weights)
 };
 
 // ---- launchers (gm_kernels.hip) -------------------------------------------

@@ -1999,6 +1999,12 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
 // sums per coarse z node, then the lane on each coarse node gathers the
 // x weights (R - |d|)/R of its 2R-1 neighbours through DPP shifts, applies
 // the y weight and adds into the pair's FP64 node box.
+// floor(u / R) for 0 <= u < 2^20 from a reciprocal of R (rcp error far below
+// the 0.5 / R distance of (u + 0.5) / R from an integer)
+__device__ __forceinline__ int fdiv_small(int u, float invR) {
+    return (int)(((float)u + 0.5f) * invR);
+}
+
 __device__ __forceinline__ double dprev(double v) {  // lane l-1 (0 in lane 0)
     const uint64_t u = __builtin_bit_cast(uint64_t, v);
     const uint32_t lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x138, 0xf, 0xf, true);
@@ -2054,21 +2060,23 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
     const int nxy = nx * ny;
     const int nn = nxy * nz;
     const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
-    int R[3], off[3];
+    // ratios, strip width and phase from the host (the launch grid follows
+    // them); fine-origin offsets in fine spacings from the descriptor
+    const int R[3] = {r.R0, r.R1, r.R2};
+    const float iRf[3] = {__builtin_amdgcn_rcpf((float)r.R0), __builtin_amdgcn_rcpf((float)r.R1),
+                          __builtin_amdgcn_rcpf((float)r.R2)};
+    int off[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        // integer ratio and offset: one hardware reciprocal is exact enough to round
-        const float ih = __builtin_amdgcn_rcpf(gf.h[a]);
-        R[a] = (int)rintf(gc.h[a] * ih);
-        off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) * ih);
-    }
-    const int W = R[0] * (61 / R[0]);
-    const int a0 = (R[0] - off[0] % R[0]) % R[0];  // first fine x on a coarse node
+    for (int a = 0; a < 3; ++a)
+        off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) * __builtin_amdgcn_rcpf(gf.h[a]));
+    const int W = r.W, a0 = r.a0;  // a0: first fine x on a coarse node
     const int s0 = a0 > 0 ? 0 : 1;
     const int nzc = (nz + ZC - 1) / ZC;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int zc = t % nzc, rest = t / nzc;
-    const int y = rest % ny, st = rest / ny;
+    const int rest = fdiv_small(t, __builtin_amdgcn_rcpf((float)nzc));
+    const int zc = t - rest * nzc;
+    const int st = fdiv_small(rest, __builtin_amdgcn_rcpf((float)ny));
+    const int y = rest - st * ny;
     const int z0 = zc * ZC;
     const int X = a0 + (st + s0 - 1) * W;  // strip start (a coarse node)
     const int lane = threadIdx.x;
@@ -2133,8 +2141,8 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
     }
 
     // ---- residuals, folded over z into coarse z nodes -------------------------
-    const int czb = (z0 + off[2]) / R[2];
-    const double invR2 = 1.0 / (double)R[2];
+    const int czb = fdiv_small(z0 + off[2], iRf[2]);
+    const double invR2 = r.invR[2];
     double zacc[NZC];
 #pragma unroll
     for (int b = 0; b < NZC; ++b) zacc[b] = 0.0;
@@ -2146,7 +2154,7 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
         if (MASS) rv -= idt * stencil_dpp<3>(Ec, Dw, i, r.mcs);
         const double v = (own && zz < nz) ? (double)rv : 0.0;
         const int uz = zz + off[2];
-        const int cz = uz / R[2];
+        const int cz = fdiv_small(uz, iRf[2]);
         const int jz = uz - cz * R[2];
         const int lz = cz - czb;
         const double w1 = (double)jz * invR2, w0 = 1.0 - w1;
@@ -2159,21 +2167,22 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
 
     // ---- x gather on the coarse-node lanes, y weight, box add -----------------
     const int uy = y + off[1];
-    const int cy = uy / R[1];
+    const int cy = fdiv_small(uy, iRf[1]);
     const int jy = uy - cy * R[1];
-    const double wy1 = (double)jy / (double)R[1], wy0 = 1.0 - wy1;
-    const double invR0 = 1.0 / (double)R[0];
+    const double wy1 = (double)jy * r.invR[1], wy0 = 1.0 - wy1;
+    const double invR0 = r.invR[0];
     // coarse-node lanes, also those just outside the fine mesh whose support
-    // reaches into it (x + off is a multiple of R there, so the division is exact)
-    const bool owner = lane >= 1 && lane <= W + 1 && (lane - 1) % R[0] == 0;
-    const int cx = (x + off[0]) / R[0];
+    // reaches into it (x + off is a multiple of R there)
+    const int lq = fdiv_small(lane + R[0] - 1, iRf[0]) - 1;  // floor((lane - 1) / R)
+    const bool owner = lane >= 1 && lane <= W + 1 && lane - 1 == lq * R[0];
+    const int cx = fdiv_small(x + off[0] + 128 * R[0], iRf[0]) - 128;
     const int lo0 = pp.ax[0].info[0], lo1 = pp.ax[1].info[0], lo2 = pp.ax[2].info[0];
     const int nb0 = pp.ax[0].info[1] + 1, nb1 = pp.ax[1].info[1] + 1, nb2 = pp.ax[2].info[1] + 1;
     const int st0 = pp.cmax[0] + 1, st1 = pp.cmax[1] + 1;
 #pragma unroll
     for (int b = 0; b < NZC; ++b) {
         const int bz = czb + b - lo2;
-        if (b > (ZC + R[2] - 2) / R[2] + 1 || bz < 0 || bz >= nb2) continue;  // uniform
+        if (b > fdiv_small(ZC + R[2] - 2, iRf[2]) + 1 || bz < 0 || bz >= nb2) continue;  // uniform
         double V = zacc[b], vp = zacc[b], vn = zacc[b];
         for (int dd = 1; dd < R[0]; ++dd) {
             vp = dprev(vp);
@@ -2203,8 +2212,8 @@ void launch_restrict_fast(const GmPair& p, const GmStepDesc* d, const DevMat* m,
         // x strips of W = R * floor(61 / R) columns starting on coarse nodes,
         // one per fine row and ZC planes
         constexpr int ZC = 4;
-        const int R0 = r.R0, a0 = r.a0;
-        const int W = R0 * (61 / R0);
+        const int a0 = r.a0;
+        const int W = r.W;
         const int nst = (nx - a0 + W - 1) / W + (a0 > 0 ? 1 : 0);
         const int nbd = nst * ny * ((nz + ZC - 1) / ZC);
         if (fl && ms)
