@@ -185,6 +185,12 @@ __device__ __forceinline__ float src_axis(const gm_grid& g, int d, int i, float 
     return s;
 }
 
+// floor(u / R) for 0 <= u < 2^20 from a reciprocal of R (rcp error far below
+// the 0.5 / R distance of (u + 0.5) / R from an integer)
+__device__ __forceinline__ int fdiv_small(int u, float invR) {
+    return (int)(((float)u + 0.5f) * invR);
+}
+
 // Tile index of this block, mapped so that each XCD (blocks are dealt to the
 // 8 XCDs round-robin) works on a contiguous run of tiles and shares halos
 // through its own L2.
@@ -721,11 +727,13 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
     const int nq = nx * nband;
     const int nzc = (nz + ZC - 1) / ZC;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int zc = t % nzc, qs = t / nzc;
+    const int qs = fdiv_small(t, __builtin_amdgcn_rcpf((float)nzc)), zc = t - qs * nzc;
     const int lane = threadIdx.x;
     const int q = DXS * qs - 1 + lane;
     const bool qv = q >= 0 && q < nq;
-    const int band = qv ? (int)((unsigned)q / (unsigned)nx) : 0;
+    int band = 0;
+    if (nq < (1 << 20)) band = qv ? fdiv_small(q, __builtin_amdgcn_rcpf((float)nx)) : 0;
+    else band = qv ? (int)((unsigned)q / (unsigned)nx) : 0;
     const int x = qv ? q - band * nx : -1;
     const int y0 = band * YB, z0 = zc * ZC;
     const bool ein = qv && x < nex;  // element with min corner x exists in x
@@ -1024,11 +1032,13 @@ __global__ void __launch_bounds__(64) k_step_l1(LSArgs a, const GmStepDesc* __re
     const int nq = nx * nband;
     const int nzc = (nz + ZC - 1) / ZC;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int zc = t % nzc, qs = t / nzc;
+    const int qs = fdiv_small(t, __builtin_amdgcn_rcpf((float)nzc)), zc = t - qs * nzc;
     const int lane = threadIdx.x;
     const int q = DXS * qs - 1 + lane;
     const bool qv = q >= 0 && q < nq;
-    const int band = qv ? (int)((unsigned)q / (unsigned)nx) : 0;
+    int band = 0;
+    if (nq < (1 << 20)) band = qv ? fdiv_small(q, __builtin_amdgcn_rcpf((float)nx)) : 0;
+    else band = qv ? (int)((unsigned)q / (unsigned)nx) : 0;
     const int x = qv ? q - band * nx : -1;
     const int y0 = band * YB, z0 = zc * ZC;
     const bool ein = qv && x < nex;
@@ -1999,12 +2009,6 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
 // sums per coarse z node, then the lane on each coarse node gathers the
 // x weights (R - |d|)/R of its 2R-1 neighbours through DPP shifts, applies
 // the y weight and adds into the pair's FP64 node box.
-// floor(u / R) for 0 <= u < 2^20 from a reciprocal of R (rcp error far below
-// the 0.5 / R distance of (u + 0.5) / R from an integer)
-__device__ __forceinline__ int fdiv_small(int u, float invR) {
-    return (int)(((float)u + 0.5f) * invR);
-}
-
 __device__ __forceinline__ double dprev(double v) {  // lane l-1 (0 in lane 0)
     const uint64_t u = __builtin_bit_cast(uint64_t, v);
     const uint32_t lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x138, 0xf, 0xf, true);
