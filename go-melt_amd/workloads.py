@@ -139,6 +139,15 @@ def production():
     return inp, raster_gcode(7.5, 12.5, 9.0, 0.1, 20)
 
 
+def production_mirror():
+    """The production meshes with the part-scale mesh centred on y = 9 mm and
+    one straight track along y = 9 mm: every level is mirror-symmetric about
+    that line, so the temperature fields are too (a full-size check)."""
+    inp, _ = production()
+    inp["Level1"]["bounds"]["y"] = [-1.0, 19.0]
+    return inp, "G0 X7.5 Y9 Z0\nG1 X12.5 Y9 Z0\n"
+
+
 def tiny(n=(8, 8, 4), n2=2, n3=2, track=1.0, props=None, velocity=100.0, dt3=1e-3):
     """Small 3-level deck: L1 over 8x8x4 mm, L2 4x4x2 mm, L3 2x2x1 mm.
 
