@@ -181,3 +181,19 @@ def test_production_size_graph_matches_direct(tmp_path):
     b = _run_gpu(inp, g, tmp_path / "d", chunks=3, graph=False)
     for lv in (1, 2, 3):
         assert np.array_equal(np.asarray(a.Levels[lv]["T0"]), np.asarray(b.Levels[lv]["T0"])), lv
+
+
+def test_driver_vs_oracle_two_layers(oracle, tmp_path):
+    """A second layer 1 mm up: the layer change (L1 activation, Level0
+    shift, window moves in z) between subcycled chunks of both layers."""
+    inp, _ = W.tiny(n=(8, 8, 4), n2=2, n3=3, track=1.6)
+    inp["Level1"]["elements"] = [8, 8, 6]
+    inp["Level1"]["bounds"]["z"] = [-4, 2]
+    inp["properties"]["layer_height"] = 1.0
+    g = "G0 X4 Y4 Z0\nG1 X5.6 Y4 Z0\nG0 X5.6 Y4.5 Z1\nG1 X4 Y4.5 Z1\n"
+    out, st = _run_both(oracle, inp, g, tmp_path)
+    assert st["subcycle"] >= 2
+    for k in (1, 2, 3):
+        a, b = out[k]
+        assert _relT(a, b) < 5e-4, (k, _relT(a, b))
+    assert np.array_equal(out["S1_3"][0], out["S1_3"][1])
