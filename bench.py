@@ -113,6 +113,13 @@ def main():
     # roofline of the dominant kernel: the L3 level step
     import ctypes as C
     ms = C.c_float()
+    mm_T = np.asarray(L[3]["T0"])  # before the timing replays below change the state
+    # GPU time of one subcycled L1 step: replays of its recorded graph (the
+    # wall-clock step above adds the window move and the host driver)
+    gms = C.c_float()
+    graph_ms = None
+    if n_sub > 0 and N.lib().gm_engine_time_kernel(run.eng.h, 0, 50, C.byref(gms)) == 0:
+        graph_ms = gms.value
     N.check(N.lib().gm_engine_time_kernel(run.eng.h, 3, a.kernel_iters, C.byref(ms)))
     nn3 = L[3]["nn"]
     bytes_per_launch = 10 * nn3          # FP32 T read + write, uint8 S1 read + write
@@ -122,7 +129,6 @@ def main():
         for line in open(TRAFFIC_FILE):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1])
-    mm_T = np.asarray(L[3]["T0"])
     result = {
         "metric": METRIC,
         "value": mm / wall,
@@ -145,6 +151,7 @@ def main():
             "subcycled_steps": n_sub,
             "single_steps": n_single,
             "peak_T_L3": float(mm_T.max()),
+            "subcycle_graph_ms": graph_ms,
             "device": N.device_name(),
         },
         "roofline": {

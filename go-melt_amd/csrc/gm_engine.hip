@@ -1550,6 +1550,24 @@ void gm_vmax(float* a, const float* b, int64_t n, hipStream_t s) {
 //           4: L3 -> L2 restriction (elements + nodes)
 extern "C" int gm_engine_time_kernel(gm_engine* e, int which, int iters, float* ms_per_launch) {
     if (iters < 1) return fail("iters must be >= 1");
+    if (which == 0) {
+        // replays of the recorded subcycle graph (the last L1 step's rows)
+        if (!e->graph_ok || !e->gexec) return fail("no recorded subcycle graph to time");
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipGraphLaunch(e->gexec, e->s));  // warm
+        HIPCHK(hipEventRecord(a, e->s));
+        for (int i = 0; i < iters; ++i) HIPCHK(hipGraphLaunch(e->gexec, e->s));
+        HIPCHK(hipEventRecord(b, e->s));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        *ms_per_launch = ms / (float)iters;
+        HIPCHK(hipEventDestroy(a));
+        HIPCHK(hipEventDestroy(b));
+        return 0;
+    }
     hipGraph_t g;
     hipGraphExec_t ge;
     HIPCHK(hipStreamSynchronize(e->s));

@@ -245,8 +245,9 @@ int gm_engine_minmax(gm_engine* e, float* minmax_host);
 /* Measurement helper: average duration (ms) of `iters` back-to-back launches
  * of one kernel of the L1 step on the engine's current state, recorded as a
  * hipGraph and timed with HIP events on the engine stream.  which = 3: L3
- * level step, 2: L2 level step, 1: L1 level step, 4: L3->L2 restriction.
- * Overwrites scratch buffers only.  Synchronous. */
+ * level step, 2: L2 level step, 1: L1 level step, 4: L3->L2 restriction
+ * (these overwrite scratch buffers only); which = 0: replays of the recorded
+ * subcycle graph (advances the engine state).  Synchronous. */
 int gm_engine_time_kernel(gm_engine* e, int which, int iters, float* ms_per_launch);
 
 /* Enable/disable hipGraph capture of gm_engine_subcycle (default enabled). */
