@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--deck", default="production", choices=["production", "single_track"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-graph-time", action="store_true",
+                   help="skip the replay timing of the subcycle graph")
     ap.add_argument("--kernel-iters", type=int, default=400)
     return ap.parse_args()
 
@@ -118,7 +120,8 @@ def main():
     # wall-clock step above adds the window move and the host driver)
     gms = C.c_float()
     graph_ms = None
-    if n_sub > 0 and N.lib().gm_engine_time_kernel(run.eng.h, 0, 50, C.byref(gms)) == 0:
+    if not a.no_graph_time and n_sub > 0 and \
+            N.lib().gm_engine_time_kernel(run.eng.h, 0, 50, C.byref(gms)) == 0:
         graph_ms = gms.value
     N.check(N.lib().gm_engine_time_kernel(run.eng.h, 3, a.kernel_iters, C.byref(ms)))
     nn3 = L[3]["nn"]
