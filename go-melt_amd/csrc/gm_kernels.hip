@@ -2096,7 +2096,6 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
     const auto rP = buf_rsrc(FLUX ? r.Tp : r.KC, nb4);
     const auto rA = buf_rsrc(MASS ? r.TpA : r.KC, nb4), rB = buf_rsrc(MASS ? r.TpB : r.KC, nb4);
     float Pw[ZC + 2][3], Dw[ZC + 2][3], Kw[ZC + 2][3], Cw[ZC + 2][3];
-    const bool sameB = FLUX && MASS && r.TpB == r.Tp;
 #pragma unroll
     for (int p = 0; p < ZC + 2; ++p) {
         const int zz = z0 - 1 + p;
@@ -2108,8 +2107,7 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
             Kw[p][rr] = buf_ld(rKC, vo);
             Cw[p][rr] = buf_ld(rKC, vo, nb4);
             Pw[p][rr] = FLUX ? buf_ld(rP, vo) : 0.0f;
-            // the corrector passes the flux T' again as the older mass T'
-            Dw[p][rr] = MASS ? buf_ld(rA, vo) - (sameB ? Pw[p][rr] : buf_ld(rB, vo)) : 0.0f;
+            Dw[p][rr] = MASS ? buf_ld(rA, vo) - buf_ld(rB, vo) : 0.0f;
         }
     }
     float dt = r.dt;
