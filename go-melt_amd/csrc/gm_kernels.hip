@@ -786,12 +786,11 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
             rS1_[i][j] = buf_ld8(rS1, oB[i][j]);
             if (ACC) {
                 // melt-time state on the L3 nodes (gathered from Level0 for
-                // the step, scattered back after it)
+                // the step, scattered back after it); accum and max_accum
+                // are read further down, only where the node is or was molten
                 const auto rS2p = buf_rsrc(a.S2prev, (uint32_t)nn);
-                const auto rA = buf_rsrc(a.accum, nb4), rM = buf_rsrc(a.max_accum, nb4);
                 rS2_[i][j] = buf_ld8(rS2p, oB[i][j]);
-                rAcc[i][j] = buf_ld(rA, oF[i][j]);
-                rMax[i][j] = buf_ld(rM, oF[i][j]);
+                rAcc[i][j] = rMax[i][j] = 0.0f;
             } else {
                 rS2_[i][j] = 0;
                 rAcc[i][j] = rMax[i][j] = 0.0f;
@@ -934,6 +933,18 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
     }
 
     lds_barrier();  // property table staged
+    if (ACC) {
+        // their latency overlaps the node updates below
+        const auto rA = buf_rsrc(a.accum, nb4), rM = buf_rsrc(a.max_accum, nb4);
+#pragma unroll
+        for (int i = 0; i < ZC; ++i)
+#pragma unroll
+            for (int j = 0; j < YB; ++j) {
+                const bool need = rS2_[i][j] != 0 || Tw[i + 1][j + 1] >= pc.Tliq;
+                rAcc[i][j] = buf_ld(rA, need ? oF[i][j] : GM_OOB);
+                rMax[i][j] = buf_ld(rM, need ? oF[i][j] : GM_OOB);
+            }
+    }
 
     // ---- outputs --------------------------------------------------------------
     const auto rTo = buf_rsrc(a.Tout, nb4), rKCo = buf_rsrc(a.kcout, 2u * nb4);
