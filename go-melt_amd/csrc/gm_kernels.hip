@@ -763,6 +763,37 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
             Cw[p][r] = buf_ld(rKC, vo, nb4);
         }
     }
+    // restricted correction on its node box
+    float corr[ZC][YB];
+#pragma unroll
+    for (int i = 0; i < ZC; ++i)
+#pragma unroll
+        for (int j = 0; j < YB; ++j) corr[i][j] = 0.0f;
+    if (a.corr || a.corr64) {
+        const int c0x = a.corr_info[0][0], c1x = a.corr_info[0][1];
+        const int c0y = a.corr_info[1][0], c1y = a.corr_info[1][1];
+        const int c0z = a.corr_info[2][0], c1z = a.corr_info[2][1];
+#pragma unroll
+        for (int i = 0; i < ZC; ++i)
+#pragma unroll
+            for (int j = 0; j < YB; ++j) {
+                const int bx = x - c0x, by = y0 + j - c0y, bz = z0 + i - c0z;
+                if ((olane && y0 + j < ny && z0 + i < nz) && bx >= 0 && bx <= c1x && by >= 0 && by <= c1y &&
+                    bz >= 0 && bz <= c1z) {
+                    const int k = bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz);
+                    if (a.corr) {
+                        corr[i][j] = a.corr[k];
+                    } else {
+                        // each box node is read by exactly one lane of the grid
+                        const double acc = a.corr64[k];
+                        a.corr64[k] = 0.0;
+                        float v = (float)acc;
+                        if (a.corrG) v += a.corrG[k] * a.corrG_scale;
+                        corr[i][j] = v;
+                    }
+                }
+            }
+    }
     // per-node inputs of the outputs
     const auto rS1 = buf_rsrc(a.S1, (uint32_t)nn);
     const auto rFa = buf_rsrc(a.bcA, nb4), rFb = buf_rsrc(a.bcB, nb4);
@@ -823,37 +854,6 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
         for (int j = 0; j < YB; ++j) sy[j] = (qv && y0 + j < ny) ? st[nx + y0 + j] : 0.0f;
 #pragma unroll
         for (int i = 0; i < ZC; ++i) sz[i] = (z0 + i < nz) ? st[nx + ny + z0 + i] : 0.0f;
-    }
-    // restricted correction on its node box
-    float corr[ZC][YB];
-#pragma unroll
-    for (int i = 0; i < ZC; ++i)
-#pragma unroll
-        for (int j = 0; j < YB; ++j) corr[i][j] = 0.0f;
-    if (a.corr || a.corr64) {
-        const int c0x = a.corr_info[0][0], c1x = a.corr_info[0][1];
-        const int c0y = a.corr_info[1][0], c1y = a.corr_info[1][1];
-        const int c0z = a.corr_info[2][0], c1z = a.corr_info[2][1];
-#pragma unroll
-        for (int i = 0; i < ZC; ++i)
-#pragma unroll
-            for (int j = 0; j < YB; ++j) {
-                const int bx = x - c0x, by = y0 + j - c0y, bz = z0 + i - c0z;
-                if (oB[i][j] != GM_OOB && bx >= 0 && bx <= c1x && by >= 0 && by <= c1y &&
-                    bz >= 0 && bz <= c1z) {
-                    const int k = bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz);
-                    if (a.corr) {
-                        corr[i][j] = a.corr[k];
-                    } else {
-                        // each box node is read by exactly one lane of the grid
-                        const double acc = a.corr64[k];
-                        a.corr64[k] = 0.0;
-                        float v = (float)acc;
-                        if (a.corrG) v += a.corrG[k] * a.corrG_scale;
-                        corr[i][j] = v;
-                    }
-                }
-            }
     }
     const PropC pc = load_pc(m);
     const float Tamb = m->T_amb;
