@@ -705,10 +705,13 @@ __device__ __forceinline__ void buf_st8(uint32_t v, __amdgpu_buffer_rsrc_t r, ui
 // square root and exponential (relative error a few 1e-7).
 __device__ __forceinline__ float face_flux_hw(const DevMat& m, float Tq) {
     Tq = (Tq > m.Tclip) ? m.Tclip : Tq;
-    const float invT = __builtin_amdgcn_rcpf(Tq);
-    const float E = m.Lev + m.cp_f * (Tq - m.T_amb);
-    const float mol = m.sqCM * __builtin_amdgcn_rsqf(Tq);
-    const float S = m.evc_cp * __expf(-m.CT * (invT - m.invTb)) * mol * E;
+    float S = 0.0f;
+    if (m.evc_cp != 0.0f) {  // evaporation (uniform: a zero coefficient skips it)
+        const float invT = __builtin_amdgcn_rcpf(Tq);
+        const float E = m.Lev + m.cp_f * (Tq - m.T_amb);
+        const float mol = m.sqCM * __builtin_amdgcn_rsqf(Tq);
+        S = m.evc_cp * __expf(-m.CT * (invT - m.invTb)) * mol * E;
+    }
     const float t2 = Tq * Tq;
     const float q = m.h_conv * (m.T_amb - Tq) + m.sigma_eps * (m.Tamb4 - t2 * t2) - S;
     return q * 1e-6f;
