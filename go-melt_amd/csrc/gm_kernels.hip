@@ -831,17 +831,8 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
             }
         }
     }
-    __shared__ float sTab[PTW] __attribute__((aligned(16)));
-    if (REC) {
-        // records known present: copy them without reading the bin count
-        const float4* src = (const float4*)&m->prec[0][0];
-        float4* dst = (float4*)sTab;
-#pragma unroll
-        for (int l = 0; l < (GM_PREC_BINS * 3 + 63) / 64; ++l)
-            if (lane + 64 * l < GM_PREC_BINS * 3) dst[lane + 64 * l] = src[lane + 64 * l];
-    } else {
-        stage_tables(m, sTab, lane, 64);
-    }
+    __shared__ float sTab[REC ? 4 : PTW] __attribute__((aligned(16)));
+    if (!REC) stage_tables(m, sTab, lane, 64);
 
     const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
     float dt = a.dt;
@@ -935,7 +926,7 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
         }
     }
 
-    lds_barrier();  // property table staged
+    if (!REC) lds_barrier();  // property table staged
     if (ACC) {
         // their latency overlaps the node updates below
         const auto rA = buf_rsrc(a.accum, nb4), rM = buf_rsrc(a.max_accum, nb4);
@@ -1002,7 +993,7 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
             const bool s1 = rS1_[i][j] != 0 || Ti >= pc.Tliq || idx < nsub;
             float kn, cn;
             bool s1n;
-            if (REC) props_rec(pc, sTab, Tnew, s1, false, kn, cn, s1n);
+            if (REC) props_rec(pc, &m->prec[0][0], Tnew, s1, false, kn, cn, s1n);
             else props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
             buf_st(Tnew, rTo, oF[i][j]);
             // S1 updated in place (S1out == S1) is written only where it flips
