@@ -1253,7 +1253,9 @@ static void launch_step_l1(const LSArgs& a0, const GmStepDesc* d, const gm_grid&
 template <bool SRC, bool ACC, bool REC>
 static void launch_dpp(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
                        hipStream_t s) {
-    constexpr int YB = 1, ZC = 3;  // measured best on a 101 x 101 x 51 mesh
+    // measured best on 101 x 101 x 51 meshes: 3 planes per brick for L3
+    // steps, 2 for L2 steps (whose correction-box reads favour more waves)
+    constexpr int YB = 1, ZC = SRC ? 3 : 2;
     const int nq = hg.n[0] * ((hg.n[1] + YB - 1) / YB);
     const int nb = ((nq + DXS - 1) / DXS) * ((hg.n[2] + ZC - 1) / ZC);
     hipLaunchKernelGGL((k_step_dpp<SRC, ACC, REC, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
