@@ -108,6 +108,8 @@ _SIGS = {
     "gm_source_l3": (_I, [_G, _M, _FP, _F, _P, _P]),
     "gm_level_source": (_I, [_G, _G, _M, _FP, _I, _P, _P]),
     "gm_conv_rad_bc": (_I, [_G, _M, _P, _I64, _P, _P]),
+    "gm_melt_pool": (_I, [_G, _P, _F, _P, _P]),
+    "gm_energy_balance": (_I, [_G, _I64, _P, _P, _F, _P, _P, _P, _P, _I, _F, _P, _P]),
     "gm_interpolate_points": (_I, [_G, _P, _G, _P, _P]),
     "gm_interpolate_matrix": (_I, [_G, _G, _P, _P, _P]),
     "gm_interpolate_w_matrix": (_I, [_P, _P, _I64, _P, _P, _P]),

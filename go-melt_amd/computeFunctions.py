@@ -755,6 +755,51 @@ def melting_temp(temps, delt_T, T_melt, accum_time, idx):
     return out
 
 
+def energyBalance(Level, ne, k, rhocp, dt, T, Fc, Corr, Tnew, bc_faces=0, T_clamp=-np.inf,
+                  stream=None):
+    """Energy audit of an explicit update T -> Tnew of a level (8 FP64 sums).
+
+    An addition with no reference counterpart; it checks the identity of
+    solveMatrixFreeFE (cF:582-642): the unconstrained update
+    Tt = T + dt (Fc + Corr - K T) / C satisfies sum C (Tt - T) = dt sum F.
+    Arguments as solveMatrixFreeFE, plus the Dirichlet faces (bits x0, x1,
+    y0, y1, z0) and the floor T_clamp whose nodes count as held.  Returns
+    [free C dT, held C dTt, dt sum F, dt sum K T, held reaction, scale,
+    residual, held count] (one call of gm_energy_balance)."""
+    g = _grid_of(Level)
+    kd, cd, Td, Tnd = _dev(k), _dev(rhocp), _dev(T), _dev(Tnew)
+    Fd = None if (np.isscalar(Fc) and Fc == 0) else _dev(Fc)
+    Cd = None if (np.isscalar(Corr) and Corr == 0) else _dev(Corr)
+    out = DeviceArray(8, np.float64)
+    sp = stream.ptr if stream is not None else None
+    N.check(N.lib().gm_energy_balance(C.byref(g), int(ne), kd.ptr, cd.ptr, float(dt), Td.ptr,
+                                      Fd.ptr if Fd else None, Cd.ptr if Cd else None, Tnd.ptr,
+                                      int(bc_faces), float(T_clamp), out.ptr, sp))
+    N.sync(sp)
+    return out.numpy(sp)
+
+
+def meltPoolDimensions(Level, T, T_liquidus, stream=None):
+    """Melt-pool size of a level's field, in mesh units (mm).
+
+    An addition with no reference counterpart: the reference only forms
+    S2 = T >= T_liquidus (cF:2585).  Returns (depth, width, length, n_molten,
+    ylo, yhi, xlo, xhi): depth below the top node plane of the lowest
+    T_liquidus crossing on a vertical grid line, and the y and x extents of
+    the molten top plane, with linear interpolation along edges (one call of
+    gm_melt_pool).  ``Level`` gives the mesh (its current ``node_coords``);
+    ``T`` is a device or host array of its nodes."""
+    g = _grid_of(Level)
+    Td = _dev(T)
+    out = DeviceArray(8)
+    sp = stream.ptr if stream is not None else None
+    N.check(N.lib().gm_melt_pool(C.byref(g), Td.ptr, float(T_liquidus), out.ptr, sp))
+    N.sync(sp)
+    d = out.numpy(sp)
+    return (float(d[0]), float(d[1]), float(d[2]), int(round(float(d[3]))), float(d[4]),
+            float(d[5]), float(d[6]), float(d[7]))
+
+
 def printLevelMaxMin(Ls, Lnames):
     """cF:3635-3665: print per-level min/max; exit(1) on NaN, <= 0 or > 1e5."""
     e = _eng(Ls)

@@ -238,6 +238,33 @@ extern "C" int gm_conv_rad_bc(const gm_grid* g, const gm_material* mat, const fl
     return 0;
 }
 
+extern "C" int gm_melt_pool(const gm_grid* g, const float* T, float T_liquidus, float* dims,
+                            void* stream) {
+    if (!g || !dims || !T) return fail("gm_melt_pool: null argument");
+    if (g->n[0] < 1 || g->n[1] < 1 || g->n[2] < 1 ||
+        (int64_t)g->n[0] * g->n[1] * g->n[2] >= (int64_t)1 << 31)
+        return fail("gm_melt_pool: node counts must be >= 1 and below 2^31 in total");
+    launch_melt_pool(*g, T, T_liquidus, dims, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_energy_balance(const gm_grid* g, int64_t ne_active, const float* k,
+                                 const float* rhocp, float dt, const float* T, const float* F,
+                                 const float* Corr, const float* T_new, int bc_faces,
+                                 float T_clamp, double* sums, void* stream) {
+    if (!g || !k || !rhocp || !T || !T_new || !sums)
+        return fail("gm_energy_balance: null argument");
+    if (g->n[0] < 2 || g->n[1] < 2 || g->n[2] < 2 ||
+        (int64_t)g->n[0] * g->n[1] * g->n[2] >= (int64_t)1 << 31)
+        return fail("gm_energy_balance: need >= 2 nodes per axis and < 2^31 nodes");
+    if (bc_faces < 0 || bc_faces > 31) return fail("gm_energy_balance: bc_faces is a 5-bit mask");
+    launch_energy(*g, ne_active, k, rhocp, T, F, Corr, T_new, bc_faces, T_clamp, dt, sums,
+                  (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 extern "C" int gm_interpolate_points(const gm_grid* src, const float* u, const gm_grid* tgt,
                                      float* out, void* stream) {
     launch_interp_grid(nullptr, -1, src, u, nullptr, -1, tgt, grid_nn(*tgt), out,

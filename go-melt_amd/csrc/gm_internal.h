@@ -274,6 +274,17 @@ void launch_melt_single(const GmStepDesc* d, const float* T3, const uint8_t* pre
                         const uint8_t* S2, float dt, float Tliq, float* accum, float* max_accum,
                         int64_t nn3, hipStream_t s);
 void launch_minmax(const float* T, int64_t n, float* out2, hipStream_t s);
+// melt-pool size of a level's field into dims (8 floats of device memory)
+void launch_melt_pool(const gm_grid& g, const float* T, float Tl, float* dims, hipStream_t s);
+// energy audit of an explicit update of one level (FP64 sums, 8 doubles)
+struct EnergyCoef {
+    double kc[8];   // unit-k box-element stiffness by neighbour pattern
+    double vol8;    // element volume / 8
+};
+EnergyCoef energy_coef(const gm_grid& g);
+void launch_energy(const gm_grid& g, int64_t ne_active, const float* k, const float* rc,
+                   const float* T, const float* F, const float* Corr, const float* Tn,
+                   int bc_faces, float T_clamp, float dt, double* sums, hipStream_t s);
 void launch_props(const DevMat& m, const float* T, const float* S1, int64_t nn, int64_t nsub,
                   float* S1o, uint8_t* S2o, float* k, float* c, hipStream_t s);
 void launch_source_l3_full(const gm_grid& g, const DevMat& m, float vx, float vy, float vz,

@@ -12,6 +12,7 @@
 // planes (k, rho*cp means) in LDS, evaluates the node properties once per
 // loaded plane node, and applies the 27-point stencil the box element's 8x8
 // matrices reduce to, summed by neighbour class.  No atomics, deterministic.
+#include <climits>
 #include <math.h>
 #include <algorithm>
 #include <vector>
@@ -2829,6 +2830,245 @@ __global__ void k_minmax(const float* __restrict__ T, int64_t n, float* out) {
 
 void launch_minmax(const float* T, int64_t n, float* out2, hipStream_t s) {
     hipLaunchKernelGGL(k_minmax, dim3(1), dim3(256), 0, s, T, n, out2);
+}
+
+// ---- melt-pool size ----------------------------------------------------------
+// The reference only forms S2 = T >= T_liquidus (cF:2585); the size below is
+// this library's own measure.  Depth: the largest distance from the mesh's top
+// plane down to where T crosses T_liquidus on a vertical edge whose upper node
+// is molten (linear along the edge; a molten bottom node counts its full
+// depth).  Width (length): the y (x) extent of the molten part of the top
+// plane, its ends placed where T crosses T_liquidus along y (x) edges, or on
+// the mesh boundary where a boundary node is molten.  Reductions in order-
+// preserving integer keys of the FP32 values (one atomic per wave and value).
+__device__ __forceinline__ int fkey(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : (i ^ 0x7fffffff);
+}
+__device__ __forceinline__ float kfloat(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff)); }
+
+// keys: [0] depth (max), [1] y lo (min), [2] y hi (max), [3] x lo (min),
+// [4] x hi (max), [5] molten node count
+__global__ void k_pool_init(int* keys) {
+    // maxima start at INT_MIN, minima (odd slots 1, 3) at INT_MAX, the count at 0
+    if (threadIdx.x < 6)
+        keys[threadIdx.x] = threadIdx.x == 5 ? 0 : ((threadIdx.x & 1) ? INT_MAX : INT_MIN);
+}
+
+__global__ void __launch_bounds__(256) k_pool(gm_grid g, const float* __restrict__ T, float Tl,
+                                              int* keys) {
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int nxy = nx * ny;
+    const int64_t nn = (int64_t)nxy * nz;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int kd = INT_MIN, kylo = INT_MAX, kyhi = INT_MIN, kxlo = INT_MAX, kxhi = INT_MIN, cnt = 0;
+    if (t < nn) {
+        int ix, iy, iz;
+        dec3(t, nx, ny, ix, iy, iz);
+        const float Ti = T[t];
+        const bool mi = Ti >= Tl;
+        cnt = mi ? 1 : 0;
+        const float ztop = g.x0[2] + (float)(nz - 1) * g.h[2];
+        if (mi) {
+            if (iz == 0) {
+                kd = fkey(ztop - g.x0[2]);
+            } else {
+                const float Tb = T[t - nxy];
+                if (!(Tb >= Tl)) {
+                    const float zc = coordf(g, 2, iz - 1) + g.h[2] * ((Tl - Tb) / (Ti - Tb));
+                    kd = fkey(ztop - zc);
+                }
+            }
+        }
+        if (iz == nz - 1) {
+            float ylo = INFINITY, yhi = -INFINITY, xlo = INFINITY, xhi = -INFINITY;
+            if (mi && iy == 0) ylo = coordf(g, 1, 0);
+            if (mi && iy == ny - 1) yhi = coordf(g, 1, ny - 1);
+            if (mi && ix == 0) xlo = coordf(g, 0, 0);
+            if (mi && ix == nx - 1) xhi = coordf(g, 0, nx - 1);
+            if (iy + 1 < ny) {
+                const float Tj = T[t + nx];
+                if (mi != (Tj >= Tl)) {
+                    const float yc = coordf(g, 1, iy) + g.h[1] * ((Tl - Ti) / (Tj - Ti));
+                    ylo = fminf(ylo, yc);
+                    yhi = fmaxf(yhi, yc);
+                }
+            }
+            if (ix + 1 < nx) {
+                const float Tj = T[t + 1];
+                if (mi != (Tj >= Tl)) {
+                    const float xc = coordf(g, 0, ix) + g.h[0] * ((Tl - Ti) / (Tj - Ti));
+                    xlo = fminf(xlo, xc);
+                    xhi = fmaxf(xhi, xc);
+                }
+            }
+            if (ylo < INFINITY) kylo = fkey(ylo);
+            if (yhi > -INFINITY) kyhi = fkey(yhi);
+            if (xlo < INFINITY) kxlo = fkey(xlo);
+            if (xhi > -INFINITY) kxhi = fkey(xhi);
+        }
+    }
+#pragma unroll
+    for (int sh = 32; sh >= 1; sh >>= 1) {
+        kd = max(kd, __shfl_xor(kd, sh, 64));
+        kylo = min(kylo, __shfl_xor(kylo, sh, 64));
+        kyhi = max(kyhi, __shfl_xor(kyhi, sh, 64));
+        kxlo = min(kxlo, __shfl_xor(kxlo, sh, 64));
+        kxhi = max(kxhi, __shfl_xor(kxhi, sh, 64));
+        cnt += __shfl_xor(cnt, sh, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (kd != INT_MIN) atomicMax(keys + 0, kd);
+        if (kylo != INT_MAX) atomicMin(keys + 1, kylo);
+        if (kyhi != INT_MIN) atomicMax(keys + 2, kyhi);
+        if (kxlo != INT_MAX) atomicMin(keys + 3, kxlo);
+        if (kxhi != INT_MIN) atomicMax(keys + 4, kxhi);
+        if (cnt) atomicAdd(keys + 5, cnt);
+    }
+}
+
+// keys -> dims: depth, width, length, molten node count, y lo, y hi, x lo, x hi
+// (sizes 0 and ends NaN where nothing is molten)
+__global__ void k_pool_finish(int* keys) {
+    if (threadIdx.x != 0) return;
+    const int k[6] = {keys[0], keys[1], keys[2], keys[3], keys[4], keys[5]};
+    float* o = (float*)keys;
+    const bool yok = k[1] != INT_MAX && k[2] != INT_MIN;
+    const bool xok = k[3] != INT_MAX && k[4] != INT_MIN;
+    o[0] = k[0] != INT_MIN ? kfloat(k[0]) : 0.0f;
+    o[1] = yok ? kfloat(k[2]) - kfloat(k[1]) : 0.0f;
+    o[2] = xok ? kfloat(k[4]) - kfloat(k[3]) : 0.0f;
+    o[3] = (float)k[5];
+    o[4] = yok ? kfloat(k[1]) : NAN;
+    o[5] = yok ? kfloat(k[2]) : NAN;
+    o[6] = xok ? kfloat(k[3]) : NAN;
+    o[7] = xok ? kfloat(k[4]) : NAN;
+}
+
+void launch_melt_pool(const gm_grid& g, const float* T, float Tl, float* dims, hipStream_t s) {
+    const int64_t nn = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    int* keys = (int*)dims;
+    hipLaunchKernelGGL(k_pool_init, dim3(1), dim3(64), 0, s, keys);
+    if (nn > 0) hipLaunchKernelGGL(k_pool, dim3(nblk(nn)), dim3(256), 0, s, g, T, Tl, keys);
+    hipLaunchKernelGGL(k_pool_finish, dim3(1), dim3(64), 0, s, keys);
+}
+
+// ---- energy balance of one explicit step ----------------------------------------
+// Audit of an explicit update T -> Tn of one level: with the lumped capacity
+// C_i = sum_e rho*cp_e V_e/8 and the stiffness K of the element-mean k
+// (the operator of solveMatrixFreeFE, cF:582-642), the unconstrained update is
+// Tt = T + dt (F - K T) / C.  Because K annihilates constants (sum_i (K T)_i
+// = 0), sum_i C_i (Tt - T)_i = dt sum_i F_i.  Nodes held by Dirichlet faces
+// or by the T >= T_clamp floor are counted with Tt (their reaction is
+// reported apart); every other node with Tn.  The relative residual
+// (sum - dt sum F) / (sum of absolute energy flows) is FP32 rounding for the
+// plain solver and measures the energy error of any other update (e.g. a
+// surrogate).  Per-node terms and sums in FP64, one atomic per wave and sum.
+// sums: [0] sum_free C (Tn - T), [1] sum_held C (Tt - T), [2] dt sum F,
+// [3] dt sum K T, [4] sum_held C (Tn - Tt), [5] scale, [6] residual,
+// [7] held nodes.
+__global__ void k_energy_init(double* sums) {
+    if (threadIdx.x < 8) sums[threadIdx.x] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_energy(gm_grid g, int64_t ne_active, EnergyCoef ec,
+                                                const float* __restrict__ k,
+                                                const float* __restrict__ rc,
+                                                const float* __restrict__ T,
+                                                const float* __restrict__ F,
+                                                const float* __restrict__ Corr,
+                                                const float* __restrict__ Tn, int bc_faces,
+                                                float T_clamp, double dt, double* sums) {
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
+    const int64_t nn = (int64_t)nx * ny * nz;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (t < nn) {
+        int ix, iy, iz;
+        dec3(t, nx, ny, ix, iy, iz);
+        double KT = 0.0, C = 0.0;
+        for (int e = 0; e < 8; ++e) {
+            // element with min corner (ix - 1 + ax, ...); this node is its local corner (1-ax, ...)
+            const int ax = e & 1, ay = (e >> 1) & 1, az = e >> 2;
+            const int ex = ix - 1 + ax, ey = iy - 1 + ay, ez = iz - 1 + az;
+            if (ex < 0 || ex >= nex || ey < 0 || ey >= ney || ez < 0 || ez >= nez) continue;
+            if ((int64_t)ex + (int64_t)nex * ((int64_t)ey + (int64_t)ney * ez) >= ne_active) continue;
+            double ks = 0.0, cs = 0.0, kt = 0.0;
+            for (int m = 0; m < 8; ++m) {
+                const int mx = m & 1, my = (m >> 1) & 1, mz = m >> 2;
+                const int64_t j = nid(ex + mx, ey + my, ez + mz, nx, ny);
+                ks += (double)k[j];
+                cs += (double)rc[j];
+                const int pat = (mx ^ (1 - ax)) | ((my ^ (1 - ay)) << 1) | ((mz ^ (1 - az)) << 2);
+                kt += ec.kc[pat] * (double)T[j];
+            }
+            KT += (ks * 0.125) * kt;
+            C += (cs * 0.125) * ec.vol8;
+        }
+        if (C > 0.0) {
+            const double Ti = T[t], Tni = Tn[t];
+            const double Fi = (F ? (double)F[t] : 0.0) + (Corr ? (double)Corr[t] : 0.0);
+            const double Tt = Ti + dt * (Fi - KT) / C;
+            const bool face = ((bc_faces & 1) && ix == 0) || ((bc_faces & 2) && ix == nex) ||
+                              ((bc_faces & 4) && iy == 0) || ((bc_faces & 8) && iy == ney) ||
+                              ((bc_faces & 16) && iz == 0);
+            const bool held = face || (Tni <= (double)T_clamp && Tt < (double)T_clamp);
+            if (held) {
+                v[1] = C * (Tt - Ti);
+                v[4] = C * (Tni - Tt);
+                v[7] = 1.0;
+            } else {
+                v[0] = C * (Tni - Ti);
+            }
+            v[2] = dt * Fi;
+            v[3] = dt * KT;
+            v[5] = fabs(dt * Fi) + fabs(held ? v[1] : v[0]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if (q == 6) continue;
+        double x = v[q];
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) x += __shfl_xor(x, sh, 64);
+        if ((threadIdx.x & 63) == 0 && x != 0.0) atomicAdd(sums + q, x);
+    }
+}
+
+__global__ void k_energy_finish(double* sums) {
+    if (threadIdx.x != 0) return;
+    const double sc = sums[5];
+    sums[6] = sc > 0.0 ? (sums[0] + sums[1] - sums[2]) / sc : 0.0;
+}
+
+EnergyCoef energy_coef(const gm_grid& g) {
+    EnergyCoef c;
+    const double h[3] = {g.h[0], g.h[1], g.h[2]};
+    for (int p = 0; p < 8; ++p) {
+        // box element, unit k: K = K1 (x) M1 (x) M1 + M1 (x) K1 (x) M1 + M1 (x) M1 (x) K1
+        // with K1 = [[1,-1],[-1,1]]/h and M1 = h [[2,1],[1,2]]/6, entry by neighbour pattern p
+        double kk[3], mm[3];
+        for (int d = 0; d < 3; ++d) {
+            const bool dif = (p >> d) & 1;
+            kk[d] = (dif ? -1.0 : 1.0) / h[d];
+            mm[d] = h[d] * (dif ? 1.0 : 2.0) / 6.0;
+        }
+        c.kc[p] = kk[0] * mm[1] * mm[2] + mm[0] * kk[1] * mm[2] + mm[0] * mm[1] * kk[2];
+    }
+    c.vol8 = h[0] * h[1] * h[2] / 8.0;
+    return c;
+}
+
+void launch_energy(const gm_grid& g, int64_t ne_active, const float* k, const float* rc,
+                   const float* T, const float* F, const float* Corr, const float* Tn,
+                   int bc_faces, float T_clamp, float dt, double* sums, hipStream_t s) {
+    const int64_t nn = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    hipLaunchKernelGGL(k_energy_init, dim3(1), dim3(64), 0, s, sums);
+    if (nn > 0)
+        hipLaunchKernelGGL(k_energy, dim3(nblk(nn)), dim3(256), 0, s, g, ne_active, energy_coef(g),
+                           k, rc, T, F, Corr, Tn, bc_faces, T_clamp, (double)dt, sums);
+    hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(64), 0, s, sums);
 }
 
 // ---- single-operator kernels (the facade's one-function entry points) ----

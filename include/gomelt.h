@@ -132,6 +132,48 @@ int gm_source_l3(const gm_grid* grid, const gm_material* mat, const float* v_hos
 int gm_level_source(const gm_grid* coarse, const gm_grid* fine, const gm_material* mat,
                     const float* rows_host, int n_pos, float* F_out, void* stream);
 
+/* Melt-pool size of a level's temperature field.  No reference counterpart:
+ * the reference only forms S2 = T >= T_liquidus (cF:2585); this is the
+ * measure the surrogate's error is judged by.  dims: 8 floats of device memory,
+ * written asynchronously on `stream`:
+ *   [0] depth  = max over vertical edges with a molten upper node of the
+ *                distance from the top node plane down to the T_liquidus
+ *                crossing (linear along the edge; a molten bottom node counts
+ *                the full mesh depth),
+ *   [1] width  = y extent of the molten part of the top node plane, its ends
+ *                at the T_liquidus crossings of y edges (or on the mesh
+ *                boundary where a boundary node is molten),
+ *   [2] length = the same along x,
+ *   [3] number of nodes with T >= T_liquidus,
+ *   [4..7] y lo, y hi, x lo, x hi of the molten top region (NaN if none).
+ * Sizes are 0 where nothing is molten. */
+int gm_melt_pool(const gm_grid* grid, const float* T, float T_liquidus, float* dims,
+                 void* stream);
+
+/* Energy balance of one explicit update T -> T_new of a level, the audit
+ * that decides whether an update other than the plain step (a surrogate) is
+ * kept.  No reference counterpart; it checks the discrete identity of
+ * solveMatrixFreeFE (cF:582-642): with the lumped capacity
+ * C_i = sum_e rhocp_e V_e / 8 and the element-mean-k stiffness K (K times a
+ * constant is 0), the unconstrained update Tt = T + dt (F + Corr - K T) / C
+ * satisfies sum_i C_i (Tt - T)_i = dt sum_i (F + Corr)_i.  k, rhocp, F, Corr
+ * have the meaning and units of gm_solve_matrix_free (F, Corr nullable).
+ * Nodes on the Dirichlet faces in bc_faces (bits 0..4: x = 0, x = max,
+ * y = 0, y = max, z = 0) and nodes held at the floor (T_new <= T_clamp while
+ * Tt < T_clamp; pass -INFINITY for no floor) count with Tt; all others with
+ * T_new.  Nodes in no active element are skipped.  sums: 8 doubles of device
+ * memory, written asynchronously on `stream`:
+ *   [0] sum over free nodes of C (T_new - T)      [1] held nodes, C (Tt - T)
+ *   [2] dt sum (F + Corr)                          [3] dt sum K T  (~ 0)
+ *   [4] held nodes, C (T_new - Tt) (energy the constraints add)
+ *   [5] scale: sum |dt (F + Corr)| + sum |C dT|    [6] residual ([0]+[1]-[2])/[5]
+ *   [7] number of held nodes.
+ * Per-node terms and sums in FP64. */
+int gm_energy_balance(const gm_grid* grid, int64_t ne_active, const float* k,
+                      const float* rhocp, float dt, const float* T, const float* F,
+                      const float* Corr, const float* T_new, int bc_faces, float T_clamp,
+                      double* sums, void* stream);
+
 /* computeConvRadBC (cF:2207-2301): adds the top-face convection, radiation
  * and evaporation flux of the top layer of elements 0..ne_active-1 to F. */
 int gm_conv_rad_bc(const gm_grid* grid, const gm_material* mat, const float* T,

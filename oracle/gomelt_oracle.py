@@ -1117,6 +1117,122 @@ def melting_temp(temps, dt, T_melt, accum_time, idx):
     return out
 
 
+def energy_balance(lv, ne, k, rhocp, dt, T, Fc, Corr, Tnew, bc_faces=0, T_clamp=-np.inf):
+    """Energy audit of an explicit update T -> Tnew (FP64).
+
+    No reference counterpart; it is the discrete identity of solveMatrixFreeFE
+    (cF:582-642): with the lumped capacity C = sum_e Me*dt (element-loop
+    scatter as in the solver) and the element-mean-k stiffness K, the
+    unconstrained update Tt = T + dt (F - K T) / C satisfies
+    sum C (Tt - T) = dt sum F, since K annihilates constants.  Nodes on the
+    faces in ``bc_faces`` (bits: x0, x1, y0, y1, z0) or held at the floor
+    (Tnew <= T_clamp while Tt < T_clamp) count with Tt, the others with Tnew.
+    Returns the 8 sums of gm_energy_balance: free C dT, held C dTt, dt sum F,
+    dt sum KT, held reaction, scale, residual, held count.
+    """
+    f64 = np.float64
+    N, dNdx, wq = computeQuad3dFemShapeFunctions(getSampleCoords(lv))
+    N, dNdx, wq = (np.asarray(a, f64) for a in (N, dNdx, wq))
+    w = wq[0, 0]
+    BTB = sum(dNdx[:, :, d].T @ dNdx[:, :, d] for d in range(3))
+    Mrow = ((N.T @ N) * w).sum(axis=0)
+    _, _, _, idx = _element_nodes(lv, ne)
+    k, rhocp, T, Tnew = (np.asarray(a, f64) for a in (k, rhocp, T, Tnew))
+    nn = len(T)
+    kvec = k[idx].mean(axis=0)
+    cvec = rhocp[idx].mean(axis=0)
+    KTe = (T[idx].T @ (BTB * w).T) * kvec[:, None]
+    KT = np.bincount(idx.T.reshape(-1), KTe.reshape(-1), minlength=nn)
+    C = np.bincount(idx.T.reshape(-1), (cvec[:, None] * Mrow[None, :]).reshape(-1), minlength=nn)
+    Fv = np.zeros(nn)
+    for a in (Fc, Corr):
+        if not np.isscalar(a):
+            Fv = Fv + np.asarray(a, f64)
+        elif a != 0:
+            Fv = Fv + float(a)
+    dt = float(dt)
+    act = C > 0
+    Tt = np.where(act, T + dt * (Fv - KT) / np.where(act, C, 1.0), T)
+    nx, ny, nz = lv["nodes"]
+    iz, iy, ix = np.unravel_index(np.arange(nn), (nz, ny, nx))
+    face = np.zeros(nn, bool)
+    for bit, m in enumerate((ix == 0, ix == nx - 1, iy == 0, iy == ny - 1, iz == 0)):
+        if bc_faces >> bit & 1:
+            face |= m
+    held = act & (face | ((Tnew <= T_clamp) & (Tt < T_clamp)))
+    free = act & ~held
+    s0 = float((C * (Tnew - T))[free].sum())
+    s1 = float((C * (Tt - T))[held].sum())
+    s2 = float(dt * Fv[act].sum())
+    s3 = float(dt * KT[act].sum())
+    s4 = float((C * (Tnew - Tt))[held].sum())
+    s5 = float(np.abs(dt * Fv[act]).sum() + np.abs(C * (Tnew - T))[free].sum()
+               + np.abs(C * (Tt - T))[held].sum())
+    s6 = (s0 + s1 - s2) / s5 if s5 > 0 else 0.0
+    return np.array([s0, s1, s2, s3, s4, s5, s6, float(held.sum())])
+
+
+def melt_pool_dims(T, axes, T_liq):
+    """Melt-pool depth, width and length of a level's field (FP64).
+
+    No reference counterpart: the reference only forms S2 = T >= T_liquidus
+    (cF:2585).  The definition is the library's own: the molten set is the
+    piecewise-linear interpolant of the nodal T along mesh edges at or above
+    T_liq.  Depth is measured from the top node plane down to the lowest
+    point of the molten set on any vertical grid line; width (length) is the
+    y (x) extent of the molten set on the top plane.  Here each grid line is
+    walked as a 1-D piecewise-linear function (a sorted list of the intervals
+    where it is >= T_liq), which is then reduced; the GPU kernel instead
+    evaluates edge crossings independently per node.
+    Returns (depth, width, length, n_molten, ylo, yhi, xlo, xhi).
+    """
+    x, y, z = (np.asarray(a, np.float64) for a in axes)
+    nx, ny, nz = len(x), len(y), len(z)
+    A = np.asarray(T, np.float64).reshape(nz, ny, nx)
+    Tl = float(T_liq)
+
+    def molten_intervals(vals, coords):
+        # maximal intervals [a, b] of coords where the linear interpolant is >= Tl
+        out = []
+        start = None
+        for i, v in enumerate(vals):
+            if v >= Tl:
+                if start is None:
+                    if i == 0:
+                        start = coords[0]
+                    else:
+                        u = vals[i - 1]
+                        start = coords[i - 1] + (coords[i] - coords[i - 1]) * (Tl - u) / (v - u)
+            elif start is not None:
+                u = vals[i - 1]
+                out.append((start, coords[i - 1] + (coords[i] - coords[i - 1]) * (Tl - u) / (v - u)))
+                start = None
+        if start is not None:
+            out.append((start, coords[-1]))
+        return out
+
+    ztop = z[-1]
+    depth = 0.0
+    for j in range(ny):
+        for i in range(nx):
+            col = A[:, j, i]
+            if not (col >= Tl).any():
+                continue
+            # the column read from the top down is a function of depth below ztop
+            for a, b in molten_intervals(col[::-1], ztop - z[::-1]):
+                depth = max(depth, b)
+    top = A[-1]
+    ys = [iv for i in range(nx) for iv in molten_intervals(top[:, i], y)]
+    xs = [iv for j in range(ny) for iv in molten_intervals(top[j, :], x)]
+    ylo = min((a for a, _ in ys), default=np.nan)
+    yhi = max((b for _, b in ys), default=np.nan)
+    xlo = min((a for a, _ in xs), default=np.nan)
+    xhi = max((b for _, b in xs), default=np.nan)
+    width = (yhi - ylo) if ys else 0.0
+    length = (xhi - xlo) if xs else 0.0
+    return depth, width, length, int((A >= Tl).sum()), ylo, yhi, xlo, xhi
+
+
 # ---------------------------------------------------------------------------
 # Toolpath  (cP:6-201)
 # ---------------------------------------------------------------------------

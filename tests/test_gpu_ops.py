@@ -163,3 +163,94 @@ def test_new_tprime(cf, oracle):
     b_tp, b_c = oracle.getNewTprime(fine, Tf, Tc, coarse, C2F)
     assert _rel(a_c, b_c) < 2e-6
     assert np.max(np.abs(a_tp - b_tp)) < 2e-3
+
+
+def _pool_level(n, x0, h):
+    ax = [np.float32(x0[d]) + np.arange(n[d], dtype=np.float32) * np.float32(h[d])
+          for d in range(3)]
+    return {"node_coords": ax, "nodes": list(n), "h": [np.float32(v) for v in h]}
+
+
+def _pool_fields(ax, rng):
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    X, Y, Z = (A.astype(np.float64) for A in (X, Y, Z))
+    ztop = float(ax[2][-1])
+    xc, yc = float(ax[0][len(ax[0]) // 3]), float(ax[1][len(ax[1]) // 2])
+    yield "gauss", 300.0 + 2500.0 * np.exp(-((X - xc) / 0.12) ** 2 - ((Y - yc) / 0.05) ** 2
+                                           - ((Z - ztop) / 0.04) ** 2)
+    yield "tent", 1900.0 - 2000.0 * np.abs(Y - yc) - 1000.0 * np.abs(X - xc) - 6000.0 * (ztop - Z)
+    yield "random", rng.uniform(1000.0, 2000.0, size=X.shape)
+    yield "cold", np.full(X.shape, 300.0)
+    yield "hot", np.full(X.shape, 2500.0)
+    buried = np.full(X.shape, 300.0)
+    buried[min(1, X.shape[0] - 1), min(2, X.shape[1] - 1), min(3, X.shape[2] - 1)] = 2000.0
+    yield "buried", buried
+
+
+@pytest.mark.parametrize("n", [(41, 33, 21), (101, 101, 51), (1, 1, 1), (5, 1, 3)])
+def test_melt_pool(cf, oracle, n):
+    """gm_melt_pool against the oracle's grid-line walk: depth, width, length,
+    ends and molten count.  FP32 coordinates and crossings: 1e-6 mm."""
+    rng = np.random.default_rng(11)
+    h = (0.0125, 0.0125, 0.00625)
+    lv = _pool_level(n, (1.25, -0.4, -0.3), h)
+    Tl = 1600.0
+    for name, T in _pool_fields(lv["node_coords"], rng):
+        T = np.ascontiguousarray(T, np.float32).reshape(-1)
+        got = cf.meltPoolDimensions(lv, T, Tl)
+        ref = oracle.melt_pool_dims(T, lv["node_coords"], Tl)
+        assert got[3] == ref[3], (name, got, ref)
+        for g_, r_ in zip(got[:3] + got[4:], ref[:3] + ref[4:]):
+            if np.isnan(r_):
+                assert np.isnan(g_), (name, got, ref)
+            else:
+                assert abs(g_ - r_) <= 1e-6, (name, got, ref)
+    # a cold field right after the hot ones: the reduction starts afresh
+    T = np.full(int(np.prod(n)), 300.0, np.float32)
+    assert cf.meltPoolDimensions(lv, T, Tl)[:4] == (0.0, 0.0, 0.0, 0)
+
+
+@pytest.mark.parametrize("elements", [(12, 10, 8), (60, 50, 30)])
+def test_energy_balance(cf, oracle, elements):
+    """gm_energy_balance against the oracle on the GPU solver's own update
+    with Dirichlet faces and a floor applied: every sum to 1e-7 of the
+    scale; the residual of the plain update is FP32 rounding (< 1e-4); an
+    error planted at an interior node moves the free sum by rho*cp*h^3*delta."""
+    inp, p, _ = _deck(oracle)
+    O = oracle
+    rng = np.random.default_rng(5)
+    lv = {"elements": list(elements),
+          "bounds": {"x": [0.0, 0.0125 * elements[0]], "y": [1.0, 1.0 + 0.0125 * elements[1]],
+                     "z": [-0.00625 * elements[2], 0.0]}}
+    lv["length"], lv["h"] = O.calc_length_h(lv)
+    lv["nodes"] = O.calcNumNodes(lv["elements"])
+    lv["ne"], lv["nn"] = int(np.prod(elements)), int(np.prod(lv["nodes"]))
+    b = lv["bounds"]
+    lv["node_coords"], lv["connect"] = O.createMesh3D(
+        (b["x"][0], b["x"][1], lv["nodes"][0]), (b["y"][0], b["y"][1], lv["nodes"][1]),
+        (b["z"][0], b["z"][1], lv["nodes"][2]))
+    nn, ne = lv["nn"], lv["ne"]
+    T = rng.uniform(300.0, 2000.0, nn).astype(np.float32)
+    k = rng.uniform(0.01, 0.03, nn).astype(np.float32)
+    rc = np.full(nn, 4e-3, np.float32)
+    Fc = rng.uniform(-1e-3, 5e-3, nn).astype(np.float32)
+    dt = 1e-6
+    Tt = cf.solveMatrixFreeFE(lv, nn, ne, k, rc, dt, T, Fc, 0)
+    nx, ny, nz = lv["nodes"]
+    iz, iy, ix = np.unravel_index(np.arange(nn), (nz, ny, nx))
+    faces = (ix == 0) | (ix == nx - 1) | (iy == 0) | (iy == ny - 1) | (iz == 0)
+    floor = np.float32(600.0)
+    Tn = np.where(faces, np.float32(300.0), np.maximum(Tt, floor)).astype(np.float32)
+    got = cf.energyBalance(lv, ne, k, rc, dt, T, Fc, 0, Tn, bc_faces=31, T_clamp=floor)
+    ref = O.energy_balance(lv, ne, k, rc, dt, T, Fc, 0.0, Tn, bc_faces=31, T_clamp=float(floor))
+    for q in (0, 1, 2, 3, 4, 5):
+        assert abs(got[q] - ref[q]) <= 1e-7 * ref[5], (q, got, ref)
+    assert got[7] == ref[7] and got[7] > faces.sum()
+    assert abs(got[6]) < 1e-4 and abs(got[6] - ref[6]) < 1e-7
+    i = 3 + nx * (3 + ny * 3)
+    Tp = Tn.copy()
+    Tp[i] += np.float32(50.0)
+    got2 = cf.energyBalance(lv, ne, k, rc, dt, T, Fc, 0, Tp, bc_faces=31, T_clamp=floor)
+    h = [float(np.float32(v)) for v in lv["h"]]
+    want = 4e-3 * h[0] * h[1] * h[2] * float(Tp[i] - Tn[i])
+    assert got2[0] - got[0] == pytest.approx(want, rel=1e-6)

@@ -463,3 +463,207 @@ def test_new_tprime_zero_for_coarse_interpolant(oracle):
 def test_window_shift_truncates_toward_zero(oracle, v, h, s):
     """(v/h + 0.01) cast to int (cF:1716-1718): -1.0 h shifts by 0, not -1."""
     assert oracle._shift(v, h) == s
+
+
+# ---------------------------------------------------------------------------
+# melt-pool size (the library's own measure; the reference forms only
+# S2 = T >= T_liquidus, cF:2585)
+# ---------------------------------------------------------------------------
+
+def _pool_mesh(n=(9, 11, 7), h=(0.1, 0.05, 0.025), x0=(0.3, -0.2, -0.15)):
+    return [x0[d] + h[d] * np.arange(n[d]) for d in range(3)]
+
+
+def _field(ax, f):
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    return f(X, Y, Z).reshape(-1)
+
+
+def test_melt_pool_linear_depth_closed_form(oracle):
+    """T linear in z: the crossing, and so the depth, is exact off the nodes."""
+    ax = _pool_mesh()
+    ztop = ax[2][-1]
+    Tl, T0, c = 1600.0, 2000.0, 4000.0  # depth (T0 - Tl) / c = 0.1 (4 elements)
+    T = _field(ax, lambda X, Y, Z: T0 + c * (Z - ztop))
+    d, w, l, n, ylo, yhi, xlo, xhi = oracle.melt_pool_dims(T, ax, Tl)
+    assert d == pytest.approx(0.1, abs=1e-12)
+    # the whole top plane is molten: extents reach the mesh boundary
+    assert (ylo, yhi, xlo, xhi) == (ax[1][0], ax[1][-1], ax[0][0], ax[0][-1])
+    assert w == pytest.approx(ax[1][-1] - ax[1][0]) and l == pytest.approx(ax[0][-1] - ax[0][0])
+    assert n == 9 * 11 * 5  # planes within 0.1 of the top: 5 of 7
+    # an off-node depth: 0.0625 = 2.5 elements
+    T = _field(ax, lambda X, Y, Z: Tl + 400.0 - 6400.0 * (ztop - Z))
+    assert oracle.melt_pool_dims(T, ax, Tl)[0] == pytest.approx(0.0625, abs=1e-12)
+
+
+def test_melt_pool_tent_width_and_depth_closed_form(oracle):
+    """A tent in y (kink on a node) and linear in z and x: width, length and
+    depth of the linear interpolant are exact."""
+    ax = _pool_mesh()
+    ztop = ax[2][-1]
+    yc, xc = ax[1][5], ax[0][4]
+    Tl, Tpk = 1600.0, 1900.0
+    T = _field(ax, lambda X, Y, Z: Tpk - 2000.0 * np.abs(Y - yc) - 1000.0 * np.abs(X - xc)
+               - 6000.0 * (ztop - Z))
+    d, w, l, n, ylo, yhi, xlo, xhi = oracle.melt_pool_dims(T, ax, Tl)
+    assert w == pytest.approx(2 * 300.0 / 2000.0, abs=1e-12)
+    assert ylo == pytest.approx(yc - 0.15, abs=1e-12) and yhi == pytest.approx(yc + 0.15, abs=1e-12)
+    assert l == pytest.approx(2 * 300.0 / 1000.0, abs=1e-12)
+    assert xlo == pytest.approx(xc - 0.3, abs=1e-12)
+    assert d == pytest.approx(300.0 / 6000.0, abs=1e-12)
+
+
+def test_melt_pool_special_cases(oracle):
+    ax = _pool_mesh()
+    ztop = ax[2][-1]
+    Tl = 1600.0
+    # nothing molten
+    d, w, l, n, ylo, yhi, xlo, xhi = oracle.melt_pool_dims(np.full(9 * 11 * 7, 300.0), ax, Tl)
+    assert (d, w, l, n) == (0.0, 0.0, 0.0, 0) and np.isnan([ylo, yhi, xlo, xhi]).all()
+    # molten to the bottom: the full mesh depth
+    d = oracle.melt_pool_dims(np.full(9 * 11 * 7, 2000.0), ax, Tl)[0]
+    assert d == pytest.approx(ztop - ax[2][0], abs=1e-12)
+    # a buried hot spot: depth counts it, the top plane has no width
+    T = np.full((7, 11, 9), 300.0)
+    T[2, 4, 3] = 2000.0
+    d, w, l, n = oracle.melt_pool_dims(T.reshape(-1), ax, Tl)[:4]
+    assert w == 0.0 and l == 0.0 and n == 1
+    # T_liquidus is crossed on the edge to the node below, 400/1700 of h
+    # under the hot node (2000 K above, 300 K below)
+    assert d == pytest.approx(ztop - ax[2][2] + 0.025 * 400.0 / 1700.0, abs=1e-12)
+    # exactly T_liquidus counts as molten (T >= T_liquidus, cF:2585)
+    T = np.full((7, 11, 9), 300.0)
+    T[-1, 5, 4] = Tl
+    d, w, l, n, ylo, yhi = oracle.melt_pool_dims(T.reshape(-1), ax, Tl)[:6]
+    assert n == 1 and w == 0.0 and ylo == yhi == pytest.approx(ax[1][5])
+
+
+def test_melt_pool_brute_force_along_grid_lines(oracle):
+    """Random fields: the extents equal those of the linear interpolant
+    sampled densely along every grid line (to the sampling step)."""
+    rng = np.random.default_rng(3)
+    ax = _pool_mesh(n=(6, 5, 4))
+    Tl = 1600.0
+    S = 2001
+    t = np.linspace(0.0, 1.0, S)
+    for trial in range(4):
+        T = rng.uniform(1000.0, 2000.0, size=(4, 5, 6))
+        T[-1, 0, :] = 500.0 if trial % 2 else T[-1, 0, :]
+        d, w, l, n, ylo, yhi, xlo, xhi = oracle.melt_pool_dims(T.reshape(-1), ax, Tl)
+        ztop = ax[2][-1]
+        depth_bf = 0.0
+        for j in range(5):
+            for i in range(6):
+                for k in range(3):
+                    v = T[k, j, i] + (T[k + 1, j, i] - T[k, j, i]) * t
+                    z = ax[2][k] + (ax[2][k + 1] - ax[2][k]) * t
+                    if (v >= Tl).any():
+                        depth_bf = max(depth_bf, ztop - z[v >= Tl].min())
+        ys, xs = [], []
+        for i in range(6):
+            for j in range(4):
+                v = T[-1, j, i] + (T[-1, j + 1, i] - T[-1, j, i]) * t
+                ys += list((ax[1][j] + (ax[1][j + 1] - ax[1][j]) * t)[v >= Tl])
+        for j in range(5):
+            for i in range(5):
+                v = T[-1, j, i] + (T[-1, j, i + 1] - T[-1, j, i]) * t
+                xs += list((ax[0][i] + (ax[0][i + 1] - ax[0][i]) * t)[v >= Tl])
+        step = max(ax[0][1] - ax[0][0], ax[1][1] - ax[1][0], ax[2][1] - ax[2][0]) / (S - 1)
+        assert d == pytest.approx(depth_bf, abs=1.01 * step)
+        assert ylo == pytest.approx(min(ys), abs=1.01 * step)
+        assert yhi == pytest.approx(max(ys), abs=1.01 * step)
+        assert xlo == pytest.approx(min(xs), abs=1.01 * step)
+        assert xhi == pytest.approx(max(xs), abs=1.01 * step)
+        assert w == pytest.approx(max(ys) - min(ys), abs=2.02 * step)
+        assert n == int((T >= Tl).sum())
+
+
+# ---------------------------------------------------------------------------
+# energy balance of an explicit update (the library's audit; its identity is
+# that of solveMatrixFreeFE, cF:582-642)
+# ---------------------------------------------------------------------------
+
+def _energy_case(O, seed=0, elements=(5, 4, 3)):
+    rng = np.random.default_rng(seed)
+    lv = _level(O, elements, {"x": [0.0, 0.25], "y": [0.0, 0.16], "z": [-0.09, 0.0]})
+    nn, ne = lv["nn"], lv["ne"]
+    T = rng.uniform(300.0, 2000.0, nn)
+    k = rng.uniform(0.01, 0.03, nn)
+    rc = rng.uniform(3e-3, 5e-3, nn)
+    Fc = rng.uniform(-1.0, 5.0, nn)
+    dt = 1e-5
+    return lv, nn, ne, T, k, rc, Fc, dt
+
+
+def test_energy_balance_of_the_solver_closes(oracle):
+    """The FP64 solver's own update closes the balance to rounding, and the
+    stiffness part sums to zero (K annihilates constants)."""
+    O = oracle
+    lv, nn, ne, T, k, rc, Fc, dt = _energy_case(O)
+    Tn = O.solveMatrixFreeFE(lv, nn, ne, k, rc, dt, T, Fc, 0.0)
+    s = O.energy_balance(lv, ne, k, rc, dt, T, Fc, 0.0, Tn)
+    assert abs(s[6]) < 1e-12
+    assert s[1] == 0.0 and s[7] == 0.0
+    assert s[0] == pytest.approx(s[2], rel=1e-11)
+    KTabs = np.abs(dt * (O.solveMatrixFreeFE(lv, nn, ne, k, rc, dt, T, 0.0, 0.0) - T)).sum()
+    assert abs(s[3]) < 1e-12 * max(KTabs, 1.0)
+
+
+def test_energy_balance_constraints_and_planted_error(oracle):
+    """Dirichlet faces and the T_amb floor are held nodes: with them named
+    the balance closes and their reaction equals C (Tn - Tt); an error of
+    delta planted at a free interior node moves the sum by exactly
+    rho*cp*h^3*delta (capacity of an interior node: 8 elements of V/8)."""
+    O = oracle
+    lv, nn, ne, T, k, rc, Fc, dt = _energy_case(O, seed=1)
+    Tt = O.solveMatrixFreeFE(lv, nn, ne, k, rc, dt, T, Fc, 0.0)
+    nx, ny, nz = lv["nodes"]
+    iz, iy, ix = np.unravel_index(np.arange(nn), (nz, ny, nx))
+    faces = (ix == 0) | (iy == ny - 1) | (iz == 0)
+    mask = 1 | 8 | 16
+    Tn = Tt.copy()
+    Tn[faces] = 300.0
+    floor = 1000.0
+    Tn = np.where(faces, Tn, np.maximum(Tn, floor))
+    held_floor = (~faces) & (Tt < floor)
+    assert held_floor.any()
+    s = O.energy_balance(lv, ne, k, rc, dt, T, Fc, 0.0, Tn, bc_faces=mask, T_clamp=floor)
+    assert abs(s[6]) < 1e-12
+    assert s[7] == faces.sum() + held_floor.sum()
+    # the reaction, from the capacities of the solver's own mass sum
+    C = dt * O.solveMatrixFreeFE(lv, nn, ne, np.zeros(nn), rc, dt, np.zeros(nn), 1.0, 0.0) ** -1
+    held = faces | held_floor
+    assert s[4] == pytest.approx(float((C * (Tn - Tt))[held].sum()), rel=1e-10)
+    # without naming the faces, their reaction shows up in the residual (face
+    # nodes at 300 K whose update fell below the floor still count as held)
+    s_nf = O.energy_balance(lv, ne, k, rc, dt, T, Fc, 0.0, Tn, bc_faces=0, T_clamp=floor)
+    react = float((C * (Tn - Tt))[faces & (Tt >= floor)].sum())
+    assert abs(react) > 1e-3 * abs(s_nf[5])
+    assert (s_nf[0] + s_nf[1] - s_nf[2]) == pytest.approx(react, rel=1e-9)
+    # a planted error at an interior node
+    i = 2 + nx * (2 + ny * 1)
+    assert 0 < ix[i] < nx - 1 and 0 < iy[i] < ny - 1 and 0 < iz[i] < nz - 1
+    rc_u = np.full(nn, 4e-3)
+    Tt2 = O.solveMatrixFreeFE(lv, nn, ne, k, rc_u, dt, T, Fc, 0.0)
+    base = O.energy_balance(lv, ne, k, rc_u, dt, T, Fc, 0.0, Tt2)
+    Tp = Tt2.copy()
+    Tp[i] += 7.0
+    s = O.energy_balance(lv, ne, k, rc_u, dt, T, Fc, 0.0, Tp)
+    h = lv["h"]
+    assert s[0] - base[0] == pytest.approx(4e-3 * h[0] * h[1] * h[2] * 7.0, rel=1e-10)
+
+
+def test_energy_balance_active_layers(oracle):
+    """Only nodes of active elements take part (L1 before its top layers are
+    built): the nodes above the active layers are skipped, and the balance
+    of the solver's update still closes."""
+    O = oracle
+    lv, nn, ne, T, k, rc, Fc, dt = _energy_case(O, seed=2)
+    nx, ny = lv["nodes"][0], lv["nodes"][1]
+    ne_a = lv["elements"][0] * lv["elements"][1] * 2
+    Tn = O.solveMatrixFreeFE(lv, nn, ne_a, k, rc, dt, T, Fc, 0.0)
+    Tn = np.where(np.isfinite(Tn), Tn, 12345.0)  # 0/0 above the active layers
+    s = O.energy_balance(lv, ne_a, k, rc, dt, T, Fc, 0.0, Tn)
+    assert abs(s[6]) < 1e-12
+    n_act = nx * ny * 3
+    assert s[2] == pytest.approx(dt * Fc[:n_act].sum(), rel=1e-12)
