@@ -116,6 +116,9 @@ def main():
     import ctypes as C
     ms = C.c_float()
     mm_T = np.asarray(L[3]["T0"])  # before the timing replays below change the state
+    run.eng.sync_coords()
+    import computeFunctions as cf
+    pool = cf.meltPoolDimensions(L[3], L[3]["T0"], run.Properties["T_liquidus"])
     # GPU time of one subcycled L1 step: replays of its recorded graph (the
     # wall-clock step above adds the window move and the host driver)
     gms = C.c_float()
@@ -154,6 +157,7 @@ def main():
             "subcycled_steps": n_sub,
             "single_steps": n_single,
             "peak_T_L3": float(mm_T.max()),
+            "melt_pool_L3_mm": {"depth": pool[0], "width": pool[1], "length": pool[2]},
             "subcycle_graph_ms": graph_ms,
             "device": N.device_name(),
         },

@@ -41,6 +41,14 @@ def _run_both(oracle, inp, gcode, tmp_path, precision="float64", graph=True):
     out["Tp3"] = (np.asarray(Lg[3]["Tprime0"], np.float64), Lo[3]["Tprime0"])
     out["acc"] = (np.asarray(Lg[0]["accum_time"], np.float64), info["accum"])
     out["maxacc"] = (np.asarray(Lg[0]["max_accum_time"], np.float64), info["max_accum"])
+    # melt-pool size of the final L3 fields, measured by the library on both
+    Tl = float(deck["properties"].get("T_liquidus", 1625.0))
+    ax = [np.asarray(a, np.float64) for a in Lo[3]["node_coords"]]
+    assert all(np.allclose(np.asarray(Lg[3]["node_coords"][d], np.float64), ax[d], atol=1e-6)
+               for d in range(3))
+    out["pool"] = (cf.meltPoolDimensions(Lg[3], Lg[3]["T0"], Tl),
+                   cf.meltPoolDimensions(Lg[3], np.asarray(out[3][1], np.float32), Tl),
+                   Lg[3]["h"])
     return out, st
 
 
@@ -106,6 +114,12 @@ def test_driver_vs_oracle(oracle, tmp_path, case):
         assert off <= max(3, a.size // 10000), (key, off)
     if case in ("melt", "tables"):
         assert np.count_nonzero(out["maxacc"][1] + out["acc"][1]) > 0  # the deck melts
+        # melt-pool depth and width of the GPU run against the oracle run's
+        # field: within one L3 element (they agree far closer)
+        pg, po, h = out["pool"]
+        print("melt pool (depth, width, length) GPU", pg[:3], "oracle", po[:3])
+        assert abs(pg[0] - po[0]) <= float(h[2]) and abs(pg[1] - po[1]) <= float(h[1]), (pg, po)
+        assert abs(pg[0] - po[0]) <= 0.05 * float(h[2]), (pg, po)
 
 
 def test_driver_vs_oracle_unaligned_windows(oracle, tmp_path):
