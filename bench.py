@@ -45,7 +45,41 @@ def parse():
     ap.add_argument("--no-graph-time", action="store_true",
                    help="skip the replay timing of the subcycle graph")
     ap.add_argument("--kernel-iters", type=int, default=400)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-instances measurement")
+    ap.add_argument("--batch-n", type=int, default=64)
     return ap.parse_args()
+
+
+def batch_measure(n, iters=50):
+    """Batched process-parameter instances: n single-track L3 windows of
+    mesh sizes drawn from a seed, one launch per step, plane depth chosen by
+    the autotuner; against one launch per instance."""
+    import numpy as np
+    import workloads as W
+    import batch as B
+    inst = W.batch_instances(n=n, seed=0)
+    bt = B.Batch.from_instances(inst)
+    for s in range(10):  # a melt pool in every instance before timing
+        bt.step(W.batch_rows(inst, s))
+    zc, tune_ms = bt.autotune(iters=20)
+    ms_b = bt.time(batched=True, iters=iters)
+    ms_u = bt.time(batched=False, iters=iters)
+    nodes = sum((e[0] + 1) * (e[1] + 1) * (e[2] + 1) for e in (b["elements"] for b in inst))
+    gbs = 10.0 * nodes / (ms_b * 1e-3) / 1e9
+    return {
+        "workload": f"{n} single-track L3 instances, element counts drawn (seed 0) from "
+                    f"{W.BATCH_SIZES}, P 100-400 W, 0.5-2 m/s, r 30-70 um",
+        "nodes": nodes,
+        "zc_autotuned": zc,
+        "autotune_ms": [float(v) for v in tune_ms],
+        "ms_per_step_batched": ms_b,
+        "ms_per_step_one_launch_per_instance": ms_u,
+        "node_updates_per_s": nodes / (ms_b * 1e-3),
+        "roofline": {"kernel": "k_step_dpp_batch", "bound": "hbm", "achieved": gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "bytes_per_launch": 10 * nodes},
+    }
 
 
 def cpu_baseline(inp, gcode, n23, mm_per_step):
@@ -179,6 +213,11 @@ def main():
             result["cpu_baseline"] = cpu_baseline(inp, gcode, N2 * N3, mm / max(a.steps, 1))
         except Exception as exc:  # the baseline must never hide the GPU line
             result["cpu_baseline"] = {"error": repr(exc)}
+    if not a.no_batch:
+        try:
+            result["batch"] = batch_measure(a.batch_n)
+        except Exception as exc:  # reported, not fatal to the headline line
+            result["batch"] = {"error": repr(exc)}
     print(json.dumps(result))
 
 

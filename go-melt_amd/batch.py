@@ -1,0 +1,91 @@
+"""Batched process-parameter instances: one kernel launch per step for many
+independent L3 windows of their own mesh size and laser (gm_batch_* in
+include/gomelt.h).  There is no CPU fallback: without libgomelt.so or a GPU
+every call raises."""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+import _native as N
+from computeFunctions import SetupProperties, _material
+
+
+class Batch:
+    """n instances; ``grids`` are gm_grid, ``props`` reference-schema
+    property dicts (one per instance)."""
+
+    def __init__(self, grids, props, stream=None):
+        N.init_device()
+        self.n = len(grids)
+        self.stream = stream or N.Stream()
+        self.grids = (N.gm_grid * self.n)(*grids)
+        self.props = [SetupProperties(p) for p in props]
+        mats = (N.gm_material * self.n)(*[_material(p) for p in self.props])
+        h = C.c_void_p()
+        N.check(N.lib().gm_batch_create(self.n, self.grids, mats, self.stream.ptr, C.byref(h)))
+        self.h = h.value
+
+    @classmethod
+    def from_instances(cls, instances, stream=None):
+        """From workloads.batch_instances(): the mesh of each instance spans
+        [0, nx h] x [0, ny h] x [-nz h, 0]."""
+        grids = []
+        for b in instances:
+            nel, h = b["elements"], b["h"]
+            grids.append(N.gm_grid.make([e + 1 for e in nel], [0.0, 0.0, -nel[2] * h], [h] * 3))
+        return cls(grids, [b["properties"] for b in instances], stream)
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.lib().gm_batch_destroy(self.h)
+        except Exception:
+            pass
+
+    def step(self, rows, batched=True):
+        r = np.ascontiguousarray(rows, np.float32).reshape(self.n, 7)
+        N.check(N.lib().gm_batch_step(self.h, N.fptr(r), 1 if batched else 0))
+
+    def set_zc(self, zc):
+        N.check(N.lib().gm_batch_set_zc(self.h, int(zc)))
+
+    def autotune(self, iters=20):
+        ms = np.zeros(3, np.float32)
+        z = C.c_int32()
+        N.check(N.lib().gm_batch_autotune(self.h, int(iters), N.fptr(ms), C.byref(z)))
+        return z.value, ms
+
+    def time(self, batched=True, iters=20):
+        ms = C.c_float()
+        N.check(N.lib().gm_batch_time(self.h, 1 if batched else 0, int(iters), C.byref(ms)))
+        return ms.value
+
+    def _field(self, i, f):
+        p = C.c_void_p()
+        n = C.c_int64()
+        N.check(N.lib().gm_batch_field(self.h, int(i), int(f), C.byref(p), C.byref(n)))
+        return N.DeviceArray(n.value, np.uint8 if f == 1 else np.float32, ptr=p.value,
+                             owner=False)
+
+    def T(self, i):
+        self.stream.sync()
+        return self._field(i, 0).numpy()
+
+    def S1(self, i):
+        self.stream.sync()
+        return self._field(i, 1).numpy()
+
+    def set_fields(self, i, T=None, S1=None, faces=None):
+        """Write T, S1 and/or the face values of instance i (host arrays)."""
+        self.stream.sync()
+        for f, v, dt in ((0, T, np.float32), (1, S1, np.uint8), (2, faces, np.float32)):
+            if v is not None:
+                d = self._field(i, f)
+                a = np.ascontiguousarray(v, dt).reshape(-1)
+                assert a.size == d.n
+                N.check(N.lib().gm_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes, self.stream.ptr))
+                self.stream.sync()  # the host array must outlive the copy
+        N.check(N.lib().gm_batch_refresh(self.h))

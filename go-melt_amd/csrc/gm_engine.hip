@@ -1647,3 +1647,280 @@ extern "C" int gm_engine_minmax(gm_engine* e, float* out) {
     HIPCHK(hipStreamSynchronize(e->s));
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Batched instances: many independent L3-type windows of their own mesh size
+// and laser, advanced by one launch per step (k_step_dpp_batch)
+// ---------------------------------------------------------------------------
+
+struct gm_batch {
+    int n = 0;
+    hipStream_t s = nullptr;
+    bool rec = false;
+    int zc = 3;                       // planes per brick of the batched launch
+    int cur = 0;                      // T[cur] holds the current field
+    std::vector<gm_grid> g;
+    std::vector<DevMat*> dm;          // per instance (laser constants differ)
+    std::vector<float*> T[2], kc[2];  // fields and their carried k | cp
+    std::vector<uint8_t*> S1, S1t;    // S1 (updated in place) and a tuning scratch
+    std::vector<float*> face, src;    // Dirichlet face values, source tables
+    std::vector<BatchItem> items;     // host copy of the step arguments
+    BatchItem* d_items = nullptr;
+    BatchItem* h_ring = nullptr;      // pinned upload slots
+    hipEvent_t ring_ev[4] = {};
+    int ring_slot = 0;
+    int* d_bstart[3] = {};            // block starts for zc = 2, 3, 4
+    int nblocks[3] = {};
+    int maxaxes = 0;
+    GmStepDesc* d_desc = nullptr;     // zeroed; the batched step reads no row from it
+    std::vector<void*> allocs;
+};
+
+static int zc_slot(int zc) { return zc == 2 ? 0 : (zc == 4 ? 2 : 1); }
+
+extern "C" int gm_batch_destroy(gm_batch* b) {
+    if (!b) return 0;
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->h_ring) (void)hipHostFree(b->h_ring);
+    for (auto& ev : b->ring_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    delete b;
+    return 0;
+}
+
+template <class P>
+static hipError_t balloc(gm_batch* b, P** p, size_t bytes) {
+    hipError_t rc = hipMalloc((void**)p, bytes ? bytes : 1);
+    if (rc == hipSuccess) b->allocs.push_back((void*)*p);
+    return rc;
+}
+
+extern "C" int gm_batch_create(int n, const gm_grid* grids, const gm_material* mats,
+                               void* stream, gm_batch** out) {
+    if (n < 1 || !grids || !mats || !out) return fail("gm_batch_create: bad arguments");
+    gm_batch* b = new gm_batch();
+    auto bad = [&](int rc) {
+        gm_batch_destroy(b);
+        return rc;
+    };
+#define BCHK(call)                                                                      \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return bad(fail(std::string(#call) + ": " + hipGetErrorString(e_)));        \
+    } while (0)
+    b->n = n;
+    b->s = (hipStream_t)stream;
+    b->g.assign(grids, grids + n);
+    const DevMat hm0 = make_devmat(mats[0]);
+    b->rec = hm0.pnb > 0;
+    for (int i = 0; i < n; ++i) {
+        const gm_grid& g = grids[i];
+        if (g.n[0] < 2 || g.n[1] < 2 || g.n[2] < 2 ||
+            (int64_t)g.n[0] * g.n[1] * g.n[2] * 8 >= (int64_t)1 << 31)
+            return bad(fail("gm_batch_create: instance meshes need >= 2 nodes per axis and < 2^28 nodes"));
+        if ((make_devmat(mats[i]).pnb > 0) != b->rec)
+            return bad(fail("gm_batch_create: all instances need the same kind of property tables"));
+        b->maxaxes = std::max(b->maxaxes, g.n[0] + g.n[1] + g.n[2]);
+    }
+    b->dm.resize(n);
+    for (int k = 0; k < 2; ++k) {
+        b->T[k].resize(n);
+        b->kc[k].resize(n);
+    }
+    b->S1.resize(n);
+    b->S1t.resize(n);
+    b->face.resize(n);
+    b->src.resize(n);
+    b->items.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const gm_grid& g = grids[i];
+        const int64_t nn = (int64_t)g.n[0] * g.n[1] * g.n[2];
+        const DevMat hm = make_devmat(mats[i]);
+        BCHK(balloc(b, &b->dm[i], sizeof(DevMat)));
+        BCHK(hipMemcpyAsync(b->dm[i], &hm, sizeof(DevMat), hipMemcpyHostToDevice, b->s));
+        for (int k = 0; k < 2; ++k) {
+            BCHK(balloc(b, &b->T[k][i], nn * 4));
+            BCHK(balloc(b, &b->kc[k][i], 2 * nn * 4));
+        }
+        BCHK(balloc(b, &b->S1[i], nn));
+        BCHK(balloc(b, &b->S1t[i], nn));
+        BCHK(balloc(b, &b->face[i], nn * 4));
+        BCHK(balloc(b, &b->src[i], (size_t)(g.n[0] + g.n[1] + g.n[2]) * 4));
+        launch_fill_f32(b->T[0][i], mats[i].T_amb, nn, b->s);
+        launch_fill_f32(b->face[i], mats[i].T_amb, nn, b->s);
+        launch_fill_u8(b->S1[i], 0, nn, b->s);
+        launch_props_kc(b->dm[i], b->T[0][i], b->S1[i], nn, 0, b->kc[0][i], b->s);
+        LSArgs a = ls_default();
+        a.use_desc = 0;
+        a.level = 3;
+        a.grid = g;
+        a.n_sub = 0;
+        a.nn_active = nn;
+        a.ne_active = (int64_t)(g.n[0] - 1) * (g.n[1] - 1) * (g.n[2] - 1);
+        a.dt_mode = GM_DT_VALUE;
+        a.src_mode = GM_SRC_L3;
+        a.src_row = 0;
+        a.srcT = b->src[i];
+        a.conv = 1;
+        a.bcA = a.bcB = b->face[i];
+        a.alpha = 1.0f;
+        a.clamp = 1;
+        // carried k, cp are in the raw units of the material tables: k / 1000
+        // and rho fold into the coefficients, as for the engine's levels
+        ls_coeffs(a, g);
+        for (int q = 0; q < 8; ++q) a.kc[q] = (float)((double)a.kc[q] / 1000.0);
+        a.mrow = (float)((double)a.mrow * mats[i].rho);
+        a.prec_ok = b->rec;
+        b->items[i].a = a;
+        b->items[i].m = b->dm[i];
+        b->items[i].w3 = a.w3;
+        memset(b->items[i].row, 0, sizeof(b->items[i].row));
+    }
+    BCHK(balloc(b, &b->d_items, sizeof(BatchItem) * n));
+    BCHK(balloc(b, &b->d_desc, sizeof(GmStepDesc)));
+    BCHK(hipMemsetAsync(b->d_desc, 0, sizeof(GmStepDesc), b->s));
+    for (int z = 0; z < 3; ++z) {
+        const int zc = z + 2;
+        std::vector<int> st(n + 1, 0);
+        for (int i = 0; i < n; ++i) st[i + 1] = st[i] + batch_bricks(grids[i], zc);
+        b->nblocks[z] = st[n];
+        BCHK(balloc(b, &b->d_bstart[z], sizeof(int) * (n + 1)));
+        BCHK(hipMemcpyAsync(b->d_bstart[z], st.data(), sizeof(int) * (n + 1),
+                              hipMemcpyHostToDevice, b->s));
+    }
+    if (hipHostMalloc((void**)&b->h_ring, sizeof(BatchItem) * n * 4, hipHostMallocDefault) !=
+        hipSuccess) {
+        b->h_ring = nullptr;
+        return bad(fail("gm_batch_create: hipHostMalloc failed"));
+    }
+    for (auto& ev : b->ring_ev) BCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    BCHK(hipStreamSynchronize(b->s));
+    *out = b;
+    return 0;
+}
+#undef BCHK
+
+// Step arguments of every instance for one step: row i of rows (x, y, z,
+// Ljump, Ldwell, dt, P), input T[cur], output T[cur ^ 1]; S1 in place or
+// into the tuning scratch.  Uploaded through a pinned slot.
+static int batch_upload(gm_batch* b, const float* rows, bool scratch_s1) {
+    for (int i = 0; i < b->n; ++i) {
+        BatchItem& it = b->items[i];
+        if (rows) memcpy(it.row, rows + 7 * i, sizeof(it.row));
+        LSArgs& a = it.a;
+        a.dt = it.row[5];
+        a.T = b->T[b->cur][i];
+        a.S1 = b->S1[i];
+        a.S1out = scratch_s1 ? b->S1t[i] : b->S1[i];
+        a.Tout = b->T[b->cur ^ 1][i];
+        a.kcin = b->kc[b->cur][i];
+        a.kcout = b->kc[b->cur ^ 1][i];
+    }
+    const int slot = b->ring_slot;
+    b->ring_slot = (slot + 1) & 3;
+    HIPCHK(hipEventSynchronize(b->ring_ev[slot]));
+    BatchItem* h = b->h_ring + (size_t)slot * b->n;
+    memcpy(h, b->items.data(), sizeof(BatchItem) * b->n);
+    HIPCHK(hipMemcpyAsync(b->d_items, h, sizeof(BatchItem) * b->n, hipMemcpyHostToDevice, b->s));
+    HIPCHK(hipEventRecord(b->ring_ev[slot], b->s));
+    return 0;
+}
+
+static void batch_launch(gm_batch* b, int batched) {
+    launch_batch_src(b->d_items, b->n, b->maxaxes, b->s);
+    if (batched) {
+        const int z = zc_slot(b->zc);
+        launch_batch_step(b->d_items, b->d_bstart[z], b->n, b->nblocks[z], b->zc, b->rec,
+                          b->d_desc, b->s);
+    } else {
+        for (int i = 0; i < b->n; ++i)
+            launch_step_zc(b->items[i].a, b->d_desc, b->g[i], b->dm[i], b->zc, b->rec, b->s);
+    }
+}
+
+extern "C" int gm_batch_step(gm_batch* b, const float* rows_host, int batched) {
+    if (!b || !rows_host) return fail("gm_batch_step: null argument");
+    if (int rc = batch_upload(b, rows_host, false)) return rc;
+    batch_launch(b, batched);
+    HIPCHK(hipGetLastError());
+    b->cur ^= 1;
+    return 0;
+}
+
+extern "C" int gm_batch_set_zc(gm_batch* b, int zc) {
+    if (!b || (zc != 2 && zc != 3 && zc != 4)) return fail("gm_batch_set_zc: zc must be 2, 3 or 4");
+    b->zc = zc;
+    return 0;
+}
+
+// Times `iters` steps from the current state (outputs into the next buffers
+// and the S1 scratch, so the state does not advance), batched or not.
+extern "C" int gm_batch_time(gm_batch* b, int batched, int iters, float* ms_per_step) {
+    if (!b || iters < 1 || !ms_per_step) return fail("gm_batch_time: bad arguments");
+    if (int rc = batch_upload(b, nullptr, true)) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    batch_launch(b, batched);  // warm-up
+    HIPCHK(hipEventRecord(e0, b->s));
+    for (int k = 0; k < iters; ++k) batch_launch(b, batched);
+    HIPCHK(hipEventRecord(e1, b->s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    *ms_per_step = ms / iters;
+    return 0;
+}
+
+// Autotuner: the planes per brick (2, 3 or 4) of the batched launch with the
+// shortest measured step on this batch; ms_host (nullable) gets the three
+// times.  The state does not advance.
+extern "C" int gm_batch_autotune(gm_batch* b, int iters, float* ms_host, int* zc_out) {
+    if (!b || iters < 1) return fail("gm_batch_autotune: bad arguments");
+    float best = 0.0f;
+    int bz = 3;
+    for (int zc = 2; zc <= 4; ++zc) {
+        b->zc = zc;
+        float ms = 0.0f;
+        if (int rc = gm_batch_time(b, 1, iters, &ms)) return rc;
+        if (ms_host) ms_host[zc - 2] = ms;
+        if (zc == 2 || ms < best) {
+            best = ms;
+            bz = zc;
+        }
+    }
+    b->zc = bz;
+    if (zc_out) *zc_out = bz;
+    return 0;
+}
+
+// Device views of instance i: field 0 = T (FP32), 1 = S1 (uint8), 2 = the
+// Dirichlet face values (FP32, writable: the faces hold them every step).
+extern "C" int gm_batch_field(gm_batch* b, int i, int field, void** ptr, int64_t* count) {
+    if (!b || i < 0 || i >= b->n || !ptr || !count) return fail("gm_batch_field: bad arguments");
+    const gm_grid& g = b->g[i];
+    *count = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    if (field == 0) *ptr = b->T[b->cur][i];
+    else if (field == 1) *ptr = b->S1[i];
+    else if (field == 2) *ptr = b->face[i];
+    else return fail("gm_batch_field: field is 0 (T), 1 (S1) or 2 (faces)");
+    return 0;
+}
+
+// After T or S1 of an instance were written through gm_batch_field: carried
+// k, cp of the current fields again.
+extern "C" int gm_batch_refresh(gm_batch* b) {
+    if (!b) return fail("gm_batch_refresh: null batch");
+    for (int i = 0; i < b->n; ++i) {
+        const gm_grid& g = b->g[i];
+        const int64_t nn = (int64_t)g.n[0] * g.n[1] * g.n[2];
+        launch_props_kc(b->dm[i], b->T[b->cur][i], b->S1[i], nn, 0, b->kc[b->cur][i], b->s);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}

@@ -163,6 +163,17 @@ struct LSArgs {
     float corrG_scale;
 };
 
+// One instance of a batched L3-type step: its step arguments (grid,
+// buffers, coefficients, dt; source tables at a.srcT with a.src_row = 0),
+// its material and laser constants, and its laser row of the step
+// (x, y, z, Ljump, Ldwell, dt, P).
+struct BatchItem {
+    LSArgs a;
+    const DevMat* m;
+    float row[7];
+    float w3;
+};
+
 // Restriction (fine -> coarse) arguments.
 struct RArgs {
     const float* T;       // fine temperature (for element k and rho*cp)
@@ -190,6 +201,14 @@ struct RArgs {
 
 // ---- launchers (gm_kernels.hip) -------------------------------------------
 namespace gm {
+
+// batched L3-type steps
+int batch_bricks(const gm_grid& g, int zc);
+void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s);
+void launch_batch_step(const BatchItem* it, const int* bstart, int n, int nblocks, int zc,
+                       bool rec, const GmStepDesc* d, hipStream_t s);
+void launch_step_zc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
+                    int zc, bool rec, hipStream_t s);
 DevMat make_devmat(const gm_material& m);
 void level_coeffs(const gm_grid& g, float kc[8], float* mrow, float* w2, float* w3);
 

@@ -718,9 +718,11 @@ __device__ __forceinline__ float face_flux_hw(const DevMat& m, float Tq) {
     return q * 1e-6f;
 }
 
+// One brick t of a subcycle level step (one wave): the body of k_step_dpp and
+// of its batched form k_step_dpp_batch.
 template <bool SRC, bool ACC, bool REC, int YB, int ZC>
-__global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __restrict__ d,
-                                                 const DevMat* __restrict__ m) {
+__device__ __forceinline__ void step_dpp_brick(const LSArgs& a, const GmStepDesc* __restrict__ d,
+                                               const DevMat* __restrict__ m, const int t) {
     // node counts from the arguments (a window move changes only the origin,
     // which this kernel does not use): no dependent descriptor read
     const int nx = a.grid.n[0], ny = a.grid.n[1], nz = a.grid.n[2];
@@ -730,7 +732,6 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
     const int nband = (ny + YB - 1) / YB;
     const int nq = nx * nband;
     const int nzc = (nz + ZC - 1) / ZC;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int qs = fdiv_small(t, __builtin_amdgcn_rcpf((float)nzc)), zc = t - qs * nzc;
     const int lane = threadIdx.x;
     const int q = DXS * qs - 1 + lane;
@@ -1020,6 +1021,31 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
     }
 }
 
+template <bool SRC, bool ACC, bool REC, int YB, int ZC>
+__global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __restrict__ d,
+                                                 const DevMat* __restrict__ m) {
+    step_dpp_brick<SRC, ACC, REC, YB, ZC>(a, d, m, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Batched L3-type steps (source, surface flux, Dirichlet faces, no correction,
+// no melt time): many instances of their own mesh size and laser in one
+// launch.  bstart[i] is the first block of instance i (bstart[n] = total);
+// the blocks are dealt out XCD-aware as in the single launch, then each finds
+// its instance (uniform binary search) and runs that instance's brick.
+template <bool REC, int ZC>
+__global__ void __launch_bounds__(64) k_step_dpp_batch(const BatchItem* __restrict__ it,
+                                                       const int* __restrict__ bstart, int n,
+                                                       const GmStepDesc* __restrict__ d) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    step_dpp_brick<true, false, REC, 1, ZC>(it[lo].a, d, it[lo].m, b - bstart[lo]);
+}
+
 // L1 step (computeL1Temperature, cF:2135-2204 with the L1 faces of
 // cF:1568-1597) in the same wave-brick form: properties from (T, S1) of every
 // loaded position (L1 carries no k, cp), active element layers below
@@ -1282,6 +1308,73 @@ void launch_level_step_kc(const LSArgs& a0, const GmStepDesc* d, const gm_grid& 
         if (a.prec_ok) launch_dpp<false, false, true>(a, d, hg, m, s);
         else launch_dpp<false, false, false>(a, d, hg, m, s);
     }
+}
+
+// ---- batched L3-type steps -------------------------------------------------
+int batch_bricks(const gm_grid& g, int zc) {
+    const int nq = g.n[0] * g.n[1];
+    return ((nq + DXS - 1) / DXS) * ((g.n[2] + zc - 1) / zc);
+}
+
+// separable source factors of each instance's row on its own mesh (as
+// k_src3_tables: x factors times 6 sqrt(3) eta P w3, then y, z); blockIdx.y
+// is the instance
+__global__ void k_batch_src(const BatchItem* __restrict__ it) {
+    const BatchItem& b = it[blockIdx.y];
+    const gm_grid& g = b.a.grid;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n[0] + g.n[1] + g.n[2]) return;
+    const DevMat* m = b.m;
+    float v;
+    if (i < g.n[0]) v = src_axis(g, 0, i, b.row[0], m->rsq, m->rc) * (m->pc6 * b.row[6] * b.w3);
+    else if (i < g.n[0] + g.n[1]) v = src_axis(g, 1, i - g.n[0], b.row[1], m->rsq, m->rc);
+    else v = src_axis(g, 2, i - g.n[0] - g.n[1], b.row[2], m->dsq, m->dc);
+    ((float*)b.a.srcT)[i] = v;
+}
+
+void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_batch_src, dim3((maxaxes + 255) / 256, n), dim3(256), 0, s, it);
+}
+
+template <int ZC>
+static void batch_step_zc(const BatchItem* it, const int* bstart, int n, int nblocks, bool rec,
+                          const GmStepDesc* d, hipStream_t s) {
+    if (rec)
+        hipLaunchKernelGGL((k_step_dpp_batch<true, ZC>), dim3(nblocks), dim3(64), 0, s, it, bstart,
+                           n, d);
+    else
+        hipLaunchKernelGGL((k_step_dpp_batch<false, ZC>), dim3(nblocks), dim3(64), 0, s, it,
+                           bstart, n, d);
+}
+
+void launch_batch_step(const BatchItem* it, const int* bstart, int n, int nblocks, int zc,
+                       bool rec, const GmStepDesc* d, hipStream_t s) {
+    if (n <= 0 || nblocks <= 0) return;
+    if (zc == 2) batch_step_zc<2>(it, bstart, n, nblocks, rec, d, s);
+    else if (zc == 4) batch_step_zc<4>(it, bstart, n, nblocks, rec, d, s);
+    else batch_step_zc<3>(it, bstart, n, nblocks, rec, d, s);
+}
+
+// the same step of one instance in a launch of its own (the unbatched form)
+template <int ZC>
+static void step_zc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
+                    bool rec, hipStream_t s) {
+    const int nb = batch_bricks(hg, ZC);
+    if (rec)
+        hipLaunchKernelGGL((k_step_dpp<true, false, true, 1, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
+    else
+        hipLaunchKernelGGL((k_step_dpp<true, false, false, 1, ZC>), dim3(nb), dim3(64), 0, s, a, d,
+                           m);
+}
+
+void launch_step_zc(const LSArgs& a0, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
+                    int zc, bool rec, hipStream_t s) {
+    LSArgs a = a0;
+    a.grid = hg;
+    if (zc == 2) step_zc<2>(a, d, hg, m, rec, s);
+    else if (zc == 4) step_zc<4>(a, d, hg, m, rec, s);
+    else step_zc<3>(a, d, hg, m, rec, s);
 }
 
 // k, cp of (T, S1) as the next level step's computeStateProperties returns them;

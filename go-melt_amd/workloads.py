@@ -191,3 +191,38 @@ def scan_length_mm(rows):
         return 0.0
     d = np.hypot(np.diff(rows[:, 0]), np.diff(rows[:, 1]))
     return float(d[rows[1:, 6] > 0].sum())
+
+
+# L3 element counts of the batched configuration (uniform draw per instance)
+BATCH_SIZES = [(32, 32, 32), (48, 48, 24), (64, 64, 32), (96, 96, 48)]
+
+
+def batch_instances(n=64, seed=0, h=0.0125, dt=1e-6, sizes=None):
+    """Independent single-track instances of an L3 window for the batched
+    step: mesh size drawn from BATCH_SIZES, P uniform 100-400 W, speed
+    uniform 0.5-2 m/s, laser radius uniform 30-70 um, from a fixed seed.
+    Spacing h (mm) in all axes, top face at z = 0, origin at 0 in x and y;
+    the track runs along x on the centre line from a quarter of the length.
+    Returns a list of dicts: elements, properties (reference schema), speed
+    (mm/s), P, x0 (track start), y, dt."""
+    rng = np.random.default_rng(seed)
+    sizes = sizes or BATCH_SIZES
+    out = []
+    for _ in range(n):
+        el = sizes[int(rng.integers(len(sizes)))]
+        P = float(rng.uniform(100.0, 400.0))
+        v = float(rng.uniform(500.0, 2000.0))
+        r = float(rng.uniform(0.030, 0.070))
+        props = _props(laser_power=P, laser_radius=r, laser_depth=0.05)
+        out.append({"elements": list(el), "properties": props, "speed": v, "P": P,
+                    "h": h, "x0": 0.25 * el[0] * h, "y": 0.5 * el[1] * h, "dt": dt})
+    return out
+
+
+def batch_rows(instances, step):
+    """Toolpath rows (x, y, z, Ljump, Ldwell, dt, P) of step `step` (0-based)
+    of every instance: the laser at x0 + v (step + 1) dt, power on."""
+    rows = np.zeros((len(instances), 7), np.float32)
+    for i, b in enumerate(instances):
+        rows[i] = (b["x0"] + b["speed"] * (step + 1) * b["dt"], b["y"], 0.0, 1, 1, b["dt"], b["P"])
+    return rows

@@ -295,6 +295,49 @@ int gm_engine_time_kernel(gm_engine* e, int which, int iters, float* ms_per_laun
 /* Enable/disable hipGraph capture of gm_engine_subcycle (default enabled). */
 int gm_engine_use_graph(gm_engine* e, int enable);
 
+/* ------------------------------------------------------------------------ */
+/* Batched instances                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Many independent process-parameter instances of an L3-type window, each of
+ * its own mesh size and laser, advanced by ONE kernel launch per step.  No
+ * reference counterpart (the reference runs one instance per process); each
+ * instance's step is the L3 step of the subcycle (computeSolutions_L3,
+ * cF:3015-3054: source cF:2960-3012, surface flux cF:2207-2301, Dirichlet
+ * W/E/S/N/bottom faces cF:1598-1620, floor at T_amb), with properties
+ * (cF:2567-2614) carried between steps.  The faces hold per-instance values
+ * (initially T_amb; writable through gm_batch_field).  Not re-entrant. */
+typedef struct gm_batch gm_batch;
+
+/* n instances: grids[i] its mesh, mats[i] its material and laser constants
+ * (all with property tables or all without).  T = T_amb, S1 = 0. */
+int gm_batch_create(int n, const gm_grid* grids, const gm_material* mats, void* stream,
+                    gm_batch** out);
+int gm_batch_destroy(gm_batch* b);
+
+/* One step of every instance: rows_host holds n toolpath rows (x, y, z,
+ * Ljump, Ldwell, dt, P), row i for instance i.  batched = 1: one launch for
+ * all instances; 0: one launch per instance (the same kernel body, for
+ * checking and timing). */
+int gm_batch_step(gm_batch* b, const float* rows_host, int batched);
+
+/* Planes per brick of the batched launch (2, 3 or 4; default 3). */
+int gm_batch_set_zc(gm_batch* b, int zc);
+
+/* Autotuner: times `iters` batched steps for each of 2, 3 and 4 planes per
+ * brick on this batch, keeps the fastest (zc_out) and returns the three
+ * times in ms_host (nullable).  The state does not advance.  Synchronous. */
+int gm_batch_autotune(gm_batch* b, int iters, float* ms_host, int* zc_out);
+
+/* Average ms per step of `iters` steps from the current state, batched or
+ * not (the state does not advance).  Synchronous. */
+int gm_batch_time(gm_batch* b, int batched, int iters, float* ms_per_step);
+
+/* Device view of instance i: field 0 = T (FP32), 1 = S1 (uint8), 2 = face
+ * values (FP32).  After writing T or S1, call gm_batch_refresh. */
+int gm_batch_field(gm_batch* b, int i, int field, void** ptr, int64_t* count);
+int gm_batch_refresh(gm_batch* b);
+
 #ifdef __cplusplus
 }
 #endif
