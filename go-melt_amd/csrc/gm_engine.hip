@@ -1660,7 +1660,8 @@ struct gm_batch {
     int zc = 3;                       // planes per brick of the batched launch
     int cur = 0;                      // T[cur] holds the current field
     std::vector<gm_grid> g;
-    std::vector<DevMat*> dm;          // per instance (laser constants differ)
+    DevMat* d_mats = nullptr;         // one per instance (laser constants differ)
+    std::vector<DevMat*> dm;          // d_mats + i
     std::vector<float*> T[2], kc[2];  // fields and their carried k | cp
     std::vector<uint8_t*> S1, S1t;    // S1 (updated in place) and a tuning scratch
     std::vector<float*> face, src;    // Dirichlet face values, source tables
@@ -1725,6 +1726,7 @@ extern "C" int gm_batch_create(int n, const gm_grid* grids, const gm_material* m
         b->maxaxes = std::max(b->maxaxes, g.n[0] + g.n[1] + g.n[2]);
     }
     b->dm.resize(n);
+    BCHK(balloc(b, &b->d_mats, sizeof(DevMat) * n));
     for (int k = 0; k < 2; ++k) {
         b->T[k].resize(n);
         b->kc[k].resize(n);
@@ -1738,7 +1740,7 @@ extern "C" int gm_batch_create(int n, const gm_grid* grids, const gm_material* m
         const gm_grid& g = grids[i];
         const int64_t nn = (int64_t)g.n[0] * g.n[1] * g.n[2];
         const DevMat hm = make_devmat(mats[i]);
-        BCHK(balloc(b, &b->dm[i], sizeof(DevMat)));
+        b->dm[i] = b->d_mats + i;
         BCHK(hipMemcpyAsync(b->dm[i], &hm, sizeof(DevMat), hipMemcpyHostToDevice, b->s));
         for (int k = 0; k < 2; ++k) {
             BCHK(balloc(b, &b->T[k][i], nn * 4));
@@ -1832,8 +1834,8 @@ static void batch_launch(gm_batch* b, int batched) {
     launch_batch_src(b->d_items, b->n, b->maxaxes, b->s);
     if (batched) {
         const int z = zc_slot(b->zc);
-        launch_batch_step(b->d_items, b->d_bstart[z], b->n, b->nblocks[z], b->zc, b->rec,
-                          b->d_desc, b->s);
+        launch_batch_step(b->d_items, b->d_mats, b->d_bstart[z], b->n, b->nblocks[z], b->zc,
+                          b->rec, b->d_desc, b->s);
     } else {
         for (int i = 0; i < b->n; ++i)
             launch_step_zc(b->items[i].a, b->d_desc, b->g[i], b->dm[i], b->zc, b->rec, b->s);

@@ -205,8 +205,8 @@ namespace gm {
 // batched L3-type steps
 int batch_bricks(const gm_grid& g, int zc);
 void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s);
-void launch_batch_step(const BatchItem* it, const int* bstart, int n, int nblocks, int zc,
-                       bool rec, const GmStepDesc* d, hipStream_t s);
+void launch_batch_step(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                       int nblocks, int zc, bool rec, const GmStepDesc* d, hipStream_t s);
 void launch_step_zc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
                     int zc, bool rec, hipStream_t s);
 DevMat make_devmat(const gm_material& m);

@@ -718,322 +718,23 @@ __device__ __forceinline__ float face_flux_hw(const DevMat& m, float Tq) {
     return q * 1e-6f;
 }
 
-// One brick t of a subcycle level step (one wave): the body of k_step_dpp and
-// of its batched form k_step_dpp_batch.
-template <bool SRC, bool ACC, bool REC, int YB, int ZC>
-__device__ __forceinline__ void step_dpp_brick(const LSArgs& a, const GmStepDesc* __restrict__ d,
-                                               const DevMat* __restrict__ m, const int t) {
-    // node counts from the arguments (a window move changes only the origin,
-    // which this kernel does not use): no dependent descriptor read
-    const int nx = a.grid.n[0], ny = a.grid.n[1], nz = a.grid.n[2];
-    const int nxy = nx * ny;
-    const int nn = nxy * nz;
-    const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
-    const int nband = (ny + YB - 1) / YB;
-    const int nq = nx * nband;
-    const int nzc = (nz + ZC - 1) / ZC;
-    const int qs = fdiv_small(t, __builtin_amdgcn_rcpf((float)nzc)), zc = t - qs * nzc;
-    const int lane = threadIdx.x;
-    const int q = DXS * qs - 1 + lane;
-    const bool qv = q >= 0 && q < nq;
-    int band = 0;
-    if (nq < (1 << 20)) band = qv ? fdiv_small(q, __builtin_amdgcn_rcpf((float)nx)) : 0;
-    else band = qv ? (int)((unsigned)q / (unsigned)nx) : 0;
-    const int x = qv ? q - band * nx : -1;
-    const int y0 = band * YB, z0 = zc * ZC;
-    const bool ein = qv && x < nex;  // element with min corner x exists in x
-    const bool olane = lane >= 1 && lane <= DXS && qv;
-    const int base = x + nx * y0 + nxy * z0;  // node (x, y0, z0)
-    const uint32_t nb4 = (uint32_t)nn * 4u;
-
-    // ---- loads: planes z0-1 .. z0+ZC, rows y0-1 .. y0+YB ---------------------
-    const auto rT = buf_rsrc(a.T, nb4), rKC = buf_rsrc(a.kcin, 2u * nb4);
-    float Tw[ZC + 2][YB + 2], Kw[ZC + 2][YB + 2], Cw[ZC + 2][YB + 2];
-    bool rowok[YB + 2];
-#pragma unroll
-    for (int r = 0; r < YB + 2; ++r) {
-        const int yy = y0 - 1 + r;
-        rowok[r] = qv && yy >= 0 && yy < ny;
-    }
-#pragma unroll
-    for (int p = 0; p < ZC + 2; ++p) {
-        const int zz = z0 - 1 + p;
-        const bool pok = zz >= 0 && zz < nz;
-#pragma unroll
-        for (int r = 0; r < YB + 2; ++r) {
-            const uint32_t vo =
-                (pok && rowok[r]) ? 4u * (uint32_t)(base + nx * (r - 1) + nxy * (p - 1)) : GM_OOB;
-            Tw[p][r] = buf_ld(rT, vo);
-            Kw[p][r] = buf_ld(rKC, vo);
-            Cw[p][r] = buf_ld(rKC, vo, nb4);
-        }
-    }
-    // restricted correction on its node box
-    float corr[ZC][YB];
-#pragma unroll
-    for (int i = 0; i < ZC; ++i)
-#pragma unroll
-        for (int j = 0; j < YB; ++j) corr[i][j] = 0.0f;
-    if (a.corr || a.corr64) {
-        const int c0x = a.corr_info[0][0], c1x = a.corr_info[0][1];
-        const int c0y = a.corr_info[1][0], c1y = a.corr_info[1][1];
-        const int c0z = a.corr_info[2][0], c1z = a.corr_info[2][1];
-#pragma unroll
-        for (int i = 0; i < ZC; ++i)
-#pragma unroll
-            for (int j = 0; j < YB; ++j) {
-                const int bx = x - c0x, by = y0 + j - c0y, bz = z0 + i - c0z;
-                if ((olane && y0 + j < ny && z0 + i < nz) && bx >= 0 && bx <= c1x && by >= 0 && by <= c1y &&
-                    bz >= 0 && bz <= c1z) {
-                    const int k = bx + a.corr_nmax[0] * (by + a.corr_nmax[1] * bz);
-                    if (a.corr) {
-                        corr[i][j] = a.corr[k];
-                    } else {
-                        // each box node is read by exactly one lane of the grid
-                        const double acc = a.corr64[k];
-                        a.corr64[k] = 0.0;
-                        float v = (float)acc;
-                        if (a.corrG) v += a.corrG[k] * a.corrG_scale;
-                        corr[i][j] = v;
-                    }
-                }
-            }
-    }
-    // per-node inputs of the outputs
-    const auto rS1 = buf_rsrc(a.S1, (uint32_t)nn);
-    const auto rFa = buf_rsrc(a.bcA, nb4), rFb = buf_rsrc(a.bcB, nb4);
-    float rFa_[ZC][YB], rFb_[ZC][YB], rAcc[ZC][YB], rMax[ZC][YB];
-    uint32_t rS1_[ZC][YB], rS2_[ZC][YB], oF[ZC][YB], oB[ZC][YB];
-    bool face_[ZC][YB];
-#pragma unroll
-    for (int i = 0; i < ZC; ++i) {
-#pragma unroll
-        for (int j = 0; j < YB; ++j) {
-            const int yy = y0 + j, zz = z0 + i;
-            const bool ok = olane && yy < ny && zz < nz;
-            const bool face = x == 0 || x == nex || yy == 0 || yy == ney || zz == 0;
-            const uint32_t idx = (uint32_t)(base + nx * j + nxy * i);
-            face_[i][j] = face;
-            oB[i][j] = ok ? idx : GM_OOB;
-            oF[i][j] = ok ? 4u * idx : GM_OOB;
-            const uint32_t fo = face ? oF[i][j] : GM_OOB;
-            rFa_[i][j] = buf_ld(rFa, fo);
-            rFb_[i][j] = buf_ld(rFb, fo);
-            rS1_[i][j] = buf_ld8(rS1, oB[i][j]);
-            if (ACC) {
-                // melt-time state on the L3 nodes (gathered from Level0 for
-                // the step, scattered back after it); accum and max_accum
-                // are read further down, only where the node is or was molten
-                const auto rS2p = buf_rsrc(a.S2prev, (uint32_t)nn);
-                rS2_[i][j] = buf_ld8(rS2p, oB[i][j]);
-                rAcc[i][j] = rMax[i][j] = 0.0f;
-            } else {
-                rS2_[i][j] = 0;
-                rAcc[i][j] = rMax[i][j] = 0.0f;
-            }
-        }
-    }
-    __shared__ float sTab[REC ? 4 : PTW] __attribute__((aligned(16)));
-    if (!REC) stage_tables(m, sTab, lane, 64);
-
-    const float* row = d->rows[a.src_row < 0 ? 0 : a.src_row];
-    float dt = a.dt;
-    if (a.dt_mode == GM_DT_ROW) dt = row[5];
-    else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
-    const float rmfac = dt / a.mrow;
-    // separable source: x and y factors per lane, z factors uniform
-    float sx = 0.0f, sy[YB], sz[ZC];
-    if (SRC) {
-        const float* st = a.srcT + (size_t)a.src_row * (nx + ny + nz);
-        sx = qv ? st[x] : 0.0f;
-#pragma unroll
-        for (int j = 0; j < YB; ++j) sy[j] = (qv && y0 + j < ny) ? st[nx + y0 + j] : 0.0f;
-#pragma unroll
-        for (int i = 0; i < ZC; ++i) sz[i] = (z0 + i < nz) ? st[nx + ny + z0 + i] : 0.0f;
-    }
-    const PropC pc = load_pc(m);
-    const float Tamb = m->T_amb;
-
-    // ---- x-y quad sums of k and cp, then element means -----------------------
-    // Q[p][r]: sum over the 4 nodes of the x-y quad with min corner (x, y0-1+r)
-    float Qk[ZC + 2][YB + 1], Qc[ZC + 2][YB + 1];
-#pragma unroll
-    for (int p = 0; p < ZC + 2; ++p) {
-#pragma unroll
-        for (int r = 0; r < YB + 1; ++r) {
-            const int yy = y0 - 1 + r;
-            const bool ok = ein && yy >= 0 && yy < ney;
-            const float ak = Kw[p][r] + Kw[p][r + 1];
-            const float ac = Cw[p][r] + Cw[p][r + 1];
-            const float qk = ak + lane_next(ak);
-            const float qc = ac + lane_next(ac);
-            Qk[p][r] = ok ? qk : 0.0f;
-            Qc[p][r] = ok ? qc : 0.0f;
-        }
-    }
-    // element layer between planes p and p+1 (min corner z = z0-1+p)
-    float Ek[ZC + 1][YB + 1], Ec[ZC + 1][YB + 1];
-#pragma unroll
-    for (int p = 0; p < ZC + 1; ++p) {
-        const int ez = z0 - 1 + p;
-        const float f = (ez >= 0 && ez < nez) ? 0.125f : 0.0f;
-#pragma unroll
-        for (int r = 0; r < YB + 1; ++r) {
-            Ek[p][r] = (Qk[p][r] + Qk[p + 1][r]) * f;
-            Ec[p][r] = (Qc[p][r] + Qc[p + 1][r]) * f;
-        }
-    }
-    // top surface flux (cF:2207-2301) of the nodes of the top plane, when it
-    // lies in this brick: face Gauss point values per face (min corner x, row
-    // y0-1+r), then each node's share from its four faces
-    float Ftop[YB];
-#pragma unroll
-    for (int j = 0; j < YB; ++j) Ftop[j] = 0.0f;
-    const int ptop = nez - (z0 - 1);  // window plane index of the top plane
-    if (ptop >= 1 && ptop <= ZC) {
-        float Fq[YB + 1][4];
-#pragma unroll
-        for (int r = 0; r < YB + 1; ++r) {
-            float t0 = 0.0f, t3 = 0.0f;
-#pragma unroll
-            for (int p = 1; p <= ZC; ++p)
-                if (p == ptop) {
-                    t0 = Tw[p][r];
-                    t3 = Tw[p][r + 1];
-                }
-            const float t1 = lane_next(t0), t2 = lane_next(t3);
-            const int yy = y0 - 1 + r;
-            const bool ok = ein && yy >= 0 && yy < ney;
-#pragma unroll
-            for (int qd = 0; qd < 4; ++qd) {
-                const float Tq = n2(qd, 0) * t0 + n2(qd, 1) * t1 + n2(qd, 2) * t2 + n2(qd, 3) * t3;
-                const float fl = face_flux_hw(*m, Tq) * a.w2;
-                Fq[r][qd] = ok ? fl : 0.0f;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < YB; ++j) {
-            float cr = 0.0f;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int bxe = e & 1, bye = e >> 1;  // 0: face on the negative side
-                const int ai = bye ? (bxe ? 0 : 1) : (bxe ? 3 : 2);
-#pragma unroll
-                for (int qd = 0; qd < 4; ++qd) {
-                    const float v = Fq[j + bye][qd];
-                    cr += n2(qd, ai) * (bxe ? v : lane_prev(v));
-                }
-            }
-            Ftop[j] = cr;
-        }
-    }
-
-    if (!REC) lds_barrier();  // property table staged
-    if (ACC) {
-        // their latency overlaps the node updates below
-        const auto rA = buf_rsrc(a.accum, nb4), rM = buf_rsrc(a.max_accum, nb4);
-#pragma unroll
-        for (int i = 0; i < ZC; ++i)
-#pragma unroll
-            for (int j = 0; j < YB; ++j) {
-                const bool need = rS2_[i][j] != 0 || Tw[i + 1][j + 1] >= pc.Tliq;
-                rAcc[i][j] = buf_ld(rA, need ? oF[i][j] : GM_OOB);
-                rMax[i][j] = buf_ld(rM, need ? oF[i][j] : GM_OOB);
-            }
-    }
-
-    // ---- outputs --------------------------------------------------------------
-    const auto rTo = buf_rsrc(a.Tout, nb4), rKCo = buf_rsrc(a.kcout, 2u * nb4);
-    const auto rS1o = buf_rsrc(a.S1out, (uint32_t)nn);
-    const bool s1_inplace = (const void*)a.S1out == a.S1;
-    const int nsub = (int)(a.n_sub < (int64_t)nn ? a.n_sub : (int64_t)nn);
-#pragma unroll
-    for (int i = 0; i < ZC; ++i) {
-#pragma unroll
-        for (int j = 0; j < YB; ++j) {
-            // elements around the node: k000 .. k111 by (x, y, z) side
-            const float k100 = Ek[i][j], k110 = Ek[i][j + 1];
-            const float k101 = Ek[i + 1][j], k111 = Ek[i + 1][j + 1];
-            const float k000 = lane_prev(k100), k010 = lane_prev(k110);
-            const float k001 = lane_prev(k101), k011 = lane_prev(k111);
-            const float cs = (Ec[i][j] + Ec[i][j + 1]) + (Ec[i + 1][j] + Ec[i + 1][j + 1]);
-            const float Msum = cs + lane_prev(cs);
-            // T of the 27 neighbours: planes i, i+1, i+2; rows j, j+1, j+2
-            const float Ti = Tw[i + 1][j + 1];
-            const float Pz00 = k000 + k001, Pz10 = k100 + k101, Pz01 = k010 + k011,
-                        Pz11 = k110 + k111;
-            const float Py00 = k000 + k010, Py10 = k100 + k110, Py01 = k001 + k011,
-                        Py11 = k101 + k111;
-            const float Px00 = k000 + k100, Px10 = k010 + k110, Px01 = k001 + k101,
-                        Px11 = k011 + k111;
-            const float Fx0 = Pz00 + Pz01, Fx1 = Pz10 + Pz11;
-            const float Fy0 = Pz00 + Pz10, Fy1 = Pz01 + Pz11;
-            const float Fz0 = Py00 + Py10, Fz1 = Py01 + Py11;
-            float KT = a.kc[0] * ((Fx0 + Fx1) * Ti);
-            KT += a.kc[1] * (Fx0 * lane_prev(Ti) + Fx1 * lane_next(Ti));
-            KT += a.kc[2] * (Fy0 * Tw[i + 1][j] + Fy1 * Tw[i + 1][j + 2]);
-            KT += a.kc[4] * (Fz0 * Tw[i][j + 1] + Fz1 * Tw[i + 2][j + 1]);
-            KT += a.kc[3] * (Pz00 * lane_prev(Tw[i + 1][j]) + Pz10 * lane_next(Tw[i + 1][j]) +
-                             Pz01 * lane_prev(Tw[i + 1][j + 2]) + Pz11 * lane_next(Tw[i + 1][j + 2]));
-            KT += a.kc[5] * (Py00 * lane_prev(Tw[i][j + 1]) + Py10 * lane_next(Tw[i][j + 1]) +
-                             Py01 * lane_prev(Tw[i + 2][j + 1]) + Py11 * lane_next(Tw[i + 2][j + 1]));
-            KT += a.kc[6] * (Px00 * Tw[i][j] + Px10 * Tw[i][j + 2] + Px01 * Tw[i + 2][j] +
-                             Px11 * Tw[i + 2][j + 2]);
-            KT += a.kc[7] * (k000 * lane_prev(Tw[i][j]) + k100 * lane_next(Tw[i][j]) +
-                             k010 * lane_prev(Tw[i][j + 2]) + k110 * lane_next(Tw[i][j + 2]) +
-                             k001 * lane_prev(Tw[i + 2][j]) + k101 * lane_next(Tw[i + 2][j]) +
-                             k011 * lane_prev(Tw[i + 2][j + 2]) + k111 * lane_next(Tw[i + 2][j + 2]));
-            float F = corr[i][j];
-            if (SRC) F += sx * sy[j] * sz[i];
-            if (i + 1 == ptop) F += Ftop[j];
-            // explicit update Ti + dt (F - K T) / M with the lumped mass
-            // M = mrow * sum of the 8 element means of rho*cp
-            float Tnew = Ti + (F - KT) * (__builtin_amdgcn_rcpf(Msum) * rmfac);
-            if (face_[i][j]) Tnew = a.alpha * rFa_[i][j] + (1.0f - a.alpha) * rFb_[i][j];
-            if (a.clamp && Tnew < Tamb) Tnew = Tamb;
-            const int idx = base + nx * j + nxy * i;
-            const bool s1 = rS1_[i][j] != 0 || Ti >= pc.Tliq || idx < nsub;
-            float kn, cn;
-            bool s1n;
-            if (REC) props_rec(pc, &m->prec[0][0], Tnew, s1, false, kn, cn, s1n);
-            else props_c(pc, sTab, Tnew, s1, false, kn, cn, s1n);
-            buf_st(Tnew, rTo, oF[i][j]);
-            // S1 updated in place (S1out == S1) is written only where it flips
-            buf_st8(s1, rS1o, (s1_inplace && s1 == (rS1_[i][j] != 0)) ? GM_OOB : oB[i][j]);
-            buf_st(kn, rKCo, oF[i][j]);
-            buf_st(cn, rKCo, oF[i][j], nb4);
-            if (ACC) {
-                // melt time: only molten nodes change accum and max_accum
-                // (max with 0 keeps max_accum >= 0), and S2 is written where
-                // it flips, so S2out may alias S2prev
-                const auto rS2o = buf_rsrc(a.S2out, (uint32_t)nn);
-                const auto rA = buf_rsrc(a.accum, nb4), rM = buf_rsrc(a.max_accum, nb4);
-                const bool s2p = rS2_[i][j] != 0;
-                const bool s2n = Ti >= pc.Tliq;
-                const bool reset = !s2p && s2n;
-                const float rr = reset ? rAcc[i][j] : 0.0f;
-                const uint32_t oa = s2n ? oF[i][j] : GM_OOB;
-                buf_st(rr > rMax[i][j] ? rr : rMax[i][j], rM, oa);
-                buf_st(rAcc[i][j] + row[5] - rr, rA, oa);
-                buf_st8(s2n, rS2o, s2n != s2p ? oB[i][j] : GM_OOB);
-            }
-        }
-    }
-}
-
 template <bool SRC, bool ACC, bool REC, int YB, int ZC>
 __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __restrict__ d,
                                                  const DevMat* __restrict__ m) {
-    step_dpp_brick<SRC, ACC, REC, YB, ZC>(a, d, m, xcd_tile(blockIdx.x, gridDim.x));
+#define GM_STEP_BRICK xcd_tile(blockIdx.x, gridDim.x)
+#include "gm_step_dpp_body.inc"
+#undef GM_STEP_BRICK
 }
 
 // Batched L3-type steps (source, surface flux, Dirichlet faces, no correction,
 // no melt time): many instances of their own mesh size and laser in one
 // launch.  bstart[i] is the first block of instance i (bstart[n] = total);
+// mats[i] is its material (the same as it[i].m);
 // the blocks are dealt out XCD-aware as in the single launch, then each finds
 // its instance (uniform binary search) and runs that instance's brick.
 template <bool REC, int ZC>
 __global__ void __launch_bounds__(64) k_step_dpp_batch(const BatchItem* __restrict__ it,
+                                                       const DevMat* __restrict__ mats,
                                                        const int* __restrict__ bstart, int n,
                                                        const GmStepDesc* __restrict__ d) {
     const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -1043,7 +744,16 @@ __global__ void __launch_bounds__(64) k_step_dpp_batch(const BatchItem* __restri
         if (bstart[mid] <= b) lo = mid;
         else hi = mid - 1;
     }
-    step_dpp_brick<true, false, REC, 1, ZC>(it[lo].a, d, it[lo].m, b - bstart[lo]);
+    constexpr bool SRC = true, ACC = false;
+    constexpr int YB = 1;
+    const LSArgs a = it[lo].a;
+    // materials indexed off the kernel argument (a known global pointer:
+    // scalar loads), not through the pointer stored in the item
+    const DevMat* __restrict__ m = mats + lo;
+    const int tb = b - bstart[lo];
+#define GM_STEP_BRICK tb
+#include "gm_step_dpp_body.inc"
+#undef GM_STEP_BRICK
 }
 
 // L1 step (computeL1Temperature, cF:2135-2204 with the L1 faces of
@@ -1338,22 +1048,22 @@ void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s) {
 }
 
 template <int ZC>
-static void batch_step_zc(const BatchItem* it, const int* bstart, int n, int nblocks, bool rec,
-                          const GmStepDesc* d, hipStream_t s) {
+static void batch_step_zc(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                          int nblocks, bool rec, const GmStepDesc* d, hipStream_t s) {
     if (rec)
-        hipLaunchKernelGGL((k_step_dpp_batch<true, ZC>), dim3(nblocks), dim3(64), 0, s, it, bstart,
-                           n, d);
+        hipLaunchKernelGGL((k_step_dpp_batch<true, ZC>), dim3(nblocks), dim3(64), 0, s, it, mats,
+                           bstart, n, d);
     else
-        hipLaunchKernelGGL((k_step_dpp_batch<false, ZC>), dim3(nblocks), dim3(64), 0, s, it,
+        hipLaunchKernelGGL((k_step_dpp_batch<false, ZC>), dim3(nblocks), dim3(64), 0, s, it, mats,
                            bstart, n, d);
 }
 
-void launch_batch_step(const BatchItem* it, const int* bstart, int n, int nblocks, int zc,
-                       bool rec, const GmStepDesc* d, hipStream_t s) {
+void launch_batch_step(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                       int nblocks, int zc, bool rec, const GmStepDesc* d, hipStream_t s) {
     if (n <= 0 || nblocks <= 0) return;
-    if (zc == 2) batch_step_zc<2>(it, bstart, n, nblocks, rec, d, s);
-    else if (zc == 4) batch_step_zc<4>(it, bstart, n, nblocks, rec, d, s);
-    else batch_step_zc<3>(it, bstart, n, nblocks, rec, d, s);
+    if (zc == 2) batch_step_zc<2>(it, mats, bstart, n, nblocks, rec, d, s);
+    else if (zc == 4) batch_step_zc<4>(it, mats, bstart, n, nblocks, rec, d, s);
+    else batch_step_zc<3>(it, mats, bstart, n, nblocks, rec, d, s);
 }
 
 // the same step of one instance in a launch of its own (the unbatched form)
