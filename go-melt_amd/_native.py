@@ -138,6 +138,11 @@ _SIGS = {
     "gm_batch_time": (_I, [_P, _I, _I, _FP]),
     "gm_batch_field": (_I, [_P, _I, _I, C.POINTER(_P), C.POINTER(_I64)]),
     "gm_batch_refresh": (_I, [_P]),
+    "gm_conv3d_fwd": (_I, [_IP, _P, _P, _P, _I, _P, _P, _P]),
+    "gm_conv3d_bwd_data": (_I, [_IP, _P, _P, _P, _P, _P, _P]),
+    "gm_conv3d_bwd_weight": (_I, [_IP, _P, _P, _P, _P]),
+    "gm_adam": (_I, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I, _P]),
+    "gm_f32_to_bf16": (_I, [_P, _P, _I64, _P]),
 }
 
 SYMBOLS = tuple(_SIGS)

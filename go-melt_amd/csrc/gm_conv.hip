@@ -1,0 +1,284 @@
+// 3x3x3 convolutions for the learned L3 operator: BF16 operands on the MFMA
+// matrix cores (v_mfma_f32_16x16x16_bf16), FP32 accumulation.
+//
+// Layouts (channels last):
+//   activations  X[vox][C], vox = ((b*D + z)*H + y)*W + x, bf16 (uint16 bits)
+//   weights      Wk[tap][Cout][Cin], tap = (dz+1)*9 + (dy+1)*3 + (dx+1), bf16
+//   outputs      FP32 (and optionally a bf16 copy for the next layer)
+// Stride 1, zero padding 1, so every layer keeps D x H x W.
+//
+// One kernel does the implicit GEMM Y[vox][o] = sum_{tap,i} X[vox+s(tap)][i]
+// Wk[tap][o][i] for the forward pass and, with the flipped and transposed
+// weights Wk'[tap][i][o] = Wk[26-tap][o][i], for the gradient of the input.
+// The weight gradient dWk[tap][o][i] = sum_vox dY[vox][o] X[vox+s(tap)][i] is
+// a second kernel that reduces over voxels (split over blocks, FP32 atomics).
+//
+// Tiles: a block of 4 waves computes 64 x 64 outputs; each wave a 32 x 32
+// quarter as 2 x 2 MFMA tiles of 16 x 16; K advances 16 at a time (one tap,
+// 16 input channels) through LDS.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gm_internal.h"
+
+namespace gm {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    // round to nearest even (finite inputs)
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+struct ConvDims {
+    int nb, D, H, W;  // batch and spatial extent
+    int ci, co;       // channels (multiples of 16)
+    int64_t nvox;     // nb * D * H * W
+};
+
+// voxel index shifted by tap, or -1 outside the volume (zero padding)
+__device__ __forceinline__ int64_t vox_shift(const ConvDims& c, int64_t v, int tap) {
+    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+    const int x = (int)(v % c.W);
+    const int64_t r = v / c.W;
+    const int y = (int)(r % c.H);
+    const int64_t r2 = r / c.H;
+    const int z = (int)(r2 % c.D);
+    const int xx = x + dx, yy = y + dy, zz = z + dz;
+    if (xx < 0 || xx >= c.W || yy < 0 || yy >= c.H || zz < 0 || zz >= c.D) return -1;
+    return v + dx + (int64_t)c.W * (dy + (int64_t)c.H * dz);
+}
+
+// Implicit-GEMM convolution.  Grid: (ceil(nvox/64), co/64 rounded up).
+// LDS tiles: sA[64 vox][16 k], sB[64 out][16 k] in bf16, K-contiguous, so a
+// lane's MFMA fragment (4 consecutive k) is one 8-byte LDS read.
+__global__ void __launch_bounds__(256) k_conv_fwd(ConvDims c, const uint16_t* __restrict__ X,
+                                                  const uint16_t* __restrict__ Wk,
+                                                  const float* __restrict__ bias, int relu,
+                                                  float* __restrict__ Y,
+                                                  uint16_t* __restrict__ Ybf) {
+    __shared__ uint16_t sA[64][16 + 4];
+    __shared__ uint16_t sB[64][16 + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t m0 = (int64_t)blockIdx.x * 64;
+    const int n0 = blockIdx.y * 64;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    v4f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    // staging: thread t copies 4 bf16 (8 bytes) of row t/4, k (t%4)*4 .. +3
+    const int srow = tid >> 2, sk = (tid & 3) * 4;
+    const int64_t vrow = m0 + srow;
+    const int kchunks = c.ci / 16;
+    for (int tap = 0; tap < 27; ++tap) {
+        const int64_t vs = vrow < c.nvox ? vox_shift(c, vrow, tap) : -1;
+        for (int kc = 0; kc < kchunks; ++kc) {
+            const int i0 = kc * 16 + sk;
+            uint2 av = make_uint2(0u, 0u), bv = make_uint2(0u, 0u);
+            if (vs >= 0) av = *(const uint2*)(X + vs * c.ci + i0);
+            const int o = n0 + srow;
+            if (o < c.co) bv = *(const uint2*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
+            __syncthreads();
+            *(uint2*)&sA[srow][sk] = av;
+            *(uint2*)&sB[srow][sk] = bv;
+            __syncthreads();
+            // fragments: A[i][k] at lane i + 16 (k/4), item k%4; B[k][j] likewise
+            const int fi = lane & 15, fk = (lane >> 4) * 4;
+            v4s fa[2], fb[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const uint2 ua = *(const uint2*)&sA[wm + t * 16 + fi][fk];
+                const uint2 ub = *(const uint2*)&sB[wn + t * 16 + fi][fk];
+                fa[t] = __builtin_bit_cast(v4s, ua);
+                fb[t] = __builtin_bit_cast(v4s, ub);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[a], fb[b], acc[a][b],
+                                                                          0, 0, 0);
+        }
+    }
+    // D[i][j] at lane j + 16 (i/4), item i%4
+    const int dj = lane & 15, di = (lane >> 4) * 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t v = m0 + wm + a * 16 + di + r;
+                const int o = n0 + wn + b * 16 + dj;
+                if (v < c.nvox && o < c.co) {
+                    float y = acc[a][b][r] + (bias ? bias[o] : 0.0f);
+                    if (relu && y < 0.0f) y = 0.0f;
+                    Y[v * c.co + o] = y;
+                    if (Ybf) Ybf[v * c.co + o] = f2bf(y);
+                }
+            }
+}
+
+// Wt[tap][i][o] = Wk[26 - tap][o][i] (weights of the input-gradient pass)
+__global__ void k_conv_wflip(int ci, int co, const uint16_t* __restrict__ Wk,
+                             uint16_t* __restrict__ Wt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)27 * ci * co;
+    if (t >= n) return;
+    const int tap = (int)(t / ((int64_t)ci * co));
+    const int rem = (int)(t - (int64_t)tap * ci * co);
+    const int i = rem / co, o = rem - i * co;
+    Wt[t] = Wk[((int64_t)(26 - tap) * co + o) * ci + i];
+}
+
+// Weight gradient: dW[tap][o][i] += sum over this block's voxels of
+// dY[v][o] X[v + s(tap)][i].  Grid: (co/64 rounded up, ci/64 rounded up,
+// 27 * vsplit); each block reduces nvox/vsplit voxels in chunks of 16 and
+// adds its 64 x 64 tile with FP32 atomics (dW zeroed by the caller).
+__global__ void __launch_bounds__(256) k_conv_wgrad(ConvDims c, const uint16_t* __restrict__ X,
+                                                    const uint16_t* __restrict__ dY, int vsplit,
+                                                    float* __restrict__ dW) {
+    __shared__ uint16_t sA[64][16 + 4];  // [o][vox]
+    __shared__ uint16_t sB[64][16 + 4];  // [i][vox]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int o0 = blockIdx.x * 64, i0 = blockIdx.y * 64;
+    const int tap = blockIdx.z % 27, part = blockIdx.z / 27;
+    const int64_t per = (c.nvox + vsplit - 1) / vsplit;
+    const int64_t vbeg = (int64_t)part * per;
+    const int64_t vend = vbeg + per < c.nvox ? vbeg + per : c.nvox;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    v4f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    // staging: thread t loads 4 channels (8 bytes) of voxel t/16, channels
+    // (t%16)*4 .. +3 of the 64-wide tile, and writes them transposed
+    const int sv = tid >> 4, sc = (tid & 15) * 4;
+    for (int64_t vb = vbeg; vb < vend; vb += 16) {
+        const int64_t v = vb + sv;
+        uint2 dv = make_uint2(0u, 0u), xv = make_uint2(0u, 0u);
+        if (v < vend) {
+            if (o0 + sc < c.co) dv = *(const uint2*)(dY + v * c.co + o0 + sc);
+            const int64_t vs = vox_shift(c, v, tap);
+            if (vs >= 0 && i0 + sc < c.ci) xv = *(const uint2*)(X + vs * c.ci + i0 + sc);
+        }
+        __syncthreads();
+        const uint16_t* d4 = (const uint16_t*)&dv;
+        const uint16_t* x4 = (const uint16_t*)&xv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            sA[sc + q][sv] = d4[q];
+            sB[sc + q][sv] = x4[q];
+        }
+        __syncthreads();
+        const int fi = lane & 15, fk = (lane >> 4) * 4;
+        v4s fa[2], fb[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            fa[t] = __builtin_bit_cast(v4s, *(const uint2*)&sA[wm + t * 16 + fi][fk]);
+            fb[t] = __builtin_bit_cast(v4s, *(const uint2*)&sB[wn + t * 16 + fi][fk]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] =
+                    __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    const int dj = lane & 15, di = (lane >> 4) * 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = o0 + wm + a * 16 + di + r;
+                const int i = i0 + wn + b * 16 + dj;
+                if (o < c.co && i < c.ci)
+                    atomicAdd(dW + ((int64_t)tap * c.co + o) * c.ci + i, acc[a][b][r]);
+            }
+}
+
+// Adam on FP32 master weights, with the bf16 copy the kernels read
+__global__ void k_adam(float* __restrict__ w, uint16_t* __restrict__ wbf,
+                       const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gi = g[t];
+    const float mi = b1 * m[t] + (1.0f - b1) * gi;
+    const float vi = b2 * v[t] + (1.0f - b2) * gi * gi;
+    m[t] = mi;
+    v[t] = vi;
+    const float wi = w[t] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    w[t] = wi;
+    if (wbf) wbf[t] = f2bf(wi);
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ a, uint16_t* __restrict__ b, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = f2bf(a[t]);
+}
+
+static ConvDims dims(const int32_t* s) {
+    ConvDims c;
+    c.nb = s[0];
+    c.D = s[1];
+    c.H = s[2];
+    c.W = s[3];
+    c.ci = s[4];
+    c.co = s[5];
+    c.nvox = (int64_t)c.nb * c.D * c.H * c.W;
+    return c;
+}
+
+bool conv_shape_ok(const int32_t* s) {
+    for (int q = 0; q < 6; ++q)
+        if (s[q] < 1) return false;
+    return s[4] % 16 == 0 && s[5] % 16 == 0 &&
+           (int64_t)s[0] * s[1] * s[2] * s[3] < ((int64_t)1 << 40);
+}
+
+void launch_conv_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk,
+                     const float* bias, int relu, float* Y, uint16_t* Ybf, hipStream_t s) {
+    const ConvDims c = dims(shape);
+    const dim3 grid((unsigned)((c.nvox + 63) / 64), (unsigned)((c.co + 63) / 64));
+    hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
+}
+
+void launch_conv_wflip(int ci, int co, const uint16_t* Wk, uint16_t* Wt, hipStream_t s) {
+    const int64_t n = (int64_t)27 * ci * co;
+    hipLaunchKernelGGL(k_conv_wflip, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ci, co,
+                       Wk, Wt);
+}
+
+void launch_conv_wgrad(const int32_t* shape, const uint16_t* X, const uint16_t* dY, int vsplit,
+                       float* dW, hipStream_t s) {
+    const ConvDims c = dims(shape);
+    const dim3 grid((unsigned)((c.co + 63) / 64), (unsigned)((c.ci + 63) / 64),
+                    (unsigned)(27 * vsplit));
+    hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
+}
+
+void launch_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, int64_t n, float lr,
+                 float b1, float b2, float eps, int step, hipStream_t s) {
+    const float c1 = 1.0f / (1.0f - powf(b1, (float)step));
+    const float c2 = 1.0f / (1.0f - powf(b2, (float)step));
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, wbf, g, m,
+                       v, n, lr, b1, b2, eps, c1, c2);
+}
+
+void launch_f32_to_bf16(const float* a, uint16_t* b, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b,
+                       n);
+}
+
+}  // namespace gm

@@ -1926,3 +1926,58 @@ extern "C" int gm_batch_refresh(gm_batch* b) {
     HIPCHK(hipGetLastError());
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Learned L3 operator: 3x3x3 BF16 MFMA convolutions, Adam
+// ---------------------------------------------------------------------------
+
+extern "C" int gm_conv3d_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk,
+                             const float* bias, int relu, float* Y, uint16_t* Ybf, void* stream) {
+    if (!shape || !X || !Wk || !Y) return fail("gm_conv3d_fwd: null argument");
+    if (!conv_shape_ok(shape))
+        return fail("gm_conv3d_fwd: shape {nb, D, H, W, cin, cout} needs positive sizes and "
+                    "channel counts that are multiples of 16");
+    launch_conv_fwd(shape, X, Wk, bias, relu, Y, Ybf, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_conv3d_bwd_data(const int32_t* shape, const uint16_t* dY, const uint16_t* Wk,
+                                  uint16_t* Wt_work, float* dX, uint16_t* dXbf, void* stream) {
+    if (!shape || !dY || !Wk || !Wt_work || !dX) return fail("gm_conv3d_bwd_data: null argument");
+    if (!conv_shape_ok(shape)) return fail("gm_conv3d_bwd_data: bad shape");
+    const int32_t sb[6] = {shape[0], shape[1], shape[2], shape[3], shape[5], shape[4]};
+    launch_conv_wflip(shape[4], shape[5], Wk, Wt_work, (hipStream_t)stream);
+    launch_conv_fwd(sb, dY, Wt_work, nullptr, 0, dX, dXbf, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_conv3d_bwd_weight(const int32_t* shape, const uint16_t* X, const uint16_t* dY,
+                                    float* dW, void* stream) {
+    if (!shape || !X || !dY || !dW) return fail("gm_conv3d_bwd_weight: null argument");
+    if (!conv_shape_ok(shape)) return fail("gm_conv3d_bwd_weight: bad shape");
+    const int64_t nvox = (int64_t)shape[0] * shape[1] * shape[2] * shape[3];
+    // about 1024 voxels per block, at most 2048 parts (grid z = 27 * parts)
+    int64_t parts = (nvox + 1023) / 1024;
+    if (parts > 2048) parts = 2048;
+    if (parts < 1) parts = 1;
+    launch_conv_wgrad(shape, X, dY, (int)parts, dW, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, int64_t n,
+                       float lr, float beta1, float beta2, float eps, int step, void* stream) {
+    if (!w || !g || !m || !v || n < 0 || step < 1) return fail("gm_adam: bad arguments");
+    if (n > 0) launch_adam(w, wbf, g, m, v, n, lr, beta1, beta2, eps, step, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_f32_to_bf16(const float* a, uint16_t* b, int64_t n, void* stream) {
+    if (!a || !b || n < 0) return fail("gm_f32_to_bf16: bad arguments");
+    if (n > 0) launch_f32_to_bf16(a, b, n, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}

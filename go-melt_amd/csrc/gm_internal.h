@@ -202,6 +202,18 @@ struct RArgs {
 // ---- launchers (gm_kernels.hip) -------------------------------------------
 namespace gm {
 
+// 3x3x3 BF16 MFMA convolutions of the learned operator (gm_conv.hip);
+// shape = {nb, D, H, W, cin, cout}
+bool conv_shape_ok(const int32_t* shape);
+void launch_conv_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk,
+                     const float* bias, int relu, float* Y, uint16_t* Ybf, hipStream_t s);
+void launch_conv_wflip(int ci, int co, const uint16_t* Wk, uint16_t* Wt, hipStream_t s);
+void launch_conv_wgrad(const int32_t* shape, const uint16_t* X, const uint16_t* dY, int vsplit,
+                       float* dW, hipStream_t s);
+void launch_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, int64_t n, float lr,
+                 float b1, float b2, float eps, int step, hipStream_t s);
+void launch_f32_to_bf16(const float* a, uint16_t* b, int64_t n, hipStream_t s);
+
 // batched L3-type steps
 int batch_bricks(const gm_grid& g, int zc);
 void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s);

@@ -338,6 +338,40 @@ int gm_batch_time(gm_batch* b, int batched, int iters, float* ms_per_step);
 int gm_batch_field(gm_batch* b, int i, int field, void** ptr, int64_t* count);
 int gm_batch_refresh(gm_batch* b);
 
+/* ------------------------------------------------------------------------ */
+/* Learned L3 operator: convolution kernels                                  */
+/* ------------------------------------------------------------------------ */
+
+/* 3x3x3 convolutions, stride 1, zero padding 1, of the learned L3 operator.
+ * No reference counterpart (the reference has no learned part).  BF16
+ * operands (uint16 bit patterns) on the MFMA matrix cores, FP32 accumulation.
+ * shape = {nb, D, H, W, cin, cout} of the forward convolution; cin and cout
+ * multiples of 16.  Channels last: X[((b*D + z)*H + y)*W + x][cin];
+ * weights Wk[tap][cout][cin], tap = (dz+1)*9 + (dy+1)*3 + (dx+1).
+ *
+ * Forward: Y[v][o] = bias[o] + sum_{tap,i} X[v + s(tap)][i] Wk[tap][o][i],
+ * then max(0, .) if relu; Y FP32, Ybf (nullable) its bf16 copy. */
+int gm_conv3d_fwd(const int32_t* shape_host, const uint16_t* X, const uint16_t* Wk,
+                  const float* bias, int relu, float* Y, uint16_t* Ybf, void* stream);
+
+/* Gradient of the input: dX[v][i] = sum_{tap,o} dY[v - s(tap)][o] Wk[tap][o][i]
+ * (Wt_work: 27*cin*cout bf16 of caller scratch for the flipped weights). */
+int gm_conv3d_bwd_data(const int32_t* shape_host, const uint16_t* dY, const uint16_t* Wk,
+                       uint16_t* Wt_work, float* dX, uint16_t* dXbf, void* stream);
+
+/* Gradient of the weights, ADDED to dW (FP32, [tap][cout][cin]):
+ * dW[tap][o][i] += sum_v dY[v][o] X[v + s(tap)][i]. */
+int gm_conv3d_bwd_weight(const int32_t* shape_host, const uint16_t* X, const uint16_t* dY,
+                         float* dW, void* stream);
+
+/* One Adam step on FP32 master weights w (and their bf16 copy wbf, nullable):
+ * m, v moments; step >= 1 (bias corrections 1 - beta^step). */
+int gm_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, int64_t n, float lr,
+            float beta1, float beta2, float eps, int step, void* stream);
+
+/* bf16 (round to nearest even) copy of FP32 values. */
+int gm_f32_to_bf16(const float* a, uint16_t* b, int64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -667,3 +667,69 @@ def test_energy_balance_active_layers(oracle):
     assert abs(s[6]) < 1e-12
     n_act = nx * ny * 3
     assert s[2] == pytest.approx(dt * Fc[:n_act].sum(), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# convolutions of the learned operator (oracle/conv_oracle.py)
+# ---------------------------------------------------------------------------
+
+def test_conv_identity_and_shift_kernels():
+    from oracle import conv_oracle as CO
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((2, 4, 5, 6, 3))
+    W = np.zeros((27, 3, 3))
+    W[13] = np.eye(3)  # centre tap: identity
+    np.testing.assert_allclose(CO.conv3d(X, W), X, atol=0, rtol=0)
+    # tap (dz, dy, dx) = (1, 0, -1): Y[z, y, x] = X[z+1, y, x-1], zero outside
+    W = np.zeros((27, 3, 3))
+    W[2 * 9 + 1 * 3 + 0] = np.eye(3)
+    Y = CO.conv3d(X, W)
+    np.testing.assert_array_equal(Y[:, :-1, :, 1:], X[:, 1:, :, :-1])
+    assert np.all(Y[:, -1] == 0) and np.all(Y[:, :, :, 0] == 0)
+    # bias and channel mixing on one tap
+    M = rng.standard_normal((2, 3))
+    W = np.zeros((27, 2, 3))
+    W[13] = M
+    np.testing.assert_allclose(CO.conv3d(X, W, [1.0, -2.0]), X @ M.T + [1.0, -2.0], rtol=1e-12)
+
+
+def test_conv_gradients_are_adjoints():
+    """<dY, conv(X; W)> = <bwd_data(dY; W), X> = <bwd_weight(X, dY), W>."""
+    from oracle import conv_oracle as CO
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((1, 3, 4, 5, 4))
+    W = rng.standard_normal((27, 6, 4))
+    dY = rng.standard_normal((1, 3, 4, 5, 6))
+    lhs = float((dY * CO.conv3d(X, W)).sum())
+    assert float((CO.conv3d_bwd_data(dY, W) * X).sum()) == pytest.approx(lhs, rel=1e-12)
+    assert float((CO.conv3d_bwd_weight(X, dY) * W).sum()) == pytest.approx(lhs, rel=1e-12)
+
+
+def test_conv_two_layer_gradients_finite_differences():
+    from oracle import conv_oracle as CO
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((1, 3, 3, 4, 2))
+    W1 = 0.3 * rng.standard_normal((27, 4, 2))
+    b1 = 0.1 * rng.standard_normal(4)
+    W2 = 0.3 * rng.standard_normal((27, 3, 4))
+    b2 = 0.1 * rng.standard_normal(3)
+    T = rng.standard_normal((1, 3, 3, 4, 3))
+    L, (dW1, db1, dW2, db2), _ = CO.two_layer_loss(X, W1, b1, W2, b2, T)
+    eps = 1e-6
+    for P, G in ((W1, dW1), (b1, db1), (W2, dW2), (b2, db2)):
+        for idx in [tuple(rng.integers(s) for s in P.shape) for _ in range(6)]:
+            old = P[idx]
+            P[idx] = old + eps
+            Lp = CO.two_layer_loss(X, W1, b1, W2, b2, T)[0]
+            P[idx] = old - eps
+            Lm = CO.two_layer_loss(X, W1, b1, W2, b2, T)[0]
+            P[idx] = old
+            assert (Lp - Lm) / (2 * eps) == pytest.approx(G[idx], rel=1e-5, abs=1e-7)
+
+
+def test_bf16_rounding():
+    from oracle import conv_oracle as CO
+    v = np.array([1.0, 1.00390625, 1.005859375, 1.0078125, -3.0, 0.0], np.float32)
+    # 1 + 2^-8 ties to even (1.0); 1 + 1.5 * 2^-8 rounds up to 1 + 2^-7
+    np.testing.assert_array_equal(CO.bf16_round(v), [1.0, 1.0, 1.0078125, 1.0078125, -3.0, 0.0])
+    assert CO.bf16_bits(np.float32(1.0)) == 0x3F80
