@@ -143,6 +143,16 @@ _SIGS = {
     "gm_conv3d_bwd_weight": (_I, [_IP, _P, _P, _P, _P]),
     "gm_adam": (_I, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I, _P]),
     "gm_f32_to_bf16": (_I, [_P, _P, _I64, _P]),
+    "gm_nn_pool2": (_I, [_I, _IP, _P, _P, _P]),
+    "gm_nn_up2": (_I, [_I, _IP, _P, _P, _P]),
+    "gm_nn_concat": (_I, [_I64, _I, _I, _P, _P, _P, _P]),
+    "gm_nn_split": (_I, [_I64, _I, _I, _P, _P, _P, _P]),
+    "gm_nn_relu_bwd": (_I, [_I64, _P, _P, _P, _P, _P, _P]),
+    "gm_nn_bias_grad": (_I, [_I64, _I, _P, _P, _P]),
+    "gm_nn_mse": (_I, [_I64, _I, _P, _P, _P, _P, _P]),
+    "gm_nn_window_input": (_I, [_IP, _FP, _P, _P, _FP, _P, _P]),
+    "gm_nn_window_apply": (_I, [_IP, _P, _P, _P, _I, _F, _F, _P, _P]),
+    "gm_nn_axpy": (_I, [_I64, _F, _P, _P, _P]),
 }
 
 SYMBOLS = tuple(_SIGS)

@@ -1981,3 +1981,96 @@ extern "C" int gm_f32_to_bf16(const float* a, uint16_t* b, int64_t n, void* stre
     HIPCHK(hipGetLastError());
     return 0;
 }
+
+// ---- elementwise and reduction kernels of the operator's network ----------
+
+static bool shape5_ok(const int32_t* s) {
+    return s && s[0] > 0 && s[1] > 0 && s[2] > 0 && s[3] > 0 && s[4] > 0;
+}
+
+extern "C" int gm_nn_pool2(int dir, const int32_t* coarse_shape, const void* in, void* out,
+                           void* stream) {
+    if (!shape5_ok(coarse_shape) || !in || !out || (dir != 0 && dir != 1))
+        return fail("gm_nn_pool2: bad arguments");
+    const int32_t* c = coarse_shape;
+    launch_nn_pool2(dir, c[0], c[1], c[2], c[3], c[4], in, out, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_up2(int dir, const int32_t* coarse_shape, const void* in, void* out,
+                         void* stream) {
+    if (!shape5_ok(coarse_shape) || !in || !out || (dir != 0 && dir != 1))
+        return fail("gm_nn_up2: bad arguments");
+    const int32_t* c = coarse_shape;
+    launch_nn_up2(dir, c[0], c[1], c[2], c[3], c[4], in, out, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_concat(int64_t nvox, int ca, int cb, const uint16_t* A, const uint16_t* B,
+                            uint16_t* Y, void* stream) {
+    if (nvox < 0 || ca < 1 || cb < 1 || !A || !B || !Y) return fail("gm_nn_concat: bad arguments");
+    if (nvox) launch_nn_concat(nvox, ca, cb, A, B, Y, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_split(int64_t nvox, int ca, int cb, const float* dY, float* dA, float* dB,
+                           void* stream) {
+    if (nvox < 0 || ca < 1 || cb < 1 || !dY || !dA || !dB) return fail("gm_nn_split: bad arguments");
+    if (nvox) launch_nn_split(nvox, ca, cb, dY, dA, dB, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_relu_bwd(int64_t n, const float* dA, const float* dA2, const float* A,
+                              float* dZ, uint16_t* dZb, void* stream) {
+    if (n < 0 || !dA || !dZ) return fail("gm_nn_relu_bwd: bad arguments");
+    if (n) launch_nn_relu_bwd(n, dA, dA2, A, dZ, dZb, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_bias_grad(int64_t nvox, int C, const float* dZ, float* db, void* stream) {
+    if (nvox < 0 || C < 1 || !dZ || !db) return fail("gm_nn_bias_grad: bad arguments");
+    if (nvox) launch_nn_bias_grad(nvox, C, dZ, db, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,
+                         double* loss, void* stream) {
+    if (nvox < 0 || C < 1 || !Y || !target || !dY || !loss) return fail("gm_nn_mse: bad arguments");
+    if (nvox) launch_nn_mse(nvox, C, Y, target, dY, loss, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_window_input(const int32_t* n_host, const float* geo_host, const float* T,
+                                  const float* Tface, const float* prm_host, uint16_t* X,
+                                  void* stream) {
+    if (!n_host || !geo_host || !T || !prm_host || !X || n_host[0] < 1 || n_host[1] < 1 ||
+        n_host[2] < 1)
+        return fail("gm_nn_window_input: bad arguments");
+    launch_nn_window_input(n_host, geo_host, T, Tface, prm_host, X, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_window_apply(const int32_t* n_host, const float* T, const float* Tface,
+                                  const float* Y, int C, float T_scale, float T_floor, float* Tn,
+                                  void* stream) {
+    if (!n_host || !T || !Y || !Tn || C < 1 || n_host[0] < 1 || n_host[1] < 1 || n_host[2] < 1)
+        return fail("gm_nn_window_apply: bad arguments");
+    launch_nn_window_apply(n_host, T, Tface, Y, C, T_scale, T_floor, Tn, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_axpy(int64_t n, float a, const float* x, float* y, void* stream) {
+    if (n < 0 || !x || !y) return fail("gm_nn_axpy: bad arguments");
+    if (n) launch_nn_axpy(n, a, x, y, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}

@@ -214,6 +214,26 @@ void launch_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, in
                  float b1, float b2, float eps, int step, hipStream_t s);
 void launch_f32_to_bf16(const float* a, uint16_t* b, int64_t n, hipStream_t s);
 
+// elementwise and reduction kernels of the operator's network (gm_nn.hip)
+void launch_nn_pool2(int dir, int nb, int D, int H, int W, int C, const void* in, void* out,
+                     hipStream_t s);
+void launch_nn_up2(int dir, int nb, int D, int H, int W, int C, const void* in, void* out,
+                   hipStream_t s);
+void launch_nn_concat(int64_t nvox, int ca, int cb, const uint16_t* A, const uint16_t* B,
+                      uint16_t* Y, hipStream_t s);
+void launch_nn_split(int64_t nvox, int ca, int cb, const float* dY, float* dA, float* dB,
+                     hipStream_t s);
+void launch_nn_relu_bwd(int64_t n, const float* dA, const float* dA2, const float* A, float* dZ,
+                        uint16_t* dZb, hipStream_t s);
+void launch_nn_bias_grad(int64_t nvox, int C, const float* dZ, float* db, hipStream_t s);
+void launch_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,
+                   double* loss, hipStream_t s);
+void launch_nn_window_input(const int32_t* n, const float* geo, const float* T, const float* Tface,
+                            const float* prm, uint16_t* X, hipStream_t s);
+void launch_nn_axpy(int64_t n, float a, const float* x, float* y, hipStream_t s);
+void launch_nn_window_apply(const int32_t* n, const float* T, const float* Tface, const float* Y,
+                            int C, float T_scale, float T_floor, float* Tn, hipStream_t s);
+
 // batched L3-type steps
 int batch_bricks(const gm_grid& g, int zc);
 void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s);

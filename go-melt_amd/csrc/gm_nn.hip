@@ -1,0 +1,302 @@
+// Elementwise and reduction kernels of the learned L3 operator's network
+// (the 3x3x3 convolutions are in gm_conv.hip).  Channels last:
+// a[((b*D + z)*H + y)*W + x][C].  Activations feeding a convolution are bf16
+// (uint16 bits); gradients are FP32 until they feed a convolution.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gm_internal.h"
+
+namespace gm {
+
+__device__ __forceinline__ float nn_bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t nn_f2bf(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+static inline unsigned nn_blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// 2x2x2 average pooling: Y[b, z, y, x, c] = mean of X over the 8 fine voxels
+// (fine extent 2D x 2H x 2W); dir 0 forward (bf16 -> bf16), dir 1 backward
+// (FP32 dY -> FP32 dX = dY / 8 on each fine voxel)
+__global__ void k_pool2(int nb, int D, int H, int W, int C, const uint16_t* __restrict__ X,
+                        uint16_t* __restrict__ Y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)nb * D * H * W * C;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    int64_t v = t / C;
+    const int x = (int)(v % W);
+    v /= W;
+    const int y = (int)(v % H);
+    v /= H;
+    const int z = (int)(v % D);
+    const int b = (int)(v / D);
+    const int W2 = 2 * W, H2 = 2 * H, D2 = 2 * D;
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int xx = 2 * x + (q & 1), yy = 2 * y + ((q >> 1) & 1), zz = 2 * z + (q >> 2);
+        s += nn_bf2f(X[((((int64_t)b * D2 + zz) * H2 + yy) * W2 + xx) * C + c]);
+    }
+    Y[t] = nn_f2bf(0.125f * s);
+}
+
+__global__ void k_pool2_bwd(int nb, int D, int H, int W, int C, const float* __restrict__ dY,
+                            float* __restrict__ dX) {
+    // one thread per fine element
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int W2 = 2 * W, H2 = 2 * H, D2 = 2 * D;
+    const int64_t n = (int64_t)nb * D2 * H2 * W2 * C;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    int64_t v = t / C;
+    const int x = (int)(v % W2);
+    v /= W2;
+    const int y = (int)(v % H2);
+    v /= H2;
+    const int z = (int)(v % D2);
+    const int b = (int)(v / D2);
+    dX[t] = 0.125f * dY[((((int64_t)b * D + z / 2) * H + y / 2) * W + x / 2) * C + c];
+}
+
+// nearest 2x upsampling (coarse extent D, H, W): forward bf16 -> bf16;
+// backward sums the 8 fine gradients (FP32)
+__global__ void k_up2(int nb, int D, int H, int W, int C, const uint16_t* __restrict__ X,
+                      uint16_t* __restrict__ Y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int W2 = 2 * W, H2 = 2 * H, D2 = 2 * D;
+    const int64_t n = (int64_t)nb * D2 * H2 * W2 * C;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    int64_t v = t / C;
+    const int x = (int)(v % W2);
+    v /= W2;
+    const int y = (int)(v % H2);
+    v /= H2;
+    const int z = (int)(v % D2);
+    const int b = (int)(v / D2);
+    Y[t] = X[((((int64_t)b * D + z / 2) * H + y / 2) * W + x / 2) * C + c];
+}
+
+__global__ void k_up2_bwd(int nb, int D, int H, int W, int C, const float* __restrict__ dY,
+                          float* __restrict__ dX) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)nb * D * H * W * C;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    int64_t v = t / C;
+    const int x = (int)(v % W);
+    v /= W;
+    const int y = (int)(v % H);
+    v /= H;
+    const int z = (int)(v % D);
+    const int b = (int)(v / D);
+    const int W2 = 2 * W, H2 = 2 * H, D2 = 2 * D;
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int xx = 2 * x + (q & 1), yy = 2 * y + ((q >> 1) & 1), zz = 2 * z + (q >> 2);
+        s += dY[((((int64_t)b * D2 + zz) * H2 + yy) * W2 + xx) * C + c];
+    }
+    dX[t] = s;
+}
+
+// channel concatenation Y = [A | B] (bf16) and its backward split (FP32)
+__global__ void k_concat(int64_t nvox, int ca, int cb, const uint16_t* __restrict__ A,
+                         const uint16_t* __restrict__ B, uint16_t* __restrict__ Y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int cc = ca + cb;
+    if (t >= nvox * cc) return;
+    const int64_t v = t / cc;
+    const int c = (int)(t - v * cc);
+    Y[t] = c < ca ? A[v * ca + c] : B[v * cb + (c - ca)];
+}
+
+__global__ void k_split(int64_t nvox, int ca, int cb, const float* __restrict__ dY,
+                        float* __restrict__ dA, float* __restrict__ dB) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int cc = ca + cb;
+    if (t >= nvox * cc) return;
+    const int64_t v = t / cc;
+    const int c = (int)(t - v * cc);
+    if (c < ca) dA[v * ca + c] = dY[t];
+    else dB[v * cb + (c - ca)] = dY[t];
+}
+
+// ReLU backward through the layer's output A (A > 0 where the input was):
+// dZ = dA (+ dA2 when given) where A > 0, else 0; FP32 and a bf16 copy
+__global__ void k_relu_bwd(int64_t n, const float* __restrict__ dA, const float* __restrict__ dA2,
+                           const float* __restrict__ A, float* __restrict__ dZ,
+                           uint16_t* __restrict__ dZb) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float g = dA[t] + (dA2 ? dA2[t] : 0.0f);
+    if (A && !(A[t] > 0.0f)) g = 0.0f;
+    dZ[t] = g;
+    if (dZb) dZb[t] = nn_f2bf(g);
+}
+
+// db[c] += sum over voxels of dZ[v][c]: each block sums 256 voxels of one
+// 64-channel slice (blockIdx.y), one atomic per channel and block
+__global__ void k_bias_grad(int64_t nvox, int C, const float* __restrict__ dZ,
+                            float* __restrict__ db) {
+    __shared__ float part[4][64];
+    const int c = blockIdx.y * 64 + (threadIdx.x & 63);
+    const int r = threadIdx.x >> 6;  // 4 row groups
+    float s = 0.0f;
+    const int64_t v0 = (int64_t)blockIdx.x * 256;
+    if (c < C)
+        for (int k = r; k < 256; k += 4) {
+            const int64_t v = v0 + k;
+            if (v < nvox) s += dZ[v * C + c];
+        }
+    part[r][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (r == 0 && c < C) {
+        const float tot = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                          part[3][threadIdx.x];
+        atomicAdd(db + c, tot);
+    }
+}
+
+// Loss on channel 0 of the head output Y (nvox x C): 0.5 mean (Y0 - target)^2
+// summed in FP64 (loss[0] += ...); dY (FP32, nvox x C) = (Y0 - target) / nvox
+// in channel 0 and zero elsewhere
+__global__ void k_mse(int64_t nvox, int C, const float* __restrict__ Y,
+                      const float* __restrict__ target, float* __restrict__ dY,
+                      double* __restrict__ loss) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (t < nvox) {
+        const float r = Y[t * C] - target[t];
+        l = 0.5 * (double)r * (double)r / (double)nvox;
+        dY[t * C] = r / (float)nvox;
+        for (int c = 1; c < C; ++c) dY[t * C + c] = 0.0f;
+    }
+#pragma unroll
+    for (int sh = 32; sh >= 1; sh >>= 1) l += __shfl_xor(l, sh, 64);
+    if ((threadIdx.x & 63) == 0 && l != 0.0) atomicAdd(loss, l);
+}
+
+// Network input of one window (nx x ny x nz nodes, x fastest), 16 channels:
+// 0: (T - T_amb) / T_scale, 1: (T_face - T_amb) / T_scale on the Dirichlet
+// faces (W, E, S, N, bottom) and 0 inside, 2: the laser intensity
+// exp(-3 ((x - vx)^2 + (y - vy)^2) / r^2 - 3 (z - vz)^2 / d^2), 3: P / P_scale,
+// 4..15: zero.  Windows are stacked along the batch dimension.
+__global__ void k_window_input(int nx, int ny, int nz, float x0, float y0, float z0, float h,
+                               const float* __restrict__ T, const float* __restrict__ Tface,
+                               float T_amb, float T_scale, float vx, float vy, float vz, float r,
+                               float d, float P, float P_scale, uint16_t* __restrict__ X) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nn = (int64_t)nx * ny * nz;
+    if (t >= nn) return;
+    const int ix = (int)(t % nx), iy = (int)((t / nx) % ny), iz = (int)(t / ((int64_t)nx * ny));
+    const bool face = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1 || iz == 0;
+    const float x = x0 + ix * h, y = y0 + iy * h, z = z0 + iz * h;
+    const float q = __expf(-3.0f * ((x - vx) * (x - vx) + (y - vy) * (y - vy)) / (r * r) -
+                           3.0f * (z - vz) * (z - vz) / (d * d));
+    uint16_t* o = X + t * 16;
+    o[0] = nn_f2bf((T[t] - T_amb) / T_scale);
+    o[1] = nn_f2bf(face && Tface ? (Tface[t] - T_amb) / T_scale : 0.0f);
+    o[2] = nn_f2bf(q);
+    o[3] = nn_f2bf(P / P_scale);
+#pragma unroll
+    for (int c = 4; c < 16; ++c) o[c] = 0;
+}
+
+// y += a * x (FP32)
+__global__ void k_axpy(int64_t n, float a, const float* __restrict__ x, float* __restrict__ y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] += a * x[t];
+}
+
+// T_next = T + T_scale * Y[v][0] on interior nodes; faces keep T_face
+__global__ void k_window_apply(int nx, int ny, int nz, const float* __restrict__ T,
+                               const float* __restrict__ Tface, const float* __restrict__ Y,
+                               int C, float T_scale, float T_floor, float* __restrict__ Tn) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nn = (int64_t)nx * ny * nz;
+    if (t >= nn) return;
+    const int ix = (int)(t % nx), iy = (int)((t / nx) % ny), iz = (int)(t / ((int64_t)nx * ny));
+    const bool face = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1 || iz == 0;
+    float v = face ? (Tface ? Tface[t] : T[t]) : T[t] + T_scale * Y[t * C];
+    Tn[t] = v < T_floor ? T_floor : v;
+}
+
+void launch_nn_pool2(int dir, int nb, int D, int H, int W, int C, const void* in, void* out,
+                     hipStream_t s) {
+    if (dir == 0) {
+        const int64_t n = (int64_t)nb * D * H * W * C;
+        hipLaunchKernelGGL(k_pool2, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W, C,
+                           (const uint16_t*)in, (uint16_t*)out);
+    } else {
+        const int64_t n = (int64_t)nb * 8 * D * H * W * C;
+        hipLaunchKernelGGL(k_pool2_bwd, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W, C,
+                           (const float*)in, (float*)out);
+    }
+}
+
+void launch_nn_up2(int dir, int nb, int D, int H, int W, int C, const void* in, void* out,
+                   hipStream_t s) {
+    if (dir == 0) {
+        const int64_t n = (int64_t)nb * 8 * D * H * W * C;
+        hipLaunchKernelGGL(k_up2, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W, C,
+                           (const uint16_t*)in, (uint16_t*)out);
+    } else {
+        const int64_t n = (int64_t)nb * D * H * W * C;
+        hipLaunchKernelGGL(k_up2_bwd, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W, C,
+                           (const float*)in, (float*)out);
+    }
+}
+
+void launch_nn_concat(int64_t nvox, int ca, int cb, const uint16_t* A, const uint16_t* B,
+                      uint16_t* Y, hipStream_t s) {
+    hipLaunchKernelGGL(k_concat, dim3(nn_blocks(nvox * (ca + cb))), dim3(256), 0, s, nvox, ca, cb,
+                       A, B, Y);
+}
+
+void launch_nn_split(int64_t nvox, int ca, int cb, const float* dY, float* dA, float* dB,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_split, dim3(nn_blocks(nvox * (ca + cb))), dim3(256), 0, s, nvox, ca, cb,
+                       dY, dA, dB);
+}
+
+void launch_nn_relu_bwd(int64_t n, const float* dA, const float* dA2, const float* A, float* dZ,
+                        uint16_t* dZb, hipStream_t s) {
+    hipLaunchKernelGGL(k_relu_bwd, dim3(nn_blocks(n)), dim3(256), 0, s, n, dA, dA2, A, dZ, dZb);
+}
+
+void launch_nn_bias_grad(int64_t nvox, int C, const float* dZ, float* db, hipStream_t s) {
+    const dim3 grid((unsigned)((nvox + 255) / 256), (unsigned)((C + 63) / 64));
+    hipLaunchKernelGGL(k_bias_grad, grid, dim3(256), 0, s, nvox, C, dZ, db);
+}
+
+void launch_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,
+                   double* loss, hipStream_t s) {
+    hipLaunchKernelGGL(k_mse, dim3(nn_blocks(nvox)), dim3(256), 0, s, nvox, C, Y, target, dY,
+                       loss);
+}
+
+void launch_nn_window_input(const int32_t* n, const float* geo, const float* T, const float* Tface,
+                            const float* prm, uint16_t* X, hipStream_t s) {
+    const int64_t nn = (int64_t)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(k_window_input, dim3(nn_blocks(nn)), dim3(256), 0, s, n[0], n[1], n[2],
+                       geo[0], geo[1], geo[2], geo[3], T, Tface, prm[0], prm[1], prm[2], prm[3],
+                       prm[4], prm[5], prm[6], prm[7], prm[8], X);
+}
+
+void launch_nn_axpy(int64_t n, float a, const float* x, float* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpy, dim3(nn_blocks(n)), dim3(256), 0, s, n, a, x, y);
+}
+
+void launch_nn_window_apply(const int32_t* n, const float* T, const float* Tface, const float* Y,
+                            int C, float T_scale, float T_floor, float* Tn, hipStream_t s) {
+    const int64_t nn = (int64_t)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(k_window_apply, dim3(nn_blocks(nn)), dim3(256), 0, s, n[0], n[1], n[2], T,
+                       Tface, Y, C, T_scale, T_floor, Tn);
+}
+
+}  // namespace gm

@@ -59,6 +59,8 @@ def conv3d_bwd_data(dY, Wk, want_bf16=True):
     _check(dY, torch.bfloat16, "dY")
     _check(Wk, torch.bfloat16, "Wk")
     co, ci = Wk.shape[1], Wk.shape[2]
+    if dY.shape[4] != co:
+        raise ValueError("conv3d_bwd_data: dY channels must equal the weights' cout")
     work = torch.empty_like(Wk)
     dX = torch.empty(dY.shape[:4] + (ci,), dtype=torch.float32, device=dY.device)
     dXb = torch.empty_like(dX, dtype=torch.bfloat16) if want_bf16 else None
@@ -74,6 +76,8 @@ def conv3d_bwd_weight(X, dY, dW):
     _check(dY, torch.bfloat16, "dY")
     _check(dW, torch.float32, "dW")
     co = dY.shape[4]
+    if tuple(X.shape[:4]) != tuple(dY.shape[:4]) or tuple(dW.shape) != (27, co, X.shape[4]):
+        raise ValueError("conv3d_bwd_weight: X, dY voxels and dW (27, co, ci) must agree")
     N.check(N.lib().gm_conv3d_bwd_weight(_shape(X, co), _ptr(X), _ptr(dY), _ptr(dW), _stream()))
     return dW
 
@@ -86,9 +90,390 @@ def adam(w, wbf, g, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
                             float(beta1), float(beta2), float(eps), int(step), _stream()))
 
 
+def zero(t):
+    """t[:] = 0 by the library (hipMemsetAsync on the current stream)."""
+    N.check(N.lib().gm_memset(_ptr(t), 0, t.numel() * t.element_size(), _stream()))
+
+
 def to_bf16(a):
     """bf16 copy (round to nearest even) of an FP32 tensor, by the library."""
     _check(a, torch.float32, "a")
     b = torch.empty_like(a, dtype=torch.bfloat16)
     N.check(N.lib().gm_f32_to_bf16(_ptr(a), _ptr(b), a.numel(), _stream()))
     return b
+
+
+# ---------------------------------------------------------------------------
+# Network helpers (each one kernel of the library)
+# ---------------------------------------------------------------------------
+
+def pool2(X):
+    """2x2x2 average pooling of a bf16 (nb, 2D, 2H, 2W, C) tensor."""
+    nb, D2, H2, W2, Cc = X.shape
+    if D2 % 2 or H2 % 2 or W2 % 2:
+        raise ValueError("pool2 needs even extents")
+    Y = torch.empty((nb, D2 // 2, H2 // 2, W2 // 2, Cc), dtype=torch.bfloat16, device=X.device)
+    sh = (C.c_int32 * 5)(nb, D2 // 2, H2 // 2, W2 // 2, Cc)
+    N.check(N.lib().gm_nn_pool2(0, sh, _ptr(X), _ptr(Y), _stream()))
+    return Y
+
+
+def pool2_bwd(dY):
+    nb, D, H, W, Cc = dY.shape
+    dX = torch.empty((nb, 2 * D, 2 * H, 2 * W, Cc), dtype=torch.float32, device=dY.device)
+    sh = (C.c_int32 * 5)(nb, D, H, W, Cc)
+    N.check(N.lib().gm_nn_pool2(1, sh, _ptr(dY), _ptr(dX), _stream()))
+    return dX
+
+
+def up2(X):
+    nb, D, H, W, Cc = X.shape
+    Y = torch.empty((nb, 2 * D, 2 * H, 2 * W, Cc), dtype=torch.bfloat16, device=X.device)
+    sh = (C.c_int32 * 5)(nb, D, H, W, Cc)
+    N.check(N.lib().gm_nn_up2(0, sh, _ptr(X), _ptr(Y), _stream()))
+    return Y
+
+
+def up2_bwd(dY):
+    nb, D2, H2, W2, Cc = dY.shape
+    dX = torch.empty((nb, D2 // 2, H2 // 2, W2 // 2, Cc), dtype=torch.float32, device=dY.device)
+    sh = (C.c_int32 * 5)(nb, D2 // 2, H2 // 2, W2 // 2, Cc)
+    N.check(N.lib().gm_nn_up2(1, sh, _ptr(dY), _ptr(dX), _stream()))
+    return dX
+
+
+def concat(A, B):
+    if tuple(A.shape[:4]) != tuple(B.shape[:4]):
+        raise ValueError(f"concat: voxel shapes {tuple(A.shape)} and {tuple(B.shape)} differ")
+    nv = A.numel() // A.shape[-1]
+    Y = torch.empty(A.shape[:4] + (A.shape[4] + B.shape[4],), dtype=torch.bfloat16,
+                    device=A.device)
+    N.check(N.lib().gm_nn_concat(nv, A.shape[4], B.shape[4], _ptr(A), _ptr(B), _ptr(Y),
+                                 _stream()))
+    return Y
+
+
+def split(dY, ca):
+    cb = dY.shape[4] - ca
+    nv = dY.numel() // dY.shape[-1]
+    dA = torch.empty(dY.shape[:4] + (ca,), dtype=torch.float32, device=dY.device)
+    dB = torch.empty(dY.shape[:4] + (cb,), dtype=torch.float32, device=dY.device)
+    N.check(N.lib().gm_nn_split(nv, ca, cb, _ptr(dY), _ptr(dA), _ptr(dB), _stream()))
+    return dA, dB
+
+
+def _same(a, b, what):
+    if b is not None and tuple(a.shape) != tuple(b.shape):
+        raise ValueError(f"{what}: shapes {tuple(a.shape)} and {tuple(b.shape)} differ")
+
+
+def relu_bwd(dA, A, dA2=None):
+    _same(dA, A, "relu_bwd")
+    _same(dA, dA2, "relu_bwd")
+    dZ = torch.empty_like(dA)
+    dZb = torch.empty_like(dA, dtype=torch.bfloat16)
+    N.check(N.lib().gm_nn_relu_bwd(dA.numel(), _ptr(dA), _ptr(dA2), _ptr(A), _ptr(dZ), _ptr(dZb),
+                                   _stream()))
+    return dZ, dZb
+
+
+def bias_grad(dZ, db):
+    if db.numel() != dZ.shape[-1]:
+        raise ValueError("bias_grad: one bias per channel")
+    N.check(N.lib().gm_nn_bias_grad(dZ.numel() // dZ.shape[-1], dZ.shape[-1], _ptr(dZ), _ptr(db),
+                                    _stream()))
+
+
+def mse(Y, target, loss):
+    """loss (FP64 device scalar) += 0.5 mean (Y[..., 0] - target)^2; returns
+    the FP32 gradient wrt Y."""
+    if target.numel() != Y.numel() // Y.shape[-1] or loss.numel() < 1:
+        raise ValueError("mse: target needs one value per voxel of Y")
+    dY = torch.empty_like(Y)
+    N.check(N.lib().gm_nn_mse(Y.numel() // Y.shape[-1], Y.shape[-1], _ptr(Y), _ptr(target),
+                              _ptr(dY), _ptr(loss), _stream()))
+    return dY
+
+
+# ---------------------------------------------------------------------------
+# The operator: a 3-D U-Net of 3x3x3 convolutions
+# ---------------------------------------------------------------------------
+
+class Conv:
+    """One 3x3x3 convolution layer: FP32 master weights, their bf16 copy,
+    FP32 gradients and Adam moments (all device tensors)."""
+
+    def __init__(self, cin, cout, relu, gen):
+        std = (2.0 / (27 * cin)) ** 0.5 if relu else (1.0 / (27 * cin)) ** 0.5
+        w = torch.randn((27, cout, cin), generator=gen, dtype=torch.float32) * std
+        self.w = w.cuda()
+        self.b = _zeros(cout)
+        self.wb = to_bf16(self.w)
+        self.relu = relu
+        self.params = [self.w, self.b]
+        self.grads = [_zeros(self.w.shape), _zeros(self.b.shape)]
+        self.moments = [(_zeros(p.shape), _zeros(p.shape)) for p in self.params]
+
+    def forward(self, Xb):
+        self.Xb = Xb
+        self.A, Ab = conv3d(Xb, self.wb, self.b, relu=self.relu)
+        return Ab
+
+    def backward(self, dA, dA2=None, need_input_grad=True):
+        """dA: FP32 gradient wrt this layer's output; returns the FP32
+        gradient wrt its input (or None)."""
+        dZ, dZb = relu_bwd(dA, self.A if self.relu else None, dA2)
+        bias_grad(dZ, self.grads[1])
+        conv3d_bwd_weight(self.Xb, dZb, self.grads[0])
+        if not need_input_grad:
+            return None
+        dX, _ = conv3d_bwd_data(dZb, self.wb, want_bf16=False)
+        return dX
+
+    def step(self, t, lr):
+        for p, g, (m, v), bf in zip(self.params, self.grads, self.moments, (self.wb, None)):
+            adam(p, bf, g, m, v, t, lr)
+        for g in self.grads:
+            zero(g)
+
+
+class UNet:
+    """3-D U-Net operator: encoder levels of two convolutions (ReLU) with
+    2x2x2 average pooling between them, decoder levels of nearest 2x
+    upsampling, a skip concatenation and two convolutions, and a linear
+    head whose channel 0 is the predicted (T_next - T) / T_scale.  Input: 16
+    channels from gm_nn_window_input."""
+
+    def __init__(self, channels=(32, 64, 128, 128), cin=16, cout=16, seed=0):
+        gen = torch.Generator().manual_seed(seed)
+        self.ch = list(channels)
+        self.enc = []
+        c = cin
+        for ch in self.ch:
+            self.enc.append((Conv(c, ch, True, gen), Conv(ch, ch, True, gen)))
+            c = ch
+        self.dec = []
+        for l in range(len(self.ch) - 2, -1, -1):
+            self.dec.append((Conv(self.ch[l] + self.ch[l + 1], self.ch[l], True, gen),
+                             Conv(self.ch[l], self.ch[l], True, gen)))
+        self.head = Conv(self.ch[0], cout, False, gen)
+        self.t = 0
+
+    def layers(self):
+        out = [c for pair in self.enc for c in pair] + [c for pair in self.dec for c in pair]
+        return out + [self.head]
+
+    def forward(self, Xb):
+        x = Xb
+        self.skips = []
+        for l, (c1, c2) in enumerate(self.enc):
+            x = c2.forward(c1.forward(x))
+            self.skips.append(x)
+            if l < len(self.enc) - 1:
+                x = pool2(x)
+        for k, (c1, c2) in enumerate(self.dec):
+            l = len(self.ch) - 2 - k
+            x = concat(self.skips[l], up2(x))
+            x = c2.forward(c1.forward(x))
+        self.head.forward(x)
+        return self.head.A
+
+    def backward(self, dY):
+        """dY: FP32 gradient wrt the head output (from mse)."""
+        g = self.head.backward(dY)
+        skip_grads = [None] * len(self.ch)
+        # decoder levels in the reverse of the forward order (finest first)
+        for k in reversed(range(len(self.dec))):
+            c1, c2 = self.dec[k]
+            l = len(self.ch) - 2 - k
+            g = c1.backward(c2.backward(g))
+            ds, du = split(g, self.ch[l])
+            skip_grads[l] = ds
+            g = up2_bwd(du)
+        for l in range(len(self.ch) - 1, -1, -1):
+            c1, c2 = self.enc[l]
+            g2 = c2.backward(g, skip_grads[l]) if l < len(self.ch) - 1 else c2.backward(g)
+            g = c1.backward(g2, need_input_grad=(l > 0))
+            if l > 0:
+                g = pool2_bwd(g)
+
+    def step(self, lr=1e-3):
+        self.t += 1
+        for c in self.layers():
+            c.step(self.t, lr)
+
+    def train_step(self, Xb, target, lr=1e-3):
+        """One Adam step on a batch; returns the loss (FP64 device scalar)."""
+        Y = self.forward(Xb)
+        loss = _zeros(1, torch.float64)
+        dY = mse(Y, target, loss)
+        self.backward(dY)
+        self.step(lr)
+        return loss
+
+    def flops_per_voxel_fwd(self, D, H, W):
+        """Multiply-adds x 2 of the convolutions for one input window."""
+        tot = 0
+        shapes = [(D, H, W)]
+        for _ in range(len(self.ch) - 1):
+            d, h, w = shapes[-1]
+            shapes.append((d // 2, h // 2, w // 2))
+        for l, (c1, c2) in enumerate(self.enc):
+            n = shapes[l][0] * shapes[l][1] * shapes[l][2]
+            tot += n * 27 * (c1.w.shape[1] * c1.w.shape[2] + c2.w.shape[1] * c2.w.shape[2])
+        for k, (c1, c2) in enumerate(self.dec):
+            l = len(self.ch) - 2 - k
+            n = shapes[l][0] * shapes[l][1] * shapes[l][2]
+            tot += n * 27 * (c1.w.shape[1] * c1.w.shape[2] + c2.w.shape[1] * c2.w.shape[2])
+        n = D * H * W
+        tot += n * 27 * self.head.w.shape[1] * self.head.w.shape[2]
+        return 2 * tot
+
+
+# ---------------------------------------------------------------------------
+# Training windows from the plain solver (batched L3 instances)
+# ---------------------------------------------------------------------------
+
+T_SCALE = 1000.0
+P_SCALE = 400.0
+
+
+def _copy_field(dev_array):
+    """Device copy (library memcpy) of a DeviceArray view as an FP32 tensor."""
+    t = torch.empty(dev_array.n, dtype=torch.float32, device="cuda")
+    N.check(N.lib().gm_memcpy_d2d(_ptr(t), C.c_void_p(dev_array.ptr), dev_array.nbytes,
+                                  _stream()))
+    return t
+
+
+def window_input(inst, grid, T, Tface, row, out):
+    """Fill out (bf16, (D, H, W, 16) view of one window) from the fields."""
+    n = (C.c_int32 * 3)(grid.n[0], grid.n[1], grid.n[2])
+    geo = (C.c_float * 4)(grid.x0[0], grid.x0[1], grid.x0[2], grid.h[0])
+    p = inst["properties"]
+    prm = (C.c_float * 9)(p.get("T_amb", 300.0), T_SCALE, float(row[0]), float(row[1]),
+                          float(row[2]), p["laser_radius"], p.get("laser_depth", 0.05),
+                          float(row[6]), P_SCALE)
+    N.check(N.lib().gm_nn_window_input(n, geo, _ptr(T), _ptr(Tface), prm, _ptr(out), _stream()))
+
+
+def make_windows(instances, K=10, warm=20, n_snap=4, gap=10):
+    """Run the plain batched L3 step on the instances and cut training pairs:
+    input (bf16, (n, D, H, W, 16)) at a step t and target (T(t+K) - T(t)) /
+    T_SCALE (FP32, (n, D*H*W)), n = len(instances) * n_snap.  Also returns the
+    fields T(t), T(t+K), the face values and the laser rows, for checks."""
+    import workloads as W
+    import batch as B
+
+    class _TorchStream:
+        """The batch runs on torch's current stream, so the copies below are
+        ordered after the steps without host synchronisation."""
+        ptr = torch.cuda.current_stream().cuda_stream
+
+        @staticmethod
+        def sync():
+            torch.cuda.current_stream().synchronize()
+
+    bt = B.Batch.from_instances(instances, stream=_TorchStream())
+    g0 = bt.grids[0]
+    D, H, Wd = g0.n[2], g0.n[1], g0.n[0]
+    for b_ in range(len(instances)):
+        g = bt.grids[b_]
+        if (g.n[0], g.n[1], g.n[2]) != (Wd, H, D):
+            raise ValueError("make_windows needs instances of one mesh size")
+    nw = len(instances) * n_snap
+    X = torch.empty((nw, D, H, Wd, 16), dtype=torch.bfloat16, device="cuda")
+    Y = torch.empty((nw, D * H * Wd), dtype=torch.float32, device="cuda")
+    keep = []
+    s = 0
+    for _ in range(warm):
+        bt.step(W.batch_rows(instances, s))
+        s += 1
+    w = 0
+    for _ in range(n_snap):
+        rows = W.batch_rows(instances, s)
+        T0 = [_copy_field(bt._field(i, 0)) for i in range(len(instances))]
+        faces = [_copy_field(bt._field(i, 2)) for i in range(len(instances))]
+        for _k in range(K):
+            bt.step(W.batch_rows(instances, s))
+            s += 1
+        for i in range(len(instances)):
+            T1 = _copy_field(bt._field(i, 0))
+            window_input(instances[i], bt.grids[i], T0[i], faces[i], rows[i], X[w])
+            zero(Y[w])
+            N.check(N.lib().gm_nn_axpy(Y[w].numel(), 1.0 / T_SCALE, _ptr(T1), _ptr(Y[w]),
+                                       _stream()))
+            N.check(N.lib().gm_nn_axpy(Y[w].numel(), -1.0 / T_SCALE, _ptr(T0[i]), _ptr(Y[w]),
+                                       _stream()))
+            keep.append((T0[i], T1, faces[i], rows[i].copy(), i))
+            w += 1
+        for _g in range(gap):
+            bt.step(W.batch_rows(instances, s))
+            s += 1
+    torch.cuda.synchronize()
+    return X, Y, keep, bt
+
+
+def apply_jump(net, X, T, Tface, grid, T_floor=300.0):
+    """T after one jump of the operator from one window's input X (bf16,
+    (1, D, H, W, 16)): T + T_SCALE * prediction inside, faces held."""
+    Yh = net.forward(X)
+    Tn = torch.empty_like(T)
+    n = (C.c_int32 * 3)(grid.n[0], grid.n[1], grid.n[2])
+    N.check(N.lib().gm_nn_window_apply(n, _ptr(T), _ptr(Tface), _ptr(Yh), Yh.shape[-1],
+                                       T_SCALE, float(T_floor), _ptr(Tn), _stream()))
+    return Tn
+
+
+def _zeros(shape, dtype=torch.float32):
+    t = torch.empty(shape, dtype=dtype, device="cuda")
+    zero(t)
+    return t
+
+
+def _gather(src, idx, out):
+    """out[k] = src[idx[k]] by library copies (rows of a contiguous tensor)."""
+    row = src[0].numel() * src.element_size()
+    for k, j in enumerate(idx):
+        N.check(N.lib().gm_memcpy_d2d(C.c_void_p(out.data_ptr() + k * row),
+                                      C.c_void_p(src.data_ptr() + int(j) * row), row, _stream()))
+    return out
+
+
+def rel_error(net, X, Y, batch=4):
+    """||prediction - target|| / ||target|| over the windows (the zero
+    predictor, i.e. T(t+K) = T(t), scores 1)."""
+    num = den = 0.0
+    for b0 in range(0, X.shape[0], batch):
+        xb = X[b0:b0 + batch]
+        yb = Y[b0:b0 + batch].reshape(-1)
+        pred = net.forward(xb)
+        loss = _zeros(1, torch.float64)
+        mse(pred, yb, loss)
+        ref = _zeros(1, torch.float64)
+        mse(_zeros(pred.shape), yb, ref)
+        num += float(loss.item())
+        den += float(ref.item())
+    return (num / den) ** 0.5 if den > 0 else 0.0
+
+
+def train(net, X, Y, iters=100, batch=4, lr=1e-3, seed=0, log=None):
+    """Adam on mini-batches drawn without replacement (reshuffled each pass);
+    returns the loss history."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = X.shape[0]
+    xb = torch.empty((batch,) + tuple(X.shape[1:]), dtype=X.dtype, device="cuda")
+    yb = torch.empty((batch,) + tuple(Y.shape[1:]), dtype=Y.dtype, device="cuda")
+    order = []
+    hist = []
+    for it in range(iters):
+        if len(order) < batch:
+            order = list(rng.permutation(n)) + order
+        idx = [order.pop() for _ in range(batch)]
+        _gather(X, idx, xb)
+        _gather(Y, idx, yb)
+        hist.append(net.train_step(xb, yb.reshape(-1), lr))
+        if log and (it % 20 == 0 or it == iters - 1):
+            log(f"iter {it} loss {float(hist[-1].item()):.4e}")
+    return [float(h.item()) for h in hist]

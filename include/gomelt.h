@@ -372,6 +372,47 @@ int gm_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, int64_t
 /* bf16 (round to nearest even) copy of FP32 values. */
 int gm_f32_to_bf16(const float* a, uint16_t* b, int64_t n, void* stream);
 
+/* Elementwise and reduction kernels of the operator's network (no reference
+ * counterpart).  Channels-last tensors; activations bf16, gradients FP32.
+ * coarse_shape = {nb, D, H, W, C} of the coarse grid (the fine one is
+ * 2D x 2H x 2W).  dir 0: forward, 1: backward. */
+/* 2x2x2 average pooling: forward bf16 fine -> bf16 coarse; backward FP32
+ * coarse gradient -> FP32 fine gradient (each fine voxel gets 1/8). */
+int gm_nn_pool2(int dir, const int32_t* coarse_shape_host, const void* in, void* out,
+                void* stream);
+/* Nearest 2x upsampling: forward bf16 coarse -> bf16 fine; backward FP32 fine
+ * gradient -> FP32 coarse gradient (sum of the 8). */
+int gm_nn_up2(int dir, const int32_t* coarse_shape_host, const void* in, void* out,
+              void* stream);
+/* Y = [A | B] along channels (bf16); the backward split of an FP32 gradient. */
+int gm_nn_concat(int64_t nvox, int ca, int cb, const uint16_t* A, const uint16_t* B,
+                 uint16_t* Y, void* stream);
+int gm_nn_split(int64_t nvox, int ca, int cb, const float* dY, float* dA, float* dB,
+                void* stream);
+/* dZ = (dA + dA2) where A > 0, else 0 (dA2, A nullable); FP32 and a bf16 copy
+ * (dZb nullable). */
+int gm_nn_relu_bwd(int64_t n, const float* dA, const float* dA2, const float* A, float* dZ,
+                   uint16_t* dZb, void* stream);
+/* db[c] += sum over voxels of dZ[v][c]. */
+int gm_nn_bias_grad(int64_t nvox, int C, const float* dZ, float* db, void* stream);
+/* loss[0] += 0.5 mean_v (Y[v][0] - target[v])^2 (FP64); dY (FP32, nvox x C)
+ * = (Y[v][0] - target[v]) / nvox in channel 0, zero elsewhere. */
+int gm_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,
+              double* loss, void* stream);
+/* Network input of one window of n = {nx, ny, nz} nodes (x fastest) at
+ * geo = {x0, y0, z0, h}: 16 bf16 channels, 0: (T - T_amb)/T_scale, 1: the
+ * Dirichlet face values (W, E, S, N, bottom) likewise, 0 inside, 2: the laser
+ * intensity exp(-3((x-vx)^2 + (y-vy)^2)/r^2 - 3(z-vz)^2/d^2), 3: P/P_scale,
+ * 4..15: 0.  prm = {T_amb, T_scale, vx, vy, vz, r, d, P, P_scale}. */
+int gm_nn_window_input(const int32_t* n_host, const float* geo_host, const float* T,
+                       const float* Tface, const float* prm_host, uint16_t* X, void* stream);
+/* Tn = max(T_floor, T + T_scale * Y[v][0]) inside; faces take Tface (or T). */
+int gm_nn_window_apply(const int32_t* n_host, const float* T, const float* Tface,
+                       const float* Y, int C, float T_scale, float T_floor, float* Tn,
+                       void* stream);
+/* y += a x (FP32). */
+int gm_nn_axpy(int64_t n, float a, const float* x, float* y, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

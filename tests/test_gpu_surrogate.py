@@ -1,0 +1,133 @@
+"""The learned L3 operator's network kernels against NumPy, and a short
+training run on windows of the plain solver (the operator must beat the
+persistence predictor T(t+K) = T(t) on held-out windows)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    import torch
+    import _native as N
+    N.init_device()
+    import surrogate as S
+    return S
+
+
+def _bf(a):
+    import torch
+    from oracle import conv_oracle as CO
+    bits = CO.bf16_bits(np.ascontiguousarray(a, np.float32))
+    return torch.from_numpy(bits.view(np.int16)).view(torch.bfloat16).cuda()
+
+
+def _np(t):
+    import torch
+    if t.dtype == torch.bfloat16:
+        return (t.view(torch.int16).cpu().numpy().astype(np.uint16).astype(np.uint32) << 16).view(np.float32)
+    return t.cpu().numpy()
+
+
+def test_network_elementwise_kernels(S):
+    """pool/unpool, upsample, concat/split, ReLU backward, bias gradient and
+    the loss against NumPy (exact or FP32 rounding)."""
+    import torch
+    from oracle import conv_oracle as CO
+    rng = np.random.default_rng(0)
+    X = CO.bf16_round(rng.standard_normal((2, 4, 6, 8, 16)).astype(np.float32))
+    P = S.pool2(_bf(X))
+    ref = X.reshape(2, 2, 2, 3, 2, 4, 2, 16).mean(axis=(2, 4, 6))
+    np.testing.assert_allclose(_np(P), CO.bf16_round(ref), rtol=0, atol=0)
+    g = rng.standard_normal((2, 2, 3, 4, 16)).astype(np.float32)
+    dX = _np(S.pool2_bwd(torch.from_numpy(g).cuda()))
+    np.testing.assert_array_equal(dX, np.repeat(np.repeat(np.repeat(g, 2, 1), 2, 2), 2, 3) / 8)
+    U = _np(S.up2(_bf(ref)))
+    np.testing.assert_array_equal(U, np.repeat(np.repeat(np.repeat(CO.bf16_round(ref), 2, 1), 2, 2), 2, 3))
+    gu = rng.standard_normal((2, 4, 6, 8, 16)).astype(np.float32)
+    du = _np(S.up2_bwd(torch.from_numpy(gu).cuda()))
+    np.testing.assert_allclose(du, gu.reshape(2, 2, 2, 3, 2, 4, 2, 16).sum(axis=(2, 4, 6)),
+                               rtol=1e-6, atol=1e-6)
+    B = CO.bf16_round(rng.standard_normal((2, 4, 6, 8, 32)).astype(np.float32))
+    Cc = _np(S.concat(_bf(X), _bf(B)))
+    np.testing.assert_array_equal(Cc, np.concatenate([X, B], axis=-1))
+    gc = rng.standard_normal((2, 4, 6, 8, 48)).astype(np.float32)
+    da, db = S.split(torch.from_numpy(gc).cuda(), 16)
+    np.testing.assert_array_equal(_np(da), gc[..., :16])
+    np.testing.assert_array_equal(_np(db), gc[..., 16:])
+    A = np.maximum(rng.standard_normal((5000, 16)).astype(np.float32), 0)
+    dA = rng.standard_normal((5000, 16)).astype(np.float32)
+    dA2 = rng.standard_normal((5000, 16)).astype(np.float32)
+    dZ, dZb = S.relu_bwd(torch.from_numpy(dA).cuda(), torch.from_numpy(A).cuda(),
+                         torch.from_numpy(dA2).cuda())
+    np.testing.assert_array_equal(_np(dZ), np.where(A > 0, dA + dA2, 0.0).astype(np.float32))
+    np.testing.assert_array_equal(_np(dZb), CO.bf16_round(_np(dZ)))
+    dbias = torch.from_numpy(np.ones(16, np.float32)).cuda()
+    S.bias_grad(dZ, dbias)
+    np.testing.assert_allclose(_np(dbias), 1 + _np(dZ).astype(np.float64).sum(0), rtol=1e-5)
+    Y = rng.standard_normal((3000, 16)).astype(np.float32)
+    T = rng.standard_normal(3000).astype(np.float32)
+    loss = torch.from_numpy(np.zeros(1)).cuda()
+    dY = _np(S.mse(torch.from_numpy(Y).cuda(), torch.from_numpy(T).cuda(), loss))
+    r = Y[:, 0].astype(np.float64) - T
+    # FP32 residuals, FP64 sum
+    assert float(loss.item()) == pytest.approx(0.5 * np.mean(r * r), rel=1e-6)
+    np.testing.assert_allclose(dY[:, 0], r / 3000, rtol=1e-6)
+    assert np.all(dY[:, 1:] == 0)
+
+
+def test_unet_gradient_direction(S):
+    """Directional derivative of the loss through the whole U-Net: the
+    kernels' gradient g predicts L(w + e d) - L(w - e d) ~ 2 e <g, d> for a
+    random weight direction d (BF16 weights and activations: 10%)."""
+    import torch
+    from oracle import conv_oracle as CO
+    rng = np.random.default_rng(3)
+    net = S.UNet(channels=(16, 32), seed=1)
+    X = _bf(0.5 * rng.standard_normal((2, 8, 8, 8, 16)).astype(np.float32))
+    T = torch.from_numpy(0.1 * rng.standard_normal(2 * 8 * 8 * 8).astype(np.float32)).cuda()
+
+    def loss():
+        out = net.forward(X)
+        l = S._zeros(1, torch.float64)
+        S.mse(out, T, l)
+        return float(l.item())
+
+    out = net.forward(X)
+    l0 = S._zeros(1, torch.float64)
+    dY = S.mse(out, T, l0)
+    net.backward(dY)
+    layers = net.layers()
+    dirs = [rng.standard_normal(tuple(c.w.shape)).astype(np.float32) for c in layers]
+    w0 = [_np(c.w).copy() for c in layers]
+    eps = 1e-1
+    vals = []
+    for sgn in (1, -1):
+        for c, w, d in zip(layers, w0, dirs):
+            c.w.copy_(torch.from_numpy(w + sgn * eps * d * np.abs(w).mean()))
+            c.wb = S.to_bf16(c.w)
+        vals.append(loss())
+    scale = [np.abs(w).mean() for w in w0]
+    gd = sum(float((_np(c.grads[0]).astype(np.float64) * d * s).sum())
+             for c, d, s in zip(layers, dirs, scale))
+    fd = (vals[0] - vals[1]) / (2 * eps)
+    assert fd == pytest.approx(gd, rel=1e-1), (fd, gd)
+
+
+def test_operator_training_beats_persistence(S):
+    """Windows of the plain batched solver (32 x 32 x 16 nodes, K = 10
+    steps); a small U-Net trained for 150 steps predicts T(t+K) better than
+    T(t) on windows of instances it did not see."""
+    import workloads as W
+    inst_tr = W.batch_instances(n=8, seed=11, sizes=[(31, 31, 15)])
+    inst_te = W.batch_instances(n=4, seed=12, sizes=[(31, 31, 15)])
+    Xtr, Ytr, _, _ = S.make_windows(inst_tr, K=10, warm=10, n_snap=3, gap=5)
+    Xte, Yte, _, _ = S.make_windows(inst_te, K=10, warm=10, n_snap=2, gap=5)
+    net = S.UNet(channels=(16, 32, 32), seed=0)
+    e0 = S.rel_error(net, Xte, Yte)
+    hist = S.train(net, Xtr, Ytr, iters=150, batch=4, lr=2e-3)
+    e1 = S.rel_error(net, Xte, Yte)
+    print("held-out relative error: untrained", e0, "trained", e1, "loss", hist[0], "->", hist[-1])
+    assert hist[-1] < 0.5 * hist[0]
+    assert e1 < 0.7
