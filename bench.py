@@ -48,7 +48,79 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-instances measurement")
     ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the learned-operator measurement")
+    ap.add_argument("--surrogate-iters", type=int, default=60)
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="print only the learned-operator measurement (one JSON line)")
     return ap.parse_args()
+
+
+def surrogate_measure(iters):
+    """Learned L3 operator (3-D U-Net, channels 32-64-128-128, BF16 MFMA
+    convolutions): trained for `iters` Adam steps (batch 4) on windows of
+    64 x 64 x 32 nodes cut from the plain batched solver (K = 10 steps per
+    jump), then one jump on held-out windows against 10 plain steps."""
+    import numpy as np
+    import torch
+    import workloads as W
+    import surrogate as S
+    import computeFunctions as cf
+    tr = W.batch_instances(n=16, seed=21, sizes=[(63, 63, 31)])
+    te = W.batch_instances(n=4, seed=22, sizes=[(63, 63, 31)])
+    Xtr, Ytr, _, _ = S.make_windows(tr, K=10, warm=20, n_snap=3, gap=10)
+    Xte, Yte, keep, bt = S.make_windows(te, K=10, warm=20, n_snap=1, gap=0)
+    net = S.UNet(channels=(32, 64, 128, 128), seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = S.train(net, Xtr, Ytr, iters=iters, batch=4, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    D, H, Wd = Xtr.shape[1:4]
+    fl_fwd = net.flops_per_voxel_fwd(D, H, Wd)
+    tflops = 3.0 * fl_fwd * 4 * iters / t_train / 1e12
+    err = S.rel_error(net, Xte, Yte)
+    # one jump against the plain solver's 10 steps, held-out windows
+    g = bt.grids[0]
+    lv = {"node_coords": g.axes(), "nodes": [g.n[0], g.n[1], g.n[2]], "h": list(g.h)}
+    Tl = te[0]["properties"]["T_liquidus"]
+    derr, werr = [], []
+    for w, (T0, T1, faces, row, i) in enumerate(keep):
+        Tn = S.apply_jump(net, Xte[w:w + 1], T0, faces, g)
+        dp = cf.meltPoolDimensions(lv, T1, Tl)
+        ds = cf.meltPoolDimensions(lv, Tn, Tl)
+        if dp[0] > 0 and dp[1] > 0:
+            derr.append(abs(ds[0] - dp[0]) / dp[0])
+            werr.append(abs(ds[1] - dp[1]) / dp[1])
+    x1 = Xte[:1].contiguous()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    net.forward(x1)
+    e0.record()
+    for _ in range(10):
+        net.forward(x1)
+    e1.record()
+    torch.cuda.synchronize()
+    jump_ms = e0.elapsed_time(e1) / 10
+    import batch as B
+    one = B.Batch.from_instances(te[:1])
+    plain_ms = 10 * one.time(batched=True, iters=50)
+    return {
+        "workload": "3-D U-Net 32-64-128-128 on 64x64x32-node L3 windows, K=10 steps per jump; "
+                    f"{Xtr.shape[0]} training windows from 16 plain batched instances, "
+                    f"{iters} Adam steps of batch 4; {len(keep)} held-out windows",
+        "train_s": t_train,
+        "train_windows_per_s": 4 * iters / t_train,
+        "loss_first_last": [hist[0], hist[-1]],
+        "heldout_rel_err_dT": err,
+        "heldout_rel_err_zero_predictor": 1.0,
+        "melt_pool_rel_err_depth": float(np.mean(derr)) if derr else None,
+        "melt_pool_rel_err_width": float(np.mean(werr)) if werr else None,
+        "jump_ms_one_window": jump_ms,
+        "plain_10_steps_ms_one_window": plain_ms,
+        "roofline": {"kernel": "k_conv_fwd / k_conv_wgrad (training step)", "bound": "mfma",
+                     "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": tflops / 2500.0, "flops_per_window_fwd": fl_fwd},
+    }
 
 
 def batch_measure(n, iters=50):
@@ -109,8 +181,12 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
-              "(instances shard with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(independent instances would shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
+    if a.surrogate_only:
+        import torch  # noqa: F401  (its HIP runtime first, before libgomelt.so)
+        print(json.dumps(surrogate_measure(a.surrogate_iters)))
+        return
     import numpy as np
     import workloads as W
     import go_melt as G
@@ -213,6 +289,20 @@ def main():
             result["cpu_baseline"] = cpu_baseline(inp, gcode, N2 * N3, mm / max(a.steps, 1))
         except Exception as exc:  # the baseline must never hide the GPU line
             result["cpu_baseline"] = {"error": repr(exc)}
+    if not a.no_surrogate:
+        # in a child process: PyTorch (which holds the operator's tensors)
+        # brings its own HIP runtime and must be loaded before this library;
+        # the plain path above runs on the system runtime
+        import subprocess
+        cmd = [sys.executable, os.path.abspath(__file__), "--surrogate-only",
+               "--surrogate-iters", str(a.surrogate_iters)]
+        try:
+            out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+            lines = [ln for ln in out.stdout.splitlines() if ln.startswith("{")]
+            result["surrogate"] = json.loads(lines[-1]) if lines else {
+                "error": f"exit {out.returncode}: {out.stderr[-400:]}"}
+        except Exception as exc:  # reported, not fatal to the headline line
+            result["surrogate"] = {"error": repr(exc)}
     if not a.no_batch:
         try:
             result["batch"] = batch_measure(a.batch_n)

@@ -264,9 +264,15 @@ class DeviceArray:
 
 
 def as_device(x, dtype=np.float32):
-    """DeviceArray for x (uploaded if it is host data); keeps a DeviceArray."""
+    """DeviceArray for x (uploaded if it is host data); keeps a DeviceArray;
+    a GPU tensor (PyTorch) of the same element type is viewed, not copied."""
     if isinstance(x, DeviceArray):
         return x
+    if getattr(x, "is_cuda", False) and hasattr(x, "data_ptr"):
+        if x.element_size() != np.dtype(dtype).itemsize or not x.is_contiguous():
+            raise GoMeltNativeError("a GPU tensor argument must be contiguous and of the "
+                                    f"expected element type ({np.dtype(dtype)})")
+        return DeviceArray(x.numel(), dtype, ptr=x.data_ptr(), owner=False)
     return DeviceArray.from_numpy(np.asarray(x), dtype=dtype)
 
 

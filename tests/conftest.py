@@ -10,6 +10,17 @@ for p in (ROOT, PKG):
         sys.path.insert(0, p)
 
 
+# PyTorch ships its own HIP runtime under the same soname as the system one
+# libgomelt.so links, and the first one loaded serves the whole process;
+# PyTorch does not start on the system one.  The tests of the learned
+# operator share the process with the other GPU tests, so PyTorch is loaded
+# first (a CPU-only PyTorch loads no HIP runtime).
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and libgomelt.so")
     config.addinivalue_line("markers", "slow: long-running (skipped unless selected)")
