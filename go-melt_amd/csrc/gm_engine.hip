@@ -1937,6 +1937,8 @@ extern "C" int gm_conv3d_fwd(const int32_t* shape, const uint16_t* X, const uint
     if (!conv_shape_ok(shape))
         return fail("gm_conv3d_fwd: shape {nb, D, H, W, cin, cout} needs positive sizes and "
                     "channel counts that are multiples of 16");
+    if (((uintptr_t)X | (uintptr_t)Wk) & 7)
+        return fail("gm_conv3d_fwd: X and Wk must be 8-byte aligned (4 bf16 per load)");
     launch_conv_fwd(shape, X, Wk, bias, relu, Y, Ybf, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return 0;
@@ -1946,6 +1948,8 @@ extern "C" int gm_conv3d_bwd_data(const int32_t* shape, const uint16_t* dY, cons
                                   uint16_t* Wt_work, float* dX, uint16_t* dXbf, void* stream) {
     if (!shape || !dY || !Wk || !Wt_work || !dX) return fail("gm_conv3d_bwd_data: null argument");
     if (!conv_shape_ok(shape)) return fail("gm_conv3d_bwd_data: bad shape");
+    if (((uintptr_t)dY | (uintptr_t)Wk | (uintptr_t)Wt_work) & 7)
+        return fail("gm_conv3d_bwd_data: dY, Wk and Wt_work must be 8-byte aligned");
     const int32_t sb[6] = {shape[0], shape[1], shape[2], shape[3], shape[5], shape[4]};
     launch_conv_wflip(shape[4], shape[5], Wk, Wt_work, (hipStream_t)stream);
     launch_conv_fwd(sb, dY, Wt_work, nullptr, 0, dX, dXbf, (hipStream_t)stream);
@@ -1957,6 +1961,8 @@ extern "C" int gm_conv3d_bwd_weight(const int32_t* shape, const uint16_t* X, con
                                     float* dW, void* stream) {
     if (!shape || !X || !dY || !dW) return fail("gm_conv3d_bwd_weight: null argument");
     if (!conv_shape_ok(shape)) return fail("gm_conv3d_bwd_weight: bad shape");
+    if (((uintptr_t)X | (uintptr_t)dY) & 7)
+        return fail("gm_conv3d_bwd_weight: X and dY must be 8-byte aligned");
     const int64_t nvox = (int64_t)shape[0] * shape[1] * shape[2] * shape[3];
     // about 1024 voxels per block, at most 2048 parts (grid z = 27 * parts)
     int64_t parts = (nvox + 1023) / 1024;

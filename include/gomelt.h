@@ -349,6 +349,8 @@ int gm_batch_refresh(gm_batch* b);
  * multiples of 16.  Channels last: X[((b*D + z)*H + y)*W + x][cin];
  * weights Wk[tap][cout][cin], tap = (dz+1)*9 + (dy+1)*3 + (dx+1).
  *
+ * The bf16 arrays are read 4 elements at a time: 8-byte aligned pointers.
+ *
  * Forward: Y[v][o] = bias[o] + sum_{tap,i} X[v + s(tap)][i] Wk[tap][o][i],
  * then max(0, .) if relu; Y FP32, Ybf (nullable) its bf16 copy. */
 int gm_conv3d_fwd(const int32_t* shape_host, const uint16_t* X, const uint16_t* Wk,
