@@ -14,8 +14,9 @@
 // a second kernel that reduces over voxels (split over blocks, FP32 atomics).
 //
 // Tiles: a block of 4 waves computes 64 x 64 outputs; each wave a 32 x 32
-// quarter as 2 x 2 MFMA tiles of 16 x 16; K advances 16 at a time (one tap,
-// 16 input channels) through LDS.
+// quarter as 2 x 2 MFMA tiles of 16 x 16; K advances through LDS one tap and
+// 32 input channels at a time (k_conv_fwd32, software-pipelined) or, for 16
+// input channels, 16 at a time (k_conv_fwd).
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -40,17 +41,35 @@ struct ConvDims {
     int64_t nvox;     // nb * D * H * W
 };
 
+// (x, y, z) of a voxel, decoded once per thread (32-bit divisions while the
+// volume has fewer than 2^32 voxels)
+struct Vox {
+    int64_t v;
+    int x, y, z;
+};
+__device__ __forceinline__ Vox vox_decode(const ConvDims& c, int64_t v) {
+    Vox r;
+    r.v = v;
+    if (c.nvox < ((int64_t)1 << 32)) {
+        const uint32_t u = (uint32_t)v, q = u / (uint32_t)c.W, q2 = q / (uint32_t)c.H;
+        r.x = (int)(u - q * (uint32_t)c.W);
+        r.y = (int)(q - q2 * (uint32_t)c.H);
+        r.z = (int)(q2 % (uint32_t)c.D);
+    } else {
+        r.x = (int)(v % c.W);
+        const int64_t q = v / c.W;
+        r.y = (int)(q % c.H);
+        r.z = (int)((q / c.H) % c.D);
+    }
+    return r;
+}
+
 // voxel index shifted by tap, or -1 outside the volume (zero padding)
-__device__ __forceinline__ int64_t vox_shift(const ConvDims& c, int64_t v, int tap) {
+__device__ __forceinline__ int64_t vox_shift(const ConvDims& c, const Vox& p, int tap) {
     const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-    const int x = (int)(v % c.W);
-    const int64_t r = v / c.W;
-    const int y = (int)(r % c.H);
-    const int64_t r2 = r / c.H;
-    const int z = (int)(r2 % c.D);
-    const int xx = x + dx, yy = y + dy, zz = z + dz;
+    const int xx = p.x + dx, yy = p.y + dy, zz = p.z + dz;
     if (xx < 0 || xx >= c.W || yy < 0 || yy >= c.H || zz < 0 || zz >= c.D) return -1;
-    return v + dx + (int64_t)c.W * (dy + (int64_t)c.H * dz);
+    return p.v + dx + (int64_t)c.W * (dy + (int64_t)c.H * dz);
 }
 
 // Implicit-GEMM convolution.  Grid: (ceil(nvox/64), co/64 rounded up).
@@ -75,9 +94,10 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvDims c, const uint16_t* __
     // staging: thread t copies 4 bf16 (8 bytes) of row t/4, k (t%4)*4 .. +3
     const int srow = tid >> 2, sk = (tid & 3) * 4;
     const int64_t vrow = m0 + srow;
+    const Vox pv = vox_decode(c, vrow < c.nvox ? vrow : 0);
     const int kchunks = c.ci / 16;
     for (int tap = 0; tap < 27; ++tap) {
-        const int64_t vs = vrow < c.nvox ? vox_shift(c, vrow, tap) : -1;
+        const int64_t vs = vrow < c.nvox ? vox_shift(c, pv, tap) : -1;
         for (int kc = 0; kc < kchunks; ++kc) {
             const int i0 = kc * 16 + sk;
             uint2 av = make_uint2(0u, 0u), bv = make_uint2(0u, 0u);
@@ -125,6 +145,84 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvDims c, const uint16_t* __
             }
 }
 
+// The same convolution for cin a multiple of 32, software-pipelined: a K
+// stage is one tap x 32 input channels (two MFMA k-steps, 8 MFMAs per wave
+// between barriers), each thread stages 16 bytes of A and of B, and the next
+// stage's global loads are in flight while the current stage computes.
+__global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* __restrict__ X,
+                                                    const uint16_t* __restrict__ Wk,
+                                                    const float* __restrict__ bias, int relu,
+                                                    float* __restrict__ Y,
+                                                    uint16_t* __restrict__ Ybf) {
+    __shared__ uint16_t sA[64][32 + 8];
+    __shared__ uint16_t sB[64][32 + 8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t m0 = (int64_t)blockIdx.x * 64;
+    const int n0 = blockIdx.y * 64;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    v4f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    // staging: thread t copies 8 bf16 (16 bytes) of row t/4, k (t%4)*8 .. +7
+    const int srow = tid >> 2, sk = (tid & 3) * 8;
+    const int64_t vrow = m0 + srow;
+    const Vox pv = vox_decode(c, vrow < c.nvox ? vrow : 0);
+    const int o = n0 + srow;
+    const int kch = c.ci / 32;
+    const int nst = 27 * kch;
+    auto fetch = [&](int st, uint4& av, uint4& bv) {
+        const int tap = st / kch, i0 = (st - tap * kch) * 32 + sk;
+        const int64_t vs = vrow < c.nvox ? vox_shift(c, pv, tap) : -1;
+        av = make_uint4(0u, 0u, 0u, 0u);
+        bv = make_uint4(0u, 0u, 0u, 0u);
+        if (vs >= 0) av = *(const uint4*)(X + vs * c.ci + i0);
+        if (o < c.co) bv = *(const uint4*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
+    };
+    uint4 av, bv;
+    fetch(0, av, bv);
+    const int fi = lane & 15, fk = (lane >> 4) * 4;
+    for (int st = 0; st < nst; ++st) {
+        __syncthreads();
+        *(uint4*)&sA[srow][sk] = av;
+        *(uint4*)&sB[srow][sk] = bv;
+        __syncthreads();
+        if (st + 1 < nst) fetch(st + 1, av, bv);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            v4s fa[2], fb[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                fa[t] = __builtin_bit_cast(v4s, *(const uint2*)&sA[wm + t * 16 + fi][ks * 16 + fk]);
+                fb[t] = __builtin_bit_cast(v4s, *(const uint2*)&sB[wn + t * 16 + fi][ks * 16 + fk]);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[a], fb[b], acc[a][b],
+                                                                          0, 0, 0);
+        }
+    }
+    const int dj = lane & 15, di = (lane >> 4) * 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t v = m0 + wm + a * 16 + di + r;
+                const int oo = n0 + wn + b * 16 + dj;
+                if (v < c.nvox && oo < c.co) {
+                    float y = acc[a][b][r] + (bias ? bias[oo] : 0.0f);
+                    if (relu && y < 0.0f) y = 0.0f;
+                    Y[v * c.co + oo] = y;
+                    if (Ybf) Ybf[v * c.co + oo] = f2bf(y);
+                }
+            }
+}
+
 // Wt[tap][i][o] = Wk[26 - tap][o][i] (weights of the input-gradient pass)
 __global__ void k_conv_wflip(int ci, int co, const uint16_t* __restrict__ Wk,
                              uint16_t* __restrict__ Wt) {
@@ -166,7 +264,7 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(ConvDims c, const uint16_t* 
         uint2 dv = make_uint2(0u, 0u), xv = make_uint2(0u, 0u);
         if (v < vend) {
             if (o0 + sc < c.co) dv = *(const uint2*)(dY + v * c.co + o0 + sc);
-            const int64_t vs = vox_shift(c, v, tap);
+            const int64_t vs = vox_shift(c, vox_decode(c, v), tap);
             if (vs >= 0 && i0 + sc < c.ci) xv = *(const uint2*)(X + vs * c.ci + i0 + sc);
         }
         __syncthreads();
@@ -251,7 +349,11 @@ void launch_conv_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk
                      const float* bias, int relu, float* Y, uint16_t* Ybf, hipStream_t s) {
     const ConvDims c = dims(shape);
     const dim3 grid((unsigned)((c.nvox + 63) / 64), (unsigned)((c.co + 63) / 64));
-    hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
+    // the pipelined kernel reads 16 bytes at a time
+    if (c.ci % 32 == 0 && (((uintptr_t)X | (uintptr_t)Wk) & 15) == 0)
+        hipLaunchKernelGGL(k_conv_fwd32, grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
+    else
+        hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
 }
 
 void launch_conv_wflip(int ci, int co, const uint16_t* Wk, uint16_t* Wt, hipStream_t s) {
