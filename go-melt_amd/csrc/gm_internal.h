@@ -62,11 +62,12 @@ struct DevMat {
     // Per-bin segment records (pnb > 0 when every bin of width 1/piw over
     // the knots, widened by a quarter bin on each side, meets at most two
     // segments A and A+1).  prec[b] = (T_{A+1} or +inf, T_A, k(T_A), k slope,
-    // cp(T_A), cp slope, k(T_{A+1}), k slope, cp(T_{A+1}), cp slope, 0, 0):
+    // cp(T_A), cp slope, k(T_{A+1}), k slope, cp(T_{A+1}), cp slope, 0 x 6;
+    // 64 bytes, so a record's offset is a shift):
     // the segment of T in bin b is A+1 when T >= prec[b][0], else A.
     int32_t pnb;
     float pT0, piw;
-    float prec[GM_PREC_BINS][12];
+    float prec[GM_PREC_BINS][16];
 };
 
 // One axis of the tabulated restriction between a fine and a coarse mesh

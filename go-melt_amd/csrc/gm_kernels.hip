@@ -225,7 +225,7 @@ __device__ __forceinline__ void lds_barrier() {
 #define QN (QX * QY)
 // LDS words of the staged property table: the per-bin segment records, or
 // the loop lookup (bin -> first segment, then the segment rows).
-#define PTW (GM_PREC_BINS * 12 > GM_LUT_BINS + 10 * GM_MAX_KNOTS ? GM_PREC_BINS * 12 \
+#define PTW (GM_PREC_BINS * 16 > GM_LUT_BINS + 10 * GM_MAX_KNOTS ? GM_PREC_BINS * 16 \
                                                                  : GM_LUT_BINS + 10 * GM_MAX_KNOTS)
 
 // computeStateProperties (cF:2584-2612) for one node, without the unit
@@ -265,7 +265,7 @@ __device__ __forceinline__ void stage_tables(const DevMat* __restrict__ m, float
     if (m->pnb > 0) {
         const float4* src = (const float4*)&m->prec[0][0];
         float4* dst = (float4*)sTab;
-        for (int l = tid; l < GM_PREC_BINS * 3; l += nthr) dst[l] = src[l];
+        for (int l = tid; l < GM_PREC_BINS * 4; l += nthr) dst[l] = src[l];
         return;
     }
     const int n = GM_LUT_BINS + 5 * m->mn;
@@ -289,7 +289,7 @@ __device__ __forceinline__ void props_rec(const PropC& c, const float* __restric
                                           bool& s1) {
     int b = (int)((T - c.pT0) * c.piw);
     b = b < 0 ? 0 : (b > c.pnb - 1 ? c.pnb - 1 : b);
-    const float4* r = (const float4*)(tab + 12 * b);
+    const float4* r = (const float4*)(tab + 16 * b);  // 64-byte records: a shift
     const float4 r0 = r[0], r1 = r[1], r2 = r[2];
     const bool up = T >= r0.x;
     const float Ts = up ? r0.x : r0.y;
@@ -834,7 +834,7 @@ __global__ void __launch_bounds__(64) k_step_l1(LSArgs a, const GmStepDesc* __re
     if (a.dt_mode == GM_DT_ROW) dt = d->rows[a.dt_index][5];
     else if (a.dt_mode == GM_DT_L1) dt = d->dt1;
     else if (a.dt_mode == GM_DT_L2) dt = d->dt2[a.dt_index];
-    const float rmfac = dt / a.mrow;
+    const float rmfac = dt * __builtin_amdgcn_rcpf(a.mrow);  // uniform: one reciprocal
     const PropC pc = load_pc(m);
     const float Tamb = m->T_amb;
 
@@ -3135,7 +3135,7 @@ DevMat make_devmat(const gm_material& mm) {
                 r[0] = a + 1 <= nU - 2 ? (float)U[a + 1] : INFINITY;
                 for (int f = 0; f < 5; ++f) r[1 + f] = (float)segv[a][f];
                 for (int f = 0; f < 4; ++f) r[6 + f] = (float)segv[c][1 + f];
-                r[10] = r[11] = 0.0f;
+                for (int f = 10; f < 16; ++f) r[f] = 0.0f;
             }
             break;
         }
@@ -3152,7 +3152,7 @@ DevMat make_devmat(const gm_material& mm) {
             r[3] = mm.k_bulk_a1;
             r[4] = mm.cp_solid_a0;
             r[5] = mm.cp_solid_a1;
-            for (int f = 6; f < 12; ++f) r[f] = 0.0f;
+            for (int f = 6; f < 16; ++f) r[f] = 0.0f;
         }
     }
     return m;
