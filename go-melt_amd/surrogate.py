@@ -477,3 +477,155 @@ def train(net, X, Y, iters=100, batch=4, lr=1e-3, seed=0, log=None):
         if log and (it % 20 == 0 or it == iters - 1):
             log(f"iter {it} loss {float(hist[-1].item()):.4e}")
     return [float(h.item()) for h in hist]
+
+
+# ---------------------------------------------------------------------------
+# Operator in the loop (batched windows) against the plain solver
+# ---------------------------------------------------------------------------
+
+def _jump_audit(inst, grid, T0, Tn, rows, K):
+    """Energy audit of a K-step jump T0 -> Tn of one window (gm_energy_balance
+    with dt_total = K dt): properties of T0, the source averaged over the
+    jump's K laser rows plus the surface flux of T0, Dirichlet faces and the
+    T_amb floor held.  Returns the relative residual.  The identity is exact
+    for one explicit step; over a jump it carries the change of the
+    conduction term across the jump, so the plain solver's own K steps score
+    a small nonzero residual that this is compared against."""
+    import computeFunctions as cf
+    p = cf.SetupProperties(inst["properties"])
+    mat = cf._material(p)
+    nn = T0.numel()
+    st = _stream()
+    S1 = _zeros(nn)
+    S1o = torch.empty(nn, dtype=torch.float32, device="cuda")
+    S2o = torch.empty(nn, dtype=torch.uint8, device="cuda")
+    k = torch.empty(nn, dtype=torch.float32, device="cuda")
+    rc = torch.empty(nn, dtype=torch.float32, device="cuda")
+    N.check(N.lib().gm_state_properties(C.byref(mat), _ptr(T0), _ptr(S1), nn, 0, _ptr(S1o),
+                                        _ptr(S2o), _ptr(k), _ptr(rc), st))
+    F = _zeros(nn)
+    Fk = torch.empty(nn, dtype=torch.float32, device="cuda")
+    for r in rows:
+        v = (C.c_float * 3)(float(r[0]), float(r[1]), float(r[2]))
+        N.check(N.lib().gm_source_l3(C.byref(grid), C.byref(mat), v, float(r[6]), _ptr(Fk), st))
+        N.check(N.lib().gm_nn_axpy(nn, 1.0 / len(rows), _ptr(Fk), _ptr(F), st))
+    ne = (grid.n[0] - 1) * (grid.n[1] - 1) * (grid.n[2] - 1)
+    N.check(N.lib().gm_conv_rad_bc(C.byref(grid), C.byref(mat), _ptr(T0), ne, _ptr(F), st))
+    sums = _zeros(8, torch.float64)
+    dt = float(rows[0][5]) * K
+    N.check(N.lib().gm_energy_balance(C.byref(grid), ne, _ptr(k), _ptr(rc), dt, _ptr(T0), _ptr(F),
+                                      None, _ptr(Tn), 31, float(p["T_amb"]), _ptr(sums), st))
+    return float(sums[6].item())
+
+
+def run_in_loop(net, instances, n_jumps, K=10, warm=20, judge_every=2, residual_limit=0.01):
+    """Advance batched windows by operator jumps of K steps next to the plain
+    batched solver from the same start.  Each jump is audited
+    (gm_energy_balance over the jump); when any window's relative residual
+    exceeds residual_limit the jump is discarded and the windows take K plain
+    steps instead (the fallback).  Every judge_every jumps the melt-pool
+    depth and width of the two runs are compared (relative error, averaged
+    over the windows that have a pool).  Returns a dict of the errors,
+    residuals (operator jumps and, for calibration, the plain K steps),
+    fallbacks and times (operator jump vs K plain steps, all windows at
+    once)."""
+    import numpy as np
+    import workloads as W
+    import batch as B
+    import computeFunctions as cf
+
+    class _TorchStream:
+        ptr = torch.cuda.current_stream().cuda_stream
+
+        @staticmethod
+        def sync():
+            torch.cuda.current_stream().synchronize()
+
+    plain = B.Batch.from_instances(instances, stream=_TorchStream())
+    sur = B.Batch.from_instances(instances, stream=_TorchStream())
+    n = len(instances)
+    g = plain.grids[0]
+    D, H, Wd = g.n[2], g.n[1], g.n[0]
+    lv = {"node_coords": g.axes(), "nodes": [g.n[0], g.n[1], g.n[2]], "h": list(g.h)}
+    Tl = instances[0]["properties"]["T_liquidus"]
+    s = 0
+    for _ in range(warm):
+        rows = W.batch_rows(instances, s)
+        plain.step(rows)
+        sur.step(rows)
+        s += 1
+    X = torch.empty((n, D, H, Wd, 16), dtype=torch.bfloat16, device="cuda")
+    t_net = t_plain = 0.0
+    fallbacks = 0
+    derr, werr, r_sur, r_plain = [], [], [], []
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    for j in range(n_jumps):
+        rows = W.batch_rows(instances, s)
+        T0 = [_copy_field(sur._field(i, 0)) for i in range(n)]
+        Tp0 = [_copy_field(plain._field(i, 0)) for i in range(n)]
+        faces = [_copy_field(sur._field(i, 2)) for i in range(n)]
+        e[0].record()
+        for i in range(n):
+            window_input(instances[i], sur.grids[i], T0[i], faces[i], rows[i], X[i])
+        Yh = net.forward(X)
+        Tn = []
+        nn3 = (C.c_int32 * 3)(g.n[0], g.n[1], g.n[2])
+        for i in range(n):
+            t = torch.empty_like(T0[i])
+            N.check(N.lib().gm_nn_window_apply(nn3, _ptr(T0[i]), _ptr(faces[i]),
+                                               C.c_void_p(Yh.data_ptr() + i * Yh[0].numel() * 4),
+                                               Yh.shape[-1], T_SCALE,
+                                               float(instances[i]["properties"]["T_amb"]),
+                                               _ptr(t), _stream()))
+            Tn.append(t)
+            d = sur._field(i, 0)
+            N.check(N.lib().gm_memcpy_d2d(C.c_void_p(d.ptr), _ptr(t), d.nbytes, _stream()))
+        N.check(N.lib().gm_batch_refresh(sur.h))
+        e[1].record()
+        jrows = [W.batch_rows(instances, s + k) for k in range(K)]
+        e[2].record()
+        for k in range(K):
+            plain.step(jrows[k])
+        e[3].record()
+        torch.cuda.synchronize()
+        t_net += e[0].elapsed_time(e[1])
+        t_plain += e[2].elapsed_time(e[3])
+        worst = 0.0
+        for i in range(n):
+            rws = [jr[i] for jr in jrows]
+            r = _jump_audit(instances[i], sur.grids[i], T0[i], Tn[i], rws, K)
+            r_sur.append(r)
+            worst = max(worst, abs(r))
+            Tp1 = _copy_field(plain._field(i, 0))
+            r_plain.append(_jump_audit(instances[i], plain.grids[i], Tp0[i], Tp1, rws, K))
+        if worst > residual_limit:
+            # fallback: back to the jump's start, K plain steps
+            fallbacks += 1
+            for i in range(n):
+                d = sur._field(i, 0)
+                N.check(N.lib().gm_memcpy_d2d(C.c_void_p(d.ptr), _ptr(T0[i]), d.nbytes,
+                                              _stream()))
+            N.check(N.lib().gm_batch_refresh(sur.h))
+            for k in range(K):
+                sur.step(jrows[k])
+        s += K
+        if (j + 1) % judge_every == 0:
+            for i in range(n):
+                dp = cf.meltPoolDimensions(lv, _copy_field(plain._field(i, 0)), Tl)
+                ds = cf.meltPoolDimensions(lv, _copy_field(sur._field(i, 0)), Tl)
+                if dp[0] > 0 and dp[1] > 0:
+                    derr.append(abs(ds[0] - dp[0]) / dp[0])
+                    werr.append(abs(ds[1] - dp[1]) / dp[1])
+    return {
+        "jumps": n_jumps, "windows": n, "K": K,
+        "melt_pool_rel_err_depth": float(np.mean(derr)) if derr else None,
+        "melt_pool_rel_err_width": float(np.mean(werr)) if werr else None,
+        "residual_limit": residual_limit,
+        "fallbacks": fallbacks,
+        "energy_residual_operator_mean_abs": float(np.mean(np.abs(r_sur))) if r_sur else None,
+        "energy_residual_plain_mean_abs": float(np.mean(np.abs(r_plain))) if r_plain else None,
+        "ms_per_jump_operator": t_net / n_jumps,
+        "ms_per_K_plain_steps": t_plain / n_jumps,
+        "final_T": ([_copy_field(sur._field(i, 0)) for i in range(n)],
+                    [_copy_field(plain._field(i, 0)) for i in range(n)]),
+    }

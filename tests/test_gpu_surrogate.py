@@ -131,3 +131,23 @@ def test_operator_training_beats_persistence(S):
     print("held-out relative error: untrained", e0, "trained", e1, "loss", hist[0], "->", hist[-1])
     assert hist[-1] < 0.5 * hist[0]
     assert e1 < 0.7
+
+
+def test_operator_in_the_loop_fallback(S):
+    """In the loop, the energy audit separates the plain solver's K steps
+    (residual 1e-7 to 1e-4) from an untrained operator's jump (residual far above
+    1%); every such jump falls back to K plain steps, so the run ends
+    bitwise on the plain solver's fields."""
+    import numpy as np
+    import workloads as W
+    inst = W.batch_instances(n=3, seed=13, sizes=[(31, 31, 15)])
+    net = S.UNet(channels=(16, 32), seed=4)
+    out = S.run_in_loop(net, inst, n_jumps=3, K=10, warm=10, judge_every=3,
+                        residual_limit=0.01)
+    print({k: v for k, v in out.items() if k != "final_T"})
+    assert out["energy_residual_plain_mean_abs"] < 1e-3
+    assert out["energy_residual_operator_mean_abs"] > 0.01
+    assert out["fallbacks"] == 3
+    sur, plain = out["final_T"]
+    for a, b in zip(sur, plain):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
