@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned-operator measurement")
-    ap.add_argument("--surrogate-iters", type=int, default=60)
+    ap.add_argument("--surrogate-iters", type=int, default=1000)
     ap.add_argument("--surrogate-only", action="store_true",
                     help="print only the learned-operator measurement (one JSON line)")
     return ap.parse_args()
@@ -58,22 +58,27 @@ def parse():
 
 def surrogate_measure(iters):
     """Learned L3 operator (3-D U-Net, channels 32-64-128-128, BF16 MFMA
-    convolutions): trained for `iters` Adam steps (batch 4) on windows of
-    64 x 64 x 32 nodes cut from the plain batched solver (K = 10 steps per
-    jump), then one jump on held-out windows against 10 plain steps."""
+    convolutions): trained for `iters` Adam steps (batch 4; learning rate
+    1e-3, halved after 400 steps) on 128 windows of 64 x 64 x 32 nodes cut
+    from the plain batched solver (K = 10 steps per jump), then one jump on
+    held-out windows against 10 plain steps, and runs in the loop with the
+    1% energy trigger and without it."""
     import numpy as np
     import torch
     import workloads as W
     import surrogate as S
     import computeFunctions as cf
-    tr = W.batch_instances(n=16, seed=21, sizes=[(63, 63, 31)])
+    tr = W.batch_instances(n=32, seed=21, sizes=[(63, 63, 31)])
     te = W.batch_instances(n=4, seed=22, sizes=[(63, 63, 31)])
-    Xtr, Ytr, _, _ = S.make_windows(tr, K=10, warm=20, n_snap=3, gap=10)
+    Xtr, Ytr, _, _ = S.make_windows(tr, K=10, warm=20, n_snap=4, gap=10)
     Xte, Yte, keep, bt = S.make_windows(te, K=10, warm=20, n_snap=1, gap=0)
     net = S.UNet(channels=(32, 64, 128, 128), seed=0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist = S.train(net, Xtr, Ytr, iters=iters, batch=4, lr=1e-3)
+    n1 = min(iters, 400)
+    hist = S.train(net, Xtr, Ytr, iters=n1, batch=4, lr=1e-3)
+    if iters > n1:
+        hist += S.train(net, Xtr, Ytr, iters=iters - n1, batch=4, lr=5e-4, seed=1)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     D, H, Wd = Xtr.shape[1:4]
@@ -106,9 +111,13 @@ def surrogate_measure(iters):
     plain_ms = 10 * one.time(batched=True, iters=50)
     loop = S.run_in_loop(net, te, n_jumps=6, K=10, warm=20, judge_every=2, residual_limit=0.01)
     loop.pop("final_T")
+    free = S.run_in_loop(net, te, n_jumps=6, K=10, warm=20, judge_every=2,
+                         residual_limit=float("inf"))
+    free.pop("final_T")
+    free["residual_limit"] = None  # no trigger (strict JSON has no Infinity)
     return {
         "workload": "3-D U-Net 32-64-128-128 on 64x64x32-node L3 windows, K=10 steps per jump; "
-                    f"{Xtr.shape[0]} training windows from 16 plain batched instances, "
+                    f"{Xtr.shape[0]} training windows from 32 plain batched instances, "
                     f"{iters} Adam steps of batch 4; {len(keep)} held-out windows",
         "train_s": t_train,
         "train_windows_per_s": 4 * iters / t_train,
@@ -120,6 +129,7 @@ def surrogate_measure(iters):
         "jump_ms_one_window": jump_ms,
         "plain_10_steps_ms_one_window": plain_ms,
         "in_the_loop": loop,
+        "in_the_loop_no_fallback": free,
         "roofline": {"kernel": "k_conv_fwd / k_conv_wgrad (training step)", "bound": "mfma",
                      "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": tflops / 2500.0, "flops_per_window_fwd": fl_fwd},
