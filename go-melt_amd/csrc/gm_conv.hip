@@ -304,6 +304,82 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(ConvDims c, const uint16_t* 
             }
 }
 
+// Weight gradient for cin and cout multiples of 32 (8 channels per 16-byte
+// load): stages of 32 voxels staged voxel-major ([vox][channel], 16-byte LDS
+// writes, no transpose), the next stage's loads in flight during the MFMAs;
+// a lane's fragment (4 consecutive voxels of one channel) is 4 LDS reads.
+__global__ void __launch_bounds__(256) k_conv_wgrad32(ConvDims c, const uint16_t* __restrict__ X,
+                                                      const uint16_t* __restrict__ dY, int vsplit,
+                                                      float* __restrict__ dW) {
+    __shared__ uint16_t sA[32][64 + 8];  // [vox][o]
+    __shared__ uint16_t sB[32][64 + 8];  // [vox][i]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int o0 = blockIdx.x * 64, i0 = blockIdx.y * 64;
+    const int tap = blockIdx.z % 27, part = blockIdx.z / 27;
+    const int64_t per = (c.nvox + vsplit - 1) / vsplit;
+    const int64_t vbeg = (int64_t)part * per;
+    const int64_t vend = vbeg + per < c.nvox ? vbeg + per : c.nvox;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    v4f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    // staging: thread t loads 8 channels (16 bytes) of voxel t/8, channels
+    // (t%8)*8 .. +7 of the 64-wide tile, for dY and for the shifted X
+    const int sv = tid >> 3, sc = (tid & 7) * 8;
+    auto fetch = [&](int64_t vb, uint4& dv, uint4& xv) {
+        const int64_t v = vb + sv;
+        dv = make_uint4(0u, 0u, 0u, 0u);
+        xv = make_uint4(0u, 0u, 0u, 0u);
+        if (v < vend) {
+            if (o0 + sc < c.co) dv = *(const uint4*)(dY + v * c.co + o0 + sc);
+            const int64_t vs = vox_shift(c, vox_decode(c, v), tap);
+            if (vs >= 0 && i0 + sc < c.ci) xv = *(const uint4*)(X + vs * c.ci + i0 + sc);
+        }
+    };
+    uint4 dv, xv;
+    fetch(vbeg, dv, xv);
+    const int fi = lane & 15, fk = (lane >> 4) * 4;
+    for (int64_t vb = vbeg; vb < vend; vb += 32) {
+        __syncthreads();
+        *(uint4*)&sA[sv][sc] = dv;
+        *(uint4*)&sB[sv][sc] = xv;
+        __syncthreads();
+        if (vb + 32 < vend) fetch(vb + 32, dv, xv);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            v4s fa[2], fb[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int ro = wm + t * 16 + fi, ri = wn + t * 16 + fi, k0 = ks * 16 + fk;
+                fa[t] = v4s{(short)sA[k0][ro], (short)sA[k0 + 1][ro], (short)sA[k0 + 2][ro],
+                            (short)sA[k0 + 3][ro]};
+                fb[t] = v4s{(short)sB[k0][ri], (short)sB[k0 + 1][ri], (short)sB[k0 + 2][ri],
+                            (short)sB[k0 + 3][ri]};
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[a], fb[b], acc[a][b],
+                                                                          0, 0, 0);
+        }
+    }
+    const int dj = lane & 15, di = (lane >> 4) * 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = o0 + wm + a * 16 + di + r;
+                const int i = i0 + wn + b * 16 + dj;
+                if (o < c.co && i < c.ci)
+                    atomicAdd(dW + ((int64_t)tap * c.co + o) * c.ci + i, acc[a][b][r]);
+            }
+}
+
 // Adam on FP32 master weights, with the bf16 copy the kernels read
 __global__ void k_adam(float* __restrict__ w, uint16_t* __restrict__ wbf,
                        const float* __restrict__ g, float* __restrict__ m,
@@ -367,7 +443,10 @@ void launch_conv_wgrad(const int32_t* shape, const uint16_t* X, const uint16_t* 
     const ConvDims c = dims(shape);
     const dim3 grid((unsigned)((c.co + 63) / 64), (unsigned)((c.ci + 63) / 64),
                     (unsigned)(27 * vsplit));
-    hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
+    if (c.ci % 32 == 0 && c.co % 32 == 0 && (((uintptr_t)X | (uintptr_t)dY) & 15) == 0)
+        hipLaunchKernelGGL(k_conv_wgrad32, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
+    else
+        hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
 }
 
 void launch_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, int64_t n, float lr,
