@@ -149,44 +149,57 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvDims c, const uint16_t* __
 // stage is one tap x 32 input channels (two MFMA k-steps, 8 MFMAs per wave
 // between barriers), each thread stages 16 bytes of A and of B, and the next
 // stage's global loads are in flight while the current stage computes.
+template <int BM, int BN>
 __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* __restrict__ X,
                                                     const uint16_t* __restrict__ Wk,
                                                     const float* __restrict__ bias, int relu,
                                                     float* __restrict__ Y,
                                                     uint16_t* __restrict__ Ybf) {
-    __shared__ uint16_t sA[64][32 + 8];
-    __shared__ uint16_t sB[64][32 + 8];
+    // BM voxels x BN outputs per block, 4 waves of 32 x 32 (BM/32 x BN/32 = 4)
+    static_assert((BM / 32) * (BN / 32) == 4, "four waves of 32 x 32");
+    constexpr int NA = BM / 64;  // 16-byte A loads per thread and stage
+    __shared__ uint16_t sA[BM][32 + 8];
+    __shared__ uint16_t sB[BN][32 + 8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t m0 = (int64_t)blockIdx.x * 64;
-    const int n0 = blockIdx.y * 64;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int64_t m0 = (int64_t)blockIdx.x * BM;
+    const int n0 = blockIdx.y * BN;
+    const int wm = (wave % (BM / 32)) * 32, wn = (wave / (BM / 32)) * 32;
     v4f acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    // staging: thread t copies 8 bf16 (16 bytes) of row t/4, k (t%4)*8 .. +7
+    // staging: 16 bytes (8 bf16) of row r, k (t%4)*8 .. +7, rows r = t/4 + 64 q
     const int srow = tid >> 2, sk = (tid & 3) * 8;
-    const int64_t vrow = m0 + srow;
-    const Vox pv = vox_decode(c, vrow < c.nvox ? vrow : 0);
-    const int o = n0 + srow;
+    Vox pv[NA];
+    int64_t vrow[NA];
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+        vrow[q] = m0 + srow + 64 * q;
+        pv[q] = vox_decode(c, vrow[q] < c.nvox ? vrow[q] : 0);
+    }
+    const int o = n0 + srow;  // B rows: threads with srow < BN
     const int kch = c.ci / 32;
     const int nst = 27 * kch;
-    auto fetch = [&](int st, uint4& av, uint4& bv) {
+    auto fetch = [&](int st, uint4* av, uint4& bv) {
         const int tap = st / kch, i0 = (st - tap * kch) * 32 + sk;
-        const int64_t vs = vrow < c.nvox ? vox_shift(c, pv, tap) : -1;
-        av = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int64_t vs = vrow[q] < c.nvox ? vox_shift(c, pv[q], tap) : -1;
+            av[q] = make_uint4(0u, 0u, 0u, 0u);
+            if (vs >= 0) av[q] = *(const uint4*)(X + vs * c.ci + i0);
+        }
         bv = make_uint4(0u, 0u, 0u, 0u);
-        if (vs >= 0) av = *(const uint4*)(X + vs * c.ci + i0);
-        if (o < c.co) bv = *(const uint4*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
+        if (srow < BN && o < c.co) bv = *(const uint4*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
     };
-    uint4 av, bv;
+    uint4 av[NA], bv;
     fetch(0, av, bv);
     const int fi = lane & 15, fk = (lane >> 4) * 4;
     for (int st = 0; st < nst; ++st) {
         __syncthreads();
-        *(uint4*)&sA[srow][sk] = av;
-        *(uint4*)&sB[srow][sk] = bv;
+#pragma unroll
+        for (int q = 0; q < NA; ++q) *(uint4*)&sA[srow + 64 * q][sk] = av[q];
+        if (srow < BN) *(uint4*)&sB[srow][sk] = bv;
         __syncthreads();
         if (st + 1 < nst) fetch(st + 1, av, bv);
 #pragma unroll
@@ -380,6 +393,83 @@ __global__ void __launch_bounds__(256) k_conv_wgrad32(ConvDims c, const uint16_t
             }
 }
 
+// Weight gradient for cin = cout = 32 (the full-resolution layers): each
+// wave computes the whole 32 x 32 tile of one tap over its own quarter of a
+// 128-voxel stage (no idle half tiles), and adds it with FP32 atomics.
+__global__ void __launch_bounds__(256) k_conv_wgrad3232(ConvDims c, const uint16_t* __restrict__ X,
+                                                        const uint16_t* __restrict__ dY,
+                                                        int vsplit, float* __restrict__ dW) {
+    __shared__ uint16_t sA[128][32 + 8];  // [vox][o]
+    __shared__ uint16_t sB[128][32 + 8];  // [vox][i]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tap = blockIdx.z % 27, part = blockIdx.z / 27;
+    const int64_t per = (c.nvox + vsplit - 1) / vsplit;
+    const int64_t vbeg = (int64_t)part * per;
+    const int64_t vend = vbeg + per < c.nvox ? vbeg + per : c.nvox;
+    v4f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    // staging: 16 bytes (8 channels) of voxels t/4 and t/4 + 64, channels (t%4)*8
+    const int sv = tid >> 2, sc = (tid & 3) * 8;
+    auto fetch = [&](int64_t vb, uint4* dv, uint4* xv) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int64_t v = vb + sv + 64 * q;
+            dv[q] = make_uint4(0u, 0u, 0u, 0u);
+            xv[q] = make_uint4(0u, 0u, 0u, 0u);
+            if (v < vend) {
+                dv[q] = *(const uint4*)(dY + v * 32 + sc);
+                const int64_t vs = vox_shift(c, vox_decode(c, v), tap);
+                if (vs >= 0) xv[q] = *(const uint4*)(X + vs * 32 + sc);
+            }
+        }
+    };
+    uint4 dv[2], xv[2];
+    fetch(vbeg, dv, xv);
+    const int fi = lane & 15, fk = (lane >> 4) * 4;
+    const int kw = wave * 32;  // this wave's voxels of the stage
+    for (int64_t vb = vbeg; vb < vend; vb += 128) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            *(uint4*)&sA[sv + 64 * q][sc] = dv[q];
+            *(uint4*)&sB[sv + 64 * q][sc] = xv[q];
+        }
+        __syncthreads();
+        if (vb + 128 < vend) fetch(vb + 128, dv, xv);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int k0 = kw + ks * 16 + fk;
+            v4s fa[2], fb[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int r = t * 16 + fi;
+                fa[t] = v4s{(short)sA[k0][r], (short)sA[k0 + 1][r], (short)sA[k0 + 2][r],
+                            (short)sA[k0 + 3][r]};
+                fb[t] = v4s{(short)sB[k0][r], (short)sB[k0 + 1][r], (short)sB[k0 + 2][r],
+                            (short)sB[k0 + 3][r]};
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[a], fb[b], acc[a][b],
+                                                                          0, 0, 0);
+        }
+    }
+    const int dj = lane & 15, di = (lane >> 4) * 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                atomicAdd(dW + ((int64_t)tap * 32 + a * 16 + di + r) * 32 + b * 16 + dj,
+                          acc[a][b][r]);
+}
+
 // Adam on FP32 master weights, with the bf16 copy the kernels read
 __global__ void k_adam(float* __restrict__ w, uint16_t* __restrict__ wbf,
                        const float* __restrict__ g, float* __restrict__ m,
@@ -425,11 +515,20 @@ void launch_conv_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk
                      const float* bias, int relu, float* Y, uint16_t* Ybf, hipStream_t s) {
     const ConvDims c = dims(shape);
     const dim3 grid((unsigned)((c.nvox + 63) / 64), (unsigned)((c.co + 63) / 64));
-    // the pipelined kernel reads 16 bytes at a time
-    if (c.ci % 32 == 0 && (((uintptr_t)X | (uintptr_t)Wk) & 15) == 0)
-        hipLaunchKernelGGL(k_conv_fwd32, grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
-    else
+    // the pipelined kernel reads 16 bytes at a time; up to 32 outputs it
+    // takes 128 voxels x 32 outputs per block (no idle half of the tile)
+    if (c.ci % 32 == 0 && (((uintptr_t)X | (uintptr_t)Wk) & 15) == 0) {
+        if (c.co <= 32) {
+            const dim3 g2((unsigned)((c.nvox + 127) / 128), 1);
+            hipLaunchKernelGGL((k_conv_fwd32<128, 32>), g2, dim3(256), 0, s, c, X, Wk, bias, relu,
+                               Y, Ybf);
+        } else {
+            hipLaunchKernelGGL((k_conv_fwd32<64, 64>), grid, dim3(256), 0, s, c, X, Wk, bias, relu,
+                               Y, Ybf);
+        }
+    } else {
         hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
+    }
 }
 
 void launch_conv_wflip(int ci, int co, const uint16_t* Wk, uint16_t* Wt, hipStream_t s) {
@@ -443,7 +542,10 @@ void launch_conv_wgrad(const int32_t* shape, const uint16_t* X, const uint16_t* 
     const ConvDims c = dims(shape);
     const dim3 grid((unsigned)((c.co + 63) / 64), (unsigned)((c.ci + 63) / 64),
                     (unsigned)(27 * vsplit));
-    if (c.ci % 32 == 0 && c.co % 32 == 0 && (((uintptr_t)X | (uintptr_t)dY) & 15) == 0)
+    if (c.ci == 32 && c.co == 32 && (((uintptr_t)X | (uintptr_t)dY) & 15) == 0)
+        hipLaunchKernelGGL(k_conv_wgrad3232, dim3(1, 1, (unsigned)(27 * vsplit)), dim3(256), 0, s,
+                           c, X, dY, vsplit, dW);
+    else if (c.ci % 32 == 0 && c.co % 32 == 0 && (((uintptr_t)X | (uintptr_t)dY) & 15) == 0)
         hipLaunchKernelGGL(k_conv_wgrad32, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
     else
         hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
