@@ -31,7 +31,8 @@ def _rel(g, r):
 # pipelined 32-channel stages with 128 x 32 tiles (cout <= 32) and 64 x 64
 # tiles, the weight gradient's 32 x 32, 32-multiple and general kernels
 SHAPES = [(2, 5, 6, 7, 16, 32), (1, 8, 8, 4, 32, 16), (1, 3, 3, 3, 16, 48), (1, 9, 7, 11, 48, 80),
-          (2, 4, 6, 8, 32, 32), (1, 5, 6, 7, 64, 96), (1, 7, 5, 9, 96, 32)]
+          (2, 4, 6, 8, 32, 32), (1, 5, 6, 7, 64, 96), (1, 7, 5, 9, 96, 32),
+          (1, 6, 8, 10, 16, 32)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
