@@ -393,25 +393,26 @@ __global__ void __launch_bounds__(256) k_conv_wgrad32(ConvDims c, const uint16_t
             }
 }
 
-// Weight gradient for cout = 32 and cin = CI (32 or 16; the full-resolution
-// layers): each wave computes the whole 32 x CI tile of one tap over its own
-// quarter of a 128-voxel stage (no idle half tiles), and adds it with FP32
-// atomics.
-template <int CI>
+// Weight gradient for cout = CO and cin = CI, each 16 or 32 (the
+// full-resolution layers): each wave computes the whole CO x CI tile of one
+// tap over its own quarter of a 128-voxel stage (no idle half tiles), and
+// adds it with FP32 atomics.
+template <int CO, int CI>
 __global__ void __launch_bounds__(256) k_conv_wgrad32x(ConvDims c, const uint16_t* __restrict__ X,
                                                        const uint16_t* __restrict__ dY,
                                                        int vsplit, float* __restrict__ dW) {
+    constexpr int NA = CO / 16;           // MFMA tiles along cout
     constexpr int NB = CI / 16;           // MFMA tiles along cin
-    __shared__ uint16_t sA[128][32 + 8];  // [vox][o]
+    __shared__ uint16_t sA[128][CO + 8];  // [vox][o]
     __shared__ uint16_t sB[128][CI + 8];  // [vox][i]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tap = blockIdx.z % 27, part = blockIdx.z / 27;
     const int64_t per = (c.nvox + vsplit - 1) / vsplit;
     const int64_t vbeg = (int64_t)part * per;
     const int64_t vend = vbeg + per < c.nvox ? vbeg + per : c.nvox;
-    v4f acc[2][NB];
+    v4f acc[NA][NB];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NA; ++a)
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     // staging: 16 bytes (8 channels) of voxels t/4 and t/4 + 64, channels (t%4)*8
@@ -423,7 +424,7 @@ __global__ void __launch_bounds__(256) k_conv_wgrad32x(ConvDims c, const uint16_
             dv[q] = make_uint4(0u, 0u, 0u, 0u);
             xv[q] = make_uint4(0u, 0u, 0u, 0u);
             if (v < vend) {
-                dv[q] = *(const uint4*)(dY + v * 32 + sc);
+                if (sc < CO) dv[q] = *(const uint4*)(dY + v * CO + sc);
                 const int64_t vs = vox_shift(c, vox_decode(c, v), tap);
                 if (vs >= 0 && sc < CI) xv[q] = *(const uint4*)(X + vs * CI + sc);
             }
@@ -437,7 +438,7 @@ __global__ void __launch_bounds__(256) k_conv_wgrad32x(ConvDims c, const uint16_
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            *(uint4*)&sA[sv + 64 * q][sc] = dv[q];
+            if (sc < CO) *(uint4*)&sA[sv + 64 * q][sc] = dv[q];
             if (sc < CI) *(uint4*)&sB[sv + 64 * q][sc] = xv[q];
         }
         __syncthreads();
@@ -445,9 +446,9 @@ __global__ void __launch_bounds__(256) k_conv_wgrad32x(ConvDims c, const uint16_
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             const int k0 = kw + ks * 16 + fk;
-            v4s fa[2], fb[NB];
+            v4s fa[NA], fb[NB];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
+            for (int t = 0; t < NA; ++t) {
                 const int r = t * 16 + fi;
                 fa[t] = v4s{(short)sA[k0][r], (short)sA[k0 + 1][r], (short)sA[k0 + 2][r],
                             (short)sA[k0 + 3][r]};
@@ -459,7 +460,7 @@ __global__ void __launch_bounds__(256) k_conv_wgrad32x(ConvDims c, const uint16_
                             (short)sB[k0 + 3][r]};
             }
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < NA; ++a)
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[a], fb[b], acc[a][b],
@@ -468,12 +469,12 @@ __global__ void __launch_bounds__(256) k_conv_wgrad32x(ConvDims c, const uint16_
     }
     const int dj = lane & 15, di = (lane >> 4) * 4;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NA; ++a)
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                atomicAdd(dW + ((int64_t)tap * 32 + a * 16 + di + r) * CI + b * 16 + dj,
+                atomicAdd(dW + ((int64_t)tap * CO + a * 16 + di + r) * CI + b * 16 + dj,
                           acc[a][b][r]);
 }
 
@@ -550,12 +551,16 @@ void launch_conv_wgrad(const int32_t* shape, const uint16_t* X, const uint16_t* 
     const dim3 grid((unsigned)((c.co + 63) / 64), (unsigned)((c.ci + 63) / 64),
                     (unsigned)(27 * vsplit));
     const bool al16 = (((uintptr_t)X | (uintptr_t)dY) & 15) == 0;
-    if (c.co == 32 && (c.ci == 32 || c.ci == 16) && al16) {
+    if ((c.co == 32 || c.co == 16) && (c.ci == 32 || c.ci == 16) && al16) {
         const dim3 g1(1, 1, (unsigned)(27 * vsplit));
-        if (c.ci == 32)
-            hipLaunchKernelGGL((k_conv_wgrad32x<32>), g1, dim3(256), 0, s, c, X, dY, vsplit, dW);
+        if (c.co == 32 && c.ci == 32)
+            hipLaunchKernelGGL((k_conv_wgrad32x<32, 32>), g1, dim3(256), 0, s, c, X, dY, vsplit, dW);
+        else if (c.co == 32)
+            hipLaunchKernelGGL((k_conv_wgrad32x<32, 16>), g1, dim3(256), 0, s, c, X, dY, vsplit, dW);
+        else if (c.ci == 32)
+            hipLaunchKernelGGL((k_conv_wgrad32x<16, 32>), g1, dim3(256), 0, s, c, X, dY, vsplit, dW);
         else
-            hipLaunchKernelGGL((k_conv_wgrad32x<16>), g1, dim3(256), 0, s, c, X, dY, vsplit, dW);
+            hipLaunchKernelGGL((k_conv_wgrad32x<16, 16>), g1, dim3(256), 0, s, c, X, dY, vsplit, dW);
     }
     else if (c.ci % 32 == 0 && c.co % 32 == 0 && al16)
         hipLaunchKernelGGL(k_conv_wgrad32, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
