@@ -109,9 +109,9 @@ def surrogate_measure(iters):
     import batch as B
     one = B.Batch.from_instances(te[:1])
     plain_ms = 10 * one.time(batched=True, iters=50)
-    loop = S.run_in_loop(net, te, n_jumps=6, K=10, warm=20, judge_every=2, residual_limit=0.01)
+    loop = S.run_in_loop(net, te, n_jumps=6, K=10, warm=20, compare_every=2, residual_limit=0.01)
     loop.pop("final_T")
-    free = S.run_in_loop(net, te, n_jumps=6, K=10, warm=20, judge_every=2,
+    free = S.run_in_loop(net, te, n_jumps=6, K=10, warm=20, compare_every=2,
                          residual_limit=float("inf"))
     free.pop("final_T")
     free["residual_limit"] = None  # no trigger (strict JSON has no Infinity)

@@ -518,12 +518,12 @@ def _jump_audit(inst, grid, T0, Tn, rows, K):
     return float(sums[6].item())
 
 
-def run_in_loop(net, instances, n_jumps, K=10, warm=20, judge_every=2, residual_limit=0.01):
+def run_in_loop(net, instances, n_jumps, K=10, warm=20, compare_every=2, residual_limit=0.01):
     """Advance batched windows by operator jumps of K steps next to the plain
     batched solver from the same start.  Each jump is audited
     (gm_energy_balance over the jump); when any window's relative residual
     exceeds residual_limit the jump is discarded and the windows take K plain
-    steps instead (the fallback).  Every judge_every jumps the melt-pool
+    steps instead (the fallback).  Every compare_every jumps the melt-pool
     depth and width of the two runs are compared (relative error, averaged
     over the windows that have a pool).  Returns a dict of the errors,
     residuals (operator jumps and, for calibration, the plain K steps),
@@ -609,7 +609,7 @@ def run_in_loop(net, instances, n_jumps, K=10, warm=20, judge_every=2, residual_
             for k in range(K):
                 sur.step(jrows[k])
         s += K
-        if (j + 1) % judge_every == 0:
+        if (j + 1) % compare_every == 0:
             for i in range(n):
                 dp = cf.meltPoolDimensions(lv, _copy_field(plain._field(i, 0)), Tl)
                 ds = cf.meltPoolDimensions(lv, _copy_field(sur._field(i, 0)), Tl)

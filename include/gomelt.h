@@ -134,7 +134,7 @@ int gm_level_source(const gm_grid* coarse, const gm_grid* fine, const gm_materia
 
 /* Melt-pool size of a level's temperature field.  No reference counterpart:
  * the reference only forms S2 = T >= T_liquidus (cF:2585); this is the
- * measure the surrogate's error is judged by.  dims: 8 floats of device memory,
+ * measure the surrogate's error is taken in.  dims: 8 floats of device memory,
  * written asynchronously on `stream`:
  *   [0] depth  = max over vertical edges with a molten upper node of the
  *                distance from the top node plane down to the T_liquidus

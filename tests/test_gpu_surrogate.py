@@ -142,7 +142,7 @@ def test_operator_in_the_loop_fallback(S):
     import workloads as W
     inst = W.batch_instances(n=3, seed=13, sizes=[(31, 31, 15)])
     net = S.UNet(channels=(16, 32), seed=4)
-    out = S.run_in_loop(net, inst, n_jumps=3, K=10, warm=10, judge_every=3,
+    out = S.run_in_loop(net, inst, n_jumps=3, K=10, warm=10, compare_every=3,
                         residual_limit=0.01)
     print({k: v for k, v in out.items() if k != "final_T"})
     assert out["energy_residual_plain_mean_abs"] < 1e-3
