@@ -147,7 +147,7 @@ def batch_measure(n, iters=50):
     bt = B.Batch.from_instances(inst)
     for s in range(10):  # a melt pool in every instance before timing
         bt.step(W.batch_rows(inst, s))
-    zc, tune_ms = bt.autotune(iters=20)
+    tile, tune_ms = bt.autotune(iters=20)
     ms_b = bt.time(batched=True, iters=iters)
     ms_u = bt.time(batched=False, iters=iters)
     nodes = sum((e[0] + 1) * (e[1] + 1) * (e[2] + 1) for e in (b["elements"] for b in inst))
@@ -156,7 +156,7 @@ def batch_measure(n, iters=50):
         "workload": f"{n} single-track L3 instances, element counts drawn (seed 0) from "
                     f"{W.BATCH_SIZES}, P 100-400 W, 0.5-2 m/s, r 30-70 um",
         "nodes": nodes,
-        "zc_autotuned": zc,
+        "brick_autotuned": {"rows": tile[0], "planes": tile[1]},
         "autotune_ms": [float(v) for v in tune_ms],
         "ms_per_step_batched": ms_b,
         "ms_per_step_one_launch_per_instance": ms_u,

@@ -133,8 +133,8 @@ _SIGS = {
     "gm_batch_create": (_I, [_I, _G, _M, _P, C.POINTER(_P)]),
     "gm_batch_destroy": (_I, [_P]),
     "gm_batch_step": (_I, [_P, _FP, _I]),
-    "gm_batch_set_zc": (_I, [_P, _I]),
-    "gm_batch_autotune": (_I, [_P, _I, _FP, _IP]),
+    "gm_batch_set_tile": (_I, [_P, _I, _I]),
+    "gm_batch_autotune": (_I, [_P, _I, _FP, _IP, _IP]),
     "gm_batch_time": (_I, [_P, _I, _I, _FP]),
     "gm_batch_field": (_I, [_P, _I, _I, C.POINTER(_P), C.POINTER(_I64)]),
     "gm_batch_refresh": (_I, [_P]),

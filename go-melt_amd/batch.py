@@ -49,14 +49,20 @@ class Batch:
         r = np.ascontiguousarray(rows, np.float32).reshape(self.n, 7)
         N.check(N.lib().gm_batch_step(self.h, N.fptr(r), 1 if batched else 0))
 
-    def set_zc(self, zc):
-        N.check(N.lib().gm_batch_set_zc(self.h, int(zc)))
+    TILES = ((1, 2), (1, 3), (1, 4), (2, 2), (2, 3))
+
+    def set_tile(self, yb, zc):
+        """Brick shape of the batched launch: yb rows x zc planes (TILES)."""
+        N.check(N.lib().gm_batch_set_tile(self.h, int(yb), int(zc)))
 
     def autotune(self, iters=20):
-        ms = np.zeros(3, np.float32)
-        z = C.c_int32()
-        N.check(N.lib().gm_batch_autotune(self.h, int(iters), N.fptr(ms), C.byref(z)))
-        return z.value, ms
+        """Times every brick shape, keeps the fastest; returns ((yb, zc), ms
+        per shape in TILES order)."""
+        ms = np.zeros(len(self.TILES), np.float32)
+        yb, zc = C.c_int32(), C.c_int32()
+        N.check(N.lib().gm_batch_autotune(self.h, int(iters), N.fptr(ms), C.byref(yb),
+                                          C.byref(zc)))
+        return (yb.value, zc.value), ms
 
     def time(self, batched=True, iters=20):
         ms = C.c_float()

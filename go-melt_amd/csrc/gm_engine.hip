@@ -1657,7 +1657,7 @@ struct gm_batch {
     int n = 0;
     hipStream_t s = nullptr;
     bool rec = false;
-    int zc = 3;                       // planes per brick of the batched launch
+    int tile = 1;                     // brick shape of the batched launch (kBatchTiles)
     int cur = 0;                      // T[cur] holds the current field
     std::vector<gm_grid> g;
     DevMat* d_mats = nullptr;         // one per instance (laser constants differ)
@@ -1670,14 +1670,13 @@ struct gm_batch {
     BatchItem* h_ring = nullptr;      // pinned upload slots
     hipEvent_t ring_ev[4] = {};
     int ring_slot = 0;
-    int* d_bstart[3] = {};            // block starts for zc = 2, 3, 4
-    int nblocks[3] = {};
+    int* d_bstart[GM_BATCH_TILES] = {};  // block starts per brick shape
+    int nblocks[GM_BATCH_TILES] = {};
     int maxaxes = 0;
     GmStepDesc* d_desc = nullptr;     // zeroed; the batched step reads no row from it
     std::vector<void*> allocs;
 };
 
-static int zc_slot(int zc) { return zc == 2 ? 0 : (zc == 4 ? 2 : 1); }
 
 extern "C" int gm_batch_destroy(gm_batch* b) {
     if (!b) return 0;
@@ -1783,10 +1782,10 @@ extern "C" int gm_batch_create(int n, const gm_grid* grids, const gm_material* m
     BCHK(balloc(b, &b->d_items, sizeof(BatchItem) * n));
     BCHK(balloc(b, &b->d_desc, sizeof(GmStepDesc)));
     BCHK(hipMemsetAsync(b->d_desc, 0, sizeof(GmStepDesc), b->s));
-    for (int z = 0; z < 3; ++z) {
-        const int zc = z + 2;
+    for (int z = 0; z < GM_BATCH_TILES; ++z) {
         std::vector<int> st(n + 1, 0);
-        for (int i = 0; i < n; ++i) st[i + 1] = st[i] + batch_bricks(grids[i], zc);
+        for (int i = 0; i < n; ++i)
+            st[i + 1] = st[i] + batch_bricks(grids[i], kBatchTiles[z][0], kBatchTiles[z][1]);
         b->nblocks[z] = st[n];
         BCHK(balloc(b, &b->d_bstart[z], sizeof(int) * (n + 1)));
         BCHK(hipMemcpyAsync(b->d_bstart[z], st.data(), sizeof(int) * (n + 1),
@@ -1833,12 +1832,12 @@ static int batch_upload(gm_batch* b, const float* rows, bool scratch_s1) {
 static void batch_launch(gm_batch* b, int batched) {
     launch_batch_src(b->d_items, b->n, b->maxaxes, b->s);
     if (batched) {
-        const int z = zc_slot(b->zc);
-        launch_batch_step(b->d_items, b->d_mats, b->d_bstart[z], b->n, b->nblocks[z], b->zc,
-                          b->rec, b->d_desc, b->s);
+        const int z = b->tile;
+        launch_batch_step(b->d_items, b->d_mats, b->d_bstart[z], b->n, b->nblocks[z], z, b->rec,
+                          b->d_desc, b->s);
     } else {
         for (int i = 0; i < b->n; ++i)
-            launch_step_zc(b->items[i].a, b->d_desc, b->g[i], b->dm[i], b->zc, b->rec, b->s);
+            launch_step_tile(b->items[i].a, b->d_desc, b->g[i], b->dm[i], b->tile, b->rec, b->s);
     }
 }
 
@@ -1851,10 +1850,14 @@ extern "C" int gm_batch_step(gm_batch* b, const float* rows_host, int batched) {
     return 0;
 }
 
-extern "C" int gm_batch_set_zc(gm_batch* b, int zc) {
-    if (!b || (zc != 2 && zc != 3 && zc != 4)) return fail("gm_batch_set_zc: zc must be 2, 3 or 4");
-    b->zc = zc;
-    return 0;
+extern "C" int gm_batch_set_tile(gm_batch* b, int yb, int zc) {
+    if (!b) return fail("gm_batch_set_tile: null batch");
+    for (int t = 0; t < GM_BATCH_TILES; ++t)
+        if (kBatchTiles[t][0] == yb && kBatchTiles[t][1] == zc) {
+            b->tile = t;
+            return 0;
+        }
+    return fail("gm_batch_set_tile: (rows, planes) must be one of (1,2) (1,3) (1,4) (2,2) (2,3)");
 }
 
 // Times `iters` steps from the current state (outputs into the next buffers
@@ -1882,22 +1885,24 @@ extern "C" int gm_batch_time(gm_batch* b, int batched, int iters, float* ms_per_
 // Autotuner: the planes per brick (2, 3 or 4) of the batched launch with the
 // shortest measured step on this batch; ms_host (nullable) gets the three
 // times.  The state does not advance.
-extern "C" int gm_batch_autotune(gm_batch* b, int iters, float* ms_host, int* zc_out) {
+extern "C" int gm_batch_autotune(gm_batch* b, int iters, float* ms_host, int* yb_out,
+                                 int* zc_out) {
     if (!b || iters < 1) return fail("gm_batch_autotune: bad arguments");
     float best = 0.0f;
-    int bz = 3;
-    for (int zc = 2; zc <= 4; ++zc) {
-        b->zc = zc;
+    int bt = 1;
+    for (int t = 0; t < GM_BATCH_TILES; ++t) {
+        b->tile = t;
         float ms = 0.0f;
         if (int rc = gm_batch_time(b, 1, iters, &ms)) return rc;
-        if (ms_host) ms_host[zc - 2] = ms;
-        if (zc == 2 || ms < best) {
+        if (ms_host) ms_host[t] = ms;
+        if (t == 0 || ms < best) {
             best = ms;
-            bz = zc;
+            bt = t;
         }
     }
-    b->zc = bz;
-    if (zc_out) *zc_out = bz;
+    b->tile = bt;
+    if (yb_out) *yb_out = kBatchTiles[bt][0];
+    if (zc_out) *zc_out = kBatchTiles[bt][1];
     return 0;
 }
 

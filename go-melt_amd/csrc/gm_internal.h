@@ -236,12 +236,14 @@ void launch_nn_window_apply(const int32_t* n, const float* T, const float* Tface
                             int C, float T_scale, float T_floor, float* Tn, hipStream_t s);
 
 // batched L3-type steps
-int batch_bricks(const gm_grid& g, int zc);
+#define GM_BATCH_TILES 5
+extern const int kBatchTiles[GM_BATCH_TILES][2];  // {rows per band, planes per brick}
+int batch_bricks(const gm_grid& g, int yb, int zc);
 void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s);
 void launch_batch_step(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
-                       int nblocks, int zc, bool rec, const GmStepDesc* d, hipStream_t s);
-void launch_step_zc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
-                    int zc, bool rec, hipStream_t s);
+                       int nblocks, int tile, bool rec, const GmStepDesc* d, hipStream_t s);
+void launch_step_tile(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
+                      int tile, bool rec, hipStream_t s);
 DevMat make_devmat(const gm_material& m);
 void level_coeffs(const gm_grid& g, float kc[8], float* mrow, float* w2, float* w3);
 

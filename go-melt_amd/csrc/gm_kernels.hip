@@ -732,7 +732,7 @@ __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __r
 // mats[i] is its material (the same as it[i].m);
 // the blocks are dealt out XCD-aware as in the single launch, then each finds
 // its instance (uniform binary search) and runs that instance's brick.
-template <bool REC, int ZC>
+template <bool REC, int YB, int ZC>
 __global__ void __launch_bounds__(64) k_step_dpp_batch(const BatchItem* __restrict__ it,
                                                        const DevMat* __restrict__ mats,
                                                        const int* __restrict__ bstart, int n,
@@ -745,7 +745,6 @@ __global__ void __launch_bounds__(64) k_step_dpp_batch(const BatchItem* __restri
         else hi = mid - 1;
     }
     constexpr bool SRC = true, ACC = false;
-    constexpr int YB = 1;
     const LSArgs a = it[lo].a;
     // materials indexed off the kernel argument (a known global pointer:
     // scalar loads), not through the pointer stored in the item
@@ -1021,10 +1020,13 @@ void launch_level_step_kc(const LSArgs& a0, const GmStepDesc* d, const gm_grid& 
 }
 
 // ---- batched L3-type steps -------------------------------------------------
-int batch_bricks(const gm_grid& g, int zc) {
-    const int nq = g.n[0] * g.n[1];
+int batch_bricks(const gm_grid& g, int yb, int zc) {
+    const int nq = g.n[0] * ((g.n[1] + yb - 1) / yb);
     return ((nq + DXS - 1) / DXS) * ((g.n[2] + zc - 1) / zc);
 }
+
+// brick shapes (rows per band x planes) the batched step offers its autotuner
+const int kBatchTiles[GM_BATCH_TILES][2] = {{1, 2}, {1, 3}, {1, 4}, {2, 2}, {2, 3}};
 
 // separable source factors of each instance's row on its own mesh (as
 // k_src3_tables: x factors times 6 sqrt(3) eta P w3, then y, z); blockIdx.y
@@ -1047,44 +1049,53 @@ void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s) {
         hipLaunchKernelGGL(k_batch_src, dim3((maxaxes + 255) / 256, n), dim3(256), 0, s, it);
 }
 
-template <int ZC>
-static void batch_step_zc(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
-                          int nblocks, bool rec, const GmStepDesc* d, hipStream_t s) {
+template <int YB, int ZC>
+static void batch_step_t(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                         int nblocks, bool rec, const GmStepDesc* d, hipStream_t s) {
     if (rec)
-        hipLaunchKernelGGL((k_step_dpp_batch<true, ZC>), dim3(nblocks), dim3(64), 0, s, it, mats,
-                           bstart, n, d);
+        hipLaunchKernelGGL((k_step_dpp_batch<true, YB, ZC>), dim3(nblocks), dim3(64), 0, s, it,
+                           mats, bstart, n, d);
     else
-        hipLaunchKernelGGL((k_step_dpp_batch<false, ZC>), dim3(nblocks), dim3(64), 0, s, it, mats,
-                           bstart, n, d);
+        hipLaunchKernelGGL((k_step_dpp_batch<false, YB, ZC>), dim3(nblocks), dim3(64), 0, s, it,
+                           mats, bstart, n, d);
 }
 
 void launch_batch_step(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
-                       int nblocks, int zc, bool rec, const GmStepDesc* d, hipStream_t s) {
+                       int nblocks, int tile, bool rec, const GmStepDesc* d, hipStream_t s) {
     if (n <= 0 || nblocks <= 0) return;
-    if (zc == 2) batch_step_zc<2>(it, mats, bstart, n, nblocks, rec, d, s);
-    else if (zc == 4) batch_step_zc<4>(it, mats, bstart, n, nblocks, rec, d, s);
-    else batch_step_zc<3>(it, mats, bstart, n, nblocks, rec, d, s);
+    switch (tile) {
+        case 0: batch_step_t<1, 2>(it, mats, bstart, n, nblocks, rec, d, s); break;
+        case 2: batch_step_t<1, 4>(it, mats, bstart, n, nblocks, rec, d, s); break;
+        case 3: batch_step_t<2, 2>(it, mats, bstart, n, nblocks, rec, d, s); break;
+        case 4: batch_step_t<2, 3>(it, mats, bstart, n, nblocks, rec, d, s); break;
+        default: batch_step_t<1, 3>(it, mats, bstart, n, nblocks, rec, d, s); break;
+    }
 }
 
 // the same step of one instance in a launch of its own (the unbatched form)
-template <int ZC>
-static void step_zc(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
-                    bool rec, hipStream_t s) {
-    const int nb = batch_bricks(hg, ZC);
+template <int YB, int ZC>
+static void step_t(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
+                   bool rec, hipStream_t s) {
+    const int nb = batch_bricks(hg, YB, ZC);
     if (rec)
-        hipLaunchKernelGGL((k_step_dpp<true, false, true, 1, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
-    else
-        hipLaunchKernelGGL((k_step_dpp<true, false, false, 1, ZC>), dim3(nb), dim3(64), 0, s, a, d,
+        hipLaunchKernelGGL((k_step_dpp<true, false, true, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d,
                            m);
+    else
+        hipLaunchKernelGGL((k_step_dpp<true, false, false, YB, ZC>), dim3(nb), dim3(64), 0, s, a,
+                           d, m);
 }
 
-void launch_step_zc(const LSArgs& a0, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
-                    int zc, bool rec, hipStream_t s) {
+void launch_step_tile(const LSArgs& a0, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
+                      int tile, bool rec, hipStream_t s) {
     LSArgs a = a0;
     a.grid = hg;
-    if (zc == 2) step_zc<2>(a, d, hg, m, rec, s);
-    else if (zc == 4) step_zc<4>(a, d, hg, m, rec, s);
-    else step_zc<3>(a, d, hg, m, rec, s);
+    switch (tile) {
+        case 0: step_t<1, 2>(a, d, hg, m, rec, s); break;
+        case 2: step_t<1, 4>(a, d, hg, m, rec, s); break;
+        case 3: step_t<2, 2>(a, d, hg, m, rec, s); break;
+        case 4: step_t<2, 3>(a, d, hg, m, rec, s); break;
+        default: step_t<1, 3>(a, d, hg, m, rec, s); break;
+    }
 }
 
 // k, cp of (T, S1) as the next level step's computeStateProperties returns them;

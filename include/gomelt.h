@@ -321,13 +321,15 @@ int gm_batch_destroy(gm_batch* b);
  * checking and timing). */
 int gm_batch_step(gm_batch* b, const float* rows_host, int batched);
 
-/* Planes per brick of the batched launch (2, 3 or 4; default 3). */
-int gm_batch_set_zc(gm_batch* b, int zc);
+/* Brick shape of the batched launch: yb rows per band x zc planes, one of
+ * (1,2) (1,3) (1,4) (2,2) (2,3); default (1,3). */
+int gm_batch_set_tile(gm_batch* b, int yb, int zc);
 
-/* Autotuner: times `iters` batched steps for each of 2, 3 and 4 planes per
- * brick on this batch, keeps the fastest (zc_out) and returns the three
- * times in ms_host (nullable).  The state does not advance.  Synchronous. */
-int gm_batch_autotune(gm_batch* b, int iters, float* ms_host, int* zc_out);
+/* Autotuner: times `iters` batched steps for each of the five brick shapes
+ * on this batch, keeps the fastest (yb_out, zc_out) and returns the five
+ * times in ms_host (nullable, in the order above).  The state does not
+ * advance.  Synchronous. */
+int gm_batch_autotune(gm_batch* b, int iters, float* ms_host, int* yb_out, int* zc_out);
 
 /* Average ms per step of `iters` steps from the current state, batched or
  * not (the state does not advance).  Synchronous. */

@@ -25,15 +25,15 @@ def _instances(n, seed, sizes=SIZES):
     return W.batch_instances(n=n, seed=seed, sizes=sizes)
 
 
-@pytest.mark.parametrize("zc", [2, 3, 4])
-def test_batched_equals_unbatched(B, zc):
+@pytest.mark.parametrize("tile", [(1, 2), (1, 3), (1, 4), (2, 2), (2, 3)])
+def test_batched_equals_unbatched(B, tile):
     """Every instance of the batched launch equals its own launch bitwise
     (same kernel body), T and S1, over steps that melt."""
     inst = _instances(8, seed=1)
     a = B.Batch.from_instances(inst)
     b = B.Batch.from_instances(inst)
-    a.set_zc(zc)
-    b.set_zc(zc)
+    a.set_tile(*tile)
+    b.set_tile(*tile)
     for s in range(30):
         rows = W.batch_rows(inst, s)
         a.step(rows, batched=True)
@@ -49,20 +49,20 @@ def test_batched_equals_unbatched(B, zc):
     assert melted > 0
 
 
-def test_batch_plane_depths_agree(B):
-    """2, 3 and 4 planes per brick compute the same nodes with the same
-    arithmetic: equal to FP32 rounding (1e-6 of the rise)."""
+def test_batch_brick_shapes_agree(B):
+    """Every brick shape computes the same nodes with the same arithmetic:
+    equal to FP32 rounding (1e-6 of the rise)."""
     inst = _instances(6, seed=2)
     runs = []
-    for zc in (2, 3, 4):
+    for tile in B.Batch.TILES:
         bt = B.Batch.from_instances(inst)
-        bt.set_zc(zc)
+        bt.set_tile(*tile)
         for s in range(6):
             bt.step(W.batch_rows(inst, s))
         runs.append([bt.T(i) for i in range(len(inst))])
     for i in range(len(inst)):
         rise = float(runs[1][i].max() - 300.0)
-        for r in (0, 2):
+        for r in range(len(runs)):
             assert float(np.max(np.abs(runs[r][i] - runs[1][i]))) <= 1e-6 * rise
 
 
@@ -71,8 +71,8 @@ def test_batch_autotune_keeps_a_valid_depth(B):
     bt = B.Batch.from_instances(inst)
     bt.step(W.batch_rows(inst, 0))
     T0 = [bt.T(i) for i in range(len(inst))]
-    zc, ms = bt.autotune(iters=5)
-    assert zc in (2, 3, 4) and np.all(ms > 0) and ms[zc - 2] == ms.min()
+    tile, ms = bt.autotune(iters=5)
+    assert tile in B.Batch.TILES and np.all(ms > 0) and ms[B.Batch.TILES.index(tile)] == ms.min()
     # tuning does not advance the state
     for i in range(len(inst)):
         assert np.array_equal(bt.T(i), T0[i])
