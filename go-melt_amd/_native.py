@@ -149,6 +149,7 @@ _SIGS = {
     "gm_nn_split": (_I, [_I64, _I, _I, _P, _P, _P, _P]),
     "gm_nn_relu_bwd": (_I, [_I64, _P, _P, _P, _P, _P, _P]),
     "gm_nn_bias_grad": (_I, [_I64, _I, _P, _P, _P]),
+    "gm_nn_relu_bias_bwd": (_I, [_I64, _I, _P, _P, _P, _P, _P, _P, _P]),
     "gm_nn_mse": (_I, [_I64, _I, _P, _P, _P, _P, _P]),
     "gm_nn_window_input": (_I, [_IP, _FP, _P, _P, _FP, _P, _P]),
     "gm_nn_window_apply": (_I, [_IP, _P, _P, _P, _I, _F, _F, _P, _P]),

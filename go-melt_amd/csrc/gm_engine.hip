@@ -2050,6 +2050,20 @@ extern "C" int gm_nn_bias_grad(int64_t nvox, int C, const float* dZ, float* db, 
     return 0;
 }
 
+extern "C" int gm_nn_relu_bias_bwd(int64_t nvox, int C, const float* dA, const float* dA2,
+                                   const float* A, float* dZ, uint16_t* dZb, float* db,
+                                   void* stream) {
+    if (nvox < 0 || C < 4 || C > 256 || (C & 3) || (256 % C) || !dA || !db)
+        return fail("gm_nn_relu_bias_bwd: needs dA, db and C a multiple of 4 dividing 256");
+    const void* ptrs[5] = {dA, dA2, A, dZ, dZb};
+    for (int i = 0; i < 5; ++i)
+        if ((uintptr_t)ptrs[i] & (i == 4 ? 7 : 15))
+            return fail("gm_nn_relu_bias_bwd: operands must be 16-byte (dZb 8-byte) aligned");
+    if (nvox) launch_nn_relu_bias_bwd(nvox, C, dA, dA2, A, dZ, dZb, db, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 extern "C" int gm_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,
                          double* loss, void* stream) {
     if (nvox < 0 || C < 1 || !Y || !target || !dY || !loss) return fail("gm_nn_mse: bad arguments");

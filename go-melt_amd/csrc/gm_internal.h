@@ -227,6 +227,8 @@ void launch_nn_split(int64_t nvox, int ca, int cb, const float* dY, float* dA, f
 void launch_nn_relu_bwd(int64_t n, const float* dA, const float* dA2, const float* A, float* dZ,
                         uint16_t* dZb, hipStream_t s);
 void launch_nn_bias_grad(int64_t nvox, int C, const float* dZ, float* db, hipStream_t s);
+void launch_nn_relu_bias_bwd(int64_t nvox, int C, const float* dA, const float* dA2,
+                             const float* A, float* dZ, uint16_t* dZb, float* db, hipStream_t s);
 void launch_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,
                    double* loss, hipStream_t s);
 void launch_nn_window_input(const int32_t* n, const float* geo, const float* T, const float* Tface,

@@ -2796,9 +2796,9 @@ __global__ void __launch_bounds__(256) k_energy(gm_grid g, int64_t ne_active, En
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
     const int64_t nn = (int64_t)nx * ny * nz;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (t < nn) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nn; t += stride) {
         int ix, iy, iz;
         dec3(t, nx, ny, ix, iy, iz);
         double KT = 0.0, C = 0.0;
@@ -2828,25 +2828,34 @@ __global__ void __launch_bounds__(256) k_energy(gm_grid g, int64_t ne_active, En
                               ((bc_faces & 4) && iy == 0) || ((bc_faces & 8) && iy == ney) ||
                               ((bc_faces & 16) && iz == 0);
             const bool held = face || (Tni <= (double)T_clamp && Tt < (double)T_clamp);
+            const double dE = held ? C * (Tt - Ti) : C * (Tni - Ti);
             if (held) {
-                v[1] = C * (Tt - Ti);
-                v[4] = C * (Tni - Tt);
-                v[7] = 1.0;
+                v[1] += dE;
+                v[4] += C * (Tni - Tt);
+                v[7] += 1.0;
             } else {
-                v[0] = C * (Tni - Ti);
+                v[0] += dE;
             }
-            v[2] = dt * Fi;
-            v[3] = dt * KT;
-            v[5] = fabs(dt * Fi) + fabs(held ? v[1] : v[0]);
+            v[2] += dt * Fi;
+            v[3] += dt * KT;
+            v[5] += fabs(dt * Fi) + fabs(dE);
         }
     }
+    // the eight sums share one cache line, so atomics on them serialise:
+    // fold the block's four waves in LDS and add once per block
+    __shared__ double part[4][8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        if (q == 6) continue;
         double x = v[q];
 #pragma unroll
         for (int sh = 32; sh >= 1; sh >>= 1) x += __shfl_xor(x, sh, 64);
-        if ((threadIdx.x & 63) == 0 && x != 0.0) atomicAdd(sums + q, x);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][q] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && threadIdx.x != 6) {
+        const int q = threadIdx.x;
+        const double x = part[0][q] + part[1][q] + part[2][q] + part[3][q];
+        if (x != 0.0) atomicAdd(sums + q, x);
     }
 }
 
@@ -2880,7 +2889,8 @@ void launch_energy(const gm_grid& g, int64_t ne_active, const float* k, const fl
     const int64_t nn = (int64_t)g.n[0] * g.n[1] * g.n[2];
     hipLaunchKernelGGL(k_energy_init, dim3(1), dim3(64), 0, s, sums);
     if (nn > 0)
-        hipLaunchKernelGGL(k_energy, dim3(nblk(nn)), dim3(256), 0, s, g, ne_active, energy_coef(g),
+        hipLaunchKernelGGL(k_energy, dim3((unsigned)std::min<int64_t>(nblk(nn), 512)), dim3(256), 0,
+                           s, g, ne_active, energy_coef(g),
                            k, rc, T, F, Corr, Tn, bc_faces, T_clamp, (double)dt, sums);
     hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(64), 0, s, sums);
 }

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "gm_internal.h"
 
 namespace gm {
@@ -162,6 +164,66 @@ __global__ void k_bias_grad(int64_t nvox, int C, const float* __restrict__ dZ,
     }
 }
 
+// ReLU backward fused with the bias gradient, for C a multiple of 4 that
+// divides 256 and 16-byte aligned operands.  Each thread takes float4 groups
+// 1024 x gridDim.x apart, so its four channels stay fixed (4 x lane mod C):
+// it sums them in registers, the wave folds lanes holding the same channels
+// (xor over lane bits >= log2(C / 4)), the 16 waves meet in LDS and the block
+// adds one atomic per channel.  dZb = bf16(g), dZ = g when given, with
+// g = (dA + dA2) where A > 0, else 0; db[c] += sum over voxels of g[v][c].
+__global__ void __launch_bounds__(1024) k_relu_bias_bwd(int64_t n4, int C,
+                                                        const float4* __restrict__ dA,
+                                                        const float4* __restrict__ dA2,
+                                                        const float4* __restrict__ A,
+                                                        float4* __restrict__ dZ,
+                                                        uint2* __restrict__ dZb,
+                                                        float* __restrict__ db) {
+    __shared__ float part[16][256];
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    const int64_t stride = (int64_t)gridDim.x * 1024;
+    for (int64_t t = (int64_t)blockIdx.x * 1024 + threadIdx.x; t < n4; t += stride) {
+        const float4 a = dA[t];
+        const float4 b = dA2 ? dA2[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 g = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+        if (A) {
+            const float4 m = A[t];
+            if (!(m.x > 0.0f)) g.x = 0.0f;
+            if (!(m.y > 0.0f)) g.y = 0.0f;
+            if (!(m.z > 0.0f)) g.z = 0.0f;
+            if (!(m.w > 0.0f)) g.w = 0.0f;
+        }
+        if (dZ) dZ[t] = g;
+        if (dZb)
+            dZb[t] = make_uint2((uint32_t)nn_f2bf(g.x) | ((uint32_t)nn_f2bf(g.y) << 16),
+                                (uint32_t)nn_f2bf(g.z) | ((uint32_t)nn_f2bf(g.w) << 16));
+        s0 += g.x;
+        s1 += g.y;
+        s2 += g.z;
+        s3 += g.w;
+    }
+    const int lanes = C >> 2;  // lanes per channel period
+    for (int sh = 32; sh >= lanes; sh >>= 1) {
+        s0 += __shfl_xor(s0, sh, 64);
+        s1 += __shfl_xor(s1, sh, 64);
+        s2 += __shfl_xor(s2, sh, 64);
+        s3 += __shfl_xor(s3, sh, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane < lanes) {
+        part[w][4 * lane + 0] = s0;
+        part[w][4 * lane + 1] = s1;
+        part[w][4 * lane + 2] = s2;
+        part[w][4 * lane + 3] = s3;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < C) {
+        float tot = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) tot += part[k][threadIdx.x];
+        atomicAdd(db + threadIdx.x, tot);
+    }
+}
+
 // Loss on channel 0 of the head output Y (nvox x C): 0.5 mean (Y0 - target)^2
 // summed in FP64 (loss[0] += ...); dY (FP32, nvox x C) = (Y0 - target) / nvox
 // in channel 0 and zero elsewhere
@@ -272,6 +334,15 @@ void launch_nn_relu_bwd(int64_t n, const float* dA, const float* dA2, const floa
 void launch_nn_bias_grad(int64_t nvox, int C, const float* dZ, float* db, hipStream_t s) {
     const dim3 grid((unsigned)((nvox + 255) / 256), (unsigned)((C + 63) / 64));
     hipLaunchKernelGGL(k_bias_grad, grid, dim3(256), 0, s, nvox, C, dZ, db);
+}
+
+void launch_nn_relu_bias_bwd(int64_t nvox, int C, const float* dA, const float* dA2,
+                             const float* A, float* dZ, uint16_t* dZb, float* db, hipStream_t s) {
+    const int64_t n4 = nvox * C / 4;
+    const int64_t blocks = std::min<int64_t>((n4 + 1023) / 1024, 256);
+    hipLaunchKernelGGL(k_relu_bias_bwd, dim3((unsigned)blocks), dim3(1024), 0, s, n4, C,
+                       (const float4*)dA, (const float4*)dA2, (const float4*)A, (float4*)dZ,
+                       (uint2*)dZb, db);
 }
 
 void launch_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,

@@ -184,6 +184,26 @@ def bias_grad(dZ, db):
                                     _stream()))
 
 
+def relu_bias_bwd(dA, A, db, dA2=None):
+    """ReLU backward and bias gradient in one pass: returns the bf16 dZ and
+    adds its per-channel sums (of the FP32 values) to db.  Channel counts the
+    fused kernel does not take go through relu_bwd and bias_grad."""
+    _same(dA, A, "relu_bias_bwd")
+    _same(dA, dA2, "relu_bias_bwd")
+    nc = dA.shape[-1]
+    if db.numel() != nc:
+        raise ValueError("relu_bias_bwd: one bias per channel")
+    aligned = all(t is None or t.data_ptr() % 16 == 0 for t in (dA, dA2, A))
+    if nc % 4 or 256 % nc or not aligned:
+        dZ, dZb = relu_bwd(dA, A, dA2)
+        bias_grad(dZ, db)
+        return dZb
+    dZb = torch.empty_like(dA, dtype=torch.bfloat16)
+    N.check(N.lib().gm_nn_relu_bias_bwd(dA.numel() // nc, nc, _ptr(dA), _ptr(dA2), _ptr(A), None,
+                                        _ptr(dZb), _ptr(db), _stream()))
+    return dZb
+
+
 def mse(Y, target, loss):
     """loss (FP64 device scalar) += 0.5 mean (Y[..., 0] - target)^2; returns
     the FP32 gradient wrt Y."""
@@ -222,8 +242,7 @@ class Conv:
     def backward(self, dA, dA2=None, need_input_grad=True):
         """dA: FP32 gradient wrt this layer's output; returns the FP32
         gradient wrt its input (or None)."""
-        dZ, dZb = relu_bwd(dA, self.A if self.relu else None, dA2)
-        bias_grad(dZ, self.grads[1])
+        dZb = relu_bias_bwd(dA, self.A if self.relu else None, self.grads[1], dA2)
         conv3d_bwd_weight(self.Xb, dZb, self.grads[0])
         if not need_input_grad:
             return None

@@ -399,6 +399,13 @@ int gm_nn_relu_bwd(int64_t n, const float* dA, const float* dA2, const float* A,
                    uint16_t* dZb, void* stream);
 /* db[c] += sum over voxels of dZ[v][c]. */
 int gm_nn_bias_grad(int64_t nvox, int C, const float* dZ, float* db, void* stream);
+/* gm_nn_relu_bwd and gm_nn_bias_grad in one pass over nvox x C values:
+ * g = (dA + dA2) where A > 0, else 0 (dA2, A nullable); dZ = g and dZb =
+ * bf16(g) when given (both nullable); db[c] += sum over voxels of g[v][c].
+ * C must be a multiple of 4 that divides 256; dA, dA2, A, dZ 16-byte and
+ * dZb 8-byte aligned.  Sums in a different order from gm_nn_bias_grad. */
+int gm_nn_relu_bias_bwd(int64_t nvox, int C, const float* dA, const float* dA2,
+                        const float* A, float* dZ, uint16_t* dZb, float* db, void* stream);
 /* loss[0] += 0.5 mean_v (Y[v][0] - target[v])^2 (FP64); dY (FP32, nvox x C)
  * = (Y[v][0] - target[v]) / nvox in channel 0, zero elsewhere. */
 int gm_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,

@@ -77,6 +77,59 @@ def test_network_elementwise_kernels(S):
     assert np.all(dY[:, 1:] == 0)
 
 
+@pytest.mark.parametrize("nc,nvox", [(16, 5000), (32, 70000), (128, 3000), (256, 1100), (4, 777)])
+@pytest.mark.parametrize("with_a,with_a2", [(True, True), (True, False), (False, False)])
+def test_relu_bias_fused(S, nc, nvox, with_a, with_a2):
+    """The fused ReLU backward + bias gradient against the two separate
+    kernels: identical bf16 and FP32 dZ, bias sums equal to FP32 rounding
+    (they add in a different order, and atomically) and to the FP64 sum of
+    dZ."""
+    import ctypes
+    import torch
+    import _native as N
+    rng = np.random.default_rng(nc + nvox)
+    dA = torch.from_numpy(rng.standard_normal((nvox, nc)).astype(np.float32)).cuda()
+    dA2 = torch.from_numpy(rng.standard_normal((nvox, nc)).astype(np.float32)).cuda() if with_a2 else None
+    A = (torch.from_numpy(np.maximum(rng.standard_normal((nvox, nc)), 0).astype(np.float32)).cuda()
+         if with_a else None)
+    db0 = torch.from_numpy(np.linspace(-1, 1, nc).astype(np.float32)).cuda()
+    dZ_ref, dZb_ref = S.relu_bwd(dA, A, dA2)
+    db_ref = db0.clone()
+    S.bias_grad(dZ_ref, db_ref)
+    db = db0.clone()
+    dZb = S.relu_bias_bwd(dA, A, db, dA2)
+    np.testing.assert_array_equal(_np(dZb), _np(dZb_ref))
+    np.testing.assert_allclose(_np(db), _np(db_ref), rtol=2e-5, atol=2e-4)
+    exact = _np(db0).astype(np.float64) + _np(dZ_ref).astype(np.float64).sum(0)
+    np.testing.assert_allclose(_np(db), exact, rtol=2e-5, atol=2e-4)
+    # the FP32 dZ output of the C entry point, called directly
+    dZ = torch.empty_like(dA)
+    db2 = db0.clone()
+    p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    N.check(N.lib().gm_nn_relu_bias_bwd(nvox, nc, p(dA), p(dA2), p(A), p(dZ), None, p(db2), st))
+    np.testing.assert_array_equal(_np(dZ), _np(dZ_ref))
+    # up to 256 blocks add their partial sums atomically, in no fixed order:
+    # each add rounds at half an ulp of the running sum
+    np.testing.assert_allclose(_np(db2), _np(db), rtol=2e-5, atol=2e-4)
+
+
+def test_relu_bias_fused_rejects(S):
+    """Channel counts outside the fused kernel's layout are refused by the
+    C entry point; the Python wrapper routes them through the two kernels."""
+    import ctypes
+    import torch
+    import _native as N
+    dA = torch.ones((10, 48), dtype=torch.float32).cuda()
+    db = torch.zeros(48, dtype=torch.float32).cuda()
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert N.lib().gm_nn_relu_bias_bwd(10, 48, p(dA), None, None, None, None, p(db), st) != 0
+    dZb = S.relu_bias_bwd(dA, None, db)
+    np.testing.assert_array_equal(_np(db), np.full(48, 10.0, np.float32))
+    np.testing.assert_array_equal(_np(dZb), np.ones((10, 48), np.float32))
+
+
 def test_unet_gradient_direction(S):
     """Directional derivative of the loss through the whole U-Net: the
     kernels' gradient g predicts L(w + e d) - L(w - e d) ~ 2 e <g, d> for a

@@ -32,6 +32,10 @@ class _Shapes:
     def bias_grad(self, dZ, db):
         assert db.numel() == dZ.shape[-1]
 
+    def relu_bias_bwd(self, dA, A, db, dA2=None):
+        self.bias_grad(dA, db)
+        return self.relu_bwd(dA, A, dA2)[1]
+
     def pool2(self, X):
         assert all(s % 2 == 0 for s in X.shape[1:4])
         return self.t.empty(X.shape[:1] + tuple(s // 2 for s in X.shape[1:4]) + X.shape[4:])
@@ -70,7 +74,8 @@ def test_unet_pass_shapes(monkeypatch, channels, window):
     import surrogate as S
     f = _Shapes(torch)
     for name in ("conv3d", "conv3d_bwd_data", "conv3d_bwd_weight", "relu_bwd", "bias_grad",
-                 "pool2", "pool2_bwd", "up2", "up2_bwd", "concat", "split", "mse", "to_bf16"):
+                 "relu_bias_bwd", "pool2", "pool2_bwd", "up2", "up2_bwd", "concat", "split", "mse",
+                 "to_bf16"):
         monkeypatch.setattr(S, name, getattr(f, name))
     monkeypatch.setattr(S, "_zeros", f.zeros)
     monkeypatch.setattr(torch.Tensor, "cuda", lambda self, *a, **k: self)
