@@ -19,6 +19,7 @@ __device__ __forceinline__ uint16_t nn_f2bf(float f) {
 }
 
 static inline unsigned nn_blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+static inline bool nn_al16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 // 2x2x2 average pooling: Y[b, z, y, x, c] = mean of X over the 8 fine voxels
 // (fine extent 2D x 2H x 2W); dir 0 forward (bf16 -> bf16), dir 1 backward
@@ -46,8 +47,16 @@ __global__ void k_pool2(int nb, int D, int H, int W, int C, const uint16_t* __re
     Y[t] = nn_f2bf(0.125f * s);
 }
 
-__global__ void k_pool2_bwd(int nb, int D, int H, int W, int C, const float* __restrict__ dY,
-                            float* __restrict__ dX) {
+// 1/8 of a gradient value, one float or four
+__device__ __forceinline__ float nn_eighth(float x) { return 0.125f * x; }
+__device__ __forceinline__ float4 nn_eighth(float4 x) {
+    return make_float4(0.125f * x.x, 0.125f * x.y, 0.125f * x.z, 0.125f * x.w);
+}
+
+// V is float, or float4 with C counted in groups of four channels
+template <typename V>
+__global__ void k_pool2_bwd(int nb, int D, int H, int W, int C, const V* __restrict__ dY,
+                            V* __restrict__ dX) {
     // one thread per fine element
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int W2 = 2 * W, H2 = 2 * H, D2 = 2 * D;
@@ -61,13 +70,15 @@ __global__ void k_pool2_bwd(int nb, int D, int H, int W, int C, const float* __r
     v /= H2;
     const int z = (int)(v % D2);
     const int b = (int)(v / D2);
-    dX[t] = 0.125f * dY[((((int64_t)b * D + z / 2) * H + y / 2) * W + x / 2) * C + c];
+    dX[t] = nn_eighth(dY[((((int64_t)b * D + z / 2) * H + y / 2) * W + x / 2) * C + c]);
 }
 
-// nearest 2x upsampling (coarse extent D, H, W): forward bf16 -> bf16;
-// backward sums the 8 fine gradients (FP32)
-__global__ void k_up2(int nb, int D, int H, int W, int C, const uint16_t* __restrict__ X,
-                      uint16_t* __restrict__ Y) {
+// nearest 2x upsampling (coarse extent D, H, W): forward bf16 -> bf16 (V is
+// one bf16, or uint4 = eight with C counted in groups of eight); backward
+// sums the 8 fine gradients (FP32)
+template <typename V>
+__global__ void k_up2(int nb, int D, int H, int W, int C, const V* __restrict__ X,
+                      V* __restrict__ Y) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int W2 = 2 * W, H2 = 2 * H, D2 = 2 * D;
     const int64_t n = (int64_t)nb * D2 * H2 * W2 * C;
@@ -106,9 +117,12 @@ __global__ void k_up2_bwd(int nb, int D, int H, int W, int C, const float* __res
     dX[t] = s;
 }
 
-// channel concatenation Y = [A | B] (bf16) and its backward split (FP32)
-__global__ void k_concat(int64_t nvox, int ca, int cb, const uint16_t* __restrict__ A,
-                         const uint16_t* __restrict__ B, uint16_t* __restrict__ Y) {
+// channel concatenation Y = [A | B] (bf16) and its backward split (FP32);
+// V is one value, or 16 bytes of them (uint4: eight bf16, float4: four
+// floats) with the channel counts in those groups
+template <typename V>
+__global__ void k_concat(int64_t nvox, int ca, int cb, const V* __restrict__ A,
+                         const V* __restrict__ B, V* __restrict__ Y) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int cc = ca + cb;
     if (t >= nvox * cc) return;
@@ -117,8 +131,9 @@ __global__ void k_concat(int64_t nvox, int ca, int cb, const uint16_t* __restric
     Y[t] = c < ca ? A[v * ca + c] : B[v * cb + (c - ca)];
 }
 
-__global__ void k_split(int64_t nvox, int ca, int cb, const float* __restrict__ dY,
-                        float* __restrict__ dA, float* __restrict__ dB) {
+template <typename V>
+__global__ void k_split(int64_t nvox, int ca, int cb, const V* __restrict__ dY,
+                        V* __restrict__ dA, V* __restrict__ dB) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int cc = ca + cb;
     if (t >= nvox * cc) return;
@@ -227,20 +242,28 @@ __global__ void __launch_bounds__(1024) k_relu_bias_bwd(int64_t n4, int C,
 // Loss on channel 0 of the head output Y (nvox x C): 0.5 mean (Y0 - target)^2
 // summed in FP64 (loss[0] += ...); dY (FP32, nvox x C) = (Y0 - target) / nvox
 // in channel 0 and zero elsewhere
-__global__ void k_mse(int64_t nvox, int C, const float* __restrict__ Y,
-                      const float* __restrict__ target, float* __restrict__ dY,
-                      double* __restrict__ loss) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_mse(int64_t nvox, int C, const float* __restrict__ Y,
+                                             const float* __restrict__ target,
+                                             float* __restrict__ dY, double* __restrict__ loss) {
+    // a grid of at most 1024 blocks; each folds its four waves in LDS and
+    // adds once (atomics on one address serialise)
+    __shared__ double part[4];
     double l = 0.0;
-    if (t < nvox) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nvox; t += stride) {
         const float r = Y[t * C] - target[t];
-        l = 0.5 * (double)r * (double)r / (double)nvox;
+        l += 0.5 * (double)r * (double)r / (double)nvox;
         dY[t * C] = r / (float)nvox;
         for (int c = 1; c < C; ++c) dY[t * C + c] = 0.0f;
     }
 #pragma unroll
     for (int sh = 32; sh >= 1; sh >>= 1) l += __shfl_xor(l, sh, 64);
-    if ((threadIdx.x & 63) == 0 && l != 0.0) atomicAdd(loss, l);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = part[0] + part[1] + part[2] + part[3];
+        if (b != 0.0) atomicAdd(loss, b);
+    }
 }
 
 // Network input of one window (nx x ny x nz nodes, x fastest), 16 channels:
@@ -294,18 +317,26 @@ void launch_nn_pool2(int dir, int nb, int D, int H, int W, int C, const void* in
         const int64_t n = (int64_t)nb * D * H * W * C;
         hipLaunchKernelGGL(k_pool2, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W, C,
                            (const uint16_t*)in, (uint16_t*)out);
+    } else if (C % 4 == 0 && nn_al16(in) && nn_al16(out)) {
+        const int64_t n = (int64_t)nb * 8 * D * H * W * (C / 4);
+        hipLaunchKernelGGL(k_pool2_bwd<float4>, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W,
+                           C / 4, (const float4*)in, (float4*)out);
     } else {
         const int64_t n = (int64_t)nb * 8 * D * H * W * C;
-        hipLaunchKernelGGL(k_pool2_bwd, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W, C,
-                           (const float*)in, (float*)out);
+        hipLaunchKernelGGL(k_pool2_bwd<float>, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W,
+                           C, (const float*)in, (float*)out);
     }
 }
 
 void launch_nn_up2(int dir, int nb, int D, int H, int W, int C, const void* in, void* out,
                    hipStream_t s) {
-    if (dir == 0) {
+    if (dir == 0 && C % 8 == 0 && nn_al16(in) && nn_al16(out)) {
+        const int64_t n = (int64_t)nb * 8 * D * H * W * (C / 8);
+        hipLaunchKernelGGL(k_up2<uint4>, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W, C / 8,
+                           (const uint4*)in, (uint4*)out);
+    } else if (dir == 0) {
         const int64_t n = (int64_t)nb * 8 * D * H * W * C;
-        hipLaunchKernelGGL(k_up2, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W, C,
+        hipLaunchKernelGGL(k_up2<uint16_t>, dim3(nn_blocks(n)), dim3(256), 0, s, nb, D, H, W, C,
                            (const uint16_t*)in, (uint16_t*)out);
     } else {
         const int64_t n = (int64_t)nb * D * H * W * C;
@@ -316,14 +347,22 @@ void launch_nn_up2(int dir, int nb, int D, int H, int W, int C, const void* in, 
 
 void launch_nn_concat(int64_t nvox, int ca, int cb, const uint16_t* A, const uint16_t* B,
                       uint16_t* Y, hipStream_t s) {
-    hipLaunchKernelGGL(k_concat, dim3(nn_blocks(nvox * (ca + cb))), dim3(256), 0, s, nvox, ca, cb,
-                       A, B, Y);
+    if (ca % 8 == 0 && cb % 8 == 0 && nn_al16(A) && nn_al16(B) && nn_al16(Y))
+        hipLaunchKernelGGL(k_concat<uint4>, dim3(nn_blocks(nvox * (ca + cb) / 8)), dim3(256), 0, s,
+                           nvox, ca / 8, cb / 8, (const uint4*)A, (const uint4*)B, (uint4*)Y);
+    else
+        hipLaunchKernelGGL(k_concat<uint16_t>, dim3(nn_blocks(nvox * (ca + cb))), dim3(256), 0, s,
+                           nvox, ca, cb, A, B, Y);
 }
 
 void launch_nn_split(int64_t nvox, int ca, int cb, const float* dY, float* dA, float* dB,
                      hipStream_t s) {
-    hipLaunchKernelGGL(k_split, dim3(nn_blocks(nvox * (ca + cb))), dim3(256), 0, s, nvox, ca, cb,
-                       dY, dA, dB);
+    if (ca % 4 == 0 && cb % 4 == 0 && nn_al16(dY) && nn_al16(dA) && nn_al16(dB))
+        hipLaunchKernelGGL(k_split<float4>, dim3(nn_blocks(nvox * (ca + cb) / 4)), dim3(256), 0, s,
+                           nvox, ca / 4, cb / 4, (const float4*)dY, (float4*)dA, (float4*)dB);
+    else
+        hipLaunchKernelGGL(k_split<float>, dim3(nn_blocks(nvox * (ca + cb))), dim3(256), 0, s,
+                           nvox, ca, cb, dY, dA, dB);
 }
 
 void launch_nn_relu_bwd(int64_t n, const float* dA, const float* dA2, const float* A, float* dZ,
@@ -347,8 +386,8 @@ void launch_nn_relu_bias_bwd(int64_t nvox, int C, const float* dA, const float* 
 
 void launch_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,
                    double* loss, hipStream_t s) {
-    hipLaunchKernelGGL(k_mse, dim3(nn_blocks(nvox)), dim3(256), 0, s, nvox, C, Y, target, dY,
-                       loss);
+    hipLaunchKernelGGL(k_mse, dim3(std::min(nn_blocks(nvox), 1024u)), dim3(256), 0, s, nvox, C, Y,
+                       target, dY, loss);
 }
 
 void launch_nn_window_input(const int32_t* n, const float* geo, const float* T, const float* Tface,

@@ -77,6 +77,35 @@ def test_network_elementwise_kernels(S):
     assert np.all(dY[:, 1:] == 0)
 
 
+def test_network_copies_odd_channels(S):
+    """Channel counts that the 16-byte copy kernels do not take (not a
+    multiple of 8 bf16 or 4 floats) run the one-value kernels: same results."""
+    import torch
+    from oracle import conv_oracle as CO
+    rng = np.random.default_rng(3)
+    X = CO.bf16_round(rng.standard_normal((2, 2, 3, 4, 6)).astype(np.float32))
+    U = _np(S.up2(_bf(X)))
+    np.testing.assert_array_equal(U, np.repeat(np.repeat(np.repeat(X, 2, 1), 2, 2), 2, 3))
+    g = rng.standard_normal((2, 2, 3, 4, 6)).astype(np.float32)
+    dX = _np(S.pool2_bwd(torch.from_numpy(g).cuda()))
+    np.testing.assert_array_equal(dX, np.repeat(np.repeat(np.repeat(g, 2, 1), 2, 2), 2, 3) / 8)
+    B = CO.bf16_round(rng.standard_normal((2, 2, 3, 4, 5)).astype(np.float32))
+    np.testing.assert_array_equal(_np(S.concat(_bf(X), _bf(B))), np.concatenate([X, B], axis=-1))
+    gc = rng.standard_normal((2, 2, 3, 4, 11)).astype(np.float32)
+    da, db = S.split(torch.from_numpy(gc).cuda(), 6)
+    np.testing.assert_array_equal(_np(da), gc[..., :6])
+    np.testing.assert_array_equal(_np(db), gc[..., 6:])
+    # a large loss: many blocks, each adding once
+    Y = rng.standard_normal((300000, 16)).astype(np.float32)
+    T = rng.standard_normal(300000).astype(np.float32)
+    loss = torch.from_numpy(np.zeros(1)).cuda()
+    dY = _np(S.mse(torch.from_numpy(Y).cuda(), torch.from_numpy(T).cuda(), loss))
+    r = Y[:, 0].astype(np.float64) - T
+    assert float(loss.item()) == pytest.approx(0.5 * np.mean(r * r), rel=1e-6)
+    np.testing.assert_allclose(dY[:, 0], r / 300000, rtol=1e-6)
+    assert np.all(dY[:, 1:] == 0)
+
+
 @pytest.mark.parametrize("nc,nvox", [(16, 5000), (32, 70000), (128, 3000), (256, 1100), (4, 777)])
 @pytest.mark.parametrize("with_a,with_a2", [(True, True), (True, False), (False, False)])
 def test_relu_bias_fused(S, nc, nvox, with_a, with_a2):
