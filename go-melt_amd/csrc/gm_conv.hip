@@ -1,5 +1,6 @@
 // 3x3x3 convolutions for the learned L3 operator: BF16 operands on the MFMA
-// matrix cores (v_mfma_f32_16x16x16_bf16), FP32 accumulation.
+// matrix cores (v_mfma_f32_16x16x32_bf16 in the forward kernel for inputs in
+// 32-channel stages, v_mfma_f32_16x16x16_bf16 elsewhere), FP32 accumulation.
 //
 // Layouts (channels last):
 //   activations  X[vox][C], vox = ((b*D + z)*H + y)*W + x, bf16 (uint16 bits)
@@ -27,6 +28,7 @@ namespace gm {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -146,9 +148,10 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvDims c, const uint16_t* __
 }
 
 // The same convolution for cin a multiple of 32, software-pipelined: a K
-// stage is one tap x 32 input channels (two MFMA k-steps, 8 MFMAs per wave
-// between barriers), each thread stages 16 bytes of A and of B, and the next
-// stage's global loads are in flight while the current stage computes.
+// stage is one tap x 32 input channels (one v_mfma_f32_16x16x32_bf16 k-step,
+// 4 MFMAs per wave between barriers), each thread stages 16 bytes of A and of
+// B, and the next stage's global loads are in flight while the current stage
+// computes.
 template <int BM, int BN>
 __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* __restrict__ X,
                                                     const uint16_t* __restrict__ Wk,
@@ -194,7 +197,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
     };
     uint4 av[NA], bv;
     fetch(0, av, bv);
-    const int fi = lane & 15, fk = (lane >> 4) * 4;
+    const int fi = lane & 15, fk8 = (lane >> 4) * 8;
     for (int st = 0; st < nst; ++st) {
         __syncthreads();
 #pragma unroll
@@ -202,21 +205,20 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
         if (srow < BN) *(uint4*)&sB[srow][sk] = bv;
         __syncthreads();
         if (st + 1 < nst) fetch(st + 1, av, bv);
+        // the stage's 32 channels in one K = 32 MFMA: each lane gives 8
+        // consecutive channels of its row (A) and column (B), the same k
+        // positions for both operands
+        v8bf fa[2], fb[2];
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            v4s fa[2], fb[2];
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                fa[t] = __builtin_bit_cast(v4s, *(const uint2*)&sA[wm + t * 16 + fi][ks * 16 + fk]);
-                fb[t] = __builtin_bit_cast(v4s, *(const uint2*)&sB[wn + t * 16 + fi][ks * 16 + fk]);
-            }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[a], fb[b], acc[a][b],
-                                                                          0, 0, 0);
+        for (int t = 0; t < 2; ++t) {
+            fa[t] = __builtin_bit_cast(v8bf, *(const uint4*)&sA[wm + t * 16 + fi][fk8]);
+            fb[t] = __builtin_bit_cast(v8bf, *(const uint4*)&sB[wn + t * 16 + fi][fk8]);
         }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
     }
     const int dj = lane & 15, di = (lane >> 4) * 4;
 #pragma unroll
