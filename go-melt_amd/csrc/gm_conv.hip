@@ -323,43 +323,58 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(ConvDims c, const uint16_t* 
 // load): stages of 32 voxels staged voxel-major ([vox][channel], 16-byte LDS
 // writes, no transpose), the next stage's loads in flight during the MFMAs;
 // a lane's fragment (4 consecutive voxels of one channel) is 4 LDS reads.
+// Tiles of OT outputs x 4096 / OT inputs: 64 x 64 (waves 2 x 2), or 32 x 128
+// (waves along the inputs) so that a 32-output layer fills whole tiles.
+template <int OT>
 __global__ void __launch_bounds__(256) k_conv_wgrad32(ConvDims c, const uint16_t* __restrict__ X,
                                                       const uint16_t* __restrict__ dY, int vsplit,
                                                       float* __restrict__ dW) {
-    __shared__ uint16_t sA[32][64 + 8];  // [vox][o]
-    __shared__ uint16_t sB[32][64 + 8];  // [vox][i]
+    constexpr int IT = 4096 / OT;  // inputs per tile
+    constexpr int NX = IT / 64;    // X loads per thread and stage
+    __shared__ uint16_t sA[32][OT + 8];  // [vox][o]
+    __shared__ uint16_t sB[32][IT + 8];  // [vox][i]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int o0 = blockIdx.x * 64, i0 = blockIdx.y * 64;
+    const int o0 = blockIdx.x * OT, i0 = blockIdx.y * IT;
     const int tap = blockIdx.z % 27, part = blockIdx.z / 27;
     const int64_t per = (c.nvox + vsplit - 1) / vsplit;
     const int64_t vbeg = (int64_t)part * per;
     const int64_t vend = vbeg + per < c.nvox ? vbeg + per : c.nvox;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int wm = OT == 64 ? (wave & 1) * 32 : 0, wn = OT == 64 ? (wave >> 1) * 32 : wave * 32;
     v4f acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    // staging: thread t loads 8 channels (16 bytes) of voxel t/8, channels
-    // (t%8)*8 .. +7 of the 64-wide tile, for dY and for the shifted X
-    const int sv = tid >> 3, sc = (tid & 7) * 8;
-    auto fetch = [&](int64_t vb, uint4& dv, uint4& xv) {
-        const int64_t v = vb + sv;
+    // staging: 16-byte chunks (8 channels) of the stage's 32 voxels, chunk q
+    // = voxel q / (T/8), channels (q % (T/8)) * 8 of the T-wide tile: dY has
+    // 4 OT chunks (threads below 4 OT), the shifted X 4 IT (NX per thread)
+    const int sva = tid / (OT / 8), sca = (tid % (OT / 8)) * 8;
+    auto fetch = [&](int64_t vb, uint4& dv, uint4* xv) {
         dv = make_uint4(0u, 0u, 0u, 0u);
-        xv = make_uint4(0u, 0u, 0u, 0u);
-        if (v < vend) {
-            if (o0 + sc < c.co) dv = *(const uint4*)(dY + v * c.co + o0 + sc);
-            const int64_t vs = vox_shift(c, vox_decode(c, v), tap);
-            if (vs >= 0 && i0 + sc < c.ci) xv = *(const uint4*)(X + vs * c.ci + i0 + sc);
+        if (tid < 4 * OT && vb + sva < vend && o0 + sca < c.co)
+            dv = *(const uint4*)(dY + (vb + sva) * c.co + o0 + sca);
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int q = tid + 256 * k, svb = q / (IT / 8), scb = (q % (IT / 8)) * 8;
+            const int64_t v = vb + svb;
+            xv[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (v < vend && i0 + scb < c.ci) {
+                const int64_t vs = vox_shift(c, vox_decode(c, v), tap);
+                if (vs >= 0) xv[k] = *(const uint4*)(X + vs * c.ci + i0 + scb);
+            }
         }
     };
-    uint4 dv, xv;
+    uint4 dv, xv[NX];
     fetch(vbeg, dv, xv);
     const int fi = lane & 15, fk = (lane >> 4) * 4;
     for (int64_t vb = vbeg; vb < vend; vb += 32) {
         __syncthreads();
-        *(uint4*)&sA[sv][sc] = dv;
-        *(uint4*)&sB[sv][sc] = xv;
+        if (tid < 4 * OT) *(uint4*)&sA[sva][sca] = dv;
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int q = tid + 256 * k;
+            *(uint4*)&sB[q / (IT / 8)][(q % (IT / 8)) * 8] = xv[k];
+        }
         __syncthreads();
         if (vb + 32 < vend) fetch(vb + 32, dv, xv);
 #pragma unroll
@@ -564,8 +579,11 @@ void launch_conv_wgrad(const int32_t* shape, const uint16_t* X, const uint16_t* 
         else
             hipLaunchKernelGGL((k_conv_wgrad32x<16, 16>), g1, dim3(256), 0, s, c, X, dY, vsplit, dW);
     }
-    else if (c.ci % 32 == 0 && c.co % 32 == 0 && al16)
-        hipLaunchKernelGGL(k_conv_wgrad32, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
+    else if (c.ci % 32 == 0 && c.co == 32 && al16) {
+        const dim3 g2(1, (unsigned)((c.ci + 127) / 128), (unsigned)(27 * vsplit));
+        hipLaunchKernelGGL(k_conv_wgrad32<32>, g2, dim3(256), 0, s, c, X, dY, vsplit, dW);
+    } else if (c.ci % 32 == 0 && c.co % 32 == 0 && al16)
+        hipLaunchKernelGGL(k_conv_wgrad32<64>, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
     else
         hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, s, c, X, dY, vsplit, dW);
 }
