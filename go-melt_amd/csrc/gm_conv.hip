@@ -151,8 +151,9 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvDims c, const uint16_t* __
 // stage is one tap x 32 input channels (one v_mfma_f32_16x16x32_bf16 k-step,
 // 4 MFMAs per wave between barriers), each thread stages 16 bytes of A and of
 // B, and the next stage's global loads are in flight while the current stage
-// computes.
-template <int BM, int BN>
+// computes.  CI16: cin = 16, and a stage is two taps x 16 channels (14
+// stages, the 28th tap zero).
+template <int BM, int BN, bool CI16 = false>
 __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* __restrict__ X,
                                                     const uint16_t* __restrict__ Wk,
                                                     const float* __restrict__ bias, int relu,
@@ -182,18 +183,27 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
         pv[q] = vox_decode(c, vrow[q] < c.nvox ? vrow[q] : 0);
     }
     const int o = n0 + srow;  // B rows: threads with srow < BN
-    const int kch = c.ci / 32;
-    const int nst = 27 * kch;
+    const int kch = CI16 ? 1 : c.ci / 32;
+    const int nst = CI16 ? 14 : 27 * kch;
     auto fetch = [&](int st, uint4* av, uint4& bv) {
-        const int tap = st / kch, i0 = (st - tap * kch) * 32 + sk;
+        int tap, i0;
+        if constexpr (CI16) {
+            tap = 2 * st + (sk >> 4);  // k 0-15: tap 2 st, k 16-31: tap 2 st + 1
+            i0 = sk & 15;
+        } else {
+            tap = st / kch;
+            i0 = (st - tap * kch) * 32 + sk;
+        }
+        const bool tv = !CI16 || tap < 27;
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
-            const int64_t vs = vrow[q] < c.nvox ? vox_shift(c, pv[q], tap) : -1;
+            const int64_t vs = vrow[q] < c.nvox && tv ? vox_shift(c, pv[q], tap) : -1;
             av[q] = make_uint4(0u, 0u, 0u, 0u);
             if (vs >= 0) av[q] = *(const uint4*)(X + vs * c.ci + i0);
         }
         bv = make_uint4(0u, 0u, 0u, 0u);
-        if (srow < BN && o < c.co) bv = *(const uint4*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
+        if (srow < BN && o < c.co && tv)
+            bv = *(const uint4*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
     };
     uint4 av[NA], bv;
     fetch(0, av, bv);
@@ -542,7 +552,8 @@ void launch_conv_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk
     const dim3 grid((unsigned)((c.nvox + 63) / 64), (unsigned)((c.co + 63) / 64));
     // the pipelined kernel reads 16 bytes at a time; up to 32 outputs it
     // takes 128 voxels x 32 outputs per block (no idle half of the tile)
-    if (c.ci % 32 == 0 && (((uintptr_t)X | (uintptr_t)Wk) & 15) == 0) {
+    const bool al16 = (((uintptr_t)X | (uintptr_t)Wk) & 15) == 0;
+    if (c.ci % 32 == 0 && al16) {
         if (c.co <= 32) {
             const dim3 g2((unsigned)((c.nvox + 127) / 128), 1);
             hipLaunchKernelGGL((k_conv_fwd32<128, 32>), g2, dim3(256), 0, s, c, X, Wk, bias, relu,
@@ -550,6 +561,16 @@ void launch_conv_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk
         } else {
             hipLaunchKernelGGL((k_conv_fwd32<64, 64>), grid, dim3(256), 0, s, c, X, Wk, bias, relu,
                                Y, Ybf);
+        }
+    } else if (c.ci == 16 && al16) {
+        // two taps x 16 channels per K = 32 stage
+        if (c.co <= 32) {
+            const dim3 g2((unsigned)((c.nvox + 127) / 128), 1);
+            hipLaunchKernelGGL((k_conv_fwd32<128, 32, true>), g2, dim3(256), 0, s, c, X, Wk, bias,
+                               relu, Y, Ybf);
+        } else {
+            hipLaunchKernelGGL((k_conv_fwd32<64, 64, true>), grid, dim3(256), 0, s, c, X, Wk, bias,
+                               relu, Y, Ybf);
         }
     } else {
         hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
