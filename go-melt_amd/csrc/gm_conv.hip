@@ -165,8 +165,12 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
     __shared__ uint16_t sA[BM][32 + 8];
     __shared__ uint16_t sB[BN][32 + 8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t m0 = (int64_t)blockIdx.x * BM;
-    const int n0 = blockIdx.y * BN;
+    // XCD-aware tiles: a run of consecutive voxel tiles (with all their output
+    // tiles) per XCD, so the tap-shifted rows of neighbouring tiles hit its L2
+    const int tile = xcd_tile(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int mt = tile / gridDim.y;
+    const int64_t m0 = (int64_t)mt * BM;
+    const int n0 = (tile - mt * gridDim.y) * BN;
     const int wm = (wave % (BM / 32)) * 32, wn = (wave / (BM / 32)) * 32;
     v4f acc[2][2];
 #pragma unroll

@@ -203,6 +203,15 @@ struct RArgs {
 // ---- launchers (gm_kernels.hip) -------------------------------------------
 namespace gm {
 
+// Tile index of block b of nb, mapped so that each XCD (blocks are dealt to
+// the 8 XCDs round-robin) works on a contiguous run of tiles and shares halos
+// through its own L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, k = b >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // 3x3x3 BF16 MFMA convolutions of the learned operator (gm_conv.hip);
 // shape = {nb, D, H, W, cin, cout}
 bool conv_shape_ok(const int32_t* shape);

@@ -192,15 +192,6 @@ __device__ __forceinline__ int fdiv_small(int u, float invR) {
     return (int)(((float)u + 0.5f) * invR);
 }
 
-// Tile index of this block, mapped so that each XCD (blocks are dealt to the
-// 8 XCDs round-robin) works on a contiguous run of tiles and shares halos
-// through its own L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-    const int q = nb >> 3, r = nb & 7;
-    const int x = b & 7, k = b >> 3;
-    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-
 // Workgroup barrier for LDS hand-offs only.  __syncthreads() also waits for
 // every outstanding global load and store of the wave (vmcnt(0)), which in a
 // plane march puts the store round trip of each plane on the critical path.
