@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libgomelt.so")
 GM_MAX_KNOTS = 16
 
 (GM_T0, GM_TPRIME0, GM_S1, GM_S2, GM_ACCUM, GM_MAX_ACCUM, GM_HIST_L2, GM_HIST_L3,
- GM_HIST_TP3) = range(9)
+ GM_HIST_TP3, GM_ACCUM_L3, GM_MAX_ACCUM_L3, GM_RESET_L3) = range(12)
 
 
 class GoMeltNativeError(RuntimeError):
@@ -130,6 +130,11 @@ _SIGS = {
     "gm_engine_minmax": (_I, [_P, _FP]),
     "gm_engine_use_graph": (_I, [_P, _I]),
     "gm_engine_time_kernel": (_I, [_P, _I, _I, _FP]),
+    "gm_engine_set_accum_mode": (_I, [_P, _I]),
+    "gm_engine_overlap_box": (_I, [_P, _I, _IP, _IP, _IP]),
+    "gm_engine_state_bytes": (_I, [_P, C.POINTER(_I64)]),
+    "gm_engine_save_state": (_I, [_P, _P]),
+    "gm_engine_load_state": (_I, [_P, _P, _I64]),
     "gm_batch_create": (_I, [_I, _G, _M, _P, C.POINTER(_P)]),
     "gm_batch_destroy": (_I, [_P]),
     "gm_batch_step": (_I, [_P, _FP, _I]),
@@ -262,6 +267,90 @@ class DeviceArray:
     def __array__(self, dtype=None, copy=None):
         a = self.numpy()
         return a.astype(dtype) if dtype is not None else a
+
+    # Host reads in the style of a jax.Array, for driver code written against
+    # the reference (go_melt.py): comparisons, arithmetic and indexing
+    # download the buffer and return host arrays, and ``.at[i].set(v)`` /
+    # ``.at[i].add(v)`` return an updated host copy (the buffer itself is
+    # not changed, as with the functional update of jax.Array).
+    __array_priority__ = 100
+    __hash__ = object.__hash__
+
+    def __getitem__(self, i):
+        return self.numpy()[i]
+
+    def _cmp(self, o, op):
+        return op(self.numpy(), np.asarray(o))
+
+    def __gt__(self, o):
+        return self._cmp(o, np.greater)
+
+    def __ge__(self, o):
+        return self._cmp(o, np.greater_equal)
+
+    def __lt__(self, o):
+        return self._cmp(o, np.less)
+
+    def __le__(self, o):
+        return self._cmp(o, np.less_equal)
+
+    def __eq__(self, o):
+        return self._cmp(o, np.equal)
+
+    def __ne__(self, o):
+        return self._cmp(o, np.not_equal)
+
+    def __mul__(self, o):
+        return self.numpy() * np.asarray(o)
+
+    __rmul__ = __mul__
+
+    def __add__(self, o):
+        return self.numpy() + np.asarray(o)
+
+    __radd__ = __add__
+
+    def __sub__(self, o):
+        return self.numpy() - np.asarray(o)
+
+    def __rsub__(self, o):
+        return np.asarray(o) - self.numpy()
+
+    @property
+    def shape(self):
+        return (self.n,)
+
+    @property
+    def at(self):
+        return _At(self.numpy())
+
+
+class _At:
+    """``a.at[idx].set(v)`` / ``.add(v)`` / ``.get()`` on a host copy."""
+
+    def __init__(self, a):
+        self._a = a
+
+    def __getitem__(self, idx):
+        return _AtIndex(self._a, idx)
+
+
+class _AtIndex:
+    def __init__(self, a, idx):
+        self._a, self._i = a, idx
+
+    def set(self, v):
+        b = self._a.copy()
+        b[self._i] = v
+        return b
+
+    def add(self, v):
+        b = self._a.copy()
+        np.add.at(b, self._i, v)
+        return b
+
+    def get(self):
+        return self._a[self._i]
 
 
 def as_device(x, dtype=np.float32):

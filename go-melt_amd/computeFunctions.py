@@ -61,17 +61,26 @@ def structure_to_dict(struct_):
                 else structure_to_dict(v)) for k, v in struct_.__dict__.items()}
 
 
-def save_object(o, filename):
-    """cF:101-112 (the reference uses dill; fields are downloaded first)."""
+def save_object(o, filename, extra=None):
+    """cF:101-112 (the reference uses dill; fields are downloaded first).
+    ``extra`` (a dict) is merged into the first level dict of a Levels list
+    at position 0 of ``o``: the driver's checkpoint stores the engine state
+    there."""
+    h = _host_copy(o)
+    if extra:
+        h[0][0].update(extra)
     with open(filename, "wb") as fh:
-        pickle.dump(_host_copy(o), fh, pickle.HIGHEST_PROTOCOL)
+        pickle.dump(h, fh, pickle.HIGHEST_PROTOCOL)
 
 
 def _host_copy(o):
     if isinstance(o, DeviceArray):
         return o.numpy()
     if isinstance(o, dict):
-        return {k: _host_copy(v) for k, v in o.items() if k != "_eng"}
+        # the engine handle and the engine's own Level0 melt-time maps (the
+        # driver checkpoints those as separate items) are not level data
+        return {k: _host_copy(v) for k, v in o.items()
+                if k not in ("_eng", "accum_time", "max_accum_time")}
     if isinstance(o, (list, tuple)):
         return type(o)(_host_copy(v) for v in o)
     return o
@@ -276,13 +285,58 @@ def _level_dict(src, p):
     return lv
 
 
+class LevelDict(dict):
+    """A level of ``Levels`` bound to the device state of an engine.
+
+    Keys that name engine fields (``T0``, ``Tprime0``, ``S1``, ``S2`` and the
+    Level0 melt-time maps) hold device views of the engine's buffers.
+    Assigning to such a key, as the reference's driver code does
+    (e.g. ``Levels[2]["Tprime0"] = ...`` at go_melt.py:366-367), copies the
+    new values INTO the engine buffer; the key keeps the device view, so the
+    assignment cannot detach the level from the state the kernels advance.
+    The overlap index sets that moveEverything updates (cF:2472-2497) are
+    read from the engine on access: ``overlapNodes`` (all windows) and, on
+    Level0, ``overlapNodes_L2``, ``idx`` and ``idx_L2`` (flat x-fastest
+    indices, getOverlapRegion cF:1642-1669)."""
+
+    _BOXES = {(0, "overlapNodes"): 0, (0, "overlapNodes_L2"): 1, (2, "overlapNodes"): 2,
+              (3, "overlapNodes"): 3}
+
+    def __getitem__(self, key):
+        eng = dict.get(self, "_eng")
+        lev = dict.get(self, "level")
+        if eng is not None:
+            if lev == 0 and key in ("idx", "idx_L2"):
+                nodes = eng.overlap_nodes(0 if key == "idx" else 1)
+                return getOverlapRegion(nodes, self["nodes"][0], self["nodes"][1])
+            which = self._BOXES.get((lev, key))
+            if which is not None:
+                return eng.overlap_nodes(which)
+        return dict.__getitem__(self, key)
+
+    def get(self, key, default=None):
+        try:
+            return self[key]
+        except KeyError:
+            return default
+
+    def __setitem__(self, key, value):
+        cur = dict.get(self, key)
+        if isinstance(cur, DeviceArray) and dict.get(self, "_eng") is not None:
+            if value is not cur:
+                v = value.numpy() if isinstance(value, DeviceArray) else np.asarray(value)
+                cur.upload(v.astype(cur.dtype).reshape(-1))
+            return
+        dict.__setitem__(self, key, value)
+
+
 def SetupLevels(solver_input, properties):
     """cF:115-264: Level0 (state grid) and Levels 1-3, their overlap index
     boxes, and the device state (T = T_amb, S1 = 0) in a new gm_engine."""
-    Levels = setup_levels_host(solver_input, properties)
+    Levels = [LevelDict(lv) for lv in setup_levels_host(solver_input, properties)]
     eng = Engine(Levels, properties, solver_input.get("nonmesh", {}))
     for lv in Levels:
-        lv["_eng"] = eng
+        dict.__setitem__(lv, "_eng", eng)
     eng.bind(Levels)
     return Levels
 
@@ -452,16 +506,46 @@ class Engine:
     def bind(self, Levels):
         """Put device views of the engine's fields into the level dicts."""
         self.Levels = Levels
-        Levels[1]["T0"] = self.field(1, N.GM_T0)
-        Levels[1]["S1"] = self.field(1, N.GM_S1)
+        put = dict.__setitem__
+        put(Levels[1], "T0", self.field(1, N.GM_T0))
+        put(Levels[1], "S1", self.field(1, N.GM_S1))
         for l in (2, 3):
-            Levels[l]["T0"] = self.field(l, N.GM_T0)
-            Levels[l]["Tprime0"] = self.field(l, N.GM_TPRIME0)
-            Levels[l]["S1"] = self.field(l, N.GM_S1)
-        Levels[3]["S2"] = self.field(3, N.GM_S2)
-        Levels[0]["S1"] = self.field(0, N.GM_S1)
-        Levels[0]["accum_time"] = self.field(0, N.GM_ACCUM)
-        Levels[0]["max_accum_time"] = self.field(0, N.GM_MAX_ACCUM)
+            put(Levels[l], "T0", self.field(l, N.GM_T0))
+            put(Levels[l], "Tprime0", self.field(l, N.GM_TPRIME0))
+            put(Levels[l], "S1", self.field(l, N.GM_S1))
+        put(Levels[3], "S2", self.field(3, N.GM_S2))
+        put(Levels[0], "S1", self.field(0, N.GM_S1))
+        put(Levels[0], "accum_time", self.field(0, N.GM_ACCUM))
+        put(Levels[0], "max_accum_time", self.field(0, N.GM_MAX_ACCUM))
+
+    def overlap_nodes(self, which):
+        """Index axes of an overlap box after the last move (see
+        gm_engine_overlap_box): 0 Level0 in L3, 1 Level0 in L2 (strided),
+        2 L1 in L2, 3 L2 in L3."""
+        lo, n, st = (np.zeros(3, np.int32) for _ in range(3))
+        N.check(N.lib().gm_engine_overlap_box(self.h, int(which), lo.ctypes.data_as(N._IP),
+                                              n.ctypes.data_as(N._IP),
+                                              st.ctypes.data_as(N._IP)))
+        return [int(lo[d]) + int(st[d]) * np.arange(int(n[d]), dtype=np.int64)
+                for d in range(3)]
+
+    def save_state(self):
+        """The engine's device state as bytes (gm_engine_save_state)."""
+        n = C.c_int64()
+        N.check(N.lib().gm_engine_state_bytes(self.h, C.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        N.check(N.lib().gm_engine_save_state(self.h, buf.ctypes.data))
+        return buf.tobytes()
+
+    def load_state(self, blob):
+        buf = np.frombuffer(blob, np.uint8)
+        N.check(N.lib().gm_engine_load_state(self.h, buf.ctypes.data, buf.size))
+        self.layer = None
+
+    def set_accum_external(self, external):
+        if bool(external) != getattr(self, "_acc_ext", False):
+            N.check(N.lib().gm_engine_set_accum_mode(self.h, int(bool(external))))
+            self._acc_ext = bool(external)
 
     def set_layer(self, tmp_ne_nn, substrate):
         key = (tuple(int(v) for v in tmp_ne_nn), tuple(int(v) for v in substrate))
@@ -652,8 +736,9 @@ def getNewTprime(Fine, FineT0, CoarseT, Coarse, C2F):
     Cd = DeviceArray.from_numpy(np.asarray(CoarseT, np.float32))
     Fd = _dev(FineT0)
     Tp = DeviceArray(gf.nn())
-    lo = np.array([int(np.asarray(Fine["overlapNodes"][d])[0]) for d in range(3)], np.int32)
-    nb = np.array([len(Fine["overlapNodes"][d]) for d in range(3)], np.int32)
+    ov = Fine["overlapNodes"]
+    lo = np.array([int(np.asarray(ov[d])[0]) for d in range(3)], np.int32)
+    nb = np.array([len(ov[d]) for d in range(3)], np.int32)
     N.check(N.lib().gm_new_tprime(C.byref(gf), Fd.ptr, C.byref(gc), Cd.ptr,
                                   lo.ctypes.data_as(N._IP), nb.ctypes.data_as(N._IP), Tp.ptr,
                                   None))
@@ -708,28 +793,56 @@ def moveEverything(v, vstart, Levels, move_v, LInterp, L1L2Eratio, L2L3Eratio, h
 def subcycleGOMELT(Levels, ne_nn, Shapes, substrate, LInterp, tmp_ne_nn, laser_position,
                    properties, laserP, subcycle, max_accum_L3=None, accum_L3=None):
     """cF:3224-3632: one L1 step with N2 x N3 subcycling (predictor, corrector,
-    melt-time accumulation) as one hipGraph launch.  Returns
-    (Levels, L2all, L3all, L3pall, max_accum, accum); the histories are device
-    views of the corrector's per-L2-substep fields, and the melt-time arrays
-    are the Level0 arrays the engine updates in place."""
+    melt-time accumulation) as one hipGraph launch.
+    Returns (Levels, L2all, L3all, L3pall, max_accum_L3, accum_L3) as the
+    reference does; L2all/L3all/L3pall are device views of the corrector's
+    per-L2-substep fields.
+
+    Melt time, as in the reference: ``max_accum_L3`` and ``accum_L3`` are the
+    caller's maps on the L3 window nodes (the driver passes
+    ``max_accum_time[Levels[0]["idx"]]``, go_melt.py:447-448); the updated
+    window values come back as host arrays (the driver scatters them,
+    go_melt.py:453-455).  Passing None for both keeps melt time in the
+    engine's own Level0 maps (``Levels[0]["accum_time"]`` etc., updated in
+    place on the device), and the last two returns are those maps."""
     e = _eng(Levels)
     e.set_layer(tmp_ne_nn, substrate)
     rows = np.array(laser_position, dtype=np.float32).reshape(-1, 7).copy()
     if laserP is not None:
         rows[:, 6] = np.asarray(laserP, dtype=np.float32).reshape(-1)
+    external = max_accum_L3 is not None or accum_L3 is not None
+    e.set_accum_external(external)
+    if external:
+        nn3 = int(Levels[3]["nn"])
+        for fld, arr in ((N.GM_MAX_ACCUM_L3, max_accum_L3), (N.GM_ACCUM_L3, accum_L3)):
+            dst = e.field(3, fld)
+            src = np.zeros(nn3, np.float32) if arr is None else np.asarray(arr, np.float32)
+            if src.size != nn3:
+                raise ValueError(f"melt-time window arrays need {nn3} values (the L3 nodes), "
+                                 f"got {src.size}")
+            N.check(N.lib().gm_memcpy_h2d(dst.ptr, np.ascontiguousarray(src).ctypes.data,
+                                          dst.nbytes, e.stream.ptr))
     N.check(N.lib().gm_engine_subcycle(e.h, N.fptr(rows), rows.shape[0]))
     L2all = [e.field(k, N.GM_HIST_L2) for k in range(e.N2)]
     L3all = [e.field(k, N.GM_HIST_L3) for k in range(e.N2)]
     L3pall = [e.field(k, N.GM_HIST_TP3) for k in range(e.N2)]
+    if external:
+        return (Levels, L2all, L3all, L3pall, e.field(3, N.GM_MAX_ACCUM_L3).numpy(e.stream.ptr),
+                e.field(3, N.GM_ACCUM_L3).numpy(e.stream.ptr))
     return (Levels, L2all, L3all, L3pall, Levels[0]["max_accum_time"],
             Levels[0]["accum_time"])
 
 
 def stepGOMELT(Levels, ne_nn, tmp_ne_nn, Shapes, LInterp, v, properties, dt, laserP,
                substrate):
-    """cF:2304-2397: equal-dt predictor/corrector over all levels (plus the
-    driver's single-step melt-time update, go_melt.py:339-357).  Returns
-    (Levels, None): the reset mask is consumed on the device."""
+    """cF:2304-2397: equal-dt predictor/corrector over all levels.
+    Returns (Levels, _resetmask) as the reference does: the L3 nodes whose S2
+    went from 0 to 1 in this step (cF:2394-2397), a device view of uint8 that
+    reads as a host array (``all_reset > 0``, go_melt.py:343).  The view
+    holds the mask of the latest step.  While the engine keeps melt time
+    itself (subcycleGOMELT never given window arrays), the step also applies
+    the driver's melt-time bookkeeping (go_melt.py:339-357) to its Level0
+    maps."""
     e = _eng(Levels)
     e.set_layer(tmp_ne_nn, substrate)
     row = np.zeros(7, np.float32)
@@ -737,7 +850,7 @@ def stepGOMELT(Levels, ne_nn, tmp_ne_nn, Shapes, LInterp, v, properties, dt, las
     row[5] = dt
     row[6] = laserP
     N.check(N.lib().gm_engine_step(e.h, N.fptr(row)))
-    return Levels, None
+    return Levels, e.field(3, N.GM_RESET_L3)
 
 
 def stepGOMELTDwellTime(Levels, tmp_ne_nn, ne_nn, properties, dt, substrate):
@@ -749,9 +862,15 @@ def stepGOMELTDwellTime(Levels, tmp_ne_nn, ne_nn, properties, dt, substrate):
 
 
 def melting_temp(temps, delt_T, T_melt, accum_time, idx):
-    """cF:3696-3712 (host form; the engine applies it on the device)."""
+    """cF:3696-3712: accum_time[idx] += (temps > T_melt) * delt_T (host form
+    for a driver that keeps its own maps; the engine applies it on the
+    device otherwise).  An ``accum_time`` with a ``.at`` functional update
+    (jax.Array style) is updated through it, as the reference does."""
+    add = np.array(np.asarray(temps) > T_melt) * delt_T
+    if hasattr(accum_time, "at"):
+        return accum_time.at[idx].add(add)
     out = np.array(accum_time, copy=True)
-    np.add.at(out, idx, (np.asarray(temps) > T_melt) * delt_T)
+    np.add.at(out, idx, add)
     return out
 
 
@@ -825,68 +944,89 @@ def printLevelMaxMin(Ls, Lnames):
 # ---------------------------------------------------------------------------
 
 
+_VTK_TYPES = {np.dtype(np.float32): "Float32", np.dtype(np.float64): "Float64",
+              np.dtype(np.int32): "Int32", np.dtype(np.int64): "Int64",
+              np.dtype(np.uint8): "UInt8", np.dtype(np.int8): "Int8"}
+
+
 def _write_vtr(path, coords, point_data):
-    """Binary-appended VTK XML RectilinearGrid (what pyevtk's gridToVTK writes)."""
+    """VTK XML RectilinearGrid with raw appended data, the layout
+    pyevtk.hl.gridToVTK writes (cF:1982 calls it; pyevtk is not needed here):
+    version 1.0, UInt64 block headers, the three coordinate axes first, then
+    the point data in the order given, each array in its own dtype with its
+    values in Fortran order (x fastest); PointData names its first array as
+    the Scalars attribute.  Writes ``path + ".vtr"``."""
     nx, ny, nz = (len(c) for c in coords)
-    arrays = [("x", np.asarray(coords[0], np.float64)), ("y", np.asarray(coords[1], np.float64)),
-              ("z", np.asarray(coords[2], np.float64))]
+    axes = [np.ascontiguousarray(np.asarray(c)) for c in coords]
+    arrays = [(n, a) for n, a in zip(("x_coordinates", "y_coordinates", "z_coordinates"), axes)]
     for name, a in point_data.items():
-        arrays.append((name, np.asarray(a, np.float64).reshape(-1)))
+        a = np.asarray(a)
+        arrays.append((name, a.reshape(-1, order="F") if a.ndim > 1 else a))
     offs, off = [], 0
     for _, a in arrays:
         offs.append(off)
         off += 8 + a.nbytes
     ext = f"0 {nx - 1} 0 {ny - 1} 0 {nz - 1}"
+
+    def darr(name, a, o):
+        return (f'<DataArray type="{_VTK_TYPES[a.dtype]}" Name="{name}" '
+                f'NumberOfComponents="1" format="appended" offset="{o}"/>\n')
+
     with open(path + ".vtr", "wb") as fh:
-        w = lambda s: fh.write(s.encode())  # noqa: E731
+        w = lambda t: fh.write(t.encode())  # noqa: E731
         w('<?xml version="1.0"?>\n<VTKFile type="RectilinearGrid" version="1.0" '
           'byte_order="LittleEndian" header_type="UInt64">\n')
-        w(f'<RectilinearGrid WholeExtent="{ext}">\n<Piece Extent="{ext}">\n<PointData>\n')
-        for (name, _), o in zip(arrays[3:], offs[3:]):
-            w(f'<DataArray type="Float64" Name="{name}" format="appended" offset="{o}"/>\n')
-        w("</PointData>\n<Coordinates>\n")
-        for (name, _), o in zip(arrays[:3], offs[:3]):
-            w(f'<DataArray type="Float64" Name="{name}_coordinates" format="appended" '
-              f'offset="{o}"/>\n')
-        w('</Coordinates>\n</Piece>\n</RectilinearGrid>\n<AppendedData encoding="raw">\n_')
+        w(f'<RectilinearGrid WholeExtent="{ext}">\n<Piece Extent="{ext}">\n<Coordinates>\n')
+        for (name, a), o in zip(arrays[:3], offs[:3]):
+            w(darr(name, a, o))
+        w("</Coordinates>\n")
+        if len(arrays) > 3:
+            w(f'<PointData Scalars="{arrays[3][0]}">\n')
+            for (name, a), o in zip(arrays[3:], offs[3:]):
+                w(darr(name, a, o))
+            w("</PointData>\n")
+        w('</Piece>\n</RectilinearGrid>\n<AppendedData encoding="raw">\n_')
         for _, a in arrays:
             fh.write(struct.pack("<Q", a.nbytes))
-            fh.write(np.ascontiguousarray(a, "<f8").tobytes(order="C"))
+            fh.write(np.ascontiguousarray(a).astype(a.dtype.newbyteorder("<")).tobytes())
         w("\n</AppendedData>\n</VTKFile>\n")
 
 
 def _fields(Level):
+    """T0 and S1 as (nx, ny, nz) float32 arrays (cF:1972-1977)."""
     nx, ny, nz = Level["nodes"]
     T = np.asarray(Level["T0"], np.float32).reshape(nz, ny, nx).transpose(2, 1, 0)
     S = np.asarray(Level["S1"]).astype(np.float32).reshape(nz, ny, nx).transpose(2, 1, 0)
     return T, S
 
 
-def saveResult(Level, save_str, record_lab, save_path, zoffset):
-    """cF:1945-1982."""
-    T, S = _fields(Level)
+def _axes(Level, zoffset):
     nc = Level["node_coords"]
-    _write_vtr(f"{save_path}{save_str}{record_lab:08}", [nc[0], nc[1], nc[2] - zoffset],
-               {"Temperature (K)": T.reshape(-1, order="F"),
-                "State (Powder/Solid)": S.reshape(-1, order="F")})
+    return [np.asarray(nc[0], np.float32), np.asarray(nc[1], np.float32),
+            (np.asarray(nc[2], np.float32) - np.float32(zoffset)).astype(np.float32)]
+
+
+def saveResult(Level, save_str, record_lab, save_path, zoffset):
+    """cF:1945-1982: <save_path><save_str><record_lab:08>.vtr with the point
+    data "Temperature (K)" and "State (Powder/Solid)"."""
+    T, S = _fields(Level)
+    _write_vtr(f"{save_path}{save_str}{record_lab:08}", _axes(Level, zoffset),
+               {"Temperature (K)": T, "State (Powder/Solid)": S})
 
 
 def saveFinalResult(Level, save_str, save_path, zoffset):
-    """cF:1985-2021."""
+    """cF:1985-2021: <save_path><save_str>Final.vtr."""
     T, S = _fields(Level)
-    nc = Level["node_coords"]
-    _write_vtr(f"{save_path}{save_str}Final", [nc[0], nc[1], nc[2] - zoffset],
-               {"Temperature (K)": T.reshape(-1, order="F"),
-                "State (Powder/Solid)": S.reshape(-1, order="F")})
+    _write_vtr(f"{save_path}{save_str}Final", _axes(Level, zoffset),
+               {"Temperature (K)": T, "State (Powder/Solid)": S})
 
 
 def saveState(Level, save_str, record_lab, save_path, zoffset):
-    """cF:2024-2057."""
+    """cF:2024-2057: the state field alone."""
     nx, ny, nz = Level["nodes"]
     S = np.asarray(Level["S1"]).astype(np.float32).reshape(nz, ny, nx).transpose(2, 1, 0)
-    nc = Level["node_coords"]
-    _write_vtr(f"{save_path}{save_str}{record_lab:08}", [nc[0], nc[1], nc[2] - zoffset],
-               {"State (Powder/Solid)": S.reshape(-1, order="F")})
+    _write_vtr(f"{save_path}{save_str}{record_lab:08}", _axes(Level, zoffset),
+               {"State (Powder/Solid)": S})
 
 
 def saveResults(Levels, Nonmesh, savenum):

@@ -131,8 +131,8 @@ struct PairMem {
     void* blob = nullptr;
 };
 
-static int pair_alloc(PairMem& pm, int coarse, int fine, const gm_grid& gc, const gm_grid& gf) {
-    GmPair& p = pm.p;
+// Sizes of a fine -> coarse pair and the bytes of its axis tables.
+static size_t pair_layout(GmPair& p, int coarse, int fine, const gm_grid& gc, const gm_grid& gf) {
     p.coarse = coarse;
     p.fine = fine;
     size_t bytes = 0;
@@ -142,9 +142,12 @@ static int pair_alloc(PairMem& pm, int coarse, int fine, const gm_grid& gc, cons
         bytes += (size_t)p.nef[d] * 4 + (size_t)p.nef[d] * 2 * 16 + (size_t)(2 + 2 * p.cmax[d]) * 4;
         bytes += 64;
     }
-    HIPCHK(hipMalloc(&pm.blob, bytes));
-    HIPCHK(hipMemset(pm.blob, 0, bytes));
-    char* b = (char*)pm.blob;
+    return bytes;
+}
+
+// Point the axis tables of a laid-out pair into `blob` (64-byte aligned).
+static void pair_bind(GmPair& p, void* blob) {
+    char* b = (char*)blob;
     for (int d = 0; d < 3; ++d) {
         p.ax[d].q = (float4*)b;
         b += (size_t)p.nef[d] * 2 * 16;
@@ -154,6 +157,13 @@ static int pair_alloc(PairMem& pm, int coarse, int fine, const gm_grid& gc, cons
         b += (size_t)(2 + 2 * p.cmax[d]) * 4;
         b = (char*)(((uintptr_t)b + 63) & ~(uintptr_t)63);
     }
+}
+
+static int pair_alloc(PairMem& pm, int coarse, int fine, const gm_grid& gc, const gm_grid& gf) {
+    const size_t bytes = pair_layout(pm.p, coarse, fine, gc, gf);
+    HIPCHK(hipMalloc(&pm.blob, bytes));
+    HIPCHK(hipMemset(pm.blob, 0, bytes));
+    pair_bind(pm.p, pm.blob);
     return 0;
 }
 
@@ -497,6 +507,8 @@ struct gm_engine {
     int strip_R0[3] = {-1, -1, -1}, strip_a0[3] = {-1, -1, -1};  // restriction strip layout
     float mcs[4][8];          // consistent mass per neighbour pattern (x rho)
     uint8_t* preS2 = nullptr;
+    uint8_t* reset3 = nullptr;  // L3 nodes whose S2 went 0 -> 1 in the last single step
+    bool acc_external = false;  // melt time kept by the caller (window arrays acc3/macc3)
     float *fa2 = nullptr, *fb2 = nullptr, *fa3 = nullptr, *fb3 = nullptr;
     float* mm = nullptr;
     PairMem pair[3];
@@ -556,6 +568,7 @@ extern "C" int gm_engine_destroy(gm_engine* e) {
 }
 
 static bool pairs_aligned(gm_engine* e);
+static int update_alignment(gm_engine* e);
 static void strip_phase(gm_engine* e, int i, int& R0, int& a0);
 
 extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, void* stream,
@@ -611,6 +624,7 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
         eng_alloc(e, (void**)&e->T1p, n1 * 4) || eng_alloc(e, (void**)&e->T1c, n1 * 4) ||
         eng_alloc(e, (void**)&e->Tp2x, n2 * 4) || eng_alloc(e, (void**)&e->Tp3x, n3 * 4) ||
         eng_alloc(e, (void**)&e->preS2, n3) || eng_alloc(e, (void**)&e->mm, 64) ||
+        eng_alloc(e, (void**)&e->reset3, n3) ||
         eng_alloc(e, (void**)&e->fa2, n2 * 4) || eng_alloc(e, (void**)&e->fb2, n2 * 4) ||
         eng_alloc(e, (void**)&e->fa3, n3 * 4) || eng_alloc(e, (void**)&e->fb3, n3 * 4) ||
         eng_alloc(e, (void**)&e->d_desc, sizeof(GmStepDesc)) ||
@@ -691,6 +705,9 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
     HIPCHK(hipMemsetAsync(e->S12, 0, n2, e->s));
     HIPCHK(hipMemsetAsync(e->S13, 0, n3, e->s));
     HIPCHK(hipMemsetAsync(e->S23, 0, n3, e->s));
+    HIPCHK(hipMemsetAsync(e->reset3, 0, n3, e->s));
+    HIPCHK(hipMemsetAsync(e->acc3, 0, n3 * 4, e->s));
+    HIPCHK(hipMemsetAsync(e->macc3, 0, n3 * 4, e->s));
     HIPCHK(hipMemsetAsync(e->L0S1, 0, n0, e->s));
     HIPCHK(hipMemsetAsync(e->acc, 0, n0 * 4, e->s));
     HIPCHK(hipMemsetAsync(e->macc, 0, n0 * 4, e->s));
@@ -752,6 +769,9 @@ extern "C" int gm_engine_field(gm_engine* e, int level, int field, void** ptr, i
         case GM_S2: p = level == 3 ? (void*)e->S23 : nullptr; break;
         case GM_ACCUM: p = level == 0 ? (void*)e->acc : nullptr; break;
         case GM_MAX_ACCUM: p = level == 0 ? (void*)e->macc : nullptr; break;
+        case GM_ACCUM_L3: p = e->acc3; n = e->nn[3]; break;
+        case GM_MAX_ACCUM_L3: p = e->macc3; n = e->nn[3]; break;
+        case GM_RESET_L3: p = e->reset3; n = e->nn[3]; break;
         default: break;
     }
     if (!p) return fail("no such field on this level");
@@ -778,6 +798,116 @@ extern "C" int gm_engine_set_layer(gm_engine* e, int64_t ne_act, int64_t nn_act,
 extern "C" int gm_engine_use_graph(gm_engine* e, int enable) {
     e->use_graph = enable != 0;
     e->graph_ok = false;
+    return 0;
+}
+
+// ---- engine state for checkpoints (go_melt.py:115-129, 390-411) -------------
+
+// The persistent device arrays of the engine except Level0 (whose S1 and
+// melt-time maps the caller saves itself, as the reference's checkpoint
+// holds Levels[0] and the two accum arrays), in a fixed order.
+static std::vector<std::pair<void*, size_t>> state_arrays(gm_engine* e) {
+    const size_t n1 = e->nn[1], n2 = e->nn[2], n3 = e->nn[3];
+    return {{e->T1, n1 * 4},  {e->S11, n1 * 4}, {e->S1st, (size_t)e->nst * n1 * 4},
+            {e->T2, n2 * 4},  {e->Tp2, n2 * 4}, {e->S12, n2},
+            {e->T3, n3 * 4},  {e->Tp3, n3 * 4}, {e->S13, n3},
+            {e->S23, n3},     {e->preS2, n3},   {e->reset3, n3},
+            {e->s2snap, n3},  {e->s2box, 64},   {e->acc3, n3 * 4},
+            {e->macc3, n3 * 4}};
+}
+
+static const uint32_t kStateMagic = 0x54534d47u;  // "GMST"
+
+static size_t state_header_bytes() {
+    return 8 + sizeof(gm_config) + sizeof(GmStepDesc) + 6 * sizeof(int);
+}
+
+extern "C" int gm_engine_state_bytes(gm_engine* e, int64_t* bytes) {
+    size_t b = state_header_bytes();
+    for (const auto& a : state_arrays(e)) b += a.second;
+    *bytes = (int64_t)b;
+    return 0;
+}
+
+extern "C" int gm_engine_save_state(gm_engine* e, void* host) {
+    char* p = (char*)host;
+    const uint32_t hdr[2] = {kStateMagic, (uint32_t)sizeof(GmStepDesc)};
+    memcpy(p, hdr, 8);
+    p += 8;
+    memcpy(p, &e->cfg, sizeof(gm_config));
+    p += sizeof(gm_config);
+    memcpy(p, &e->host, sizeof(GmStepDesc));
+    p += sizeof(GmStepDesc);
+    memcpy(p, e->s3, 3 * sizeof(int));
+    memcpy(p + 3 * sizeof(int), e->s2, 3 * sizeof(int));
+    p += 6 * sizeof(int);
+    for (const auto& a : state_arrays(e)) {
+        HIPCHK(hipMemcpyAsync(p, a.first, a.second, hipMemcpyDeviceToHost, e->s));
+        p += a.second;
+    }
+    HIPCHK(hipStreamSynchronize(e->s));
+    return 0;
+}
+
+extern "C" int gm_engine_load_state(gm_engine* e, const void* host, int64_t bytes) {
+    int64_t want = 0;
+    gm_engine_state_bytes(e, &want);
+    if (bytes != want) return fail("engine state: size does not match this deck");
+    const char* p = (const char*)host;
+    uint32_t hdr[2];
+    memcpy(hdr, p, 8);
+    if (hdr[0] != kStateMagic || hdr[1] != (uint32_t)sizeof(GmStepDesc))
+        return fail("engine state: not a state of this library");
+    p += 8;
+    if (memcmp(p, &e->cfg, sizeof(gm_config)) != 0)
+        return fail("engine state: saved from a different deck");
+    p += sizeof(gm_config);
+    memcpy(&e->host, p, sizeof(GmStepDesc));
+    p += sizeof(GmStepDesc);
+    memcpy(e->s3, p, 3 * sizeof(int));
+    memcpy(e->s2, p + 3 * sizeof(int), 3 * sizeof(int));
+    p += 6 * sizeof(int);
+    HIPCHK(hipStreamSynchronize(e->s));
+    for (const auto& a : state_arrays(e)) {
+        HIPCHK(hipMemcpy(a.first, p, a.second, hipMemcpyHostToDevice));
+        p += a.second;
+    }
+    // derived state: restriction path and strip layout, descriptor, tables
+    e->graph_ok = false;
+    e->fast_restrict = !pairs_aligned(e);  // force update_alignment to clear partials
+    if (update_alignment(e)) return -1;
+    if (upload_desc(e, desc_bytes(0))) return -1;
+    {
+        const GmPair ps[3] = {e->pair[0].p, e->pair[1].p, e->pair[2].p};
+        launch_axis_tables(ps, 3, e->d_desc, e->s);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->s));
+    return 0;
+}
+
+extern "C" int gm_engine_set_accum_mode(gm_engine* e, int external) {
+    const bool x = external != 0;
+    if (x != e->acc_external) {
+        e->acc_external = x;
+        e->graph_ok = false;  // the recorded step gathers/scatters Level0 or not
+    }
+    return 0;
+}
+
+extern "C" int gm_engine_overlap_box(gm_engine* e, int which, int32_t* lo, int32_t* n,
+                                     int32_t* stride) {
+    const GmStepDesc& h = e->host;
+    const gm_config& c = e->cfg;
+    for (int d = 0; d < 3; ++d) {
+        switch (which) {
+            case 0: lo[d] = h.l0_lo3[d]; n[d] = c.ov_n[0][d]; stride[d] = 1; break;
+            case 1: lo[d] = h.l0_lo2[d]; n[d] = c.ov_n[1][d]; stride[d] = h.l0_st2[d]; break;
+            case 2: lo[d] = h.ov2.lo[d]; n[d] = h.ov2.n[d]; stride[d] = 1; break;
+            case 3: lo[d] = h.ov3.lo[d]; n[d] = h.ov3.n[d]; stride[d] = 1; break;
+            default: return fail("overlap box: which must be 0..3");
+        }
+    }
     return 0;
 }
 
@@ -1298,7 +1428,9 @@ static void subcycle_body(gm_engine* e) {
     launch_face_bc(d, 2, 1, e->cfg.level[2], e->T1c, e->T1, e->fa2, e->fb2, s);
 
     // ---- Part 2: restart L2/L3 from the start-of-step state ------------------
-    launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, n3, 1, s);
+    // melt time of the L3 nodes: gathered from Level0 unless the caller
+    // passed the window arrays (the driver's max_accum_time[idx], cF:3224-3238)
+    if (!e->acc_external) launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, n3, 1, s);
     T2cur = e->T2;
     S12cur = e->S12;
     T3cur = e->T3;
@@ -1337,7 +1469,7 @@ static void subcycle_body(gm_engine* e) {
         S23cur = S23in;
         Tp3cur = e->histTp[k];
     }
-    launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, n3, 0, s);
+    if (!e->acc_external) launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, n3, 0, s);
     // final L2 T' and L1 T0 (cF:3625-3627), then commit the end-of-step state
     new_tprime(e, 2, T2cur, e->T1c, e->Tp2);
     {
@@ -1393,6 +1525,13 @@ extern "C" int gm_engine_subcycle(gm_engine* e, const float* rows, int n_rows) {
 }
 
 // ---- stepGOMELT (cF:2304-2397) ----------------------------------------------
+
+// _resetmask of cF:2394-2397: (1 - 2 S2_before) * S2_after == 1.
+__global__ void k_reset_mask(const uint8_t* __restrict__ pre, const uint8_t* __restrict__ post,
+                             uint8_t* __restrict__ reset, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) reset[t] = (uint8_t)((pre[t] == 0) && (post[t] != 0));
+}
 
 static void solutions(gm_engine* e, const float* V1, const float* V2, float* T1o, float* T2o,
                       float* T3o) {
@@ -1475,8 +1614,12 @@ extern "C" int gm_engine_step(gm_engine* e, const float* row) {
     launch_copy_f32(e->T1, T1o, n1, s);
     // Level0 state and the driver's melt-time bookkeeping (go_melt.py:339-357)
     launch_l0_scatter(d, e->S13, e->S23, e->L0S1, e->s2snap, e->s2box, n3, s);
-    launch_melt_single(d, e->T3, e->preS2, e->S23, row[5], e->mat.T_liquidus, e->acc, e->macc,
-                       n3, s);
+    // the reset mask stepGOMELT returns (cF:2394-2397): S2 went 0 -> 1
+    hipLaunchKernelGGL(k_reset_mask, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s,
+                       e->preS2, e->S23, e->reset3, n3);
+    if (!e->acc_external)
+        launch_melt_single(d, e->T3, e->preS2, e->S23, row[5], e->mat.T_liquidus, e->acc,
+                           e->macc, n3, s);
     HIPCHK(hipGetLastError());
     return 0;
 }

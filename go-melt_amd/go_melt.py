@@ -19,6 +19,7 @@ from __future__ import annotations
 import copy
 import json
 import os
+import pickle
 import sys
 import time
 from pathlib import Path
@@ -32,9 +33,14 @@ import computeFunctions as cf  # noqa: E402
 from createPath import count_lines, parsingGcode  # noqa: E402
 
 
-def _chunks(path, n):
-    """Yield lists of up to n parsed rows from the toolpath file."""
+def _chunks(path, n, skip_rows=0):
+    """Yield lists of up to n parsed rows from the toolpath file, from row
+    ``skip_rows`` on: the rows have a fixed width, so a resumed run seeks
+    ``skip_rows * line_length`` bytes (gm:124-126)."""
     with open(path, "r") as fh:
+        if skip_rows:
+            line_len = len(fh.readline())
+            fh.seek(skip_rows * line_len)
         while True:
             rows = []
             for _ in range(n):
@@ -52,8 +58,12 @@ class TimeLoop:
 
     level_names = ["L1", "L2", "L3"]
 
-    def __init__(self, solver_input, quiet=False):
+    def __init__(self, solver_input, quiet=False, write_outputs=True):
+        """``write_outputs`` False skips the per-layer Level0 state and melt-time
+        files (gm:243-261) and the final files; the timed benchmark loop uses
+        it (the Level0 map of the production deck is 10^9 nodes)."""
         self.tstart = time.time()
+        self.write_outputs = write_outputs
         self.say = (lambda *a, **k: None) if quiet else print
         self.quiet = quiet
         self.Properties = SetupProperties(solver_input.get("properties", {}))
@@ -85,9 +95,61 @@ class TimeLoop:
         self.stats = {"single": 0, "subcycle": 0, "dwell": 0, "chunks": [], "mm": 0.0}
         self.prev_row = None
         self.done = False
+        self.start_row = 0
+        if nm["layer_num"] > 0:
+            self._resume()
+
+    def _checkpoint_path(self, layer):
+        d = Path(self.Nonmesh["save_path"] + "checkpoint").absolute()
+        return d, d.joinpath(f"Checkpoint{str(layer).zfill(4)}.pkl")
+
+    def _resume(self):
+        """Load Checkpoint<layer_num>.pkl and continue after its last row
+        (gm:115-129).  The payload is the reference's five items; Levels[0]
+        also carries the engine's device state (window positions, fields of
+        the three levels), which the reference keeps inside its Levels."""
+        nm = self.Nonmesh
+        self.say(f"Checkpoint loading for start of Layer {nm['layer_num']}")
+        _, path = self._checkpoint_path(nm["layer_num"])
+        with open(path, "rb") as fh:
+            Ls, accum, max_accum, time_inc, record_inc = pickle.load(fh)
+        self.eng.load_state(Ls[0]["_engine_state"])
+        L = self.Levels
+        L[0]["S1"] = np.asarray(Ls[0]["S1"])
+        L[0]["accum_time"] = np.asarray(accum)
+        L[0]["max_accum_time"] = np.asarray(max_accum)
+        L[0]["node_coords"] = copy.deepcopy(Ls[0]["node_coords"])
+        self.eng.sync_coords()
+        self.load_chkpt = True
+        self.time_inc += int(time_inc)
+        self.record_inc = int(record_inc)
+        self.start_row = int(time_inc)
+
+    def _save_checkpoint(self):
+        """gm:390-401: [Levels, accum_time, max_accum_time, time_inc,
+        record_inc] after the chunk in which a new layer started."""
+        nm, L = self.Nonmesh, self.Levels
+        cdir, path = self._checkpoint_path(nm["layer_num"])
+        os.makedirs(cdir, exist_ok=True)
+        self.eng.sync()
+        save_object([L, np.asarray(L[0]["accum_time"]), np.asarray(L[0]["max_accum_time"]),
+                     self.time_inc, self.record_inc], path,
+                    extra={"_engine_state": self.eng.save_state()})
+        self.say("Saved Checkpoint")
+
+    def _save_layer_outputs(self):
+        """gm:243-261: Level0 state and the melt-time map of the layer that
+        ends, written before Level0 shifts down one layer."""
+        if not self.write_outputs:
+            return
+        L, nm = self.Levels, self.Nonmesh
+        self.eng.sync()
+        saveState(L[0], "Level0_", nm["layer_num"], nm["save_path"], 0)
+        acc = np.maximum(np.asarray(L[0]["accum_time"]), np.asarray(L[0]["max_accum_time"]))
+        np.savez(nm["save_path"] + "accum_time" + str(nm["layer_num"]).zfill(4), accum_time=acc)
 
     def chunks(self):
-        return _chunks(self.Nonmesh["toolpath"], self.subcycle[2])
+        return _chunks(self.Nonmesh["toolpath"], self.subcycle[2], self.start_row)
 
     # -- one chunk of N2*N3 rows ------------------------------------------
     def step_chunk(self, rows):
@@ -116,10 +178,7 @@ class TimeLoop:
             if self.new_checkpoint:
                 nm["layer_num"] += 1
                 self.say(f"Saving checkpoint for Layer {nm['layer_num']}")
-                cdir = Path(nm["save_path"] + "checkpoint").absolute()
-                os.makedirs(cdir, exist_ok=True)
-                save_object([L, self.time_inc, self.record_inc],
-                            cdir.joinpath(f"Checkpoint{str(nm['layer_num']).zfill(4)}.pkl"))
+                self._save_checkpoint()
                 if nm["layer_num"] == self.layer_check:
                     keep_going = False
                 self.new_checkpoint = False
@@ -132,8 +191,7 @@ class TimeLoop:
             _, self.Shapes, self.LInterp, self.move_hist = moveEverything(
                 la[0, :], self.laser_start, L, self.move_hist, self.LInterp, eng.r12, eng.r23,
                 P["layer_height"])
-            subcycleGOMELT(L, self.ne_nn, self.Shapes, self.substrate, self.LInterp,
-                           self.tmp_ne_nn, la, P, la[:, 6], sub)
+            self.run_subcycle(la)
             self.stats["subcycle"] += 1
             self.time_inc += la.shape[0]
             self.record_inc += la.shape[0]
@@ -172,6 +230,7 @@ class TimeLoop:
                 tmp[2] = (tmp[2] + np.float32(P["layer_height"])).astype(np.float32)
                 kidx += 1
             if not self.load_chkpt:
+                self._save_layer_outputs()
                 cf.N.check(cf.N.lib().gm_engine_layer_change(eng.h, kidx, float(tmp[2][0]),
                                                              float(pos[2]), 1))
                 L[1]["node_coords"] = copy.deepcopy(tmp)
@@ -191,8 +250,9 @@ class TimeLoop:
             cf.N.check(cf.N.lib().gm_engine_level0_substrate(eng.h, self.substrate[0]))
             self.say("Start of new layer")
         if self.wait_inc <= nm["wait_time"]:
-            stepGOMELT(L, self.ne_nn, self.tmp_ne_nn, self.Shapes, self.LInterp, pos, P, pos[5],
-                       pos[6], self.substrate)
+            _, all_reset = stepGOMELT(L, self.ne_nn, self.tmp_ne_nn, self.Shapes, self.LInterp,
+                                      pos, P, pos[5], pos[6], self.substrate)
+            self.accum_single(pos, all_reset)
             self.stats["single"] += 1
             if nm["info_T"]:
                 self.say(f"Step {self.time_inc + 1} / {self.total_t_inc}")
@@ -209,13 +269,27 @@ class TimeLoop:
         self.time_inc += 1
         self.record_inc += 1
 
+    # -- melt-time bookkeeping (gm:339-357, 436-456) ------------------------
+    # The engine keeps the melt-time maps on its own Level0 (subcycleGOMELT
+    # without window arrays; stepGOMELT applies gm:339-357 on the device).
+    # A driver that keeps its own maps, as the reference does, overrides
+    # these two methods (tests/test_gpu_reference_driver.py runs the
+    # reference's lines that way).
+    def run_subcycle(self, la):
+        subcycleGOMELT(self.Levels, self.ne_nn, self.Shapes, self.substrate, self.LInterp,
+                       self.tmp_ne_nn, la, self.Properties, la[:, 6], self.subcycle)
+
+    def accum_single(self, pos, all_reset):
+        pass
+
     def finish(self):
         """Final output (gm:495-530)."""
         L, nm = self.Levels, self.Nonmesh
         self.eng.sync()
         self.eng.sync_coords()
-        if nm["output_files"] == 1:
-            saveState(L[0], "Level0_", nm["layer_num"], nm["save_path"], 0)
+        if not self.write_outputs:
+            return
+        saveState(L[0], "Level0_", nm["layer_num"], nm["save_path"], 0)
         saveResultsFinal(L, nm)
         np.savez(f"{nm['save_path']}FinalTemperatureFields", L1T=np.asarray(L[1]["T0"]),
                  L2T=np.asarray(L[2]["T0"]), L3T=np.asarray(L[3]["T0"]))

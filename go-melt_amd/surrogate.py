@@ -393,13 +393,10 @@ def make_windows(instances, K=10, warm=20, n_snap=4, gap=10):
         def sync():
             torch.cuda.current_stream().synchronize()
 
+    check_one_mesh_size(instances)
     bt = B.Batch.from_instances(instances, stream=_TorchStream())
     g0 = bt.grids[0]
     D, H, Wd = g0.n[2], g0.n[1], g0.n[0]
-    for b_ in range(len(instances)):
-        g = bt.grids[b_]
-        if (g.n[0], g.n[1], g.n[2]) != (Wd, H, D):
-            raise ValueError("make_windows needs instances of one mesh size")
     nw = len(instances) * n_snap
     X = torch.empty((nw, D, H, Wd, 16), dtype=torch.bfloat16, device="cuda")
     Y = torch.empty((nw, D * H * Wd), dtype=torch.float32, device="cuda")
@@ -537,6 +534,19 @@ def _jump_audit(inst, grid, T0, Tn, rows, K):
     return float(sums[6].item())
 
 
+def check_one_mesh_size(instances):
+    """Raise ValueError unless every instance has the same element counts: the
+    operator's window tensors and the per-window kernels (gm_nn_window_input,
+    gm_nn_window_apply) take one (D, H, W) for the whole batch."""
+    if not instances:
+        raise ValueError("no instances")
+    el0 = list(instances[0]["elements"])
+    for i, b in enumerate(instances):
+        if list(b["elements"]) != el0:
+            raise ValueError(f"instance {i} has elements {list(b['elements'])}, instance 0 "
+                             f"{el0}: the learned operator needs windows of one mesh size")
+
+
 def run_in_loop(net, instances, n_jumps, K=10, warm=20, compare_every=2, residual_limit=0.01):
     """Advance batched windows by operator jumps of K steps next to the plain
     batched solver from the same start.  Each jump is audited
@@ -560,6 +570,8 @@ def run_in_loop(net, instances, n_jumps, K=10, warm=20, compare_every=2, residua
         def sync():
             torch.cuda.current_stream().synchronize()
 
+    # the network input and output buffers below have one window shape
+    check_one_mesh_size(instances)
     plain = B.Batch.from_instances(instances, stream=_TorchStream())
     sur = B.Batch.from_instances(instances, stream=_TorchStream())
     n = len(instances)

@@ -227,7 +227,10 @@ enum {
     GM_MAX_ACCUM = 5, /* Level0 FP32 max accumulated melt time               */
     GM_HIST_L2 = 6,   /* corrector L2 T after L2 substep `level` (L2all)     */
     GM_HIST_L3 = 7,   /* corrector L3 T after L2 substep `level` (L3all)     */
-    GM_HIST_TP3 = 8   /* corrector L3 T' after L2 substep `level` (L3pall)   */
+    GM_HIST_TP3 = 8,  /* corrector L3 T' after L2 substep `level` (L3pall)   */
+    GM_ACCUM_L3 = 9,  /* FP32 melt time on the L3 window nodes (level ignored)*/
+    GM_MAX_ACCUM_L3 = 10, /* FP32 max melt time on the L3 window nodes        */
+    GM_RESET_L3 = 11  /* uint8 reset mask of the last gm_engine_step         */
 };
 
 /* SetupLevels (cF:115-264) + getStaticNodesAndElements/getStaticSubcycle:
@@ -261,9 +264,44 @@ int gm_engine_move(gm_engine* e, const float* v_host, const float* vstart_host,
  * given layer state records a hipGraph; later calls replay it. */
 int gm_engine_subcycle(gm_engine* e, const float* rows_host, int n_rows);
 
-/* stepGOMELT (cF:2304-2397) for one toolpath row; also applies the melt-time
- * bookkeeping of the single-step driver path (go_melt.py:339-357). */
+/* stepGOMELT (cF:2304-2397) for one toolpath row.  Writes the reference's
+ * return value _resetmask (cF:2394-2397: L3 nodes whose S2 went 0 -> 1) into
+ * the GM_RESET_L3 field.  In the engine's own melt-time mode (the default,
+ * see gm_engine_set_accum_mode) it also applies the single-step driver's
+ * melt-time bookkeeping (go_melt.py:339-357) to the Level0 maps. */
 int gm_engine_step(gm_engine* e, const float* row_host);
+
+/* Engine state for the driver's per-layer checkpoint (go_melt.py:390-411)
+ * and its resume (go_melt.py:115-129): every persistent array of the three
+ * levels (T0, T', S1, S2, the L1 S1 storage rows, the L3 melt-time window)
+ * and the window positions and overlap boxes, in one host blob of
+ * gm_engine_state_bytes bytes.  Level0 (S1 and the melt-time maps) is not in
+ * it: the checkpoint stores those as the reference does (Levels[0],
+ * accum_time, max_accum_time).  load_state accepts only a blob saved by an
+ * engine of the same deck (gm_config).  Both are synchronous. */
+int gm_engine_state_bytes(gm_engine* e, int64_t* bytes);
+int gm_engine_save_state(gm_engine* e, void* host);
+int gm_engine_load_state(gm_engine* e, const void* host, int64_t bytes);
+
+/* Who keeps the melt-time maps.  external = 0 (default): the engine keeps
+ * them on its Level0 (GM_ACCUM, GM_MAX_ACCUM); gm_engine_subcycle gathers
+ * the L3 window from them and scatters it back, gm_engine_step updates them.
+ * external = 1: the caller keeps them, as the reference driver does
+ * (go_melt.py:339-357, 436-456): before gm_engine_subcycle it writes the
+ * window values max_accum_time[idx], accum_time[idx] into GM_MAX_ACCUM_L3 and
+ * GM_ACCUM_L3, after it reads the updated window back from them
+ * (subcycleGOMELT's inputs and returns, cF:3224-3238, 3568-3578);
+ * gm_engine_step leaves melt time alone. */
+int gm_engine_set_accum_mode(gm_engine* e, int external);
+
+/* Current overlap index boxes after the last window move (moveEverything
+ * cF:2477-2497 and update_overlap_nodes_coords*, cF:1727-1845): which = 0:
+ * Level0 nodes of the L3 window (Levels[0]["overlapNodes"], whose flat
+ * indices are Levels[0]["idx"]); 1: Level0 nodes of the L2 window, strided
+ * ("overlapNodes_L2", "idx_L2"); 2: L1 nodes inside L2; 3: L2 nodes inside
+ * L3.  Axis d covers lo[d] + stride[d] * (0 .. n[d]-1).  3 ints each, host. */
+int gm_engine_overlap_box(gm_engine* e, int which, int32_t* lo_host, int32_t* n_host,
+                          int32_t* stride_host);
 
 /* stepGOMELTDwellTime (cF:2617-2664): L1-only step of length dt. */
 int gm_engine_step_dwell(gm_engine* e, float dt);

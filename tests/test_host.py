@@ -192,3 +192,15 @@ def test_device_code_has_no_calls_or_recursion(tmp_path):
         asm = out.read_text()
         assert "s_swappc_b64" not in asm, f"{src}: device function call"
         assert "has_recursion, 1" not in asm, f"{src}: recursive device function"
+
+
+def test_operator_loop_rejects_mixed_window_sizes():
+    """run_in_loop / make_windows take one window shape for the whole batch;
+    mixed mesh sizes are refused on the host before any launch."""
+    import surrogate as S
+    same = W.batch_instances(n=3, seed=1, sizes=[(32, 32, 16)])
+    S.check_one_mesh_size(same)
+    mixed = W.batch_instances(n=8, seed=0)
+    assert len({tuple(b["elements"]) for b in mixed}) > 1
+    with pytest.raises(ValueError, match="one mesh size"):
+        S.check_one_mesh_size(mixed)
