@@ -494,7 +494,8 @@ class Engine:
         n = C.c_int64()
         N.check(N.lib().gm_engine_field(self.h, level, field, C.byref(p), C.byref(n)))
         dt = np.float32
-        if field in (N.GM_S1, N.GM_S2) and not (field == N.GM_S1 and level == 1):
+        if (field in (N.GM_S1, N.GM_S2) and not (field == N.GM_S1 and level == 1)) or \
+                field == N.GM_RESET_L3:
             dt = np.uint8
         return DeviceArray(n.value, dt, ptr=p.value, owner=False)
 

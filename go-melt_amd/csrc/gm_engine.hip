@@ -28,8 +28,10 @@ static int fail(const std::string& msg) {
 #define HIPCHK(call)                                                                    \
     do {                                                                                \
         hipError_t e_ = (call);                                                         \
-        if (e_ != hipSuccess)                                                           \
+        if (e_ != hipSuccess) {                                                         \
+            (void)hipGetLastError(); /* the caller gets the error; do not leave it */   \
             return fail(std::string(#call) + ": " + hipGetErrorString(e_));             \
+        }                                                                               \
     } while (0)
 
 extern "C" const char* gm_last_error(void) { return g_err.c_str(); }

@@ -48,10 +48,10 @@ def test_dwell_steps_vs_oracle(oracle, tmp_path):
     from createPath import read_rows
     inp, g = _dwell_deck()
     deck = _write(inp, g, tmp_path)
+    st = {}
+    Lg = G.go_melt(deck, quiet=True, stats=st)  # writes the toolpath (cP:6-188)
     rows = read_rows(deck["nonmesh"]["toolpath"])
     assert np.count_nonzero(rows[:, 4] == 0) > inp["nonmesh"]["wait_time"]
-    st = {}
-    Lg = G.go_melt(deck, quiet=True, stats=st)
     Lo, info = oracle.go_melt(deck, rows=rows)
     assert st["chunks"] == info["chunks"]
     assert st["dwell"] == info["dwell"] and st["dwell"] >= 3, (st["dwell"], info["dwell"])
