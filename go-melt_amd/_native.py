@@ -114,6 +114,7 @@ _SIGS = {
     "gm_interpolate_matrix": (_I, [_G, _G, _P, _P, _P]),
     "gm_interpolate_w_matrix": (_I, [_P, _P, _I64, _P, _P, _P]),
     "gm_restrict": (_I, [_G, _G, _M, _P, _P, _I64, _P, _P, _P, _F, _P, _P]),
+    "gm_restrict_noalloc": (_I, [_G, _G, _M, _P, _P, _I64, _P, _P, _P, _F, _P, _P]),
     "gm_new_tprime": (_I, [_G, _P, _G, _P, _IP, _IP, _P, _P]),
     "gm_engine_create": (_I, [C.POINTER(gm_config), _M, _P, C.POINTER(_P)]),
     "gm_engine_destroy": (_I, [_P]),

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,10 +41,13 @@ extern "C" const char* gm_last_error(void) { return g_err.c_str(); }
 // Device, memory, streams
 // ---------------------------------------------------------------------------
 
+static int scratch_ptr(void** out);
+
 extern "C" int gm_device_init(int device) {
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipFree(nullptr));
-    return 0;
+    void* p = nullptr;
+    return scratch_ptr(&p);  // the device's address of the no-allocation scratch
 }
 
 extern "C" int gm_device_name(char* buf, int len) {
@@ -434,6 +438,102 @@ extern "C" int gm_restrict(const gm_grid* coarse, const gm_grid* fine, const gm_
     HIPCHK(hipFree(part));
     HIPCHK(hipFree(V));
     HIPCHK(hipFree(pm.blob));
+    return 0;
+}
+
+// ---- gm_restrict without allocation ---------------------------------------
+// Scratch of gm_restrict_noalloc: a step descriptor holding the two grids, the
+// pair's axis tables, the FP64 element partials and the coarse node box, laid
+// out one after the other (256-byte aligned) in one static device array.
+// Every GPU holds its own copy until the process ends; host code reaches it
+// only through the per-device address from hipGetSymbolAddress.
+#define GM_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+#define GM_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) uint8_t scratch[GM_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[GM_NOALLOC_MAX_DEVICES];
+
+static int scratch_ptr(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= GM_NOALLOC_MAX_DEVICES) return fail("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+__global__ void k_put_grids12(GmStepDesc* d, gm_grid g1, gm_grid g2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d->g[1] = g1;
+        d->g[2] = g2;
+    }
+}
+
+__global__ void k_zero_words(uint32_t* p, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0u;
+}
+
+extern "C" int gm_restrict_noalloc(const gm_grid* coarse, const gm_grid* fine,
+                                   const gm_material* mat, const float* T, const float* S1,
+                                   int64_t nsub, const float* Tp, const float* TpA,
+                                   const float* TpB, float dt, float* V_out, void* stream) {
+    if (!coarse || !fine || !mat || !T || !S1 || !V_out)
+        return fail("gm_restrict_noalloc: null argument");
+    for (int d = 0; d < 3; ++d)
+        if (coarse->n[d] < 2 || fine->n[d] < 2)
+            return fail("gm_restrict_noalloc: need >= 2 nodes per axis on both grids");
+    hipStream_t s = (hipStream_t)stream;
+    void* base = nullptr;
+    if (scratch_ptr(&base)) return -1;
+    GmPair p{};
+    const size_t blob_bytes = pair_layout(p, 1, 2, *coarse, *fine);
+    const size_t o_blob = align256(sizeof(GmStepDesc));
+    const size_t o_part = o_blob + align256(blob_bytes);
+    const size_t o_V = o_part + align256((size_t)pair_elems(p) * 8 * sizeof(double));
+    const size_t total = o_V + align256((size_t)pair_nodes(p) * 4);
+    if (total > GM_NOALLOC_SCRATCH_BYTES)
+        return fail("gm_restrict_noalloc: this pair needs " + std::to_string(total) +
+                    " bytes of scratch; the limit is " +
+                    std::to_string(GM_NOALLOC_SCRATCH_BYTES) + " (use gm_restrict)");
+    char* b = (char*)base;
+    GmStepDesc* d = (GmStepDesc*)b;
+    pair_bind(p, b + o_blob);
+    double* part = (double*)(b + o_part);
+    float* V = (float*)(b + o_V);
+    // the descriptor's grids and cleared axis tables, written by kernels (no
+    // host copy, no memset): the scratch keeps the previous call's contents
+    hipLaunchKernelGGL(k_put_grids12, dim3(1), dim3(64), 0, s, d, *coarse, *fine);
+    const int64_t words = (int64_t)(blob_bytes / 4);
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s,
+                       (uint32_t*)(b + o_blob), words);
+    launch_axis_tables(&p, 1, d, s);
+    RArgs r;
+    memset(&r, 0, sizeof(r));
+    r.T = T;
+    r.S1f = S1;
+    r.n_sub = nsub;
+    r.Tp = Tp;
+    r.TpA = TpA;
+    r.TpB = TpB;
+    r.dt = dt;
+    r.dt_mode = GM_DT_VALUE;
+    float kc[8], mrow, w2, w3;
+    level_coeffs(*fine, kc, &mrow, &w2, &w3);
+    r.wf = w3;
+    launch_restrict(p, d, make_devmat(*mat), r, part, s);
+    launch_restrict_nodes(p, part, nullptr, nullptr, nullptr, V, nullptr, nullptr, 0, 0, 0.0f, s);
+    const int64_t n = grid_nn(*coarse);
+    hipLaunchKernelGGL(k_box_to_full, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, V,
+                       *coarse, V_out);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

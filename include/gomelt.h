@@ -205,6 +205,22 @@ int gm_restrict(const gm_grid* coarse, const gm_grid* fine, const gm_material* m
                 const float* T, const float* S1, int64_t n_substrate, const float* Tp,
                 const float* TpA, const float* TpB, float dt, float* V, void* stream);
 
+/* gm_restrict without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as gm_restrict, but the descriptor, axis
+ * tables, FP64 element partials and node box live in a static device array of
+ * 16 MiB per GPU (held until the process ends), so the call allocates and
+ * frees nothing, copies nothing from the host and does not synchronise: the
+ * result is ready when `stream` is.  A pair whose scratch would exceed 16 MiB
+ * is rejected with an error before any launch (use gm_restrict for it).  All
+ * calls on one GPU share that GPU's array: calls that overlap in time on one
+ * GPU (other streams, other threads) race on it.  Like gm_restrict, V is
+ * added to, so the caller clears it first. */
+int gm_restrict_noalloc(const gm_grid* coarse, const gm_grid* fine, const gm_material* mat,
+                        const float* T, const float* S1, int64_t n_substrate, const float* Tp,
+                        const float* TpA, const float* TpB, float dt, float* V, void* stream);
+
 /* getNewTprime (cF:2060-2099): CoarseT[overlap box] <- fine field interpolated
  * at those coarse nodes; then Tprime = FineT - prolong(CoarseT). */
 int gm_new_tprime(const gm_grid* fine, const float* FineT, const gm_grid* coarse,
