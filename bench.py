@@ -14,7 +14,8 @@ The JSON line also carries element updates per second, the roofline of the
 L3 level-step kernel (algorithmic 10 B/node over the mean launch duration,
 measured here with HIP events on the engine stream; HBM traffic per launch
 from the FETCH_SIZE/WRITE_SIZE profile in profiles/) and the CPU baseline (the
-NumPy oracle on one core, a bounded sample of the same L1 step, extrapolated).
+reference algorithm's element loops restated in C++/OpenMP, FP32, on all host
+cores, timed over one whole L1 step of the same deck).
 """
 
 from __future__ import annotations
@@ -167,26 +168,65 @@ def batch_measure(n, iters=50):
     }
 
 
-def cpu_baseline(inp, gcode, n23, mm_per_step):
-    """NumPy oracle (FP32, one core) on a bounded sample of one L1 step."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(inp, gcode, n23):
+    """The reference algorithm on the host cores: the C++/OpenMP FP32
+    restatement of its element loops (oracle/cpu_port.cpp), driven by the
+    oracle's orchestration (moveEverything + subcycleGOMELT).  Sample: one
+    whole subcycled L1 step (the rows of the deck's second chunk), timed, not
+    extrapolated."""
     import numpy as np
+    import workloads as W
     from oracle import gomelt_oracle as O
-    O.set_precision(np.float32)
-    nm = O.SetupNonmesh(dict(inp["nonmesh"], save_path="/tmp/"))
-    p = O.SetupProperties(inp["properties"])
-    _, text = O.parsingGcode(dict(nm, toolpath=None), p, None, gcode_text=gcode)
-    lines = text.splitlines()[n23:2 * n23]
-    rows = np.array([[float(v) for v in ln.split(",")] for ln in lines])
-    t = O.sample_l1_step_cost(inp, rows)
+    from oracle import cpu_port
+    cpu_port.install(O)  # FP32 element loops in C++
+    try:
+        nm = O.SetupNonmesh(dict(inp["nonmesh"], save_path="/tmp/"))
+        p = O.SetupProperties(inp["properties"])
+        L = O.SetupLevels(inp, p)
+        ne_nn = O.getStaticNodesAndElements(L)
+        sub = O.getStaticSubcycle(nm)
+        _, text = O.parsingGcode(dict(nm, toolpath=None), p, L[2]["h"], gcode_text=gcode)
+        allrows = np.array([[float(v) for v in ln.split(",")]
+                            for ln in text.splitlines()[:2 * n23]])
+        lp = allrows[n23:2 * n23]
+        r12 = [int(np.round(L[1]["h"][i] / L[2]["h"][i])) for i in range(2)] + \
+              [int(np.round(p["layer_height"] / L[2]["h"][2]))]
+        r23 = [int(np.round(L[2]["h"][i] / L[3]["h"][i])) for i in range(3)]
+        vstart = np.array(p["laser_center"], np.float64) if p["laser_center"] else allrows[0]
+        tmp = O.calcStaticTmpNodesAndElements(L, lp[0])
+        substrate = O.getSubstrateNodes(L)
+        t0 = time.perf_counter()
+        L, Shapes, LI, _ = O.moveEverything(lp[0], vstart, L, [0, 0, 0], [None, None], r12,
+                                            r23, p["layer_height"])
+        n3 = L[0]["idx"].size
+        O.subcycleGOMELT(L, ne_nn, Shapes, substrate, LI, tmp, lp, p, lp[:, 6], sub,
+                         np.zeros(n3, np.float32), np.zeros(n3, np.float32))
+        t1 = time.perf_counter()
+        mm = W.scan_length_mm(allrows[n23 - 1:2 * n23])
+        cores = cpu_port.threads()
+    finally:
+        cpu_port.uninstall(O)
+        O.set_precision("float64")
     return {
-        "value": mm_per_step / t["step_s"],
+        "value": mm / (t1 - t0),
         "unit": "mm/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": ("NumPy oracle, FP32, 1 core, same deck: moveEverything + L1 predictor "
-                   "(sources, T' flux restriction, solve) + one Part-1 L2 substep (L2 solve, "
-                   f"{t['N3']} L3 solves, T'), measured {t['measured_s']:.1f} s; L1-step time "
-                   f"extrapolated as move + 2*L1 + 2*N2*substep = {t['step_s']:.1f} s"),
+        "cpu_model": _cpu_model(),
+        "sample": (f"one whole subcycled L1 step of the same deck ({n23} toolpath rows, "
+                   f"{mm:.3f} mm of laser path): moveEverything + subcycleGOMELT of the "
+                   f"oracle's orchestration with the C++/OpenMP FP32 element loops "
+                   f"(oracle/cpu_port.cpp) on {cores} threads, {t1 - t0:.2f} s"),
     }
 
 
@@ -299,7 +339,7 @@ def main():
     }
     if not a.no_cpu:
         try:
-            result["cpu_baseline"] = cpu_baseline(inp, gcode, N2 * N3, mm / max(a.steps, 1))
+            result["cpu_baseline"] = cpu_baseline(inp, gcode, N2 * N3)
         except Exception as exc:  # the baseline must never hide the GPU line
             result["cpu_baseline"] = {"error": repr(exc)}
     if not a.no_surrogate:

@@ -1496,70 +1496,3 @@ def go_melt(solver_input, rows=None, max_rows=None, log=None):
 def load_input(path):
     with open(path) as fh:
         return json.load(fh)
-
-
-# ---------------------------------------------------------------------------
-# Bounded timing sample of one subcycled L1 step (CPU baseline of bench.py)
-# ---------------------------------------------------------------------------
-
-def sample_l1_step_cost(solver_input, rows):
-    """Time the pieces of one subcycled L1 step of this oracle on ``rows``
-    (N2*N3 toolpath rows): the window move, the L1 predictor (sources, flux
-    restriction, solve) and ONE Part-1 L2 substep (L2 source/restriction/solve,
-    N3 L3 solves, T').  Returns a dict of seconds and the extrapolated cost of
-    the whole step, move + 2 * L1 + 2 * N2 * (L2 substep)."""
-    import time as _t
-    p = SetupProperties(solver_input.get("properties", {}))
-    Levels = SetupLevels(solver_input, p)
-    nm = SetupNonmesh(dict(solver_input.get("nonmesh", {}), save_path="/tmp/"))
-    ne_nn = getStaticNodesAndElements(Levels)
-    sub = getStaticSubcycle(nm)
-    N2, N3 = sub[0], sub[1]
-    r12 = [int(np.round(Levels[1]["h"][i] / Levels[2]["h"][i])) for i in range(2)] + \
-          [int(np.round(p["layer_height"] / Levels[2]["h"][2]))]
-    r23 = [int(np.round(Levels[2]["h"][i] / Levels[3]["h"][i])) for i in range(3)]
-    lp = np.asarray(rows, dtype=np.float64)
-    vstart = np.array(p["laser_center"], dtype=np.float64) if p["laser_center"] else lp[0]
-    LInterp = [None, None]
-    tmp = calcStaticTmpNodesAndElements(Levels, lp[0])
-    substrate = getSubstrateNodes(Levels)
-    t0 = _t.perf_counter()
-    Levels, Shapes, LInterp, _ = moveEverything(lp[0], vstart, Levels, [0, 0, 0], LInterp, r12,
-                                                r23, p["layer_height"])
-    t1 = _t.perf_counter()
-    L1, L2, L3 = Levels[1], Levels[2], Levels[3]
-    P = lp[:, 6]
-    _, _, L3k, _ = computeStateProperties(L3["T0"], L3["S1"], p, substrate[3])
-    _, _, L2k, _ = computeStateProperties(L2["T0"], L2["S1"], p, substrate[2])
-    _, _, L1k, L1rc = computeStateProperties(L1["T0"], L1["S1"], p, substrate[1])
-    L1F = computeLevelSource(Levels, ne_nn, lp, Shapes[1], p, P)
-    L1F = computeConvRadBC(L1, L1["T0"], tmp[0], ne_nn[2], p, L1F)
-    L1V = computeL1TprimeTerms_Part1(Levels, ne_nn, L3k, Shapes, L2k)
-    L1T = _clamp(computeL1Temperature(Levels, ne_nn, tmp, L1F, L1V, L1k, L1rc,
-                                      float(F(lp[:, 5]).sum()), p), p)
-    t2 = _t.perf_counter()
-    rws = np.arange(N3)
-    _, _, L3k2, _ = computeStateProperties(L3["T0"], L3["S1"], p, substrate[3])
-    S12, _, L2k2, L2rc2 = computeStateProperties(L2["T0"], L2["S1"], p, substrate[2])
-    L2F = computeLevelSource(Levels, ne_nn, lp[rws], Shapes[2], p, P[rws])
-    L2F = computeConvRadBC(L2, L2["T0"], ne_nn[0], ne_nn[3], p, L2F)
-    L2V = computeL2TprimeTerms_Part1(Levels, ne_nn, L3["Tprime0"], L3k2, Shapes)
-    BC = (F(1.0 / N2) * L1T + F(1 - 1.0 / N2) * L1["T0"]).astype(FLOAT)
-    L2T = _clamp(computeL2Temperature(BC, LInterp[0], Levels, ne_nn, L2["T0"], L2F, L2V, L2k2,
-                                      L2rc2, float(F(lp[rws, 5]).sum())), p)
-    T3, S13 = L3["T0"], L3["S1"]
-    for j in range(N3):
-        S13n, _, k3, c3 = computeStateProperties(T3, S13, p, substrate[3])
-        F3 = computeSourcesL3(L3, lp[j], ne_nn, p, P[j])
-        F3 = computeConvRadBC(L3, T3, ne_nn[1], ne_nn[4], p, F3)
-        a3 = (j + 1) / float(N3)
-        BC3 = (F(a3) * L2T + F(1 - a3) * L2["T0"]).astype(FLOAT)
-        T3 = _clamp(computeSolutions_L3(BC3, LInterp[1], Levels, ne_nn, T3, F3, k3, c3, lp[j, 5]),
-                    p)
-        S13 = S13n
-    getNewTprime(L3, T3, L2T, L2, LInterp[1])
-    t3 = _t.perf_counter()
-    out = {"move_s": t1 - t0, "l1_s": t2 - t1, "l2_substep_s": t3 - t2, "N2": N2, "N3": N3}
-    out["step_s"] = out["move_s"] + 2 * out["l1_s"] + 2 * N2 * out["l2_substep_s"]
-    out["measured_s"] = t3 - t0
-    return out
