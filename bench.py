@@ -51,54 +51,98 @@ def parse():
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned-operator measurement")
-    ap.add_argument("--surrogate-iters", type=int, default=1000)
+    ap.add_argument("--surrogate-iters", type=int, default=1250)
+    ap.add_argument("--surrogate-windows", type=int, default=20000)
+    ap.add_argument("--c5-steps", type=int, default=20)
     ap.add_argument("--surrogate-only", action="store_true",
                     help="print only the learned-operator measurement (one JSON line)")
     return ap.parse_args()
 
 
-def surrogate_measure(iters):
+def c5_run(net, steps, warmup=3, residual_limit=0.01, judge_every=200, lines=2):
+    """The part-scale deck (L1 400 x 400 x 100 elements, the operator's
+    64 x 64 x 32-node L3 window), its first `lines` raster lines of layer 1:
+    `warmup` chunks (the first is single-step), then `steps` timed L1 steps,
+    plain (net None) or with the operator advancing the L3 window."""
+    import gc
+    import torch
+    import go_melt as G
+    import computeFunctions as cf
+    import surrogate as S
+    import workloads as W
+    inp, gcode = W.part_scale(lines=lines, layers=1)
+    work = tempfile.mkdtemp(prefix="gomelt_c5_")
+    with open(W.write_deck(inp, gcode, work)) as fh:
+        deck = json.load(fh)
+    run = G.TimeLoop(deck, quiet=True, write_outputs=False)
+    op = None
+    if net is not None:
+        op = S.EngineOperator(net, run.Properties, residual_limit=residual_limit,
+                              judge_every=judge_every)
+    cf.attach_l3_operator(run.Levels, op)
+    it = run.chunks()
+    for _ in range(warmup):
+        run.step_chunk(next(it))
+    run.eng.sync()
+    torch.cuda.synchronize()
+    if op is not None:
+        op.stats.update({"jumps": 0, "fallbacks": 0, "residuals": [], "judged": []})
+    mm0, n0 = run.stats["mm"], run.stats["subcycle"]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if not run.step_chunk(next(it)):
+            break
+    run.eng.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"scan_mm": run.stats["mm"] - mm0, "subcycled_steps": run.stats["subcycle"] - n0,
+           "wall_s": wall, "mm_per_s": (run.stats["mm"] - mm0) / wall}
+    if op is not None:
+        import numpy as np
+        st = op.stats
+        res = np.abs(np.asarray(st["residuals"], np.float64))
+        out.update({"jumps": st["jumps"], "fallbacks": st["fallbacks"],
+                    "residual_median": float(np.median(res)) if res.size else None,
+                    "residual_max": float(np.max(res)) if res.size else None,
+                    "judged": st["judged"]})
+    del run, op
+    gc.collect()
+    return out
+
+
+def surrogate_measure(n_windows, iters, c5_steps):
     """Learned L3 operator (3-D U-Net, channels 32-64-128-128, BF16 MFMA
-    convolutions): trained for `iters` Adam steps (batch 4; learning rate
-    1e-3, halved after 400 steps) on 128 windows of 64 x 64 x 32 nodes cut
-    from the plain batched solver (K = 10 steps per jump), then one jump on
-    held-out windows against 10 plain steps, and runs in the loop with the
-    1% energy trigger and without it."""
+    convolutions, FP32 accumulation, Adam): trained on windows of the plain
+    three-level engine (every L2 substep of seeded single-track and raster
+    runs: the window's field, new face values, laser row -> the field after
+    the N3 plain L3 solves), batch 16; then run in the driver's moving L3
+    window of the part-scale deck, audited by the energy balance (1% limit)
+    and judged by the plain solves every 200 jumps, next to the plain run."""
     import numpy as np
     import torch
-    import workloads as W
     import surrogate as S
-    import computeFunctions as cf
-    tr = W.batch_instances(n=32, seed=21, sizes=[(63, 63, 31)])
-    te = W.batch_instances(n=4, seed=22, sizes=[(63, 63, 31)])
-    Xtr, Ytr, _, _ = S.make_windows(tr, K=10, warm=20, n_snap=4, gap=10)
-    Xte, Yte, keep, bt = S.make_windows(te, K=10, warm=20, n_snap=1, gap=0)
     net = S.UNet(channels=(32, 64, 128, 128), seed=0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    n1 = min(iters, 400)
-    hist = S.train(net, Xtr, Ytr, iters=n1, batch=4, lr=1e-3)
+    X, Y, seeds = S.engine_windows(n_windows)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    seeds = np.asarray(seeds)
+    held = seeds == seeds[-1]  # the last run's windows are held out
+    ntr = int(np.count_nonzero(~held))
+    Xtr, Ytr, Xte, Yte = X[:ntr], Y[:ntr], X[ntr:], Y[ntr:]
+    t0 = time.perf_counter()
+    n1 = min(iters, max(iters // 2, 1))
+    hist = S.train(net, Xtr, Ytr, iters=n1, batch=16, lr=1e-3)
     if iters > n1:
-        hist += S.train(net, Xtr, Ytr, iters=iters - n1, batch=4, lr=5e-4, seed=1)
+        hist += S.train(net, Xtr, Ytr, iters=iters - n1, batch=16, lr=3e-4, seed=1)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    D, H, Wd = Xtr.shape[1:4]
+    D, H, Wd = X.shape[1:4]
     fl_fwd = net.flops_per_voxel_fwd(D, H, Wd)
-    tflops = 3.0 * fl_fwd * 4 * iters / t_train / 1e12
-    err = S.rel_error(net, Xte, Yte)
-    # one jump against the plain solver's 10 steps, held-out windows
-    g = bt.grids[0]
-    lv = {"node_coords": g.axes(), "nodes": [g.n[0], g.n[1], g.n[2]], "h": list(g.h)}
-    Tl = te[0]["properties"]["T_liquidus"]
-    derr, werr = [], []
-    for w, (T0, T1, faces, row, i) in enumerate(keep):
-        Tn = S.apply_jump(net, Xte[w:w + 1], T0, faces, g)
-        dp = cf.meltPoolDimensions(lv, T1, Tl)
-        ds = cf.meltPoolDimensions(lv, Tn, Tl)
-        if dp[0] > 0 and dp[1] > 0:
-            derr.append(abs(ds[0] - dp[0]) / dp[0])
-            werr.append(abs(ds[1] - dp[1]) / dp[1])
-    x1 = Xte[:1].contiguous()
+    tflops = 3.0 * fl_fwd * 16 * iters / t_train / 1e12
+    err = S.rel_error(net, Xte, Yte, batch=16) if Xte.shape[0] else None
+    x1 = Xte[:1].contiguous() if Xte.shape[0] else Xtr[:1].contiguous()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     net.forward(x1)
     e0.record()
@@ -107,30 +151,26 @@ def surrogate_measure(iters):
     e1.record()
     torch.cuda.synchronize()
     jump_ms = e0.elapsed_time(e1) / 10
-    import batch as B
-    one = B.Batch.from_instances(te[:1])
-    plain_ms = 10 * one.time(batched=True, iters=50)
-    loop = S.run_in_loop(net, te, n_jumps=6, K=10, warm=20, compare_every=2, residual_limit=0.01)
-    loop.pop("final_T")
-    free = S.run_in_loop(net, te, n_jumps=6, K=10, warm=20, compare_every=2,
-                         residual_limit=float("inf"))
-    free.pop("final_T")
-    free["residual_limit"] = None  # no trigger (strict JSON has no Infinity)
+    plain = c5_run(None, c5_steps)
+    sur = c5_run(net, c5_steps)
     return {
-        "workload": "3-D U-Net 32-64-128-128 on 64x64x32-node L3 windows, K=10 steps per jump; "
-                    f"{Xtr.shape[0]} training windows from 32 plain batched instances, "
-                    f"{iters} Adam steps of batch 4; {len(keep)} held-out windows",
+        "workload": ("3-D U-Net 32-64-128-128 on 64x64x32-node L3 windows, one jump per L2 "
+                     f"substep (K = N3 = 10 L3 steps); {X.shape[0]} training windows from "
+                     f"{len(set(seeds.tolist()))} seeded plain three-level runs "
+                     f"(held out: the last run, {Xte.shape[0]} windows), {iters} Adam steps "
+                     "of batch 16; in the loop: part-scale deck (L1 400x400x100 elements, "
+                     f"first 2 raster lines of layer 1), {c5_steps} L1 steps timed"),
+        "windows": int(X.shape[0]),
+        "windows_s": t_data,
         "train_s": t_train,
-        "train_windows_per_s": 4 * iters / t_train,
+        "train_windows_per_s": 16 * iters / t_train,
         "loss_first_last": [hist[0], hist[-1]],
         "heldout_rel_err_dT": err,
         "heldout_rel_err_zero_predictor": 1.0,
-        "melt_pool_rel_err_depth": float(np.mean(derr)) if derr else None,
-        "melt_pool_rel_err_width": float(np.mean(werr)) if werr else None,
         "jump_ms_one_window": jump_ms,
-        "plain_10_steps_ms_one_window": plain_ms,
-        "in_the_loop": loop,
-        "in_the_loop_no_fallback": free,
+        "c5_plain": plain,
+        "c5_operator": sur,
+        "c5_speedup_scan_mm_per_s": sur["mm_per_s"] / plain["mm_per_s"],
         "roofline": {"kernel": "k_conv_fwd / k_conv_wgrad (training step)", "bound": "mfma",
                      "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": tflops / 2500.0, "flops_per_window_fwd": fl_fwd},
@@ -238,7 +278,8 @@ def main():
         sys.exit(2)
     if a.surrogate_only:
         import torch  # noqa: F401  (its HIP runtime first, before libgomelt.so)
-        print(json.dumps(surrogate_measure(a.surrogate_iters)))
+        print(json.dumps(surrogate_measure(a.surrogate_windows, a.surrogate_iters,
+                                           a.c5_steps)))
         return
     import numpy as np
     import workloads as W
@@ -348,7 +389,8 @@ def main():
         # the plain path above runs on the system runtime
         import subprocess
         cmd = [sys.executable, os.path.abspath(__file__), "--surrogate-only",
-               "--surrogate-iters", str(a.surrogate_iters)]
+               "--surrogate-iters", str(a.surrogate_iters),
+               "--surrogate-windows", str(a.surrogate_windows), "--c5-steps", str(a.c5_steps)]
         try:
             out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
             lines = [ln for ln in out.stdout.splitlines() if ln.startswith("{")]

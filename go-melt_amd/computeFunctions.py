@@ -563,6 +563,15 @@ class Engine:
         self.stream.sync()
 
 
+def attach_l3_operator(Levels, operator):
+    """Let ``operator`` (surrogate.EngineOperator, or None to detach) advance
+    the L3 window in subcycleGOMELT: the subcycle then runs stage by stage and
+    the operator takes the N3 L3 solves of each L2 substep (an addition with no
+    reference counterpart; the reference always runs them, cF:3367-3412,
+    3530-3590)."""
+    _eng(Levels).l3_operator = operator
+
+
 def _eng(Levels):
     for lv in Levels:
         if isinstance(lv, dict) and "_eng" in lv:
@@ -823,7 +832,11 @@ def subcycleGOMELT(Levels, ne_nn, Shapes, substrate, LInterp, tmp_ne_nn, laser_p
                                  f"got {src.size}")
             N.check(N.lib().gm_memcpy_h2d(dst.ptr, np.ascontiguousarray(src).ctypes.data,
                                           dst.nbytes, e.stream.ptr))
-    N.check(N.lib().gm_engine_subcycle(e.h, N.fptr(rows), rows.shape[0]))
+    op = getattr(e, "l3_operator", None)
+    if op is None:
+        N.check(N.lib().gm_engine_subcycle(e.h, N.fptr(rows), rows.shape[0]))
+    else:
+        op.subcycle(e, rows)
     L2all = [e.field(k, N.GM_HIST_L2) for k in range(e.N2)]
     L3all = [e.field(k, N.GM_HIST_L3) for k in range(e.N2)]
     L3pall = [e.field(k, N.GM_HIST_TP3) for k in range(e.N2)]

@@ -562,6 +562,21 @@ extern "C" int gm_new_tprime(const gm_grid* fine, const float* FineT, const gm_g
 // The engine
 // ---------------------------------------------------------------------------
 
+// Loop state of a subcycled L1 step run stage by stage (see stage_prologue).
+struct SubcycleState {
+    const float* T2cur;
+    const uint8_t* S12cur;
+    const float* T3cur;
+    const float* Tp3cur;
+    const uint8_t* S13cur;
+    const uint8_t* S23cur;
+    int b2, b3;
+    // outputs of the current substep's L2 stage
+    float* T2new;
+    uint8_t* S12new;
+    float* T3out;  // where the L3 stage leaves the substep's final L3 field
+};
+
 struct gm_engine {
     gm_config cfg;
     gm_material mat;
@@ -624,6 +639,9 @@ struct gm_engine {
     int ring_k = 0;
     // window state
     int s3[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+    // staged subcycle (gm_engine_stage_*): state, and the next stage expected
+    SubcycleState stg{};
+    int stg_active = 0, stg_part = 0, stg_k = 0, stg_phase = 0;
     // graph
     bool use_graph = true;
     bool graph_ok = false;
@@ -1446,13 +1464,32 @@ static void l2_substep(gm_engine* e, const float* T2cur, const uint8_t* S12cur, 
     run_l2_kc(e, T2cur, S12cur, S12new, e->V2, alpha, T2new, k);
 }
 
-static void subcycle_body(gm_engine* e) {
+// The subcycled L1 step runs as a fixed sequence of stages: the prologue
+// (sources, L1 predictor), per L2 substep k of Part 1 the L2 stage
+// (restriction, L2 solve, L3 face values) and the L3 stage (N3 L3 solves,
+// then T' and injection), the middle (L1 corrector), Part 2 likewise, then the
+// epilogue.  subcycle_body runs them all (the recorded graph); the staged
+// entry points let a caller replace the N3 L3 solves of a substep by its own
+// update of the L3 window (the learned operator).
+
+static void stage_reset(gm_engine* e, SubcycleState& st) {
+    st.T2cur = e->T2;
+    st.S12cur = e->S12;
+    st.T3cur = e->T3;
+    st.Tp3cur = e->Tp3;
+    st.S13cur = e->S13;
+    st.S23cur = e->S23;
+    st.b2 = st.b3 = 0;
+    st.T2new = nullptr;
+    st.S12new = nullptr;
+    st.T3out = nullptr;
+}
+
+static void stage_prologue(gm_engine* e, SubcycleState& st) {
     const int N2 = e->N2, N3 = e->N3, N23 = e->N23;
-    const int64_t n1 = e->nn[1], n2 = e->nn[2], n3 = e->nn[3];
+    const int64_t n2 = e->nn[2], n3 = e->nn[3];
     GmStepDesc* d = e->d_desc;
     hipStream_t s = e->s;
-    const float fN2 = (float)N2, fN3 = (float)N3;
-
     // L1 S1 from the L2 overlap, then substrate (cF:3273-3278)
     l1_s1_update(e);
     // k, cp of the start-of-step L3 state for the carried-property L3 steps
@@ -1481,81 +1518,68 @@ static void subcycle_body(gm_engine* e) {
     // L1 predictor; its L2 face values for the Part-1 L2 substeps
     run_l1(e, e->V1, e->T1p, GM_DT_L1, 0.0f, 1);
     launch_face_bc(d, 2, 1, e->cfg.level[2], e->T1p, e->T1, e->fa2, e->fb2, s);
+    stage_reset(e, st);
+}
 
-    // ---- Part 1: L2 x N2 { L3 x N3 } -----------------------------------------
-    const float* T2cur = e->T2;
-    const uint8_t* S12cur = e->S12;
-    const float* T3cur = e->T3;
-    const float* Tp3cur = e->Tp3;
-    const uint8_t* S13cur = e->S13;
-    int b2 = 0, b3 = 0;
-    for (int k = 0; k < N2; ++k) {
-        const float* Gk = e->G2 + (size_t)k * N3 * 3 * e->gmax[2];
-        eng_restrict(e, 2,
-                        rargs(e, 3, T3cur, S13cur, Tp3cur, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f));
-        float* T2new = e->T2b[b2];
-        uint8_t* S12new = e->S12b[b2];
-        b2 ^= 1;
-        l2_substep(e, T2cur, S12cur, S12new, Gk, (float)(k + 1) / fN2, T2new, k);
-        launch_face_bc(d, 3, 2, e->cfg.level[3], T2new, T2cur, e->fa3, e->fb3, s);
-        const float* T3in = T3cur;
-        const uint8_t* S13in = S13cur;
-        for (int j = 0; j < N3; ++j) {
-            float* T3o = e->T3b[b3];
+// Part 1 (part = 1, cF:3333-3420) or Part 2 (part = 2, cF:3466-3608) L2 stage
+// of substep k: restriction of the L3 T' terms, L2 solve, L3 face values.
+static void stage_l2(gm_engine* e, SubcycleState& st, int part, int k) {
+    const float fN2 = (float)e->N2;
+    const float* Gk = e->G2 + (size_t)k * e->N3 * 3 * e->gmax[2];
+    if (part == 1) {
+        eng_restrict(e, 2, rargs(e, 3, st.T3cur, st.S13cur, st.Tp3cur, nullptr, nullptr,
+                                 GM_DT_VALUE, 0, 1.0f));
+        st.T2new = e->T2b[st.b2];
+    } else {
+        // flux + mass restriction onto L2 (cF:3496-3508)
+        eng_restrict(e, 2, rargs(e, 3, st.T3cur, st.S13cur, st.Tp3cur, e->histTp[k],
+                                 st.Tp3cur, GM_DT_L2, k, 0.0f));
+        st.T2new = e->L2all[k];
+    }
+    st.S12new = e->S12b[st.b2];
+    st.b2 ^= 1;
+    l2_substep(e, st.T2cur, st.S12cur, st.S12new, Gk, (float)(k + 1) / fN2, st.T2new, k);
+    launch_face_bc(e->d_desc, 3, 2, e->cfg.level[3], st.T2new, st.T2cur, e->fa3, e->fb3, e->s);
+    // the buffer that will hold the substep's final L3 field
+    if (part == 1) st.T3out = e->T3b[(st.b3 + e->N3 - 1) & 1];
+    else st.T3out = e->L3all[k];
+}
+
+// L3 stage, closing: T' and injection into the new L2 field, then the
+// substep's state becomes the next one's input.
+static void stage_l3_close(gm_engine* e, SubcycleState& st, int part, int k,
+                           const uint8_t* S13fin, const uint8_t* S23fin) {
+    new_tprime(e, 3, st.T3out, st.T2new, e->histTp[k], st.S12new);
+    st.T2cur = st.T2new;
+    st.S12cur = st.S12new;
+    st.T3cur = st.T3out;
+    st.S13cur = S13fin;
+    if (part == 2) st.S23cur = S23fin;
+    st.Tp3cur = e->histTp[k];
+}
+
+// L3 stage of substep k: the N3 L3 solves (computeSolutions_L3 cF:3015-3054
+// with the time-blended faces), then T'.
+static void stage_l3_plain(gm_engine* e, SubcycleState& st, int part, int k) {
+    const int N3 = e->N3;
+    const float fN3 = (float)N3;
+    const float* T3in = st.T3cur;
+    const uint8_t* S13in = st.S13cur;
+    const uint8_t* S23in = st.S23cur;
+    for (int j = 0; j < N3; ++j) {
+        if (part == 1) {
+            float* T3o = e->T3b[st.b3];
             // L3 S1 of Part 1 in a work buffer: the first step writes all of
             // it, later steps update it in place (written where it flips)
             uint8_t* S13o = e->S13b[0];
-            b3 ^= 1;
+            st.b3 ^= 1;
             run_l3_kc(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, nullptr,
                       nullptr);
             T3in = T3o;
             S13in = S13o;
-        }
-        new_tprime(e, 3, T3in, T2new, e->histTp[k], S12new);
-        T2cur = T2new;
-        S12cur = S12new;
-        T3cur = T3in;
-        S13cur = S13in;
-        Tp3cur = e->histTp[k];
-    }
-    // L2 T' and L1 injection, then the L1 mass terms and corrector (cF:3433-3456)
-    new_tprime(e, 2, T2cur, e->T1p, e->Tp2x);
-    eng_restrict(e, 1,
-                    rargs(e, 3, e->T3, e->S13, nullptr, Tp3cur, e->Tp3, GM_DT_L1, 0, 0.0f));
-    eng_restrict(e, 0,
-                    rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_L1, 0, 0.0f));
-    eng_restrict_nodes(e, 0, 1, e->V1, e->V1m,
-                          -1, nullptr, 0, 0, 0.0f);
-    run_l1(e, e->V1m, e->T1c, GM_DT_L1, 0.0f, 1);
-    launch_face_bc(d, 2, 1, e->cfg.level[2], e->T1c, e->T1, e->fa2, e->fb2, s);
-
-    // ---- Part 2: restart L2/L3 from the start-of-step state ------------------
-    // melt time of the L3 nodes: gathered from Level0 unless the caller
-    // passed the window arrays (the driver's max_accum_time[idx], cF:3224-3238)
-    if (!e->acc_external) launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, n3, 1, s);
-    T2cur = e->T2;
-    S12cur = e->S12;
-    T3cur = e->T3;
-    Tp3cur = e->Tp3;
-    S13cur = e->S13;
-    const uint8_t* S23cur = e->S23;
-    int b2s = 0, b3s = 0;
-    for (int k = 0; k < N2; ++k) {
-        const float* Gk = e->G2 + (size_t)k * N3 * 3 * e->gmax[2];
-        // flux + mass restriction onto L2 (cF:3496-3508)
-        eng_restrict(e, 2,
-                        rargs(e, 3, T3cur, S13cur, Tp3cur, e->histTp[k], Tp3cur, GM_DT_L2, k, 0.0f));
-        float* T2new = e->L2all[k];
-        uint8_t* S12new = e->S12b[b2s];
-        b2s ^= 1;
-        l2_substep(e, T2cur, S12cur, S12new, Gk, (float)(k + 1) / fN2, T2new, k);
-        launch_face_bc(d, 3, 2, e->cfg.level[3], T2new, T2cur, e->fa3, e->fb3, s);
-        const float* T3in = T3cur;
-        const uint8_t* S13in = S13cur;
-        const uint8_t* S23in = S23cur;
-        for (int j = 0; j < N3; ++j) {
-            float* T3o = (j == N3 - 1) ? e->L3all[k] : e->T3b[b3s];
-            if (j != N3 - 1) b3s ^= 1;
+        } else {
+            float* T3o = (j == N3 - 1) ? e->L3all[k] : e->T3b[st.b3];
+            if (j != N3 - 1) st.b3 ^= 1;
             uint8_t* S13o = e->S13;  // S1 and the melt state of Part 2 updated in place
             uint8_t* S23o = e->S23;
             run_l3_kc(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, S23in, S23o);
@@ -1563,23 +1587,95 @@ static void subcycle_body(gm_engine* e) {
             S13in = S13o;
             S23in = S23o;
         }
-        new_tprime(e, 3, T3in, T2new, e->histTp[k], S12new);
-        T2cur = T2new;
-        S12cur = S12new;
-        T3cur = T3in;
-        S13cur = S13in;
-        S23cur = S23in;
-        Tp3cur = e->histTp[k];
     }
+    stage_l3_close(e, st, part, k, S13in, S23in);
+}
+
+// Phase flags and melt time of one jump of the L3 window over the N3 steps of
+// L2 substep k (a caller's update T0 -> T1 in place of the N3 solves):
+//   S1 = S1in | (T0 >= T_liq) | (T1 >= T_liq), substrate nodes solid;
+//   with S2 (Part 2): a node is molten over the jump if T0 or T1 >= T_liq;
+//   one that melts anew hands its accumulated time to max_accum and
+//   restarts, as the plain step's per-step rule does, then adds the
+//   substep's time dt2[k].
+__global__ void k_jump_state(const GmStepDesc* __restrict__ d, const DevMat* __restrict__ m,
+                             int k, const float* __restrict__ T0, const float* __restrict__ T1,
+                             const uint8_t* __restrict__ S1in, uint8_t* __restrict__ S1out,
+                             int64_t nsub, uint8_t* __restrict__ S2, float* __restrict__ acc,
+                             float* __restrict__ macc, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float Tl = m->T_liq;
+    const bool hot = T0[t] >= Tl || T1[t] >= Tl;
+    S1out[t] = (uint8_t)((S1in[t] != 0 || hot || t < nsub) ? 1 : 0);
+    if (S2) {
+        const bool s2p = S2[t] != 0;
+        if (hot) {
+            const float a = acc[t], mx = macc[t];
+            const float rr = s2p ? 0.0f : a;
+            macc[t] = rr > mx ? rr : mx;
+            acc[t] = a + d->dt2[k] - rr;
+        }
+        S2[t] = (uint8_t)hot;
+    }
+}
+
+static void launch_jump_state(const GmStepDesc* d, const DevMat* m, int k, const float* T0,
+                              const float* T1, const uint8_t* S1in, uint8_t* S1out, int64_t nsub,
+                              uint8_t* S2, float* acc, float* macc, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_jump_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, m, k,
+                       T0, T1, S1in, S1out, nsub, S2, acc, macc, n);
+}
+
+// L3 stage of substep k when the caller has written the substep's final L3
+// field into st.T3out (one jump over the N3 steps): phase flags, melt time
+// and the carried k, cp of that field, then T'.
+static void stage_l3_external(gm_engine* e, SubcycleState& st, int part, int k) {
+    const int64_t n3 = e->nn[3];
+    uint8_t* S13o = part == 1 ? e->S13b[0] : e->S13;
+    launch_jump_state(e->d_desc, e->d_dm, k, st.T3cur, st.T3out, st.S13cur, S13o,
+                      e->host.sub[3], part == 2 ? e->S23 : nullptr,
+                      part == 2 ? e->acc3 : nullptr, part == 2 ? e->macc3 : nullptr, n3, e->s);
+    launch_props_kc(e->d_dm, st.T3out, S13o, n3, e->host.sub[3], kc_of(e, st.T3out), e->s,
+                    nullptr, nullptr, 0, 0, nullptr);
+    if (part == 1) st.b3 ^= (e->N3 & 1);  // the buffer parity the N3 solves leave
+    else st.b3 ^= ((e->N3 - 1) & 1);
+    stage_l3_close(e, st, part, k, S13o, part == 2 ? e->S23 : nullptr);
+}
+
+static void stage_mid(gm_engine* e, SubcycleState& st) {
+    GmStepDesc* d = e->d_desc;
+    hipStream_t s = e->s;
+    // L2 T' and L1 injection, then the L1 mass terms and corrector (cF:3433-3456)
+    new_tprime(e, 2, st.T2cur, e->T1p, e->Tp2x);
+    eng_restrict(e, 1,
+                    rargs(e, 3, e->T3, e->S13, nullptr, st.Tp3cur, e->Tp3, GM_DT_L1, 0, 0.0f));
+    eng_restrict(e, 0,
+                    rargs(e, 2, e->T2, e->S12, nullptr, e->Tp2x, e->Tp2, GM_DT_L1, 0, 0.0f));
+    eng_restrict_nodes(e, 0, 1, e->V1, e->V1m,
+                          -1, nullptr, 0, 0, 0.0f);
+    run_l1(e, e->V1m, e->T1c, GM_DT_L1, 0.0f, 1);
+    launch_face_bc(d, 2, 1, e->cfg.level[2], e->T1c, e->T1, e->fa2, e->fb2, s);
+    // ---- Part 2: restart L2/L3 from the start-of-step state ------------------
+    // melt time of the L3 nodes: gathered from Level0 unless the caller
+    // passed the window arrays (the driver's max_accum_time[idx], cF:3224-3238)
+    if (!e->acc_external) launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, e->nn[3], 1, s);
+    stage_reset(e, st);
+}
+
+static void stage_epilogue(gm_engine* e, SubcycleState& st) {
+    const int64_t n1 = e->nn[1], n2 = e->nn[2], n3 = e->nn[3];
+    GmStepDesc* d = e->d_desc;
+    hipStream_t s = e->s;
     if (!e->acc_external) launch_acc_l3(d, e->acc, e->macc, e->acc3, e->macc3, n3, 0, s);
     // final L2 T' and L1 T0 (cF:3625-3627), then commit the end-of-step state
-    new_tprime(e, 2, T2cur, e->T1c, e->Tp2);
+    new_tprime(e, 2, st.T2cur, e->T1c, e->Tp2);
     {
         CopyList L;
-        L.c[0] = CopyItem{e->T2, T2cur, n2 * 4};
-        L.c[1] = CopyItem{e->S12, S12cur, n2};
-        L.c[2] = CopyItem{e->T3, T3cur, n3 * 4};
-        L.c[3] = CopyItem{e->Tp3, Tp3cur, n3 * 4};
+        L.c[0] = CopyItem{e->T2, st.T2cur, n2 * 4};
+        L.c[1] = CopyItem{e->S12, st.S12cur, n2};
+        L.c[2] = CopyItem{e->T3, st.T3cur, n3 * 4};
+        L.c[3] = CopyItem{e->Tp3, st.Tp3cur, n3 * 4};
         L.c[4] = CopyItem{e->T1, e->T1c, n1 * 4};
         launch_copy_multi(L, 5, s);
     }
@@ -1588,7 +1684,23 @@ static void subcycle_body(gm_engine* e) {
     e->use_kc = false;
 }
 
-extern "C" int gm_engine_subcycle(gm_engine* e, const float* rows, int n_rows) {
+static void subcycle_body(gm_engine* e) {
+    SubcycleState st;
+    stage_prologue(e, st);
+    // ---- Part 1: L2 x N2 { L3 x N3 } -----------------------------------------
+    for (int k = 0; k < e->N2; ++k) {
+        stage_l2(e, st, 1, k);
+        stage_l3_plain(e, st, 1, k);
+    }
+    stage_mid(e, st);
+    for (int k = 0; k < e->N2; ++k) {
+        stage_l2(e, st, 2, k);
+        stage_l3_plain(e, st, 2, k);
+    }
+    stage_epilogue(e, st);
+}
+
+static int subcycle_rows(gm_engine* e, const float* rows, int n_rows) {
     if (n_rows != e->N23) return fail("subcycle needs exactly N2*N3 toolpath rows");
     GmStepDesc& h = e->host;
     h.nrows = n_rows;
@@ -1603,7 +1715,12 @@ extern "C" int gm_engine_subcycle(gm_engine* e, const float* rows, int n_rows) {
         for (int j = 0; j < e->N3; ++j) s2 += rows[7 * (k * e->N3 + j) + 5];
         h.dt2[k] = s2;
     }
-    if (upload_desc(e, desc_bytes(n_rows))) return -1;
+    return upload_desc(e, desc_bytes(n_rows));
+}
+
+extern "C" int gm_engine_subcycle(gm_engine* e, const float* rows, int n_rows) {
+    if (e->stg_active) return fail("a staged subcycle is in progress");
+    if (subcycle_rows(e, rows, n_rows)) return -1;
     if (!e->use_graph) {
         subcycle_body(e);
         HIPCHK(hipGetLastError());
@@ -1623,6 +1740,70 @@ extern "C" int gm_engine_subcycle(gm_engine* e, const float* rows, int n_rows) {
         e->graph_ok = true;
     }
     HIPCHK(hipGraphLaunch(e->gexec, e->s));
+    return 0;
+}
+
+// ---- the subcycled L1 step stage by stage ------------------------------------
+// begin (rows, prologue) -> 2*N2 x { l2, l3 } with the middle and the
+// epilogue run by the l3 stage that closes Part 1 and Part 2.  Kernels are
+// launched directly (no graph), in the order gm_engine_subcycle records.
+
+extern "C" int gm_engine_stage_begin(gm_engine* e, const float* rows, int n_rows) {
+    if (e->stg_active) return fail("a staged subcycle is already in progress");
+    if (subcycle_rows(e, rows, n_rows)) return -1;
+    stage_prologue(e, e->stg);
+    e->stg_active = 1;
+    e->stg_part = 1;
+    e->stg_k = 0;
+    e->stg_phase = 0;
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_engine_stage_l2(gm_engine* e, int32_t* part, int32_t* k) {
+    if (!e->stg_active || e->stg_phase != 0) return fail("gm_engine_stage_l2 out of order");
+    stage_l2(e, e->stg, e->stg_part, e->stg_k);
+    e->stg_phase = 1;
+    if (part) *part = e->stg_part;
+    if (k) *k = e->stg_k;
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_engine_stage_l3(gm_engine* e, int external) {
+    if (!e->stg_active || e->stg_phase != 1) return fail("gm_engine_stage_l3 out of order");
+    if (external) stage_l3_external(e, e->stg, e->stg_part, e->stg_k);
+    else stage_l3_plain(e, e->stg, e->stg_part, e->stg_k);
+    e->stg_phase = 0;
+    if (++e->stg_k == e->N2) {
+        e->stg_k = 0;
+        if (e->stg_part == 1) {
+            stage_mid(e, e->stg);
+            e->stg_part = 2;
+        } else {
+            stage_epilogue(e, e->stg);
+            e->stg_active = 0;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_engine_stage_field(gm_engine* e, int which, void** ptr, int64_t* count) {
+    if (!e->stg_active || e->stg_phase != 1)
+        return fail("gm_engine_stage_field: only between the l2 and l3 stages");
+    const SubcycleState& st = e->stg;
+    void* p = nullptr;
+    switch (which) {
+        case GM_STAGE_L3_IN: p = (void*)st.T3cur; break;
+        case GM_STAGE_L3_OUT: p = (void*)st.T3out; break;
+        case GM_STAGE_FACE_NEW: p = (void*)e->fa3; break;
+        case GM_STAGE_FACE_OLD: p = (void*)e->fb3; break;
+        case GM_STAGE_S1_IN: p = (void*)st.S13cur; break;
+        default: return fail("gm_engine_stage_field: no such field");
+    }
+    *ptr = p;
+    *count = e->nn[3];
     return 0;
 }
 
@@ -2339,6 +2520,13 @@ extern "C" int gm_nn_window_apply(const int32_t* n_host, const float* T, const f
     if (!n_host || !T || !Y || !Tn || C < 1 || n_host[0] < 1 || n_host[1] < 1 || n_host[2] < 1)
         return fail("gm_nn_window_apply: bad arguments");
     launch_nn_window_apply(n_host, T, Tface, Y, C, T_scale, T_floor, Tn, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_u8_to_f32(int64_t n, const uint8_t* x, float* y, void* stream) {
+    if (n < 0 || !x || !y) return fail("gm_u8_to_f32: bad arguments");
+    if (n) launch_nn_u8_to_f32(n, x, y, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return 0;
 }

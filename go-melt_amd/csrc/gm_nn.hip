@@ -398,6 +398,16 @@ void launch_nn_window_input(const int32_t* n, const float* geo, const float* T, 
                        prm[4], prm[5], prm[6], prm[7], prm[8], X);
 }
 
+// y = (float)x for 0/1 phase bytes
+__global__ void k_u8_to_f32(int64_t n, const uint8_t* __restrict__ x, float* __restrict__ y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = (float)x[t];
+}
+
+void launch_nn_u8_to_f32(int64_t n, const uint8_t* x, float* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_u8_to_f32, dim3(nn_blocks(n)), dim3(256), 0, s, n, x, y);
+}
+
 void launch_nn_axpy(int64_t n, float a, const float* x, float* y, hipStream_t s) {
     hipLaunchKernelGGL(k_axpy, dim3(nn_blocks(n)), dim3(256), 0, s, n, a, x, y);
 }

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 import _native as N
@@ -660,3 +661,228 @@ def run_in_loop(net, instances, n_jumps, K=10, warm=20, compare_every=2, residua
         "final_T": ([_copy_field(sur._field(i, 0)) for i in range(n)],
                     [_copy_field(plain._field(i, 0)) for i in range(n)]),
     }
+
+
+# ---------------------------------------------------------------------------
+# Operator in the three-level engine's moving L3 window
+# ---------------------------------------------------------------------------
+
+class _Raw:
+    """A device pointer with a tensor's data_ptr(), for the wrappers above."""
+
+    def __init__(self, ptr):
+        self.p = int(ptr)
+
+    def data_ptr(self):
+        return self.p
+
+
+def _stage_field(e, which):
+    p = C.c_void_p()
+    n = C.c_int64()
+    N.check(N.lib().gm_engine_stage_field(e.h, int(which), C.byref(p), C.byref(n)))
+    return _Raw(p.value), int(n.value)
+
+
+class EngineOperator:
+    """The learned operator inside the engine's moving L3 window.
+
+    Attached to a facade engine (``computeFunctions.attach_l3_operator``),
+    it runs subcycleGOMELT (cF:3224-3632) stage by stage and replaces the N3
+    L3 solves of every L2 substep (cF:3367-3412 in Part 1, cF:3530-3590 in
+    Part 2) by one jump: the network sees the window's field at the start of
+    the substep, the substep's new L3 face values (the L2-interpolated
+    Dirichlet values the last of the N3 solves uses), the laser position and
+    power of the substep's first row, and returns the field at its end.
+
+    Each jump is audited with gm_energy_balance over the substep (properties
+    and surface flux of the start field, the source averaged over the N3
+    rows, faces and the T_amb floor held); when the relative residual exceeds
+    ``residual_limit`` the jump is dropped and the N3 plain solves run
+    (the fallback).  Every ``judge_every`` jumps the plain solves run from the
+    same start as well, and the jump is judged against them (peak T,
+    melt-pool depth and width of the L3 window); the run continues on the
+    plain field there.  ``record`` (a callable) receives (input field, face
+    values, rows of the substep, plain output field) at every substep whose
+    plain solves ran: the windows the operator is trained on.
+    ``net`` None runs the plain solves everywhere (with ``record``, the
+    training-window generator)."""
+
+    def __init__(self, net, properties, residual_limit=0.01, judge_every=200, record=None):
+        self.net = net
+        self.p = properties
+        self.limit = float(residual_limit)
+        self.judge_every = int(judge_every) if judge_every else 0
+        self.record = record
+        self.X = None
+        self.stats = {"jumps": 0, "fallbacks": 0, "residuals": [], "judged": []}
+        self._tmp = {}
+
+    def _buf(self, name, n, dtype=torch.float32):
+        t = self._tmp.get(name)
+        if t is None or t.numel() != n or t.dtype != dtype:
+            t = torch.empty(n, dtype=dtype, device="cuda")
+            self._tmp[name] = t
+        return t
+
+    def _audit(self, e, g, T0, Tn, S1u8, rows):
+        import computeFunctions as cf
+        lib = N.lib()
+        nn = g.n[0] * g.n[1] * g.n[2]
+        st = _stream()
+        mat = cf._material(self.p)
+        S1 = self._buf("S1", nn)
+        N.check(lib.gm_u8_to_f32(nn, _ptr(S1u8), _ptr(S1), st))
+        S1o, k, rc = self._buf("S1o", nn), self._buf("k", nn), self._buf("rc", nn)
+        S2o = self._buf("S2o", nn, torch.uint8)
+        N.check(lib.gm_state_properties(C.byref(mat), _ptr(T0), _ptr(S1), nn, 0, _ptr(S1o),
+                                        _ptr(S2o), _ptr(k), _ptr(rc), st))
+        F, Fk = self._buf("F", nn), self._buf("Fk", nn)
+        zero(F)
+        for r in rows:
+            v = (C.c_float * 3)(float(r[0]), float(r[1]), float(r[2]))
+            N.check(lib.gm_source_l3(C.byref(g), C.byref(mat), v, float(r[6]), _ptr(Fk), st))
+            N.check(lib.gm_nn_axpy(nn, 1.0 / len(rows), _ptr(Fk), _ptr(F), st))
+        ne = (g.n[0] - 1) * (g.n[1] - 1) * (g.n[2] - 1)
+        N.check(lib.gm_conv_rad_bc(C.byref(g), C.byref(mat), _ptr(T0), ne, _ptr(F), st))
+        sums = self._buf("sums", 8, torch.float64)
+        zero(sums)
+        dt = float(np.float32(rows[:, 5]).sum())
+        N.check(lib.gm_energy_balance(C.byref(g), ne, _ptr(k), _ptr(rc), dt, _ptr(T0), _ptr(F),
+                                      None, _ptr(Tn), 31, float(self.p["T_amb"]), _ptr(sums), st))
+        return float(sums[6].item())
+
+    def _pool(self, g, T):
+        import computeFunctions as cf
+        lv = {"node_coords": g.axes(), "nodes": [g.n[0], g.n[1], g.n[2]], "h": list(g.h)}
+        return cf.meltPoolDimensions(lv, T, self.p["T_liquidus"])
+
+    def subcycle(self, e, rows):
+        """One L1 step (N2*N3 rows) with the operator in the L3 window."""
+        lib = N.lib()
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        with torch.cuda.stream(torch.cuda.ExternalStream(e.stream.ptr)):
+            N.check(lib.gm_engine_stage_begin(e.h, N.fptr(rows), rows.shape[0]))
+            part, k = C.c_int32(), C.c_int32()
+            for _ in range(2 * e.N2):
+                N.check(lib.gm_engine_stage_l2(e.h, C.byref(part), C.byref(k)))
+                self._substep(e, rows[k.value * e.N3:(k.value + 1) * e.N3])
+
+    def _substep(self, e, rows):
+        lib = N.lib()
+        T0, n3 = _stage_field(e, N.GM_STAGE_L3_IN)
+        out, _ = _stage_field(e, N.GM_STAGE_L3_OUT)
+        fa, _ = _stage_field(e, N.GM_STAGE_FACE_NEW)
+        s1, _ = _stage_field(e, N.GM_STAGE_S1_IN)
+        g = e.grid(3)
+        if self.net is None:
+            if self.record is not None:
+                T0c = self._copy(T0, n3)
+                fac = self._copy(fa, n3)
+                N.check(lib.gm_engine_stage_l3(e.h, 0))
+                self.record(T0c, fac, rows.copy(), self._copy(out, n3))
+            else:
+                N.check(lib.gm_engine_stage_l3(e.h, 0))
+            return
+        shape = (1, g.n[2], g.n[1], g.n[0], 16)
+        if self.X is None or tuple(self.X.shape) != shape:
+            self.X = torch.empty(shape, dtype=torch.bfloat16, device="cuda")
+        window_input({"properties": self.p}, g, T0, fa, rows[0], self.X[0])
+        Yh = self.net.forward(self.X)
+        nn3 = (C.c_int32 * 3)(g.n[0], g.n[1], g.n[2])
+        N.check(lib.gm_nn_window_apply(nn3, _ptr(T0), _ptr(fa), _ptr(Yh), Yh.shape[-1], T_SCALE,
+                                       float(self.p["T_amb"]), _ptr(out), _stream()))
+        res = self._audit(e, g, T0, out, s1, rows)
+        self.stats["jumps"] += 1
+        self.stats["residuals"].append(res)
+        if self.judge_every and self.stats["jumps"] % self.judge_every == 0:
+            Tj = self._copy(out, n3)
+            T0c = self._copy(T0, n3)  # the N3 solves reuse the start field's buffer
+            N.check(lib.gm_engine_stage_l3(e.h, 0))
+            Tp = self._copy(out, n3)
+            # the audit of the plain solves over the same substep, for scale
+            res_plain = self._audit(e, g, T0c, Tp, s1, rows)
+            e.sync()
+            pj, pp = self._pool(g, Tj), self._pool(g, Tp)
+            # peak temperatures on the host (a judged jump is rare)
+            peak_j = float(np.max(N.DeviceArray(n3, ptr=Tj.data_ptr(), owner=False).numpy()))
+            peak_p = float(np.max(N.DeviceArray(n3, ptr=Tp.data_ptr(), owner=False).numpy()))
+            rel = lambda a, b: abs(a - b) / b if b > 0 else (0.0 if a == 0 else float("inf"))
+            self.stats["judged"].append({
+                "jump": self.stats["jumps"], "residual": res, "residual_plain": res_plain,
+                "peak_T_rel_err": rel(peak_j, peak_p),
+                "depth_rel_err": rel(pj[0], pp[0]), "width_rel_err": rel(pj[1], pp[1]),
+                "depth_mm": [pj[0], pp[0]], "width_mm": [pj[1], pp[1]]})
+            return
+        if not (abs(res) <= self.limit):
+            self.stats["fallbacks"] += 1
+            N.check(lib.gm_engine_stage_l3(e.h, 0))
+            return
+        N.check(lib.gm_engine_stage_l3(e.h, 1))
+
+    def _copy(self, raw, n):
+        t = torch.empty(n, dtype=torch.float32, device="cuda")
+        N.check(N.lib().gm_memcpy_d2d(_ptr(t), C.c_void_p(raw.p), 4 * n, _stream()))
+        return t
+
+
+def engine_windows(n_windows, seed0=0, n_l1=(80, 80, 40), length=4.0, per_run=1000,
+                   max_runs=400, log=None):
+    """Training pairs from the plain three-level engine: runs of
+    workloads.window_deck(seed) for seed = seed0, seed0 + 1, ... (P, speed and
+    path drawn from the seed), and at every L2 substep of every subcycled L1
+    step the window's field at the substep's start, its new face values and
+    first laser row (the input, as EngineOperator builds it) and the field
+    after the N3 plain L3 solves (target (T1 - T0) / T_SCALE), at most
+    `per_run` windows from each run.  Returns
+    X (bf16, (n, 32, 64, 64, 16)), Y (FP32, (n, 64*64*32)) and the seed of each
+    window."""
+    import json
+    import tempfile
+    import go_melt as G
+    import computeFunctions as cf
+    import workloads as W
+    e3 = W.OP_L3_ELEMENTS
+    D, H, Wd = e3[2] + 1, e3[1] + 1, e3[0] + 1
+    X = torch.empty((n_windows, D, H, Wd, 16), dtype=torch.bfloat16, device="cuda")
+    Y = torch.empty((n_windows, D * H * Wd), dtype=torch.float32, device="cuda")
+    seeds = []
+    state = {"w": 0}
+    seed = seed0
+    while state["w"] < n_windows and seed - seed0 < max_runs:
+        inp, g = W.window_deck(seed, n_l1=n_l1, length=length)
+        d = tempfile.mkdtemp(prefix="gomelt_win_")
+        with open(W.write_deck(inp, g, d)) as fh:
+            deck = json.load(fh)
+        run = G.TimeLoop(deck, quiet=True, write_outputs=False)
+        p = run.Properties
+
+        state["run_w"] = 0
+
+        def rec(T0, face, rows, T1, run=run, p=p, seed=seed):
+            w = state["w"]
+            if w >= n_windows or state["run_w"] >= per_run:
+                return
+            state["run_w"] += 1
+            window_input({"properties": p}, run.eng.grid(3), T0, face, rows[0], X[w])
+            zero(Y[w])
+            N.check(N.lib().gm_nn_axpy(Y[w].numel(), 1.0 / T_SCALE, _ptr(T1), _ptr(Y[w]),
+                                       _stream()))
+            N.check(N.lib().gm_nn_axpy(Y[w].numel(), -1.0 / T_SCALE, _ptr(T0), _ptr(Y[w]),
+                                       _stream()))
+            seeds.append(seed)
+            state["w"] = w + 1
+
+        cf.attach_l3_operator(run.Levels, EngineOperator(None, p, record=rec))
+        for rows in run.chunks():
+            if state["w"] >= n_windows or state["run_w"] >= per_run or \
+                    not run.step_chunk(rows):
+                break
+        run.eng.sync()
+        if log:
+            log(f"run {seed}: {state['w']} windows")
+        del run
+        seed += 1
+    w = state["w"]
+    torch.cuda.synchronize()
+    return X[:w], Y[:w], seeds

@@ -226,3 +226,78 @@ def batch_rows(instances, step):
     for i, b in enumerate(instances):
         rows[i] = (b["x0"] + b["speed"] * (step + 1) * b["dt"], b["y"], 0.0, 1, 1, b["dt"], b["P"])
     return rows
+
+
+# ---------------------------------------------------------------------------
+# Decks of the learned operator: training windows and the part-scale run
+# ---------------------------------------------------------------------------
+
+# L3 window of the operator: 64 x 64 x 32 nodes (63 x 63 x 31 elements) at
+# 12.5 um; L2 63 x 63 x 31 elements at 50 um; L1 at 100 um
+OP_L3_ELEMENTS = (63, 63, 31)
+OP_H3, OP_H2, OP_H1 = 0.0125, 0.05, 0.1
+
+
+def _op_windows(x, y):
+    """L2 and L3 windows of the operator decks with the laser at (x, y) on the
+    top surface: L3 starts 32 L3 spacings before the laser in x and y, L2 at
+    an L1 node 16 L1 spacings before it."""
+    e3 = OP_L3_ELEMENTS
+    x3, y3 = round(x - 32 * OP_H3, 6), round(y - 32 * OP_H3, 6)
+    # L3 origin on an L2 node, L2 origin on an L1 node
+    x2 = round(np.floor((x3 - 24 * OP_H2) / OP_H1 + 1e-9) * OP_H1, 6)
+    y2 = round(np.floor((y3 - 24 * OP_H2) / OP_H1 + 1e-9) * OP_H1, 6)
+    x3 = round(x2 + round((x3 - x2) / OP_H2) * OP_H2, 6)
+    y3 = round(y2 + round((y3 - y2) / OP_H2) * OP_H2, 6)
+    L2 = _level(e3, (x2, x2 + e3[0] * OP_H2), (y2, y2 + e3[1] * OP_H2), (-e3[2] * OP_H2, 0))
+    L3 = _level(e3, (x3, x3 + e3[0] * OP_H3), (y3, y3 + e3[1] * OP_H3), (-e3[2] * OP_H3, 0))
+    return L2, L3
+
+
+def window_deck(seed, n_l1=(80, 80, 40), length=4.0):
+    """A single-track (even seed) or two-line raster (odd seed) run whose L3
+    window is the operator's 64 x 64 x 32 nodes; P uniform 150-350 W and speed
+    uniform 0.5-1.5 m/s from the seed.  The runs the training windows come
+    from (the plain solver at every L2 substep)."""
+    rng = np.random.default_rng(1000 + seed)
+    P = float(rng.uniform(150.0, 350.0))
+    v = float(rng.uniform(500.0, 1500.0))
+    ext = [n_l1[0] * OP_H1, n_l1[1] * OP_H1, n_l1[2] * OP_H1]
+    x0, y0 = 0.3 * ext[0], 0.5 * ext[1] - 0.1
+    L2, L3 = _op_windows(x0, y0)
+    props = _props(laser_power=P, laser_radius=0.05, laser_depth=0.05, emissivity=0.4,
+                   laser_center=[x0, y0, 0.0, 0, 0, 0, 0])
+    inp = {
+        "Level1": _level(n_l1, (0, ext[0]), (0, ext[1]), (-ext[2], 0), 300.0),
+        "Level2": L2,
+        "Level3": L3,
+        "properties": props,
+        "nonmesh": _nonmesh(laser_velocity=v),
+    }
+    if seed % 2 == 0:
+        g = f"G0 X{x0:.4f} Y{y0:.4f} Z0\nG1 X{x0 + length:.4f} Y{y0:.4f} Z0\n"
+    else:
+        g = raster_gcode(x0, x0 + 0.5 * length, y0, 0.1, 2)
+    return inp, g
+
+
+def part_scale(lines=30, layers=5, line_mm=10.0, hatch=0.1, P=250.0):
+    """The surrogate-in-the-loop part: L1 400 x 400 x 100 elements over
+    40 x 40 x 10 mm, the operator's L2/L3 windows, `layers` layers of 50 um
+    (each activating its L1 elements as the laser reaches it), each a
+    serpentine raster of `lines` lines of `line_mm` mm at `hatch` spacing,
+    1 m/s, P = 250 W."""
+    x0, y0 = 15.0, 15.0
+    L2, L3 = _op_windows(x0, y0)
+    props = _props(laser_power=P, laser_radius=0.05, laser_depth=0.05, emissivity=0.4,
+                   laser_center=[x0, y0, 0.0, 0, 0, 0, 0], layer_height=0.05)
+    inp = {
+        "Level1": _level((400, 400, 100), (0, 40), (0, 40), (-10, 0), 300.0),
+        "Level2": L2,
+        "Level3": L3,
+        "properties": props,
+        "nonmesh": _nonmesh(),
+    }
+    g = "".join(raster_gcode(x0, x0 + line_mm, y0, hatch, lines, z=0.05 * k)
+                for k in range(layers))
+    return inp, g

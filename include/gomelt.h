@@ -249,6 +249,16 @@ enum {
     GM_RESET_L3 = 11  /* uint8 reset mask of the last gm_engine_step         */
 };
 
+/* Buffers of a staged subcycle between its l2 and l3 stages (all FP32 with the
+ * L3 node count, except GM_STAGE_S1_IN, uint8). */
+enum {
+    GM_STAGE_L3_IN = 0,    /* L3 field at the start of the L2 substep           */
+    GM_STAGE_L3_OUT = 1,   /* where the substep's final L3 field goes           */
+    GM_STAGE_FACE_NEW = 2, /* L3 face values from the substep's new L2 field    */
+    GM_STAGE_FACE_OLD = 3, /* ... from its start-of-substep L2 field            */
+    GM_STAGE_S1_IN = 4     /* L3 phase flags at the start of the substep        */
+};
+
 /* SetupLevels (cF:115-264) + getStaticNodesAndElements/getStaticSubcycle:
  * allocates all level state on the current device and initialises it to
  * T_amb, S1 = 0.  `stream` is kept and used by every engine call. */
@@ -279,6 +289,22 @@ int gm_engine_move(gm_engine* e, const float* v_host, const float* vstart_host,
  * accumulation on Level0 and the Level0 S1/S2 update.  The first call with a
  * given layer state records a hipGraph; later calls replay it. */
 int gm_engine_subcycle(gm_engine* e, const float* rows_host, int n_rows);
+
+/* subcycleGOMELT (cF:3224-3632) stage by stage, for a caller that advances the
+ * L3 window of some L2 substeps itself (the learned operator in place of the
+ * N3 L3 solves of cF:3367-3412 and cF:3530-3590).  stage_begin takes the rows
+ * and runs the prologue; then, 2*N2 times (Part 1, then Part 2), stage_l2 runs
+ * the L2 substep (restriction, L2 solve, L3 face values; part, k returned) and
+ * stage_l3 finishes it: external = 0 runs the N3 L3 solves, external = 1
+ * takes the final L3 field the caller wrote into GM_STAGE_L3_OUT (phase
+ * flags, melt time over the substep, carried k and cp), then T'.  The l3
+ * stage closing Part 1 runs the L1 corrector, the one closing Part 2 the
+ * epilogue.  With external = 0 everywhere the result equals
+ * gm_engine_subcycle's; kernels are launched directly, not from a graph. */
+int gm_engine_stage_begin(gm_engine* e, const float* rows_host, int n_rows);
+int gm_engine_stage_l2(gm_engine* e, int32_t* part_host, int32_t* k_host);
+int gm_engine_stage_l3(gm_engine* e, int external);
+int gm_engine_stage_field(gm_engine* e, int which, void** ptr, int64_t* count);
 
 /* stepGOMELT (cF:2304-2397) for one toolpath row.  Writes the reference's
  * return value _resetmask (cF:2394-2397: L3 nodes whose S2 went 0 -> 1) into
@@ -477,6 +503,11 @@ int gm_nn_window_apply(const int32_t* n_host, const float* T, const float* Tface
                        void* stream);
 /* y += a x (FP32). */
 int gm_nn_axpy(int64_t n, float a, const float* x, float* y, void* stream);
+
+/* y[i] = x[i] for 0/1 phase bytes (the engine's uint8 S1) read as FP32 S1
+ * (the form gm_state_properties takes, cF:2589).  No reference counterpart:
+ * the reference keeps S1 as float throughout. */
+int gm_u8_to_f32(int64_t n, const uint8_t* x, float* y, void* stream);
 
 #ifdef __cplusplus
 }

@@ -993,10 +993,11 @@ def stepGOMELT(Levels, ne_nn, tmp_ne_nn, Shapes, LInterp, v, p, dt, laserP, subs
     (Levels[3]["Tprime0"], Levels[2]["Tprime0"], Levels[2]["T0"],
      Levels[1]["T0"]) = getBothNewTprimes(Levels, Levels[3]["T0"], L2T, LInterp[1], L1T,
                                           LInterp[0])
+    # Level0 state (cF:2385-2392); updated in place (the reference's
+    # functional .at[].set builds new arrays of the same values)
     L0 = Levels[0]
-    L0["S1"] = L0["S1"].copy()
     L0["S1"][L0["idx"]] = Levels[3]["S1"]
-    L0["S2"] = np.zeros_like(L0["S2"])
+    L0["S2"].fill(False)
     L0["S2"][L0["idx"]] = Levels[3]["S2"]
     resetmask = ((1 - 2 * preS2.astype(np.int64)) * Levels[3]["S2"]) == 1
     return Levels, resetmask
@@ -1102,10 +1103,10 @@ def subcycleGOMELT(Levels, ne_nn, Shapes, substrate, LInterp, tmp_ne_nn, laser_p
     L2["T0"], L2["S1"] = c["L2T"], c["L2S1"]
     L3["T0"], L3["Tprime0"], L3["S1"], L3["S2"] = c["L3T"], c["L3Tp"], c["L3S1"], c["L3S2"]
     L2["Tprime0"], L1["T0"] = getNewTprime(L2, L2["T0"], L1T, L1, LInterp[0])
+    # Level0 state (cF:3628-3630), updated in place
     L0 = Levels[0]
-    L0["S1"] = L0["S1"].copy()
     L0["S1"][L0["idx"]] = L3["S1"]
-    L0["S2"] = np.zeros_like(L0["S2"])
+    L0["S2"].fill(False)
     L0["S2"][L0["idx"]] = L3["S2"]
     return (Levels, np.stack(L2all), np.stack(L3all), np.stack(L3pall), c["max"], c["acc"])
 
@@ -1350,11 +1351,13 @@ def parsingGcode(Nonmesh, Properties, L2h, gcode_text=None):
 # Driver  (gm:16-530), without file output
 # ---------------------------------------------------------------------------
 
-def go_melt(solver_input, rows=None, max_rows=None, log=None):
+def go_melt(solver_input, rows=None, max_rows=None, log=None, log_rows=False):
     """gm:16-530: the time loop (single-step / subcycle / dwell), no output files.
 
     ``rows`` may pass pre-parsed toolpath rows (n, 7); otherwise the G-code of
     ``solver_input['nonmesh']`` is parsed.  Returns (Levels, info dict).
+    ``log(time_inc, Levels)`` is called after every chunk (and, with
+    ``log_rows``, after every single-step row).
     Layer changes, dwell and the checkpoint-driven single-step rule follow
     the reference; checkpoint, VTK and npz files are not written.
     """
@@ -1452,8 +1455,12 @@ def go_melt(solver_input, rows=None, max_rows=None, log=None):
                     racc = accum[idx] * (reset > 0)
                     max_accum[idx] = np.maximum(racc, max_accum[idx])
                     accum[idx] = accum[idx] - racc
-                    accum = melting_temp(Levels[3]["T0"], lp[5], p["T_liquidus"], accum, idx)
+                    # melting_temp (cF:3696-3712), in place on the driver's map
+                    np.add.at(accum, idx,
+                              (Levels[3]["T0"] > p["T_liquidus"]).astype(FLOAT) * F(lp[5]))
                     info["single"] += 1
+                    if log_rows and log is not None:
+                        log(time_inc + 1, Levels)
                 else:
                     if (not (Levels[2]["Tprime0"] == 0).all()
                             and not (Levels[3]["Tprime0"] == 0).all()):

@@ -65,7 +65,7 @@ def main():
     def log(ti, Levels):
         print(f"{name} {prec}: row {ti} at {time.time() - t0:.0f} s", flush=True)
 
-    Lo, info = O.go_melt(deck, rows=rows, max_rows=max_rows, log=log)
+    Lo, info = O.go_melt(deck, rows=rows, max_rows=max_rows, log=log, log_rows=True)
     L0, L3 = Lo[0], Lo[3]
     out = {"rows": np.int64(min(len(rows), max_rows or len(rows))),
            "chunks": np.array([c == "single" for c in info["chunks"]], np.uint8)}
