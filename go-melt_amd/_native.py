@@ -18,7 +18,7 @@ GM_MAX_KNOTS = 16
 (GM_T0, GM_TPRIME0, GM_S1, GM_S2, GM_ACCUM, GM_MAX_ACCUM, GM_HIST_L2, GM_HIST_L3,
  GM_HIST_TP3, GM_ACCUM_L3, GM_MAX_ACCUM_L3, GM_RESET_L3) = range(12)
 (GM_STAGE_L3_IN, GM_STAGE_L3_OUT, GM_STAGE_FACE_NEW, GM_STAGE_FACE_OLD,
- GM_STAGE_S1_IN) = range(5)
+ GM_STAGE_S1_IN, GM_STAGE_L3_LAST) = range(6)
 
 
 class GoMeltNativeError(RuntimeError):
@@ -163,6 +163,7 @@ _SIGS = {
     "gm_nn_window_apply": (_I, [_IP, _P, _P, _P, _I, _F, _F, _P, _P]),
     "gm_nn_axpy": (_I, [_I64, _F, _P, _P, _P]),
     "gm_u8_to_f32": (_I, [_I64, _P, _P, _P]),
+    "gm_nn_mse_energy": (_I, [_I64, _I, _I, _I, _I, _F, _P, _P, _P, _P, _P, _P]),
     "gm_engine_stage_begin": (_I, [_P, _P, _I]),
     "gm_engine_stage_l2": (_I, [_P, _P, _P]),
     "gm_engine_stage_l3": (_I, [_P, _I]),

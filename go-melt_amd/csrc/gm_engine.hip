@@ -574,7 +574,9 @@ struct SubcycleState {
     // outputs of the current substep's L2 stage
     float* T2new;
     uint8_t* S12new;
-    float* T3out;  // where the L3 stage leaves the substep's final L3 field
+    float* T3out;  // where the N3 plain solves leave the substep's final L3 field
+    float* T3ext;  // where a caller writes its own final L3 field (never T3cur)
+    const float* T3last;  // the final L3 field of the last closed substep
 };
 
 struct gm_engine {
@@ -610,6 +612,8 @@ struct gm_engine {
     float* Tp2x = nullptr;
     std::vector<float*> L2all, L3all, histTp;
     float* T3b[2] = {nullptr, nullptr};
+    float* T3x = nullptr;  // a caller's L3 jump of a Part-1 substep whose plain final
+                           // buffer is its start buffer (even N3)
     uint8_t* S13b[2] = {nullptr, nullptr};
     uint8_t* S23b[2] = {nullptr, nullptr};
     float* Tp3x = nullptr;
@@ -747,6 +751,7 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
         eng_alloc(e, (void**)&e->reset3, n3) ||
         eng_alloc(e, (void**)&e->fa2, n2 * 4) || eng_alloc(e, (void**)&e->fb2, n2 * 4) ||
         eng_alloc(e, (void**)&e->fa3, n3 * 4) || eng_alloc(e, (void**)&e->fb3, n3 * 4) ||
+        eng_alloc(e, (void**)&e->T3x, n3 * 4) ||
         eng_alloc(e, (void**)&e->d_desc, sizeof(GmStepDesc)) ||
         eng_alloc(e, (void**)&e->d_dm, sizeof(DevMat)) ||
         eng_alloc(e, (void**)&e->acc3, n3 * 4) || eng_alloc(e, (void**)&e->macc3, n3 * 4) ||
@@ -775,7 +780,7 @@ extern "C" int gm_engine_create(const gm_config* cfg, const gm_material* mat, vo
         // k, cp companions of every L2 and L3 temperature buffer a
         // carried-property step reads or writes
         std::vector<std::pair<const float*, int64_t>> keys = {
-            {e->T3, n3}, {e->T3b[0], n3}, {e->T3b[1], n3},
+            {e->T3, n3}, {e->T3b[0], n3}, {e->T3b[1], n3}, {e->T3x, n3},
             {e->T2, n2}, {e->T2b[0], n2}, {e->T2b[1], n2}};
         for (int i = 0; i < e->N2; ++i) {
             keys.push_back({e->L3all[i], n3});
@@ -1483,6 +1488,7 @@ static void stage_reset(gm_engine* e, SubcycleState& st) {
     st.T2new = nullptr;
     st.S12new = nullptr;
     st.T3out = nullptr;
+    st.T3ext = nullptr;
 }
 
 static void stage_prologue(gm_engine* e, SubcycleState& st) {
@@ -1540,19 +1546,24 @@ static void stage_l2(gm_engine* e, SubcycleState& st, int part, int k) {
     st.b2 ^= 1;
     l2_substep(e, st.T2cur, st.S12cur, st.S12new, Gk, (float)(k + 1) / fN2, st.T2new, k);
     launch_face_bc(e->d_desc, 3, 2, e->cfg.level[3], st.T2new, st.T2cur, e->fa3, e->fb3, e->s);
-    // the buffer that will hold the substep's final L3 field
+    // the buffer that will hold the substep's final L3 field; a caller's own
+    // field goes there too unless that is the start field's buffer (Part 1 with
+    // an even N3 ends its solves in the buffer it started from), which must
+    // stay intact for a fallback to the plain solves
     if (part == 1) st.T3out = e->T3b[(st.b3 + e->N3 - 1) & 1];
     else st.T3out = e->L3all[k];
+    st.T3ext = st.T3out == st.T3cur ? e->T3x : st.T3out;
 }
 
 // L3 stage, closing: T' and injection into the new L2 field, then the
 // substep's state becomes the next one's input.
 static void stage_l3_close(gm_engine* e, SubcycleState& st, int part, int k,
-                           const uint8_t* S13fin, const uint8_t* S23fin) {
-    new_tprime(e, 3, st.T3out, st.T2new, e->histTp[k], st.S12new);
+                           const uint8_t* S13fin, const uint8_t* S23fin, float* T3fin) {
+    new_tprime(e, 3, T3fin, st.T2new, e->histTp[k], st.S12new);
     st.T2cur = st.T2new;
     st.S12cur = st.S12new;
-    st.T3cur = st.T3out;
+    st.T3cur = T3fin;
+    st.T3last = T3fin;
     st.S13cur = S13fin;
     if (part == 2) st.S23cur = S23fin;
     st.Tp3cur = e->histTp[k];
@@ -1588,7 +1599,7 @@ static void stage_l3_plain(gm_engine* e, SubcycleState& st, int part, int k) {
             S23in = S23o;
         }
     }
-    stage_l3_close(e, st, part, k, S13in, S23in);
+    stage_l3_close(e, st, part, k, S13in, S23in, st.T3out);
 }
 
 // Phase flags and melt time of one jump of the L3 window over the N3 steps of
@@ -1628,19 +1639,22 @@ static void launch_jump_state(const GmStepDesc* d, const DevMat* m, int k, const
 }
 
 // L3 stage of substep k when the caller has written the substep's final L3
-// field into st.T3out (one jump over the N3 steps): phase flags, melt time
+// field into st.T3ext (one jump over the N3 steps): phase flags, melt time
 // and the carried k, cp of that field, then T'.
 static void stage_l3_external(gm_engine* e, SubcycleState& st, int part, int k) {
     const int64_t n3 = e->nn[3];
+    float* T3fin = st.T3ext;
     uint8_t* S13o = part == 1 ? e->S13b[0] : e->S13;
-    launch_jump_state(e->d_desc, e->d_dm, k, st.T3cur, st.T3out, st.S13cur, S13o,
+    launch_jump_state(e->d_desc, e->d_dm, k, st.T3cur, T3fin, st.S13cur, S13o,
                       e->host.sub[3], part == 2 ? e->S23 : nullptr,
                       part == 2 ? e->acc3 : nullptr, part == 2 ? e->macc3 : nullptr, n3, e->s);
-    launch_props_kc(e->d_dm, st.T3out, S13o, n3, e->host.sub[3], kc_of(e, st.T3out), e->s,
+    launch_props_kc(e->d_dm, T3fin, S13o, n3, e->host.sub[3], kc_of(e, T3fin), e->s,
                     nullptr, nullptr, 0, 0, nullptr);
-    if (part == 1) st.b3 ^= (e->N3 & 1);  // the buffer parity the N3 solves leave
+    // the ping-pong parity the N3 solves would leave (the next solves start in
+    // the other buffer than the one holding the field; with T3x both are free)
+    if (part == 1) st.b3 ^= (e->N3 & 1);
     else st.b3 ^= ((e->N3 - 1) & 1);
-    stage_l3_close(e, st, part, k, S13o, part == 2 ? e->S23 : nullptr);
+    stage_l3_close(e, st, part, k, S13o, part == 2 ? e->S23 : nullptr, T3fin);
 }
 
 static void stage_mid(gm_engine* e, SubcycleState& st) {
@@ -1751,6 +1765,7 @@ extern "C" int gm_engine_subcycle(gm_engine* e, const float* rows, int n_rows) {
 extern "C" int gm_engine_stage_begin(gm_engine* e, const float* rows, int n_rows) {
     if (e->stg_active) return fail("a staged subcycle is already in progress");
     if (subcycle_rows(e, rows, n_rows)) return -1;
+    e->stg.T3last = nullptr;
     stage_prologue(e, e->stg);
     e->stg_active = 1;
     e->stg_part = 1;
@@ -1790,13 +1805,19 @@ extern "C" int gm_engine_stage_l3(gm_engine* e, int external) {
 }
 
 extern "C" int gm_engine_stage_field(gm_engine* e, int which, void** ptr, int64_t* count) {
+    const SubcycleState& st = e->stg;
+    if (which == GM_STAGE_L3_LAST) {
+        if (!st.T3last) return fail("gm_engine_stage_field: no L3 stage has run yet");
+        *ptr = (void*)st.T3last;
+        *count = e->nn[3];
+        return 0;
+    }
     if (!e->stg_active || e->stg_phase != 1)
         return fail("gm_engine_stage_field: only between the l2 and l3 stages");
-    const SubcycleState& st = e->stg;
     void* p = nullptr;
     switch (which) {
         case GM_STAGE_L3_IN: p = (void*)st.T3cur; break;
-        case GM_STAGE_L3_OUT: p = (void*)st.T3out; break;
+        case GM_STAGE_L3_OUT: p = (void*)st.T3ext; break;
         case GM_STAGE_FACE_NEW: p = (void*)e->fa3; break;
         case GM_STAGE_FACE_OLD: p = (void*)e->fb3; break;
         case GM_STAGE_S1_IN: p = (void*)st.S13cur; break;
@@ -2499,6 +2520,18 @@ extern "C" int gm_nn_mse(int64_t nvox, int C, const float* Y, const float* targe
                          double* loss, void* stream) {
     if (nvox < 0 || C < 1 || !Y || !target || !dY || !loss) return fail("gm_nn_mse: bad arguments");
     if (nvox) launch_nn_mse(nvox, C, Y, target, dY, loss, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_nn_mse_energy(int64_t nw, int nb, int C, int nx, int ny, float lambda,
+                                const float* Y, const float* target, double* wsum, float* dY,
+                                double* loss, void* stream) {
+    if (nw < 1 || nb < 1 || nb > 65535 || C < 1 || nx < 1 || ny < 1 ||
+        nw % ((int64_t)nx * ny) || !Y || !target || !wsum || !dY || !loss)
+        return fail("gm_nn_mse_energy: bad arguments");
+    launch_nn_mse_energy(nw, nb, C, nx, ny, lambda, Y, target, wsum, dY, loss,
+                         (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -244,6 +244,9 @@ void launch_nn_window_input(const int32_t* n, const float* geo, const float* T, 
                             const float* prm, uint16_t* X, hipStream_t s);
 void launch_nn_axpy(int64_t n, float a, const float* x, float* y, hipStream_t s);
 void launch_nn_u8_to_f32(int64_t n, const uint8_t* x, float* y, hipStream_t s);
+void launch_nn_mse_energy(int64_t nw, int nb, int C, int nx, int ny, float lambda,
+                          const float* Y, const float* target, double* wsum, float* dY,
+                          double* loss, hipStream_t s);
 void launch_nn_window_apply(const int32_t* n, const float* T, const float* Tface, const float* Y,
                             int C, float T_scale, float T_floor, float* Tn, hipStream_t s);
 

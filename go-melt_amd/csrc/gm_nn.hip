@@ -266,6 +266,84 @@ __global__ void __launch_bounds__(256) k_mse(int64_t nvox, int C, const float* _
     }
 }
 
+// Energy-consistent loss of nb windows of nw voxels (nvox = nb nw; nx x ny x nz
+// nodes, x fastest), r = Y[., 0] - target:  L = 0.5 mean_v r_v^2 +
+// 0.5 lambda mean_w m_w^2, m_w the sum of r over the window's free voxels (not
+// on the Dirichlet faces W, E, S, N, bottom, whose outputs a jump does not use)
+// divided by nw: the window's heat-content error, which the energy audit of a
+// jump sums as C (T_new - T) over the free nodes.  Pass 1 sums r per window
+// (FP64, one atomic per block and window); pass 2 writes
+// dL/dY = (r_v + lambda m_w [v free]) / nvox and adds the loss.
+__device__ __forceinline__ bool window_face(int64_t t, int nx, int ny, int64_t nw) {
+    const int nz = (int)(nw / ((int64_t)nx * ny));
+    const int ix = (int)(t % nx), iy = (int)((t / nx) % ny), iz = (int)(t / ((int64_t)nx * ny));
+    (void)nz;
+    return ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1 || iz == 0;
+}
+
+__global__ void __launch_bounds__(256) k_window_rsum(int64_t nw, int nb, int C, int nx, int ny,
+                                                     const float* __restrict__ Y,
+                                                     const float* __restrict__ target,
+                                                     double* __restrict__ wsum) {
+    __shared__ double part[4];
+    const int w = blockIdx.y;
+    double l = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nw; t += stride) {
+        const int64_t v = (int64_t)w * nw + t;
+        if (!window_face(t, nx, ny, nw)) l += (double)(Y[v * C] - target[v]);
+    }
+#pragma unroll
+    for (int sh = 32; sh >= 1; sh >>= 1) l += __shfl_xor(l, sh, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = part[0] + part[1] + part[2] + part[3];
+        if (b != 0.0) atomicAdd(&wsum[w], b);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_mse_energy(int64_t nw, int nb, int C, int nx, int ny,
+                                                    float lambda,
+                                                    const float* __restrict__ Y,
+                                                    const float* __restrict__ target,
+                                                    const double* __restrict__ wsum,
+                                                    float* __restrict__ dY,
+                                                    double* __restrict__ loss) {
+    __shared__ double part[4];
+    const int w = blockIdx.y;
+    const int64_t nvox = nw * nb;
+    const double m = wsum[w] / (double)nw;
+    double l = (blockIdx.x == 0 && threadIdx.x == 0) ? 0.5 * (double)lambda * m * m / nb : 0.0;
+    const float mf = (float)m;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nw; t += stride) {
+        const int64_t v = (int64_t)w * nw + t;
+        const float r = Y[v * C] - target[v];
+        l += 0.5 * (double)r * (double)r / (double)nvox;
+        dY[v * C] = (r + (window_face(t, nx, ny, nw) ? 0.0f : lambda * mf)) / (float)nvox;
+        for (int c = 1; c < C; ++c) dY[v * C + c] = 0.0f;
+    }
+#pragma unroll
+    for (int sh = 32; sh >= 1; sh >>= 1) l += __shfl_xor(l, sh, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = part[0] + part[1] + part[2] + part[3];
+        if (b != 0.0) atomicAdd(loss, b);
+    }
+}
+
+void launch_nn_mse_energy(int64_t nw, int nb, int C, int nx, int ny, float lambda,
+                          const float* Y, const float* target, double* wsum, float* dY,
+                          double* loss, hipStream_t s) {
+    const unsigned gx = std::min(nn_blocks(nw), 256u);
+    hipLaunchKernelGGL(k_window_rsum, dim3(gx, nb), dim3(256), 0, s, nw, nb, C, nx, ny, Y,
+                       target, wsum);
+    hipLaunchKernelGGL(k_mse_energy, dim3(gx, nb), dim3(256), 0, s, nw, nb, C, nx, ny, lambda, Y,
+                       target, wsum, dY, loss);
+}
+
 // Network input of one window (nx x ny x nz nodes, x fastest), 16 channels:
 // 0: (T - T_amb) / T_scale, 1: (T_face - T_amb) / T_scale on the Dirichlet
 // faces (W, E, S, N, bottom) and 0 inside, 2: the laser intensity

@@ -216,6 +216,23 @@ def mse(Y, target, loss):
     return dY
 
 
+def mse_energy(Y, target, loss, lam):
+    """loss += 0.5 mean (Y[..., 0] - target)^2 + 0.5 lam mean over the windows
+    of m_w^2, m_w the sum of the error over window w's free voxels (off the
+    Dirichlet faces) over its voxel count; Y is (nb, D, H, W, C).  Returns
+    dloss/dY."""
+    nb, D, H, Wd = Y.shape[:4]
+    nvox = nb * D * H * Wd
+    if target.numel() != nvox or loss.numel() < 1:
+        raise ValueError("mse_energy: target needs one value per voxel of Y")
+    dY = torch.empty_like(Y)
+    ws = torch.empty(nb, dtype=torch.float64, device=Y.device)
+    zero(ws)
+    N.check(N.lib().gm_nn_mse_energy(D * H * Wd, nb, Y.shape[-1], Wd, H, float(lam), _ptr(Y),
+                                     _ptr(target), _ptr(ws), _ptr(dY), _ptr(loss), _stream()))
+    return dY
+
+
 # ---------------------------------------------------------------------------
 # The operator: a 3-D U-Net of 3x3x3 convolutions
 # ---------------------------------------------------------------------------
@@ -322,11 +339,13 @@ class UNet:
         for c in self.layers():
             c.step(self.t, lr)
 
-    def train_step(self, Xb, target, lr=1e-3):
-        """One Adam step on a batch; returns the loss (FP64 device scalar)."""
+    def train_step(self, Xb, target, lr=1e-3, energy=0.0):
+        """One Adam step on a batch; returns the loss (FP64 device scalar).
+        energy > 0 adds the penalty on each window's mean error (mse_energy)."""
         Y = self.forward(Xb)
         loss = _zeros(1, torch.float64)
-        dY = mse(Y, target, loss)
+        dY = mse_energy(Y, target, loss, energy) if energy > 0 else \
+            mse(Y, target, loss)
         self.backward(dY)
         self.step(lr)
         return loss
@@ -474,9 +493,10 @@ def rel_error(net, X, Y, batch=4):
     return (num / den) ** 0.5 if den > 0 else 0.0
 
 
-def train(net, X, Y, iters=100, batch=4, lr=1e-3, seed=0, log=None):
+def train(net, X, Y, iters=100, batch=4, lr=1e-3, seed=0, log=None, energy=0.0):
     """Adam on mini-batches drawn without replacement (reshuffled each pass);
-    returns the loss history."""
+    returns the loss history.  energy: weight of the windows' mean-error
+    penalty (mse_energy), 0 for the plain MSE."""
     import numpy as np
     rng = np.random.default_rng(seed)
     n = X.shape[0]
@@ -490,7 +510,7 @@ def train(net, X, Y, iters=100, batch=4, lr=1e-3, seed=0, log=None):
         idx = [order.pop() for _ in range(batch)]
         _gather(X, idx, xb)
         _gather(Y, idx, yb)
-        hist.append(net.train_step(xb, yb.reshape(-1), lr))
+        hist.append(net.train_step(xb, yb.reshape(-1), lr, energy))
         if log and (it % 20 == 0 or it == iters - 1):
             log(f"iter {it} loss {float(hist[-1].item()):.4e}")
     return [float(h.item()) for h in hist]
@@ -780,7 +800,8 @@ class EngineOperator:
                 T0c = self._copy(T0, n3)
                 fac = self._copy(fa, n3)
                 N.check(lib.gm_engine_stage_l3(e.h, 0))
-                self.record(T0c, fac, rows.copy(), self._copy(out, n3))
+                last, _ = _stage_field(e, N.GM_STAGE_L3_LAST)
+                self.record(T0c, fac, rows.copy(), self._copy(last, n3))
             else:
                 N.check(lib.gm_engine_stage_l3(e.h, 0))
             return
@@ -797,9 +818,10 @@ class EngineOperator:
         self.stats["residuals"].append(res)
         if self.judge_every and self.stats["jumps"] % self.judge_every == 0:
             Tj = self._copy(out, n3)
-            T0c = self._copy(T0, n3)  # the N3 solves reuse the start field's buffer
+            T0c = self._copy(T0, n3)  # the N3 solves may reuse the start field's buffer
             N.check(lib.gm_engine_stage_l3(e.h, 0))
-            Tp = self._copy(out, n3)
+            last, _ = _stage_field(e, N.GM_STAGE_L3_LAST)
+            Tp = self._copy(last, n3)
             # the audit of the plain solves over the same substep, for scale
             res_plain = self._audit(e, g, T0c, Tp, s1, rows)
             e.sync()

@@ -256,7 +256,8 @@ enum {
     GM_STAGE_L3_OUT = 1,   /* where the substep's final L3 field goes           */
     GM_STAGE_FACE_NEW = 2, /* L3 face values from the substep's new L2 field    */
     GM_STAGE_FACE_OLD = 3, /* ... from its start-of-substep L2 field            */
-    GM_STAGE_S1_IN = 4     /* L3 phase flags at the start of the substep        */
+    GM_STAGE_S1_IN = 4,    /* L3 phase flags at the start of the substep        */
+    GM_STAGE_L3_LAST = 5   /* the final L3 field of the last finished substep   */
 };
 
 /* SetupLevels (cF:115-264) + getStaticNodesAndElements/getStaticSubcycle:
@@ -297,7 +298,10 @@ int gm_engine_subcycle(gm_engine* e, const float* rows_host, int n_rows);
  * the L2 substep (restriction, L2 solve, L3 face values; part, k returned) and
  * stage_l3 finishes it: external = 0 runs the N3 L3 solves, external = 1
  * takes the final L3 field the caller wrote into GM_STAGE_L3_OUT (phase
- * flags, melt time over the substep, carried k and cp), then T'.  The l3
+ * flags, melt time over the substep, carried k and cp), then T'.  The
+ * GM_STAGE_L3_OUT buffer is never the start field's, so the plain solves can
+ * still run after a caller wrote it; their final field (and, after an
+ * external stage, the caller's) is GM_STAGE_L3_LAST once the stage is done.  The l3
  * stage closing Part 1 runs the L1 corrector, the one closing Part 2 the
  * epilogue.  With external = 0 everywhere the result equals
  * gm_engine_subcycle's; kernels are launched directly, not from a graph. */
@@ -490,6 +494,18 @@ int gm_nn_relu_bias_bwd(int64_t nvox, int C, const float* dA, const float* dA2,
  * = (Y[v][0] - target[v]) / nvox in channel 0, zero elsewhere. */
 int gm_nn_mse(int64_t nvox, int C, const float* Y, const float* target, float* dY,
               double* loss, void* stream);
+
+/* Energy-consistent loss of nb windows of nw = nx*ny*nz voxels each (x
+ * fastest; Y: nb*nw*C FP32, channel 0 used; target: nb*nw): loss +=
+ * 0.5 mean_v r_v^2 + 0.5 lambda mean_w m_w^2, with r = Y - target and m_w the
+ * sum of r over the window's free voxels (not on the faces x = 0, x = max,
+ * y = 0, y = max, z = 0) divided by nw: the window's heat-content error, which
+ * the energy audit of a jump measures; dY = (r_v + lambda m_w [v free]) /
+ * (nb nw) on channel 0, 0 on the others.  wsum: nb FP64 of device memory,
+ * zeroed by the caller.  No reference counterpart (training loss of the
+ * learned operator). */
+int gm_nn_mse_energy(int64_t nw, int nb, int C, int nx, int ny, float lambda, const float* Y,
+                     const float* target, double* wsum, float* dY, double* loss, void* stream);
 /* Network input of one window of n = {nx, ny, nz} nodes (x fastest) at
  * geo = {x0, y0, z0, h}: 16 bf16 channels, 0: (T - T_amb)/T_scale, 1: the
  * Dirichlet face values (W, E, S, N, bottom) likewise, 0 inside, 2: the laser

@@ -21,12 +21,12 @@ import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _deck(tmp_path, track=1.6):
+def _deck(tmp_path, track=1.6, n3=3):
     """L1 16 x 16 x 8 elements (0.5 mm), L2 16 x 16 x 8 (0.25 mm) and an L3 of
     16 x 16 x 8 nodes (15 x 15 x 7 elements, 0.125 mm: even node counts for
     the network's pooling), every window on the lattice of the next coarser
     level; dt3 = 0.25 ms keeps the explicit step stable."""
-    inp, g = W.tiny(n=(16, 16, 8), n2=2, n3=3, track=track, dt3=2.5e-4,
+    inp, g = W.tiny(n=(16, 16, 8), n2=2, n3=n3, track=track, dt3=2.5e-4,
                     props={"laser_power": 600.0, "emissivity": 0.4})
     inp["Level3"] = {"elements": [15, 15, 7],
                      "bounds": {"x": [3.125, 5.0], "y": [3.125, 5.0], "z": [-0.875, 0.0]}}
@@ -78,10 +78,14 @@ def _net(seed=0):
     return S.UNet(channels=(16, 32), seed=seed)
 
 
-def test_untrained_operator_falls_back_everywhere(tmp_path):
+# N3 = 4: the Part-1 solves of a substep end in the buffer they started from,
+# so the operator's field goes to a third buffer and the fallback still starts
+# from the substep's start field
+@pytest.mark.parametrize("n3", [3, 4])
+def test_untrained_operator_falls_back_everywhere(tmp_path, n3):
     import surrogate as S
     import computeFunctions as cf
-    deck = _deck(tmp_path)
+    deck = _deck(tmp_path, n3=n3)
     plain = _fields(_run(deck))
     p = cf.SetupProperties(deck["properties"])
     op = S.EngineOperator(_net(), p, residual_limit=0.01, judge_every=0)
@@ -91,6 +95,8 @@ def test_untrained_operator_falls_back_everywhere(tmp_path):
           np.round(np.abs(st["residuals"]), 3))
     assert st["jumps"] == 2 * 2 * run.stats["subcycle"] and st["jumps"] > 0
     assert st["fallbacks"] == st["jumps"]
+    # the untrained jumps fail the audit by far; the plain solves pass it
+    assert min(abs(r) for r in st["residuals"]) > 0.05
     got = _fields(run)
     for key in plain:
         scale = max(float(np.max(np.abs(plain[key]))), 1e-30)
@@ -129,6 +135,22 @@ def test_recorder_sees_every_substep(tmp_path):
     n3 = int(run.Levels[3]["nn"])
     assert len(seen) == 2 * 2 * run.stats["subcycle"]
     assert all(s == (n3, n3, (3, 7), n3) for s in seen)
+
+
+def test_jump_audit_of_plain_solves(tmp_path):
+    """The audit of every judged substep's plain solves (N3 = 4, both parts):
+    the energy identity holds for them to well under the 1% limit, so the
+    trigger separates jumps by their own error."""
+    import surrogate as S
+    import computeFunctions as cf
+    deck = _deck(tmp_path, n3=4)
+    p = cf.SetupProperties(deck["properties"])
+    op = S.EngineOperator(_net(2), p, residual_limit=float("inf"), judge_every=1)
+    _run(deck, operator=op)
+    plain = [abs(j["residual_plain"]) for j in op.stats["judged"]]
+    print("plain-solve residuals", np.round(plain, 5))
+    assert len(plain) == op.stats["jumps"] > 0
+    assert max(plain) < 0.01
 
 
 def test_engine_windows_train_and_run_in_the_loop(tmp_path):

@@ -239,3 +239,46 @@ def test_operator_in_the_loop_fallback(S):
     sur, plain = out["final_T"]
     for a, b in zip(sur, plain):
         assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
+def test_energy_consistent_loss(S):
+    """mse_energy against NumPy: loss 0.5 mean r^2 + 0.5 lam mean_w m_w^2 and
+    gradient (r + lam m_w [free]) / nvox on channel 0 (r = Y[..., 0] - target,
+    m_w the sum of r over window w's voxels off the faces x = 0, x = max,
+    y = 0, y = max, z = 0, over the window's voxel count); a planted uniform
+    offset on one window's free voxels moves m_w by the offset times their
+    share, an offset on its faces leaves m_w alone."""
+    import torch
+    rng = np.random.default_rng(21)
+    nb, D, H, Wd, C, lam = 3, 10, 20, 25, 16, 250.0
+    nw = D * H * Wd
+    Y = rng.standard_normal((nb, D, H, Wd, C)).astype(np.float32)
+    T = rng.standard_normal(nb * nw).astype(np.float32)
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(Wd), indexing="ij")
+    free = ~((x == 0) | (x == Wd - 1) | (y == 0) | (y == H - 1) | (z == 0))
+    free = free.reshape(-1)
+
+    def ref(Yn):
+        r = Yn[..., 0].astype(np.float64).reshape(nb, nw) - T.astype(np.float64).reshape(nb, nw)
+        m = (r * free).sum(axis=1) / nw
+        return r, m, 0.5 * np.mean(r ** 2) + 0.5 * lam * np.mean(m ** 2)
+
+    loss = S._zeros(1, torch.float64)
+    dY = _np(S.mse_energy(torch.from_numpy(Y).cuda(), torch.from_numpy(T).cuda(), loss, lam))
+    r, m, want = ref(Y)
+    # FP32 residuals, FP64 sums
+    assert float(loss.item()) == pytest.approx(want, rel=1e-6)
+    g = (r + lam * m[:, None] * free[None, :]) / (nb * nw)
+    # FP32 sum of two terms of size <~ 4 over nvox: a few ulps of 4 / 15000
+    np.testing.assert_allclose(dY[..., 0].reshape(nb, nw), g, rtol=2e-5, atol=1e-10)
+    assert not np.any(dY[..., 1:])
+    for where, moves in ((free, True), (~free, False)):
+        Y2 = Y.copy()
+        v = Y2[1, ..., 0].reshape(-1).copy()
+        v[where] += 0.01
+        Y2[1, ..., 0] = v.reshape(D, H, Wd)
+        l2 = S._zeros(1, torch.float64)
+        S.mse_energy(torch.from_numpy(Y2).cuda(), torch.from_numpy(T).cuda(), l2, lam)
+        r2, m2, want2 = ref(Y2)
+        assert float(l2.item()) == pytest.approx(want2, rel=1e-6)
+        assert (abs(m2[1] - m[1]) > 1e-3) == moves
