@@ -49,6 +49,11 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-instances measurement")
     ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--no-stacks", action="store_true",
+                    help="skip the batched three-level stacks measurement")
+    ap.add_argument("--stacks-chunks", type=int, default=3)
+    ap.add_argument("--stacks-only", action="store_true",
+                    help="print only the stacks measurement (one JSON line)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned-operator measurement")
     ap.add_argument("--surrogate-iters", type=int, default=1250)
@@ -208,6 +213,56 @@ def batch_measure(n, iters=50):
     }
 
 
+def stacks_measure(n, chunks=3, seed=0):
+    """The batched configuration as three-level stacks: n single-track runs,
+    each its own L1/L2/L3 stack (element counts drawn from the batch sizes),
+    advanced in lockstep with the L3 solves of all runs in one launch per
+    solve; against the same runs advanced one after another through their own
+    recorded subcycles.  Both advance every run by `chunks` subcycled L1
+    steps after two warm-up chunks."""
+    import numpy as np
+    import workloads as W
+    import batch as B
+    decks = []
+    with tempfile.TemporaryDirectory() as tmp:
+        for i, (inp, g) in enumerate(W.stack_decks(n=n, seed=seed)):
+            with open(W.write_deck(inp, g, os.path.join(tmp, f"s{i}"))) as fh:
+                decks.append(json.load(fh))
+        t0 = time.perf_counter()
+        st = B.Stacks(decks)
+        build_s = time.perf_counter() - t0
+        st.step()       # the single-step chunk
+        st.step_each()  # records every run's subcycle graph
+        st.sync()
+        mm_per_chunk = 0.0
+        l3_nodes = 0
+        for r in st.runs:
+            nm = r.Nonmesh
+            rows = r.subcycle[2]
+            mm_per_chunk += float(nm["laser_velocity"]) * float(nm["timestep_L3"]) * rows
+            l3_nodes += int(np.prod([len(v) for v in r.Levels[3]["node_coords"]]))
+        ms = {}
+        for mode in ("batched", "sequential"):  # every engine is on the Stacks' stream
+            t0 = time.perf_counter()
+            for _ in range(chunks):
+                (st.step if mode == "batched" else st.step_each)()
+            st.sync()
+            ms[mode] = (time.perf_counter() - t0) * 1e3 / chunks
+        n23 = st.runs[0].subcycle[2]
+    return {
+        "workload": f"{n} single-track three-level stacks, L3 element counts drawn (seed "
+                    f"{seed}) from {W.BATCH_SIZES}, L2 and L1 4x and 16x the L3 extent, "
+                    f"P 100-400 W, 0.5-2 m/s, r 30-70 um",
+        "setup_s": build_s,
+        "ms_per_l1_step_batched": ms["batched"],
+        "ms_per_l1_step_sequential": ms["sequential"],
+        "speedup": ms["sequential"] / ms["batched"],
+        "scan_mm_per_s_batched": mm_per_chunk / (ms["batched"] * 1e-3),
+        "scan_mm_per_s_sequential": mm_per_chunk / (ms["sequential"] * 1e-3),
+        "l3_node_updates_per_s_batched": l3_nodes * n23 / (ms["batched"] * 1e-3),
+    }
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -280,6 +335,9 @@ def main():
         import torch  # noqa: F401  (its HIP runtime first, before libgomelt.so)
         print(json.dumps(surrogate_measure(a.surrogate_windows, a.surrogate_iters,
                                            a.c5_steps)))
+        return
+    if a.stacks_only:
+        print(json.dumps(stacks_measure(a.batch_n, chunks=a.stacks_chunks)))
         return
     import numpy as np
     import workloads as W
@@ -403,6 +461,11 @@ def main():
             result["batch"] = batch_measure(a.batch_n)
         except Exception as exc:  # reported, not fatal to the headline line
             result["batch"] = {"error": repr(exc)}
+    if not a.no_stacks:
+        try:
+            result["stacks"] = stacks_measure(a.batch_n, chunks=a.stacks_chunks)
+        except Exception as exc:  # reported, not fatal to the headline line
+            result["stacks"] = {"error": repr(exc)}
     print(json.dumps(result))
 
 

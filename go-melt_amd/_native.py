@@ -168,6 +168,9 @@ _SIGS = {
     "gm_engine_stage_l2": (_I, [_P, _P, _P]),
     "gm_engine_stage_l3": (_I, [_P, _I]),
     "gm_engine_stage_field": (_I, [_P, _I, _P, _P]),
+    "gm_multi_create": (_I, [_P, _I, _P]),
+    "gm_multi_l3": (_I, [_P]),
+    "gm_multi_destroy": (_I, [_P]),
 }
 
 SYMBOLS = tuple(_SIGS)

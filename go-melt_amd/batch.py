@@ -95,3 +95,105 @@ class Batch:
                 N.check(N.lib().gm_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes, self.stream.ptr))
                 self.stream.sync()  # the host array must outlive the copy
         N.check(N.lib().gm_batch_refresh(self.h))
+
+
+class _Deferred:
+    """Operator hook of one run in a Stacks: its subcycleGOMELT only takes the
+    rows and runs the prologue; Stacks.step runs the L2 and L3 stages of all
+    runs together."""
+
+    def __init__(self, owner):
+        self.owner = owner
+
+    def subcycle(self, e, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        N.check(N.lib().gm_engine_stage_begin(e.h, N.fptr(rows), rows.shape[0]))
+        self.owner._pending.append(e)
+
+
+class Stacks:
+    """Independent three-level runs (each its own L1/L2/L3 stack, the
+    reference's time loop gm:134-493 per run) advanced together.
+
+    One TimeLoop per deck, all engines on one stream.  Each step gives every
+    run its next chunk of N2*N3 toolpath rows; a single-step chunk runs as in
+    the plain driver, a subcycled one (subcycleGOMELT cF:3224-3632) stage by
+    stage in lockstep with the others: per L2 substep the L2 stage of each run,
+    then the N3 L3 solves of all subcycling runs as N3 batched launches
+    (gm_multi_l3).  Every run ends where its own unbatched run ends."""
+
+    def __init__(self, decks):
+        import computeFunctions as cf
+        import go_melt as G
+        N.init_device()
+        self.stream = N.Stream()
+        with cf.engines_on_stream(self.stream):
+            self.runs = [G.TimeLoop(d, quiet=True, write_outputs=False) for d in decks]
+        self.n = len(self.runs)
+        self._pending = []
+        for r in self.runs:
+            cf.attach_l3_operator(r.Levels, _Deferred(self))
+        self._iters = [r.chunks() for r in self.runs]
+        self._multi = None
+
+    def _multi_of(self, engines):
+        key = tuple(e.h for e in engines)
+        if self._multi is None or self._multi[0] != key:
+            if self._multi is not None:
+                N.lib().gm_multi_destroy(self._multi[1])
+            arr = (C.c_void_p * len(key))(*key)
+            h = C.c_void_p()
+            N.check(N.lib().gm_multi_create(arr, len(key), C.byref(h)))
+            self._multi = (key, h.value)
+        return self._multi[1]
+
+    def step(self):
+        """One chunk of every run; False when a run's toolpath has ended."""
+        lib = N.lib()
+        self._pending = []
+        going = True
+        for r, it in zip(self.runs, self._iters):
+            rows = next(it, None)
+            if not rows:
+                going = False
+                continue
+            going = r.step_chunk(rows) and going
+        pend = self._pending
+        if pend:
+            n2 = pend[0].N2
+            multi = self._multi_of(pend)
+            for _ in range(2 * n2):
+                for e in pend:
+                    N.check(lib.gm_engine_stage_l2(e.h, None, None))
+                N.check(lib.gm_multi_l3(multi))
+        self._pending = []
+        return going
+
+    def step_each(self):
+        """One chunk of every run, one run after another through its own
+        recorded subcycle (no batching): the sequential baseline of step."""
+        import computeFunctions as cf
+        going = True
+        for r, it in zip(self.runs, self._iters):
+            rows = next(it, None)
+            if not rows:
+                going = False
+                continue
+            cf.attach_l3_operator(r.Levels, None)
+            try:
+                going = r.step_chunk(rows) and going
+            finally:
+                cf.attach_l3_operator(r.Levels, _Deferred(self))
+        return going
+
+    def sync(self):
+        self.stream.sync()
+        for r in self.runs:
+            r.eng.sync_coords()
+
+    def __del__(self):
+        try:
+            if self._multi is not None:
+                N.lib().gm_multi_destroy(self._multi[1])
+        except Exception:
+            pass

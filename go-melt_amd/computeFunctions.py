@@ -421,6 +421,25 @@ def getSubstrateNodes(Levels):
 # ---------------------------------------------------------------------------
 
 
+_ENGINE_STREAM = []  # a stream new engines share (engines_on_stream)
+
+
+class engines_on_stream:
+    """Context in which SetupLevels creates its engines on the given stream
+    (an N.Stream), for runs advanced together (batch.Stacks)."""
+
+    def __init__(self, stream):
+        self.stream = stream
+
+    def __enter__(self):
+        _ENGINE_STREAM.append(self.stream)
+        return self.stream
+
+    def __exit__(self, *exc):
+        _ENGINE_STREAM.pop()
+        return False
+
+
 class Engine:
     """One gm_engine: the three levels, Level0 and the scratch of the L1 step."""
 
@@ -428,7 +447,7 @@ class Engine:
         N.init_device()
         L0, L1, L2, L3 = Levels
         self.props = p
-        self.stream = N.Stream()
+        self.stream = _ENGINE_STREAM[-1] if _ENGINE_STREAM else N.Stream()
         self.mat = _material(p)
         cfg = N.gm_config()
         for l, lv in enumerate(Levels):

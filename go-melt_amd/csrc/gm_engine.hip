@@ -1295,8 +1295,9 @@ static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8
 }
 
 // L3 solve of the subcycle with the nodal properties carried between steps
-static void run_l3_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
-                      int row, float alpha, float* Tout, const uint8_t* S2prev, uint8_t* S2out) {
+static LSArgs l3_args_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
+                         int row, float alpha, float* Tout, const uint8_t* S2prev,
+                         uint8_t* S2out) {
     LSArgs a = ls_default();
     a.level = 3;
     a.T = Tin;
@@ -1328,6 +1329,12 @@ static void run_l3_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8
     a.w2 = e->w2[3];
     a.w3 = e->w3[3];
     a.prec_ok = e->dm.pnb > 0;
+    return a;
+}
+
+static void run_l3_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
+                      int row, float alpha, float* Tout, const uint8_t* S2prev, uint8_t* S2out) {
+    const LSArgs a = l3_args_kc(e, Tin, S1in, S1out, row, alpha, Tout, S2prev, S2out);
     launch_level_step_kc(a, e->d_desc, e->cfg.level[3], e->d_dm, e->s);
 }
 
@@ -1569,37 +1576,63 @@ static void stage_l3_close(gm_engine* e, SubcycleState& st, int part, int k,
     st.Tp3cur = e->histTp[k];
 }
 
+// Buffers of the N3 L3 solves of substep k, solve by solve: Part 1 ping-pongs
+// the field between the two work buffers and keeps S1 in a work buffer (the
+// first solve writes all of it, later ones update it in place, written where
+// it flips); Part 2 leaves its last field in the substep's history buffer and
+// updates S1 and the melt state in place.
+struct L3Walk {
+    const float* T3in;
+    const uint8_t* S13in;
+    const uint8_t* S23in;
+};
+struct L3StepBufs {
+    const float* Tin;
+    const uint8_t* S1in;
+    uint8_t* S1out;
+    float* Tout;
+    const uint8_t* S2in;
+    uint8_t* S2out;
+};
+
+static L3Walk l3_walk_begin(const SubcycleState& st) { return {st.T3cur, st.S13cur, st.S23cur}; }
+
+static L3StepBufs l3_walk_step(gm_engine* e, SubcycleState& st, L3Walk& w, int part, int k,
+                               int j) {
+    L3StepBufs b;
+    b.Tin = w.T3in;
+    b.S1in = w.S13in;
+    if (part == 1) {
+        b.Tout = e->T3b[st.b3];
+        st.b3 ^= 1;
+        b.S1out = e->S13b[0];
+        b.S2in = nullptr;
+        b.S2out = nullptr;
+    } else {
+        b.Tout = (j == e->N3 - 1) ? e->L3all[k] : e->T3b[st.b3];
+        if (j != e->N3 - 1) st.b3 ^= 1;
+        b.S1out = e->S13;
+        b.S2in = w.S23in;
+        b.S2out = e->S23;
+    }
+    w.T3in = b.Tout;
+    w.S13in = b.S1out;
+    if (part == 2) w.S23in = b.S2out;
+    return b;
+}
+
 // L3 stage of substep k: the N3 L3 solves (computeSolutions_L3 cF:3015-3054
 // with the time-blended faces), then T'.
 static void stage_l3_plain(gm_engine* e, SubcycleState& st, int part, int k) {
     const int N3 = e->N3;
     const float fN3 = (float)N3;
-    const float* T3in = st.T3cur;
-    const uint8_t* S13in = st.S13cur;
-    const uint8_t* S23in = st.S23cur;
+    L3Walk w = l3_walk_begin(st);
     for (int j = 0; j < N3; ++j) {
-        if (part == 1) {
-            float* T3o = e->T3b[st.b3];
-            // L3 S1 of Part 1 in a work buffer: the first step writes all of
-            // it, later steps update it in place (written where it flips)
-            uint8_t* S13o = e->S13b[0];
-            st.b3 ^= 1;
-            run_l3_kc(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, nullptr,
-                      nullptr);
-            T3in = T3o;
-            S13in = S13o;
-        } else {
-            float* T3o = (j == N3 - 1) ? e->L3all[k] : e->T3b[st.b3];
-            if (j != N3 - 1) st.b3 ^= 1;
-            uint8_t* S13o = e->S13;  // S1 and the melt state of Part 2 updated in place
-            uint8_t* S23o = e->S23;
-            run_l3_kc(e, T3in, S13in, S13o, k * N3 + j, (float)(j + 1) / fN3, T3o, S23in, S23o);
-            T3in = T3o;
-            S13in = S13o;
-            S23in = S23o;
-        }
+        const L3StepBufs b = l3_walk_step(e, st, w, part, k, j);
+        run_l3_kc(e, b.Tin, b.S1in, b.S1out, k * N3 + j, (float)(j + 1) / fN3, b.Tout, b.S2in,
+                  b.S2out);
     }
-    stage_l3_close(e, st, part, k, S13in, S23in, st.T3out);
+    stage_l3_close(e, st, part, k, w.S13in, w.S23in, st.T3out);
 }
 
 // Phase flags and melt time of one jump of the L3 window over the N3 steps of
@@ -1785,10 +1818,9 @@ extern "C" int gm_engine_stage_l2(gm_engine* e, int32_t* part, int32_t* k) {
     return 0;
 }
 
-extern "C" int gm_engine_stage_l3(gm_engine* e, int external) {
-    if (!e->stg_active || e->stg_phase != 1) return fail("gm_engine_stage_l3 out of order");
-    if (external) stage_l3_external(e, e->stg, e->stg_part, e->stg_k);
-    else stage_l3_plain(e, e->stg, e->stg_part, e->stg_k);
+// after an L3 stage: the next substep, the middle after Part 1, the
+// epilogue after Part 2
+static void stage_advance(gm_engine* e) {
     e->stg_phase = 0;
     if (++e->stg_k == e->N2) {
         e->stg_k = 0;
@@ -1799,6 +1831,146 @@ extern "C" int gm_engine_stage_l3(gm_engine* e, int external) {
             stage_epilogue(e, e->stg);
             e->stg_active = 0;
         }
+    }
+}
+
+extern "C" int gm_engine_stage_l3(gm_engine* e, int external) {
+    if (!e->stg_active || e->stg_phase != 1) return fail("gm_engine_stage_l3 out of order");
+    if (external) stage_l3_external(e, e->stg, e->stg_part, e->stg_k);
+    else stage_l3_plain(e, e->stg, e->stg_part, e->stg_k);
+    stage_advance(e);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---- many three-level engines with their L3 solves batched ---------------------
+// The engines share one stream and run their subcycles stage by stage in
+// lockstep; at each L2 substep the N3 L3 solves of all of them go out as N3
+// launches of the batched step (one launch per solve for every engine).
+
+struct gm_multi {
+    int n = 0;
+    hipStream_t s = nullptr;
+    std::vector<gm_engine*> e;
+    DevMat* d_mats = nullptr;
+    BatchItem* d_items = nullptr;
+    BatchItem* h_ring = nullptr;  // pinned, 4 slots of n items
+    hipEvent_t ring_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int ring_slot = 0;
+    int* d_bstart = nullptr;
+    int nblocks = 0;
+    bool rec = false;
+    std::vector<BatchItem> items;
+};
+
+extern "C" int gm_multi_destroy(gm_multi* m) {
+    if (!m) return 0;
+    if (m->s) (void)hipStreamSynchronize(m->s);
+    if (m->d_mats) (void)hipFree(m->d_mats);
+    if (m->d_items) (void)hipFree(m->d_items);
+    if (m->d_bstart) (void)hipFree(m->d_bstart);
+    if (m->h_ring) (void)hipHostFree(m->h_ring);
+    for (auto& ev : m->ring_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    delete m;
+    return 0;
+}
+
+extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out) {
+    if (n < 1 || !engines || !out) return fail("gm_multi_create: bad arguments");
+    gm_multi* m = new gm_multi();
+    auto bad = [&](int rc) {
+        gm_multi_destroy(m);
+        return rc;
+    };
+    m->n = n;
+    m->e.assign(engines, engines + n);
+    gm_engine* e0 = m->e[0];
+    m->s = e0->s;
+    m->rec = e0->dm.pnb > 0;
+    std::vector<int> st(n + 1, 0);
+    std::vector<DevMat> hm(n);
+    for (int i = 0; i < n; ++i) {
+        gm_engine* e = m->e[i];
+        if (!e) return bad(fail("gm_multi_create: null engine"));
+        if (e->s != m->s) return bad(fail("gm_multi_create: the engines must share one stream"));
+        if (e->N2 != e0->N2 || e->N3 != e0->N3)
+            return bad(fail("gm_multi_create: the engines need the same N2 and N3"));
+        if ((e->dm.pnb > 0) != m->rec)
+            return bad(fail("gm_multi_create: all engines need the same kind of property tables"));
+        const gm_grid& g = e->cfg.level[3];
+        if ((int64_t)g.n[0] * g.n[1] * g.n[2] * 8 >= (int64_t)1 << 31)
+            return bad(fail("gm_multi_create: L3 meshes must stay below 2^28 nodes"));
+        hm[i] = e->dm;
+        st[i + 1] = st[i] + batch_bricks(g, 1, 3);
+    }
+    m->nblocks = st[n];
+    m->items.resize(n);
+#define MCHK(call)                                                                      \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return bad(fail(std::string(#call) + ": " + hipGetErrorString(e_)));        \
+    } while (0)
+    MCHK(hipMalloc(&m->d_mats, sizeof(DevMat) * n));
+    MCHK(hipMalloc(&m->d_items, sizeof(BatchItem) * n));
+    MCHK(hipMalloc(&m->d_bstart, sizeof(int) * (n + 1)));
+    MCHK(hipMemcpy(m->d_mats, hm.data(), sizeof(DevMat) * n, hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(m->d_bstart, st.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    MCHK(hipHostMalloc((void**)&m->h_ring, sizeof(BatchItem) * n * 4, hipHostMallocDefault));
+    for (auto& ev : m->ring_ev) MCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+#undef MCHK
+    *out = m;
+    return 0;
+}
+
+extern "C" int gm_multi_l3(gm_multi* m) {
+    if (!m) return fail("gm_multi_l3: null");
+    gm_engine* e0 = m->e[0];
+    const int part = e0->stg_part, k = e0->stg_k;
+    for (gm_engine* e : m->e)
+        if (!e->stg_active || e->stg_phase != 1 || e->stg_part != part || e->stg_k != k)
+            return fail("gm_multi_l3: every engine must be between the l2 and l3 stages of "
+                        "the same substep");
+    const int N3 = e0->N3;
+    const float fN3 = (float)N3;
+    std::vector<L3Walk> w(m->n);
+    for (int i = 0; i < m->n; ++i) w[i] = l3_walk_begin(m->e[i]->stg);
+    for (int j = 0; j < N3; ++j) {
+        const int row = k * N3 + j;
+        for (int i = 0; i < m->n; ++i) {
+            gm_engine* e = m->e[i];
+            const L3StepBufs b = l3_walk_step(e, e->stg, w[i], part, k, j);
+            BatchItem& it = m->items[i];
+            it.a = l3_args_kc(e, b.Tin, b.S1in, b.S1out, row, (float)(j + 1) / fN3, b.Tout, b.S2in,
+                              b.S2out);
+            it.a.grid = e->cfg.level[3];  // node counts (a move changes the origin only)
+            it.m = e->d_dm;
+            for (int c = 0; c < 7; ++c) it.row[c] = e->host.rows[row][c];
+            // the row's dt by value: the launch's desc is the first engine's
+            it.a.dt_mode = GM_DT_VALUE;
+            it.a.dt = it.row[5];
+            it.w3 = e->w3[3];
+        }
+        const int slot = m->ring_slot;
+        m->ring_slot = (slot + 1) & 3;
+        HIPCHK(hipEventSynchronize(m->ring_ev[slot]));
+        BatchItem* h = m->h_ring + (size_t)slot * m->n;
+        memcpy(h, m->items.data(), sizeof(BatchItem) * m->n);
+        HIPCHK(hipMemcpyAsync(m->d_items, h, sizeof(BatchItem) * m->n, hipMemcpyHostToDevice,
+                              m->s));
+        HIPCHK(hipEventRecord(m->ring_ev[slot], m->s));
+        if (part == 1)
+            launch_batch_step(m->d_items, m->d_mats, m->d_bstart, m->n, m->nblocks, 1, m->rec,
+                              e0->d_desc, m->s);
+        else
+            launch_batch_step_acc(m->d_items, m->d_mats, m->d_bstart, m->n, m->nblocks, m->rec,
+                                  e0->d_desc, m->s);
+    }
+    for (int i = 0; i < m->n; ++i) {
+        gm_engine* e = m->e[i];
+        stage_l3_close(e, e->stg, part, k, w[i].S13in, w[i].S23in, e->stg.T3out);
+        stage_advance(e);
     }
     HIPCHK(hipGetLastError());
     return 0;

@@ -257,6 +257,8 @@ int batch_bricks(const gm_grid& g, int yb, int zc);
 void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s);
 void launch_batch_step(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
                        int nblocks, int tile, bool rec, const GmStepDesc* d, hipStream_t s);
+void launch_batch_step_acc(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                           int nblocks, bool rec, const GmStepDesc* d, hipStream_t s);
 void launch_step_tile(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
                       int tile, bool rec, hipStream_t s);
 DevMat make_devmat(const gm_material& m);

@@ -713,38 +713,67 @@ template <bool SRC, bool ACC, bool REC, int YB, int ZC>
 __global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __restrict__ d,
                                                  const DevMat* __restrict__ m) {
 #define GM_STEP_BRICK xcd_tile(blockIdx.x, gridDim.x)
+#define GM_STEP_ROW d->rows[a.src_row < 0 ? 0 : a.src_row]
 #include "gm_step_dpp_body.inc"
+#undef GM_STEP_ROW
 #undef GM_STEP_BRICK
 }
 
-// Batched L3-type steps (source, surface flux, Dirichlet faces, no correction,
-// no melt time): many instances of their own mesh size and laser in one
-// launch.  bstart[i] is the first block of instance i (bstart[n] = total);
-// mats[i] is its material (the same as it[i].m);
-// the blocks are dealt out XCD-aware as in the single launch, then each finds
-// its instance (uniform binary search) and runs that instance's brick.
+// Batched L3-type steps (source, surface flux, Dirichlet faces, no
+// correction): many instances of their own mesh size and laser in one launch.
+// bstart[i] is the first block of instance i (bstart[n] = total); mats[i] is
+// its material (the same as it[i].m).  The blocks are dealt out XCD-aware as
+// in the single launch, then each finds its instance (uniform binary search)
+// and runs that instance's brick.  Items carry their time step by value
+// (dt_mode GM_DT_VALUE); the desc's rows are read only under GM_DT_ROW.
+// Both kernels include the body directly (no shared device function), so
+// each compiles to the same instruction DAG as the single launch of the same
+// template arguments and an instance's result equals its own launch bitwise.
+#define GM_BATCH_FIND_INSTANCE                                           \
+    const int b = xcd_tile(blockIdx.x, gridDim.x);                       \
+    int lo = 0, hi = n - 1;                                              \
+    while (lo < hi) {                                                    \
+        const int mid = (lo + hi + 1) >> 1;                              \
+        if (bstart[mid] <= b) lo = mid;                                  \
+        else hi = mid - 1;                                               \
+    }                                                                    \
+    const LSArgs a = it[lo].a;                                           \
+    /* materials indexed off the kernel argument (a known global pointer: \
+       scalar loads), not through the pointer stored in the item */      \
+    const DevMat* __restrict__ m = mats + lo;                            \
+    const int tb = b - bstart[lo];
+
 template <bool REC, int YB, int ZC>
 __global__ void __launch_bounds__(64) k_step_dpp_batch(const BatchItem* __restrict__ it,
                                                        const DevMat* __restrict__ mats,
                                                        const int* __restrict__ bstart, int n,
                                                        const GmStepDesc* __restrict__ d) {
-    const int b = xcd_tile(blockIdx.x, gridDim.x);
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (bstart[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
     constexpr bool SRC = true, ACC = false;
-    const LSArgs a = it[lo].a;
-    // materials indexed off the kernel argument (a known global pointer:
-    // scalar loads), not through the pointer stored in the item
-    const DevMat* __restrict__ m = mats + lo;
-    const int tb = b - bstart[lo];
+    GM_BATCH_FIND_INSTANCE
 #define GM_STEP_BRICK tb
+#define GM_STEP_ROW d->rows[a.src_row < 0 ? 0 : a.src_row]
 #include "gm_step_dpp_body.inc"
+#undef GM_STEP_ROW
 #undef GM_STEP_BRICK
 }
+
+// The same with melt time (Part 2 of the subcycle), bricks of 1 row x 3
+// planes: the melt time adds the item's own toolpath row dt.
+template <bool REC>
+__global__ void __launch_bounds__(64) k_step_dpp_batch_acc(const BatchItem* __restrict__ it,
+                                                           const DevMat* __restrict__ mats,
+                                                           const int* __restrict__ bstart, int n,
+                                                           const GmStepDesc* __restrict__ d) {
+    constexpr bool SRC = true, ACC = true;
+    constexpr int YB = 1, ZC = 3;
+    GM_BATCH_FIND_INSTANCE
+#define GM_STEP_BRICK tb
+#define GM_STEP_ROW it[lo].row
+#include "gm_step_dpp_body.inc"
+#undef GM_STEP_ROW
+#undef GM_STEP_BRICK
+}
+#undef GM_BATCH_FIND_INSTANCE
 
 // L1 step (computeL1Temperature, cF:2135-2204 with the L1 faces of
 // cF:1568-1597) in the same wave-brick form: properties from (T, S1) of every
@@ -1049,6 +1078,17 @@ static void batch_step_t(const BatchItem* it, const DevMat* mats, const int* bst
     else
         hipLaunchKernelGGL((k_step_dpp_batch<false, YB, ZC>), dim3(nblocks), dim3(64), 0, s, it,
                            mats, bstart, n, d);
+}
+
+void launch_batch_step_acc(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                           int nblocks, bool rec, const GmStepDesc* d, hipStream_t s) {
+    if (n <= 0 || nblocks <= 0) return;
+    if (rec)
+        hipLaunchKernelGGL(k_step_dpp_batch_acc<true>, dim3(nblocks), dim3(64), 0, s, it, mats,
+                           bstart, n, d);
+    else
+        hipLaunchKernelGGL(k_step_dpp_batch_acc<false>, dim3(nblocks), dim3(64), 0, s, it, mats,
+                           bstart, n, d);
 }
 
 void launch_batch_step(const BatchItem* it, const DevMat* mats, const int* bstart, int n,

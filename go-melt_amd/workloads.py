@@ -301,3 +301,44 @@ def part_scale(lines=30, layers=5, line_mm=10.0, hatch=0.1, P=250.0):
     g = "".join(raster_gcode(x0, x0 + line_mm, y0, hatch, lines, z=0.05 * k)
                 for k in range(layers))
     return inp, g
+
+
+def stack_decks(n=64, seed=0, h=0.0125, dt=1e-6, sizes=None, track_frac=0.5):
+    """The batched configuration as three-level stacks: n single-track runs,
+    each with its own L1/L2/L3 meshes of the L3 element counts drawn from
+    BATCH_SIZES (the same counts on every level), L2 and L1 spanning 4x and
+    16x the L3 extent (spacings h, 4h, 16h), P uniform 100-400 W, speed
+    uniform 0.5-2 m/s and laser radius uniform 30-70 um, from a fixed seed
+    (the same draws as batch_instances).  The track runs along x through the
+    middle of L1 for `track_frac` of its length.  Returns a list of
+    (solver_input, gcode)."""
+    out = []
+    for b in batch_instances(n=n, seed=seed, h=h, dt=dt, sizes=sizes):
+        ex, ey, ez = b["elements"]
+        h1, h2 = 16 * h, 4 * h
+        ext1 = [ex * h1, ey * h1, ez * h1]
+        # laser start a quarter of the way along x, centred in y, on L1 nodes
+        x0 = round(0.25 * ex) * h1
+        y0 = round(0.5 * ey) * h1
+        # L2 and L3 centred on the laser, origins on the next coarser lattice
+        x2 = x0 - round(0.5 * ex) * h2
+        y2 = y0 - round(0.5 * ey) * h2
+        x3 = x0 - round(0.5 * ex) * h
+        y3 = y0 - round(0.5 * ey) * h
+        x0, y0, x2, y2, x3, y3 = (round(v, 9) for v in (x0, y0, x2, y2, x3, y3))
+        ext1 = [round(v, 9) for v in ext1]
+        props = dict(b["properties"])
+        props["laser_center"] = [x0, y0, 0.0, 0, 0, 0, 0]
+        r9 = lambda v: round(v, 9)  # noqa: E731
+        inp = {
+            "Level1": _level((ex, ey, ez), (0, ext1[0]), (0, ext1[1]), (-ext1[2], 0), 300.0),
+            "Level2": _level((ex, ey, ez), (x2, r9(x2 + ex * h2)), (y2, r9(y2 + ey * h2)),
+                             (r9(-ez * h2), 0)),
+            "Level3": _level((ex, ey, ez), (x3, r9(x3 + ex * h)), (y3, r9(y3 + ey * h)),
+                             (r9(-ez * h), 0)),
+            "properties": props,
+            "nonmesh": _nonmesh(laser_velocity=b["speed"], timestep_L3=dt),
+        }
+        g = f"G0 X{x0:.6f} Y{y0:.6f} Z0\nG1 X{x0 + track_frac * ext1[0]:.6f} Y{y0:.6f} Z0\n"
+        out.append((inp, g))
+    return out

@@ -310,6 +310,22 @@ int gm_engine_stage_l2(gm_engine* e, int32_t* part_host, int32_t* k_host);
 int gm_engine_stage_l3(gm_engine* e, int external);
 int gm_engine_stage_field(gm_engine* e, int which, void** ptr, int64_t* count);
 
+/* Many independent three-level runs advanced together (the batched
+ * configuration: each instance its own L1/L2/L3 stack, subcycleGOMELT
+ * cF:3224-3632 per instance).  The engines share one stream and run their
+ * subcycles stage by stage in lockstep (gm_engine_stage_begin and
+ * gm_engine_stage_l2 on each); gm_multi_l3 then runs the N3 L3 solves of the
+ * current L2 substep of every engine as N3 launches of the batched step (all
+ * engines' bricks in one launch per solve, melt time in Part 2) and finishes
+ * each engine's L3 stage as gm_engine_stage_l3(e, 0) would.  Every engine
+ * must be between the l2 and l3 stages of the same (part, k), with equal N2,
+ * N3 and the same kind of property tables.  No reference counterpart (the
+ * reference runs one instance per process). */
+typedef struct gm_multi gm_multi;
+int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out);
+int gm_multi_l3(gm_multi* m);
+int gm_multi_destroy(gm_multi* m);
+
 /* stepGOMELT (cF:2304-2397) for one toolpath row.  Writes the reference's
  * return value _resetmask (cF:2394-2397: L3 nodes whose S2 went 0 -> 1) into
  * the GM_RESET_L3 field.  In the engine's own melt-time mode (the default,
