@@ -146,6 +146,7 @@ _SIGS = {
     "gm_batch_time": (_I, [_P, _I, _I, _FP]),
     "gm_batch_field": (_I, [_P, _I, _I, C.POINTER(_P), C.POINTER(_I64)]),
     "gm_batch_refresh": (_I, [_P]),
+    "gm_batch_jump_state": (_I, [_P, _I, _P]),
     "gm_conv3d_fwd": (_I, [_IP, _P, _P, _P, _I, _P, _P, _P]),
     "gm_conv3d_bwd_data": (_I, [_IP, _P, _P, _P, _P, _P, _P]),
     "gm_conv3d_bwd_weight": (_I, [_IP, _P, _P, _P, _P]),

@@ -2551,6 +2551,17 @@ extern "C" int gm_batch_refresh(gm_batch* b) {
     return 0;
 }
 
+extern "C" int gm_batch_jump_state(gm_batch* b, int i, const float* T0) {
+    if (!b || !T0) return fail("gm_batch_jump_state: null argument");
+    if (i < 0 || i >= b->n) return fail("gm_batch_jump_state: no such instance");
+    const gm_grid& g = b->g[i];
+    const int64_t nn = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    launch_jump_state(b->d_desc, b->dm[i], 0, T0, b->T[b->cur][i], b->S1[i], b->S1[i], 0,
+                      nullptr, nullptr, nullptr, nn, b->s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Learned L3 operator: 3x3x3 BF16 MFMA convolutions, Adam
 // ---------------------------------------------------------------------------

@@ -632,6 +632,8 @@ def run_in_loop(net, instances, n_jumps, K=10, warm=20, compare_every=2, residua
             Tn.append(t)
             d = sur._field(i, 0)
             N.check(N.lib().gm_memcpy_d2d(C.c_void_p(d.ptr), _ptr(t), d.nbytes, _stream()))
+            # nodes the jump took over the liquidus keep their melted flag
+            N.check(N.lib().gm_batch_jump_state(sur.h, i, _ptr(T0[i])))
         N.check(N.lib().gm_batch_refresh(sur.h))
         e[1].record()
         jrows = [W.batch_rows(instances, s + k) for k in range(K)]

@@ -440,6 +440,13 @@ int gm_batch_time(gm_batch* b, int batched, int iters, float* ms_per_step);
 int gm_batch_field(gm_batch* b, int i, int field, void** ptr, int64_t* count);
 int gm_batch_refresh(gm_batch* b);
 
+/* After a caller has replaced instance i's T by its own update of a start
+ * field T0 (an operator jump over several steps): S1 |= (T0 >= T_liquidus) |
+ * (T >= T_liquidus), so a node that melted inside the jump stays melted, as
+ * the per-step phase rule of the plain step keeps it (cF:2567-2614).  Call
+ * gm_batch_refresh after it. */
+int gm_batch_jump_state(gm_batch* b, int i, const float* T0);
+
 /* ------------------------------------------------------------------------ */
 /* Learned L3 operator: convolution kernels                                  */
 /* ------------------------------------------------------------------------ */

@@ -129,3 +129,39 @@ def test_batch_instance_vs_oracle(B, oracle, table):
         rise = float(np.max(T) - p["T_amb"])
         assert rise > 100.0
         assert float(np.max(np.abs(Tg - T))) / rise < 1e-4, (i, float(np.max(np.abs(Tg - T))) / rise)
+
+
+def test_jump_state_keeps_melted_nodes(B):
+    """A caller's multi-step update written over an instance's T (an operator
+    jump) keeps the melted flag of every node that is above T_liquidus at the
+    jump's start or end (gm_batch_jump_state): where the plain steps over the
+    same span have flagged a node and the field is above T_liquidus, the
+    jumped state has it flagged too; nowhere else does the flag change."""
+    import _native as N
+    inst = _instances(3, seed=4, sizes=[(40, 36, 20)])
+    a = B.Batch.from_instances(inst)
+    b = B.Batch.from_instances(inst)
+    for s in range(15):
+        rows = W.batch_rows(inst, s)
+        a.step(rows)
+        b.step(rows)
+    T0 = [a.T(i) for i in range(3)]
+    S0 = [a.S1(i) for i in range(3)]
+    for s in range(15, 25):
+        b.step(W.batch_rows(inst, s))
+    grew = 0
+    for i in range(3):
+        T1 = b.T(i)
+        a.set_fields(i, T=T1)
+        dev = N.DeviceArray.from_numpy(T0[i])
+        N.check(N.lib().gm_batch_jump_state(a.h, i, dev.ptr))
+        N.check(N.lib().gm_batch_refresh(a.h))
+        S = a.S1(i)
+        Tl = float(inst[i]["properties"]["T_liquidus"])
+        want = (S0[i] != 0) | (T0[i] >= Tl) | (T1 >= Tl)
+        assert np.array_equal(S != 0, want), i
+        both = (b.S1(i) != 0) & (T1 >= Tl)
+        assert np.all(S[both] != 0), i
+        grew += int(np.count_nonzero(want & (S0[i] == 0)))
+        del dev
+    assert grew > 0  # the span melts nodes that were powder at its start
