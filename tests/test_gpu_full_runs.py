@@ -133,6 +133,18 @@ def test_single_track_full_run(tmp_path):
     assert run.stats["chunks"].count("single") == 1 and run.stats["subcycle"] == 49
     _check("single_track", run, deck, f32, f64, (1, 2, 3))
     _check_accum(run, f64, 1e-6)
+    # physics: the deck's properties and laser are Rosenthal's (constant k,
+    # rho c, eta P = 70 W, 1 m/s); a Gaussian source of 50 um radius on a
+    # finite body is not a point source on a half-space, so only a loose
+    # agreement of the final pool with Rosenthal's is asserted (measured
+    # 0.05% on depth and width)
+    import computeFunctions as cf
+    from rosenthal import rosenthal_pool
+    _, depth, width, _ = rosenthal_pool()
+    L3 = run.Levels[3]
+    pg = cf.meltPoolDimensions(L3, L3["T0"], float(deck["properties"]["T_liquidus"]))
+    print(f"single_track: melt pool depth, width {pg[:2]} Rosenthal {depth:.6f}, {width:.6f}")
+    assert abs(pg[0] / depth - 1.0) < 0.05 and abs(pg[1] / width - 1.0) < 0.05
 
 
 def test_production_window(tmp_path):

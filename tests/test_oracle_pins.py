@@ -513,6 +513,26 @@ def test_melt_pool_tent_width_and_depth_closed_form(oracle):
     assert d == pytest.approx(300.0 / 6000.0, abs=1e-12)
 
 
+def test_melt_pool_rosenthal_field(oracle):
+    """The measure on a realistic pool: Rosenthal's field sampled on a 5 um
+    grid (source a third of a spacing off the nodes), against the isotherm's
+    depth, width and length from root finding.  The linear interpolation
+    along grid lines is second-order accurate: depth and width agree to
+    1e-4 of the pool size here (measured 5e-5), the length to 1e-3 (4.6e-4;
+    its ends are where T varies fastest along x)."""
+    from rosenthal import ROSENTHAL, rosenthal_pool
+    T, depth, width, length = rosenthal_pool()
+    h = 0.005
+    x = np.arange(-0.5, 0.05 + 1e-12, h) + h / 3
+    y = np.arange(-0.15, 0.15 + 1e-12, h)
+    z = np.arange(-0.15, 1e-12, h)
+    F = np.minimum(_field((x, y, z), lambda X, Y, Z: T(X, np.sqrt(Y * Y + Z * Z))), 1e6)
+    d, w, l = oracle.melt_pool_dims(F, (x, y, z), ROSENTHAL["Tl"])[:3]
+    assert d == pytest.approx(depth, rel=1e-4)
+    assert w == pytest.approx(width, rel=1e-4)
+    assert l == pytest.approx(length, rel=1e-3)
+
+
 def test_melt_pool_special_cases(oracle):
     ax = _pool_mesh()
     ztop = ax[2][-1]
