@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-stacks", action="store_true",
                     help="skip the batched three-level stacks measurement")
     ap.add_argument("--stacks-chunks", type=int, default=3)
+    ap.add_argument("--stacks-streams", type=int, default=4)
     ap.add_argument("--stacks-only", action="store_true",
                     help="print only the stacks measurement (one JSON line)")
     ap.add_argument("--no-surrogate", action="store_true",
@@ -213,13 +214,16 @@ def batch_measure(n, iters=50):
     }
 
 
-def stacks_measure(n, chunks=3, seed=0):
+def stacks_measure(n, chunks=3, seed=0, streams=4):
     """The batched configuration as three-level stacks: n single-track runs,
     each its own L1/L2/L3 stack (element counts drawn from the batch sizes),
     advanced in lockstep with the L3 solves of all runs in one launch per
-    solve; against the same runs advanced one after another through their own
-    recorded subcycles.  Both advance every run by `chunks` subcycled L1
-    steps after two warm-up chunks."""
+    solve, the runs dealt over `streams` streams; against the same runs on
+    one stream, one after another through their own recorded subcycles.
+    Each pass builds its stacks, runs the single-step chunk and one
+    subcycled warm-up chunk, then times `chunks` subcycled L1 steps of every
+    run."""
+    import gc
     import numpy as np
     import workloads as W
     import batch as B
@@ -228,38 +232,39 @@ def stacks_measure(n, chunks=3, seed=0):
         for i, (inp, g) in enumerate(W.stack_decks(n=n, seed=seed)):
             with open(W.write_deck(inp, g, os.path.join(tmp, f"s{i}"))) as fh:
                 decks.append(json.load(fh))
-        t0 = time.perf_counter()
-        st = B.Stacks(decks)
-        build_s = time.perf_counter() - t0
-        st.step()       # the single-step chunk
-        st.step_each()  # records every run's subcycle graph
-        st.sync()
-        mm_per_chunk = 0.0
-        l3_nodes = 0
-        for r in st.runs:
-            nm = r.Nonmesh
-            rows = r.subcycle[2]
-            mm_per_chunk += float(nm["laser_velocity"]) * float(nm["timestep_L3"]) * rows
-            l3_nodes += int(np.prod([len(v) for v in r.Levels[3]["node_coords"]]))
-        ms = {}
-        for mode in ("batched", "sequential"):  # every engine is on the Stacks' stream
+        ms, info = {}, {}
+        for mode, ns in (("batched", streams), ("sequential", 1)):
+            t0 = time.perf_counter()
+            st = B.Stacks(decks, streams=ns)
+            info["setup_s"] = time.perf_counter() - t0
+            adv = st.step if mode == "batched" else st.step_each
+            st.step()  # the single-step chunk
+            adv()      # warm-up (records the subcycle graphs of the sequential pass)
+            st.sync()
             t0 = time.perf_counter()
             for _ in range(chunks):
-                (st.step if mode == "batched" else st.step_each)()
+                adv()
             st.sync()
             ms[mode] = (time.perf_counter() - t0) * 1e3 / chunks
-        n23 = st.runs[0].subcycle[2]
+            info["mm"] = sum(float(r.Nonmesh["laser_velocity"]) * float(r.Nonmesh["timestep_L3"])
+                             * r.subcycle[2] for r in st.runs)
+            info["l3_nodes"] = sum(int(np.prod([len(v) for v in r.Levels[3]["node_coords"]]))
+                                   for r in st.runs)
+            info["n23"] = st.runs[0].subcycle[2]
+            del st, adv
+            gc.collect()
     return {
         "workload": f"{n} single-track three-level stacks, L3 element counts drawn (seed "
                     f"{seed}) from {W.BATCH_SIZES}, L2 and L1 4x and 16x the L3 extent, "
                     f"P 100-400 W, 0.5-2 m/s, r 30-70 um",
-        "setup_s": build_s,
+        "streams": streams,
+        "setup_s": info["setup_s"],
         "ms_per_l1_step_batched": ms["batched"],
         "ms_per_l1_step_sequential": ms["sequential"],
         "speedup": ms["sequential"] / ms["batched"],
-        "scan_mm_per_s_batched": mm_per_chunk / (ms["batched"] * 1e-3),
-        "scan_mm_per_s_sequential": mm_per_chunk / (ms["sequential"] * 1e-3),
-        "l3_node_updates_per_s_batched": l3_nodes * n23 / (ms["batched"] * 1e-3),
+        "scan_mm_per_s_batched": info["mm"] / (ms["batched"] * 1e-3),
+        "scan_mm_per_s_sequential": info["mm"] / (ms["sequential"] * 1e-3),
+        "l3_node_updates_per_s_batched": info["l3_nodes"] * info["n23"] / (ms["batched"] * 1e-3),
     }
 
 
@@ -337,7 +342,8 @@ def main():
                                            a.c5_steps)))
         return
     if a.stacks_only:
-        print(json.dumps(stacks_measure(a.batch_n, chunks=a.stacks_chunks)))
+        print(json.dumps(stacks_measure(a.batch_n, chunks=a.stacks_chunks,
+                                        streams=a.stacks_streams)))
         return
     import numpy as np
     import workloads as W
@@ -397,6 +403,10 @@ def main():
         for line in open(TRAFFIC_FILE):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1])
+    def ext(k):
+        b = inp[f"Level{k}"]["bounds"]
+        return "x".join(f"{b[ax][1] - b[ax][0]:g}" for ax in ("x", "y", "z"))
+
     result = {
         "metric": METRIC,
         "value": mm / wall,
@@ -411,9 +421,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic (generated toolpath and input deck; no dataset)",
         "config": {
-            "workload": f"{a.deck} deck: L1 {L[1]['elements']}, L2 {L[2]['elements']}, "
-                        f"L3 {L[3]['elements']} elements, N2={N2}, N3={N3}, "
-                        f"{a.steps * N2 * N3} L3 steps timed",
+            "workload": f"{a.deck} deck: L1 {L[1]['elements']} elements over {ext(1)} mm, "
+                        f"L2 {L[2]['elements']} over {ext(2)} mm, L3 {L[3]['elements']} over "
+                        f"{ext(3)} mm, N2={N2}, N3={N3}, {a.steps * N2 * N3} L3 steps timed"
+                        + ("; L2 spans 5x5x2.5 mm instead of 4x4x2 (an L1/L2 spacing ratio of "
+                           "2.5 is not representable in the reference's window bookkeeping)"
+                           if a.deck == "production" else ""),
             "scan_mm": mm,
             "element_updates_per_s": elem_updates / wall,
             "subcycled_steps": n_sub,
@@ -463,7 +476,8 @@ def main():
             result["batch"] = {"error": repr(exc)}
     if not a.no_stacks:
         try:
-            result["stacks"] = stacks_measure(a.batch_n, chunks=a.stacks_chunks)
+            result["stacks"] = stacks_measure(a.batch_n, chunks=a.stacks_chunks,
+                                              streams=a.stacks_streams)
         except Exception as exc:  # reported, not fatal to the headline line
             result["stacks"] = {"error": repr(exc)}
     print(json.dumps(result))

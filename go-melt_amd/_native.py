@@ -104,6 +104,7 @@ _SIGS = {
     "gm_event_create": (_I, [C.POINTER(_P)]),
     "gm_event_destroy": (_I, [_P]),
     "gm_event_record": (_I, [_P, _P]),
+    "gm_stream_wait_event": (_I, [_P, _P]),
     "gm_event_elapsed_ms": (_I, [_P, _P, _FP]),
     "gm_state_properties": (_I, [_M, _P, _P, _I64, _I64, _P, _P, _P, _P, _P]),
     "gm_solve_matrix_free": (_I, [_G, _I64, _P, _P, _F, _P, _P, _P, _P, _P]),
@@ -171,6 +172,7 @@ _SIGS = {
     "gm_engine_stage_field": (_I, [_P, _I, _P, _P]),
     "gm_multi_create": (_I, [_P, _I, _P]),
     "gm_multi_l3": (_I, [_P]),
+    "gm_multi_subcycle": (_I, [_P]),
     "gm_multi_destroy": (_I, [_P]),
 }
 
@@ -405,6 +407,11 @@ class Event:
 
     def record(self, stream):
         check(lib().gm_event_record(self.ptr, stream.ptr if isinstance(stream, Stream) else stream))
+
+    def wait_on(self, stream):
+        """Later work on ``stream`` waits for this event."""
+        check(lib().gm_stream_wait_event(stream.ptr if isinstance(stream, Stream) else stream,
+                                         self.ptr))
 
     def elapsed_ms(self, later):
         ms = C.c_float()

@@ -1,7 +1,11 @@
-"""Batched process-parameter instances: one kernel launch per step for many
-independent L3 windows of their own mesh size and laser (gm_batch_* in
-include/gomelt.h).  There is no CPU fallback: without libgomelt.so or a GPU
-every call raises."""
+"""Batched process-parameter instances.
+
+* Batch: one kernel launch per step for many independent L3 windows of their
+  own mesh size and laser (gm_batch_* in include/gomelt.h).
+* Stacks: many independent three-level runs advanced in lockstep, the L3
+  solves of all of them one launch per solve (gm_multi_*).
+
+There is no CPU fallback: without libgomelt.so or a GPU every call raises."""
 
 from __future__ import annotations
 
@@ -115,26 +119,31 @@ class Stacks:
     """Independent three-level runs (each its own L1/L2/L3 stack, the
     reference's time loop gm:134-493 per run) advanced together.
 
-    One TimeLoop per deck, all engines on one stream.  Each step gives every
+    One TimeLoop per deck, the engines dealt over `streams` streams.  Each step gives every
     run its next chunk of N2*N3 toolpath rows; a single-step chunk runs as in
     the plain driver, a subcycled one (subcycleGOMELT cF:3224-3632) stage by
     stage in lockstep with the others: per L2 substep the L2 stage of each run,
     then the N3 L3 solves of all subcycling runs as N3 batched launches
-    (gm_multi_l3).  Every run ends where its own unbatched run ends."""
+    (gm_multi_subcycle).  Every run ends where its own unbatched run ends."""
 
-    def __init__(self, decks):
+    def __init__(self, decks, streams=4):
         import computeFunctions as cf
         import go_melt as G
         N.init_device()
-        self.stream = N.Stream()
-        with cf.engines_on_stream(self.stream):
-            self.runs = [G.TimeLoop(d, quiet=True, write_outputs=False) for d in decks]
+        # runs dealt round-robin over a few streams, so the small per-run
+        # kernels (L2 stages, moves, single steps) of different runs overlap
+        self.streams = [N.Stream() for _ in range(max(1, min(int(streams), len(decks))))]
+        self.runs = []
+        for i, d in enumerate(decks):
+            with cf.engines_on_stream(self.streams[i % len(self.streams)]):
+                self.runs.append(G.TimeLoop(d, quiet=True, write_outputs=False))
         self.n = len(self.runs)
         self._pending = []
         for r in self.runs:
             cf.attach_l3_operator(r.Levels, _Deferred(self))
         self._iters = [r.chunks() for r in self.runs]
         self._multi = None
+        self._ev = [N.Event(), N.Event()]
 
     def _multi_of(self, engines):
         key = tuple(e.h for e in engines)
@@ -160,34 +169,38 @@ class Stacks:
             going = r.step_chunk(rows) and going
         pend = self._pending
         if pend:
-            n2 = pend[0].N2
-            multi = self._multi_of(pend)
-            for _ in range(2 * n2):
-                for e in pend:
-                    N.check(lib.gm_engine_stage_l2(e.h, None, None))
-                N.check(lib.gm_multi_l3(multi))
+            N.check(lib.gm_multi_subcycle(self._multi_of(pend)))
         self._pending = []
         return going
 
     def step_each(self):
         """One chunk of every run, one run after another through its own
-        recorded subcycle (no batching): the sequential baseline of step."""
+        recorded subcycle (no batching): the sequential baseline of step.
+        Each run's work waits on the GPU for the previous run's (events
+        across the streams), the host does not wait."""
         import computeFunctions as cf
         going = True
-        for r, it in zip(self.runs, self._iters):
+        prev = None
+        for i, (r, it) in enumerate(zip(self.runs, self._iters)):
             rows = next(it, None)
             if not rows:
                 going = False
                 continue
+            s = r.eng.stream
+            if prev is not None and prev is not s:
+                self._ev[i % 2].wait_on(s)
             cf.attach_l3_operator(r.Levels, None)
             try:
                 going = r.step_chunk(rows) and going
             finally:
                 cf.attach_l3_operator(r.Levels, _Deferred(self))
+            self._ev[(i + 1) % 2].record(s)
+            prev = s
         return going
 
     def sync(self):
-        self.stream.sync()
+        for s in self.streams:
+            s.sync()
         for r in self.runs:
             r.eng.sync_coords()
 

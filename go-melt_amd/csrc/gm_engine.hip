@@ -109,6 +109,10 @@ extern "C" int gm_event_record(void* ev, void* s) {
     HIPCHK(hipEventRecord((hipEvent_t)ev, (hipStream_t)s));
     return 0;
 }
+extern "C" int gm_stream_wait_event(void* s, void* ev) {
+    HIPCHK(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)ev, 0));
+    return 0;
+}
 extern "C" int gm_event_elapsed_ms(void* a, void* b, float* ms) {
     HIPCHK(hipEventSynchronize((hipEvent_t)b));
     HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
@@ -1861,11 +1865,20 @@ struct gm_multi {
     int nblocks = 0;
     bool rec = false;
     std::vector<BatchItem> items;
+    // the engines' other streams: the batched solves on s wait for their
+    // L2 stages, their L3 closes wait for the batched solves
+    std::vector<hipStream_t> xs;
+    std::vector<hipEvent_t> xev;
+    hipEvent_t mev = nullptr;
 };
 
 extern "C" int gm_multi_destroy(gm_multi* m) {
     if (!m) return 0;
     if (m->s) (void)hipStreamSynchronize(m->s);
+    for (hipStream_t x : m->xs) (void)hipStreamSynchronize(x);
+    for (hipEvent_t ev : m->xev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (m->mev) (void)hipEventDestroy(m->mev);
     if (m->d_mats) (void)hipFree(m->d_mats);
     if (m->d_items) (void)hipFree(m->d_items);
     if (m->d_bstart) (void)hipFree(m->d_bstart);
@@ -1893,7 +1906,8 @@ extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out)
     for (int i = 0; i < n; ++i) {
         gm_engine* e = m->e[i];
         if (!e) return bad(fail("gm_multi_create: null engine"));
-        if (e->s != m->s) return bad(fail("gm_multi_create: the engines must share one stream"));
+        if (e->s != m->s && std::find(m->xs.begin(), m->xs.end(), e->s) == m->xs.end())
+            m->xs.push_back(e->s);
         if (e->N2 != e0->N2 || e->N3 != e0->N3)
             return bad(fail("gm_multi_create: the engines need the same N2 and N3"));
         if ((e->dm.pnb > 0) != m->rec)
@@ -1919,6 +1933,9 @@ extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out)
     MCHK(hipMemcpy(m->d_bstart, st.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     MCHK(hipHostMalloc((void**)&m->h_ring, sizeof(BatchItem) * n * 4, hipHostMallocDefault));
     for (auto& ev : m->ring_ev) MCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    m->xev.assign(m->xs.size(), nullptr);
+    for (auto& ev : m->xev) MCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    MCHK(hipEventCreateWithFlags(&m->mev, hipEventDisableTiming));
 #undef MCHK
     *out = m;
     return 0;
@@ -1936,6 +1953,10 @@ extern "C" int gm_multi_l3(gm_multi* m) {
     const float fN3 = (float)N3;
     std::vector<L3Walk> w(m->n);
     for (int i = 0; i < m->n; ++i) w[i] = l3_walk_begin(m->e[i]->stg);
+    for (size_t i = 0; i < m->xs.size(); ++i) {
+        HIPCHK(hipEventRecord(m->xev[i], m->xs[i]));
+        HIPCHK(hipStreamWaitEvent(m->s, m->xev[i], 0));
+    }
     for (int j = 0; j < N3; ++j) {
         const int row = k * N3 + j;
         for (int i = 0; i < m->n; ++i) {
@@ -1967,10 +1988,32 @@ extern "C" int gm_multi_l3(gm_multi* m) {
             launch_batch_step_acc(m->d_items, m->d_mats, m->d_bstart, m->n, m->nblocks, m->rec,
                                   e0->d_desc, m->s);
     }
+    if (!m->xs.empty()) {
+        HIPCHK(hipEventRecord(m->mev, m->s));
+        for (hipStream_t x : m->xs) HIPCHK(hipStreamWaitEvent(x, m->mev, 0));
+    }
     for (int i = 0; i < m->n; ++i) {
         gm_engine* e = m->e[i];
         stage_l3_close(e, e->stg, part, k, w[i].S13in, w[i].S23in, e->stg.T3out);
         stage_advance(e);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_multi_subcycle(gm_multi* m) {
+    if (!m) return fail("gm_multi_subcycle: null");
+    for (gm_engine* e : m->e)
+        if (!e->stg_active || e->stg_phase != 0 || e->stg_part != 1 || e->stg_k != 0)
+            return fail("gm_multi_subcycle: every engine must have just begun a staged subcycle "
+                        "(gm_engine_stage_begin)");
+    const int n2 = m->e[0]->N2;
+    for (int sub = 0; sub < 2 * n2; ++sub) {
+        for (gm_engine* e : m->e) {
+            stage_l2(e, e->stg, e->stg_part, e->stg_k);
+            e->stg_phase = 1;
+        }
+        if (int rc = gm_multi_l3(m)) return rc;
     }
     HIPCHK(hipGetLastError());
     return 0;

@@ -709,13 +709,19 @@ __device__ __forceinline__ float face_flux_hw(const DevMat& m, float Tq) {
     return q * 1e-6f;
 }
 
-template <bool SRC, bool ACC, bool REC, int YB, int ZC>
-__global__ void __launch_bounds__(64) k_step_dpp(LSArgs a, const GmStepDesc* __restrict__ d,
-                                                 const DevMat* __restrict__ m) {
-#define GM_STEP_BRICK xcd_tile(blockIdx.x, gridDim.x)
+// WPB waves per block, each its own brick (block b holds bricks
+// WPB*tile(b) .. WPB*tile(b) + WPB - 1; bricks past the last are empty).
+template <bool SRC, bool ACC, bool REC, int YB, int ZC, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_step_dpp(LSArgs a, const GmStepDesc* __restrict__ d,
+                                                       const DevMat* __restrict__ m) {
+#define GM_STEP_BRICK \
+    (WPB == 1 ? xcd_tile(blockIdx.x, gridDim.x) \
+              : xcd_tile(blockIdx.x, gridDim.x) * WPB + (int)(threadIdx.x >> 6))
+#define GM_STEP_LANE (WPB == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63))
 #define GM_STEP_ROW d->rows[a.src_row < 0 ? 0 : a.src_row]
 #include "gm_step_dpp_body.inc"
 #undef GM_STEP_ROW
+#undef GM_STEP_LANE
 #undef GM_STEP_BRICK
 }
 
@@ -751,9 +757,11 @@ __global__ void __launch_bounds__(64) k_step_dpp_batch(const BatchItem* __restri
     constexpr bool SRC = true, ACC = false;
     GM_BATCH_FIND_INSTANCE
 #define GM_STEP_BRICK tb
+#define GM_STEP_LANE (int)threadIdx.x
 #define GM_STEP_ROW d->rows[a.src_row < 0 ? 0 : a.src_row]
 #include "gm_step_dpp_body.inc"
 #undef GM_STEP_ROW
+#undef GM_STEP_LANE
 #undef GM_STEP_BRICK
 }
 
@@ -768,9 +776,11 @@ __global__ void __launch_bounds__(64) k_step_dpp_batch_acc(const BatchItem* __re
     constexpr int YB = 1, ZC = 3;
     GM_BATCH_FIND_INSTANCE
 #define GM_STEP_BRICK tb
+#define GM_STEP_LANE (int)threadIdx.x
 #define GM_STEP_ROW it[lo].row
 #include "gm_step_dpp_body.inc"
 #undef GM_STEP_ROW
+#undef GM_STEP_LANE
 #undef GM_STEP_BRICK
 }
 #undef GM_BATCH_FIND_INSTANCE
@@ -1006,15 +1016,28 @@ static void launch_step_l1(const LSArgs& a0, const GmStepDesc* d, const gm_grid&
         hipLaunchKernelGGL((k_step_l1<false, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
 }
 
+// waves (bricks) per block of the subcycle steps, measured on the production
+// deck: 4 for L3 steps (6.69 us against 6.87 us with 1, 7.29 with 2, 7.88
+// with 8), 1 for L2 steps (4 made the L2 step slower)
+#ifndef GM_L3_WPB
+#define GM_L3_WPB 4
+#endif
+#ifndef GM_L2_WPB
+#define GM_L2_WPB 1
+#endif
+
 template <bool SRC, bool ACC, bool REC>
 static void launch_dpp(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
                        hipStream_t s) {
     // measured best on 101 x 101 x 51 meshes: 3 planes per brick for L3
     // steps, 2 for L2 steps (whose correction-box reads favour more waves)
     constexpr int YB = 1, ZC = SRC ? 3 : 2;
+    constexpr int WPB = SRC ? GM_L3_WPB : GM_L2_WPB;
     const int nq = hg.n[0] * ((hg.n[1] + YB - 1) / YB);
-    const int nb = ((nq + DXS - 1) / DXS) * ((hg.n[2] + ZC - 1) / ZC);
-    hipLaunchKernelGGL((k_step_dpp<SRC, ACC, REC, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
+    const int nbr = ((nq + DXS - 1) / DXS) * ((hg.n[2] + ZC - 1) / ZC);
+    const int nb = (nbr + WPB - 1) / WPB;
+    hipLaunchKernelGGL((k_step_dpp<SRC, ACC, REC, YB, ZC, WPB>), dim3(nb), dim3(64 * WPB), 0, s,
+                       a, d, m);
 }
 
 void launch_level_step_kc(const LSArgs& a0, const GmStepDesc* d, const gm_grid& hg,
@@ -1109,10 +1132,10 @@ static void step_t(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, cons
                    bool rec, hipStream_t s) {
     const int nb = batch_bricks(hg, YB, ZC);
     if (rec)
-        hipLaunchKernelGGL((k_step_dpp<true, false, true, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d,
+        hipLaunchKernelGGL((k_step_dpp<true, false, true, YB, ZC, 1>), dim3(nb), dim3(64), 0, s, a, d,
                            m);
     else
-        hipLaunchKernelGGL((k_step_dpp<true, false, false, YB, ZC>), dim3(nb), dim3(64), 0, s, a,
+        hipLaunchKernelGGL((k_step_dpp<true, false, false, YB, ZC, 1>), dim3(nb), dim3(64), 0, s, a,
                            d, m);
 }
 

@@ -102,6 +102,7 @@ int gm_stream_sync(void* stream);
 int gm_event_create(void** ev);
 int gm_event_destroy(void* ev);
 int gm_event_record(void* ev, void* stream);
+int gm_stream_wait_event(void* stream, void* ev); /* later work on stream waits for ev */
 int gm_event_elapsed_ms(void* start, void* stop, float* ms);
 
 /* ------------------------------------------------------------------------ */
@@ -312,8 +313,10 @@ int gm_engine_stage_field(gm_engine* e, int which, void** ptr, int64_t* count);
 
 /* Many independent three-level runs advanced together (the batched
  * configuration: each instance its own L1/L2/L3 stack, subcycleGOMELT
- * cF:3224-3632 per instance).  The engines share one stream and run their
- * subcycles stage by stage in lockstep (gm_engine_stage_begin and
+ * cF:3224-3632 per instance).  The engines run their subcycles stage by
+ * stage in lockstep, each on its own stream (several may share one); the
+ * batched solves run on the first engine's stream, after the other streams'
+ * L2 stages and before their L3 closes (events).  Stages: (gm_engine_stage_begin and
  * gm_engine_stage_l2 on each); gm_multi_l3 then runs the N3 L3 solves of the
  * current L2 substep of every engine as N3 launches of the batched step (all
  * engines' bricks in one launch per solve, melt time in Part 2) and finishes
@@ -324,6 +327,11 @@ int gm_engine_stage_field(gm_engine* e, int which, void** ptr, int64_t* count);
 typedef struct gm_multi gm_multi;
 int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out);
 int gm_multi_l3(gm_multi* m);
+/* The whole subcycle of every engine after gm_engine_stage_begin on each:
+ * for each of the 2*N2 L2 substeps the L2 stage of every engine, then
+ * gm_multi_l3.  Equivalent to the caller's loop over gm_engine_stage_l2 and
+ * gm_multi_l3, without a host call per engine and substep. */
+int gm_multi_subcycle(gm_multi* m);
 int gm_multi_destroy(gm_multi* m);
 
 /* stepGOMELT (cF:2304-2397) for one toolpath row.  Writes the reference's
