@@ -1016,11 +1016,16 @@ static void launch_step_l1(const LSArgs& a0, const GmStepDesc* d, const gm_grid&
         hipLaunchKernelGGL((k_step_l1<false, YB, ZC>), dim3(nb), dim3(64), 0, s, a, d, m);
 }
 
-// waves (bricks) per block of the subcycle steps, measured on the production
-// deck: 4 for L3 steps (6.69 us against 6.87 us with 1, 7.29 with 2, 7.88
-// with 8), 1 for L2 steps (4 made the L2 step slower)
+// waves (bricks) per block of the subcycle steps.  Measured on the
+// production deck on two machines: four per block ran the L3 step in
+// 6.69 us on both, one per block in 6.87 us on one and 6.51 us on the other,
+// where it also made the whole L1 step 6% faster (2.14 ms against 2.27); one
+// per block is kept for all levels.
 #ifndef GM_L3_WPB
-#define GM_L3_WPB 4
+#define GM_L3_WPB 1
+#endif
+#ifndef GM_L3ACC_WPB
+#define GM_L3ACC_WPB GM_L3_WPB
 #endif
 #ifndef GM_L2_WPB
 #define GM_L2_WPB 1
@@ -1032,7 +1037,7 @@ static void launch_dpp(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, 
     // measured best on 101 x 101 x 51 meshes: 3 planes per brick for L3
     // steps, 2 for L2 steps (whose correction-box reads favour more waves)
     constexpr int YB = 1, ZC = SRC ? 3 : 2;
-    constexpr int WPB = SRC ? GM_L3_WPB : GM_L2_WPB;
+    constexpr int WPB = SRC ? (ACC ? GM_L3ACC_WPB : GM_L3_WPB) : GM_L2_WPB;
     const int nq = hg.n[0] * ((hg.n[1] + YB - 1) / YB);
     const int nbr = ((nq + DXS - 1) / DXS) * ((hg.n[2] + ZC - 1) / ZC);
     const int nb = (nbr + WPB - 1) / WPB;
