@@ -109,6 +109,7 @@ _SIGS = {
     "gm_state_properties": (_I, [_M, _P, _P, _I64, _I64, _P, _P, _P, _P, _P]),
     "gm_solve_matrix_free": (_I, [_G, _I64, _P, _P, _F, _P, _P, _P, _P, _P]),
     "gm_source_l3": (_I, [_G, _M, _FP, _F, _P, _P]),
+    "gm_source_l3_mean": (_I, [_G, _M, _FP, _I, _P, _P]),
     "gm_level_source": (_I, [_G, _G, _M, _FP, _I, _P, _P]),
     "gm_conv_rad_bc": (_I, [_G, _M, _P, _I64, _P, _P]),
     "gm_melt_pool": (_I, [_G, _P, _F, _P, _P]),

@@ -251,6 +251,15 @@ extern "C" int gm_source_l3(const gm_grid* g, const gm_material* mat, const floa
     return 0;
 }
 
+extern "C" int gm_source_l3_mean(const gm_grid* g, const gm_material* mat,
+                                 const float* rows_host, int n_rows, float* F, void* stream) {
+    if (!g || !mat || !rows_host || !F || n_rows < 1)
+        return fail("gm_source_l3_mean: null argument or no rows");
+    launch_source_l3_mean(*g, make_devmat(*mat), rows_host, n_rows, F, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 extern "C" int gm_conv_rad_bc(const gm_grid* g, const gm_material* mat, const float* T,
                               int64_t ne_active, float* F, void* stream) {
     launch_convrad_full(*g, make_devmat(*mat), T, ne_active, F, (hipStream_t)stream);

@@ -360,6 +360,8 @@ void launch_props(const DevMat& m, const float* T, const float* S1, int64_t nn, 
                   float* S1o, uint8_t* S2o, float* k, float* c, hipStream_t s);
 void launch_source_l3_full(const gm_grid& g, const DevMat& m, float vx, float vy, float vz,
                            float P, float* F, hipStream_t s);
+void launch_source_l3_mean(const gm_grid& g, const DevMat& m, const float* rows, int n_rows,
+                           float* F, hipStream_t s);
 void launch_convrad_full(const gm_grid& g, const DevMat& m, const float* T, int64_t ne_active,
                          float* F, hipStream_t s);
 void launch_interp_matrix(const gm_grid& src, const gm_grid& tgt, float* Nc, int32_t* node,

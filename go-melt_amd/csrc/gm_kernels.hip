@@ -2994,6 +2994,52 @@ void launch_source_l3_full(const gm_grid& g, const DevMat& m, float vx, float vy
     hipLaunchKernelGGL(k_source_l3_full, dim3(nblk(n)), dim3(256), 0, s, g, m, vx, vy, vz, P, w3, F);
 }
 
+// Source averaged over up to 32 laser positions (one launch per pack):
+// F += a * source(position r) in position order, F cleared by the first pack.
+struct SrcPack {
+    float x[32], y[32], z[32], P[32];
+    int n;
+    float a;
+    int accumulate;
+};
+
+__global__ void k_source_l3_mean(gm_grid g, DevMat m, SrcPack pk, float w3,
+                                 float* __restrict__ F) {
+    const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int ix, iy, iz;
+    dec3(t, g.n[0], g.n[1], ix, iy, iz);
+    float acc = pk.accumulate ? F[t] : 0.0f;
+    for (int r = 0; r < pk.n; ++r) {
+        const float sx = src_axis(g, 0, ix, pk.x[r], m.rsq, m.rc) * (m.pc6 * pk.P[r]);
+        const float sy = src_axis(g, 1, iy, pk.y[r], m.rsq, m.rc);
+        const float sz = src_axis(g, 2, iz, pk.z[r], m.dsq, m.dc);
+        acc += pk.a * (sx * sy * sz * w3);
+    }
+    F[t] = acc;
+}
+
+void launch_source_l3_mean(const gm_grid& g, const DevMat& m, const float* rows, int n_rows,
+                           float* F, hipStream_t s) {
+    const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const float w3 = (g.h[0] * 0.5f) * (g.h[1] * 0.5f) * (g.h[2] * 0.5f);
+    SrcPack pk;
+    pk.a = 1.0f / (float)n_rows;
+    for (int r0 = 0; r0 < n_rows; r0 += 32) {
+        pk.n = n_rows - r0 < 32 ? n_rows - r0 : 32;
+        pk.accumulate = r0 > 0;
+        for (int r = 0; r < pk.n; ++r) {
+            const float* row = rows + 7 * (r0 + r);
+            pk.x[r] = row[0];
+            pk.y[r] = row[1];
+            pk.z[r] = row[2];
+            pk.P[r] = row[6];
+        }
+        hipLaunchKernelGGL(k_source_l3_mean, dim3(nblk(n)), dim3(256), 0, s, g, m, pk, w3, F);
+    }
+}
+
 __global__ void k_convrad_full(gm_grid g, DevMat m, const float* __restrict__ T, int top,
                                float w2, float* __restrict__ F) {
     const int nx = g.n[0], ny = g.n[1];

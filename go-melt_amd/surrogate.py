@@ -730,9 +730,16 @@ class EngineOperator:
     ``net`` None runs the plain solves everywhere (with ``record``, the
     training-window generator)."""
 
-    def __init__(self, net, properties, residual_limit=0.01, judge_every=200, record=None):
+    def __init__(self, net, properties, residual_limit=0.01, judge_every=200, record=None,
+                 graph=True):
         self.net = net
         self.p = properties
+        # the network's forward pass recorded once as a graph and replayed
+        # on every jump (its launches then cost no host time)
+        self.graph = bool(graph)
+        self._g = None
+        self._gout = None
+        self._gx = None
         self.limit = float(residual_limit)
         self.judge_every = int(judge_every) if judge_every else 0
         self.record = record
@@ -759,12 +766,10 @@ class EngineOperator:
         S2o = self._buf("S2o", nn, torch.uint8)
         N.check(lib.gm_state_properties(C.byref(mat), _ptr(T0), _ptr(S1), nn, 0, _ptr(S1o),
                                         _ptr(S2o), _ptr(k), _ptr(rc), st))
-        F, Fk = self._buf("F", nn), self._buf("Fk", nn)
-        zero(F)
-        for r in rows:
-            v = (C.c_float * 3)(float(r[0]), float(r[1]), float(r[2]))
-            N.check(lib.gm_source_l3(C.byref(g), C.byref(mat), v, float(r[6]), _ptr(Fk), st))
-            N.check(lib.gm_nn_axpy(nn, 1.0 / len(rows), _ptr(Fk), _ptr(F), st))
+        F = self._buf("F", nn)
+        r32 = np.ascontiguousarray(rows, dtype=np.float32)
+        N.check(lib.gm_source_l3_mean(C.byref(g), C.byref(mat), N.fptr(r32), r32.shape[0],
+                                      _ptr(F), st))
         ne = (g.n[0] - 1) * (g.n[1] - 1) * (g.n[2] - 1)
         N.check(lib.gm_conv_rad_bc(C.byref(g), C.byref(mat), _ptr(T0), ne, _ptr(F), st))
         sums = self._buf("sums", 8, torch.float64)
@@ -811,7 +816,7 @@ class EngineOperator:
         if self.X is None or tuple(self.X.shape) != shape:
             self.X = torch.empty(shape, dtype=torch.bfloat16, device="cuda")
         window_input({"properties": self.p}, g, T0, fa, rows[0], self.X[0])
-        Yh = self.net.forward(self.X)
+        Yh = self._forward()
         nn3 = (C.c_int32 * 3)(g.n[0], g.n[1], g.n[2])
         N.check(lib.gm_nn_window_apply(nn3, _ptr(T0), _ptr(fa), _ptr(Yh), Yh.shape[-1], T_SCALE,
                                        float(self.p["T_amb"]), _ptr(out), _stream()))
@@ -848,6 +853,28 @@ class EngineOperator:
         t = torch.empty(n, dtype=torch.float32, device="cuda")
         N.check(N.lib().gm_memcpy_d2d(_ptr(t), C.c_void_p(raw.p), 4 * n, _stream()))
         return t
+
+    def _forward(self):
+        """The network on self.X: eager, or the replay of its recorded graph
+        (recorded on first use; the network's kernels allocate through
+        PyTorch and clear nothing with memset, so the graph holds kernels
+        only).  The output tensor is the graph's own, reused by every
+        replay."""
+        if not self.graph:
+            return self.net.forward(self.X)
+        if self._g is None or self._gx is not self.X:
+            cur = torch.cuda.current_stream()
+            side = torch.cuda.Stream()
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):  # warm-up outside the capture
+                self.net.forward(self.X)
+            cur.wait_stream(side)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                out = self.net.forward(self.X)
+            self._g, self._gout, self._gx = g, out, self.X
+        self._g.replay()
+        return self._gout
 
 
 def engine_windows(n_windows, seed0=0, n_l1=(80, 80, 40), length=4.0, per_run=1000,

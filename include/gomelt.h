@@ -127,6 +127,13 @@ int gm_solve_matrix_free(const gm_grid* grid, int64_t ne_active, const float* k,
 int gm_source_l3(const gm_grid* grid, const gm_material* mat, const float* v_host,
                  float P, float* F_out, void* stream);
 
+/* The same source averaged over n_rows laser rows (row stride 7 floats: x, y,
+ * z, Ljump, Ldwell, dt, P), F = sum_r (1/n_rows) F_r in row order; the mean
+ * source the L3 solves of an L2 substep see (cF:3367-3412), for the energy
+ * audit of a jump over them.  Not a reference function. */
+int gm_source_l3_mean(const gm_grid* grid, const gm_material* mat, const float* rows_host,
+                      int n_rows, float* F_out, void* stream);
+
 /* computeLevelSource (cF:2667-2730): coarse source averaged over n_pos laser
  * rows (row stride 7 floats: x, y, z, Ljump, Ldwell, dt, P), evaluated at the
  * fine mesh's Gauss points and restricted with the coarse shape functions. */

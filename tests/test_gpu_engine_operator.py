@@ -122,6 +122,32 @@ def test_operator_jumps_kept_without_limit(tmp_path):
         assert np.all(np.isfinite(f[k])) and float(np.min(f[k])) >= 300.0 - 1e-3, k
 
 
+def test_forward_graph_replays_equal_eager(tmp_path):
+    """The operator's forward pass recorded as a graph and replayed on every
+    jump (with the engine's stage work and the audits between the replays)
+    gives the run the eager forward gives: every jump kept, the residuals and
+    the final fields equal up to the order of the FP64 atomic sums in the
+    restrictions and the audit."""
+    import surrogate as S
+    import computeFunctions as cf
+    deck = _deck(tmp_path)
+    p = cf.SetupProperties(deck["properties"])
+    out = []
+    for graph in (False, True):
+        op = S.EngineOperator(_net(1), p, residual_limit=float("inf"), judge_every=0,
+                              graph=graph)
+        run = _run(deck, operator=op)
+        assert op.stats["jumps"] >= 4 and op.stats["fallbacks"] == 0
+        assert (op._g is not None) == graph
+        out.append((_fields(run), list(op.stats["residuals"])))
+    (fa, ra), (fb, rb) = out
+    np.testing.assert_allclose(ra, rb, rtol=1e-9, atol=1e-12)
+    for key in fa:
+        scale = max(float(np.max(np.abs(fa[key]))), 1e-30)
+        assert float(np.max(np.abs(fa[key] - fb[key]))) <= 1e-6 * scale, key
+    print("bitwise:", {k: bool(np.array_equal(fa[k], fb[k])) for k in fa})
+
+
 def test_recorder_sees_every_substep(tmp_path):
     import surrogate as S
     deck = _deck(tmp_path)

@@ -81,6 +81,30 @@ def test_source_l3(cf, oracle):
     assert _rel(a, b) < 2e-6
 
 
+@pytest.mark.parametrize("n_rows", [1, 10, 37])
+def test_source_l3_mean(cf, oracle, n_rows):
+    """gm_source_l3_mean: the mean of computeSourcesL3 (cF:2960-3012) over a
+    substep's laser rows in one launch per 32 rows, against the mean of the
+    oracle's per-row sources."""
+    import ctypes as C
+    import _native as N
+    _, p, L = _deck(oracle, n=(16, 12, 8))
+    rng = np.random.default_rng(n_rows)
+    rows = np.zeros((n_rows, 7), np.float32)
+    rows[:, 0] = 4.0 + rng.uniform(-0.4, 0.4, n_rows)
+    rows[:, 1] = 4.0 + rng.uniform(-0.4, 0.4, n_rows)
+    rows[:, 6] = rng.uniform(100, 300, n_rows)
+    g = cf._grid_of(L[3])
+    out = N.DeviceArray(g.nn())
+    N.check(N.lib().gm_source_l3_mean(C.byref(g), C.byref(cf._material(p)),
+                                      N.fptr(rows), n_rows, out.ptr, None))
+    N.sync(None)
+    ne_nn = (0, 0, 0, 0, L[3]["nn"])
+    b = sum(oracle.computeSourcesL3(L[3], r[:3].astype(np.float64), ne_nn, p, float(r[6]))
+            for r in rows) / n_rows
+    assert _rel(out.numpy(), b) < 2e-6
+
+
 @pytest.mark.parametrize("pair", [(1, 3), (2, 3), (1, 2)])
 def test_level_source(cf, oracle, pair):
     _, p, L = _deck(oracle, n=(12, 12, 6))
