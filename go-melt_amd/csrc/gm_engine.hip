@@ -1915,6 +1915,8 @@ extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out)
     for (int i = 0; i < n; ++i) {
         gm_engine* e = m->e[i];
         if (!e) return bad(fail("gm_multi_create: null engine"));
+        if (std::find(m->e.begin(), m->e.begin() + i, e) != m->e.begin() + i)
+            return bad(fail("gm_multi_create: an engine is listed twice"));
         if (e->s != m->s && std::find(m->xs.begin(), m->xs.end(), e->s) == m->xs.end())
             m->xs.push_back(e->s);
         if (e->N2 != e0->N2 || e->N3 != e0->N3)
