@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--surrogate-iters", type=int, default=1250)
     ap.add_argument("--surrogate-windows", type=int, default=20000)
     ap.add_argument("--c5-steps", type=int, default=20)
+    ap.add_argument("--surrogate-channels", default="32,64,128,128",
+                    help="channels per U-Net level")
     ap.add_argument("--surrogate-only", action="store_true",
                     help="print only the learned-operator measurement (one JSON line)")
     return ap.parse_args()
@@ -116,7 +118,7 @@ def c5_run(net, steps, warmup=3, residual_limit=0.01, judge_every=200, lines=2):
     return out
 
 
-def surrogate_measure(n_windows, iters, c5_steps):
+def surrogate_measure(n_windows, iters, c5_steps, channels=(32, 64, 128, 128)):
     """Learned L3 operator (3-D U-Net, channels 32-64-128-128, BF16 MFMA
     convolutions, FP32 accumulation, Adam): trained on windows of the plain
     three-level engine (every L2 substep of seeded single-track and raster
@@ -127,7 +129,7 @@ def surrogate_measure(n_windows, iters, c5_steps):
     import numpy as np
     import torch
     import surrogate as S
-    net = S.UNet(channels=(32, 64, 128, 128), seed=0)
+    net = S.UNet(channels=tuple(channels), seed=0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     X, Y, seeds = S.engine_windows(n_windows)
@@ -160,7 +162,8 @@ def surrogate_measure(n_windows, iters, c5_steps):
     plain = c5_run(None, c5_steps)
     sur = c5_run(net, c5_steps)
     return {
-        "workload": ("3-D U-Net 32-64-128-128 on 64x64x32-node L3 windows, one jump per L2 "
+        "workload": (f"3-D U-Net {'-'.join(str(c) for c in channels)} on 64x64x32-node L3 "
+                     "windows, one jump per L2 "
                      f"substep (K = N3 = 10 L3 steps); {X.shape[0]} training windows from "
                      f"{len(set(seeds.tolist()))} seeded plain three-level runs "
                      f"(held out: the last run, {Xte.shape[0]} windows), {iters} Adam steps "
@@ -330,6 +333,13 @@ def cpu_baseline(inp, gcode, n23):
     }
 
 
+def _channels(text):
+    ch = tuple(int(v) for v in text.split(","))
+    if len(ch) < 2 or any(c % 16 for c in ch):
+        raise SystemExit("--surrogate-channels: two or more levels, multiples of 16")
+    return ch
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -339,7 +349,7 @@ def main():
     if a.surrogate_only:
         import torch  # noqa: F401  (its HIP runtime first, before libgomelt.so)
         print(json.dumps(surrogate_measure(a.surrogate_windows, a.surrogate_iters,
-                                           a.c5_steps)))
+                                           a.c5_steps, _channels(a.surrogate_channels))))
         return
     if a.stacks_only:
         print(json.dumps(stacks_measure(a.batch_n, chunks=a.stacks_chunks,
@@ -461,7 +471,8 @@ def main():
         import subprocess
         cmd = [sys.executable, os.path.abspath(__file__), "--surrogate-only",
                "--surrogate-iters", str(a.surrogate_iters),
-               "--surrogate-windows", str(a.surrogate_windows), "--c5-steps", str(a.c5_steps)]
+               "--surrogate-windows", str(a.surrogate_windows), "--c5-steps", str(a.c5_steps),
+               "--surrogate-channels", a.surrogate_channels]
         try:
             out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
             lines = [ln for ln in out.stdout.splitlines() if ln.startswith("{")]
