@@ -148,12 +148,13 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvDims c, const uint16_t* __
 }
 
 // The same convolution for cin a multiple of 32, software-pipelined: a K
-// stage is one tap x 32 input channels (one v_mfma_f32_16x16x32_bf16 k-step,
-// 4 MFMAs per wave between barriers), each thread stages 16 bytes of A and of
-// B, and the next stage's global loads are in flight while the current stage
-// computes.  CI16: cin = 16, and a stage is two taps x 16 channels (14
-// stages, the 28th tap zero).
-template <int BM, int BN, bool CI16 = false>
+// stage is G sub-stages of one tap x 32 input channels (one
+// v_mfma_f32_16x16x32_bf16 k-step each, 4 MFMAs per wave and sub-stage), each
+// thread stages 16 bytes of A and of B per sub-stage, and the next stage's
+// global loads are in flight while the current stage computes.  Sub-stages
+// past the last tap are zero.  CI16: cin = 16, and a sub-stage is two taps x
+// 16 channels (14 sub-stages, the 28th tap zero).
+template <int BM, int BN, bool CI16 = false, int G = 1>
 __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* __restrict__ X,
                                                     const uint16_t* __restrict__ Wk,
                                                     const float* __restrict__ bias, int relu,
@@ -161,9 +162,10 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
                                                     uint16_t* __restrict__ Ybf) {
     // BM voxels x BN outputs per block, 4 waves of 32 x 32 (BM/32 x BN/32 = 4)
     static_assert((BM / 32) * (BN / 32) == 4, "four waves of 32 x 32");
-    constexpr int NA = BM / 64;  // 16-byte A loads per thread and stage
-    __shared__ uint16_t sA[BM][32 + 8];
-    __shared__ uint16_t sB[BN][32 + 8];
+    constexpr int NA = BM / 64;  // 16-byte A loads per thread and sub-stage
+    constexpr int KW = G * 32 + 8;
+    __shared__ uint16_t sA[BM][KW];
+    __shared__ uint16_t sB[BN][KW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // XCD-aware tiles: a run of consecutive voxel tiles (with all their output
     // tiles) per XCD, so the tap-shifted rows of neighbouring tiles hit its L2
@@ -188,51 +190,65 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
     }
     const int o = n0 + srow;  // B rows: threads with srow < BN
     const int kch = CI16 ? 1 : c.ci / 32;
-    const int nst = CI16 ? 14 : 27 * kch;
-    auto fetch = [&](int st, uint4* av, uint4& bv) {
-        int tap, i0;
-        if constexpr (CI16) {
-            tap = 2 * st + (sk >> 4);  // k 0-15: tap 2 st, k 16-31: tap 2 st + 1
-            i0 = sk & 15;
-        } else {
-            tap = st / kch;
-            i0 = (st - tap * kch) * 32 + sk;
-        }
-        const bool tv = !CI16 || tap < 27;
+    const int nsub = CI16 ? 14 : 27 * kch;
+    const int nst = (nsub + G - 1) / G;
+    auto fetch = [&](int st, uint4 (*av)[NA], uint4* bv) {
 #pragma unroll
-        for (int q = 0; q < NA; ++q) {
-            const int64_t vs = vrow[q] < c.nvox && tv ? vox_shift(c, pv[q], tap) : -1;
-            av[q] = make_uint4(0u, 0u, 0u, 0u);
-            if (vs >= 0) av[q] = *(const uint4*)(X + vs * c.ci + i0);
+        for (int g = 0; g < G; ++g) {
+            const int sub = st * G + g;
+            int tap, i0;
+            if constexpr (CI16) {
+                tap = 2 * sub + (sk >> 4);  // k 0-15: tap 2 sub, k 16-31: tap 2 sub + 1
+                i0 = sk & 15;
+            } else {
+                tap = sub / kch;
+                i0 = (sub - tap * kch) * 32 + sk;
+            }
+            const bool tv = sub < nsub && tap < 27;
+#pragma unroll
+            for (int q = 0; q < NA; ++q) {
+                const int64_t vs = vrow[q] < c.nvox && tv ? vox_shift(c, pv[q], tap) : -1;
+                av[g][q] = make_uint4(0u, 0u, 0u, 0u);
+                if (vs >= 0) av[g][q] = *(const uint4*)(X + vs * c.ci + i0);
+            }
+            bv[g] = make_uint4(0u, 0u, 0u, 0u);
+            if (srow < BN && o < c.co && tv)
+                bv[g] = *(const uint4*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
         }
-        bv = make_uint4(0u, 0u, 0u, 0u);
-        if (srow < BN && o < c.co && tv)
-            bv = *(const uint4*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
     };
-    uint4 av[NA], bv;
+    uint4 av[G][NA], bv[G];
     fetch(0, av, bv);
     const int fi = lane & 15, fk8 = (lane >> 4) * 8;
     for (int st = 0; st < nst; ++st) {
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < NA; ++q) *(uint4*)&sA[srow + 64 * q][sk] = av[q];
-        if (srow < BN) *(uint4*)&sB[srow][sk] = bv;
+        for (int g = 0; g < G; ++g) {
+#pragma unroll
+            for (int q = 0; q < NA; ++q) *(uint4*)&sA[srow + 64 * q][g * 32 + sk] = av[g][q];
+            if (srow < BN) *(uint4*)&sB[srow][g * 32 + sk] = bv[g];
+        }
         __syncthreads();
         if (st + 1 < nst) fetch(st + 1, av, bv);
-        // the stage's 32 channels in one K = 32 MFMA: each lane gives 8
+        // each sub-stage's 32 channels in one K = 32 MFMA: each lane gives 8
         // consecutive channels of its row (A) and column (B), the same k
         // positions for both operands
-        v8bf fa[2], fb[2];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            fa[t] = __builtin_bit_cast(v8bf, *(const uint4*)&sA[wm + t * 16 + fi][fk8]);
-            fb[t] = __builtin_bit_cast(v8bf, *(const uint4*)&sB[wn + t * 16 + fi][fk8]);
+        for (int g = 0; g < G; ++g) {
+            v8bf fa[2], fb[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                fa[t] = __builtin_bit_cast(v8bf,
+                                           *(const uint4*)&sA[wm + t * 16 + fi][g * 32 + fk8]);
+                fb[t] = __builtin_bit_cast(v8bf,
+                                           *(const uint4*)&sB[wn + t * 16 + fi][g * 32 + fk8]);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
     }
     const int dj = lane & 15, di = (lane >> 4) * 4;
 #pragma unroll
@@ -250,6 +266,22 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
                     if (Ybf) Ybf[v * c.co + oo] = f2bf(y);
                 }
             }
+}
+
+template <int BM, int BN, bool CI16>
+static void launch_fwd32(const dim3& grid, const ConvDims& c, const uint16_t* X,
+                         const uint16_t* Wk, const float* bias, int relu, float* Y,
+                         uint16_t* Ybf, hipStream_t s) {
+    // Few voxels (below 2^18; one window is 2^17 at full resolution) leave the chip with
+    // few resident blocks; three sub-stages per barrier pair then hide more
+    // of each stage's latency (batch-1 forward 0.86 -> 0.73 ms).  With many
+    // voxels the one-sub-stage kernel is faster (more resident blocks).
+    if (c.nvox < ((int64_t)1 << 18))
+        hipLaunchKernelGGL((k_conv_fwd32<BM, BN, CI16, 3>), grid, dim3(256), 0, s, c, X, Wk, bias,
+                           relu, Y, Ybf);
+    else
+        hipLaunchKernelGGL((k_conv_fwd32<BM, BN, CI16, 1>), grid, dim3(256), 0, s, c, X, Wk, bias,
+                           relu, Y, Ybf);
 }
 
 // Wt[tap][i][o] = Wk[26 - tap][o][i] (weights of the input-gradient pass)
@@ -560,21 +592,17 @@ void launch_conv_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk
     if (c.ci % 32 == 0 && al16) {
         if (c.co <= 32) {
             const dim3 g2((unsigned)((c.nvox + 127) / 128), 1);
-            hipLaunchKernelGGL((k_conv_fwd32<128, 32>), g2, dim3(256), 0, s, c, X, Wk, bias, relu,
-                               Y, Ybf);
+            launch_fwd32<128, 32, false>(g2, c, X, Wk, bias, relu, Y, Ybf, s);
         } else {
-            hipLaunchKernelGGL((k_conv_fwd32<64, 64>), grid, dim3(256), 0, s, c, X, Wk, bias, relu,
-                               Y, Ybf);
+            launch_fwd32<64, 64, false>(grid, c, X, Wk, bias, relu, Y, Ybf, s);
         }
     } else if (c.ci == 16 && al16) {
-        // two taps x 16 channels per K = 32 stage
+        // two taps x 16 channels per K = 32 sub-stage
         if (c.co <= 32) {
             const dim3 g2((unsigned)((c.nvox + 127) / 128), 1);
-            hipLaunchKernelGGL((k_conv_fwd32<128, 32, true>), g2, dim3(256), 0, s, c, X, Wk, bias,
-                               relu, Y, Ybf);
+            launch_fwd32<128, 32, true>(g2, c, X, Wk, bias, relu, Y, Ybf, s);
         } else {
-            hipLaunchKernelGGL((k_conv_fwd32<64, 64, true>), grid, dim3(256), 0, s, c, X, Wk, bias,
-                               relu, Y, Ybf);
+            launch_fwd32<64, 64, true>(grid, c, X, Wk, bias, relu, Y, Ybf, s);
         }
     } else {
         hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);

@@ -37,7 +37,9 @@ SHAPES = [(2, 5, 6, 7, 16, 32), (1, 8, 8, 4, 32, 16), (1, 3, 3, 3, 16, 48), (1, 
           (2, 3, 4, 32, 32, 32), (1, 3, 2, 64, 64, 16), (1, 2, 64, 2, 96, 32),
           # a larger volume: 32768 voxels, 128 outputs
           (2, 8, 32, 64, 32, 128),
-          # 2^18 voxels: 4096-voxel parts in the weight gradient
+          # 2^18 voxels: 4096-voxel parts in the weight gradient, and the
+          # forward kernel with one sub-stage per barrier pair (the smaller
+          # volumes above take three)
           (1, 8, 128, 256, 32, 32)]
 
 
