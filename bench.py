@@ -138,6 +138,8 @@ def surrogate_measure(n_windows, iters, c5_steps, channels=(32, 64, 128, 128)):
     seeds = np.asarray(seeds)
     held = seeds == seeds[-1]  # the last run's windows are held out
     ntr = int(np.count_nonzero(~held))
+    if ntr < 16:  # one run only (a small window count): hold out its last quarter
+        ntr = max(16, (3 * X.shape[0]) // 4)
     Xtr, Ytr, Xte, Yte = X[:ntr], Y[:ntr], X[ntr:], Y[ntr:]
     t0 = time.perf_counter()
     n1 = min(iters, max(iters // 2, 1))

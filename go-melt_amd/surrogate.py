@@ -500,6 +500,8 @@ def train(net, X, Y, iters=100, batch=4, lr=1e-3, seed=0, log=None, energy=0.0):
     import numpy as np
     rng = np.random.default_rng(seed)
     n = X.shape[0]
+    if n < batch:
+        raise ValueError(f"train: {n} windows for batches of {batch}")
     xb = torch.empty((batch,) + tuple(X.shape[1:]), dtype=X.dtype, device="cuda")
     yb = torch.empty((batch,) + tuple(Y.shape[1:]), dtype=Y.dtype, device="cuda")
     order = []

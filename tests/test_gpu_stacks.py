@@ -72,3 +72,29 @@ def test_stacks_equal_their_own_runs(tmp_path, monkeypatch):
             assert float(np.max(np.abs(a - b))) <= 1e-9 + 1e-6 * float(np.max(b)), (i, key)
         del ref
     print(f"worst relative difference {worst:.3g}; {bitwise} of {3 * len(decks)} fields bitwise")
+
+
+def test_stacks_one_after_another_across_streams(tmp_path):
+    """Stacks.step_each (the sequential baseline: each run's chunk waits on
+    the GPU for the run before it, across streams) ends every run where its
+    own run ends."""
+    import batch as B
+    decks = []
+    for j in range(3):
+        inp, g = W.stack_decks(n=1, seed=40 + j, sizes=[W.BATCH_SIZES[0]])[0]
+        with open(W.write_deck(inp, g, str(tmp_path / f"e{j}"))) as fh:
+            decks.append(json.load(fh))
+    st = B.Stacks(decks, streams=2)
+    assert len({id(r.eng.stream) for r in st.runs}) == 2
+    st.step()  # the single-step chunk
+    for _ in range(2):
+        assert st.step_each()
+    st.sync()
+    for i, deck in enumerate(decks):
+        run = st.runs[i]
+        ref = _alone(deck, 3)
+        assert run.stats["chunks"] == ref.stats["chunks"], (i, run.stats, ref.stats)
+        for k in (1, 2, 3):
+            a = np.asarray(run.Levels[k]["T0"], np.float64)
+            b = np.asarray(ref.Levels[k]["T0"], np.float64)
+            assert float(np.max(np.abs(a - b))) <= 1e-6 * (float(np.max(b)) - 300.0), (i, k)
