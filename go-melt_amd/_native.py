@@ -150,6 +150,8 @@ _SIGS = {
     "gm_batch_refresh": (_I, [_P]),
     "gm_batch_jump_state": (_I, [_P, _I, _P]),
     "gm_conv3d_fwd": (_I, [_IP, _P, _P, _P, _I, _P, _P, _P]),
+    "gm_conv3d_fwd_ws_bytes": (_I, [_IP, C.POINTER(_I64)]),
+    "gm_conv3d_fwd_ws": (_I, [_IP, _P, _P, _P, _I, _P, _P, _P, _I64, _P]),
     "gm_conv3d_bwd_data": (_I, [_IP, _P, _P, _P, _P, _P, _P]),
     "gm_conv3d_bwd_weight": (_I, [_IP, _P, _P, _P, _P]),
     "gm_adam": (_I, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I, _P]),

@@ -154,12 +154,17 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvDims c, const uint16_t* __
 // global loads are in flight while the current stage computes.  Sub-stages
 // past the last tap are zero.  CI16: cin = 16, and a sub-stage is two taps x
 // 16 channels (14 sub-stages, the 28th tap zero).
+//
+// Split K (part != nullptr): block z of gridDim.z runs stages z*spb ..
+// z*spb + spb - 1 and writes its raw sums to part[z][vox][out]; the output
+// is then k_conv_part_sum's fixed-order sum of the parts.
 template <int BM, int BN, bool CI16 = false, int G = 1>
 __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* __restrict__ X,
                                                     const uint16_t* __restrict__ Wk,
                                                     const float* __restrict__ bias, int relu,
                                                     float* __restrict__ Y,
-                                                    uint16_t* __restrict__ Ybf) {
+                                                    uint16_t* __restrict__ Ybf,
+                                                    float* __restrict__ part, int spb) {
     // BM voxels x BN outputs per block, 4 waves of 32 x 32 (BM/32 x BN/32 = 4)
     static_assert((BM / 32) * (BN / 32) == 4, "four waves of 32 x 32");
     constexpr int NA = BM / 64;  // 16-byte A loads per thread and sub-stage
@@ -192,6 +197,8 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
     const int kch = CI16 ? 1 : c.ci / 32;
     const int nsub = CI16 ? 14 : 27 * kch;
     const int nst = (nsub + G - 1) / G;
+    const int st0 = (int)blockIdx.z * spb;
+    const int st1 = st0 + spb < nst ? st0 + spb : nst;
     auto fetch = [&](int st, uint4 (*av)[NA], uint4* bv) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -217,9 +224,9 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
         }
     };
     uint4 av[G][NA], bv[G];
-    fetch(0, av, bv);
+    fetch(st0, av, bv);
     const int fi = lane & 15, fk8 = (lane >> 4) * 8;
-    for (int st = 0; st < nst; ++st) {
+    for (int st = st0; st < st1; ++st) {
         __syncthreads();
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -228,7 +235,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
             if (srow < BN) *(uint4*)&sB[srow][g * 32 + sk] = bv[g];
         }
         __syncthreads();
-        if (st + 1 < nst) fetch(st + 1, av, bv);
+        if (st + 1 < st1) fetch(st + 1, av, bv);
         // each sub-stage's 32 channels in one K = 32 MFMA: each lane gives 8
         // consecutive channels of its row (A) and column (B), the same k
         // positions for both operands
@@ -260,6 +267,10 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
                 const int64_t v = m0 + wm + a * 16 + di + r;
                 const int oo = n0 + wn + b * 16 + dj;
                 if (v < c.nvox && oo < c.co) {
+                    if (part) {
+                        part[(int64_t)blockIdx.z * c.nvox * c.co + v * c.co + oo] = acc[a][b][r];
+                        continue;
+                    }
                     float y = acc[a][b][r] + (bias ? bias[oo] : 0.0f);
                     if (relu && y < 0.0f) y = 0.0f;
                     Y[v * c.co + oo] = y;
@@ -268,20 +279,81 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
             }
 }
 
+// Y = sum over the S parts (in part order) + bias, relu, bf16 copy
+__global__ void k_conv_part_sum(const float* __restrict__ part, int S, int64_t n, int co,
+                                const float* __restrict__ bias, int relu, float* __restrict__ Y,
+                                uint16_t* __restrict__ Ybf) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float y = 0.0f;
+    for (int z = 0; z < S; ++z) y += part[(int64_t)z * n + t];
+    if (bias) y += bias[t % co];
+    if (relu && y < 0.0f) y = 0.0f;
+    Y[t] = y;
+    if (Ybf) Ybf[t] = f2bf(y);
+}
+
+// Launch plan of the pipelined forward kernel: tile shape, sub-stages per
+// barrier pair and the K split.
+struct FwdPlan {
+    bool fwd32 = false, ci16 = false, small = false;
+    int bm = 64, bn = 64;
+    dim3 grid;
+    int nst = 0;     // stages of the whole K
+    int splits = 1;  // K splits (grid z)
+    int spb = 1;     // stages per split
+};
+
+static FwdPlan plan_fwd(const ConvDims& c, bool al16) {
+    FwdPlan p;
+    p.grid = dim3((unsigned)((c.nvox + 63) / 64), (unsigned)((c.co + 63) / 64));
+    if (!al16 || !(c.ci % 32 == 0 || c.ci == 16)) return p;
+    p.fwd32 = true;
+    p.ci16 = c.ci == 16;
+    // the pipelined kernel reads 16 bytes at a time; up to 32 outputs it
+    // takes 128 voxels x 32 outputs per block (no idle half of the tile)
+    if (c.co <= 32) {
+        p.bm = 128;
+        p.bn = 32;
+        p.grid = dim3((unsigned)((c.nvox + 127) / 128), 1);
+    }
+    // Few voxels (below 2^18; one window is 2^17 at full resolution) leave
+    // the chip with few resident blocks; three sub-stages per barrier pair
+    // then hide more of each stage's latency (batch-1 forward 0.86 -> 0.73
+    // ms).  With many voxels one sub-stage is faster (more resident blocks).
+    p.small = c.nvox < ((int64_t)1 << 18);
+    const int G = p.small ? 3 : 1;
+    const int nsub = p.ci16 ? 14 : 27 * (c.ci / 32);
+    p.nst = (nsub + G - 1) / G;
+    p.spb = p.nst;
+    // Coarse volumes give fewer blocks than the chip has CUs: split K over
+    // up to 16 blocks per tile, aiming at ~512 blocks, each with at least
+    // two stages.
+    const int64_t blocks = (int64_t)p.grid.x * p.grid.y;
+    if (blocks < 128) {
+        int S = (int)(512 / blocks);
+        if (S > 16) S = 16;
+        if (S > p.nst / 2) S = p.nst / 2;
+        if (S >= 2) {
+            p.spb = (p.nst + S - 1) / S;
+            p.splits = (p.nst + p.spb - 1) / p.spb;
+        }
+    }
+    return p;
+}
+
 template <int BM, int BN, bool CI16>
-static void launch_fwd32(const dim3& grid, const ConvDims& c, const uint16_t* X,
+static void launch_fwd32(const FwdPlan& p, const ConvDims& c, const uint16_t* X,
                          const uint16_t* Wk, const float* bias, int relu, float* Y,
-                         uint16_t* Ybf, hipStream_t s) {
-    // Few voxels (below 2^18; one window is 2^17 at full resolution) leave the chip with
-    // few resident blocks; three sub-stages per barrier pair then hide more
-    // of each stage's latency (batch-1 forward 0.86 -> 0.73 ms).  With many
-    // voxels the one-sub-stage kernel is faster (more resident blocks).
-    if (c.nvox < ((int64_t)1 << 18))
-        hipLaunchKernelGGL((k_conv_fwd32<BM, BN, CI16, 3>), grid, dim3(256), 0, s, c, X, Wk, bias,
-                           relu, Y, Ybf);
+                         uint16_t* Ybf, float* part, hipStream_t s) {
+    const dim3 g(p.grid.x, p.grid.y, part ? p.splits : 1);
+    const int spb = part ? p.spb : p.nst;
+    if (p.small)
+        hipLaunchKernelGGL((k_conv_fwd32<BM, BN, CI16, 3>), g, dim3(256), 0, s, c, X, Wk, bias,
+                           relu, Y, Ybf, part, spb);
     else
-        hipLaunchKernelGGL((k_conv_fwd32<BM, BN, CI16, 1>), grid, dim3(256), 0, s, c, X, Wk, bias,
-                           relu, Y, Ybf);
+        hipLaunchKernelGGL((k_conv_fwd32<BM, BN, CI16, 1>), g, dim3(256), 0, s, c, X, Wk, bias,
+                           relu, Y, Ybf, part, spb);
 }
 
 // Wt[tap][i][o] = Wk[26 - tap][o][i] (weights of the input-gradient pass)
@@ -582,31 +654,49 @@ bool conv_shape_ok(const int32_t* s) {
            (int64_t)s[0] * s[1] * s[2] * s[3] < ((int64_t)1 << 40);
 }
 
+static void launch_plan(const FwdPlan& p, const ConvDims& c, const uint16_t* X,
+                        const uint16_t* Wk, const float* bias, int relu, float* Y, uint16_t* Ybf,
+                        float* part, hipStream_t s) {
+    if (!p.fwd32) {
+        hipLaunchKernelGGL(k_conv_fwd, p.grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
+        return;
+    }
+    if (p.bm == 128) {
+        if (p.ci16) launch_fwd32<128, 32, true>(p, c, X, Wk, bias, relu, Y, Ybf, part, s);
+        else launch_fwd32<128, 32, false>(p, c, X, Wk, bias, relu, Y, Ybf, part, s);
+    } else {
+        if (p.ci16) launch_fwd32<64, 64, true>(p, c, X, Wk, bias, relu, Y, Ybf, part, s);
+        else launch_fwd32<64, 64, false>(p, c, X, Wk, bias, relu, Y, Ybf, part, s);
+    }
+    if (part) {
+        const int64_t n = c.nvox * c.co;
+        hipLaunchKernelGGL(k_conv_part_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           part, p.splits, n, c.co, bias, relu, Y, Ybf);
+    }
+}
+
 void launch_conv_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk,
                      const float* bias, int relu, float* Y, uint16_t* Ybf, hipStream_t s) {
     const ConvDims c = dims(shape);
-    const dim3 grid((unsigned)((c.nvox + 63) / 64), (unsigned)((c.co + 63) / 64));
-    // the pipelined kernel reads 16 bytes at a time; up to 32 outputs it
-    // takes 128 voxels x 32 outputs per block (no idle half of the tile)
     const bool al16 = (((uintptr_t)X | (uintptr_t)Wk) & 15) == 0;
-    if (c.ci % 32 == 0 && al16) {
-        if (c.co <= 32) {
-            const dim3 g2((unsigned)((c.nvox + 127) / 128), 1);
-            launch_fwd32<128, 32, false>(g2, c, X, Wk, bias, relu, Y, Ybf, s);
-        } else {
-            launch_fwd32<64, 64, false>(grid, c, X, Wk, bias, relu, Y, Ybf, s);
-        }
-    } else if (c.ci == 16 && al16) {
-        // two taps x 16 channels per K = 32 sub-stage
-        if (c.co <= 32) {
-            const dim3 g2((unsigned)((c.nvox + 127) / 128), 1);
-            launch_fwd32<128, 32, true>(g2, c, X, Wk, bias, relu, Y, Ybf, s);
-        } else {
-            launch_fwd32<64, 64, true>(grid, c, X, Wk, bias, relu, Y, Ybf, s);
-        }
-    } else {
-        hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
-    }
+    launch_plan(plan_fwd(c, al16), c, X, Wk, bias, relu, Y, Ybf, nullptr, s);
+}
+
+int64_t conv_fwd_split_bytes(const int32_t* shape) {
+    const ConvDims c = dims(shape);
+    const FwdPlan p = plan_fwd(c, true);
+    return p.fwd32 && p.splits > 1 ? (int64_t)p.splits * c.nvox * c.co * 4 : 0;
+}
+
+void launch_conv_fwd_ws(const int32_t* shape, const uint16_t* X, const uint16_t* Wk,
+                        const float* bias, int relu, float* Y, uint16_t* Ybf, void* ws,
+                        int64_t ws_bytes, hipStream_t s) {
+    const ConvDims c = dims(shape);
+    const bool al16 = (((uintptr_t)X | (uintptr_t)Wk) & 15) == 0;
+    const FwdPlan p = plan_fwd(c, al16);
+    const bool split = p.fwd32 && p.splits > 1 && ws &&
+                       ws_bytes >= (int64_t)p.splits * c.nvox * c.co * 4;
+    launch_plan(p, c, X, Wk, bias, relu, Y, Ybf, split ? (float*)ws : nullptr, s);
 }
 
 void launch_conv_wflip(int ci, int co, const uint16_t* Wk, uint16_t* Wt, hipStream_t s) {

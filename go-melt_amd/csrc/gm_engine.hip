@@ -2633,6 +2633,28 @@ extern "C" int gm_conv3d_fwd(const int32_t* shape, const uint16_t* X, const uint
     return 0;
 }
 
+extern "C" int gm_conv3d_fwd_ws_bytes(const int32_t* shape, int64_t* bytes) {
+    if (!shape || !bytes) return fail("gm_conv3d_fwd_ws_bytes: null argument");
+    if (!conv_shape_ok(shape)) return fail("gm_conv3d_fwd_ws_bytes: bad shape");
+    *bytes = conv_fwd_split_bytes(shape);
+    return 0;
+}
+
+extern "C" int gm_conv3d_fwd_ws(const int32_t* shape, const uint16_t* X, const uint16_t* Wk,
+                                const float* bias, int relu, float* Y, uint16_t* Ybf, void* ws,
+                                int64_t ws_bytes, void* stream) {
+    if (!shape || !X || !Wk || !Y) return fail("gm_conv3d_fwd_ws: null argument");
+    if (!conv_shape_ok(shape))
+        return fail("gm_conv3d_fwd_ws: shape {nb, D, H, W, cin, cout} needs positive sizes and "
+                    "channel counts that are multiples of 16");
+    if (((uintptr_t)X | (uintptr_t)Wk) & 7)
+        return fail("gm_conv3d_fwd_ws: X and Wk must be 8-byte aligned (4 bf16 per load)");
+    if (ws && ((uintptr_t)ws & 15)) return fail("gm_conv3d_fwd_ws: ws must be 16-byte aligned");
+    launch_conv_fwd_ws(shape, X, Wk, bias, relu, Y, Ybf, ws, ws_bytes, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 extern "C" int gm_conv3d_bwd_data(const int32_t* shape, const uint16_t* dY, const uint16_t* Wk,
                                   uint16_t* Wt_work, float* dX, uint16_t* dXbf, void* stream) {
     if (!shape || !dY || !Wk || !Wt_work || !dX) return fail("gm_conv3d_bwd_data: null argument");

@@ -217,6 +217,11 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 bool conv_shape_ok(const int32_t* shape);
 void launch_conv_fwd(const int32_t* shape, const uint16_t* X, const uint16_t* Wk,
                      const float* bias, int relu, float* Y, uint16_t* Ybf, hipStream_t s);
+// bytes of the K-split workspace the forward would use for this shape (0: no split)
+int64_t conv_fwd_split_bytes(const int32_t* shape);
+void launch_conv_fwd_ws(const int32_t* shape, const uint16_t* X, const uint16_t* Wk,
+                        const float* bias, int relu, float* Y, uint16_t* Ybf, void* ws,
+                        int64_t ws_bytes, hipStream_t s);
 void launch_conv_wflip(int ci, int co, const uint16_t* Wk, uint16_t* Wt, hipStream_t s);
 void launch_conv_wgrad(const int32_t* shape, const uint16_t* X, const uint16_t* dY, int vsplit,
                        float* dW, hipStream_t s);

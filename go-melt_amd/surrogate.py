@@ -50,8 +50,16 @@ def conv3d(X, Wk, bias=None, relu=False, want_bf16=True):
     Yb = torch.empty_like(Y, dtype=torch.bfloat16) if want_bf16 else None
     if bias is not None:
         _check(bias, torch.float32, "bias")
-    N.check(N.lib().gm_conv3d_fwd(_shape(X, co), _ptr(X), _ptr(Wk), _ptr(bias), int(relu),
-                                  _ptr(Y), _ptr(Yb), _stream()))
+    shape = _shape(X, co)
+    nb = C.c_int64()
+    N.check(N.lib().gm_conv3d_fwd_ws_bytes(shape, C.byref(nb)))
+    if nb.value > 0:  # few output tiles: the K sum split over blocks
+        ws = torch.empty(nb.value, dtype=torch.uint8, device=X.device)
+        N.check(N.lib().gm_conv3d_fwd_ws(shape, _ptr(X), _ptr(Wk), _ptr(bias), int(relu),
+                                         _ptr(Y), _ptr(Yb), _ptr(ws), nb.value, _stream()))
+    else:
+        N.check(N.lib().gm_conv3d_fwd(shape, _ptr(X), _ptr(Wk), _ptr(bias), int(relu),
+                                      _ptr(Y), _ptr(Yb), _stream()))
     return Y, Yb
 
 

@@ -480,6 +480,17 @@ int gm_batch_jump_state(gm_batch* b, int i, const float* T0);
 int gm_conv3d_fwd(const int32_t* shape_host, const uint16_t* X, const uint16_t* Wk,
                   const float* bias, int relu, float* Y, uint16_t* Ybf, void* stream);
 
+/* The same forward pass with a caller's workspace: on volumes that give the
+ * chip few output tiles (coarse U-Net levels, one window) the K sum (taps x
+ * input channels) is split over blocks that write partial sums into ws,
+ * which a second kernel adds in a fixed order (+ bias, ReLU).
+ * gm_conv3d_fwd_ws_bytes gives the workspace size the split needs (0: no
+ * split for this shape); with a smaller or null ws the call runs unsplit. */
+int gm_conv3d_fwd_ws_bytes(const int32_t* shape_host, int64_t* bytes_host);
+int gm_conv3d_fwd_ws(const int32_t* shape_host, const uint16_t* X, const uint16_t* Wk,
+                     const float* bias, int relu, float* Y, uint16_t* Ybf, void* ws,
+                     int64_t ws_bytes, void* stream);
+
 /* Gradient of the input: dX[v][i] = sum_{tap,o} dY[v - s(tap)][o] Wk[tap][o][i]
  * (Wt_work: 27*cin*cout bf16 of caller scratch for the flipped weights). */
 int gm_conv3d_bwd_data(const int32_t* shape_host, const uint16_t* dY, const uint16_t* Wk,
