@@ -256,6 +256,7 @@ def stacks_measure(n, chunks=3, seed=0, streams=4):
             info["l3_nodes"] = sum(int(np.prod([len(v) for v in r.Levels[3]["node_coords"]]))
                                    for r in st.runs)
             info["n23"] = st.runs[0].subcycle[2]
+            st.close()
             del st, adv
             gc.collect()
     return {

@@ -107,12 +107,13 @@ class _Deferred:
     runs together."""
 
     def __init__(self, owner):
-        self.owner = owner
+        import weakref
+        self.owner = weakref.ref(owner)  # no cycle through the runs' engines
 
     def subcycle(self, e, rows):
         rows = np.ascontiguousarray(rows, dtype=np.float32)
         N.check(N.lib().gm_engine_stage_begin(e.h, N.fptr(rows), rows.shape[0]))
-        self.owner._pending.append(e)
+        self.owner()._pending.append(e)
 
 
 class Stacks:
@@ -204,9 +205,18 @@ class Stacks:
         for r in self.runs:
             r.eng.sync_coords()
 
+    def close(self):
+        """Release the batched-solve object and detach the runs (their
+        engines are released with them)."""
+        import computeFunctions as cf
+        if self._multi is not None:
+            N.lib().gm_multi_destroy(self._multi[1])
+            self._multi = None
+        for r in self.runs:
+            cf.attach_l3_operator(r.Levels, None)
+
     def __del__(self):
         try:
-            if self._multi is not None:
-                N.lib().gm_multi_destroy(self._multi[1])
+            self.close()
         except Exception:
             pass

@@ -1883,8 +1883,11 @@ struct gm_multi {
 
 extern "C" int gm_multi_destroy(gm_multi* m) {
     if (!m) return 0;
-    if (m->s) (void)hipStreamSynchronize(m->s);
-    for (hipStream_t x : m->xs) (void)hipStreamSynchronize(x);
+    // wait for this object's own work only (the engines' streams may be gone):
+    // the last batched launch and the item uploads
+    if (m->mev) (void)hipEventSynchronize(m->mev);
+    for (auto& ev : m->ring_ev)
+        if (ev) (void)hipEventSynchronize(ev);
     for (hipEvent_t ev : m->xev)
         if (ev) (void)hipEventDestroy(ev);
     if (m->mev) (void)hipEventDestroy(m->mev);
@@ -1999,10 +2002,8 @@ extern "C" int gm_multi_l3(gm_multi* m) {
             launch_batch_step_acc(m->d_items, m->d_mats, m->d_bstart, m->n, m->nblocks, m->rec,
                                   e0->d_desc, m->s);
     }
-    if (!m->xs.empty()) {
-        HIPCHK(hipEventRecord(m->mev, m->s));
-        for (hipStream_t x : m->xs) HIPCHK(hipStreamWaitEvent(x, m->mev, 0));
-    }
+    HIPCHK(hipEventRecord(m->mev, m->s));
+    for (hipStream_t x : m->xs) HIPCHK(hipStreamWaitEvent(x, m->mev, 0));
     for (int i = 0; i < m->n; ++i) {
         gm_engine* e = m->e[i];
         stage_l3_close(e, e->stg, part, k, w[i].S13in, w[i].S23in, e->stg.T3out);
