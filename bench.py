@@ -240,7 +240,7 @@ def stacks_measure(n, chunks=3, seed=0, streams=4):
         ms, info = {}, {}
         for mode, ns in (("batched", streams), ("sequential", 1)):
             t0 = time.perf_counter()
-            st = B.Stacks(decks, streams=ns)
+            st = B.Stacks(decks, streams=ns, tile="auto" if mode == "batched" else None)
             info["setup_s"] = time.perf_counter() - t0
             adv = st.step if mode == "batched" else st.step_each
             st.step()  # the single-step chunk
@@ -251,6 +251,9 @@ def stacks_measure(n, chunks=3, seed=0, streams=4):
                 adv()
             st.sync()
             ms[mode] = (time.perf_counter() - t0) * 1e3 / chunks
+            if mode == "batched":
+                info["tile"] = {"rows": int(st.tile[0]), "planes": int(st.tile[1])}
+                info["tile_ms"] = st.tile_ms
             info["mm"] = sum(float(r.Nonmesh["laser_velocity"]) * float(r.Nonmesh["timestep_L3"])
                              * r.subcycle[2] for r in st.runs)
             info["l3_nodes"] = sum(int(np.prod([len(v) for v in r.Levels[3]["node_coords"]]))
@@ -264,6 +267,8 @@ def stacks_measure(n, chunks=3, seed=0, streams=4):
                     f"{seed}) from {W.BATCH_SIZES}, L2 and L1 4x and 16x the L3 extent, "
                     f"P 100-400 W, 0.5-2 m/s, r 30-70 um",
         "streams": streams,
+        "part1_brick_autotuned": info["tile"],
+        "autotune_ms": info["tile_ms"],
         "setup_s": info["setup_s"],
         "ms_per_l1_step_batched": ms["batched"],
         "ms_per_l1_step_sequential": ms["sequential"],

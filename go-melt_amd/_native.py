@@ -176,6 +176,7 @@ _SIGS = {
     "gm_multi_create": (_I, [_P, _I, _P]),
     "gm_multi_l3": (_I, [_P]),
     "gm_multi_subcycle": (_I, [_P]),
+    "gm_multi_set_tile": (_I, [_P, _I, _I]),
     "gm_multi_destroy": (_I, [_P]),
 }
 

@@ -127,10 +127,11 @@ class Stacks:
     then the N3 L3 solves of all subcycling runs as N3 batched launches
     (gm_multi_subcycle).  Every run ends where its own unbatched run ends."""
 
-    def __init__(self, decks, streams=4):
+    def __init__(self, decks, streams=4, tile="auto"):
         import computeFunctions as cf
         import go_melt as G
         N.init_device()
+        self.decks = decks
         # runs dealt round-robin over a few streams, so the small per-run
         # kernels (L2 stages, moves, single steps) of different runs overlap
         self.streams = [N.Stream() for _ in range(max(1, min(int(streams), len(decks))))]
@@ -145,6 +146,30 @@ class Stacks:
         self._iters = [r.chunks() for r in self.runs]
         self._multi = None
         self._ev = [N.Event(), N.Event()]
+        # brick shape of the batched Part-1 solves: the autotuner's pick on
+        # the runs' L3 windows ("auto"), a given (rows, planes), or None (1 x 3)
+        self.tile_ms = None
+        self.tile = self.autotune_tile() if tile == "auto" else tile
+
+    def autotune_tile(self, iters=20):
+        """gm_batch_autotune on a batch of the runs' L3 windows (their meshes,
+        materials and first laser position, a few steps in): returns the
+        fastest (rows, planes); the times per shape go to self.tile_ms."""
+        import computeFunctions as cf
+        grids = [cf._grid_of(r.Levels[3]) for r in self.runs]
+        bt = Batch(grids, [d["properties"] for d in self.decks])
+        rows = np.zeros((self.n, 7), np.float32)
+        for i, d in enumerate(self.decks):
+            p = d["properties"]
+            rows[i, :3] = np.asarray(p.get("laser_center", [0.0, 0.0, 0.0])[:3], np.float32)
+            rows[i, 5] = float(d["nonmesh"].get("timestep_L3", 1e-6))
+            rows[i, 6] = float(p.get("laser_power", 0.0))
+        for _ in range(5):
+            bt.step(rows)
+        tile, ms = bt.autotune(iters)
+        self.tile_ms = [float(v) for v in ms]
+        del bt
+        return tile
 
     def _multi_of(self, engines):
         key = tuple(e.h for e in engines)
@@ -155,6 +180,8 @@ class Stacks:
             h = C.c_void_p()
             N.check(N.lib().gm_multi_create(arr, len(key), C.byref(h)))
             self._multi = (key, h.value)
+            if self.tile is not None:
+                N.check(N.lib().gm_multi_set_tile(h.value, int(self.tile[0]), int(self.tile[1])))
         return self._multi[1]
 
     def step(self):

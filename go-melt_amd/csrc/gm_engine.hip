@@ -1870,8 +1870,11 @@ struct gm_multi {
     BatchItem* h_ring = nullptr;  // pinned, 4 slots of n items
     hipEvent_t ring_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int ring_slot = 0;
-    int* d_bstart = nullptr;
+    int* d_bstart = nullptr;  // block starts of the 1 x 3 bricks (Part 2)
     int nblocks = 0;
+    int tile1 = 1;               // brick shape of the Part-1 solves (kBatchTiles)
+    int* d_bstart1 = nullptr;    // its block starts
+    int nblocks1 = 0;
     bool rec = false;
     std::vector<BatchItem> items;
     // the engines' other streams: the batched solves on s wait for their
@@ -1894,6 +1897,7 @@ extern "C" int gm_multi_destroy(gm_multi* m) {
     if (m->d_mats) (void)hipFree(m->d_mats);
     if (m->d_items) (void)hipFree(m->d_items);
     if (m->d_bstart) (void)hipFree(m->d_bstart);
+    if (m->d_bstart1) (void)hipFree(m->d_bstart1);
     if (m->h_ring) (void)hipHostFree(m->h_ring);
     for (auto& ev : m->ring_ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -1995,7 +1999,10 @@ extern "C" int gm_multi_l3(gm_multi* m) {
         HIPCHK(hipMemcpyAsync(m->d_items, h, sizeof(BatchItem) * m->n, hipMemcpyHostToDevice,
                               m->s));
         HIPCHK(hipEventRecord(m->ring_ev[slot], m->s));
-        if (part == 1)
+        if (part == 1 && m->d_bstart1)
+            launch_batch_step(m->d_items, m->d_mats, m->d_bstart1, m->n, m->nblocks1, m->tile1,
+                              m->rec, e0->d_desc, m->s);
+        else if (part == 1)
             launch_batch_step(m->d_items, m->d_mats, m->d_bstart, m->n, m->nblocks, 1, m->rec,
                               e0->d_desc, m->s);
         else
@@ -2010,6 +2017,25 @@ extern "C" int gm_multi_l3(gm_multi* m) {
         stage_advance(e);
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_multi_set_tile(gm_multi* m, int yb, int zc) {
+    if (!m) return fail("gm_multi_set_tile: null");
+    int z = -1;
+    for (int t = 0; t < GM_BATCH_TILES; ++t)
+        if (kBatchTiles[t][0] == yb && kBatchTiles[t][1] == zc) z = t;
+    if (z < 0)
+        return fail("gm_multi_set_tile: (rows, planes) must be one of (1,2) (1,3) (1,4) (2,2) (2,3)");
+    std::vector<int> st(m->n + 1, 0);
+    for (int i = 0; i < m->n; ++i)
+        st[i + 1] = st[i] + batch_bricks(m->e[i]->cfg.level[3], yb, zc);
+    // the previous table may still be read by launches in flight
+    HIPCHK(hipStreamSynchronize(m->s));
+    if (!m->d_bstart1) HIPCHK(hipMalloc(&m->d_bstart1, sizeof(int) * (m->n + 1)));
+    HIPCHK(hipMemcpy(m->d_bstart1, st.data(), sizeof(int) * (m->n + 1), hipMemcpyHostToDevice));
+    m->tile1 = z;
+    m->nblocks1 = st[m->n];
     return 0;
 }
 

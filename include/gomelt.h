@@ -339,6 +339,11 @@ int gm_multi_l3(gm_multi* m);
  * gm_multi_l3.  Equivalent to the caller's loop over gm_engine_stage_l2 and
  * gm_multi_l3, without a host call per engine and substep. */
 int gm_multi_subcycle(gm_multi* m);
+/* Brick shape (rows per band x planes, as gm_batch_set_tile) of the Part-1
+ * batched solves, e.g. the one gm_batch_autotune picks for the same L3
+ * windows; Part 2 (melt time) keeps 1 x 3.  Synchronises the first engine's
+ * stream. */
+int gm_multi_set_tile(gm_multi* m, int yb, int zc);
 int gm_multi_destroy(gm_multi* m);
 
 /* stepGOMELT (cF:2304-2397) for one toolpath row.  Writes the reference's
