@@ -213,3 +213,60 @@ def test_engine_windows_train_and_run_in_the_loop(tmp_path):
     f = _fields(run)
     for k in (1, 2, 3):
         assert np.all(np.isfinite(f[k])) and float(np.min(f[k])) >= 300.0 - 1e-3, k
+
+
+def test_staged_and_batched_calls_refuse_misuse(tmp_path):
+    """Host-side checks of the staged subcycle and the batched solves: stages
+    out of order, the recorded subcycle during an open staged one, and
+    batched solves over engines that are not at the same substep, or listed
+    twice, are refused with an error (no kernel runs for them).  The checks
+    run inside a subcycled L1 step of the driver, whose stages then complete
+    as the plain ones."""
+    import ctypes as C
+    import _native as N
+    import go_melt as G
+    import computeFunctions as cf
+    deck = _deck(tmp_path)
+    other = G.TimeLoop(deck, quiet=True, write_outputs=False)
+    e1 = other.eng
+    seen = []
+
+    class Probe:
+        def subcycle(self, e, rows):
+            lib = N.lib()
+            rows = np.ascontiguousarray(rows, dtype=np.float32)
+            with pytest.raises(N.GoMeltNativeError, match="out of order"):
+                N.check(lib.gm_engine_stage_l2(e.h, None, None))
+            N.check(lib.gm_engine_stage_begin(e.h, N.fptr(rows), rows.shape[0]))
+            with pytest.raises(N.GoMeltNativeError):
+                N.check(lib.gm_engine_subcycle(e.h, N.fptr(rows), rows.shape[0]))
+            h = C.c_void_p()
+            arr = (C.c_void_p * 2)(e.h, e.h)
+            with pytest.raises(N.GoMeltNativeError, match="twice"):
+                N.check(lib.gm_multi_create(arr, 2, C.byref(h)))
+            arr = (C.c_void_p * 2)(e.h, e1.h)
+            N.check(lib.gm_multi_create(arr, 2, C.byref(h)))
+            try:
+                # the other engine has not begun a staged subcycle
+                with pytest.raises(N.GoMeltNativeError, match="just begun"):
+                    N.check(lib.gm_multi_subcycle(h.value))
+                with pytest.raises(N.GoMeltNativeError, match="same substep"):
+                    N.check(lib.gm_multi_l3(h.value))
+            finally:
+                lib.gm_multi_destroy(h.value)
+            for _ in range(2 * e.N2):
+                N.check(lib.gm_engine_stage_l2(e.h, None, None))
+                N.check(lib.gm_engine_stage_l3(e.h, 0))
+            seen.append(rows.shape[0])
+
+    run = G.TimeLoop(deck, quiet=True, write_outputs=False)
+    cf.attach_l3_operator(run.Levels, Probe())
+    it = run.chunks()
+    for _ in range(4):  # until one subcycled L1 step has run
+        assert run.step_chunk(next(it))
+        if seen:
+            break
+    run.eng.sync()
+    assert seen == [run.eng.N2 * run.eng.N3]
+    for k in (1, 2, 3):
+        assert np.all(np.isfinite(np.asarray(run.Levels[k]["T0"]))), k
