@@ -53,6 +53,8 @@ def parse():
                     help="skip the batched three-level stacks measurement")
     ap.add_argument("--stacks-chunks", type=int, default=3)
     ap.add_argument("--stacks-streams", type=int, default=4)
+    ap.add_argument("--stacks-no-sequential", action="store_true",
+                    help="time only the batched pass of the stacks")
     ap.add_argument("--stacks-only", action="store_true",
                     help="print only the stacks measurement (one JSON line)")
     ap.add_argument("--no-surrogate", action="store_true",
@@ -219,7 +221,7 @@ def batch_measure(n, iters=50):
     }
 
 
-def stacks_measure(n, chunks=3, seed=0, streams=4):
+def stacks_measure(n, chunks=3, seed=0, streams=4, sequential=True):
     """The batched configuration as three-level stacks: n single-track runs,
     each its own L1/L2/L3 stack (element counts drawn from the batch sizes),
     advanced in lockstep with the L3 solves of all runs in one launch per
@@ -238,7 +240,8 @@ def stacks_measure(n, chunks=3, seed=0, streams=4):
             with open(W.write_deck(inp, g, os.path.join(tmp, f"s{i}"))) as fh:
                 decks.append(json.load(fh))
         ms, info = {}, {}
-        for mode, ns in (("batched", streams), ("sequential", 1)):
+        modes = (("batched", streams), ("sequential", 1)) if sequential else (("batched", streams),)
+        for mode, ns in modes:
             t0 = time.perf_counter()
             st = B.Stacks(decks, streams=ns, tile="auto" if mode == "batched" else None)
             info["setup_s"] = time.perf_counter() - t0
@@ -271,10 +274,10 @@ def stacks_measure(n, chunks=3, seed=0, streams=4):
         "autotune_ms": info["tile_ms"],
         "setup_s": info["setup_s"],
         "ms_per_l1_step_batched": ms["batched"],
-        "ms_per_l1_step_sequential": ms["sequential"],
-        "speedup": ms["sequential"] / ms["batched"],
+        "ms_per_l1_step_sequential": ms.get("sequential"),
+        "speedup": ms["sequential"] / ms["batched"] if sequential else None,
         "scan_mm_per_s_batched": info["mm"] / (ms["batched"] * 1e-3),
-        "scan_mm_per_s_sequential": info["mm"] / (ms["sequential"] * 1e-3),
+        "scan_mm_per_s_sequential": info["mm"] / (ms["sequential"] * 1e-3) if sequential else None,
         "l3_node_updates_per_s_batched": info["l3_nodes"] * info["n23"] / (ms["batched"] * 1e-3),
     }
 
@@ -361,7 +364,8 @@ def main():
         return
     if a.stacks_only:
         print(json.dumps(stacks_measure(a.batch_n, chunks=a.stacks_chunks,
-                                        streams=a.stacks_streams)))
+                                        streams=a.stacks_streams,
+                                        sequential=not a.stacks_no_sequential)))
         return
     import numpy as np
     import workloads as W
