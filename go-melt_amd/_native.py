@@ -177,6 +177,7 @@ _SIGS = {
     "gm_multi_l3": (_I, [_P]),
     "gm_multi_subcycle": (_I, [_P]),
     "gm_multi_set_tile": (_I, [_P, _I, _I]),
+    "gm_multi_set_batch_l2": (_I, [_P, _I]),
     "gm_multi_destroy": (_I, [_P]),
 }
 

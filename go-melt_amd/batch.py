@@ -123,11 +123,12 @@ class Stacks:
     One TimeLoop per deck, the engines dealt over `streams` streams.  Each step gives every
     run its next chunk of N2*N3 toolpath rows; a single-step chunk runs as in
     the plain driver, a subcycled one (subcycleGOMELT cF:3224-3632) stage by
-    stage in lockstep with the others: per L2 substep the L2 stage of each run,
-    then the N3 L3 solves of all subcycling runs as N3 batched launches
-    (gm_multi_subcycle).  Every run ends where its own unbatched run ends."""
+    stage in lockstep with the others: per L2 substep the restriction of each
+    run, then one batched launch each for the L2 solves, the L3 face values,
+    the N3 L3 solves and T' + injection of all subcycling runs
+    (gm_multi_subcycle; batch_l2=False batches the L3 solves only).  Every run ends where its own unbatched run ends."""
 
-    def __init__(self, decks, streams=4, tile="auto"):
+    def __init__(self, decks, streams=4, tile="auto", batch_l2=True):
         import computeFunctions as cf
         import go_melt as G
         N.init_device()
@@ -140,6 +141,7 @@ class Stacks:
             with cf.engines_on_stream(self.streams[i % len(self.streams)]):
                 self.runs.append(G.TimeLoop(d, quiet=True, write_outputs=False))
         self.n = len(self.runs)
+        self.batch_l2 = bool(batch_l2)
         self._pending = []
         for r in self.runs:
             cf.attach_l3_operator(r.Levels, _Deferred(self))
@@ -182,6 +184,7 @@ class Stacks:
             self._multi = (key, h.value)
             if self.tile is not None:
                 N.check(N.lib().gm_multi_set_tile(h.value, int(self.tile[0]), int(self.tile[1])))
+            N.check(N.lib().gm_multi_set_batch_l2(h.value, 1 if self.batch_l2 else 0))
         return self._multi[1]
 
     def step(self):

@@ -1267,9 +1267,9 @@ static float* kc_of(gm_engine* e, const float* T) {
 }
 
 // L2 solve of the subcycle with the nodal properties carried between steps
-static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
-                      const float* corr, float alpha, float* Tout, int dt_index,
-                      const float* F = nullptr, float gscale = 0.0f) {
+static LSArgs l2_args_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
+                         const float* corr, float alpha, float* Tout, int dt_index,
+                         const float* F = nullptr, float gscale = 0.0f) {
     LSArgs a = ls_default();
     a.level = 2;
     a.T = Tin;
@@ -1304,6 +1304,13 @@ static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8
     a.w2 = e->w2[2];
     a.w3 = e->w3[2];
     a.prec_ok = e->dm.pnb > 0;
+    return a;
+}
+
+static void run_l2_kc(gm_engine* e, const float* Tin, const uint8_t* S1in, uint8_t* S1out,
+                      const float* corr, float alpha, float* Tout, int dt_index,
+                      const float* F = nullptr, float gscale = 0.0f) {
+    const LSArgs a = l2_args_kc(e, Tin, S1in, S1out, corr, alpha, Tout, dt_index, F, gscale);
     launch_level_step_kc(a, e->d_desc, e->cfg.level[2], e->d_dm, e->s);
 }
 
@@ -1549,9 +1556,9 @@ static void stage_prologue(gm_engine* e, SubcycleState& st) {
 
 // Part 1 (part = 1, cF:3333-3420) or Part 2 (part = 2, cF:3466-3608) L2 stage
 // of substep k: restriction of the L3 T' terms, L2 solve, L3 face values.
-static void stage_l2(gm_engine* e, SubcycleState& st, int part, int k) {
-    const float fN2 = (float)e->N2;
-    const float* Gk = e->G2 + (size_t)k * e->N3 * 3 * e->gmax[2];
+// the L2 stage before the L2 solve: restriction of the L3 T' terms onto L2
+// and the buffers of the substep's new L2 field
+static void stage_l2_pre(gm_engine* e, SubcycleState& st, int part, int k) {
     if (part == 1) {
         eng_restrict(e, 2, rargs(e, 3, st.T3cur, st.S13cur, st.Tp3cur, nullptr, nullptr,
                                  GM_DT_VALUE, 0, 1.0f));
@@ -1564,22 +1571,47 @@ static void stage_l2(gm_engine* e, SubcycleState& st, int part, int k) {
     }
     st.S12new = e->S12b[st.b2];
     st.b2 ^= 1;
-    l2_substep(e, st.T2cur, st.S12cur, st.S12new, Gk, (float)(k + 1) / fN2, st.T2new, k);
-    launch_face_bc(e->d_desc, 3, 2, e->cfg.level[3], st.T2new, st.T2cur, e->fa3, e->fb3, e->s);
-    // the buffer that will hold the substep's final L3 field; a caller's own
-    // field goes there too unless that is the start field's buffer (Part 1 with
-    // an even N3 ends its solves in the buffer it started from), which must
-    // stay intact for a fallback to the plain solves
+}
+
+// after the L2 solve and the L3 face values: the buffer that will hold the
+// substep's final L3 field; a caller's own field goes there too unless that
+// is the start field's buffer (Part 1 with an even N3 ends its solves in the
+// buffer it started from), which must stay intact for a fallback to the
+// plain solves
+static void stage_l2_post(gm_engine* e, SubcycleState& st, int part, int k) {
     if (part == 1) st.T3out = e->T3b[(st.b3 + e->N3 - 1) & 1];
     else st.T3out = e->L3all[k];
     st.T3ext = st.T3out == st.T3cur ? e->T3x : st.T3out;
 }
 
+// Part 1 (part = 1, cF:3333-3420) or Part 2 (part = 2, cF:3466-3608) L2 stage
+// of substep k: restriction of the L3 T' terms, L2 solve, L3 face values.
+static void stage_l2(gm_engine* e, SubcycleState& st, int part, int k) {
+    const float fN2 = (float)e->N2;
+    const float* Gk = e->G2 + (size_t)k * e->N3 * 3 * e->gmax[2];
+    stage_l2_pre(e, st, part, k);
+    l2_substep(e, st.T2cur, st.S12cur, st.S12new, Gk, (float)(k + 1) / fN2, st.T2new, k);
+    launch_face_bc(e->d_desc, 3, 2, e->cfg.level[3], st.T2new, st.T2cur, e->fa3, e->fb3, e->s);
+    stage_l2_post(e, st, part, k);
+}
+
+// the L2 solve of stage_l2 as arguments of the batched L2 step (the
+// node-aligned path, l2_substep's fast branch), dt by value
+static LSArgs stage_l2_args(gm_engine* e, const SubcycleState& st, int k) {
+    const float gscale = e->w3[3] / (float)e->N3;
+    LSArgs a = l2_args_kc(e, st.T2cur, st.S12cur, st.S12new, nullptr,
+                          (float)(k + 1) / (float)e->N2, st.T2new, k,
+                          e->F2 + (size_t)k * pair_nodes(e->pair[2].p), gscale);
+    a.grid = e->cfg.level[2];
+    a.dt_mode = GM_DT_VALUE;
+    a.dt = e->host.dt2[k];
+    return a;
+}
+
 // L3 stage, closing: T' and injection into the new L2 field, then the
 // substep's state becomes the next one's input.
-static void stage_l3_close(gm_engine* e, SubcycleState& st, int part, int k,
-                           const uint8_t* S13fin, const uint8_t* S23fin, float* T3fin) {
-    new_tprime(e, 3, T3fin, st.T2new, e->histTp[k], st.S12new);
+static void stage_l3_close_state(gm_engine* e, SubcycleState& st, int part, int k,
+                                 const uint8_t* S13fin, const uint8_t* S23fin, float* T3fin) {
     st.T2cur = st.T2new;
     st.S12cur = st.S12new;
     st.T3cur = T3fin;
@@ -1587,6 +1619,29 @@ static void stage_l3_close(gm_engine* e, SubcycleState& st, int part, int k,
     st.S13cur = S13fin;
     if (part == 2) st.S23cur = S23fin;
     st.Tp3cur = e->histTp[k];
+}
+
+static void stage_l3_close(gm_engine* e, SubcycleState& st, int part, int k,
+                           const uint8_t* S13fin, const uint8_t* S23fin, float* T3fin) {
+    new_tprime(e, 3, T3fin, st.T2new, e->histTp[k], st.S12new);
+    stage_l3_close_state(e, st, part, k, S13fin, S23fin, T3fin);
+}
+
+// new_tprime's node-aligned launch as an item of the batched T' + injection
+static TpItem stage_tprime_item(gm_engine* e, const SubcycleState& st, int k, float* T3fin,
+                                int* nblocks) {
+    TpItem t;
+    t.d = e->d_desc;
+    t.fT = T3fin;
+    t.cT = st.T2new;
+    t.Tp = e->histTp[k];
+    t.m = e->d_dm;
+    t.cS1 = st.S12new;
+    t.c_nsub = e->host.sub[2];
+    t.cKC = e->use_kc ? kc_of(e, st.T2new) : nullptr;
+    const GmBox& b = e->host.ov3;
+    *nblocks = tprime_blocks(e->cfg.level[3], (int64_t)b.n[0] * b.n[1] * b.n[2], &t.ntp);
+    return t;
 }
 
 // Buffers of the N3 L3 solves of substep k, solve by solve: Part 1 ping-pongs
@@ -1882,6 +1937,17 @@ struct gm_multi {
     std::vector<hipStream_t> xs;
     std::vector<hipEvent_t> xev;
     hipEvent_t mev = nullptr;
+    // batched L2 stage and T' (multi_substep_batched): per substep one upload
+    // of the L2-step, face and T' items and their block starts (slot layout
+    // at off_*), from a 4-slot pinned ring
+    bool batch_l2 = true;
+    int* d_bst_l2 = nullptr;
+    int nblk_l2 = 0;
+    char* d_sub2 = nullptr;
+    char* h_ring2 = nullptr;
+    hipEvent_t ring2_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int ring2_slot = 0;
+    size_t sub2_bytes = 0, off_l2 = 0, off_face = 0, off_tp = 0, off_bf = 0, off_bt = 0;
 };
 
 extern "C" int gm_multi_destroy(gm_multi* m) {
@@ -1898,6 +1964,14 @@ extern "C" int gm_multi_destroy(gm_multi* m) {
     if (m->d_items) (void)hipFree(m->d_items);
     if (m->d_bstart) (void)hipFree(m->d_bstart);
     if (m->d_bstart1) (void)hipFree(m->d_bstart1);
+    for (auto& ev : m->ring2_ev)
+        if (ev) {
+            (void)hipEventSynchronize(ev);
+            (void)hipEventDestroy(ev);
+        }
+    if (m->h_ring2) (void)hipHostFree(m->h_ring2);
+    if (m->d_sub2) (void)hipFree(m->d_sub2);
+    if (m->d_bst_l2) (void)hipFree(m->d_bst_l2);
     if (m->h_ring) (void)hipHostFree(m->h_ring);
     for (auto& ev : m->ring_ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -1954,27 +2028,55 @@ extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out)
     m->xev.assign(m->xs.size(), nullptr);
     for (auto& ev : m->xev) MCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     MCHK(hipEventCreateWithFlags(&m->mev, hipEventDisableTiming));
+    {
+        std::vector<int> b2(n + 1, 0);
+        for (int i = 0; i < n; ++i) b2[i + 1] = b2[i] + batch_bricks(m->e[i]->cfg.level[2], 1, 2);
+        m->nblk_l2 = b2[n];
+        MCHK(hipMalloc(&m->d_bst_l2, sizeof(int) * (n + 1)));
+        MCHK(hipMemcpy(m->d_bst_l2, b2.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+        size_t o = 0;
+        m->off_l2 = o;
+        o = align256(o + sizeof(BatchItem) * n);
+        m->off_face = o;
+        o = align256(o + sizeof(FaceItem) * n);
+        m->off_tp = o;
+        o = align256(o + sizeof(TpItem) * n);
+        m->off_bf = o;
+        o = align256(o + sizeof(int) * (n + 1));
+        m->off_bt = o;
+        o = align256(o + sizeof(int) * (n + 1));
+        m->sub2_bytes = o;
+        MCHK(hipMalloc(&m->d_sub2, m->sub2_bytes));
+        MCHK(hipHostMalloc((void**)&m->h_ring2, m->sub2_bytes * 4, hipHostMallocDefault));
+        for (auto& ev : m->ring2_ev) MCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
 #undef MCHK
     *out = m;
     return 0;
 }
 
-extern "C" int gm_multi_l3(gm_multi* m) {
-    if (!m) return fail("gm_multi_l3: null");
-    gm_engine* e0 = m->e[0];
-    const int part = e0->stg_part, k = e0->stg_k;
-    for (gm_engine* e : m->e)
-        if (!e->stg_active || e->stg_phase != 1 || e->stg_part != part || e->stg_k != k)
-            return fail("gm_multi_l3: every engine must be between the l2 and l3 stages of "
-                        "the same substep");
-    const int N3 = e0->N3;
-    const float fN3 = (float)N3;
-    std::vector<L3Walk> w(m->n);
-    for (int i = 0; i < m->n; ++i) w[i] = l3_walk_begin(m->e[i]->stg);
+// the other streams' work so far before what m->s runs next
+static int multi_join_in(gm_multi* m) {
     for (size_t i = 0; i < m->xs.size(); ++i) {
         HIPCHK(hipEventRecord(m->xev[i], m->xs[i]));
         HIPCHK(hipStreamWaitEvent(m->s, m->xev[i], 0));
     }
+    return 0;
+}
+
+// m->s's work so far before what the other streams run next
+static int multi_join_out(gm_multi* m) {
+    HIPCHK(hipEventRecord(m->mev, m->s));
+    for (hipStream_t x : m->xs) HIPCHK(hipStreamWaitEvent(x, m->mev, 0));
+    return 0;
+}
+
+// the N3 batched L3 solves of substep (part, k) on m->s; w walks each
+// engine's buffers
+static int multi_l3_solves(gm_multi* m, int part, int k, std::vector<L3Walk>& w) {
+    gm_engine* e0 = m->e[0];
+    const int N3 = e0->N3;
+    const float fN3 = (float)N3;
     for (int j = 0; j < N3; ++j) {
         const int row = k * N3 + j;
         for (int i = 0; i < m->n; ++i) {
@@ -2009,14 +2111,90 @@ extern "C" int gm_multi_l3(gm_multi* m) {
             launch_batch_step_acc(m->d_items, m->d_mats, m->d_bstart, m->n, m->nblocks, m->rec,
                                   e0->d_desc, m->s);
     }
-    HIPCHK(hipEventRecord(m->mev, m->s));
-    for (hipStream_t x : m->xs) HIPCHK(hipStreamWaitEvent(x, m->mev, 0));
+    return 0;
+}
+
+extern "C" int gm_multi_l3(gm_multi* m) {
+    if (!m) return fail("gm_multi_l3: null");
+    gm_engine* e0 = m->e[0];
+    const int part = e0->stg_part, k = e0->stg_k;
+    for (gm_engine* e : m->e)
+        if (!e->stg_active || e->stg_phase != 1 || e->stg_part != part || e->stg_k != k)
+            return fail("gm_multi_l3: every engine must be between the l2 and l3 stages of "
+                        "the same substep");
+    std::vector<L3Walk> w(m->n);
+    for (int i = 0; i < m->n; ++i) w[i] = l3_walk_begin(m->e[i]->stg);
+    if (int rc = multi_join_in(m)) return rc;
+    if (int rc = multi_l3_solves(m, part, k, w)) return rc;
+    if (int rc = multi_join_out(m)) return rc;
     for (int i = 0; i < m->n; ++i) {
         gm_engine* e = m->e[i];
         stage_l3_close(e, e->stg, part, k, w[i].S13in, w[i].S23in, e->stg.T3out);
         stage_advance(e);
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// One whole substep of every engine with the L2 solves, the L3 face values
+// and T' + injection batched too (engines on the node-aligned path): the
+// restrictions per engine on its stream, then on m->s the batched L2 step,
+// face values, N3 L3 solves and T' + injection.
+static int multi_substep_batched(gm_multi* m) {
+    gm_engine* e0 = m->e[0];
+    const int part = e0->stg_part, k = e0->stg_k, n = m->n;
+    const int slot = m->ring2_slot;
+    m->ring2_slot = (slot + 1) & 3;
+    HIPCHK(hipEventSynchronize(m->ring2_ev[slot]));
+    char* h = m->h_ring2 + (size_t)slot * m->sub2_bytes;
+    BatchItem* l2 = (BatchItem*)(h + m->off_l2);
+    FaceItem* fi = (FaceItem*)(h + m->off_face);
+    TpItem* tp = (TpItem*)(h + m->off_tp);
+    int* bf = (int*)(h + m->off_bf);
+    int* bt = (int*)(h + m->off_bt);
+    bf[0] = bt[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        gm_engine* e = m->e[i];
+        SubcycleState& st = e->stg;
+        stage_l2_pre(e, st, part, k);
+        l2[i].a = stage_l2_args(e, st, k);
+        l2[i].m = e->d_dm;
+        memset(l2[i].row, 0, sizeof(l2[i].row));
+        l2[i].w3 = e->w3[2];
+        fi[i] = FaceItem{e->d_desc, st.T2new, st.T2cur, e->fa3, e->fb3};
+        bf[i + 1] = bf[i] + face_bc_blocks(e->cfg.level[3]);
+        stage_l2_post(e, st, part, k);
+        int nb = 0;
+        tp[i] = stage_tprime_item(e, st, k, st.T3out, &nb);
+        bt[i + 1] = bt[i] + nb;
+        e->stg_phase = 1;
+    }
+    if (int rc = multi_join_in(m)) return rc;
+    HIPCHK(hipMemcpyAsync(m->d_sub2, h, m->sub2_bytes, hipMemcpyHostToDevice, m->s));
+    HIPCHK(hipEventRecord(m->ring2_ev[slot], m->s));
+    char* d = m->d_sub2;
+    launch_batch_step_l2((const BatchItem*)(d + m->off_l2), m->d_mats, m->d_bst_l2, n,
+                         m->nblk_l2, m->rec, e0->d_desc, m->s);
+    launch_face_bc_batch((const FaceItem*)(d + m->off_face), (const int*)(d + m->off_bf), n,
+                         bf[n], m->s);
+    std::vector<L3Walk> w(n);
+    for (int i = 0; i < n; ++i) w[i] = l3_walk_begin(m->e[i]->stg);
+    if (int rc = multi_l3_solves(m, part, k, w)) return rc;
+    launch_tprime_batch((const TpItem*)(d + m->off_tp), (const int*)(d + m->off_bt), n, bt[n],
+                        m->s);
+    if (int rc = multi_join_out(m)) return rc;
+    for (int i = 0; i < n; ++i) {
+        gm_engine* e = m->e[i];
+        stage_l3_close_state(e, e->stg, part, k, w[i].S13in, w[i].S23in, e->stg.T3out);
+        stage_advance(e);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gm_multi_set_batch_l2(gm_multi* m, int on) {
+    if (!m) return fail("gm_multi_set_batch_l2: null");
+    m->batch_l2 = on != 0;
     return 0;
 }
 
@@ -2046,7 +2224,13 @@ extern "C" int gm_multi_subcycle(gm_multi* m) {
             return fail("gm_multi_subcycle: every engine must have just begun a staged subcycle "
                         "(gm_engine_stage_begin)");
     const int n2 = m->e[0]->N2;
+    bool batched = m->batch_l2 && m->d_sub2;
+    for (gm_engine* e : m->e) batched = batched && e->fast_restrict && e->use_kc;
     for (int sub = 0; sub < 2 * n2; ++sub) {
+        if (batched) {
+            if (int rc = multi_substep_batched(m)) return rc;
+            continue;
+        }
         for (gm_engine* e : m->e) {
             stage_l2(e, e->stg, e->stg_part, e->stg_k);
             e->stg_phase = 1;

@@ -175,6 +175,28 @@ struct BatchItem {
     float w3;
 };
 
+// L3 face values of one engine (k_face_bc_batch): L2 fields A (new), B (old)
+struct FaceItem {
+    const GmStepDesc* d;
+    const float* A;
+    const float* B;
+    float* fa;
+    float* fb;
+};
+
+// T' and injection of one engine's L3 window into its L2 field (k_tprime_batch)
+struct TpItem {
+    const GmStepDesc* d;
+    const float* fT;
+    float* cT;
+    float* Tp;
+    int ntp;
+    const DevMat* m;
+    const uint8_t* cS1;
+    int64_t c_nsub;
+    float* cKC;
+};
+
 // Restriction (fine -> coarse) arguments.
 struct RArgs {
     const float* T;       // fine temperature (for element k and rho*cp)
@@ -264,6 +286,15 @@ void launch_batch_step(const BatchItem* it, const DevMat* mats, const int* bstar
                        int nblocks, int tile, bool rec, const GmStepDesc* d, hipStream_t s);
 void launch_batch_step_acc(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
                            int nblocks, bool rec, const GmStepDesc* d, hipStream_t s);
+// batched L2 steps (bricks 1 x 2), face values and T' + injection of many engines
+void launch_batch_step_l2(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                          int nblocks, bool rec, const GmStepDesc* d, hipStream_t s);
+void launch_face_bc_batch(const FaceItem* it, const int* bstart, int n, int nblocks,
+                          hipStream_t s);
+int face_bc_blocks(const gm_grid& fine_grid);
+void launch_tprime_batch(const TpItem* it, const int* bstart, int n, int nblocks,
+                         hipStream_t s);
+int tprime_blocks(const gm_grid& fine_grid, int64_t nbox, int* ntp_out);
 void launch_step_tile(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
                       int tile, bool rec, hipStream_t s);
 DevMat make_devmat(const gm_material& m);

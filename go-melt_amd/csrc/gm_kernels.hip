@@ -783,6 +783,25 @@ __global__ void __launch_bounds__(64) k_step_dpp_batch_acc(const BatchItem* __re
 #undef GM_STEP_LANE
 #undef GM_STEP_BRICK
 }
+// The same for the L2 steps of the subcycle (no source, no melt time; the
+// restricted terms from each item's FP64 node box), bricks of 1 row x 2
+// planes as the single launch; items carry dt by value.
+template <bool REC>
+__global__ void __launch_bounds__(64) k_step_dpp_batch_l2(const BatchItem* __restrict__ it,
+                                                          const DevMat* __restrict__ mats,
+                                                          const int* __restrict__ bstart, int n,
+                                                          const GmStepDesc* __restrict__ d) {
+    constexpr bool SRC = false, ACC = false;
+    constexpr int YB = 1, ZC = 2;
+    GM_BATCH_FIND_INSTANCE
+#define GM_STEP_BRICK tb
+#define GM_STEP_LANE (int)threadIdx.x
+#define GM_STEP_ROW d->rows[a.src_row < 0 ? 0 : a.src_row]
+#include "gm_step_dpp_body.inc"
+#undef GM_STEP_ROW
+#undef GM_STEP_LANE
+#undef GM_STEP_BRICK
+}
 #undef GM_BATCH_FIND_INSTANCE
 
 // L1 step (computeL1Temperature, cF:2135-2204 with the L1 faces of
@@ -1119,6 +1138,17 @@ void launch_batch_step_acc(const BatchItem* it, const DevMat* mats, const int* b
                            bstart, n, d);
 }
 
+void launch_batch_step_l2(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                          int nblocks, bool rec, const GmStepDesc* d, hipStream_t s) {
+    if (n <= 0 || nblocks <= 0) return;
+    if (rec)
+        hipLaunchKernelGGL(k_step_dpp_batch_l2<true>, dim3(nblocks), dim3(64), 0, s, it, mats,
+                           bstart, n, d);
+    else
+        hipLaunchKernelGGL(k_step_dpp_batch_l2<false>, dim3(nblocks), dim3(64), 0, s, it, mats,
+                           bstart, n, d);
+}
+
 void launch_batch_step(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
                        int nblocks, int tile, bool rec, const GmStepDesc* d, hipStream_t s) {
     if (n <= 0 || nblocks <= 0) return;
@@ -1287,13 +1317,13 @@ void launch_acc_l3(const GmStepDesc* d, float* accum, float* max_accum, float* a
 // A and B prolonged separately (interpolatePointsMatrix rule, cF:1060-1104)
 // onto full-size arrays at the face nodes (cF:1598-1620); the level step
 // blends them as alpha*A + (1-alpha)*B.
-__global__ void k_face_bc(const GmStepDesc* __restrict__ d, int fine, int coarse,
-                          const float* __restrict__ A, const float* __restrict__ B,
-                          float* __restrict__ fa, float* __restrict__ fb) {
+__device__ __forceinline__ void face_bc_at(const GmStepDesc* __restrict__ d, int fine,
+                                           int coarse, const float* __restrict__ A,
+                                           const float* __restrict__ B, float* __restrict__ fa,
+                                           float* __restrict__ fb, int t) {
     const gm_grid& g = d->g[fine];
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int nW = ny * nz, nS = nx * nz, nB = nx * ny;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
     int ix, iy, iz;
     if (t < 2 * nW) {
         ix = t < nW ? 0 : nx - 1;
@@ -1324,6 +1354,38 @@ __global__ void k_face_bc(const GmStepDesc* __restrict__ d, int fine, int coarse
     const int idx = ix + nx * iy + nx * ny * iz;
     fa[idx] = sa;
     fb[idx] = sb;
+}
+
+__global__ void k_face_bc(const GmStepDesc* __restrict__ d, int fine, int coarse,
+                          const float* __restrict__ A, const float* __restrict__ B,
+                          float* __restrict__ fa, float* __restrict__ fb) {
+    face_bc_at(d, fine, coarse, A, B, fa, fb, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Face values of many engines' L3 windows from their L2 fields in one
+// launch: bstart[i] is the first block of item i (256 threads per block).
+__global__ void __launch_bounds__(256) k_face_bc_batch(const FaceItem* __restrict__ it,
+                                                       const int* __restrict__ bstart, int n) {
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const FaceItem x = it[lo];
+    face_bc_at(x.d, 3, 2, x.A, x.B, x.fa, x.fb, (b - bstart[lo]) * 256 + (int)threadIdx.x);
+}
+
+void launch_face_bc_batch(const FaceItem* it, const int* bstart, int n, int nblocks,
+                          hipStream_t s) {
+    if (n <= 0 || nblocks <= 0) return;
+    hipLaunchKernelGGL(k_face_bc_batch, dim3(nblocks), dim3(256), 0, s, it, bstart, n);
+}
+
+int face_bc_blocks(const gm_grid& gf) {
+    const int n = 2 * gf.n[1] * gf.n[2] + 2 * gf.n[0] * gf.n[2] + gf.n[0] * gf.n[1];
+    return (n + 255) / 256;
 }
 
 void launch_face_bc(const GmStepDesc* d, int fine, int coarse, const gm_grid& gf, const float* A,
@@ -2342,11 +2404,13 @@ __device__ __forceinline__ void tp_axis(int i, int f0, int R, float invR, int bl
     in1 = c1 - blo >= 0 && c1 - blo < bn && j1 >= 0 && j1 < nf;
 }
 
-__global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level,
-                                int coarse_level, const float* __restrict__ fT,
-                                float* __restrict__ cT, float* __restrict__ Tp, int ntp_blocks,
-                                const DevMat* __restrict__ m, const uint8_t* __restrict__ cS1,
-                                int64_t c_nsub, float* __restrict__ cKC) {
+__device__ __forceinline__ void tprime_inject_blk(const GmStepDesc* __restrict__ d,
+                                                  int fine_level, int coarse_level,
+                                                  const float* __restrict__ fT,
+                                                  float* __restrict__ cT, float* __restrict__ Tp,
+                                                  int ntp_blocks, const DevMat* __restrict__ m,
+                                                  const uint8_t* __restrict__ cS1, int64_t c_nsub,
+                                                  float* __restrict__ cKC, int blk) {
     const gm_grid& fg = d->g[fine_level];
     const gm_grid& cg = d->g[coarse_level];
     const GmBox& b = fine_level == 3 ? d->ov3 : d->ov2;
@@ -2361,15 +2425,15 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
         fr[a] = (int)rintf(cg.h[a] * ih);
     }
     const int fnx = fg.n[0], fny = fg.n[1], fnxy = fnx * fny;
-    if ((int)blockIdx.x < ntp_blocks) {
+    if (blk < ntp_blocks) {
         // T' = T_fine - I(T_coarse after injection) at the fine nodes of one
         // plane (z uniform in the block): with the coarse nodes on every R-th
         // fine node, the coarse element of a fine node and its trilinear
         // weights follow from the fine index; corners inside the injected box
         // take the fine value (cF:2060-2099).
         const int nqb = (fnxy + 255) / 256;
-        const int iz = blockIdx.x / nqb;
-        const int q = (blockIdx.x - iz * nqb) * 256 + threadIdx.x;
+        const int iz = blk / nqb;
+        const int q = (blk - iz * nqb) * 256 + threadIdx.x;
         if (q >= fnxy) return;
         const int iy = (int)((unsigned)q / (unsigned)fnx), ix = q - iy * fnx;
         int cx0, cx1, cy0, cy1, cz0, cz1, jx0, jx1, jy0, jy1, jz0, jz1;
@@ -2418,7 +2482,7 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
         stage_tables(m, sTab, threadIdx.x, blockDim.x);
         __syncthreads();
     }
-    const int k = ((int)blockIdx.x - ntp_blocks) * 256 + threadIdx.x;
+    const int k = (blk - ntp_blocks) * 256 + threadIdx.x;
     const int nb = b.n[0] * b.n[1] * b.n[2];
     if (k >= nb) return;
     const int bz = k / (b.n[0] * b.n[1]), rem = k - bz * (b.n[0] * b.n[1]);
@@ -2439,6 +2503,44 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
         cKC[ci] = kk;
         cKC[nnc + ci] = cc;
     }
+}
+
+__global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level,
+                                int coarse_level, const float* __restrict__ fT,
+                                float* __restrict__ cT, float* __restrict__ Tp, int ntp_blocks,
+                                const DevMat* __restrict__ m, const uint8_t* __restrict__ cS1,
+                                int64_t c_nsub, float* __restrict__ cKC) {
+    tprime_inject_blk(d, fine_level, coarse_level, fT, cT, Tp, ntp_blocks, m, cS1, c_nsub, cKC,
+                      (int)blockIdx.x);
+}
+
+// T' and injection of many engines' L3 windows into their L2 fields in one
+// launch: bstart[i] is the first block of item i.
+__global__ void __launch_bounds__(256) k_tprime_batch(const TpItem* __restrict__ it,
+                                                      const int* __restrict__ bstart, int n) {
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const TpItem x = it[lo];
+    tprime_inject_blk(x.d, 3, 2, x.fT, x.cT, x.Tp, x.ntp, x.m, x.cS1, x.c_nsub, x.cKC,
+                      b - bstart[lo]);
+}
+
+void launch_tprime_batch(const TpItem* it, const int* bstart, int n, int nblocks,
+                         hipStream_t s) {
+    if (n <= 0 || nblocks <= 0) return;
+    hipLaunchKernelGGL(k_tprime_batch, dim3(nblocks), dim3(256), 0, s, it, bstart, n);
+}
+
+int tprime_blocks(const gm_grid& fine_grid, int64_t nbox, int* ntp_out) {
+    const int nxy = fine_grid.n[0] * fine_grid.n[1];
+    const int ntp = ((nxy + 255) / 256) * fine_grid.n[2];
+    if (ntp_out) *ntp_out = ntp;
+    return ntp + (int)((nbox + 255) / 256);
 }
 
 void launch_tprime_inject(const GmStepDesc* d, int fine_level, int coarse_level,

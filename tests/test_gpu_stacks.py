@@ -44,12 +44,15 @@ def _alone(deck, chunks):
     return run
 
 
-def test_stacks_equal_their_own_runs(tmp_path, monkeypatch):
+@pytest.mark.parametrize("batch_l2", [True, False])
+def test_stacks_equal_their_own_runs(tmp_path, monkeypatch, batch_l2):
+    """batch_l2: the L2 solves, face values and T' + injection batched across
+    the stacks too, or only the L3 solves."""
     import batch as B
     monkeypatch.delenv("GOMELT_NO_GRAPH", raising=False)
     decks = _decks(tmp_path)
     chunks = 4  # one single-step chunk, then three subcycled L1 steps
-    st = B.Stacks(decks)
+    st = B.Stacks(decks, batch_l2=batch_l2)
     for _ in range(chunks):
         assert st.step()
     st.sync()
