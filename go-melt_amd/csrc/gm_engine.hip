@@ -1922,7 +1922,7 @@ struct gm_multi {
     std::vector<gm_engine*> e;
     DevMat* d_mats = nullptr;
     BatchItem* d_items = nullptr;
-    BatchItem* h_ring = nullptr;  // pinned, 4 slots of n items
+    BatchItem* h_ring = nullptr;  // pinned, 4 slots of the N3 x n items of a substep
     hipEvent_t ring_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int ring_slot = 0;
     int* d_bstart = nullptr;  // block starts of the 1 x 3 bricks (Part 2)
@@ -1931,7 +1931,6 @@ struct gm_multi {
     int* d_bstart1 = nullptr;    // its block starts
     int nblocks1 = 0;
     bool rec = false;
-    std::vector<BatchItem> items;
     // the engines' other streams: the batched solves on s wait for their
     // L2 stages, their L3 closes wait for the batched solves
     std::vector<hipStream_t> xs;
@@ -2011,7 +2010,6 @@ extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out)
         st[i + 1] = st[i] + batch_bricks(g, 1, 3);
     }
     m->nblocks = st[n];
-    m->items.resize(n);
 #define MCHK(call)                                                                      \
     do {                                                                                \
         hipError_t e_ = (call);                                                         \
@@ -2019,11 +2017,12 @@ extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out)
             return bad(fail(std::string(#call) + ": " + hipGetErrorString(e_)));        \
     } while (0)
     MCHK(hipMalloc(&m->d_mats, sizeof(DevMat) * n));
-    MCHK(hipMalloc(&m->d_items, sizeof(BatchItem) * n));
+    MCHK(hipMalloc(&m->d_items, sizeof(BatchItem) * n * e0->N3));
     MCHK(hipMalloc(&m->d_bstart, sizeof(int) * (n + 1)));
     MCHK(hipMemcpy(m->d_mats, hm.data(), sizeof(DevMat) * n, hipMemcpyHostToDevice));
     MCHK(hipMemcpy(m->d_bstart, st.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-    MCHK(hipHostMalloc((void**)&m->h_ring, sizeof(BatchItem) * n * 4, hipHostMallocDefault));
+    MCHK(hipHostMalloc((void**)&m->h_ring, sizeof(BatchItem) * n * e0->N3 * 4,
+                       hipHostMallocDefault));
     for (auto& ev : m->ring_ev) MCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     m->xev.assign(m->xs.size(), nullptr);
     for (auto& ev : m->xev) MCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -2075,14 +2074,19 @@ static int multi_join_out(gm_multi* m) {
 // engine's buffers
 static int multi_l3_solves(gm_multi* m, int part, int k, std::vector<L3Walk>& w) {
     gm_engine* e0 = m->e[0];
-    const int N3 = e0->N3;
+    const int N3 = e0->N3, n = m->n;
     const float fN3 = (float)N3;
+    // the items of all N3 solves, built in a pinned slot and uploaded at once
+    const int slot = m->ring_slot;
+    m->ring_slot = (slot + 1) & 3;
+    HIPCHK(hipEventSynchronize(m->ring_ev[slot]));
+    BatchItem* h = m->h_ring + (size_t)slot * n * N3;
     for (int j = 0; j < N3; ++j) {
         const int row = k * N3 + j;
-        for (int i = 0; i < m->n; ++i) {
+        for (int i = 0; i < n; ++i) {
             gm_engine* e = m->e[i];
             const L3StepBufs b = l3_walk_step(e, e->stg, w[i], part, k, j);
-            BatchItem& it = m->items[i];
+            BatchItem& it = h[(size_t)j * n + i];
             it.a = l3_args_kc(e, b.Tin, b.S1in, b.S1out, row, (float)(j + 1) / fN3, b.Tout, b.S2in,
                               b.S2out);
             it.a.grid = e->cfg.level[3];  // node counts (a move changes the origin only)
@@ -2093,23 +2097,21 @@ static int multi_l3_solves(gm_multi* m, int part, int k, std::vector<L3Walk>& w)
             it.a.dt = it.row[5];
             it.w3 = e->w3[3];
         }
-        const int slot = m->ring_slot;
-        m->ring_slot = (slot + 1) & 3;
-        HIPCHK(hipEventSynchronize(m->ring_ev[slot]));
-        BatchItem* h = m->h_ring + (size_t)slot * m->n;
-        memcpy(h, m->items.data(), sizeof(BatchItem) * m->n);
-        HIPCHK(hipMemcpyAsync(m->d_items, h, sizeof(BatchItem) * m->n, hipMemcpyHostToDevice,
-                              m->s));
-        HIPCHK(hipEventRecord(m->ring_ev[slot], m->s));
+    }
+    HIPCHK(hipMemcpyAsync(m->d_items, h, sizeof(BatchItem) * n * N3, hipMemcpyHostToDevice,
+                          m->s));
+    HIPCHK(hipEventRecord(m->ring_ev[slot], m->s));
+    for (int j = 0; j < N3; ++j) {
+        const BatchItem* it = m->d_items + (size_t)j * n;
         if (part == 1 && m->d_bstart1)
-            launch_batch_step(m->d_items, m->d_mats, m->d_bstart1, m->n, m->nblocks1, m->tile1,
-                              m->rec, e0->d_desc, m->s);
-        else if (part == 1)
-            launch_batch_step(m->d_items, m->d_mats, m->d_bstart, m->n, m->nblocks, 1, m->rec,
+            launch_batch_step(it, m->d_mats, m->d_bstart1, n, m->nblocks1, m->tile1, m->rec,
                               e0->d_desc, m->s);
+        else if (part == 1)
+            launch_batch_step(it, m->d_mats, m->d_bstart, n, m->nblocks, 1, m->rec, e0->d_desc,
+                              m->s);
         else
-            launch_batch_step_acc(m->d_items, m->d_mats, m->d_bstart, m->n, m->nblocks, m->rec,
-                                  e0->d_desc, m->s);
+            launch_batch_step_acc(it, m->d_mats, m->d_bstart, n, m->nblocks, m->rec, e0->d_desc,
+                                  m->s);
     }
     return 0;
 }
