@@ -725,29 +725,50 @@ __global__ void __launch_bounds__(64 * WPB) k_step_dpp(LSArgs a, const GmStepDes
 #undef GM_STEP_BRICK
 }
 
+// Item of block b in a batched launch: the last i < n with bstart[i] <= b
+// (bstart non-decreasing from bstart[0] = 0, so an item with no blocks is
+// skipped), and its first block *b0.  One load of 64 starts per lane group
+// and a wave ballot instead of a chain of dependent loads: b is uniform, so
+// every lane of the wave gets the same answer.  Called by every wave of the
+// block.
+__device__ __forceinline__ int batch_item_of(const int* __restrict__ bstart, int n, int b,
+                                             int* b0) {
+    const int lane = (int)(threadIdx.x & 63);
+    int cnt = 0, vlast = 0, clast = 0;
+    for (int c = 0; c < n; c += 64) {
+        const int i = c + lane;
+        const int v = i < n ? bstart[i] : 0x7fffffff;
+        const unsigned long long bal = __ballot(v <= b);
+        if (bal) {
+            cnt += __popcll(bal);
+            vlast = v;
+            clast = c;
+        }
+    }
+    const int lo = cnt - 1;
+    *b0 = __builtin_amdgcn_readlane(vlast, lo - clast);
+    return lo;
+}
+
 // Batched L3-type steps (source, surface flux, Dirichlet faces, no
 // correction): many instances of their own mesh size and laser in one launch.
 // bstart[i] is the first block of instance i (bstart[n] = total); mats[i] is
 // its material (the same as it[i].m).  The blocks are dealt out XCD-aware as
-// in the single launch, then each finds its instance (uniform binary search)
-// and runs that instance's brick.  Items carry their time step by value
+// in the single launch, then each finds its instance (batch_item_of) and
+// runs that instance's brick.  Items carry their time step by value
 // (dt_mode GM_DT_VALUE); the desc's rows are read only under GM_DT_ROW.
 // Both kernels include the body directly (no shared device function), so
 // each compiles to the same instruction DAG as the single launch of the same
 // template arguments and an instance's result equals its own launch bitwise.
 #define GM_BATCH_FIND_INSTANCE                                           \
     const int b = xcd_tile(blockIdx.x, gridDim.x);                       \
-    int lo = 0, hi = n - 1;                                              \
-    while (lo < hi) {                                                    \
-        const int mid = (lo + hi + 1) >> 1;                              \
-        if (bstart[mid] <= b) lo = mid;                                  \
-        else hi = mid - 1;                                               \
-    }                                                                    \
+    int b0_;                                                             \
+    const int lo = batch_item_of(bstart, n, b, &b0_);                    \
     const LSArgs a = it[lo].a;                                           \
     /* materials indexed off the kernel argument (a known global pointer: \
        scalar loads), not through the pointer stored in the item */      \
     const DevMat* __restrict__ m = mats + lo;                            \
-    const int tb = b - bstart[lo];
+    const int tb = b - b0_;
 
 template <bool REC, int YB, int ZC>
 __global__ void __launch_bounds__(64) k_step_dpp_batch(const BatchItem* __restrict__ it,
@@ -1367,14 +1388,10 @@ __global__ void k_face_bc(const GmStepDesc* __restrict__ d, int fine, int coarse
 __global__ void __launch_bounds__(256) k_face_bc_batch(const FaceItem* __restrict__ it,
                                                        const int* __restrict__ bstart, int n) {
     const int b = (int)blockIdx.x;
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (bstart[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
+    int b0;
+    const int lo = batch_item_of(bstart, n, b, &b0);
     const FaceItem x = it[lo];
-    face_bc_at(x.d, 3, 2, x.A, x.B, x.fa, x.fb, (b - bstart[lo]) * 256 + (int)threadIdx.x);
+    face_bc_at(x.d, 3, 2, x.A, x.B, x.fa, x.fb, (b - b0) * 256 + (int)threadIdx.x);
 }
 
 void launch_face_bc_batch(const FaceItem* it, const int* bstart, int n, int nblocks,
@@ -2519,15 +2536,10 @@ __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level
 __global__ void __launch_bounds__(256) k_tprime_batch(const TpItem* __restrict__ it,
                                                       const int* __restrict__ bstart, int n) {
     const int b = (int)blockIdx.x;
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (bstart[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
+    int b0;
+    const int lo = batch_item_of(bstart, n, b, &b0);
     const TpItem x = it[lo];
-    tprime_inject_blk(x.d, 3, 2, x.fT, x.cT, x.Tp, x.ntp, x.m, x.cS1, x.c_nsub, x.cKC,
-                      b - bstart[lo]);
+    tprime_inject_blk(x.d, 3, 2, x.fT, x.cT, x.Tp, x.ntp, x.m, x.cS1, x.c_nsub, x.cKC, b - b0);
 }
 
 void launch_tprime_batch(const TpItem* it, const int* bstart, int n, int nblocks,
