@@ -2408,14 +2408,15 @@ void launch_tprime(const GmStepDesc* d, int fine_level, int coarse_level, const 
 __device__ __forceinline__ void tp_axis(int i, int f0, int R, float invR, int blo, int bn, int nc,
                                         int nf, int& c0, int& c1, float& w, bool& in0, bool& in1,
                                         int& j0, int& j1) {
+    // 24-bit products (full rate): indices and ratios stay far below 2^23
     const int rel = i - f0 + R;                           // >= 1: f0 < R
     const int q = (int)(((float)rel + 0.5f) * invR);      // floor(rel / R), rel small
-    w = (float)(rel - q * R) * invR;
+    w = (float)(rel - __mul24(q, R)) * invR;
     c0 = blo + q - 1;                                     // coarse node at or below node i
     c0 = c0 < 0 ? 0 : (c0 > nc - 1 ? nc - 1 : c0);        // inside the mesh when aligned
     c1 = c0 + 1 < nc ? c0 + 1 : nc - 1;                   // weight w = 0 when clamped
-    j0 = f0 + (c0 - blo) * R;
-    j1 = f0 + (c1 - blo) * R;
+    j0 = f0 + __mul24(c0 - blo, R);
+    j1 = f0 + __mul24(c1 - blo, R);
     // injected: in the overlap box and on a fine node
     in0 = c0 - blo >= 0 && c0 - blo < bn && j0 >= 0 && j0 < nf;
     in1 = c1 - blo >= 0 && c1 - blo < bn && j1 >= 0 && j1 < nf;
@@ -2432,27 +2433,38 @@ __device__ __forceinline__ void tprime_inject_blk(const GmStepDesc* __restrict__
     const gm_grid& cg = d->g[coarse_level];
     const GmBox& b = fine_level == 3 ? d->ov3 : d->ov2;
     const gm_grid& xyz = fine_level == 3 ? d->ov3_xyz : d->ov2_xyz;
-    // fine node of box node 0 and the fine stride of one coarse spacing
+    // fine node of box node 0 and the fine stride of one coarse spacing, the
+    // same for every lane: kept in scalar registers
     int f0[3], fr[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         // integer offset and ratio: one hardware reciprocal is exact enough to round
         const float ih = __builtin_amdgcn_rcpf(fg.h[a]);
-        f0[a] = (int)rintf((xyz.x0[a] - fg.x0[a]) * ih);
-        fr[a] = (int)rintf(cg.h[a] * ih);
+        f0[a] = __builtin_amdgcn_readfirstlane((int)rintf((xyz.x0[a] - fg.x0[a]) * ih));
+        fr[a] = __builtin_amdgcn_readfirstlane((int)rintf(cg.h[a] * ih));
     }
-    const int fnx = fg.n[0], fny = fg.n[1], fnxy = fnx * fny;
+    // node counts in scalar registers (read through a pointer loaded from
+    // memory in the batched launch, where the compiler cannot see that they
+    // are the same in every lane): buffer resources and scalar offsets are
+    // built from them
+    const int fnx = __builtin_amdgcn_readfirstlane(fg.n[0]);
+    const int fny = __builtin_amdgcn_readfirstlane(fg.n[1]);
+    const int fnz = __builtin_amdgcn_readfirstlane(fg.n[2]);
+    const int fnxy = fnx * fny;
     if (blk < ntp_blocks) {
         // T' = T_fine - I(T_coarse after injection) at the fine nodes of one
         // plane (z uniform in the block): with the coarse nodes on every R-th
         // fine node, the coarse element of a fine node and its trilinear
         // weights follow from the fine index; corners inside the injected box
-        // take the fine value (cF:2060-2099).
+        // take the fine value (cF:2060-2099).  Buffer loads with 32-bit byte
+        // offsets, the plane's part of each offset in a scalar register.
         const int nqb = (fnxy + 255) / 256;
         const int iz = blk / nqb;
         const int q = (blk - iz * nqb) * 256 + threadIdx.x;
         if (q >= fnxy) return;
-        const int iy = (int)((unsigned)q / (unsigned)fnx), ix = q - iy * fnx;
+        const int iy = fnxy < (1 << 20) ? fdiv_small(q, __builtin_amdgcn_rcpf((float)fnx))
+                                        : (int)((unsigned)q / (unsigned)fnx);
+        const int ix = q - iy * fnx;
         int cx0, cx1, cy0, cy1, cz0, cz1, jx0, jx1, jy0, jy1, jz0, jz1;
         float wx, wy, wz;
         bool ix0, ix1, iy0, iy1, iz0, iz1;
@@ -2462,34 +2474,57 @@ __device__ __forceinline__ void tprime_inject_blk(const GmStepDesc* __restrict__
                 iy0, iy1, jy0, jy1);
         tp_axis(iz, f0[2], fr[2], __builtin_amdgcn_rcpf((float)fr[2]), b.lo[2], b.n[2], cg.n[2], fg.n[2], cz0, cz1, wz,
                 iz0, iz1, jz0, jz1);
-        const int t = q + fnxy * iz;
+        // the z axis is the block's: its results to scalar registers
+        cz0 = __builtin_amdgcn_readfirstlane(cz0);
+        cz1 = __builtin_amdgcn_readfirstlane(cz1);
+        jz0 = __builtin_amdgcn_readfirstlane(jz0);
+        jz1 = __builtin_amdgcn_readfirstlane(jz1);
+        wz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wz)));
+        iz0 = __builtin_amdgcn_readfirstlane((int)iz0) != 0;
+        iz1 = __builtin_amdgcn_readfirstlane((int)iz1) != 0;
+        const uint32_t nf4 = 4u * (uint32_t)fnxy * (uint32_t)fnz;
+        const auto rF = buf_rsrc(fT, nf4);
+        const uint32_t zt = 4u * (uint32_t)(fnxy * iz);
+        const uint32_t z0 = 4u * (uint32_t)(fnxy * jz0), z1 = 4u * (uint32_t)(fnxy * jz1);
+        const uint32_t y0 = 4u * __umul24((unsigned)fnx, (unsigned)jy0);
+        const uint32_t y1 = 4u * __umul24((unsigned)fnx, (unsigned)jy1);
+        const uint32_t x0 = 4u * (uint32_t)jx0, x1 = 4u * (uint32_t)jx1;
         float v;
         if (ix0 && ix1 && iy0 && iy1 && iz0 && iz1) {
             // all corners injected (inside the window): fine values only
-            const int r00 = fnx * jy0 + fnxy * jz0, r10 = fnx * jy1 + fnxy * jz0;
-            const int r01 = fnx * jy0 + fnxy * jz1, r11 = fnx * jy1 + fnxy * jz1;
-            const float a00 = (1.0f - wx) * fT[jx0 + r00] + wx * fT[jx1 + r00];
-            const float a10 = (1.0f - wx) * fT[jx0 + r10] + wx * fT[jx1 + r10];
-            const float a01 = (1.0f - wx) * fT[jx0 + r01] + wx * fT[jx1 + r01];
-            const float a11 = (1.0f - wx) * fT[jx0 + r11] + wx * fT[jx1 + r11];
+            const float a00 = (1.0f - wx) * buf_ld(rF, x0 + y0, z0) + wx * buf_ld(rF, x1 + y0, z0);
+            const float a10 = (1.0f - wx) * buf_ld(rF, x0 + y1, z0) + wx * buf_ld(rF, x1 + y1, z0);
+            const float a01 = (1.0f - wx) * buf_ld(rF, x0 + y0, z1) + wx * buf_ld(rF, x1 + y0, z1);
+            const float a11 = (1.0f - wx) * buf_ld(rF, x0 + y1, z1) + wx * buf_ld(rF, x1 + y1, z1);
             v = (1.0f - wz) * ((1.0f - wy) * a00 + wy * a10) + wz * ((1.0f - wy) * a01 + wy * a11);
         } else {
-            const int cnx = cg.n[0], cnxy = cg.n[0] * cg.n[1];
+            // corners outside the injected box read the coarse field (whose
+            // clamped node indices are inside it); both loads are issued, the
+            // fine one of such a corner at an offset past the end (zero)
+            const int cnx = __builtin_amdgcn_readfirstlane(cg.n[0]);
+            const int cnxy = cnx * __builtin_amdgcn_readfirstlane(cg.n[1]);
+            const auto rC =
+                buf_rsrc(cT, 4u * (uint32_t)cnxy * (uint32_t)__builtin_amdgcn_readfirstlane(cg.n[2]));
+            const uint32_t cy0o = 4u * __umul24((unsigned)cnx, (unsigned)cy0);
+            const uint32_t cy1o = 4u * __umul24((unsigned)cnx, (unsigned)cy1);
+            const uint32_t cz0o = 4u * (uint32_t)(cnxy * cz0), cz1o = 4u * (uint32_t)(cnxy * cz1);
             v = 0.0f;
 #pragma unroll
             for (int qd = 0; qd < 8; ++qd) {
                 const int dx = qd & 1, dy = (qd >> 1) & 1, dz = qd >> 2;
                 const bool inj = (dx ? ix1 : ix0) && (dy ? iy1 : iy0) && (dz ? iz1 : iz0);
-                const float u = inj ? fT[(dx ? jx1 : jx0) + fnx * (dy ? jy1 : jy0) +
-                                         fnxy * (dz ? jz1 : jz0)]
-                                    : cT[(dx ? cx1 : cx0) + cnx * (dy ? cy1 : cy0) +
-                                         cnxy * (dz ? cz1 : cz0)];
+                // an injected corner is a node of the fine field
+                const float uf =
+                    buf_ld(rF, inj ? (dx ? x1 : x0) + (dy ? y1 : y0) + (dz ? z1 : z0) : GM_OOB);
+                const float uc = buf_ld(rC, 4u * (uint32_t)(dx ? cx1 : cx0) + (dy ? cy1o : cy0o),
+                                        dz ? cz1o : cz0o);
+                const float u = inj ? uf : uc;
                 const float N =
                     (dx ? wx : 1.0f - wx) * (dy ? wy : 1.0f - wy) * (dz ? wz : 1.0f - wz);
                 v += N * u;
             }
         }
-        Tp[t] = fT[t] - v;
+        buf_st(buf_ld(rF, 4u * (uint32_t)q, zt) - v, buf_rsrc(Tp, nf4), 4u * (uint32_t)q, zt);
         return;
     }
     // injection of the fine values into the coarse overlap box; the coarse
