@@ -49,6 +49,28 @@ def test_batched_equals_unbatched(B, tile):
     assert melted > 0
 
 
+def test_batched_equals_unbatched_past_64_instances(B):
+    """A batch of more than 64 instances (the block-to-instance lookup then
+    spans more than one group of 64 block starts), small meshes of mixed
+    sizes: every instance equals its own launch bitwise."""
+    sizes = [(8, 8, 4), (12, 10, 8), (16, 12, 6), (10, 14, 4)]
+    inst = _instances(131, seed=5, sizes=sizes)
+    a = B.Batch.from_instances(inst)
+    b = B.Batch.from_instances(inst)
+    for tile in ((1, 3), (2, 3)):
+        a.set_tile(*tile)
+        b.set_tile(*tile)
+        for s in range(8):
+            rows = W.batch_rows(inst, s)
+            a.step(rows, batched=True)
+            b.step(rows, batched=False)
+    for i in range(len(inst)):
+        Ta, Tb = a.T(i), b.T(i)
+        assert np.array_equal(Ta, Tb), (i, float(np.max(np.abs(Ta - Tb))))
+        assert np.array_equal(a.S1(i), b.S1(i)), i
+        assert Ta.max() > 300.0, i
+
+
 def test_batch_brick_shapes_agree(B):
     """Every brick shape computes the same nodes with the same arithmetic:
     equal to FP32 rounding (1e-6 of the rise)."""
