@@ -294,6 +294,7 @@ void launch_face_bc_batch(const FaceItem* it, const int* bstart, int n, int nblo
 int face_bc_blocks(const gm_grid& fine_grid);
 void launch_tprime_batch(const TpItem* it, const int* bstart, int n, int nblocks,
                          hipStream_t s);
+// blocks of one item in launch_tprime_batch; *ntp_out: its T' blocks of the single launch
 int tprime_blocks(const gm_grid& fine_grid, int64_t nbox, int* ntp_out);
 void launch_step_tile(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,
                       int tile, bool rec, hipStream_t s);

@@ -2422,132 +2422,141 @@ __device__ __forceinline__ void tp_axis(int i, int f0, int R, float invR, int bl
     in1 = c1 - blo >= 0 && c1 - blo < bn && j1 >= 0 && j1 < nf;
 }
 
-__device__ __forceinline__ void tprime_inject_blk(const GmStepDesc* __restrict__ d,
-                                                  int fine_level, int coarse_level,
-                                                  const float* __restrict__ fT,
-                                                  float* __restrict__ cT, float* __restrict__ Tp,
-                                                  int ntp_blocks, const DevMat* __restrict__ m,
-                                                  const uint8_t* __restrict__ cS1, int64_t c_nsub,
-                                                  float* __restrict__ cKC, int blk) {
+// Geometry of a fine/coarse pair for T' and injection, the same for every
+// lane: kept in scalar registers (in the batched launch the descriptor is
+// read through a pointer loaded from memory, where the compiler cannot see
+// that the values are uniform; buffer resources and scalar offsets are built
+// from them).
+struct TpGeom {
+    int f0[3], fr[3];  // fine node of overlap-box node 0; fine nodes per coarse spacing
+    int fn[3], cn[3];  // fine and coarse node counts
+    int blo[3], bn[3]; // overlap box in coarse nodes
+};
+
+__device__ __forceinline__ TpGeom tp_geom(const GmStepDesc* __restrict__ d, int fine_level,
+                                          int coarse_level) {
     const gm_grid& fg = d->g[fine_level];
     const gm_grid& cg = d->g[coarse_level];
     const GmBox& b = fine_level == 3 ? d->ov3 : d->ov2;
     const gm_grid& xyz = fine_level == 3 ? d->ov3_xyz : d->ov2_xyz;
-    // fine node of box node 0 and the fine stride of one coarse spacing, the
-    // same for every lane: kept in scalar registers
-    int f0[3], fr[3];
+    TpGeom g;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         // integer offset and ratio: one hardware reciprocal is exact enough to round
         const float ih = __builtin_amdgcn_rcpf(fg.h[a]);
-        f0[a] = __builtin_amdgcn_readfirstlane((int)rintf((xyz.x0[a] - fg.x0[a]) * ih));
-        fr[a] = __builtin_amdgcn_readfirstlane((int)rintf(cg.h[a] * ih));
+        g.f0[a] = __builtin_amdgcn_readfirstlane((int)rintf((xyz.x0[a] - fg.x0[a]) * ih));
+        g.fr[a] = __builtin_amdgcn_readfirstlane((int)rintf(cg.h[a] * ih));
+        g.fn[a] = __builtin_amdgcn_readfirstlane(fg.n[a]);
+        g.cn[a] = __builtin_amdgcn_readfirstlane(cg.n[a]);
+        g.blo[a] = __builtin_amdgcn_readfirstlane(b.lo[a]);
+        g.bn[a] = __builtin_amdgcn_readfirstlane(b.n[a]);
     }
-    // node counts in scalar registers (read through a pointer loaded from
-    // memory in the batched launch, where the compiler cannot see that they
-    // are the same in every lane): buffer resources and scalar offsets are
-    // built from them
-    const int fnx = __builtin_amdgcn_readfirstlane(fg.n[0]);
-    const int fny = __builtin_amdgcn_readfirstlane(fg.n[1]);
-    const int fnz = __builtin_amdgcn_readfirstlane(fg.n[2]);
-    const int fnxy = fnx * fny;
-    if (blk < ntp_blocks) {
-        // T' = T_fine - I(T_coarse after injection) at the fine nodes of one
-        // plane (z uniform in the block): with the coarse nodes on every R-th
-        // fine node, the coarse element of a fine node and its trilinear
-        // weights follow from the fine index; corners inside the injected box
-        // take the fine value (cF:2060-2099).  Buffer loads with 32-bit byte
-        // offsets, the plane's part of each offset in a scalar register.
-        const int nqb = (fnxy + 255) / 256;
-        const int iz = blk / nqb;
-        const int q = (blk - iz * nqb) * 256 + threadIdx.x;
-        if (q >= fnxy) return;
-        const int iy = fnxy < (1 << 20) ? fdiv_small(q, __builtin_amdgcn_rcpf((float)fnx))
-                                        : (int)((unsigned)q / (unsigned)fnx);
-        const int ix = q - iy * fnx;
-        int cx0, cx1, cy0, cy1, cz0, cz1, jx0, jx1, jy0, jy1, jz0, jz1;
-        float wx, wy, wz;
-        bool ix0, ix1, iy0, iy1, iz0, iz1;
-        tp_axis(ix, f0[0], fr[0], __builtin_amdgcn_rcpf((float)fr[0]), b.lo[0], b.n[0], cg.n[0], fg.n[0], cx0, cx1, wx,
-                ix0, ix1, jx0, jx1);
-        tp_axis(iy, f0[1], fr[1], __builtin_amdgcn_rcpf((float)fr[1]), b.lo[1], b.n[1], cg.n[1], fg.n[1], cy0, cy1, wy,
-                iy0, iy1, jy0, jy1);
-        tp_axis(iz, f0[2], fr[2], __builtin_amdgcn_rcpf((float)fr[2]), b.lo[2], b.n[2], cg.n[2], fg.n[2], cz0, cz1, wz,
-                iz0, iz1, jz0, jz1);
-        // the z axis is the block's: its results to scalar registers
-        cz0 = __builtin_amdgcn_readfirstlane(cz0);
-        cz1 = __builtin_amdgcn_readfirstlane(cz1);
-        jz0 = __builtin_amdgcn_readfirstlane(jz0);
-        jz1 = __builtin_amdgcn_readfirstlane(jz1);
-        wz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wz)));
-        iz0 = __builtin_amdgcn_readfirstlane((int)iz0) != 0;
-        iz1 = __builtin_amdgcn_readfirstlane((int)iz1) != 0;
-        const uint32_t nf4 = 4u * (uint32_t)fnxy * (uint32_t)fnz;
-        const auto rF = buf_rsrc(fT, nf4);
-        const uint32_t zt = 4u * (uint32_t)(fnxy * iz);
-        const uint32_t z0 = 4u * (uint32_t)(fnxy * jz0), z1 = 4u * (uint32_t)(fnxy * jz1);
-        const uint32_t y0 = 4u * __umul24((unsigned)fnx, (unsigned)jy0);
-        const uint32_t y1 = 4u * __umul24((unsigned)fnx, (unsigned)jy1);
-        const uint32_t x0 = 4u * (uint32_t)jx0, x1 = 4u * (uint32_t)jx1;
-        float v;
-        if (ix0 && ix1 && iy0 && iy1 && iz0 && iz1) {
-            // all corners injected (inside the window): fine values only
-            const float a00 = (1.0f - wx) * buf_ld(rF, x0 + y0, z0) + wx * buf_ld(rF, x1 + y0, z0);
-            const float a10 = (1.0f - wx) * buf_ld(rF, x0 + y1, z0) + wx * buf_ld(rF, x1 + y1, z0);
-            const float a01 = (1.0f - wx) * buf_ld(rF, x0 + y0, z1) + wx * buf_ld(rF, x1 + y0, z1);
-            const float a11 = (1.0f - wx) * buf_ld(rF, x0 + y1, z1) + wx * buf_ld(rF, x1 + y1, z1);
-            v = (1.0f - wz) * ((1.0f - wy) * a00 + wy * a10) + wz * ((1.0f - wy) * a01 + wy * a11);
-        } else {
-            // corners outside the injected box read the coarse field (whose
-            // clamped node indices are inside it); both loads are issued, the
-            // fine one of such a corner at an offset past the end (zero)
-            const int cnx = __builtin_amdgcn_readfirstlane(cg.n[0]);
-            const int cnxy = cnx * __builtin_amdgcn_readfirstlane(cg.n[1]);
-            const auto rC =
-                buf_rsrc(cT, 4u * (uint32_t)cnxy * (uint32_t)__builtin_amdgcn_readfirstlane(cg.n[2]));
-            const uint32_t cy0o = 4u * __umul24((unsigned)cnx, (unsigned)cy0);
-            const uint32_t cy1o = 4u * __umul24((unsigned)cnx, (unsigned)cy1);
-            const uint32_t cz0o = 4u * (uint32_t)(cnxy * cz0), cz1o = 4u * (uint32_t)(cnxy * cz1);
-            v = 0.0f;
+    return g;
+}
+
+// T' = T_fine - I(T_coarse after injection) at the fine nodes of block blk
+// of a plane (z uniform in the block): with the coarse nodes on every R-th
+// fine node, the coarse element of a fine node and its trilinear weights
+// follow from the fine index; corners inside the injected box take the fine
+// value (cF:2060-2099).  Buffer loads with 32-bit byte offsets, the plane's
+// part of each offset in a scalar register.
+__device__ __forceinline__ void tprime_blk(const TpGeom& g, const float* __restrict__ fT,
+                                           const float* __restrict__ cT, float* __restrict__ Tp,
+                                           int blk) {
+    const int fnx = g.fn[0], fnxy = g.fn[0] * g.fn[1];
+    const int nqb = (fnxy + 255) / 256;
+    const int iz = blk / nqb;
+    const int q = (blk - iz * nqb) * 256 + threadIdx.x;
+    if (q >= fnxy) return;
+    const int iy = fnxy < (1 << 20) ? fdiv_small(q, __builtin_amdgcn_rcpf((float)fnx))
+                                    : (int)((unsigned)q / (unsigned)fnx);
+    const int ix = q - iy * fnx;
+    int cx0, cx1, cy0, cy1, cz0, cz1, jx0, jx1, jy0, jy1, jz0, jz1;
+    float wx, wy, wz;
+    bool ix0, ix1, iy0, iy1, iz0, iz1;
+    tp_axis(ix, g.f0[0], g.fr[0], __builtin_amdgcn_rcpf((float)g.fr[0]), g.blo[0], g.bn[0],
+            g.cn[0], g.fn[0], cx0, cx1, wx, ix0, ix1, jx0, jx1);
+    tp_axis(iy, g.f0[1], g.fr[1], __builtin_amdgcn_rcpf((float)g.fr[1]), g.blo[1], g.bn[1],
+            g.cn[1], g.fn[1], cy0, cy1, wy, iy0, iy1, jy0, jy1);
+    tp_axis(iz, g.f0[2], g.fr[2], __builtin_amdgcn_rcpf((float)g.fr[2]), g.blo[2], g.bn[2],
+            g.cn[2], g.fn[2], cz0, cz1, wz, iz0, iz1, jz0, jz1);
+    // the z axis is the block's: its results to scalar registers
+    cz0 = __builtin_amdgcn_readfirstlane(cz0);
+    cz1 = __builtin_amdgcn_readfirstlane(cz1);
+    jz0 = __builtin_amdgcn_readfirstlane(jz0);
+    jz1 = __builtin_amdgcn_readfirstlane(jz1);
+    wz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wz)));
+    iz0 = __builtin_amdgcn_readfirstlane((int)iz0) != 0;
+    iz1 = __builtin_amdgcn_readfirstlane((int)iz1) != 0;
+    const uint32_t nf4 = 4u * (uint32_t)fnxy * (uint32_t)g.fn[2];
+    const auto rF = buf_rsrc(fT, nf4);
+    const uint32_t zt = 4u * (uint32_t)(fnxy * iz);
+    const uint32_t z0 = 4u * (uint32_t)(fnxy * jz0), z1 = 4u * (uint32_t)(fnxy * jz1);
+    const uint32_t y0 = 4u * __umul24((unsigned)fnx, (unsigned)jy0);
+    const uint32_t y1 = 4u * __umul24((unsigned)fnx, (unsigned)jy1);
+    const uint32_t x0 = 4u * (uint32_t)jx0, x1 = 4u * (uint32_t)jx1;
+    float v;
+    if (ix0 && ix1 && iy0 && iy1 && iz0 && iz1) {
+        // all corners injected (inside the window): fine values only
+        const float a00 = (1.0f - wx) * buf_ld(rF, x0 + y0, z0) + wx * buf_ld(rF, x1 + y0, z0);
+        const float a10 = (1.0f - wx) * buf_ld(rF, x0 + y1, z0) + wx * buf_ld(rF, x1 + y1, z0);
+        const float a01 = (1.0f - wx) * buf_ld(rF, x0 + y0, z1) + wx * buf_ld(rF, x1 + y0, z1);
+        const float a11 = (1.0f - wx) * buf_ld(rF, x0 + y1, z1) + wx * buf_ld(rF, x1 + y1, z1);
+        v = (1.0f - wz) * ((1.0f - wy) * a00 + wy * a10) + wz * ((1.0f - wy) * a01 + wy * a11);
+    } else {
+        // corners outside the injected box read the coarse field (whose
+        // clamped node indices are inside it); both loads are issued, the
+        // fine one of such a corner at an offset past the end (zero)
+        const int cnx = g.cn[0], cnxy = g.cn[0] * g.cn[1];
+        const auto rC = buf_rsrc(cT, 4u * (uint32_t)cnxy * (uint32_t)g.cn[2]);
+        const uint32_t cy0o = 4u * __umul24((unsigned)cnx, (unsigned)cy0);
+        const uint32_t cy1o = 4u * __umul24((unsigned)cnx, (unsigned)cy1);
+        const uint32_t cz0o = 4u * (uint32_t)(cnxy * cz0), cz1o = 4u * (uint32_t)(cnxy * cz1);
+        v = 0.0f;
 #pragma unroll
-            for (int qd = 0; qd < 8; ++qd) {
-                const int dx = qd & 1, dy = (qd >> 1) & 1, dz = qd >> 2;
-                const bool inj = (dx ? ix1 : ix0) && (dy ? iy1 : iy0) && (dz ? iz1 : iz0);
-                // an injected corner is a node of the fine field
-                const float uf =
-                    buf_ld(rF, inj ? (dx ? x1 : x0) + (dy ? y1 : y0) + (dz ? z1 : z0) : GM_OOB);
-                const float uc = buf_ld(rC, 4u * (uint32_t)(dx ? cx1 : cx0) + (dy ? cy1o : cy0o),
-                                        dz ? cz1o : cz0o);
-                const float u = inj ? uf : uc;
-                const float N =
-                    (dx ? wx : 1.0f - wx) * (dy ? wy : 1.0f - wy) * (dz ? wz : 1.0f - wz);
-                v += N * u;
-            }
+        for (int qd = 0; qd < 8; ++qd) {
+            const int dx = qd & 1, dy = (qd >> 1) & 1, dz = qd >> 2;
+            const bool inj = (dx ? ix1 : ix0) && (dy ? iy1 : iy0) && (dz ? iz1 : iz0);
+            // an injected corner is a node of the fine field
+            const float uf =
+                buf_ld(rF, inj ? (dx ? x1 : x0) + (dy ? y1 : y0) + (dz ? z1 : z0) : GM_OOB);
+            const float uc = buf_ld(rC, 4u * (uint32_t)(dx ? cx1 : cx0) + (dy ? cy1o : cy0o),
+                                    dz ? cz1o : cz0o);
+            const float u = inj ? uf : uc;
+            const float N =
+                (dx ? wx : 1.0f - wx) * (dy ? wy : 1.0f - wy) * (dz ? wz : 1.0f - wz);
+            v += N * u;
         }
-        buf_st(buf_ld(rF, 4u * (uint32_t)q, zt) - v, buf_rsrc(Tp, nf4), 4u * (uint32_t)q, zt);
-        return;
     }
-    // injection of the fine values into the coarse overlap box; the coarse
-    // level's carried k, cp follow the injected temperatures
+    buf_st(buf_ld(rF, 4u * (uint32_t)q, zt) - v, buf_rsrc(Tp, nf4), 4u * (uint32_t)q, zt);
+}
+
+// Injection of the fine values into the coarse overlap box, block ib of the
+// box's nodes; the coarse level's carried k, cp follow the injected
+// temperatures.
+__device__ __forceinline__ void inject_blk(const TpGeom& g, const float* __restrict__ fT,
+                                           float* __restrict__ cT, const DevMat* __restrict__ m,
+                                           const uint8_t* __restrict__ cS1, int64_t c_nsub,
+                                           float* __restrict__ cKC, int ib) {
     __shared__ float sTab[PTW] __attribute__((aligned(16)));
     if (cKC) {
         stage_tables(m, sTab, threadIdx.x, blockDim.x);
         __syncthreads();
     }
-    const int k = (blk - ntp_blocks) * 256 + threadIdx.x;
-    const int nb = b.n[0] * b.n[1] * b.n[2];
+    const int k = ib * 256 + threadIdx.x;
+    const int nb = g.bn[0] * g.bn[1] * g.bn[2];
     if (k >= nb) return;
-    const int bz = k / (b.n[0] * b.n[1]), rem = k - bz * (b.n[0] * b.n[1]);
-    const int by = rem / b.n[0], bx = rem - by * b.n[0];
-    const int cx = b.lo[0] + bx, cy = b.lo[1] + by, cz = b.lo[2] + bz;
-    if (cx < 0 || cx >= cg.n[0] || cy < 0 || cy >= cg.n[1] || cz < 0 || cz >= cg.n[2]) return;
-    const int jx = f0[0] + bx * fr[0], jy = f0[1] + by * fr[1], jz = f0[2] + bz * fr[2];
-    if (jx < 0 || jx >= fg.n[0] || jy < 0 || jy >= fg.n[1] || jz < 0 || jz >= fg.n[2]) return;
-    const int64_t ci = nid(cx, cy, cz, cg.n[0], cg.n[1]);
-    const float v = fT[jx + fnx * jy + fnxy * jz];
+    const int bz = k / (g.bn[0] * g.bn[1]), rem = k - bz * (g.bn[0] * g.bn[1]);
+    const int by = rem / g.bn[0], bx = rem - by * g.bn[0];
+    const int cx = g.blo[0] + bx, cy = g.blo[1] + by, cz = g.blo[2] + bz;
+    if (cx < 0 || cx >= g.cn[0] || cy < 0 || cy >= g.cn[1] || cz < 0 || cz >= g.cn[2]) return;
+    const int jx = g.f0[0] + bx * g.fr[0], jy = g.f0[1] + by * g.fr[1], jz = g.f0[2] + bz * g.fr[2];
+    if (jx < 0 || jx >= g.fn[0] || jy < 0 || jy >= g.fn[1] || jz < 0 || jz >= g.fn[2]) return;
+    const int64_t ci = nid(cx, cy, cz, g.cn[0], g.cn[1]);
+    const float v = fT[jx + g.fn[0] * jy + g.fn[0] * g.fn[1] * jz];
     cT[ci] = v;
     if (cKC) {
-        const int64_t nnc = (int64_t)cg.n[0] * cg.n[1] * cg.n[2];
+        const int64_t nnc = (int64_t)g.cn[0] * g.cn[1] * g.cn[2];
         const PropC pc = load_pc(m);
         float kk, cc;
         bool s1;
@@ -2557,24 +2566,43 @@ __device__ __forceinline__ void tprime_inject_blk(const GmStepDesc* __restrict__
     }
 }
 
+// Blocks below ntp_blocks: T' (one plane's 256 fine nodes each); the rest
+// inject (256 overlap-box nodes each).
 __global__ void k_tprime_inject(const GmStepDesc* __restrict__ d, int fine_level,
                                 int coarse_level, const float* __restrict__ fT,
                                 float* __restrict__ cT, float* __restrict__ Tp, int ntp_blocks,
                                 const DevMat* __restrict__ m, const uint8_t* __restrict__ cS1,
                                 int64_t c_nsub, float* __restrict__ cKC) {
-    tprime_inject_blk(d, fine_level, coarse_level, fT, cT, Tp, ntp_blocks, m, cS1, c_nsub, cKC,
-                      (int)blockIdx.x);
+    const TpGeom g = tp_geom(d, fine_level, coarse_level);
+    const int blk = (int)blockIdx.x;
+    if (blk < ntp_blocks) tprime_blk(g, fT, cT, Tp, blk);
+    else inject_blk(g, fT, cT, m, cS1, c_nsub, cKC, blk - ntp_blocks);
 }
 
 // T' and injection of many engines' L3 windows into their L2 fields in one
-// launch: bstart[i] is the first block of item i.
+// launch: bstart[i] is the first block of item i.  A block runs
+// GM_TP_PER_BLOCK consecutive T' blocks of the single launch (the item
+// lookup and the descriptor reads once for all of them, their loads in
+// flight together), or one injection block.
+#define GM_TP_PER_BLOCK 4
 __global__ void __launch_bounds__(256) k_tprime_batch(const TpItem* __restrict__ it,
                                                       const int* __restrict__ bstart, int n) {
     const int b = (int)blockIdx.x;
     int b0;
     const int lo = batch_item_of(bstart, n, b, &b0);
     const TpItem x = it[lo];
-    tprime_inject_blk(x.d, 3, 2, x.fT, x.cT, x.Tp, x.ntp, x.m, x.cS1, x.c_nsub, x.cKC, b - b0);
+    const TpGeom g = tp_geom(x.d, 3, 2);
+    const int t = b - b0;
+    const int ntpb = (x.ntp + GM_TP_PER_BLOCK - 1) / GM_TP_PER_BLOCK;
+    if (t < ntpb) {
+#pragma unroll
+        for (int s = 0; s < GM_TP_PER_BLOCK; ++s) {
+            const int blk = t * GM_TP_PER_BLOCK + s;
+            if (blk < x.ntp) tprime_blk(g, x.fT, x.cT, x.Tp, blk);
+        }
+    } else {
+        inject_blk(g, x.fT, x.cT, x.m, x.cS1, x.c_nsub, x.cKC, t - ntpb);
+    }
 }
 
 void launch_tprime_batch(const TpItem* it, const int* bstart, int n, int nblocks,
@@ -2587,7 +2615,7 @@ int tprime_blocks(const gm_grid& fine_grid, int64_t nbox, int* ntp_out) {
     const int nxy = fine_grid.n[0] * fine_grid.n[1];
     const int ntp = ((nxy + 255) / 256) * fine_grid.n[2];
     if (ntp_out) *ntp_out = ntp;
-    return ntp + (int)((nbox + 255) / 256);
+    return (ntp + GM_TP_PER_BLOCK - 1) / GM_TP_PER_BLOCK + (int)((nbox + 255) / 256);
 }
 
 void launch_tprime_inject(const GmStepDesc* d, int fine_level, int coarse_level,
