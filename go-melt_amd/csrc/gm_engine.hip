@@ -1916,14 +1916,20 @@ extern "C" int gm_engine_stage_l3(gm_engine* e, int external) {
 // lockstep; at each L2 substep the N3 L3 solves of all of them go out as N3
 // launches of the batched step (one launch per solve for every engine).
 
+// Pinned item slots of a gm_multi: the host may run this many substeps ahead
+// of the GPU before it waits (a whole L1 step of N2 <= 16 substeps per part,
+// so that the driver's host work for the next L1 step overlaps the GPU's
+// substeps of this one).
+constexpr int kMultiRing = 32;
+
 struct gm_multi {
     int n = 0;
     hipStream_t s = nullptr;
     std::vector<gm_engine*> e;
     DevMat* d_mats = nullptr;
     BatchItem* d_items = nullptr;
-    BatchItem* h_ring = nullptr;  // pinned, 4 slots of the N3 x n items of a substep
-    hipEvent_t ring_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    BatchItem* h_ring = nullptr;  // pinned, kMultiRing slots of the N3 x n items of a substep
+    hipEvent_t ring_ev[kMultiRing] = {};
     int ring_slot = 0;
     int* d_bstart = nullptr;  // block starts of the 1 x 3 bricks (Part 2)
     int nblocks = 0;
@@ -1938,13 +1944,13 @@ struct gm_multi {
     hipEvent_t mev = nullptr;
     // batched L2 stage and T' (multi_substep_batched): per substep one upload
     // of the L2-step, face and T' items and their block starts (slot layout
-    // at off_*), from a 4-slot pinned ring
+    // at off_*), from a pinned ring of kMultiRing slots
     bool batch_l2 = true;
     int* d_bst_l2 = nullptr;
     int nblk_l2 = 0;
     char* d_sub2 = nullptr;
     char* h_ring2 = nullptr;
-    hipEvent_t ring2_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ring2_ev[kMultiRing] = {};
     int ring2_slot = 0;
     size_t sub2_bytes = 0, off_l2 = 0, off_face = 0, off_tp = 0, off_bf = 0, off_bt = 0;
 };
@@ -2021,7 +2027,7 @@ extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out)
     MCHK(hipMalloc(&m->d_bstart, sizeof(int) * (n + 1)));
     MCHK(hipMemcpy(m->d_mats, hm.data(), sizeof(DevMat) * n, hipMemcpyHostToDevice));
     MCHK(hipMemcpy(m->d_bstart, st.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-    MCHK(hipHostMalloc((void**)&m->h_ring, sizeof(BatchItem) * n * e0->N3 * 4,
+    MCHK(hipHostMalloc((void**)&m->h_ring, sizeof(BatchItem) * n * e0->N3 * kMultiRing,
                        hipHostMallocDefault));
     for (auto& ev : m->ring_ev) MCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     m->xev.assign(m->xs.size(), nullptr);
@@ -2046,7 +2052,7 @@ extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out)
         o = align256(o + sizeof(int) * (n + 1));
         m->sub2_bytes = o;
         MCHK(hipMalloc(&m->d_sub2, m->sub2_bytes));
-        MCHK(hipHostMalloc((void**)&m->h_ring2, m->sub2_bytes * 4, hipHostMallocDefault));
+        MCHK(hipHostMalloc((void**)&m->h_ring2, m->sub2_bytes * kMultiRing, hipHostMallocDefault));
         for (auto& ev : m->ring2_ev) MCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     }
 #undef MCHK
@@ -2078,7 +2084,7 @@ static int multi_l3_solves(gm_multi* m, int part, int k, std::vector<L3Walk>& w)
     const float fN3 = (float)N3;
     // the items of all N3 solves, built in a pinned slot and uploaded at once
     const int slot = m->ring_slot;
-    m->ring_slot = (slot + 1) & 3;
+    m->ring_slot = (slot + 1) % kMultiRing;
     HIPCHK(hipEventSynchronize(m->ring_ev[slot]));
     BatchItem* h = m->h_ring + (size_t)slot * n * N3;
     for (int j = 0; j < N3; ++j) {
@@ -2146,7 +2152,7 @@ static int multi_substep_batched(gm_multi* m) {
     gm_engine* e0 = m->e[0];
     const int part = e0->stg_part, k = e0->stg_k, n = m->n;
     const int slot = m->ring2_slot;
-    m->ring2_slot = (slot + 1) & 3;
+    m->ring2_slot = (slot + 1) % kMultiRing;
     HIPCHK(hipEventSynchronize(m->ring2_ev[slot]));
     char* h = m->h_ring2 + (size_t)slot * m->sub2_bytes;
     BatchItem* l2 = (BatchItem*)(h + m->off_l2);
