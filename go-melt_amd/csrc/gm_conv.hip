@@ -15,7 +15,8 @@
 // a second kernel that reduces over voxels (split over blocks, FP32 atomics).
 //
 // Tiles: a block of 4 waves computes 64 x 64 outputs; each wave a 32 x 32
-// quarter as 2 x 2 MFMA tiles of 16 x 16; K advances through LDS one tap and
+// quarter as 2 x 2 MFMA tiles of 16 x 16 (k_conv_fwd32 on large volumes with
+// up to 32 outputs: waves of 64 voxels x 32 outputs); K advances through LDS one tap and
 // 32 input channels at a time (k_conv_fwd32, software-pipelined) or, for 16
 // input channels, 16 at a time (k_conv_fwd).
 
@@ -149,25 +150,34 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvDims c, const uint16_t* __
 
 // The same convolution for cin a multiple of 32, software-pipelined: a K
 // stage is G sub-stages of one tap x 32 input channels (one
-// v_mfma_f32_16x16x32_bf16 k-step each, 4 MFMAs per wave and sub-stage), each
-// thread stages 16 bytes of A and of B per sub-stage, and the next stage's
-// global loads are in flight while the current stage computes.  Sub-stages
-// past the last tap are zero.  CI16: cin = 16, and a sub-stage is two taps x
-// 16 channels (14 sub-stages, the 28th tap zero).
+// v_mfma_f32_16x16x32_bf16 k-step each), each thread stages 16 bytes of A and
+// of B per sub-stage and row pass, and the next stage's global loads are in
+// flight while the current stage computes.  Sub-stages past the last tap are
+// zero.  CI16: cin = 16, and a sub-stage is two taps x 16 channels (14
+// sub-stages, the 28th tap zero).
+//
+// Tiles: BM voxels x BN outputs per block, four waves of WM x WN, each wave
+// (WM/16) x (WN/16) MFMA tiles of 16 x 16.  Larger wave tiles read fewer LDS
+// bytes per MFMA (a 64 x 64 wave tile reads 128 bytes per lane for 16 MFMAs,
+// a 32 x 32 one 64 bytes for 4).  Every output's K sum runs through the same
+// stages in the same order whatever the tile shape, so the shapes give the
+// same bits.
 //
 // Split K (part != nullptr): block z of gridDim.z runs stages z*spb ..
 // z*spb + spb - 1 and writes its raw sums to part[z][vox][out]; the output
 // is then k_conv_part_sum's fixed-order sum of the parts.
-template <int BM, int BN, bool CI16 = false, int G = 1>
+template <int BM, int BN, bool CI16 = false, int G = 1, int WM = 32, int WN = 32>
 __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* __restrict__ X,
                                                     const uint16_t* __restrict__ Wk,
                                                     const float* __restrict__ bias, int relu,
                                                     float* __restrict__ Y,
                                                     uint16_t* __restrict__ Ybf,
                                                     float* __restrict__ part, int spb) {
-    // BM voxels x BN outputs per block, 4 waves of 32 x 32 (BM/32 x BN/32 = 4)
-    static_assert((BM / 32) * (BN / 32) == 4, "four waves of 32 x 32");
-    constexpr int NA = BM / 64;  // 16-byte A loads per thread and sub-stage
+    static_assert((BM / WM) * (BN / WN) == 4, "four waves");
+    static_assert(BM % 64 == 0 && WM % 16 == 0 && WN % 16 == 0, "tile shape");
+    constexpr int NA = BM / 64;             // A rows per thread (row passes)
+    constexpr int NB = BN < 64 ? 1 : BN / 64;  // B row passes
+    constexpr int TM = WM / 16, TN = WN / 16;
     constexpr int KW = G * 32 + 8;
     __shared__ uint16_t sA[BM][KW];
     __shared__ uint16_t sB[BN][KW];
@@ -178,12 +188,12 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
     const int mt = tile / gridDim.y;
     const int64_t m0 = (int64_t)mt * BM;
     const int n0 = (tile - mt * gridDim.y) * BN;
-    const int wm = (wave % (BM / 32)) * 32, wn = (wave / (BM / 32)) * 32;
-    v4f acc[2][2];
+    const int wm = (wave % (BM / WM)) * WM, wn = (wave / (BM / WM)) * WN;
+    v4f acc[TM][TN];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int b = 0; b < TN; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     // staging: 16 bytes (8 bf16) of row r, k (t%4)*8 .. +7, rows r = t/4 + 64 q
     const int srow = tid >> 2, sk = (tid & 3) * 8;
     Vox pv[NA];
@@ -193,13 +203,12 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
         vrow[q] = m0 + srow + 64 * q;
         pv[q] = vox_decode(c, vrow[q] < c.nvox ? vrow[q] : 0);
     }
-    const int o = n0 + srow;  // B rows: threads with srow < BN
     const int kch = CI16 ? 1 : c.ci / 32;
     const int nsub = CI16 ? 14 : 27 * kch;
     const int nst = (nsub + G - 1) / G;
     const int st0 = (int)blockIdx.z * spb;
     const int st1 = st0 + spb < nst ? st0 + spb : nst;
-    auto fetch = [&](int st, uint4 (*av)[NA], uint4* bv) {
+    auto fetch = [&](int st, uint4 (*av)[NA], uint4 (*bv)[NB]) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int sub = st * G + g;
@@ -218,12 +227,16 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
                 av[g][q] = make_uint4(0u, 0u, 0u, 0u);
                 if (vs >= 0) av[g][q] = *(const uint4*)(X + vs * c.ci + i0);
             }
-            bv[g] = make_uint4(0u, 0u, 0u, 0u);
-            if (srow < BN && o < c.co && tv)
-                bv[g] = *(const uint4*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const int br = srow + 64 * q, o = n0 + br;  // B row (output)
+                bv[g][q] = make_uint4(0u, 0u, 0u, 0u);
+                if (br < BN && o < c.co && tv)
+                    bv[g][q] = *(const uint4*)(Wk + ((int64_t)tap * c.co + o) * c.ci + i0);
+            }
         }
     };
-    uint4 av[G][NA], bv[G];
+    uint4 av[G][NA], bv[G][NB];
     fetch(st0, av, bv);
     const int fi = lane & 15, fk8 = (lane >> 4) * 8;
     for (int st = st0; st < st1; ++st) {
@@ -232,7 +245,9 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
         for (int g = 0; g < G; ++g) {
 #pragma unroll
             for (int q = 0; q < NA; ++q) *(uint4*)&sA[srow + 64 * q][g * 32 + sk] = av[g][q];
-            if (srow < BN) *(uint4*)&sB[srow][g * 32 + sk] = bv[g];
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+                if (srow + 64 * q < BN) *(uint4*)&sB[srow + 64 * q][g * 32 + sk] = bv[g][q];
         }
         __syncthreads();
         if (st + 1 < st1) fetch(st + 1, av, bv);
@@ -241,27 +256,28 @@ __global__ void __launch_bounds__(256) k_conv_fwd32(ConvDims c, const uint16_t* 
         // positions for both operands
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            v8bf fa[2], fb[2];
+            v8bf fa[TM], fb[TN];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
+            for (int t = 0; t < TM; ++t)
                 fa[t] = __builtin_bit_cast(v8bf,
                                            *(const uint4*)&sA[wm + t * 16 + fi][g * 32 + fk8]);
+#pragma unroll
+            for (int t = 0; t < TN; ++t)
                 fb[t] = __builtin_bit_cast(v8bf,
                                            *(const uint4*)&sB[wn + t * 16 + fi][g * 32 + fk8]);
-            }
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < TM; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int b = 0; b < TN; ++b)
                     acc[a][b] =
                         __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
     }
     const int dj = lane & 15, di = (lane >> 4) * 4;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < TN; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t v = m0 + wm + a * 16 + di + r;
@@ -296,7 +312,7 @@ __global__ void k_conv_part_sum(const float* __restrict__ part, int S, int64_t n
 // Launch plan of the pipelined forward kernel: tile shape, sub-stages per
 // barrier pair and the K split.
 struct FwdPlan {
-    bool fwd32 = false, ci16 = false, small = false;
+    bool fwd32 = false, ci16 = false, small = false, wide = false;
     int bm = 64, bn = 64;
     dim3 grid;
     int nst = 0;     // stages of the whole K
@@ -322,6 +338,17 @@ static FwdPlan plan_fwd(const ConvDims& c, bool al16) {
     // then hide more of each stage's latency (batch-1 forward 0.86 -> 0.73
     // ms).  With many voxels one sub-stage is faster (more resident blocks).
     p.small = c.nvox < ((int64_t)1 << 18);
+    // Many voxels and up to 32 outputs: blocks of 256 voxels x 32 outputs,
+    // each wave 64 x 32 (fewer LDS bytes per MFMA; 8-11% faster on 2^21
+    // voxels).  Wider outputs keep 64 x 64 blocks of 32 x 32 waves: 256 x 64
+    // and 128 x 128 blocks of 64 x 64 waves measured 5-10% slower and the
+    // same there.
+    p.wide = !p.small && c.co <= 32;
+    if (p.wide) {
+        p.bm = 256;
+        p.bn = 32;
+        p.grid = dim3((unsigned)((c.nvox + 255) / 256), 1);
+    }
     const int G = p.small ? 3 : 1;
     const int nsub = p.ci16 ? 14 : 27 * (c.ci / 32);
     p.nst = (nsub + G - 1) / G;
@@ -354,6 +381,21 @@ static void launch_fwd32(const FwdPlan& p, const ConvDims& c, const uint16_t* X,
     else
         hipLaunchKernelGGL((k_conv_fwd32<BM, BN, CI16, 1>), g, dim3(256), 0, s, c, X, Wk, bias,
                            relu, Y, Ybf, part, spb);
+}
+
+// many voxels, up to 32 outputs: 256 x 32 blocks of 64 x 32 waves, one
+// sub-stage per barrier pair
+static void launch_fwd32_wide(const FwdPlan& p, const ConvDims& c, const uint16_t* X,
+                              const uint16_t* Wk, const float* bias, int relu, float* Y,
+                              uint16_t* Ybf, float* part, bool ci16, hipStream_t s) {
+    const dim3 g(p.grid.x, p.grid.y, part ? p.splits : 1);
+    const int spb = part ? p.spb : p.nst;
+    if (ci16)
+        hipLaunchKernelGGL((k_conv_fwd32<256, 32, true, 1, 64, 32>), g, dim3(256), 0, s, c, X,
+                           Wk, bias, relu, Y, Ybf, part, spb);
+    else
+        hipLaunchKernelGGL((k_conv_fwd32<256, 32, false, 1, 64, 32>), g, dim3(256), 0, s, c, X,
+                           Wk, bias, relu, Y, Ybf, part, spb);
 }
 
 // Wt[tap][i][o] = Wk[26 - tap][o][i] (weights of the input-gradient pass)
@@ -661,7 +703,10 @@ static void launch_plan(const FwdPlan& p, const ConvDims& c, const uint16_t* X,
         hipLaunchKernelGGL(k_conv_fwd, p.grid, dim3(256), 0, s, c, X, Wk, bias, relu, Y, Ybf);
         return;
     }
-    if (p.bm == 128) {
+    if (p.wide) {
+        if (p.ci16) launch_fwd32_wide(p, c, X, Wk, bias, relu, Y, Ybf, part, true, s);
+        else launch_fwd32_wide(p, c, X, Wk, bias, relu, Y, Ybf, part, false, s);
+    } else if (p.bm == 128) {
         if (p.ci16) launch_fwd32<128, 32, true>(p, c, X, Wk, bias, relu, Y, Ybf, part, s);
         else launch_fwd32<128, 32, false>(p, c, X, Wk, bias, relu, Y, Ybf, part, s);
     } else {

@@ -39,8 +39,10 @@ SHAPES = [(2, 5, 6, 7, 16, 32), (1, 8, 8, 4, 32, 16), (1, 3, 3, 3, 16, 48), (1, 
           (2, 8, 32, 64, 32, 128),
           # 2^18 voxels: 4096-voxel parts in the weight gradient, and the
           # forward kernel with one sub-stage per barrier pair (the smaller
-          # volumes above take three)
-          (1, 8, 128, 256, 32, 32)]
+          # volumes above take three): 256 x 32 blocks of 64 x 32 waves (up to
+          # 32 outputs, also with 16 input channels), 64 x 64 blocks beyond
+          (1, 8, 128, 256, 32, 32), (1, 8, 128, 256, 16, 32), (1, 8, 128, 256, 64, 64),
+          (2, 8, 64, 128, 128, 128)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
