@@ -270,7 +270,7 @@ def stacks_measure(n, chunks=3, seed=0, streams=4, sequential=True):
                     f"{seed}) from {W.BATCH_SIZES}, L2 and L1 4x and 16x the L3 extent, "
                     f"P 100-400 W, 0.5-2 m/s, r 30-70 um",
         "streams": streams,
-        "part1_brick_autotuned": info["tile"],
+        "brick_autotuned": info["tile"],
         "autotune_ms": info["tile_ms"],
         "setup_s": info["setup_s"],
         "ms_per_l1_step_batched": ms["batched"],

@@ -148,8 +148,8 @@ class Stacks:
         self._iters = [r.chunks() for r in self.runs]
         self._multi = None
         self._ev = [N.Event(), N.Event()]
-        # brick shape of the batched Part-1 solves: the autotuner's pick on
-        # the runs' L3 windows ("auto"), a given (rows, planes), or None (1 x 3)
+        # brick shape of the batched L3 solves: the autotuner's pick on the
+        # runs' L3 windows ("auto"), a given (rows, planes), or None (1 x 3)
         self.tile_ms = None
         self.tile = self.autotune_tile() if tile == "auto" else tile
 

@@ -2107,17 +2107,16 @@ static int multi_l3_solves(gm_multi* m, int part, int k, std::vector<L3Walk>& w)
     HIPCHK(hipMemcpyAsync(m->d_items, h, sizeof(BatchItem) * n * N3, hipMemcpyHostToDevice,
                           m->s));
     HIPCHK(hipEventRecord(m->ring_ev[slot], m->s));
+    // brick shape of both parts' solves: the set one, else 1 x 3
+    const bool t1 = m->d_bstart1 != nullptr;
+    const int* bst = t1 ? m->d_bstart1 : m->d_bstart;
+    const int nbk = t1 ? m->nblocks1 : m->nblocks, tile = t1 ? m->tile1 : 1;
     for (int j = 0; j < N3; ++j) {
         const BatchItem* it = m->d_items + (size_t)j * n;
-        if (part == 1 && m->d_bstart1)
-            launch_batch_step(it, m->d_mats, m->d_bstart1, n, m->nblocks1, m->tile1, m->rec,
-                              e0->d_desc, m->s);
-        else if (part == 1)
-            launch_batch_step(it, m->d_mats, m->d_bstart, n, m->nblocks, 1, m->rec, e0->d_desc,
-                              m->s);
+        if (part == 1)
+            launch_batch_step(it, m->d_mats, bst, n, nbk, tile, m->rec, e0->d_desc, m->s);
         else
-            launch_batch_step_acc(it, m->d_mats, m->d_bstart, n, m->nblocks, m->rec, e0->d_desc,
-                                  m->s);
+            launch_batch_step_acc(it, m->d_mats, bst, n, nbk, tile, m->rec, e0->d_desc, m->s);
     }
     return 0;
 }

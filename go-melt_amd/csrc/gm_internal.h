@@ -285,7 +285,7 @@ void launch_batch_src(const BatchItem* it, int n, int maxaxes, hipStream_t s);
 void launch_batch_step(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
                        int nblocks, int tile, bool rec, const GmStepDesc* d, hipStream_t s);
 void launch_batch_step_acc(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
-                           int nblocks, bool rec, const GmStepDesc* d, hipStream_t s);
+                           int nblocks, int tile, bool rec, const GmStepDesc* d, hipStream_t s);
 // batched L2 steps (bricks 1 x 2), face values and T' + injection of many engines
 void launch_batch_step_l2(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
                           int nblocks, bool rec, const GmStepDesc* d, hipStream_t s);

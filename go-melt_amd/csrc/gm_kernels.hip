@@ -786,15 +786,14 @@ __global__ void __launch_bounds__(64) k_step_dpp_batch(const BatchItem* __restri
 #undef GM_STEP_BRICK
 }
 
-// The same with melt time (Part 2 of the subcycle), bricks of 1 row x 3
+// The same with melt time (Part 2 of the subcycle), bricks of YB rows x ZC
 // planes: the melt time adds the item's own toolpath row dt.
-template <bool REC>
+template <bool REC, int YB, int ZC>
 __global__ void __launch_bounds__(64) k_step_dpp_batch_acc(const BatchItem* __restrict__ it,
                                                            const DevMat* __restrict__ mats,
                                                            const int* __restrict__ bstart, int n,
                                                            const GmStepDesc* __restrict__ d) {
     constexpr bool SRC = true, ACC = true;
-    constexpr int YB = 1, ZC = 3;
     GM_BATCH_FIND_INSTANCE
 #define GM_STEP_BRICK tb
 #define GM_STEP_LANE (int)threadIdx.x
@@ -1148,15 +1147,27 @@ static void batch_step_t(const BatchItem* it, const DevMat* mats, const int* bst
                            mats, bstart, n, d);
 }
 
-void launch_batch_step_acc(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
-                           int nblocks, bool rec, const GmStepDesc* d, hipStream_t s) {
-    if (n <= 0 || nblocks <= 0) return;
+template <int YB, int ZC>
+static void batch_step_acc_t(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                             int nblocks, bool rec, const GmStepDesc* d, hipStream_t s) {
     if (rec)
-        hipLaunchKernelGGL(k_step_dpp_batch_acc<true>, dim3(nblocks), dim3(64), 0, s, it, mats,
-                           bstart, n, d);
+        hipLaunchKernelGGL((k_step_dpp_batch_acc<true, YB, ZC>), dim3(nblocks), dim3(64), 0, s, it,
+                           mats, bstart, n, d);
     else
-        hipLaunchKernelGGL(k_step_dpp_batch_acc<false>, dim3(nblocks), dim3(64), 0, s, it, mats,
-                           bstart, n, d);
+        hipLaunchKernelGGL((k_step_dpp_batch_acc<false, YB, ZC>), dim3(nblocks), dim3(64), 0, s,
+                           it, mats, bstart, n, d);
+}
+
+void launch_batch_step_acc(const BatchItem* it, const DevMat* mats, const int* bstart, int n,
+                           int nblocks, int tile, bool rec, const GmStepDesc* d, hipStream_t s) {
+    if (n <= 0 || nblocks <= 0) return;
+    switch (tile) {
+        case 0: batch_step_acc_t<1, 2>(it, mats, bstart, n, nblocks, rec, d, s); break;
+        case 2: batch_step_acc_t<1, 4>(it, mats, bstart, n, nblocks, rec, d, s); break;
+        case 3: batch_step_acc_t<2, 2>(it, mats, bstart, n, nblocks, rec, d, s); break;
+        case 4: batch_step_acc_t<2, 3>(it, mats, bstart, n, nblocks, rec, d, s); break;
+        default: batch_step_acc_t<1, 3>(it, mats, bstart, n, nblocks, rec, d, s); break;
+    }
 }
 
 void launch_batch_step_l2(const BatchItem* it, const DevMat* mats, const int* bstart, int n,

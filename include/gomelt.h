@@ -339,10 +339,9 @@ int gm_multi_l3(gm_multi* m);
  * gm_multi_l3.  Equivalent to the caller's loop over gm_engine_stage_l2 and
  * gm_multi_l3, without a host call per engine and substep. */
 int gm_multi_subcycle(gm_multi* m);
-/* Brick shape (rows per band x planes, as gm_batch_set_tile) of the Part-1
- * batched solves, e.g. the one gm_batch_autotune picks for the same L3
- * windows; Part 2 (melt time) keeps 1 x 3.  Synchronises the first engine's
- * stream. */
+/* Brick shape (rows per band x planes, as gm_batch_set_tile) of the batched
+ * L3 solves of both parts, e.g. the one gm_batch_autotune picks for the same
+ * L3 windows (default 1 x 3).  Synchronises the first engine's stream. */
 int gm_multi_set_tile(gm_multi* m, int yb, int zc);
 /* With on != 0 (the default) gm_multi_subcycle also batches, per substep,
  * the L2 solves, the L3 face values and T' + injection of all engines (one

@@ -7,7 +7,7 @@ Every instance is compared with its own unbatched run (one engine, its own
 stream, the recorded subcycle graph) over the same chunks: the batched step
 runs the same per-node arithmetic on the same inputs, so the fields agree to
 FP32 rounding (the restriction sums add FP64 atomics in no fixed order, and
-the autotuned brick shape of the batched Part-1 solves groups a node's sums
+the autotuned brick shape of the batched solves groups a node's sums
 as its own kernel's shape does only for 1 x 3 bricks)."""
 import json
 
