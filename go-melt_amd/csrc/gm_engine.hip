@@ -1413,30 +1413,48 @@ static bool pairs_aligned(gm_engine* e) {
 
 // Restricted T' terms of pair i into its partial buffer: node-centric FP64
 // node box when the pairs are aligned, element partials otherwise.
+// the node-aligned restriction's arguments of pair i (fast_restrict engines)
+static RArgs fast_rargs(gm_engine* e, int i, RArgs r) {
+    const GmPair& p = e->pair[i].p;
+    if (e->use_kc && p.fine >= 2) r.KC = kc_of(e, r.T);
+    r.R0 = e->strip_R0[i];
+    r.a0 = e->strip_a0[i];
+    r.dpp_ok = r.R0 > 0;
+    if (r.dpp_ok) {
+        const gm_grid& gc = e->host.g[p.coarse];
+        const gm_grid& gf = e->host.g[p.fine];
+        r.R1 = (int)std::lround((double)gc.h[1] / gf.h[1]);
+        r.R2 = (int)std::lround((double)gc.h[2] / gf.h[2]);
+        r.W = r.R0 * (61 / r.R0);
+        r.invR[0] = 1.0 / r.R0;
+        r.invR[1] = 1.0 / r.R1;
+        r.invR[2] = 1.0 / r.R2;
+        r.dpp_ok = r.R1 >= 1 && r.R2 >= 1;
+    }
+    memcpy(r.kcs, e->kc[p.fine], sizeof(r.kcs));
+    memcpy(r.mcs, e->mcs[p.fine], sizeof(r.mcs));
+    return r;
+}
+
 static void eng_restrict(gm_engine* e, int i, RArgs r) {
     const GmPair& p = e->pair[i].p;
-    if (e->fast_restrict) {
-        if (e->use_kc && p.fine >= 2) r.KC = kc_of(e, r.T);
-        r.R0 = e->strip_R0[i];
-        r.a0 = e->strip_a0[i];
-        r.dpp_ok = r.R0 > 0;
-        if (r.dpp_ok) {
-            const gm_grid& gc = e->host.g[p.coarse];
-            const gm_grid& gf = e->host.g[p.fine];
-            r.R1 = (int)std::lround((double)gc.h[1] / gf.h[1]);
-            r.R2 = (int)std::lround((double)gc.h[2] / gf.h[2]);
-            r.W = r.R0 * (61 / r.R0);
-            r.invR[0] = 1.0 / r.R0;
-            r.invR[1] = 1.0 / r.R1;
-            r.invR[2] = 1.0 / r.R2;
-            r.dpp_ok = r.R1 >= 1 && r.R2 >= 1;
-        }
-        memcpy(r.kcs, e->kc[p.fine], sizeof(r.kcs));
-        memcpy(r.mcs, e->mcs[p.fine], sizeof(r.mcs));
-        launch_restrict_fast(p, e->d_desc, e->d_dm, r, e->part[i], e->s);
-    } else {
+    if (e->fast_restrict)
+        launch_restrict_fast(p, e->d_desc, e->d_dm, fast_rargs(e, i, r), e->part[i], e->s);
+    else
         launch_restrict(p, e->d_desc, e->dm, r, e->part[i], e->s);
-    }
+}
+
+// eng_restrict of pair i as an item of the batched restriction, when the
+// pair takes the form the batched launch runs
+static bool restrict_item(gm_engine* e, int i, const RArgs& r0, RItem* it) {
+    if (!e->fast_restrict) return false;
+    const RArgs r = fast_rargs(e, i, r0);
+    if (!restrict_batchable(r)) return false;
+    it->p = e->pair[i].p;
+    it->d = e->d_desc;
+    it->r = r;
+    it->acc = e->part[i];
+    return true;
 }
 
 static void eng_restrict_nodes(gm_engine* e, int outer, int inner, const float* Vbase, float* V,
@@ -1558,19 +1576,25 @@ static void stage_prologue(gm_engine* e, SubcycleState& st) {
 // of substep k: restriction of the L3 T' terms, L2 solve, L3 face values.
 // the L2 stage before the L2 solve: restriction of the L3 T' terms onto L2
 // and the buffers of the substep's new L2 field
-static void stage_l2_pre(gm_engine* e, SubcycleState& st, int part, int k) {
+// the restriction's arguments of stage_l2_pre (and its buffer bookkeeping)
+static RArgs stage_l2_rargs(gm_engine* e, SubcycleState& st, int part, int k) {
+    RArgs r;
     if (part == 1) {
-        eng_restrict(e, 2, rargs(e, 3, st.T3cur, st.S13cur, st.Tp3cur, nullptr, nullptr,
-                                 GM_DT_VALUE, 0, 1.0f));
+        r = rargs(e, 3, st.T3cur, st.S13cur, st.Tp3cur, nullptr, nullptr, GM_DT_VALUE, 0, 1.0f);
         st.T2new = e->T2b[st.b2];
     } else {
         // flux + mass restriction onto L2 (cF:3496-3508)
-        eng_restrict(e, 2, rargs(e, 3, st.T3cur, st.S13cur, st.Tp3cur, e->histTp[k],
-                                 st.Tp3cur, GM_DT_L2, k, 0.0f));
+        r = rargs(e, 3, st.T3cur, st.S13cur, st.Tp3cur, e->histTp[k], st.Tp3cur, GM_DT_L2, k,
+                  0.0f);
         st.T2new = e->L2all[k];
     }
     st.S12new = e->S12b[st.b2];
     st.b2 ^= 1;
+    return r;
+}
+
+static void stage_l2_pre(gm_engine* e, SubcycleState& st, int part, int k) {
+    eng_restrict(e, 2, stage_l2_rargs(e, st, part, k));
 }
 
 // after the L2 solve and the L3 face values: the buffer that will hold the
@@ -1953,6 +1977,9 @@ struct gm_multi {
     hipEvent_t ring2_ev[kMultiRing] = {};
     int ring2_slot = 0;
     size_t sub2_bytes = 0, off_l2 = 0, off_face = 0, off_tp = 0, off_bf = 0, off_bt = 0;
+    size_t off_ri = 0, off_br = 0;  // restriction items and their block starts
+    // batched work on s not yet waited for by the other streams
+    bool out_pending = false;
 };
 
 extern "C" int gm_multi_destroy(gm_multi* m) {
@@ -2050,6 +2077,10 @@ extern "C" int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out)
         o = align256(o + sizeof(int) * (n + 1));
         m->off_bt = o;
         o = align256(o + sizeof(int) * (n + 1));
+        m->off_ri = o;
+        o = align256(o + sizeof(RItem) * n);
+        m->off_br = o;
+        o = align256(o + sizeof(int) * (n + 1));
         m->sub2_bytes = o;
         MCHK(hipMalloc(&m->d_sub2, m->sub2_bytes));
         MCHK(hipHostMalloc((void**)&m->h_ring2, m->sub2_bytes * kMultiRing, hipHostMallocDefault));
@@ -2143,10 +2174,14 @@ extern "C" int gm_multi_l3(gm_multi* m) {
     return 0;
 }
 
-// One whole substep of every engine with the L2 solves, the L3 face values
-// and T' + injection batched too (engines on the node-aligned path): the
-// restrictions per engine on its stream, then on m->s the batched L2 step,
-// face values, N3 L3 solves and T' + injection.
+// One whole substep of every engine with the L2 stage and T' batched too
+// (engines on the node-aligned path): on m->s one launch each for the
+// restrictions of the L3 T' terms, the L2 solves, the face values, the N3 L3
+// solves and T' + injection.  An engine whose restriction the batched launch
+// does not run restricts on its own stream, and the streams are joined around
+// it; otherwise m->s waits for the other streams only at the first substep of
+// each part (after the prologue or the middle of the L1 step) and they wait
+// for it after the last (before the middle or the epilogue).
 static int multi_substep_batched(gm_multi* m) {
     gm_engine* e0 = m->e[0];
     const int part = e0->stg_part, k = e0->stg_k, n = m->n;
@@ -2159,11 +2194,27 @@ static int multi_substep_batched(gm_multi* m) {
     TpItem* tp = (TpItem*)(h + m->off_tp);
     int* bf = (int*)(h + m->off_bf);
     int* bt = (int*)(h + m->off_bt);
-    bf[0] = bt[0] = 0;
+    RItem* ri = (RItem*)(h + m->off_ri);
+    int* br = (int*)(h + m->off_br);
+    bf[0] = bt[0] = br[0] = 0;
+    int nri = 0;
+    bool own = false;  // an engine restricted on its own stream
     for (int i = 0; i < n; ++i) {
         gm_engine* e = m->e[i];
         SubcycleState& st = e->stg;
-        stage_l2_pre(e, st, part, k);
+        const RArgs ra = stage_l2_rargs(e, st, part, k);
+        if (restrict_item(e, 2, ra, &ri[nri])) {
+            br[nri + 1] = br[nri] + restrict_batch_blocks(ri[nri].p, ri[nri].r);
+            ++nri;
+        } else {
+            // its inputs come from the batched work on m->s
+            if (m->out_pending) {
+                if (int rc = multi_join_out(m)) return rc;
+                m->out_pending = false;
+            }
+            eng_restrict(e, 2, ra);
+            own = true;
+        }
         l2[i].a = stage_l2_args(e, st, k);
         l2[i].m = e->d_dm;
         memset(l2[i].row, 0, sizeof(l2[i].row));
@@ -2176,10 +2227,14 @@ static int multi_substep_batched(gm_multi* m) {
         bt[i + 1] = bt[i] + nb;
         e->stg_phase = 1;
     }
-    if (int rc = multi_join_in(m)) return rc;
+    if (k == 0 || own)
+        if (int rc = multi_join_in(m)) return rc;
     HIPCHK(hipMemcpyAsync(m->d_sub2, h, m->sub2_bytes, hipMemcpyHostToDevice, m->s));
     HIPCHK(hipEventRecord(m->ring2_ev[slot], m->s));
     char* d = m->d_sub2;
+    // all items of one part have the same terms: flux (Part 1), flux + mass
+    launch_restrict_batch((const RItem*)(d + m->off_ri), (const int*)(d + m->off_br), nri,
+                          br[nri], true, part == 2, m->s);
     launch_batch_step_l2((const BatchItem*)(d + m->off_l2), m->d_mats, m->d_bst_l2, n,
                          m->nblk_l2, m->rec, e0->d_desc, m->s);
     launch_face_bc_batch((const FaceItem*)(d + m->off_face), (const int*)(d + m->off_bf), n,
@@ -2189,7 +2244,11 @@ static int multi_substep_batched(gm_multi* m) {
     if (int rc = multi_l3_solves(m, part, k, w)) return rc;
     launch_tprime_batch((const TpItem*)(d + m->off_tp), (const int*)(d + m->off_bt), n, bt[n],
                         m->s);
-    if (int rc = multi_join_out(m)) return rc;
+    m->out_pending = true;
+    if (k == e0->N2 - 1 || own) {
+        if (int rc = multi_join_out(m)) return rc;
+        m->out_pending = false;
+    }
     for (int i = 0; i < n; ++i) {
         gm_engine* e = m->e[i];
         stage_l3_close_state(e, e->stg, part, k, w[i].S13in, w[i].S23in, e->stg.T3out);

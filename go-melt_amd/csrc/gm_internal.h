@@ -222,6 +222,15 @@ struct RArgs {
     double invR[3];       // 1 / ratio per axis (FP64 weights)
 };
 
+// One engine's restriction in a batched launch (launch_restrict_batch): the
+// arguments of its single launch.
+struct RItem {
+    GmPair p;
+    const GmStepDesc* d;
+    RArgs r;
+    double* acc;
+};
+
 // ---- launchers (gm_kernels.hip) -------------------------------------------
 namespace gm {
 
@@ -294,6 +303,12 @@ void launch_face_bc_batch(const FaceItem* it, const int* bstart, int n, int nblo
 int face_bc_blocks(const gm_grid& fine_grid);
 void launch_tprime_batch(const TpItem* it, const int* bstart, int n, int nblocks,
                          hipStream_t s);
+// the node-aligned restriction with carried k, cp (the form the batched
+// launch runs), its blocks, and the batched launch
+bool restrict_batchable(const RArgs& r);
+int restrict_batch_blocks(const GmPair& p, const RArgs& r);
+void launch_restrict_batch(const RItem* it, const int* bstart, int n, int nblocks, bool flux,
+                           bool mass, hipStream_t s);
 // blocks of one item in launch_tprime_batch; *ntp_out: its T' blocks of the single launch
 int tprime_blocks(const gm_grid& fine_grid, int64_t nbox, int* ntp_out);
 void launch_step_tile(const LSArgs& a, const GmStepDesc* d, const gm_grid& hg, const DevMat* m,

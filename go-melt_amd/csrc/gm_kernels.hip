@@ -1952,8 +1952,12 @@ __global__ void __launch_bounds__(256) k_restrict_kc(GmPair pp, const GmStepDesc
         sX[az][yy][ax] = sum;
     }
     lds_barrier();
-    const int lo0 = pp.ax[0].info[0], lo1 = pp.ax[1].info[0], lo2 = pp.ax[2].info[0];
-    const int nb0 = pp.ax[0].info[1] + 1, nb1 = pp.ax[1].info[1] + 1, nb2 = pp.ax[2].info[1] + 1;
+    const int lo0 = __builtin_amdgcn_readfirstlane(pp.ax[0].info[0]);
+    const int lo1 = __builtin_amdgcn_readfirstlane(pp.ax[1].info[0]);
+    const int lo2 = __builtin_amdgcn_readfirstlane(pp.ax[2].info[0]);
+    const int nb0 = __builtin_amdgcn_readfirstlane(pp.ax[0].info[1]) + 1;
+    const int nb1 = __builtin_amdgcn_readfirstlane(pp.ax[1].info[1]) + 1;
+    const int nb2 = __builtin_amdgcn_readfirstlane(pp.ax[2].info[1]) + 1;
     const int st0 = pp.cmax[0] + 1, st1 = pp.cmax[1] + 1;
     for (int cI = tid; cI < NAZ * NAY * NAX; cI += 256) {
         const int az = cI / (NAY * NAX), rem = cI % (NAY * NAX);
@@ -2025,13 +2029,22 @@ __device__ __forceinline__ float stencil_dpp(const float (*E)[2], const float (*
     return s;
 }
 
+// One wave's brick t of the node-centric restriction of pair pp (the body of
+// k_restrict_dpp and of its batched form).  Values read through d, pp and r
+// are the same in every lane; they are moved to scalar registers where they
+// size buffers or offsets, since in the batched launch the compiler cannot see
+// that.
 template <bool FLUX, bool MASS, int ZC>
-__global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc* __restrict__ d,
-                                                     RArgs r, double* __restrict__ acc) {
+__device__ __forceinline__ void restrict_dpp_brick(const GmPair& pp,
+                                                   const GmStepDesc* __restrict__ d,
+                                                   const RArgs& r, double* __restrict__ acc,
+                                                   int t) {
     constexpr int NZC = ZC + 1;  // coarse z nodes a brick can touch (ratio 1)
-    const gm_grid gf = d->g[pp.fine];
-    const gm_grid gc = d->g[pp.coarse];
-    const int nx = gf.n[0], ny = gf.n[1], nz = gf.n[2];
+    const gm_grid& gf = d->g[pp.fine];
+    const gm_grid& gc = d->g[pp.coarse];
+    const int nx = __builtin_amdgcn_readfirstlane(gf.n[0]);
+    const int ny = __builtin_amdgcn_readfirstlane(gf.n[1]);
+    const int nz = __builtin_amdgcn_readfirstlane(gf.n[2]);
     const int nxy = nx * ny;
     const int nn = nxy * nz;
     const int nex = nx - 1, ney = ny - 1, nez = nz - 1;
@@ -2043,11 +2056,11 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
     int off[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-        off[a] = (int)rintf((gf.x0[a] - gc.x0[a]) * __builtin_amdgcn_rcpf(gf.h[a]));
+        off[a] = __builtin_amdgcn_readfirstlane(
+            (int)rintf((gf.x0[a] - gc.x0[a]) * __builtin_amdgcn_rcpf(gf.h[a])));
     const int W = r.W, a0 = r.a0;  // a0: first fine x on a coarse node
     const int s0 = a0 > 0 ? 0 : 1;
     const int nzc = (nz + ZC - 1) / ZC;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int rest = fdiv_small(t, __builtin_amdgcn_rcpf((float)nzc));
     const int zc = t - rest * nzc;
     const int st = fdiv_small(rest, __builtin_amdgcn_rcpf((float)ny));
@@ -2151,8 +2164,12 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
     const int lq = fdiv_small(lane + R[0] - 1, iRf[0]) - 1;  // floor((lane - 1) / R)
     const bool owner = lane >= 1 && lane <= W + 1 && lane - 1 == lq * R[0];
     const int cx = fdiv_small(x + off[0] + 128 * R[0], iRf[0]) - 128;
-    const int lo0 = pp.ax[0].info[0], lo1 = pp.ax[1].info[0], lo2 = pp.ax[2].info[0];
-    const int nb0 = pp.ax[0].info[1] + 1, nb1 = pp.ax[1].info[1] + 1, nb2 = pp.ax[2].info[1] + 1;
+    const int lo0 = __builtin_amdgcn_readfirstlane(pp.ax[0].info[0]);
+    const int lo1 = __builtin_amdgcn_readfirstlane(pp.ax[1].info[0]);
+    const int lo2 = __builtin_amdgcn_readfirstlane(pp.ax[2].info[0]);
+    const int nb0 = __builtin_amdgcn_readfirstlane(pp.ax[0].info[1]) + 1;
+    const int nb1 = __builtin_amdgcn_readfirstlane(pp.ax[1].info[1]) + 1;
+    const int nb2 = __builtin_amdgcn_readfirstlane(pp.ax[2].info[1]) + 1;
     const int st0 = pp.cmax[0] + 1, st1 = pp.cmax[1] + 1;
 #pragma unroll
     for (int b = 0; b < NZC; ++b) {
@@ -2177,6 +2194,52 @@ __global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc
     }
 }
 
+template <bool FLUX, bool MASS, int ZC>
+__global__ void __launch_bounds__(64) k_restrict_dpp(GmPair pp, const GmStepDesc* __restrict__ d,
+                                                     RArgs r, double* __restrict__ acc) {
+    restrict_dpp_brick<FLUX, MASS, ZC>(pp, d, r, acc, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// The restriction of many engines' pairs in one launch (items as the single
+// launch's arguments, node-aligned pairs with carried k, cp): bstart[i] is
+// the first block of item i.
+template <bool FLUX, bool MASS, int ZC>
+__global__ void __launch_bounds__(64) k_restrict_dpp_batch(const RItem* __restrict__ it,
+                                                           const int* __restrict__ bstart,
+                                                           int n) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    int b0;
+    const int lo = batch_item_of(bstart, n, b, &b0);
+    const RItem& x = it[lo];
+    restrict_dpp_brick<FLUX, MASS, ZC>(x.p, x.d, x.r, x.acc, b - b0);
+}
+
+// bricks of the node-aligned restriction of pair p with arguments r
+static int restrict_dpp_nbd(const GmPair& p, const RArgs& r, int ZC) {
+    const int nx = p.nef[0] + 1, ny = p.nef[1] + 1, nz = p.nef[2] + 1;
+    const int nst = (nx - r.a0 + r.W - 1) / r.W + (r.a0 > 0 ? 1 : 0);
+    return nst * ny * ((nz + ZC - 1) / ZC);
+}
+
+bool restrict_batchable(const RArgs& r) { return r.KC && r.dpp_ok && (r.Tp || r.TpA); }
+
+int restrict_batch_blocks(const GmPair& p, const RArgs& r) { return restrict_dpp_nbd(p, r, 4); }
+
+void launch_restrict_batch(const RItem* it, const int* bstart, int n, int nblocks, bool flux,
+                           bool mass, hipStream_t s) {
+    if (n <= 0 || nblocks <= 0) return;
+    constexpr int ZC = 4;
+    if (flux && mass)
+        hipLaunchKernelGGL((k_restrict_dpp_batch<true, true, ZC>), dim3(nblocks), dim3(64), 0, s,
+                           it, bstart, n);
+    else if (flux)
+        hipLaunchKernelGGL((k_restrict_dpp_batch<true, false, ZC>), dim3(nblocks), dim3(64), 0, s,
+                           it, bstart, n);
+    else
+        hipLaunchKernelGGL((k_restrict_dpp_batch<false, true, ZC>), dim3(nblocks), dim3(64), 0, s,
+                           it, bstart, n);
+}
+
 void launch_restrict_fast(const GmPair& p, const GmStepDesc* d, const DevMat* m, const RArgs& r,
                           double* acc, hipStream_t s) {
     // grid of the fine level: the pair's element counts + 1
@@ -2187,10 +2250,7 @@ void launch_restrict_fast(const GmPair& p, const GmStepDesc* d, const DevMat* m,
         // x strips of W = R * floor(61 / R) columns starting on coarse nodes,
         // one per fine row and ZC planes
         constexpr int ZC = 4;
-        const int a0 = r.a0;
-        const int W = r.W;
-        const int nst = (nx - a0 + W - 1) / W + (a0 > 0 ? 1 : 0);
-        const int nbd = nst * ny * ((nz + ZC - 1) / ZC);
+        const int nbd = restrict_dpp_nbd(p, r, ZC);
         if (fl && ms)
             hipLaunchKernelGGL((k_restrict_dpp<true, true, ZC>), dim3(nbd), dim3(64), 0, s, p, d, r,
                                acc);

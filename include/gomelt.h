@@ -344,10 +344,12 @@ int gm_multi_subcycle(gm_multi* m);
  * L3 windows (default 1 x 3).  Synchronises the first engine's stream. */
 int gm_multi_set_tile(gm_multi* m, int yb, int zc);
 /* With on != 0 (the default) gm_multi_subcycle also batches, per substep,
- * the L2 solves, the L3 face values and T' + injection of all engines (one
- * launch each, after each engine's restriction on its own stream), when
- * every engine is on the node-aligned path; otherwise, or with on = 0, those
- * run per engine as in gm_engine_stage_l2 / _l3. */
+ * the restrictions of the L3 T' terms, the L2 solves, the L3 face values and
+ * T' + injection of all engines (one launch each on the first engine's
+ * stream; an engine whose restriction is not of the node-aligned form with
+ * carried properties restricts on its own stream), when every engine is on
+ * the node-aligned path; otherwise, or with on = 0, those run per engine as
+ * in gm_engine_stage_l2 / _l3. */
 int gm_multi_set_batch_l2(gm_multi* m, int on);
 int gm_multi_destroy(gm_multi* m);
 
