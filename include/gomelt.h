@@ -336,8 +336,11 @@ int gm_multi_create(gm_engine* const* engines, int n, gm_multi** out);
 int gm_multi_l3(gm_multi* m);
 /* The whole subcycle of every engine after gm_engine_stage_begin on each:
  * for each of the 2*N2 L2 substeps the L2 stage of every engine, then
- * gm_multi_l3.  Equivalent to the caller's loop over gm_engine_stage_l2 and
- * gm_multi_l3, without a host call per engine and substep. */
+ * gm_multi_l3.  Gives what the caller's loop over gm_engine_stage_l2 and
+ * gm_multi_l3 gives (to FP32 rounding; see gm_multi_set_batch_l2), without a
+ * host call per engine and substep.  It does not wait for the GPU: the host
+ * may run up to 32 substeps ahead (pinned item slots), and each engine's
+ * stream is ordered after the batched work when the call returns. */
 int gm_multi_subcycle(gm_multi* m);
 /* Brick shape (rows per band x planes, as gm_batch_set_tile) of the batched
  * L3 solves of both parts, e.g. the one gm_batch_autotune picks for the same
