@@ -123,10 +123,11 @@ class Stacks:
     One TimeLoop per deck, the engines dealt over `streams` streams.  Each step gives every
     run its next chunk of N2*N3 toolpath rows; a single-step chunk runs as in
     the plain driver, a subcycled one (subcycleGOMELT cF:3224-3632) stage by
-    stage in lockstep with the others: per L2 substep the restriction of each
-    run, then one batched launch each for the L2 solves, the L3 face values,
-    the N3 L3 solves and T' + injection of all subcycling runs
-    (gm_multi_subcycle; batch_l2=False batches the L3 solves only).  Every run ends where its own unbatched run ends."""
+    stage in lockstep with the others: per L2 substep one batched launch each
+    for the restrictions of the L3 T' terms, the L2 solves, the L3 face
+    values, the N3 L3 solves and T' + injection of all subcycling runs
+    (gm_multi_subcycle; batch_l2=False batches the L3 solves only).  Every
+    run ends where its own unbatched run ends."""
 
     def __init__(self, decks, streams=4, tile="auto", batch_l2=True):
         import computeFunctions as cf
