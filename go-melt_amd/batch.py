@@ -2,8 +2,8 @@
 
 * Batch: one kernel launch per step for many independent L3 windows of their
   own mesh size and laser (gm_batch_* in include/gomelt.h).
-* Stacks: many independent three-level runs advanced in lockstep, the L3
-  solves of all of them one launch per solve (gm_multi_*).
+* Stacks: many independent three-level runs advanced in lockstep, each
+  substep stage of all of them one launch (gm_multi_*).
 
 There is no CPU fallback: without libgomelt.so or a GPU every call raises."""
 
