@@ -178,6 +178,7 @@ _SIGS = {
     "gm_multi_subcycle": (_I, [_P]),
     "gm_multi_set_tile": (_I, [_P, _I, _I]),
     "gm_multi_set_batch_l2": (_I, [_P, _I]),
+    "gm_multi_set_batch_restrict": (_I, [_P, _I]),
     "gm_multi_destroy": (_I, [_P]),
 }
 

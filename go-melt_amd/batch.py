@@ -126,10 +126,11 @@ class Stacks:
     stage in lockstep with the others: per L2 substep one batched launch each
     for the restrictions of the L3 T' terms, the L2 solves, the L3 face
     values, the N3 L3 solves and T' + injection of all subcycling runs
-    (gm_multi_subcycle; batch_l2=False batches the L3 solves only).  Every
-    run ends where its own unbatched run ends."""
+    (gm_multi_subcycle; batch_l2=False batches the L3 solves only,
+    batch_restrict=False leaves each run's restriction on its own stream).
+    Every run ends where its own unbatched run ends."""
 
-    def __init__(self, decks, streams=4, tile="auto", batch_l2=True):
+    def __init__(self, decks, streams=4, tile="auto", batch_l2=True, batch_restrict=True):
         import computeFunctions as cf
         import go_melt as G
         N.init_device()
@@ -143,6 +144,7 @@ class Stacks:
                 self.runs.append(G.TimeLoop(d, quiet=True, write_outputs=False))
         self.n = len(self.runs)
         self.batch_l2 = bool(batch_l2)
+        self.batch_restrict = bool(batch_restrict)
         self._pending = []
         for r in self.runs:
             cf.attach_l3_operator(r.Levels, _Deferred(self))
@@ -186,6 +188,7 @@ class Stacks:
             if self.tile is not None:
                 N.check(N.lib().gm_multi_set_tile(h.value, int(self.tile[0]), int(self.tile[1])))
             N.check(N.lib().gm_multi_set_batch_l2(h.value, 1 if self.batch_l2 else 0))
+            N.check(N.lib().gm_multi_set_batch_restrict(h.value, 1 if self.batch_restrict else 0))
         return self._multi[1]
 
     def step(self):

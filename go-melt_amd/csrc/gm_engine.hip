@@ -1970,6 +1970,7 @@ struct gm_multi {
     // of the L2-step, face and T' items and their block starts (slot layout
     // at off_*), from a pinned ring of kMultiRing slots
     bool batch_l2 = true;
+    bool batch_restrict = true;  // the restrictions in the batched launch too
     int* d_bst_l2 = nullptr;
     int nblk_l2 = 0;
     char* d_sub2 = nullptr;
@@ -2203,7 +2204,7 @@ static int multi_substep_batched(gm_multi* m) {
         gm_engine* e = m->e[i];
         SubcycleState& st = e->stg;
         const RArgs ra = stage_l2_rargs(e, st, part, k);
-        if (restrict_item(e, 2, ra, &ri[nri])) {
+        if (m->batch_restrict && restrict_item(e, 2, ra, &ri[nri])) {
             br[nri + 1] = br[nri] + restrict_batch_blocks(ri[nri].p, ri[nri].r);
             ++nri;
         } else {
@@ -2261,6 +2262,12 @@ static int multi_substep_batched(gm_multi* m) {
 extern "C" int gm_multi_set_batch_l2(gm_multi* m, int on) {
     if (!m) return fail("gm_multi_set_batch_l2: null");
     m->batch_l2 = on != 0;
+    return 0;
+}
+
+extern "C" int gm_multi_set_batch_restrict(gm_multi* m, int on) {
+    if (!m) return fail("gm_multi_set_batch_restrict: null");
+    m->batch_restrict = on != 0;
     return 0;
 }
 

@@ -354,6 +354,11 @@ int gm_multi_set_tile(gm_multi* m, int yb, int zc);
  * the node-aligned path; otherwise, or with on = 0, those run per engine as
  * in gm_engine_stage_l2 / _l3. */
 int gm_multi_set_batch_l2(gm_multi* m, int on);
+/* With on = 0 the batched substep leaves each engine's restriction of the
+ * L3 T' terms on its own stream (the streams joined around it every
+ * substep), as for an engine whose restriction the batched launch does not
+ * run; default on. */
+int gm_multi_set_batch_restrict(gm_multi* m, int on);
 int gm_multi_destroy(gm_multi* m);
 
 /* stepGOMELT (cF:2304-2397) for one toolpath row.  Writes the reference's

@@ -44,15 +44,18 @@ def _alone(deck, chunks):
     return run
 
 
-@pytest.mark.parametrize("batch_l2", [True, False])
-def test_stacks_equal_their_own_runs(tmp_path, monkeypatch, batch_l2):
-    """batch_l2: the L2 solves, face values and T' + injection batched across
-    the stacks too, or only the L3 solves."""
+@pytest.mark.parametrize("batch_l2,batch_restrict", [(True, True), (True, False), (False, True)])
+def test_stacks_equal_their_own_runs(tmp_path, monkeypatch, batch_l2, batch_restrict):
+    """batch_l2: the restrictions, L2 solves, face values and T' + injection
+    batched across the stacks too, or only the L3 solves; batch_restrict
+    False: each stack restricts on its own stream inside the batched
+    substep (the path of a stack whose restriction the batched launch does
+    not run)."""
     import batch as B
     monkeypatch.delenv("GOMELT_NO_GRAPH", raising=False)
     decks = _decks(tmp_path)
     chunks = 4  # one single-step chunk, then three subcycled L1 steps
-    st = B.Stacks(decks, batch_l2=batch_l2)
+    st = B.Stacks(decks, batch_l2=batch_l2, batch_restrict=batch_restrict)
     for _ in range(chunks):
         assert st.step()
     st.sync()
