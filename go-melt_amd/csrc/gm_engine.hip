@@ -1572,11 +1572,8 @@ static void stage_prologue(gm_engine* e, SubcycleState& st) {
     stage_reset(e, st);
 }
 
-// Part 1 (part = 1, cF:3333-3420) or Part 2 (part = 2, cF:3466-3608) L2 stage
-// of substep k: restriction of the L3 T' terms, L2 solve, L3 face values.
-// the L2 stage before the L2 solve: restriction of the L3 T' terms onto L2
-// and the buffers of the substep's new L2 field
-// the restriction's arguments of stage_l2_pre (and its buffer bookkeeping)
+// The L2 stage before the L2 solve: the arguments of the restriction of the
+// L3 T' terms onto L2, and the buffers of the substep's new L2 field.
 static RArgs stage_l2_rargs(gm_engine* e, SubcycleState& st, int part, int k) {
     RArgs r;
     if (part == 1) {
